@@ -1,0 +1,218 @@
+/*
+ * ppafm_hip.h -- C ABI of libppafm_hip.so, the MI355X (gfx950) implementation
+ * of the probe-particle hot path: LJ + point-charge Coulomb force-field grid,
+ * FIRE relaxation of the probe through the interpolated grid, and the z
+ * convolution that turns the force stack into a frequency shift.
+ *
+ * Two groups of entry points.
+ *
+ * (1) Reference-compatible host ABI.  Same names, argument lists and
+ *     semantics as the ctypes bindings of ppafm/core.py:33-324 over
+ *     ppafm/cpp/ProbeParticle.cpp.  Arrays are caller-owned, C-contiguous
+ *     float64 host arrays with the reference layouts (grids (nx,ny,nz,3) with
+ *     z fastest).  Like the reference these keep process-wide state set by the
+ *     set* calls and are not reentrant.  Work runs on the GPU in FP64.
+ *
+ * (2) Device ABI (ppafm_* prefix).  Plain device pointers and sizes, explicit
+ *     descriptors, an explicit HIP stream (NULL = default stream), an int
+ *     status (0 = success; ppafm_last_error() describes a failure).  The grid
+ *     is FP32 float4 (Fx,Fy,Fz,E) with z fastest; relaxation arithmetic is
+ *     FP64.  These have no counterpart in the reference; they are what the
+ *     Python mirror of ppafm.ocl.AFMulator and the batched path call.
+ */
+#ifndef PPAFM_HIP_H
+#define PPAFM_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------
+ * (1) Reference-compatible host ABI
+ * ---------------------------------------------------------------------- */
+
+/* Tip radial spline (ProbeParticle.cpp:58-62).  Only NULL (harmonic radial
+ * spring) is supported; a non-NULL spline makes the relax calls fail. */
+typedef struct {
+    int rff_n;
+    double *rff_xs;
+    double *rff_ydys;
+} ppafm_spline_params;
+
+/* ProbeParticle.cpp:358  setGridN(int* n): grid points along a, b, c. */
+void setGridN(int *n);
+/* ProbeParticle.cpp:365  setGridCell(double* cell): 3x3 lattice, rows a,b,c. */
+void setGridCell(double *cell);
+/* ProbeParticle.cpp:338  setFF_Fpointer: (nx,ny,nz,3) float64 force grid. */
+void setFF_Fpointer(double *gridF);
+/* ProbeParticle.cpp:343  setFF_Epointer: (nx,ny,nz) float64 energy grid. */
+void setFF_Epointer(double *gridE);
+/* ProbeParticle.cpp:348 / 353 */
+void deleteFF_Fpointer(void);
+void deleteFF_Epointer(void);
+/* ProbeParticle.cpp:371  setTip(lRad, kRad, rPP0[3], kSpring[3]); rPP0.z is
+ * moved onto the sphere of radius lRad when |kRad| > 1e-8 (makeConsistent). */
+void setTip(double lRad, double kRad, double *rPP0, double *kSpring);
+/* ProbeParticle.cpp:322  setRelax(maxIters, convF2, dt, damping). */
+void setRelax(int maxIters, double convF2, double dt, double damping);
+/* ProbeParticle.cpp:331  setFIRE(finc, fdec, falpha). */
+void setFIRE(double finc, double fdec, double falpha);
+/* ProbeParticle.cpp:508  F (and E if set) += LJ grid of natoms atoms;
+ * Rs (natoms,3) relative to the grid origin, cLJs (natoms,2) = (c6, c12). */
+void getLennardJonesFF(int natoms, double *Rs, double *cLJs);
+/* ProbeParticle.cpp:540  F (and E) += Coulomb grid for a unit probe charge;
+ * kQQs (natoms) = Q_i * 14.399645; kind 0 = s, 1 = pz, 2 = dz2. */
+void getCoulombFF(int natoms, double *Rs, double *kQQs, int kind);
+/* ProbeParticle.cpp:633  relax one stroke of nstep tip positions; rs, fs
+ * (nstep,3) out; returns the sum of FIRE iteration indices. */
+int relaxTipStroke(int probeStart, int relaxAlg, int nstep, double *rTips, double *rs, double *fs,
+                   ppafm_spline_params *splineParams);
+/* ProbeParticle.cpp:682  relax nx*ny strokes; rTips, rs, fs (nx,ny,nz,3);
+ * returns 0 like the reference (ppafm_last_itrsum() gives the sum). */
+int relaxTipStrokes_omp(int nx, int ny, int probeStart, int relaxAlg, int nstep, double *rTips, double *rs,
+                        double *fs, ppafm_spline_params *splineParams);
+
+/* ------------------------------------------------------------------------
+ * (2) Device ABI
+ * ---------------------------------------------------------------------- */
+
+/* Regular grid: n points along the rows of cell (a, b, c); point (i,j,k) sits
+ * at i*cell[0..2]/n0 + j*cell[3..5]/n1 + k*cell[6..8]/n2 relative to the grid
+ * origin; all positions below are relative to the grid origin. */
+typedef struct {
+    int n[3];
+    double cell[9];
+} ppafm_grid;
+
+/* Force-field model of one grid build.
+ *  mode 0 (ProbeParticle semantics): F = clamp(F_LJ) + probe_q * clamp(F_El)
+ *         with R2SAFE = (double)1e-8f, clamp = io.limit_vec_field to fmax
+ *         (fmax <= 0: no clamp); F_El is the unit-charge field of the
+ *         stencil below (kind 0: one point, weight 1).
+ *  mode 1 (FF.cl evalLJC_QZs_noPos semantics): F = F_LJ + F_El, R2SAFE 1e-6,
+ *         no clamp; the stencil carries the tip charges Qs (weights) at
+ *         QZs (z offsets).
+ * Coulomb charges enter as atoms[:,3] * coulomb_const. */
+typedef struct {
+    int mode;
+    int with_lj;
+    int n_stencil;          /* 0..4 Coulomb evaluation points */
+    int with_energy;        /* write E into the .w channel (else 0) */
+    double stencil_w[4];
+    double stencil_dz[4];
+    double coulomb_const;
+    double probe_q;
+    double fmax;
+} ppafm_ff_model;
+
+/* One atom as the grid kernels read it: position relative to the grid
+ * origin, charge already multiplied by the Coulomb constant, LJ c6/c12. */
+typedef struct {
+    double x, y, z, q, c6, c12;
+} ppafm_atom;
+
+/* Pack host arrays atoms_xyzq (natoms,4) and cLJs (natoms,2, may be NULL)
+ * into natoms ppafm_atom records at out_host, q scaled by coulomb_const. */
+int ppafm_pack_atoms(int natoms, const double *atoms_xyzq, const double *cLJs, double coulomb_const,
+                     ppafm_atom *out_host);
+
+/* Build the FP32 float4 grid from device-resident atoms (no host work, no
+ * synchronisation: capturable in a HIP graph). */
+int ppafm_ff_build_dev(const ppafm_grid *grid, const ppafm_ff_model *model, int natoms, const ppafm_atom *atoms_dev,
+                       void *out_dev, void *stream);
+
+/* Build the FP32 float4 grid out[(ix*ny+iy)*nz+iz] on the device.
+ * atoms_xyzq (natoms,4) and cLJs (natoms,2) are HOST float64 arrays
+ * (positions relative to the grid origin).  Waits for earlier work on
+ * the stream before it stages the atoms. */
+int ppafm_ff_build(const ppafm_grid *grid, const ppafm_ff_model *model, int natoms, const double *atoms_xyzq,
+                   const double *cLJs, void *out_dev, void *stream);
+
+/* Relaxation semantics.
+ *  mode 0 (ProbeParticle.cpp): FIRE state (dt, alpha) reset for every tip
+ *         position; output force = grid force sampled at the final position.
+ *  mode 1 (relax.cl relaxStrokesTilted): dt and alpha persist along the
+ *         stroke, dt >= dtmin_frac*dt0, F2SAFE added under the FIRE sqrt;
+ *         output force = last evaluated grid force rotated into the tip
+ *         frame, .w = energy. */
+typedef struct {
+    int mode;
+    int alg;                /* 1 = FIRE, 0 = damped leap-frog (mode 0 only) */
+    int max_iters;
+    int probe_start;        /* mode 0: -1, 0, 1 as relaxTipStroke */
+    double conv_f2, dt, damping, finc, fdec, falpha, dtmin_frac, f2safe;
+} ppafm_relax_params;
+
+/* Tip: rot rows A,B,C (identity for mode 0), d0 = equilibrium offset in the
+ * tip frame, kspring (3), krad, lrad, surf_ff (mode 1 constant force along C). */
+typedef struct {
+    double rot[9];
+    double d0[3];
+    double kspring[3];
+    double krad, lrad, surf_ff;
+} ppafm_tip;
+
+/* Scan: nx*ny columns of nz tip positions.  If tips_dev is non-NULL it holds
+ * explicit float64 positions (nx,ny,nz,3) in stroke order; otherwise the
+ * position of step s of column (i,j) is p0 + i*da + j*db + s*dstep. */
+typedef struct {
+    int nx, ny, nz;
+    double p0[3], da[3], db[3], dstep[3];
+} ppafm_scan;
+
+/* Outputs (device, any may be NULL): pos (nx,ny,nz,3) float64 or float32
+ * (pos_f32 != 0), force (nx,ny,nz,4) float32, iters (nx,ny,nz) int32,
+ * counters[2] uint64 accumulated: total force evaluations, total positions.
+ * Layout index ((i*ny + j)*nz + s) (column-major over the scan lattice
+ * matches (nx,ny,nz) C order). */
+typedef struct {
+    void *pos;
+    int pos_f32;
+    void *force;
+    int *iters;
+    unsigned long long *counters;
+} ppafm_relax_out;
+
+int ppafm_relax(const ppafm_grid *grid, const void *grid_dev, const ppafm_relax_params *rp, const ppafm_tip *tip,
+                const ppafm_scan *scan, const double *tips_dev, const ppafm_relax_out *out, void *stream);
+
+/* Valid-mode correlation along z for ncol columns (convolveZ, relax.cl:571-608;
+ * with reversed weights it is np.convolve(..., 'valid') of common.py:429):
+ *   out[c*nzout + j] = scale * sum_{k<nw} in[c*in_stride + (j+k)*in_step + in_offset] * w[k]
+ * with nzout = nz - nw + 1.  in/out are float32 (f64 = 0) or float64
+ * (f64 = 1) device arrays; w is a host array of nw <= 64 weights. */
+int ppafm_convolve_z(long ncol, int nz, int in_stride, int in_step, int in_offset, const void *in_dev, int nw,
+                     const double *w, double scale, void *out_dev, int f64, void *stream);
+
+/* io.limit_vec_field (io.py:766-778) in place on a float64 (npts,3) device
+ * array: vectors longer than fmax are scaled to length fmax. */
+int ppafm_limit_vec_field(double *F_dev, long npts, double fmax, void *stream);
+
+/* Memory helpers so Python callers need no other GPU runtime. */
+int ppafm_malloc(void **ptr, unsigned long long bytes);
+int ppafm_free(void *ptr);
+int ppafm_memcpy_h2d(void *dst, const void *src, unsigned long long bytes, void *stream);
+int ppafm_memcpy_d2h(void *dst, const void *src, unsigned long long bytes, void *stream);
+int ppafm_memset(void *dst, int value, unsigned long long bytes, void *stream);
+int ppafm_synchronize(void *stream);
+int ppafm_device_count(void);
+/* HIP events for timing (elapsed waits for ev1) and non-blocking streams of
+ * default priority. */
+int ppafm_event_create(void **ev);
+int ppafm_event_record(void *ev, void *stream);
+int ppafm_event_elapsed_ms(void *ev0, void *ev1, float *ms);
+int ppafm_event_destroy(void *ev);
+int ppafm_stream_create(void **stream);
+int ppafm_stream_destroy(void *stream);
+const char *ppafm_last_error(void);
+/* Clear the calling thread's last error (the reference-compatible void
+ * entry points report failures only through ppafm_last_error and stderr). */
+void ppafm_clear_error(void);
+/* Sum of iteration indices of the last relaxTipStrokes_omp call. */
+long long ppafm_last_itrsum(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PPAFM_HIP_H */

@@ -1,0 +1,615 @@
+/*
+ * pp_oracle.c -- FP64 CPU oracle for the probe-particle hot path.
+ *
+ * TEST INFRASTRUCTURE.  This file is the checker, never the product: only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
+ * the library built from it.  The product path (ppafm_amd) runs on the GPU
+ * and fails loudly when its HIP library is missing; it never calls in here.
+ *
+ * What it restates (semantics, written from scratch; file:line into the
+ * reference ppafm v0.4.0 tree):
+ *   - grid geometry   GridShape::setCell, Mat3::invert_T_to   Grid.h:29-36, Mat3.h:275-286
+ *   - LJ pair term    addAtomLJ                               Forces.h:120-132
+ *   - Coulomb pair    addAtomCoulomb (+ pz / dz2 stencils)    Forces.h:237-244, ProbeParticle.cpp:178-209
+ *   - grid iteration  iterateGrid3D_omp + evalCell            Grid.h:142-170, ProbeParticle.cpp:262-278
+ *   - force clamp     io.limit_vec_field                      io.py:766-778
+ *   - interpolation   SET_GRID_IND + interpolate3DvecWrap     Grid.h:66-111
+ *   - tip force       getPPforce + forceRSpring               ProbeParticle.cpp:282-294, Forces.h:85-89
+ *   - FIRE            FIREstate::move, relaxProbe             ProbeParticle.cpp:142-174, 297-313
+ *   - strokes         relaxTipStroke / relaxTipStrokes_omp    ProbeParticle.cpp:633-701
+ *   - OpenCL-mode grid  evalLJC_QZs_noPos (FP64, x-fastest)   ocl/cl/FF.cl:115-130, 366-423
+ *   - OpenCL-mode FIRE  relaxStrokesTilted, update_FIRE       ocl/cl/relax.cl:37-41, 90-126, 391-464
+ *   - z convolution     convolveZ (valid-mode correlation)    ocl/cl/relax.cl:571-608
+ *
+ * The operation order of every floating-point expression on the CPU-mode
+ * path follows the reference so that, built with gcc -O2 (no FMA
+ * contraction on x86-64 baseline), results agree with the compiled
+ * reference to the last bit.  Unlike the reference there is no global
+ * state: every call takes its grid, tip and relaxation parameters.
+ *
+ * Parity pins (tests/test_oracle_cpu.py): compiled reference C++ (oracle/_ref),
+ * committed golden fixtures, closed forms (single atom, affine fields,
+ * harmonic tip minimum, 1-D force balance by bisection), F = -grad E.
+ * The OpenCL-mode functions are pinned only by the mathematics (the
+ * reference OpenCL path cannot run here); see DESIGN.md.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* Forces.h:12 writes R2SAFE as the float literal 1.0e-8f; promoted to double
+ * it is 9.99999993922529e-09, which is what the reference adds to r^2. */
+#define ORC_R2SAFE ((double)1.0e-8f)
+
+typedef struct { double x, y, z; } v3;
+
+static inline double v3_dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline double v3_norm2(v3 a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
+
+/* ---------------------------------------------------------------- geometry */
+
+/* dcell[r] = cell[r]/n[r]; dinv = (dcell^-1)^T with the cofactor order of
+ * Mat3::invert_T_to (Mat3.h:254-286). Both 3x3 row-major, rows a,b,c. */
+void orc_grid_shape(const int n[3], const double cell[9], double dcell[9], double dinv[9])
+{
+    for (int r = 0; r < 3; r++) {
+        double s = 1.0 / n[r];
+        for (int c = 0; c < 3; c++) dcell[3 * r + c] = cell[3 * r + c] * s;
+    }
+    const double xx = dcell[0], xy = dcell[1], xz = dcell[2];
+    const double yx = dcell[3], yy = dcell[4], yz = dcell[5];
+    const double zx = dcell[6], zy = dcell[7], zz = dcell[8];
+    double c_xx = yy * zz - yz * zy;
+    double c_yx = yz * zx - yx * zz;
+    double c_zx = yx * zy - yy * zx;
+    double idet = 1 / (xx * c_xx + xy * c_yx + xz * c_zx);
+    dinv[0] = (yy * zz - yz * zy) * idet;  /* xx */
+    dinv[3] = (xz * zy - xy * zz) * idet;  /* yx */
+    dinv[6] = (xy * yz - xz * yy) * idet;  /* zx */
+    dinv[1] = (yz * zx - yx * zz) * idet;  /* xy */
+    dinv[4] = (xx * zz - xz * zx) * idet;  /* yy */
+    dinv[7] = (xz * yx - xx * yz) * idet;  /* zy */
+    dinv[2] = (yx * zy - yy * zx) * idet;  /* xz */
+    dinv[5] = (xy * zx - xx * zy) * idet;  /* yz */
+    dinv[8] = (xx * yy - xy * yx) * idet;  /* zz */
+}
+
+/* ---------------------------------------------------------- pair functions */
+
+static inline double pair_lj(v3 d, v3 *f, double c6, double c12)
+{
+    double ir2 = 1.0 / (v3_norm2(d) + ORC_R2SAFE);
+    double ir6 = ir2 * ir2 * ir2;
+    double e6 = c6 * ir6;
+    double e12 = c12 * ir6 * ir6;
+    double s = (12 * e12 - 6 * e6) * ir2;
+    f->x += d.x * s; f->y += d.y * s; f->z += d.z * s;
+    return e12 - e6;
+}
+
+static inline double pair_coul(v3 d, v3 *f, double kqq)
+{
+    double ir2 = 1.0 / (v3_norm2(d) + ORC_R2SAFE);
+    double ir = sqrt(ir2);
+    double e = ir * kqq;
+    double s = e * ir2;
+    f->x += d.x * s; f->y += d.y * s; f->z += d.z * s;
+    return e;
+}
+
+/* Finite-difference multipole probes, dstep 0.1 A (ProbeParticle.cpp:178-209). */
+static inline double pair_coul_kind(v3 d, v3 *fout, double kqq, int kind)
+{
+    if (kind == 0) return pair_coul(d, fout, kqq);
+    v3 f = {0, 0, 0};
+    double e = 0;
+    if (kind == 1) {
+        d.z -= 0.1;     e += pair_coul(d, &f, -kqq);
+        d.z += 2 * 0.1; e += pair_coul(d, &f, +kqq);
+        fout->x += f.x * 10.0; fout->y += f.y * 10.0; fout->z += f.z * 10.0;
+        return e * 10.0;
+    }
+    e += pair_coul(d, &f, -2 * kqq);
+    d.z -= 0.1;     e += pair_coul(d, &f, kqq);
+    d.z += 2 * 0.1; e += pair_coul(d, &f, kqq);
+    fout->x += f.x * 100.0; fout->y += f.y * 100.0; fout->z += f.z * 100.0;
+    return e * 100.0;
+}
+
+/* ------------------------------------------------------------- grid build */
+
+/* model 0 = LJ (coefs c6,c12 per atom), 1/2/3 = Coulomb kind 0/1/2 (coef kqq).
+ * Accumulates (+=) into F[(ix*ny+iy)*nz+iz][3] and E[...] where non-NULL. */
+static void grid_eval(int model, const int n[3], const double cell[9], int natoms,
+                      const double *Rs, const double *coefs, double *F, double *E)
+{
+    double dcell[9], dinv[9];
+    orc_grid_shape(n, cell, dcell, dinv);
+    const int nx = n[0], ny = n[1], nz = n[2];
+    #pragma omp parallel for collapse(3) schedule(static)
+    for (int ia = 0; ia < nx; ia++)
+        for (int ib = 0; ib < ny; ib++)
+            for (int ic = 0; ic < nz; ic++) {
+                v3 p;
+                p.x = dcell[0] * ia + dcell[3] * ib + dcell[6] * ic;
+                p.y = dcell[1] * ia + dcell[4] * ib + dcell[7] * ic;
+                p.z = dcell[2] * ia + dcell[5] * ib + dcell[8] * ic;
+                v3 f = {0, 0, 0};
+                double e = 0;
+                for (int i = 0; i < natoms; i++) {
+                    v3 d = {p.x - Rs[3 * i], p.y - Rs[3 * i + 1], p.z - Rs[3 * i + 2]};
+                    if (model == 0) e += pair_lj(d, &f, coefs[2 * i], coefs[2 * i + 1]);
+                    else            e += pair_coul_kind(d, &f, coefs[i], model - 1);
+                }
+                long ig = ((long)ia * ny + ib) * nz + ic;
+                if (F) { F[3 * ig] += f.x; F[3 * ig + 1] += f.y; F[3 * ig + 2] += f.z; }
+                if (E) E[ig] += e;
+            }
+}
+
+/* ProbeParticle.cpp:508-513; Rs already shifted by -lvec[0] (HighLevel.py:277). */
+void orc_lj_ff(const int n[3], const double cell[9], int natoms, const double *Rs,
+               const double *cLJs, double *F, double *E)
+{
+    grid_eval(0, n, cell, natoms, Rs, cLJs, F, E);
+}
+
+/* ProbeParticle.cpp:540-554; kQQs = Qs * 14.399645 for a unit probe charge. */
+void orc_coulomb_ff(const int n[3], const double cell[9], int natoms, const double *Rs,
+                    const double *kQQs, int kind, double *F, double *E)
+{
+    if (kind < 0 || kind > 2) return;
+    grid_eval(1 + kind, n, cell, natoms, Rs, kQQs, F, E);
+}
+
+/* io.py:766-778: scale vectors longer than Fmax to length Fmax, in place. */
+void orc_limit_vec_field(long npts, double *F, double fmax)
+{
+    #pragma omp parallel for schedule(static)
+    for (long i = 0; i < npts; i++) {
+        double fr = sqrt(F[3 * i] * F[3 * i] + F[3 * i + 1] * F[3 * i + 1] + F[3 * i + 2] * F[3 * i + 2]);
+        if (fr > fmax) {
+            double s = fmax / fr;
+            F[3 * i] *= s; F[3 * i + 1] *= s; F[3 * i + 2] *= s;
+        }
+    }
+}
+
+/* ---------------------------------------------------------- interpolation */
+
+/* One axis of SET_GRID_IND (Grid.h:66-69): lower index lo, upper index hi,
+ * weights wlo (for lo) and whi (for hi), correct for negative coordinates. */
+static inline void wrap_axis(double r, int n, int *lo, int *hi, double *wlo, double *whi)
+{
+    int i;
+    if (r >= 0) {
+        i = (int)r;
+        *whi = r - i;
+        *wlo = 1 - *whi;
+        i %= n;
+    } else {
+        int t = (int)r;
+        *wlo = t - r;
+        *whi = 1 - *wlo;
+        i = t % n + n - 1;
+    }
+    *lo = i;
+    *hi = (i + 1) % n;
+}
+
+/* interpolate3DvecWrap (Grid.h:93-111), same corner order. g = grid coords. */
+static inline v3 interp_wrap(const double *F, const int n[3], v3 g)
+{
+    int ix, jx, iy, jy, iz, jz;
+    double mx, tx, my, ty, mz, tz;
+    wrap_axis(g.x, n[0], &ix, &jx, &mx, &tx);
+    wrap_axis(g.y, n[1], &iy, &jy, &my, &ty);
+    wrap_axis(g.z, n[2], &iz, &jz, &mz, &tz);
+    const long ny = n[1], nz = n[2];
+#define G3(a, b, c) (F + 3 * (((long)(a) * ny + (b)) * nz + (c)))
+    double mxmy = mx * my, mxty = mx * ty, txmy = tx * my, txty = tx * ty;
+    const double *c[8] = {G3(ix, iy, iz), G3(ix, iy, jz), G3(ix, jy, iz), G3(ix, jy, jz),
+                          G3(jx, jy, iz), G3(jx, jy, jz), G3(jx, iy, iz), G3(jx, iy, jz)};
+    const double w[8] = {mxmy * mz, mxmy * tz, mxty * mz, mxty * tz,
+                         txty * mz, txty * tz, txmy * mz, txmy * tz};
+#undef G3
+    v3 o = {c[0][0] * w[0], c[0][1] * w[0], c[0][2] * w[0]};
+    for (int k = 1; k < 8; k++) { o.x += c[k][0] * w[k]; o.y += c[k][1] * w[k]; o.z += c[k][2] * w[k]; }
+    return o;
+}
+
+/* Periodic trilinear sample at Cartesian r (grid origin at 0). */
+void orc_interp(const int n[3], const double cell[9], const double *F, int npts,
+                const double *rs, double *out)
+{
+    double dcell[9], dinv[9];
+    orc_grid_shape(n, cell, dcell, dinv);
+    for (int i = 0; i < npts; i++) {
+        v3 r = {rs[3 * i], rs[3 * i + 1], rs[3 * i + 2]};
+        v3 g = {v3_dot(r, (v3){dinv[0], dinv[1], dinv[2]}),
+                v3_dot(r, (v3){dinv[3], dinv[4], dinv[5]}),
+                v3_dot(r, (v3){dinv[6], dinv[7], dinv[8]})};
+        v3 f = interp_wrap(F, n, g);
+        out[3 * i] = f.x; out[3 * i + 1] = f.y; out[3 * i + 2] = f.z;
+    }
+}
+
+/* ------------------------------------------------------------ relaxation */
+
+/* Tip: lRadial, kRadial, rPP0 (already on the sphere, TIP::makeConsistent,
+ * ProbeParticle.cpp:64-69), kSpring; all in eV/A units. */
+typedef struct {
+    double lrad, krad;
+    double rpp0[3];
+    double kspring[3];
+} orc_tip;
+
+/* RELAX + FIRE namespaces (ProbeParticle.cpp:74-114). */
+typedef struct {
+    int max_iters;
+    double conv_f2, dt, damping;
+    double finc, fdec, falpha;
+} orc_relax;
+
+typedef struct {
+    const double *F;
+    int n[3];
+    double dinv[9];
+} orc_field;
+
+/* getPPforce (ProbeParticle.cpp:282-294), harmonic radial spring. */
+static inline v3 pp_force(const orc_field *ff, const orc_tip *tip, v3 rtip, v3 r)
+{
+    v3 g = {v3_dot(r, (v3){ff->dinv[0], ff->dinv[1], ff->dinv[2]}),
+            v3_dot(r, (v3){ff->dinv[3], ff->dinv[4], ff->dinv[5]}),
+            v3_dot(r, (v3){ff->dinv[6], ff->dinv[7], ff->dinv[8]})};
+    v3 d = {r.x - rtip.x, r.y - rtip.y, r.z - rtip.z};
+    v3 f = interp_wrap(ff->F, ff->n, g);
+    double l = sqrt(v3_norm2(d));
+    double s = tip->krad * (l - tip->lrad) / l;
+    f.x += d.x * s; f.y += d.y * s; f.z += d.z * s;
+    d.x -= tip->rpp0[0]; d.y -= tip->rpp0[1]; d.z -= tip->rpp0[2];
+    f.x += d.x * tip->kspring[0]; f.y += d.y * tip->kspring[1]; f.z += d.z * tip->kspring[2];
+    return f;
+}
+
+/* relaxProbe (ProbeParticle.cpp:297-313) with FIREstate::move (142-174) for
+ * alg 1, damped leap-frog RELAX::move (84-88) for alg 0.  Returns the
+ * iteration index at which it stopped, like the reference; *nevals gets the
+ * number of force evaluations made. */
+static int relax_probe(const orc_field *ff, const orc_tip *tip, const orc_relax *rp, int alg,
+                       v3 rtip, v3 *r, int *nevals)
+{
+    v3 v = {0, 0, 0};
+    const double dtmax = rp->dt, a0 = rp->damping;
+    double dt = dtmax, acoef = a0;
+    int it;
+    *nevals = 0;
+    for (it = 0; it < rp->max_iters; it++) {
+        v3 f = pp_force(ff, tip, rtip, *r);
+        (*nevals)++;
+        if (alg == 1) {
+            double ff2 = v3_norm2(f), vv = v3_norm2(v), vf = v3_dot(f, v);
+            if (vf < 0) {
+                v.x = 0; v.y = 0; v.z = 0;
+                dt = dt * rp->fdec;
+                acoef = a0;
+            } else {
+                double cf = acoef * sqrt(vv / ff2);
+                double cv = 1 - acoef;
+                v.x *= cv; v.y *= cv; v.z *= cv;
+                v.x += f.x * cf; v.y += f.y * cf; v.z += f.z * cf;
+                dt = fmin(dt * rp->finc, dtmax);
+                acoef = acoef * rp->falpha;
+            }
+            v.x += f.x * dt; v.y += f.y * dt; v.z += f.z * dt;
+            r->x += v.x * dt; r->y += v.y * dt; r->z += v.z * dt;
+        } else {
+            double c = 1 - rp->damping;
+            v.x *= c; v.y *= c; v.z *= c;
+            v.x += f.x * rp->dt; v.y += f.y * rp->dt; v.z += f.z * rp->dt;
+            r->x += v.x * rp->dt; r->y += v.y * rp->dt; r->z += v.z * rp->dt;
+        }
+        if (v3_norm2(f) < rp->conv_f2) break;
+    }
+    return it;
+}
+
+/* relaxTipStroke (ProbeParticle.cpp:633-676) for one column of nstep tip
+ * positions.  iters[i] (optional) = iteration index returned per step,
+ * nevals[i] (optional) = force evaluations of FIRE per step (the final
+ * sample evaluation not included).  Returns the sum of iters. */
+static int relax_stroke(const orc_field *ff, const orc_tip *tip, const orc_relax *rp,
+                        int probe_start, int alg, int nstep, const double *rtips,
+                        double *rs, double *fs, int *iters, int *nevals)
+{
+    v3 rtip = {rtips[0], rtips[1], rtips[2]};
+    v3 rp_ = {rtip.x + tip->rpp0[0], rtip.y + tip->rpp0[1], rtip.z + tip->rpp0[2]};
+    int itrsum = 0;
+    for (int i = 0; i < nstep; i++) {
+        v3 nt = {rtips[3 * i], rtips[3 * i + 1], rtips[3 * i + 2]};
+        if (probe_start == -1) {
+            rtip = nt;
+        } else if (probe_start == 0) {
+            rtip = nt;
+            rp_.x = rtip.x + tip->rpp0[0]; rp_.y = rtip.y + tip->rpp0[1]; rp_.z = rtip.z + tip->rpp0[2];
+        } else if (probe_start == 1) {
+            v3 drp = {rp_.x - rtip.x, rp_.y - rtip.y, rp_.z - rtip.z};
+            rtip = nt;
+            rp_.x = rtip.x + drp.x; rp_.y = rtip.y + drp.y; rp_.z = rtip.z + drp.z;
+        }
+        int ne = 0;
+        int itr = relax_probe(ff, tip, rp, alg, rtip, &rp_, &ne);
+        v3 g = {v3_dot(rp_, (v3){ff->dinv[0], ff->dinv[1], ff->dinv[2]}),
+                v3_dot(rp_, (v3){ff->dinv[3], ff->dinv[4], ff->dinv[5]}),
+                v3_dot(rp_, (v3){ff->dinv[6], ff->dinv[7], ff->dinv[8]})};
+        v3 fsamp = interp_wrap(ff->F, ff->n, g);
+        rs[3 * i] = rp_.x; rs[3 * i + 1] = rp_.y; rs[3 * i + 2] = rp_.z;
+        fs[3 * i] = fsamp.x; fs[3 * i + 1] = fsamp.y; fs[3 * i + 2] = fsamp.z;
+        if (iters) iters[i] = itr;
+        if (nevals) nevals[i] = ne;
+        itrsum += itr;
+    }
+    return itrsum;
+}
+
+static void make_field(orc_field *ff, const int n[3], const double cell[9], const double *F)
+{
+    double dcell[9];
+    ff->F = F;
+    ff->n[0] = n[0]; ff->n[1] = n[1]; ff->n[2] = n[2];
+    orc_grid_shape(n, cell, dcell, ff->dinv);
+}
+
+static void make_tip(orc_tip *t, const double tip[8])
+{
+    /* tip = {lRadial, kRadial, rPP0x, rPP0y, rPP0z, kSx, kSy, kSz}, rPP0 as given
+     * to setTip; the z component is put on the sphere as TIP::makeConsistent does. */
+    t->lrad = tip[0]; t->krad = tip[1];
+    t->rpp0[0] = tip[2]; t->rpp0[1] = tip[3]; t->rpp0[2] = tip[4];
+    t->kspring[0] = tip[5]; t->kspring[1] = tip[6]; t->kspring[2] = tip[7];
+    if (fabs(t->krad) > 1e-8)
+        t->rpp0[2] = -sqrt(t->lrad * t->lrad - t->rpp0[0] * t->rpp0[0] - t->rpp0[1] * t->rpp0[1]);
+}
+
+static void make_relax(orc_relax *r, const double relax[7])
+{
+    /* relax = {maxIters, convF2, dt, damping, finc, fdec, falpha} */
+    r->max_iters = (int)relax[0];
+    r->conv_f2 = relax[1]; r->dt = relax[2]; r->damping = relax[3];
+    r->finc = relax[4]; r->fdec = relax[5]; r->falpha = relax[6];
+}
+
+/* One stroke (relaxTipStroke).  Returns sum of iteration indices. */
+int orc_relax_stroke(const int n[3], const double cell[9], const double *F, const double tip[8],
+                     const double relax[7], int probe_start, int alg, int nstep,
+                     const double *rtips, double *rs, double *fs, int *iters, int *nevals)
+{
+    orc_field ff; orc_tip t; orc_relax r;
+    make_field(&ff, n, cell, F); make_tip(&t, tip); make_relax(&r, relax);
+    return relax_stroke(&ff, &t, &r, probe_start, alg, nstep, rtips, rs, fs, iters, nevals);
+}
+
+/* relaxTipStrokes_omp (ProbeParticle.cpp:682-701): rTips/rs/fs are
+ * (nx,ny,nz,3) C order; stroke offset (ix + iy*nx)*nz as in the reference
+ * (a bijection of the columns).  iters/nevals optional (nx*ny*nz). Returns
+ * the total iteration sum (the reference returns 0; the sum is additive). */
+long orc_relax_strokes(int nx, int ny, int nz, const int n[3], const double cell[9], const double *F,
+                       const double tip[8], const double relax[7], int probe_start, int alg,
+                       const double *rtips, double *rs, double *fs, int *iters, int *nevals)
+{
+    orc_field ff; orc_tip t; orc_relax r;
+    make_field(&ff, n, cell, F); make_tip(&t, tip); make_relax(&r, relax);
+    long total = 0;
+    #pragma omp parallel for collapse(2) schedule(static) reduction(+ : total)
+    for (int ix = 0; ix < nx; ix++)
+        for (int iy = 0; iy < ny; iy++) {
+            long off = ((long)ix + (long)iy * nx) * nz;
+            total += relax_stroke(&ff, &t, &r, probe_start, alg, nz, rtips + 3 * off, rs + 3 * off,
+                                  fs + 3 * off, iters ? iters + off : 0, nevals ? nevals + off : 0);
+        }
+    return total;
+}
+
+/* Same strokes over a subset of columns (list of (ix,iy) pairs) -- used to
+ * time a bounded sample of a large scan. */
+long orc_relax_strokes_subset(int ncols, const int *cols, int nx, int nz, const int n[3],
+                              const double cell[9], const double *F, const double tip[8],
+                              const double relax[7], int probe_start, int alg,
+                              const double *rtips, double *rs, double *fs, int *iters)
+{
+    orc_field ff; orc_tip t; orc_relax r;
+    make_field(&ff, n, cell, F); make_tip(&t, tip); make_relax(&r, relax);
+    long total = 0;
+    #pragma omp parallel for schedule(dynamic, 4) reduction(+ : total)
+    for (int c = 0; c < ncols; c++) {
+        long off = ((long)cols[2 * c] + (long)cols[2 * c + 1] * nx) * nz;
+        total += relax_stroke(&ff, &t, &r, probe_start, alg, nz, rtips + 3 * off, rs + 3 * off,
+                              fs + 3 * off, iters ? iters + off : 0, 0);
+    }
+    return total;
+}
+
+/* ------------------------------------------------------- OpenCL-mode grid */
+
+/* evalLJC_QZs_noPos (FF.cl:366-423) restated in FP64: LJ (getLJ, FF.cl:123-130)
+ * plus four z-offset point charges of the tip (getCoulomb, FF.cl:115-121),
+ * R2SAFE = 1e-6, Coulomb constant 14.399644, no clamp.  Output FE is
+ * (nx*ny*nz, 4) with x fastest: index ia + nx*(ib + ny*ic).  atoms is
+ * (N,4) xyzq in the grid frame *before* subtracting the origin p0. */
+void orc_ljc_qzs_ff(const int n[3], const double p0[3], const double dA[3], const double dB[3],
+                    const double dC[3], int natoms, const double *atoms, const double *cLJs,
+                    const double Qs[4], const double QZs[4], double *FE)
+{
+    const int nx = n[0], ny = n[1], nz = n[2];
+    const double r2safe = (double)1e-6f, kc = (double)14.399644f;
+    double q[4];
+    for (int k = 0; k < 4; k++) q[k] = Qs[k] * kc;
+    #pragma omp parallel for collapse(3) schedule(static)
+    for (int ic = 0; ic < nz; ic++)
+        for (int ib = 0; ib < ny; ib++)
+            for (int ia = 0; ia < nx; ia++) {
+                double px = p0[0] + dA[0] * ia + dB[0] * ib + dC[0] * ic;
+                double py = p0[1] + dA[1] * ia + dB[1] * ib + dC[1] * ic;
+                double pz = p0[2] + dA[2] * ia + dB[2] * ib + dC[2] * ic;
+                double fx = 0, fy = 0, fz = 0, e = 0;
+                for (int i = 0; i < natoms; i++) {
+                    const double *a = atoms + 4 * i;
+                    double dx = px - a[0], dy = py - a[1], dz = pz - a[2];
+                    double ir2 = 1.0 / (dx * dx + dy * dy + dz * dz + r2safe);
+                    double ir6 = ir2 * ir2 * ir2;
+                    double c6 = cLJs[2 * i], c12 = cLJs[2 * i + 1];
+                    e += (c12 * ir6 - c6) * ir6;
+                    double s = (12.0 * c12 * ir6 - 6.0 * c6) * ir6 * ir2;
+                    fx += s * dx; fy += s * dy; fz += s * dz;
+                    for (int k = 0; k < 4; k++) {
+                        if (q[k] == 0) continue;
+                        double dzk = dz + QZs[k];
+                        double jr2 = 1.0 / (dx * dx + dy * dy + dzk * dzk + r2safe);
+                        double ek = a[3] * sqrt(jr2);
+                        double sk = ek * jr2 * q[k];
+                        fx += sk * dx; fy += sk * dy; fz += sk * dzk;
+                        e += ek * q[k];
+                    }
+                }
+                long ig = (long)ia + (long)nx * ((long)ib + (long)ny * ic);
+                FE[4 * ig] = fx; FE[4 * ig + 1] = fy; FE[4 * ig + 2] = fz; FE[4 * ig + 3] = e;
+            }
+}
+
+/* -------------------------------------------------- OpenCL-mode relaxation */
+
+/* relaxStrokesTilted (relax.cl:391-464) restated in FP64 with exact software
+ * periodic trilinear interpolation in place of the hardware image filter
+ * (interpFE, relax.cl:64-73: normalized coordinates + half texel with
+ * ADDRESS_REPEAT and LINEAR filtering samples grid points floor(u) and
+ * floor(u)+1 of u = pos.dinv, i.e. the same corners and weights as
+ * Grid.h:66-111, but the hardware rounds the weights).
+ *
+ * FE grid: (nx,ny,nz,4) with x fastest (evalLJC_QZs_noPos layout).
+ * poss: (npix,4) tip anchor positions (relative to the grid origin).
+ * tipRot rows A,B,C (3x4; C[3] = -zstep), stiffness (4), dpos0 (4, tip frame),
+ * relax_params (dt, damp), surfFF (x used), maxIters, convF2, F2SAFE.
+ * Outputs FEs and paths (npix*nz, 4).  nevals (npix*nz, optional). */
+static inline void fe_interp_xfast(const double *FE, const int n[3], double gx, double gy, double gz,
+                                   double out[4])
+{
+    int i0[3], i1[3];
+    double w0[3], w1[3];
+    double g[3] = {gx, gy, gz};
+    for (int a = 0; a < 3; a++) {
+        double fl = floor(g[a]);
+        double t = g[a] - fl;
+        long il = (long)fl % n[a];
+        if (il < 0) il += n[a];
+        i0[a] = (int)il;
+        i1[a] = (int)((il + 1) % n[a]);
+        w0[a] = 1 - t; w1[a] = t;
+    }
+    for (int c = 0; c < 4; c++) out[c] = 0;
+    for (int k = 0; k < 8; k++) {
+        int ix = (k & 1) ? i1[0] : i0[0];
+        int iy = (k & 2) ? i1[1] : i0[1];
+        int iz = (k & 4) ? i1[2] : i0[2];
+        double w = ((k & 1) ? w1[0] : w0[0]) * ((k & 2) ? w1[1] : w0[1]) * ((k & 4) ? w1[2] : w0[2]);
+        const double *p = FE + 4 * ((long)ix + (long)n[0] * ((long)iy + (long)n[1] * iz));
+        for (int c = 0; c < 4; c++) out[c] += w * p[c];
+    }
+}
+
+void orc_relax_strokes_tilted(int npix, int nz, const int n[3], const double *FE, const double dinv[9],
+                              const double tipRot[12], const double stiffness[4], const double dpos0[4],
+                              const double relax_params[2], double surf_ff, int max_iters,
+                              double conv_f2, double f2safe, const double *poss, double *FEs,
+                              double *paths, int *nevals)
+{
+    #pragma omp parallel for schedule(dynamic, 16)
+    for (int ip = 0; ip < npix; ip++) {
+        const double *A = tipRot, *B = tipRot + 4, *C = tipRot + 8;
+        double dtip[3] = {C[0] * C[3], C[1] * C[3], C[2] * C[3]};
+        /* dpos0_ = rotMatT(dpos0) = A*x + B*y + C*z */
+        double d0[3];
+        for (int c = 0; c < 3; c++) d0[c] = A[c] * dpos0[0] + B[c] * dpos0[1] + C[c] * dpos0[2];
+        double tp[3] = {poss[4 * ip], poss[4 * ip + 1], poss[4 * ip + 2]};
+        double pos[3] = {tp[0] + d0[0], tp[1] + d0[1], tp[2] + d0[2]};
+        double dt = relax_params[0], damp = relax_params[1];
+        const double dtmax = dt, dtmin = dtmax * 0.1, damp0 = damp;
+        for (int iz = 0; iz < nz; iz++) {
+            double fe[4] = {0, 0, 0, 0};
+            double v[3] = {0, 0, 0};
+            int ne = 0;
+            for (int i = 0; i < max_iters; i++) {
+                double gx = pos[0] * dinv[0] + pos[1] * dinv[1] + pos[2] * dinv[2];
+                double gy = pos[0] * dinv[3] + pos[1] * dinv[4] + pos[2] * dinv[5];
+                double gz = pos[0] * dinv[6] + pos[1] * dinv[7] + pos[2] * dinv[8];
+                fe_interp_xfast(FE, n, gx, gy, gz, fe);
+                ne++;
+                double f[3] = {fe[0], fe[1], fe[2]};
+                double dp[3] = {pos[0] - tp[0], pos[1] - tp[1], pos[2] - tp[2]};
+                double dq[3] = {dp[0] * A[0] + dp[1] * A[1] + dp[2] * A[2],
+                                dp[0] * B[0] + dp[1] * B[1] + dp[2] * B[2],
+                                dp[0] * C[0] + dp[1] * C[1] + dp[2] * C[2]};
+                double rl = sqrt(dq[0] * dq[0] + dq[1] * dq[1] + dq[2] * dq[2]);
+                double sr = stiffness[3] * (rl - dpos0[3]) / rl;
+                double ft[3];
+                for (int c = 0; c < 3; c++) ft[c] = (dq[c] - dpos0[c]) * stiffness[c] + dq[c] * sr;
+                for (int c = 0; c < 3; c++) f[c] += A[c] * ft[0] + B[c] * ft[1] + C[c] * ft[2];
+                for (int c = 0; c < 3; c++) f[c] += C[c] * surf_ff;
+                /* update_FIRE (relax.cl:105-126) */
+                double ff2 = f[0] * f[0] + f[1] * f[1] + f[2] * f[2];
+                double vv = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+                double vf = v[0] * f[0] + v[1] * f[1] + v[2] * f[2];
+                if (vf < 0) {
+                    v[0] = v[1] = v[2] = 0;
+                    dt = fmax(dtmin, dt * 0.5);
+                    damp = damp0;
+                } else {
+                    double cf = damp * sqrt(vv / (ff2 + f2safe));
+                    for (int c = 0; c < 3; c++) v[c] = v[c] * (1 - damp) + f[c] * cf;
+                    dt = fmin(dtmax, dt * 1.1);
+                    damp *= 0.99;
+                }
+                for (int c = 0; c < 3; c++) { v[c] += f[c] * dt; pos[c] += v[c] * dt; }
+                if (ff2 < conv_f2) break;
+            }
+            long ind = (long)ip * nz + iz;
+            FEs[4 * ind + 0] = fe[0] * A[0] + fe[1] * A[1] + fe[2] * A[2];
+            FEs[4 * ind + 1] = fe[0] * B[0] + fe[1] * B[1] + fe[2] * B[2];
+            FEs[4 * ind + 2] = fe[0] * C[0] + fe[1] * C[1] + fe[2] * C[2];
+            FEs[4 * ind + 3] = fe[3];
+            paths[4 * ind + 0] = pos[0]; paths[4 * ind + 1] = pos[1]; paths[4 * ind + 2] = pos[2];
+            paths[4 * ind + 3] = 0;
+            if (nevals) nevals[ind] = ne;
+            for (int c = 0; c < 3; c++) { tp[c] += dtip[c]; pos[c] += dtip[c]; }
+        }
+    }
+}
+
+/* ------------------------------------------------------------ z convolution */
+
+/* Valid-mode correlation along the last axis: out[j] = sum_k in[j+k] * w[k],
+ * nzout = nzin - nw + 1 (convolveZ, relax.cl:571-608, which is the same as
+ * numpy.convolve(in, w[::-1], 'valid')).  ncol columns, stride nzin / nzout. */
+void orc_convolve_z(long ncol, int nzin, int nw, const double *in, const double *w, double *out)
+{
+    int nzout = nzin - nw + 1;
+    #pragma omp parallel for schedule(static)
+    for (long c = 0; c < ncol; c++)
+        for (int j = 0; j < nzout; j++) {
+            double s = 0;
+            for (int k = 0; k < nw; k++) s += in[c * nzin + j + k] * w[k];
+            out[c * nzout + j] = s;
+        }
+}
+
+int orc_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}

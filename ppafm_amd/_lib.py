@@ -1,0 +1,155 @@
+"""ctypes binding of libppafm_hip.so (declarations in include/ppafm_hip.h).
+
+The library is the product path: importing this module fails loudly when it
+has not been built (``python -m ppafm_amd.build``); nothing falls back to CPU.
+"""
+
+import ctypes
+import pathlib
+
+import numpy as np
+
+LIB_PATH = pathlib.Path(__file__).resolve().parent / "libppafm_hip.so"
+
+c_int, c_double, c_long, c_void_p = ctypes.c_int, ctypes.c_double, ctypes.c_long, ctypes.c_void_p
+c_ull = ctypes.c_ulonglong
+_dp = ctypes.POINTER(c_double)
+_ip = ctypes.POINTER(c_int)
+
+
+class SplineParams(ctypes.Structure):
+    _fields_ = [("rff_n", c_int), ("rff_xs", _dp), ("rff_ydys", _dp)]
+
+
+class Grid(ctypes.Structure):
+    _fields_ = [("n", c_int * 3), ("cell", c_double * 9)]
+
+
+class FFModel(ctypes.Structure):
+    _fields_ = [
+        ("mode", c_int),
+        ("with_lj", c_int),
+        ("n_stencil", c_int),
+        ("with_energy", c_int),
+        ("stencil_w", c_double * 4),
+        ("stencil_dz", c_double * 4),
+        ("coulomb_const", c_double),
+        ("probe_q", c_double),
+        ("fmax", c_double),
+    ]
+
+
+class Atom(ctypes.Structure):
+    _fields_ = [("x", c_double), ("y", c_double), ("z", c_double), ("q", c_double), ("c6", c_double), ("c12", c_double)]
+
+
+class RelaxParams(ctypes.Structure):
+    _fields_ = [
+        ("mode", c_int),
+        ("alg", c_int),
+        ("max_iters", c_int),
+        ("probe_start", c_int),
+        ("conv_f2", c_double),
+        ("dt", c_double),
+        ("damping", c_double),
+        ("finc", c_double),
+        ("fdec", c_double),
+        ("falpha", c_double),
+        ("dtmin_frac", c_double),
+        ("f2safe", c_double),
+    ]
+
+
+class Tip(ctypes.Structure):
+    _fields_ = [("rot", c_double * 9), ("d0", c_double * 3), ("kspring", c_double * 3), ("krad", c_double), ("lrad", c_double), ("surf_ff", c_double)]
+
+
+class Scan(ctypes.Structure):
+    _fields_ = [("nx", c_int), ("ny", c_int), ("nz", c_int), ("p0", c_double * 3), ("da", c_double * 3), ("db", c_double * 3), ("dstep", c_double * 3)]
+
+
+class RelaxOut(ctypes.Structure):
+    _fields_ = [("pos", c_void_p), ("pos_f32", c_int), ("force", c_void_p), ("iters", c_void_p), ("counters", c_void_p)]
+
+
+# (name, restype, argtypes) of every exported symbol declared in the header
+SIGNATURES = [
+    ("setGridN", None, [_ip]),
+    ("setGridCell", None, [_dp]),
+    ("setFF_Fpointer", None, [_dp]),
+    ("setFF_Epointer", None, [_dp]),
+    ("deleteFF_Fpointer", None, []),
+    ("deleteFF_Epointer", None, []),
+    ("setTip", None, [c_double, c_double, _dp, _dp]),
+    ("setRelax", None, [c_int, c_double, c_double, c_double]),
+    ("setFIRE", None, [c_double, c_double, c_double]),
+    ("getLennardJonesFF", None, [c_int, _dp, _dp]),
+    ("getCoulombFF", None, [c_int, _dp, _dp, c_int]),
+    ("relaxTipStroke", c_int, [c_int, c_int, c_int, _dp, _dp, _dp, ctypes.POINTER(SplineParams)]),
+    ("relaxTipStrokes_omp", c_int, [c_int, c_int, c_int, c_int, c_int, _dp, _dp, _dp, ctypes.POINTER(SplineParams)]),
+    ("ppafm_pack_atoms", c_int, [c_int, _dp, _dp, c_double, c_void_p]),
+    ("ppafm_ff_build_dev", c_int, [ctypes.POINTER(Grid), ctypes.POINTER(FFModel), c_int, c_void_p, c_void_p, c_void_p]),
+    ("ppafm_ff_build", c_int, [ctypes.POINTER(Grid), ctypes.POINTER(FFModel), c_int, _dp, _dp, c_void_p, c_void_p]),
+    (
+        "ppafm_relax",
+        c_int,
+        [ctypes.POINTER(Grid), c_void_p, ctypes.POINTER(RelaxParams), ctypes.POINTER(Tip), ctypes.POINTER(Scan), c_void_p, ctypes.POINTER(RelaxOut), c_void_p],
+    ),
+    ("ppafm_convolve_z", c_int, [c_long, c_int, c_int, c_int, c_int, c_void_p, c_int, _dp, c_double, c_void_p, c_int, c_void_p]),
+    ("ppafm_limit_vec_field", c_int, [c_void_p, c_long, c_double, c_void_p]),
+    ("ppafm_malloc", c_int, [ctypes.POINTER(c_void_p), c_ull]),
+    ("ppafm_free", c_int, [c_void_p]),
+    ("ppafm_memcpy_h2d", c_int, [c_void_p, c_void_p, c_ull, c_void_p]),
+    ("ppafm_memcpy_d2h", c_int, [c_void_p, c_void_p, c_ull, c_void_p]),
+    ("ppafm_memset", c_int, [c_void_p, c_int, c_ull, c_void_p]),
+    ("ppafm_synchronize", c_int, [c_void_p]),
+    ("ppafm_device_count", c_int, []),
+    ("ppafm_event_create", c_int, [ctypes.POINTER(c_void_p)]),
+    ("ppafm_event_record", c_int, [c_void_p, c_void_p]),
+    ("ppafm_event_elapsed_ms", c_int, [c_void_p, c_void_p, ctypes.POINTER(ctypes.c_float)]),
+    ("ppafm_event_destroy", c_int, [c_void_p]),
+    ("ppafm_stream_create", c_int, [ctypes.POINTER(c_void_p)]),
+    ("ppafm_stream_destroy", c_int, [c_void_p]),
+    ("ppafm_last_error", ctypes.c_char_p, []),
+    ("ppafm_clear_error", None, []),
+    ("ppafm_last_itrsum", ctypes.c_longlong, []),
+]
+
+_lib = None
+
+
+class HipError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise HipError(f"libppafm_hip.so is not built ({LIB_PATH}); run `python -m ppafm_amd.build`")
+        L = ctypes.CDLL(str(LIB_PATH))
+        for name, res, args in SIGNATURES:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc, what=""):
+    if rc != 0:
+        msg = lib().ppafm_last_error().decode(errors="replace")
+        raise HipError(f"{what} failed ({rc}): {msg}")
+
+
+def dptr(a):
+    """ctypes double* of a C-contiguous float64 numpy array (or None)."""
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def iptr(a):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_ip)

@@ -1,0 +1,95 @@
+// Valid-mode z correlation for Fz -> df.
+//
+// convolveZ (ppafm/ocl/cl/relax.cl:571-608) correlates each pixel column of
+// the force stack with the df weights; Fz2df (ppafm/common.py:421-430) is
+// the same with the weights reversed (np.convolve 'valid').  One thread per
+// output value, weights in the kernel argument block, FP64 accumulation.
+// HBM bound and tiny next to the relaxation; kept separate so it can read
+// any channel of any force layout.
+
+#include "pp_common.hpp"
+
+// Explicit __fmaf_rn where a fused multiply-add is wanted; no implicit
+// contraction, so FP64 expressions round like the CPU reference.
+#pragma clang fp contract(off)
+
+namespace pp {
+
+namespace {
+
+constexpr int kMaxW = 64;
+
+struct ConvArgs {
+    long ncol;
+    int nz, nzout, nw;
+    int in_stride, in_step, in_offset;
+    double scale;
+    double w[kMaxW];
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void convolve_z_kernel(ConvArgs A, const T *__restrict__ in, T *__restrict__ out)
+{
+    const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long total = A.ncol * A.nzout;
+    if (gid >= total) return;
+    const long c = gid / A.nzout;
+    const int j = (int)(gid % A.nzout);
+    const T *p = in + c * A.in_stride + A.in_offset + (long)j * A.in_step;
+    double acc = 0;
+    for (int k = 0; k < A.nw; k++) acc += (double)p[(long)k * A.in_step] * A.w[k];
+    out[gid] = (T)(acc * A.scale);
+}
+
+// io.limit_vec_field (io.py:766-778) on a float64 (npts,3) device array.
+__global__ __launch_bounds__(256) void limit_vec_kernel(double *__restrict__ F, long npts, double fmax)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npts) return;
+    const double x = F[3 * i], y = F[3 * i + 1], z = F[3 * i + 2];
+    const double fr = sqrt(x * x + y * y + z * z);
+    if (fr > fmax) {
+        const double s = fmax / fr;
+        F[3 * i] = x * s; F[3 * i + 1] = y * s; F[3 * i + 2] = z * s;
+    }
+}
+
+}  // namespace
+
+int launch_limit_vec_field(double *F, long npts, double fmax, hipStream_t st)
+{
+    PP_REQUIRE(npts >= 0, "npts must be >= 0");
+    if (npts == 0) return 0;
+    const long nblk = (npts + 255) / 256;
+    PP_REQUIRE(nblk < (1L << 31), "array too large");
+    hipLaunchKernelGGL(limit_vec_kernel, dim3((unsigned)nblk), dim3(256), 0, st, F, npts, fmax);
+    return 0;
+}
+
+int launch_convolve_z(long ncol, int nz, int in_stride, int in_step, int in_offset, const void *in, int nw,
+                      const double *w, double scale, void *out, int f64, hipStream_t st)
+{
+    PP_REQUIRE(nw >= 1 && nw <= kMaxW, "number of df weights must be 1..64");
+    PP_REQUIRE(nz >= nw, "nz must be >= the number of weights");
+    PP_REQUIRE(ncol >= 0, "ncol must be >= 0");
+    PP_REQUIRE(in_step >= 1 && in_offset >= 0 && in_stride >= (long)(nz - 1) * in_step + in_offset + 1,
+               "input strides inconsistent with nz");
+    ConvArgs A;
+    A.ncol = ncol; A.nz = nz; A.nzout = nz - nw + 1; A.nw = nw;
+    A.in_stride = in_stride; A.in_step = in_step; A.in_offset = in_offset;
+    A.scale = scale;
+    for (int k = 0; k < kMaxW; k++) A.w[k] = k < nw ? w[k] : 0.0;
+    const long total = ncol * A.nzout;
+    if (total == 0) return 0;
+    const long nblk = (total + 255) / 256;
+    PP_REQUIRE(nblk < (1L << 31), "too many columns");
+    if (f64)
+        hipLaunchKernelGGL(convolve_z_kernel<double>, dim3((unsigned)nblk), dim3(256), 0, st, A,
+                           (const double *)in, (double *)out);
+    else
+        hipLaunchKernelGGL(convolve_z_kernel<float>, dim3((unsigned)nblk), dim3(256), 0, st, A,
+                           (const float *)in, (float *)out);
+    return 0;
+}
+
+}  // namespace pp

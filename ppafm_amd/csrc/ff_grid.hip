@@ -1,0 +1,458 @@
+// Force-field grid build on gfx950.
+//
+// Pair terms follow ppafm/cpp/Forces.h:120-132 (addAtomLJ) and :237-244
+// (addAtomCoulomb) with the finite-difference probe stencils of
+// ProbeParticle.cpp:193-209, or ppafm/ocl/cl/FF.cl:115-130 (getLJ,
+// getCoulomb) for the OpenCL-mode model; the grid iteration replaces
+// iterateGrid3D_omp (Grid.h:142-170) and evalLJC_QZs_noPos (FF.cl:366-423).
+//
+// Kernel ff_build_f32 (device path, FP32 VALU bound):
+//   * a block owns a brick of 4x4 grid columns x 64 z points; lanes run along
+//     z so every float4 store instruction writes 4 runs of 16 consecutive
+//     points (z fastest);
+//   * atoms are staged in LDS per tile; every lane reads the same atom
+//     (LDS broadcast) and applies it to its 4 points;
+//   * coordinates are relative to the brick centre and atom offsets are kept
+//     as FP32 hi + lo, so the pair distance carries FP32 relative accuracy
+//     even 40 A away from the grid origin;
+//   * pair math is FP32 (one v_rsq per pair serves LJ and Coulomb), partial
+//     sums are folded into FP64 accumulators every 32 atoms;
+//   * pairs closer than 0.2 A (where r^-13 overflows FP32) are skipped in
+//     the FP32 pass and added in FP64 by a second pass that runs only for
+//     the flagged points;
+//   * the epilogue applies the per-field clamp of io.limit_vec_field
+//     (io.py:766-778) and combines FF = clamp(LJ) + q clamp(El)
+//     (HighLevel.py:143-145) before the single FP32 store.
+//
+// Kernel ff_accum_f64 (host ABI): FP64 pair math, += into the caller's
+// float64 (nx,ny,nz,3) grid, one point per thread.
+
+#include "pp_common.hpp"
+
+// Explicit __fmaf_rn where a fused multiply-add is wanted; no implicit
+// contraction, so FP64 expressions round like the CPU reference.
+#pragma clang fp contract(off)
+
+namespace pp {
+
+namespace {
+
+constexpr int kColsX = 4, kColsY = 4;
+constexpr int kZLanes = 16;
+constexpr int kK = 4;  // z points per thread, strided by kZLanes
+constexpr int kZBrick = kZLanes * kK;
+constexpr int kThreads = kColsX * kColsY * kZLanes;  // 256
+constexpr int kTile = 128;                           // atoms per LDS tile
+constexpr int kFold = 32;                            // FP32 partial-sum length
+constexpr float kNear2 = 0.04f;                      // (0.2 A)^2
+
+struct FFArgs {
+    int n[3];
+    int nbx, nby, nbz;
+    int natoms;
+    int ns;
+    double dcell[9];
+    float sw[4], sdz[4];
+    float r2safe;
+    double r2safe_d;
+    int mode;
+    int with_lj;
+    double probe_q;
+    double fmax;
+};
+
+struct AtomLds {
+    float4 hi;  // x, y, z (hi part, brick-local), charge * coulomb constant
+    float4 lo;  // x, y, z (lo part), unused
+    float4 lj;  // 6*c6, 12*c12, c6, c12
+};
+
+__device__ __forceinline__ float r2_f32(float dx, float dy, float dz, float r2safe)
+{
+    return __fmaf_rn(dx, dx, __fmaf_rn(dy, dy, __fmaf_rn(dz, dz, r2safe)));
+}
+
+__device__ __forceinline__ void clamp_vec(double &x, double &y, double &z, double fmax)
+{
+    if (fmax > 0) {
+        double fr = sqrt(x * x + y * y + z * z);
+        if (fr > fmax) {
+            double s = fmax / fr;
+            x *= s; y *= s; z *= s;
+        }
+    }
+}
+
+// LJ: include the LJ term.  NS: number of Coulomb stencil points.  Z0:
+// stencil point 0 sits at zero z offset and reuses the LJ pair's rsq.
+// WE: accumulate the energy channel.
+template <bool LJ, int NS, bool Z0, bool WE>
+__global__ __launch_bounds__(kThreads) void ff_build_f32(FFArgs A, const AtomRec *__restrict__ atoms,
+                                                         float4 *__restrict__ out)
+{
+    __shared__ AtomLds lds[kTile];
+
+    const int t = threadIdx.x;
+    const int col = t / kZLanes;
+    const int zl = t % kZLanes;
+    int b = blockIdx.x;
+    const int bz = b % A.nbz; b /= A.nbz;
+    const int by = b % A.nby;
+    const int bx = b / A.nby;
+    const int ix = bx * kColsX + (col % kColsX);
+    const int iy = by * kColsY + (col / kColsX);
+    const int z0 = bz * kZBrick;
+
+    // brick centre (fractional grid index) and its Cartesian position
+    const double cx = bx * kColsX + 0.5 * (kColsX - 1);
+    const double cy = by * kColsY + 0.5 * (kColsY - 1);
+    const double cz = z0 + 0.5 * (kZBrick - 1);
+    const double *D = A.dcell;
+    const double ccx = D[0] * cx + D[3] * cy + D[6] * cz;
+    const double ccy = D[1] * cx + D[4] * cy + D[7] * cz;
+    const double ccz = D[2] * cx + D[5] * cy + D[8] * cz;
+
+    // this lane's points, brick-local
+    float px[kK], py[kK], pz[kK];
+#pragma unroll
+    for (int k = 0; k < kK; k++) {
+        const double fx = ix - cx, fy = iy - cy, fz = (z0 + zl + kZLanes * k) - cz;
+        px[k] = (float)(D[0] * fx + D[3] * fy + D[6] * fz);
+        py[k] = (float)(D[1] * fx + D[4] * fy + D[7] * fz);
+        pz[k] = (float)(D[2] * fx + D[5] * fy + D[8] * fz);
+    }
+    // z sub-rows holding at least one valid point (uniform per block)
+    const int nk = min(kK, (A.n[2] - z0 + kZLanes - 1) / kZLanes);
+    const float near2s = kNear2 + A.r2safe;
+
+    double aL[kK][4], aE[kK][4];  // FP64 accumulators Fx, Fy, Fz, E of the LJ and El fields
+#pragma unroll
+    for (int k = 0; k < kK; k++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) { aL[k][c] = 0; aE[k][c] = 0; }
+    unsigned near_mask = 0;
+
+    for (int i0 = 0; i0 < A.natoms; i0 += kTile) {
+        const int nt = min(kTile, A.natoms - i0);
+        __syncthreads();
+        for (int j = t; j < nt; j += kThreads) {
+            const AtomRec a = atoms[i0 + j];
+            const double rx = a.x - ccx, ry = a.y - ccy, rz = a.z - ccz;
+            const float hx = (float)rx, hy = (float)ry, hz = (float)rz;
+            AtomLds r;
+            r.hi = make_float4(hx, hy, hz, (float)a.q);
+            r.lo = make_float4((float)(rx - hx), (float)(ry - hy), (float)(rz - hz), 0.f);
+            r.lj = make_float4((float)(6.0 * a.c6), (float)(12.0 * a.c12), (float)a.c6, (float)a.c12);
+            lds[j] = r;
+        }
+        __syncthreads();
+
+        for (int j0 = 0; j0 < nt; j0 += kFold) {
+            const int j1 = min(nt, j0 + kFold);
+            float sL[kK][4], sE[kK][4];
+#pragma unroll
+            for (int k = 0; k < kK; k++)
+#pragma unroll
+                for (int c = 0; c < 4; c++) { sL[k][c] = 0.f; sE[k][c] = 0.f; }
+
+            for (int j = j0; j < j1; j++) {
+                const float4 hi = lds[j].hi;
+                const float4 lo = lds[j].lo;
+                const float4 lj = lds[j].lj;
+#pragma unroll
+                for (int k = 0; k < kK; k++) {
+                    if (k >= nk) break;
+                    const float dx = (px[k] - hi.x) - lo.x;
+                    const float dy = (py[k] - hi.y) - lo.y;
+                    const float dz = (pz[k] - hi.z) - lo.z;
+                    float ir0 = 0.f, ir20 = 0.f;
+                    if (LJ || Z0) {
+                        float r2 = r2_f32(dx, dy, dz, A.r2safe);
+                        const bool nr = r2 < near2s;
+                        near_mask |= nr ? (1u << k) : 0u;
+                        r2 = nr ? 1.0e30f : r2;
+                        ir0 = __builtin_amdgcn_rsqf(r2);
+                        ir20 = ir0 * ir0;
+                    }
+                    if (LJ) {
+                        const float ir6 = ir20 * ir20 * ir20;
+                        const float s = __fmaf_rn(lj.y, ir6, -lj.x) * ir6 * ir20;
+                        sL[k][0] = __fmaf_rn(dx, s, sL[k][0]);
+                        sL[k][1] = __fmaf_rn(dy, s, sL[k][1]);
+                        sL[k][2] = __fmaf_rn(dz, s, sL[k][2]);
+                        if (WE) sL[k][3] = __fmaf_rn(__fmaf_rn(lj.w, ir6, -lj.z), ir6, sL[k][3]);
+                    }
+#pragma unroll
+                    for (int s_ = 0; s_ < NS; s_++) {
+                        float ir, ir2, dzs;
+                        if (Z0 && s_ == 0) {
+                            ir = ir0; ir2 = ir20; dzs = dz;
+                        } else {
+                            dzs = dz + A.sdz[s_];
+                            float r2 = r2_f32(dx, dy, dzs, A.r2safe);
+                            const bool nr = r2 < near2s;
+                            near_mask |= nr ? (1u << k) : 0u;
+                            r2 = nr ? 1.0e30f : r2;
+                            ir = __builtin_amdgcn_rsqf(r2);
+                            ir2 = ir * ir;
+                        }
+                        const float e = (hi.w * A.sw[s_]) * ir;
+                        const float s = e * ir2;
+                        sE[k][0] = __fmaf_rn(dx, s, sE[k][0]);
+                        sE[k][1] = __fmaf_rn(dy, s, sE[k][1]);
+                        sE[k][2] = __fmaf_rn(dzs, s, sE[k][2]);
+                        if (WE) sE[k][3] += e;
+                    }
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kK; k++)
+#pragma unroll
+                for (int c = 0; c < 4; c++) { aL[k][c] += (double)sL[k][c]; aE[k][c] += (double)sE[k][c]; }
+        }
+    }
+
+    // FP64 pass over the pairs the FP32 pass skipped (same FP32 test decides)
+    if (near_mask) {
+        for (int i = 0; i < A.natoms; i++) {
+            const AtomRec a = atoms[i];
+            const double rx = a.x - ccx, ry = a.y - ccy, rz = a.z - ccz;
+            const float hx = (float)rx, hy = (float)ry, hz = (float)rz;
+            const float lx = (float)(rx - hx), ly = (float)(ry - hy), lz = (float)(rz - hz);
+#pragma unroll
+            for (int k = 0; k < kK; k++) {
+                if (!(near_mask & (1u << k))) continue;
+                const float dx = (px[k] - hx) - lx, dy = (py[k] - hy) - ly, dz = (pz[k] - hz) - lz;
+                const double fx = ix - cx, fy = iy - cy, fz = (z0 + zl + kZLanes * k) - cz;
+                const double ddx = (D[0] * fx + D[3] * fy + D[6] * fz) - rx;
+                const double ddy = (D[1] * fx + D[4] * fy + D[7] * fz) - ry;
+                const double ddz = (D[2] * fx + D[5] * fy + D[8] * fz) - rz;
+                const bool n0 = (LJ || Z0) && r2_f32(dx, dy, dz, A.r2safe) < near2s;
+                if (LJ && n0) {
+                    const double ir2 = 1.0 / (ddx * ddx + ddy * ddy + ddz * ddz + A.r2safe_d);
+                    const double ir6 = ir2 * ir2 * ir2;
+                    const double e6 = a.c6 * ir6, e12 = a.c12 * ir6 * ir6;
+                    const double s = (12 * e12 - 6 * e6) * ir2;
+                    aL[k][0] += ddx * s; aL[k][1] += ddy * s; aL[k][2] += ddz * s;
+                    if (WE) aL[k][3] += e12 - e6;
+                }
+#pragma unroll
+                for (int s_ = 0; s_ < NS; s_++) {
+                    bool nr;
+                    double ddzs;
+                    if (Z0 && s_ == 0) {
+                        nr = n0; ddzs = ddz;
+                    } else {
+                        nr = r2_f32(dx, dy, dz + A.sdz[s_], A.r2safe) < near2s;
+                        ddzs = ddz + (double)A.sdz[s_];
+                    }
+                    if (nr) {
+                        const double ir2 = 1.0 / (ddx * ddx + ddy * ddy + ddzs * ddzs + A.r2safe_d);
+                        const double e = sqrt(ir2) * (a.q * (double)A.sw[s_]);
+                        const double s = e * ir2;
+                        aE[k][0] += ddx * s; aE[k][1] += ddy * s; aE[k][2] += ddzs * s;
+                        if (WE) aE[k][3] += e;
+                    }
+                }
+            }
+        }
+    }
+
+    if (ix >= A.n[0] || iy >= A.n[1]) return;
+#pragma unroll
+    for (int k = 0; k < kK; k++) {
+        const int iz = z0 + zl + kZLanes * k;
+        if (iz >= A.n[2]) break;
+        double fx, fy, fz, e;
+        if (A.mode == 0) {
+            clamp_vec(aL[k][0], aL[k][1], aL[k][2], A.fmax);
+            clamp_vec(aE[k][0], aE[k][1], aE[k][2], A.fmax);
+            fx = aL[k][0] + A.probe_q * aE[k][0];
+            fy = aL[k][1] + A.probe_q * aE[k][1];
+            fz = aL[k][2] + A.probe_q * aE[k][2];
+            e = aL[k][3] + A.probe_q * aE[k][3];
+        } else {
+            fx = aL[k][0] + aE[k][0];
+            fy = aL[k][1] + aE[k][1];
+            fz = aL[k][2] + aE[k][2];
+            e = aL[k][3] + aE[k][3];
+        }
+        const long ig = ((long)ix * A.n[1] + iy) * A.n[2] + iz;
+        out[ig] = make_float4((float)fx, (float)fy, (float)fz, WE ? (float)e : 0.f);
+    }
+}
+
+// ------------------------------------------------------------------ FP64
+
+struct F64Args {
+    int n[3];
+    long npts;
+    int natoms;
+    int model;  // 0 LJ, 1..3 Coulomb kind 0..2
+    double dcell[9];
+};
+
+constexpr int kTile64 = 256;
+
+__device__ __forceinline__ double coul64(double dx, double dy, double dz, double kqq, double &fx, double &fy,
+                                         double &fz, double r2safe)
+{
+    double ir2 = 1.0 / ((dx * dx + dy * dy + dz * dz) + r2safe);
+    double ir = sqrt(ir2);
+    double e = ir * kqq;
+    double s = e * ir2;
+    fx += dx * s; fy += dy * s; fz += dz * s;
+    return e;
+}
+
+__global__ __launch_bounds__(256) void ff_accum_f64(F64Args A, const AtomRec *__restrict__ atoms,
+                                                    double *__restrict__ F, double *__restrict__ E)
+{
+    __shared__ double4 sx[kTile64];
+    __shared__ double2 sc[kTile64];
+    const double r2safe = (double)1.0e-8f;  // Forces.h:12
+    const long ig = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = ig < A.npts;
+    const long igc = live ? ig : 0;
+    const int ic = (int)(igc % A.n[2]);
+    const long r = igc / A.n[2];
+    const int ib = (int)(r % A.n[1]);
+    const int ia = (int)(r / A.n[1]);
+    const double *D = A.dcell;
+    // iterateGrid3D_omp order: pos0 + a*ia + b*ib + c*ic
+    const double px = D[0] * ia + D[3] * ib + D[6] * ic;
+    const double py = D[1] * ia + D[4] * ib + D[7] * ic;
+    const double pz = D[2] * ia + D[5] * ib + D[8] * ic;
+    double fx = 0, fy = 0, fz = 0, e = 0;
+    for (int i0 = 0; i0 < A.natoms; i0 += kTile64) {
+        const int nt = min(kTile64, A.natoms - i0);
+        __syncthreads();
+        for (int j = threadIdx.x; j < nt; j += blockDim.x) {
+            const AtomRec a = atoms[i0 + j];
+            sx[j] = make_double4(a.x, a.y, a.z, a.q);
+            sc[j] = make_double2(a.c6, a.c12);
+        }
+        __syncthreads();
+        for (int j = 0; j < nt; j++) {
+            const double4 a = sx[j];
+            const double dx = px - a.x, dy = py - a.y, dz = pz - a.z;
+            if (A.model == 0) {
+                const double2 c = sc[j];
+                double ir2 = 1.0 / ((dx * dx + dy * dy + dz * dz) + r2safe);
+                double ir6 = ir2 * ir2 * ir2;
+                double e6 = c.x * ir6, e12 = c.y * ir6 * ir6;
+                double s = (12 * e12 - 6 * e6) * ir2;
+                fx += dx * s; fy += dy * s; fz += dz * s;
+                e += e12 - e6;
+            } else if (A.model == 1) {
+                e += coul64(dx, dy, dz, a.w, fx, fy, fz, r2safe);
+            } else if (A.model == 2) {
+                double gx = 0, gy = 0, gz = 0, ee = 0;
+                ee += coul64(dx, dy, dz - 0.1, -a.w, gx, gy, gz, r2safe);
+                ee += coul64(dx, dy, (dz - 0.1) + 2 * 0.1, a.w, gx, gy, gz, r2safe);
+                fx += gx * 10.0; fy += gy * 10.0; fz += gz * 10.0;
+                e += ee * 10.0;
+            } else {
+                double gx = 0, gy = 0, gz = 0, ee = 0;
+                ee += coul64(dx, dy, dz, -2 * a.w, gx, gy, gz, r2safe);
+                ee += coul64(dx, dy, dz - 0.1, a.w, gx, gy, gz, r2safe);
+                ee += coul64(dx, dy, (dz - 0.1) + 2 * 0.1, a.w, gx, gy, gz, r2safe);
+                fx += gx * 100.0; fy += gy * 100.0; fz += gz * 100.0;
+                e += ee * 100.0;
+            }
+        }
+    }
+    if (!live) return;
+    if (F) { F[3 * ig] += fx; F[3 * ig + 1] += fy; F[3 * ig + 2] += fz; }
+    if (E) E[ig] += e;
+}
+
+template <bool LJ, int NS, bool Z0, bool WE>
+void launch_t(const FFArgs &A, const AtomRec *atoms, float4 *out, hipStream_t st)
+{
+    dim3 grid((unsigned)A.nbx * A.nby * A.nbz), block(kThreads);
+    hipLaunchKernelGGL((ff_build_f32<LJ, NS, Z0, WE>), grid, block, 0, st, A, atoms, out);
+}
+
+template <bool LJ, bool WE>
+int launch_ns(int ns, bool z0, const FFArgs &A, const AtomRec *atoms, float4 *out, hipStream_t st)
+{
+    switch (ns) {
+    case 0: launch_t<LJ, 0, false, WE>(A, atoms, out, st); break;
+    case 1: z0 ? launch_t<LJ, 1, true, WE>(A, atoms, out, st) : launch_t<LJ, 1, false, WE>(A, atoms, out, st); break;
+    case 2: z0 ? launch_t<LJ, 2, true, WE>(A, atoms, out, st) : launch_t<LJ, 2, false, WE>(A, atoms, out, st); break;
+    case 3: z0 ? launch_t<LJ, 3, true, WE>(A, atoms, out, st) : launch_t<LJ, 3, false, WE>(A, atoms, out, st); break;
+    case 4: z0 ? launch_t<LJ, 4, true, WE>(A, atoms, out, st) : launch_t<LJ, 4, false, WE>(A, atoms, out, st); break;
+    default: set_error("n_stencil must be 0..4"); return -2;
+    }
+    return 0;
+}
+
+int first_zero_index(const ppafm_ff_model &m)
+{
+    for (int s = 0; s < m.n_stencil; s++)
+        if (m.stencil_w[s] != 0 && m.stencil_dz[s] == 0) return s;
+    return -1;
+}
+
+}  // namespace
+
+int launch_ff_build_f32(const GridGeom &g, const ppafm_ff_model &m, int natoms, const AtomRec *atoms_dev,
+                        float4 *out, hipStream_t st)
+{
+    PP_REQUIRE(g.n[0] > 0 && g.n[1] > 0 && g.n[2] > 0, "grid dimensions must be positive");
+    PP_REQUIRE(m.n_stencil >= 0 && m.n_stencil <= 4, "n_stencil must be 0..4");
+    PP_REQUIRE(natoms >= 0, "natoms must be >= 0");
+    FFArgs A;
+    for (int i = 0; i < 3; i++) A.n[i] = g.n[i];
+    A.nbx = (g.n[0] + kColsX - 1) / kColsX;
+    A.nby = (g.n[1] + kColsY - 1) / kColsY;
+    A.nbz = (g.n[2] + kZBrick - 1) / kZBrick;
+    PP_REQUIRE((long)A.nbx * A.nby * A.nbz < (1L << 31), "grid too large for one launch");
+    A.natoms = natoms;
+    for (int i = 0; i < 9; i++) A.dcell[i] = g.dcell[i];
+    // stencil points with zero weight are dropped; the first zero-offset point
+    // goes first so that it can share the LJ pair's rsq
+    const int zi = first_zero_index(m);
+    int ns = 0;
+    if (zi >= 0) { A.sw[ns] = (float)m.stencil_w[zi]; A.sdz[ns] = 0.f; ns++; }
+    for (int s = 0; s < m.n_stencil; s++)
+        if (m.stencil_w[s] != 0 && s != zi) {
+            A.sw[ns] = (float)m.stencil_w[s]; A.sdz[ns] = (float)m.stencil_dz[s]; ns++;
+        }
+    for (int s = ns; s < 4; s++) { A.sw[s] = 0.f; A.sdz[s] = 0.f; }
+    A.ns = ns;
+    const bool z0 = zi >= 0;
+    A.mode = m.mode;
+    A.r2safe_d = m.mode == 0 ? (double)1.0e-8f : (double)1.0e-6f;  // Forces.h:12 / FF.cl:1
+    A.r2safe = (float)A.r2safe_d;
+    A.with_lj = m.with_lj;
+    A.probe_q = m.probe_q;
+    A.fmax = m.fmax;
+    const bool we = m.with_energy != 0;
+    if (m.with_lj) {
+        if (we) return launch_ns<true, true>(A.ns, z0, A, atoms_dev, out, st);
+        return launch_ns<true, false>(A.ns, z0, A, atoms_dev, out, st);
+    }
+    if (we) return launch_ns<false, true>(A.ns, z0, A, atoms_dev, out, st);
+    return launch_ns<false, false>(A.ns, z0, A, atoms_dev, out, st);
+}
+
+int launch_ff_accum_f64(const GridGeom &g, int model, int natoms, const AtomRec *atoms_dev, double *F, double *E,
+                        hipStream_t st)
+{
+    PP_REQUIRE(model >= 0 && model <= 3, "model must be 0 (LJ) or 1..3 (Coulomb kind 0..2)");
+    F64Args A;
+    for (int i = 0; i < 3; i++) A.n[i] = g.n[i];
+    A.npts = (long)g.n[0] * g.n[1] * g.n[2];
+    A.natoms = natoms;
+    A.model = model;
+    for (int i = 0; i < 9; i++) A.dcell[i] = g.dcell[i];
+    const long nblk = (A.npts + 255) / 256;
+    PP_REQUIRE(nblk < (1L << 31), "grid too large for one launch");
+    if (A.npts == 0) return 0;
+    hipLaunchKernelGGL(ff_accum_f64, dim3((unsigned)nblk), dim3(256), 0, st, A, atoms_dev, F, E);
+    return 0;
+}
+
+}  // namespace pp

@@ -1,0 +1,82 @@
+// Internal helpers shared by the kernels and the C ABI of libppafm_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/ppafm_hip.h"
+
+namespace pp {
+
+// Last error message of the calling thread (ppafm_last_error).
+void set_error(const std::string &msg);
+const char *get_error();
+
+#define PP_TRY_HIP(expr)                                                                         \
+    do {                                                                                         \
+        hipError_t _e = (expr);                                                                  \
+        if (_e != hipSuccess) {                                                                  \
+            ::pp::set_error(std::string(#expr) + ": " + hipGetErrorString(_e));                \
+            return -1;                                                                           \
+        }                                                                                        \
+    } while (0)
+
+#define PP_REQUIRE(cond, msg)                                                                    \
+    do {                                                                                         \
+        if (!(cond)) {                                                                           \
+            ::pp::set_error(msg);                                                                \
+            return -2;                                                                           \
+        }                                                                                        \
+    } while (0)
+
+// Grid geometry in the form the kernels use: step vectors (rows of cell/n)
+// and the map from a Cartesian offset to fractional grid indices, computed
+// on the host in FP64 like GridShape::setCell (Grid.h:29-36).
+struct GridGeom {
+    int n[3];
+    double dcell[9];  // rows: a/na, b/nb, c/nc
+    double dinv[9];   // (dcell^-1)^T: row r dotted with a position gives index coordinate r
+};
+
+GridGeom make_geom(const int n[3], const double cell[9]);
+
+// Atom record for the FP32 grid kernel: coordinates relative to the brick
+// centre are formed in FP64 on the fly and split hi + lo, so that
+// d = (p - hi) - lo keeps FP32 relative accuracy in the pair distance.
+using AtomRec = ppafm_atom;  // x, y, z (grid frame), q * coulomb constant, c6, c12
+
+hipStream_t as_stream(void *s);
+
+}  // namespace pp
+
+// Launchers implemented in the .hip files.
+namespace pp {
+
+// FP32 device grid (z fastest, float4).
+int launch_ff_build_f32(const GridGeom &g, const ppafm_ff_model &m, int natoms, const AtomRec *atoms_dev,
+                        float4 *out, hipStream_t st);
+// FP64 accumulate (+=) into double3 grid F (and E) for the host ABI.
+// model: 0 = LJ, 1..3 = Coulomb kind 0..2.
+int launch_ff_accum_f64(const GridGeom &g, int model, int natoms, const AtomRec *atoms_dev, double *F, double *E,
+                        hipStream_t st);
+
+struct RelaxLaunch {
+    GridGeom g;
+    ppafm_relax_params rp;
+    ppafm_tip tip;
+    ppafm_scan scan;
+    const double *tips;  // optional explicit tip positions (nx,ny,nz,3)
+    ppafm_relax_out out;
+};
+int launch_relax_f32grid(const RelaxLaunch &L, const float4 *grid, hipStream_t st);
+int launch_relax_f64grid(const RelaxLaunch &L, const double *grid, double *pos, double *force, int *iters,
+                         hipStream_t st);
+
+int launch_limit_vec_field(double *F, long npts, double fmax, hipStream_t st);
+int launch_convolve_z(long ncol, int nz, int in_stride, int in_step, int in_offset, const void *in, int nw,
+                      const double *w, double scale, void *out, int f64, hipStream_t st);
+
+}  // namespace pp

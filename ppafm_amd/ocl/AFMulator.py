@@ -1,0 +1,360 @@
+"""Drop-in for ``ppafm.ocl.AFMulator.AFMulator`` (ocl/AFMulator.py:23-719),
+point-charge method, running on MI355X through libppafm_hip.so.
+
+Same constructor keywords, call signature, output shape and errors.  The
+force field is the OpenCL-mode model (LJ + tip charges Qs at z offsets QZs,
+evalLJC_QZs_noPos, FF.cl:366-423), the relaxation follows relaxStrokesTilted
+(relax.cl:391-464: dt 0.5, dtmin 0.05, 128 steps, dt/alpha kept along the
+stroke) and the df stack is convolveZ with get_simple_df_weight
+(relax.cl:571-608, AFMulator.py:226-229).  Interpolation is exact software
+trilinear, where the reference uses the hardware image filter.
+
+Not on this path (raise NotImplementedError): Hartree potentials, tip
+densities ``rho`` / ``rho_delta`` and the FDBM method.
+"""
+
+import os
+import warnings
+
+import numpy as np
+
+from .. import common
+from .. import device as D
+
+VALID_SIZES = np.array([16, 32, 64, 128, 192, 256, 384, 512, 768, 1024, 1536, 2048])
+
+
+class AFMulator:
+    bMergeConv = False
+    relaxParams = [0.5, 0.1, 0.1 * 0.2, 0.1 * 5.0]
+    bSaveFF = False
+
+    def __init__(
+        self,
+        pixPerAngstrome=10,
+        lvec=None,
+        scan_dim=(128, 128, 30),
+        scan_window=((2.0, 2.0, 7.0), (18.0, 18.0, 10.0)),
+        iZPP=8,
+        QZs=[0.1, 0, -0.1, 0],
+        Qs=[-10, 20, -10, 0],
+        rho=None,
+        sigma=0.71,
+        rho_delta=None,
+        A_pauli=18.0,
+        B_pauli=1.0,
+        fdbm_vdw_type="D3",
+        d3_params="PBE",
+        lj_vdw_damp=2,
+        df_steps=10,
+        tipR0=[0.0, 0.0, 3.0],
+        tipStiffness=[0.25, 0.25, 0.0, 30.0],
+        npbc=(1, 1, 0),
+        f0Cantilever=30300,
+        kCantilever=1800,
+        colorscale="gray",
+        minimize_memory=False,
+    ):
+        self.stiffness = np.array(tipStiffness, dtype=np.float32) / -common.eVA_Nm
+        self.relax_params = np.array(self.relaxParams, dtype=np.float32)
+        self.iZPP = iZPP
+        self.tipR0 = tipR0
+        self.f0Cantilever = f0Cantilever
+        self.kCantilever = kCantilever
+        self.npbc = npbc
+        self.sigma = sigma
+        self.A_pauli = A_pauli
+        self.B_pauli = B_pauli
+        self.fdbm_vdw_type = fdbm_vdw_type
+        self.d3_params = d3_params
+        self.lj_vdw_damp = lj_vdw_damp
+        self.sample_lvec = None
+        self.colorscale = colorscale
+        self.minimize_memory = minimize_memory
+        self._grid = None
+        self._nDim_alloc = None
+        self._out = None
+        self.setScanWindow(scan_window, scan_dim, df_steps)
+        self.setLvec(lvec, pixPerAngstrome)
+        self.setRho(rho, sigma, B_pauli)
+        self.setRhoDelta(rho_delta)
+        self.setQs(Qs, QZs)
+        self.typeParams = common.loadSpecies("atomtypes.ini")
+
+    # ------------------------------------------------------------------ call
+    def eval(self, xyzs, Zs, qs, rho_sample=None, sample_lvec=None, rot=np.eye(3), rot_center=None, REAs=None, X=None, plot_to_dir=None):
+        self.prepareFF(xyzs, Zs, qs, rho_sample, sample_lvec, rot, rot_center, REAs)
+        self.prepareScanner()
+        X = self.evalAFM(X)
+        if plot_to_dir:
+            raise NotImplementedError("plotting is not on this path")
+        return X
+
+    def __call__(self, *args, **kwargs):
+        return self.eval(*args, **kwargs)
+
+    def eval_(self, mol):
+        return self.eval(mol.xyzs, mol.Zs, mol.qs)
+
+    # ----------------------------------------------------------------- setup
+    def setLvec(self, lvec=None, pixPerAngstrome=None):
+        if pixPerAngstrome is not None:
+            self.pixPerAngstrome = pixPerAngstrome
+        if lvec is not None:
+            self.lvec = np.array(lvec, dtype=np.float64)
+        else:
+            self.lvec = get_lvec(self.scan_window, tipR0=self.tipR0, pixPerAngstrome=self.pixPerAngstrome)
+        self.nDim = common.genFFSampling(self.lvec, pixPerAngstrome=self.pixPerAngstrome)
+
+    def setScanWindow(self, scan_window=None, scan_dim=None, df_steps=None):
+        if scan_window is not None:
+            self.scan_window = scan_window
+        if scan_dim is not None:
+            self.scan_dim = tuple(scan_dim)
+        if df_steps is not None:
+            if df_steps <= 0 or df_steps > self.scan_dim[2]:
+                raise ValueError(f"df_steps should be between 1 and scan_dim[2]({self.scan_dim[2]}), but got {df_steps}.")
+            self.df_steps = df_steps
+        self.dz = (self.scan_window[1][2] - self.scan_window[0][2]) / self.scan_dim[2]
+        self.dfWeight = common.get_simple_df_weight(self.df_steps, dz=self.dz).astype(np.float32)
+        self.dfWeight *= common.eVA_Nm * self.f0Cantilever / self.kCantilever
+        self.amplitude = self.dz * len(self.dfWeight)
+        self._out = None
+
+    def setRho(self, rho=None, sigma=0.71, B_pauli=1.0):
+        if rho is not None and len(rho) > 0:
+            raise NotImplementedError("tip charge densities (rho) are not on this path; use point charges Qs/QZs")
+        self._rho = None
+        self.rho = None
+
+    def setBPauli(self, B_pauli=1.0):
+        self.B_pauli = B_pauli
+
+    def setRhoDelta(self, rho_delta=None):
+        if rho_delta is not None:
+            raise NotImplementedError("rho_delta (FDBM) is not on this path")
+        self.rho_delta = None
+
+    def setQs(self, Qs, QZs):
+        if (len(Qs) != 4) or (len(QZs) != 4):
+            raise ValueError("Qs and Qzs must have length 4")
+        self.Qs = Qs
+        self.QZs = QZs
+
+    def setStiffness(self, tipStiffness):
+        self.stiffness = np.array(tipStiffness, dtype=np.float32) / -common.eVA_Nm
+
+    # --------------------------------------------------------------- imaging
+    def prepareFF(self, xyzs, Zs, qs, rho_sample=None, sample_lvec=None, rot=np.eye(3), rot_center=None, REAs=None):
+        self.check_scan_window()
+        if rho_sample is not None:
+            raise NotImplementedError("FDBM (rho_sample) is not on this path")
+        if qs is not None and not isinstance(qs, np.ndarray) and not isinstance(qs, (list, tuple)):
+            raise NotImplementedError("Hartree potentials are not on this path; pass point charges")
+        xyzs = np.asarray(xyzs, dtype=np.float64)
+        qs = np.zeros(len(Zs)) if qs is None else np.asarray(qs, dtype=np.float64)
+        if sample_lvec is not None:
+            self.sample_lvec = np.asarray(sample_lvec, dtype=np.float64)
+        npbc = self.npbc if self.sample_lvec is not None else (0, 0, 0)
+        if rot_center is None:
+            rot_center = xyzs.mean(axis=0)
+        if REAs is None:
+            REAs = common.getAtomsREA(self.iZPP, Zs, self.typeParams, alphaFac=-1.0)
+        cLJs = common.REA2LJ(REAs)
+        if sum(npbc) > 0:
+            Zs, xyzs, qs, cLJs, REAs = common.PBCAtoms3D_np(Zs, xyzs, qs, cLJs, REAs, self.sample_lvec, npbc=npbc)
+        # atoms are rotated about rot_center (field.py:1694-1697), kept in FP32
+        # like the reference's device buffers (field.py:926-931)
+        xyzs = np.asarray(xyzs, dtype=np.float64) - rot_center
+        xyzs = np.dot(xyzs, np.asarray(rot).T) + rot_center
+        xyzs = xyzs.astype(np.float32).astype(np.float64)
+        qs = np.asarray(qs, dtype=np.float32).astype(np.float64)
+        cLJs = np.asarray(cLJs, dtype=np.float32).astype(np.float64)
+        lvec0 = self.lvec[0].astype(np.float32).astype(np.float64)
+        xyzq = np.concatenate([xyzs - lvec0[None, :], qs[:, None]], axis=1)
+        charged = not np.allclose(qs, 0)
+        stencil = list(zip(np.float32(self.Qs).tolist(), np.float32(self.QZs).tolist())) if charged else []
+        self._model = D.ff_model(1, with_lj=True, stencil=stencil, with_energy=True, fmax=None)
+        n = tuple(int(v) for v in self.nDim[:3])
+        if self._grid is None or self._nDim_alloc != n:
+            self._grid = D.DeviceArray(n + (4,), np.float32)
+            self._nDim_alloc = n
+        cell = self.lvec[1:].astype(np.float32).astype(np.float64)
+        D.ff_build(n, cell, xyzq, cLJs, self._model, out=self._grid)
+
+    def prepareScanner(self):
+        self.pos0 = np.array([0, 0, self.scan_window[1][2]]) - self.lvec[0]
+        rot = np.eye(3)
+        zstep = self.dz
+        tipR0 = self.tipR0
+        d0 = np.array([tipR0[0], tipR0[1], -np.sqrt(tipR0[2] ** 2 - tipR0[0] ** 2 - tipR0[1] ** 2)], dtype=np.float32)
+        st = self.stiffness
+        self._tip = D.tip_desc(d0.astype(np.float64), st[:3].astype(np.float64), float(st[3]), float(np.float32(tipR0[2])), rot=rot)
+        nx, ny, nz = self.scan_dim
+        start = self.scan_window[0][:2]
+        end = self.scan_window[1][:2]
+        p0 = np.array([self.pos0[0] + start[0], self.pos0[1] + start[1], self.pos0[2]])
+        self._scan = D.scan_desc(
+            nx, ny, nz, p0=p0,
+            da=((end[0] - start[0]) / max(nx - 1, 1), 0, 0),
+            db=(0, (end[1] - start[1]) / max(ny - 1, 1), 0),
+            dstep=(0, 0, -np.float32(zstep)),
+        )
+        rp = self.relax_params
+        self._rp = D.relax_params(mode=1, alg=1, max_iters=128, conv_f2=float(np.float32(1e-8)), dt=float(rp[0]), damping=float(rp[1]), dtmin_frac=float(np.float32(0.1)), f2safe=float(np.float32(1e-8)))
+
+    def evalAFM(self, X=None):
+        nx, ny, nz = self.scan_dim
+        if self._out is None or self._out[0].shape != (nx, ny, nz, 4):
+            self._out = (D.DeviceArray((nx, ny, nz, 4), np.float32), D.DeviceArray((nx, ny, nz, 3), np.float32))
+        force, paths = self._out
+        cell = self.lvec[1:].astype(np.float32).astype(np.float64)
+        D.relax(tuple(int(v) for v in self.nDim[:3]), cell, self._grid, self._rp, self._tip, self._scan, pos=paths, force=force)
+        self._paths = paths
+        nw = len(self.dfWeight)
+        nzout = nz - nw + 1
+        conv = D.convolve_z_dev(force, nx * ny, nz, self.dfWeight.astype(np.float64), scale=1.0, in_stride=nz * 4, in_step=4, in_offset=2)
+        Xd = conv.download().reshape(nx, ny, nzout)
+        if X is None:
+            X = Xd.copy()
+        else:
+            if X.shape != tuple(self.scan_dim[:2]) + (nzout,):
+                raise ValueError(
+                    f"Expected an array of shape {tuple(self.scan_dim[:2]) + (nzout,)} for storing AFM image, but got an array of shape {X.shape} instead."
+                )
+            X[:, :, :] = Xd
+        return X
+
+    # ------------------------------------------------------------ accessors
+    def downloadFF(self):
+        """Force field (nx, ny, nz, 4) float32 (Fx, Fy, Fz, E)."""
+        return self._grid.download()
+
+    def downloadPaths(self):
+        """Probe positions (nx, ny, nz, 3) in the molecule frame (relax.py:282-305)."""
+        return self._paths.download() + self.lvec[0].astype(np.float32)
+
+    # ------------------------------------------------------------ params I/O
+    @classmethod
+    def from_params(cls, file_path="./params.ini"):
+        parameters, sample_lvec = _get_params(file_path)
+        afmulator = cls(**parameters)
+        afmulator.sample_lvec = sample_lvec
+        return afmulator
+
+    def load_params(self, file_path="./params.ini"):
+        parameters, sample_lvec = _get_params(file_path)
+        if parameters["tipStiffness"] is not None:
+            self.stiffness = np.array(parameters["tipStiffness"], dtype=np.float32) / -common.eVA_Nm
+        self.iZPP = parameters["iZPP"]
+        self.tipR0 = parameters["tipR0"]
+        self.f0Cantilever = parameters["f0Cantilever"]
+        self.kCantilever = parameters["kCantilever"]
+        self.npbc = parameters["npbc"]
+        self.A_pauli = parameters["A_pauli"]
+        self.setScanWindow(parameters["scan_window"], parameters["scan_dim"], parameters["df_steps"])
+        self.setLvec(parameters["lvec"], parameters["pixPerAngstrome"])
+        self.sample_lvec = sample_lvec
+        self.colorscale = parameters["colorscale"]
+
+    def save_params(self, file_path="./params.ini"):
+        k = self.stiffness * -common.eVA_Nm
+        nDim = self.nDim
+        scan_step = (
+            (self.scan_window[1][0] - self.scan_window[0][0]) / (self.scan_dim[0] - 1),
+            (self.scan_window[1][1] - self.scan_window[0][1]) / (self.scan_dim[1] - 1),
+            (self.scan_window[1][2] - self.scan_window[0][2]) / self.scan_dim[2],
+        )
+        with open(file_path, "w") as f:
+            f.write(f"probeType {self.iZPP}\n")
+            f.write(f"stiffness {k[0]} {k[1]} {k[3]}\n")
+            f.write(f"r0Probe {self.tipR0[0]} {self.tipR0[1]} {self.tipR0[2]}\n")
+            f.write(f"PBC {(np.array(self.npbc) > 0).any()}\n")
+            f.write(f"nPBC {self.npbc[0]} {self.npbc[1]} {self.npbc[2]}\n")
+            if self.sample_lvec is not None:
+                for name, row in zip(("gridA", "gridB", "gridC"), self.sample_lvec):
+                    f.write(f"{name} {row[0]} {row[1]} {row[2]}\n")
+            for name, row in zip(("FFgrid0", "FFgridA", "FFgridB", "FFgridC"), self.lvec):
+                f.write(f"{name} {row[0]} {row[1]} {row[2]}\n")
+            f.write(f"gridN {nDim[0]} {nDim[1]} {nDim[2]}\n")
+            f.write(f"scanMin {self.scan_window[0][0]} {self.scan_window[0][1]} {self.scan_window[0][2]}\n")
+            f.write(f"scanMax {self.scan_window[1][0]} {self.scan_window[1][1]} {self.scan_window[1][2]}\n")
+            f.write(f"scanStep {scan_step[0]} {scan_step[1]} {scan_step[2]}\n")
+            f.write(f"kCantilever {self.kCantilever}\n")
+            f.write(f"f0Cantilever {self.f0Cantilever}\n")
+            f.write(f"Amplitude {self.amplitude}\n")
+            f.write(f"Apauli {self.A_pauli}\n")
+            f.write(f"Bpauli {self.B_pauli}\n")
+
+    def check_scan_window(self):
+        for i, dim in enumerate("xyz"):
+            if self.npbc[i]:
+                continue
+            lvec_start = self.lvec[0, i]
+            lvec_end = lvec_start + self.lvec[i + 1, i]
+            scan_start = self.scan_window[0][i] - 0.3
+            scan_end = self.scan_window[1][i] + 0.3
+            if i == 2:
+                scan_start -= self.tipR0[2]
+                scan_end -= self.tipR0[2]
+            if (scan_start < lvec_start) or (scan_end > lvec_end):
+                warnings.warn(f"The edge of the scan window in {dim} dimension is very close or extends over the boundary of the force-field grid which is not periodic in {dim} dimension.")
+
+
+def _get_params(file_path):
+    parameters = common.PpafmParameters.from_file(file_path)
+    lvec = np.array([parameters.FFgrid0, parameters.FFgridA, parameters.FFgridB, parameters.FFgridC])
+    if (lvec < 0).any():
+        lvec = None
+    sample_lvec = np.array([parameters.gridA, parameters.gridB, parameters.gridC])
+    if (np.array(parameters.gridN) == 0).any() or lvec is None:
+        pixPerAngstrome = 10
+    else:
+        rx, ry, rz = (round(parameters.gridN[i] / np.linalg.norm(lvec[i + 1])) for i in range(3))
+        pixPerAngstrome = rx if np.allclose([rx, ry], rz) else int(np.max([rx, ry, rz]))
+    scan_window = (parameters.scanMin, parameters.scanMax)
+    scan_dim = (
+        round((scan_window[1][0] - scan_window[0][0]) / parameters.scanStep[0]) + 1,
+        round((scan_window[1][1] - scan_window[0][1]) / parameters.scanStep[1]) + 1,
+        round((scan_window[1][2] - scan_window[0][2]) / parameters.scanStep[2]),
+    )
+    from ..io import ELEMENT_Z
+
+    iZPP = parameters.probeType
+    iZPP = ELEMENT_Z[iZPP] if iZPP in ELEMENT_Z else int(iZPP)
+    tipStiffness = parameters.stiffness
+    if (np.array(tipStiffness) < 0).any():
+        tipStiffness = [0.25, 0.25, 0.0, 30.0]
+    else:
+        tipStiffness = np.insert(tipStiffness, 2, 0.0)
+    afmulator_params = {
+        "lvec": lvec,
+        "pixPerAngstrome": pixPerAngstrome,
+        "scan_dim": scan_dim,
+        "scan_window": scan_window,
+        "iZPP": iZPP,
+        "rho": None,
+        "sigma": parameters.sigma,
+        "A_pauli": parameters.Apauli,
+        "B_pauli": parameters.Bpauli,
+        "df_steps": round(parameters.Amplitude / parameters.scanStep[2]),
+        "tipR0": parameters.r0Probe,
+        "tipStiffness": tipStiffness,
+        "npbc": tuple(int(v) * int(parameters.PBC) for v in parameters.nPBC),
+        "f0Cantilever": parameters.f0Cantilever,
+        "kCantilever": parameters.kCantilever,
+        "colorscale": parameters.colorscale,
+    }
+    return afmulator_params, sample_lvec
+
+
+def get_lvec(scan_window, pad=(3.0, 3.0, 5.0), tipR0=(0.0, 0.0, 3.0), pixPerAngstrome=10):
+    pad = np.array(pad)
+    tipR0 = np.array(tipR0)
+    center = (np.array(scan_window[0]) + np.array(scan_window[1])) / 2
+    box_size = (np.array(scan_window[1]) - np.array(scan_window[0])) + 2 * pad
+    nDim = (pixPerAngstrome * box_size).round().astype(np.int32)
+    nDim = np.array([VALID_SIZES[VALID_SIZES >= d][0] for d in nDim])
+    box_size = nDim / pixPerAngstrome
+    origin = center - box_size / 2 - tipR0
+    return np.array([origin, [box_size[0], 0, 0], [0, box_size[1], 0], [0, 0, box_size[2]]])

@@ -1,0 +1,127 @@
+"""Device-resident relaxed scan with the ProbeParticle (CPU) semantics.
+
+One object owns the FP32 force-field grid, the relaxation outputs and the df
+stack on the GPU and runs the three stages of the path back to back on one
+stream:
+
+  1. grid build   FF = clamp(LJ) + q * clamp(El)  (HighLevel.py:143-145,260-387)
+  2. relaxation   relaxTipStrokes_omp semantics    (ProbeParticle.cpp:633-701)
+  3. Fz -> df     Fz2df with get_df_weight         (common.py:333-430)
+
+Positions are relative to the grid origin inside; the public accessors return
+them in the molecule frame (origin added back, HighLevel.py:170-172).
+"""
+
+import numpy as np
+
+from . import common as PPU
+from . import device as D
+
+
+class PlainScan:
+    """Relaxed AFM scan of one molecule, ProbeParticle semantics, on the GPU.
+
+    lvec: (4,3) grid origin and cell; n: grid points; scan: dict with
+    ``scan_n`` (nx, ny), ``scan_x``/``scan_y`` apex ranges, ``z_top``, ``nz``,
+    ``dz`` (apex descends by dz per step).  Tip: tipR0 (x0, y0, L), k_lat,
+    k_rad in N/m; probe element iZPP and charge probe_q (monopole unless
+    ``tip`` says "pz"/"dz2").  df: amplitude A (Fz2df ``amplitude``), k0, f0.
+    """
+
+    def __init__(self, lvec, n, scan, tipR0=(0.0, 0.0, 3.0), k_lat=0.25, k_rad=30.0, iZPP=8, probe_q=-0.1, tip="s",
+                 amplitude=0.8, k0=1800.0, f0=30300.0, Fmax=10.0, relax=None, species=None, stream=None):
+        self.lvec = np.asarray(lvec, dtype=np.float64)
+        self.n = tuple(int(v) for v in n)
+        self.scan = dict(scan)
+        self.stream = stream
+        self.iZPP = iZPP
+        self.species = species or PPU.loadSpecies()
+        self.model = D.ff_model(0, with_lj=True, stencil=D.PROBE_STENCILS[tip], probe_q=probe_q, fmax=Fmax)
+        relax = relax or {}
+        self.rp = D.relax_params(mode=0, **relax)
+        kS = np.array([k_lat, k_lat, 0.0]) / -PPU.eVA_Nm
+        self.tip = D.cpu_tip(tipR0[2], k_rad / -PPU.eVA_Nm, (tipR0[0], tipR0[1]), kS)
+        nx, ny = self.scan["scan_n"]
+        nz = self.scan["nz"]
+        x0, x1 = self.scan["scan_x"]
+        y0, y1 = self.scan["scan_y"]
+        o = self.lvec[0]
+        self.scan_d = D.scan_desc(
+            nx, ny, nz,
+            p0=(x0 - o[0], y0 - o[1], self.scan["z_top"] - o[2]),
+            da=((x1 - x0) / max(nx - 1, 1), 0, 0),
+            db=(0, (y1 - y0) / max(ny - 1, 1), 0),
+            dstep=(0, 0, -self.scan["dz"]),
+        )
+        self.grid = D.DeviceArray(self.n + (4,), np.float32)
+        self.pos = D.DeviceArray((nx, ny, nz, 3), np.float32)
+        self.force = D.DeviceArray((nx, ny, nz, 4), np.float32)
+        self.iters = D.DeviceArray((nx, ny, nz), np.int32)
+        self.counters = D.DeviceArray((2,), np.uint64)
+        # Fz2df: correlating the stroke-ordered (z descending) Fz with the
+        # weights as returned equals np.convolve on the ascending stack.
+        self.w = PPU.get_df_weight(amplitude, dz=self.scan["dz"])
+        self.df_scale = 16.0217656 * f0 / k0
+        self.nzout = nz - len(self.w) + 1
+        self.df = D.DeviceArray((nx * ny, max(self.nzout, 1)), np.float32)
+        self.atoms = None
+        self.natoms = 0
+
+    # ---------------------------------------------------------------- inputs
+    def set_molecule(self, xyzs, Zs, qs):
+        xyzs = np.asarray(xyzs, dtype=np.float64)
+        cLJs = PPU.getAtomsLJ(self.iZPP, Zs, self.species)
+        xyzq = np.concatenate([xyzs - self.lvec[0][None, :], np.asarray(qs, dtype=np.float64)[:, None]], axis=1)
+        packed = D.pack_atoms(xyzq, cLJs, self.model.coulomb_const)
+        self.natoms = len(packed)
+        self.atoms = D.DeviceArray.from_host(packed if len(packed) else np.zeros((1, 6)))
+
+    # ---------------------------------------------------------------- stages
+    def build_grid(self):
+        import ctypes
+
+        from ._lib import check, lib
+
+        g = D.grid_desc(self.n, self.lvec[1:])
+        check(lib().ppafm_ff_build_dev(ctypes.byref(g), ctypes.byref(self.model), self.natoms, self.atoms.ptr, self.grid.ptr, self.stream), "ff_build")
+
+    def relax(self, counters=True):
+        D.relax(self.n, self.lvec[1:], self.grid, self.rp, self.tip, self.scan_d, pos=self.pos, force=self.force, iters=self.iters, counters=self.counters if counters else None, stream=self.stream)
+
+    def to_df(self):
+        if self.nzout < 1:
+            return
+        nx, ny = self.scan["scan_n"]
+        nz = self.scan["nz"]
+        D.convolve_z_dev(self.force, nx * ny, nz, self.w, scale=self.df_scale, in_stride=nz * 4, in_step=4, in_offset=2, out=self.df, stream=self.stream)
+
+    def run(self):
+        self.build_grid()
+        self.relax()
+        self.to_df()
+
+    def reset_counters(self):
+        self.counters.zero(self.stream)
+
+    # --------------------------------------------------------------- outputs
+    def grid_host(self):
+        return self.grid.download()
+
+    def positions(self):
+        """Relaxed probe positions (nx, ny, nz, 3) in stroke order, molecule frame."""
+        return self.pos.download().astype(np.float64) + self.lvec[0][None, None, None, :]
+
+    def forces(self):
+        return self.force.download()
+
+    def iterations(self):
+        return self.iters.download()
+
+    def df_host(self):
+        """df (nx, ny, nzout) with z ascending, like Fz2df on the reversed stack."""
+        nx, ny = self.scan["scan_n"]
+        return self.df.download().reshape(nx, ny, self.nzout)[:, :, ::-1].copy()
+
+    def counts(self):
+        c = self.counters.download()
+        return int(c[0]), int(c[1])

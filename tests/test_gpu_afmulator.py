@@ -1,0 +1,91 @@
+"""GPU parity of the AFMulator mirror (OpenCL-mode semantics) against the FP64
+oracle restatement of evalLJC_QZs_noPos, relaxStrokesTilted and convolveZ.
+
+The reference OpenCL path cannot run here and samples the grid through the
+hardware image filter; the oracle uses exact trilinear weights, as does the
+GPU kernel, so this test pins the GPU code to the OpenCL-mode algorithm, not
+to the vendor filter (parity with the hardware filter is unpinned)."""
+
+import numpy as np
+import pytest
+
+import oracle as O
+from ppafm_amd import common as C
+from ppafm_amd import molgen as M
+from ppafm_amd.ocl.AFMulator import AFMulator
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def run():
+    print("case: AFMulator 32x32x20 scan, 20-atom molecule, default dz2-like tip charges", flush=True)
+    xyz, Zs, qs = M.molecule(20, 11)
+    afm = AFMulator(scan_dim=(32, 32, 20), scan_window=((-2.0, -2.0, 7.0), (2.0, 2.0, 9.0)), df_steps=10)
+    X = afm(xyz, Zs, qs)
+    return afm, xyz, Zs, qs, X
+
+
+def _oracle_inputs(afm, xyz, Zs, qs):
+    REAs = C.getAtomsREA(afm.iZPP, Zs, afm.typeParams, alphaFac=-1.0)
+    cLJs = C.REA2LJ(REAs).astype(np.float32).astype(np.float64)
+    x32 = xyz.astype(np.float32).astype(np.float64)
+    q32 = np.asarray(qs, np.float32).astype(np.float64)
+    lvec = afm.lvec.astype(np.float32).astype(np.float64)
+    atoms = np.concatenate([x32, q32[:, None]], axis=1)
+    return atoms, cLJs, lvec
+
+
+def test_afm_grid(run):
+    afm, xyz, Zs, qs, X = run
+    atoms, cLJs, lvec = _oracle_inputs(afm, xyz, Zs, qs)
+    FE = O.ljc_qzs_ff(afm.nDim, lvec, atoms, cLJs, np.float32(afm.Qs).astype(np.float64), np.float32(afm.QZs).astype(np.float64))
+    G = afm.downloadFF()
+    d = np.linalg.norm(G[..., :3] - FE[..., :3], axis=-1) / np.maximum(np.linalg.norm(FE[..., :3], axis=-1), 1e-3)
+    de = np.abs(G[..., 3] - FE[..., 3]) / np.maximum(np.abs(FE[..., 3]), 1e-3)
+    print(f"AFM grid rel err: F max {d.max():.3g} p99 {np.percentile(d, 99):.3g}; E max {de.max():.3g}")
+    assert np.isfinite(G).all()
+    assert d.max() < 1e-4
+
+
+def test_afm_relax_and_df(run):
+    afm, xyz, Zs, qs, X = run
+    atoms, cLJs, lvec = _oracle_inputs(afm, xyz, Zs, qs)
+    FE = O.ljc_qzs_ff(afm.nDim, lvec, atoms, cLJs, np.float32(afm.Qs).astype(np.float64), np.float32(afm.QZs).astype(np.float64))
+    # feed the oracle the GPU's FP32 grid so the comparison isolates the relaxation
+    G = afm.downloadFF().astype(np.float64)
+    nx, ny, nz = afm.scan_dim
+    cell = lvec[1:]
+    dinv = np.linalg.inv(cell).T * np.array(afm.nDim[:3])[:, None]
+    xs = np.linspace(afm.scan_window[0][0], afm.scan_window[1][0], nx)
+    ys = np.linspace(afm.scan_window[0][1], afm.scan_window[1][1], ny)
+    poss = np.zeros((nx, ny, 4))
+    poss[..., 0] = xs[:, None] - afm.lvec[0][0]
+    poss[..., 1] = ys[None, :] - afm.lvec[0][1]
+    poss[..., 2] = afm.scan_window[1][2] - afm.lvec[0][2]
+    tipRot = np.zeros((3, 4))
+    tipRot[:, :3] = np.eye(3)
+    tipRot[2, 3] = -np.float32(afm.dz)
+    t = afm.tipR0
+    dpos0 = np.array([t[0], t[1], -np.sqrt(t[2] ** 2 - t[0] ** 2 - t[1] ** 2), t[2]], dtype=np.float32).astype(np.float64)
+    FEs, paths, nev = O.relax_tilted(G, dinv, poss.reshape(-1, 4), nz, tipRot, afm.stiffness.astype(np.float64), dpos0, afm.relax_params[:2].astype(np.float64), max_iters=128, conv_f2=float(np.float32(1e-8)), f2safe=float(np.float32(1e-8)))
+    p_gpu = afm.downloadPaths() - afm.lvec[0].astype(np.float32)
+    dp = np.linalg.norm(p_gpu.reshape(-1, nz, 3) - paths[..., :3], axis=-1)
+    print(f"AFM paths |dr| max {dp.max():.3g} p99 {np.percentile(dp, 99):.3g}; mean evals {nev.mean():.1f}")
+    assert dp.max() < 1e-3
+    Xref = O.convolve_z(FEs[..., 2].reshape(nx, ny, nz), afm.dfWeight.astype(np.float64))
+    dX = np.abs(X - Xref)
+    print(f"AFM df max |dX| {dX.max():.3g} Hz (range {Xref.min():.3g} .. {Xref.max():.3g})")
+    assert X.shape == (nx, ny, nz - afm.df_steps + 1)
+    assert dX.max() < 1e-3
+
+
+def test_afm_output_array_and_errors(run):
+    afm, xyz, Zs, qs, X = run
+    out = np.zeros_like(X)
+    r = afm(xyz, Zs, qs, X=out)
+    assert r is out and np.allclose(out, X)
+    with pytest.raises(ValueError):
+        afm(xyz, Zs, qs, X=np.zeros((3, 3, 3), np.float32))
+    with pytest.raises(ValueError):
+        afm.setScanWindow(df_steps=0)
