@@ -1,0 +1,271 @@
+"""Benchmark of the probe-particle hot path on one MI355X.
+
+A step is one pass of the path over the 120-atom production workload
+(256x256x320 grid at 0.1 A, 256x256 pixels x 60 z-steps, plain FIRE to
+|F| < 1e-4 eV/A, 9-tap Giessibl df): force-field grid build, relaxation of
+all 3,932,160 tip positions and the Fz -> df convolution, with the atoms
+already resident on the GPU.  ``value`` is relaxed tip positions per second
+of the whole step (grid build included).
+
+Timing: HIP events recorded on the stream the kernels are launched on,
+bracketed by device synchronisation; the relaxation kernel is also timed on
+its own for the roofline line (algorithmic bytes from device-side counters
+of force evaluations).  The CPU baseline is the reference C++ (compiled by
+``make -C oracle ref``; the FP64 oracle port if that is absent) timed with
+OpenMP on this host's cores on a bounded sample of the same workload.
+
+Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+"""
+
+import argparse
+import contextlib
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "relaxed tip positions/s & FF grid pts/s, learned+batched vs plain FIRE"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=10)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--grid", choices=["f64", "f32"], default="f64", help="grid storage of the measured plain path")
+    return p.parse_args()
+
+
+@contextlib.contextmanager
+def quiet_stdout():
+    """Silence C-level printf (the reference prints progress) on fd 1."""
+    libc = ctypes.CDLL(None)
+    sys.stdout.flush()
+    libc.fflush(None)
+    saved = os.dup(1)
+    devnull = os.open(os.devnull, os.O_WRONLY)
+    os.dup2(devnull, 1)
+    try:
+        yield
+    finally:
+        libc.fflush(None)
+        os.dup2(saved, 1)
+        os.close(saved)
+        os.close(devnull)
+
+
+def cpu_cores():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def run_gpu(args, cfg, xyz, Zs, qs, lvec):
+    from ppafm_amd import device as D
+    from ppafm_amd import molgen as M
+    from ppafm_amd.scan import PlainScan
+
+    s = PlainScan(lvec, cfg["n"], cfg, tipR0=M.TIP_R0, k_lat=M.K_LAT, k_rad=M.K_RAD, iZPP=M.PROBE_Z, probe_q=M.PROBE_Q, amplitude=M.DF_AMPLITUDE, grid_f64=args.grid == "f64")
+    s.set_molecule(xyz, Zs, qs)
+    for _ in range(args.warmup):
+        s.run()
+    D.synchronize()
+    ev = [[D.Event() for _ in range(4)] for _ in range(args.steps)]
+    start, stop = D.Event(), D.Event()
+    s.reset_counters()
+    D.synchronize()
+    t0 = time.perf_counter()
+    start.record()
+    for e in ev:
+        e[0].record()
+        s.build_grid()
+        e[1].record()
+        s.relax()
+        e[2].record()
+        s.to_df()
+        e[3].record()
+    stop.record()
+    D.synchronize()
+    wall = time.perf_counter() - t0
+    t_total = start.elapsed_ms(stop) / args.steps
+    t_grid = float(np.mean([e[0].elapsed_ms(e[1]) for e in ev]))
+    t_relax = float(np.mean([e[1].elapsed_ms(e[2]) for e in ev]))
+    t_df = float(np.mean([e[2].elapsed_ms(e[3]) for e in ev]))
+    nev, npos = s.counts()
+    nev //= args.steps
+    npos //= args.steps
+    iters = s.iterations()
+    return dict(scan=s, t_step=t_total, t_grid=t_grid, t_relax=t_relax, t_df=t_df, wall_step_ms=1e3 * wall / args.steps, n_eval=nev, n_pos=npos, mean_iters=float(iters.mean()))
+
+
+def run_cpu(args, cfg, xyz, Zs, qs, lvec, grid_host):
+    """Reference C++ (or FP64 port) on a bounded sample of the workload."""
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    cores = cpu_cores()
+    os.environ["OMP_NUM_THREADS"] = str(cores)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    from ppafm_amd import common as C
+    from ppafm_amd import molgen as M
+
+    cLJs = C.getAtomsLJ(M.PROBE_Z, Zs, C.loadSpecies())
+    Rs0 = np.ascontiguousarray(xyz - lvec[0])
+    kq = np.ascontiguousarray(qs * C.CoulombConst)
+    nx, ny, nz = cfg["n"]
+    kind = "reference" if O.have_reference() else "port"
+    # grid: a z-slab of the same grid (same spacing), scaled by nz / kz
+    kz = 64
+    n_s = np.array([nx, ny, kz], dtype=np.int32)
+    cell_s = np.array(lvec[1:]) * np.array([1.0, 1.0, kz / nz])[:, None]
+    F = np.zeros((nx, ny, kz, 3))
+    with quiet_stdout():
+        if kind == "reference":
+            ref = O.Reference()
+            ref.set_grid(n_s, cell_s, F)
+            t0 = time.perf_counter()
+            ref.lj(Rs0, cLJs)
+            ref.coulomb(Rs0, kq, 0)
+            t_grid_s = time.perf_counter() - t0
+        else:
+            t0 = time.perf_counter()
+            O.lj_ff(n_s, cell_s, Rs0, cLJs, F=F)
+            O.coulomb_ff(n_s, cell_s, Rs0, kq, F=F)
+            t_grid_s = time.perf_counter() - t0
+    t_grid = t_grid_s * nz / kz
+    # relaxation: every k-th pixel column on the full grid (the GPU's FP64 grid)
+    FF = np.ascontiguousarray(grid_host[..., :3], dtype=np.float64)
+    tips = M.scan_tips(cfg) - lvec[0]
+    sx, sy = cfg["scan_n"]
+    stride = 2
+    cols = [(i, j) for i in range(0, sx, stride) for j in range(0, sy, stride)]
+    sub = np.ascontiguousarray(np.stack([np.stack([tips[i, j] for j in range(0, sy, stride)]) for i in range(0, sx, stride)]))
+    tip = O.tip_params(M.TIP_R0[2], M.K_RAD / -C.eVA_Nm, (0.0, 0.0, 0.0), np.array([M.K_LAT, M.K_LAT, 0.0]) / -C.eVA_Nm)
+    with quiet_stdout():
+        if kind == "reference":
+            ref.set_grid(np.array(cfg["n"], dtype=np.int32), lvec[1:], FF)
+            ref.set_tip(M.TIP_R0[2], M.K_RAD / -C.eVA_Nm, np.zeros(3), np.array([M.K_LAT, M.K_LAT, 0.0]) / -C.eVA_Nm)
+            ref.set_relax(1000, 1e-8, 0.1, 0.1)
+            t0 = time.perf_counter()
+            ref.relax_strokes(sub)
+            t_rel_s = time.perf_counter() - t0
+        else:
+            t0 = time.perf_counter()
+            O.relax_strokes(cfg["n"], lvec[1:], FF, tip, sub, want_iters=False)
+            t_rel_s = time.perf_counter() - t0
+    frac = len(cols) / (sx * sy)
+    t_relax = t_rel_s / frac
+    n_pos = sx * sy * cfg["nz"]
+    value = n_pos / (t_grid + t_relax)
+    sample = (f"grid: {kz} of {nz} z-layers ({100 * kz / nz:.2f}%, LJ + Coulomb) scaled by {nz / kz:g}; "
+              f"relaxation: every {stride}th pixel in x and y ({len(cols)} of {sx * sy} columns, {100 * frac:.2f}%, all {cfg['nz']} z-steps) "
+              f"scaled by {1 / frac:g}; CPU {cpu_model()}; measured {t_grid_s:.2f} s + {t_rel_s:.2f} s")
+    return dict(value=value, unit="relaxed tip positions/s", cores=cores, kind=kind, sample=sample, grid_pts_per_s=nx * ny * nz / t_grid, t_grid_s=t_grid, t_relax_s=t_relax)
+
+
+def traffic_from_profiles():
+    """HBM bytes per relaxation launch from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
+    path = os.path.join(ROOT, "profiles", "relax_c1_pmc.json")
+    try:
+        d = json.load(open(path))
+        return float(d["hbm_bytes_per_launch"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).", file=sys.stderr)
+        sys.exit(2)
+    from ppafm_amd import molgen as M
+
+    cfg = M.CONFIGS[1]
+    xyz, Zs, qs = M.config_molecule(1, 0, cfg["natoms"])
+    lvec = M.config_lvec(cfg)
+    g = run_gpu(args, cfg, xyz, Zs, qs, lvec)
+    npts = int(np.prod(cfg["n"]))
+    n_pos = g["n_pos"]
+    value = n_pos / (g["t_step"] * 1e-3)
+    corner_bytes = 8 if args.grid == "f64" else 4
+    alg_bytes = 8 * 3 * corner_bytes * g["n_eval"] + 24 * n_pos
+    achieved = alg_bytes / (g["t_relax"] * 1e-3) / 1e9
+    traffic = traffic_from_profiles()
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "relaxed tip positions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": g["t_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.grid == "f64" else "f32 grid / f64 relax",
+        "data": "synthetic",
+        "config": {
+            "workload": "plain FIRE, one 120-atom molecule: LJ+Coulomb grid 256x256x320 at 0.1 A, 256x256 pixels x 60 z-steps, 9-tap df",
+            "molecule_seed": 1000000,
+            "grid_storage": args.grid,
+            "grid_points": npts,
+            "tip_positions": n_pos,
+        },
+        "grid_pts_per_s": npts / (g["t_grid"] * 1e-3),
+        "relax_positions_per_s": n_pos / (g["t_relax"] * 1e-3),
+        "stage_ms": {"grid": g["t_grid"], "relax": g["t_relax"], "df": g["t_df"]},
+        "host_wall_ms_per_step": g["wall_step_ms"],
+        "force_evals_per_step": g["n_eval"],
+        "mean_fire_iterations": g["mean_iters"],
+        "roofline": {
+            "kernel": "relax_kernel (FIRE)",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": alg_bytes,
+            "bytes_model": f"8 corners x 3 x {corner_bytes} B per force evaluation + 24 B per relaxed position",
+        },
+        "grid_build": {
+            "bound": "valu",
+            "pairs": npts * len(xyz),
+            "achieved_tflops": 32 * npts * len(xyz) / (g["t_grid"] * 1e-3) / 1e12,
+            "peak_tflops": 78.6 if args.grid == "f64" else 157.3,
+            "note": "32 algorithmic FLOP per atom-point pair (LJ + Coulomb, force + energy)",
+        },
+    }
+    if not args.no_cpu:
+        grid_host = g["scan"].grid_host()
+        c = run_cpu(args, cfg, xyz, Zs, qs, lvec, grid_host)
+        out["cpu_baseline"] = {"value": c["value"], "unit": c["unit"], "cores": c["cores"], "kind": c["kind"], "sample": c["sample"], "grid_pts_per_s": c["grid_pts_per_s"]}
+        out["speedup_vs_cpu"] = value / c["value"]
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

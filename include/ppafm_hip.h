@@ -78,10 +78,13 @@ int relaxTipStrokes_omp(int nx, int ny, int probeStart, int relaxAlg, int nstep,
 
 /* Regular grid: n points along the rows of cell (a, b, c); point (i,j,k) sits
  * at i*cell[0..2]/n0 + j*cell[3..5]/n1 + k*cell[6..8]/n2 relative to the grid
- * origin; all positions below are relative to the grid origin. */
+ * origin; all positions below are relative to the grid origin.  The device
+ * grid holds (Fx, Fy, Fz, E) per point, z fastest, as float4 (f64 = 0) or
+ * double4 (f64 = 1; every pair summed in FP64). */
 typedef struct {
     int n[3];
     double cell[9];
+    int f64;
 } ppafm_grid;
 
 /* Force-field model of one grid build.
@@ -103,6 +106,9 @@ typedef struct {
     double coulomb_const;
     double probe_q;
     double fmax;
+    double fp64_radius;     /* atom-point pairs closer than this [A] are evaluated
+                               in FP64 (at least 0.2 A beyond the largest stencil
+                               offset); the rest in FP32 with FP64 accumulation */
 } ppafm_ff_model;
 
 /* One atom as the grid kernels read it: position relative to the grid
@@ -187,6 +193,10 @@ int ppafm_convolve_z(long ncol, int nz, int in_stride, int in_step, int in_offse
 /* io.limit_vec_field (io.py:766-778) in place on a float64 (npts,3) device
  * array: vectors longer than fmax are scaled to length fmax. */
 int ppafm_limit_vec_field(double *F_dev, long npts, double fmax, void *stream);
+
+/* Device-to-device copy by a plain kernel (bytes a multiple of 16), the
+ * known-size reference for HBM byte counters. */
+int ppafm_copy_kernel(const void *src_dev, void *dst_dev, unsigned long long bytes, void *stream);
 
 /* Memory helpers so Python callers need no other GPU runtime. */
 int ppafm_malloc(void **ptr, unsigned long long bytes);

@@ -22,7 +22,7 @@ class SplineParams(ctypes.Structure):
 
 
 class Grid(ctypes.Structure):
-    _fields_ = [("n", c_int * 3), ("cell", c_double * 9)]
+    _fields_ = [("n", c_int * 3), ("cell", c_double * 9), ("f64", c_int)]
 
 
 class FFModel(ctypes.Structure):
@@ -36,6 +36,7 @@ class FFModel(ctypes.Structure):
         ("coulomb_const", c_double),
         ("probe_q", c_double),
         ("fmax", c_double),
+        ("fp64_radius", c_double),
     ]
 
 
@@ -97,6 +98,7 @@ SIGNATURES = [
     ),
     ("ppafm_convolve_z", c_int, [c_long, c_int, c_int, c_int, c_int, c_void_p, c_int, _dp, c_double, c_void_p, c_int, c_void_p]),
     ("ppafm_limit_vec_field", c_int, [c_void_p, c_long, c_double, c_void_p]),
+    ("ppafm_copy_kernel", c_int, [c_void_p, c_void_p, c_ull, c_void_p]),
     ("ppafm_malloc", c_int, [ctypes.POINTER(c_void_p), c_ull]),
     ("ppafm_free", c_int, [c_void_p]),
     ("ppafm_memcpy_h2d", c_int, [c_void_p, c_void_p, c_ull, c_void_p]),

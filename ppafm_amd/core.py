@@ -36,18 +36,36 @@ def setGridCell(cell):
     lib().setGridCell(_lib.dptr(np.ascontiguousarray(cell)))
 
 
+def _release(key, token):
+    # Clear the library pointer only if the collected array is still the one
+    # registered; the reference's finalizer (core.py:61,71) clears it even
+    # when a newer array has been set since, which leaves the grid unset.
+    cur = _keep.get(key)
+    if cur is not None and cur[1] == token:
+        (lib().deleteFF_Fpointer if key == "F" else lib().deleteFF_Epointer)()
+        _keep.pop(key, None)
+
+
+_token = [0]
+
+
+def _register(key, arr):
+    _token[0] += 1
+    tok = _token[0]
+    _keep[key] = (weakref.ref(arr), tok)
+    weakref.finalize(arr, _release, key, tok)
+
+
 def setFF_Fpointer(gridF):
     _c64(gridF, 4, "gridF")
     lib().setFF_Fpointer(_lib.dptr(gridF))
-    _keep["F"] = gridF
-    weakref.finalize(gridF, deleteFF_Fpointer)
+    _register("F", gridF)
 
 
 def setFF_Epointer(gridE):
     _c64(gridE, 3, "gridE")
     lib().setFF_Epointer(_lib.dptr(gridE))
-    _keep["E"] = gridE
-    weakref.finalize(gridE, deleteFF_Epointer)
+    _register("E", gridE)
 
 
 def deleteFF_Fpointer():

@@ -307,7 +307,7 @@ int ppafm_ff_build_dev(const ppafm_grid *grid, const ppafm_ff_model *model, int 
     PP_REQUIRE(grid && model && out_dev, "null argument");
     PP_REQUIRE(natoms == 0 || atoms_dev, "null atoms");
     const GridGeom g = make_geom(grid->n, grid->cell);
-    int e = launch_ff_build_f32(g, *model, natoms, (const AtomRec *)atoms_dev, (float4 *)out_dev, as_stream(stream));
+    int e = launch_ff_build(g, *model, natoms, (const AtomRec *)atoms_dev, out_dev, grid->f64 != 0, as_stream(stream));
     if (e) return e;
     PP_TRY_HIP(hipGetLastError());
     return 0;
@@ -357,7 +357,7 @@ int ppafm_relax(const ppafm_grid *grid, const void *grid_dev, const ppafm_relax_
     L.scan = *scan;
     L.tips = tips_dev;
     L.out = *out;
-    int e = launch_relax_f32grid(L, (const float4 *)grid_dev, as_stream(stream));
+    int e = launch_relax_dev(L, grid_dev, grid->f64 != 0, as_stream(stream));
     if (e) return e;
     PP_TRY_HIP(hipGetLastError());
     return 0;
@@ -378,6 +378,15 @@ int ppafm_limit_vec_field(double *F_dev, long npts, double fmax, void *stream)
 {
     PP_REQUIRE(F_dev || npts == 0, "null argument");
     int e = launch_limit_vec_field(F_dev, npts, fmax, as_stream(stream));
+    if (e) return e;
+    PP_TRY_HIP(hipGetLastError());
+    return 0;
+}
+
+int ppafm_copy_kernel(const void *src_dev, void *dst_dev, unsigned long long bytes, void *stream)
+{
+    PP_REQUIRE(bytes % 16 == 0, "copy size must be a multiple of 16 bytes");
+    int e = launch_copy(src_dev, dst_dev, (long)(bytes / 16), as_stream(stream));
     if (e) return e;
     PP_TRY_HIP(hipGetLastError());
     return 0;

@@ -7,6 +7,8 @@
 // HBM bound and tiny next to the relaxation; kept separate so it can read
 // any channel of any force layout.
 
+#include <algorithm>
+
 #include "pp_common.hpp"
 
 // Explicit __fmaf_rn where a fused multiply-add is wanted; no implicit
@@ -54,7 +56,22 @@ __global__ __launch_bounds__(256) void limit_vec_kernel(double *__restrict__ F, 
     }
 }
 
+// Plain device copy (float4 granularity), used to calibrate HBM byte counters.
+__global__ __launch_bounds__(256) void copy_kernel(const float4 *__restrict__ src, float4 *__restrict__ dst, long n)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) dst[i] = src[i];
+}
+
 }  // namespace
+
+int launch_copy(const void *src, void *dst, long n16, hipStream_t st)
+{
+    PP_REQUIRE(n16 >= 0, "size must be >= 0");
+    if (n16 == 0) return 0;
+    const long nblk = std::min<long>((n16 + 255) / 256, 256L * 64);
+    hipLaunchKernelGGL(copy_kernel, dim3((unsigned)nblk), dim3(256), 0, st, (const float4 *)src, (float4 *)dst, n16);
+    return 0;
+}
 
 int launch_limit_vec_field(double *F, long npts, double fmax, hipStream_t st)
 {

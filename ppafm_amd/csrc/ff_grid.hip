@@ -44,7 +44,7 @@ constexpr int kZBrick = kZLanes * kK;
 constexpr int kThreads = kColsX * kColsY * kZLanes;  // 256
 constexpr int kTile = 128;                           // atoms per LDS tile
 constexpr int kFold = 32;                            // FP32 partial-sum length
-constexpr float kNear2 = 0.04f;                      // (0.2 A)^2
+constexpr double kMinFp64Radius = 0.2;               // r^-13 overflows FP32 below ~0.04 A
 
 struct FFArgs {
     int n[3];
@@ -55,6 +55,7 @@ struct FFArgs {
     float sw[4], sdz[4];
     float r2safe;
     double r2safe_d;
+    float near2;  // pairs with |d|^2 below this go to FP64
     int mode;
     int with_lj;
     double probe_q;
@@ -83,6 +84,32 @@ __device__ __forceinline__ void clamp_vec(double &x, double &y, double &z, doubl
     }
 }
 
+// FP64 contribution of one atom to one point: LJ at d and the Coulomb
+// stencil at d + (0, 0, sdz[s]); ProbeParticle.cpp:262-278 / Forces.h:120-132,
+// 237-244 in the order of the oracle.
+template <bool LJ, int NS, bool WE>
+__device__ __forceinline__ void pair_fp64(const FFArgs &A, double dx, double dy, double dz, double q, double c6,
+                                          double c12, double aL[4], double aE[4])
+{
+    if (LJ) {
+        const double ir2 = 1.0 / ((dx * dx + dy * dy + dz * dz) + A.r2safe_d);
+        const double ir6 = ir2 * ir2 * ir2;
+        const double e6 = c6 * ir6, e12 = c12 * ir6 * ir6;
+        const double s = (12 * e12 - 6 * e6) * ir2;
+        aL[0] += dx * s; aL[1] += dy * s; aL[2] += dz * s;
+        if (WE) aL[3] += e12 - e6;
+    }
+#pragma unroll
+    for (int s_ = 0; s_ < NS; s_++) {
+        const double dzs = dz + (double)A.sdz[s_];
+        const double ir2 = 1.0 / ((dx * dx + dy * dy + dzs * dzs) + A.r2safe_d);
+        const double e = sqrt(ir2) * (q * (double)A.sw[s_]);
+        const double s = e * ir2;
+        aE[0] += dx * s; aE[1] += dy * s; aE[2] += dzs * s;
+        if (WE) aE[3] += e;
+    }
+}
+
 // LJ: include the LJ term.  NS: number of Coulomb stencil points.  Z0:
 // stencil point 0 sits at zero z offset and reuses the LJ pair's rsq.
 // WE: accumulate the energy channel.
@@ -91,6 +118,7 @@ __global__ __launch_bounds__(kThreads) void ff_build_f32(FFArgs A, const AtomRec
                                                          float4 *__restrict__ out)
 {
     __shared__ AtomLds lds[kTile];
+    __shared__ double4 ldsd[kTile];  // q, c6, c12 in FP64 for the FP64 pairs
 
     const int t = threadIdx.x;
     const int col = t / kZLanes;
@@ -111,26 +139,26 @@ __global__ __launch_bounds__(kThreads) void ff_build_f32(FFArgs A, const AtomRec
     const double ccx = D[0] * cx + D[3] * cy + D[6] * cz;
     const double ccy = D[1] * cx + D[4] * cy + D[7] * cz;
     const double ccz = D[2] * cx + D[5] * cy + D[8] * cz;
+    const double fx = ix - cx, fy = iy - cy;
 
     // this lane's points, brick-local
     float px[kK], py[kK], pz[kK];
 #pragma unroll
     for (int k = 0; k < kK; k++) {
-        const double fx = ix - cx, fy = iy - cy, fz = (z0 + zl + kZLanes * k) - cz;
+        const double fz = (z0 + zl + kZLanes * k) - cz;
         px[k] = (float)(D[0] * fx + D[3] * fy + D[6] * fz);
         py[k] = (float)(D[1] * fx + D[4] * fy + D[7] * fz);
         pz[k] = (float)(D[2] * fx + D[5] * fy + D[8] * fz);
     }
     // z sub-rows holding at least one valid point (uniform per block)
     const int nk = min(kK, (A.n[2] - z0 + kZLanes - 1) / kZLanes);
-    const float near2s = kNear2 + A.r2safe;
+    const float near2s = A.near2 + A.r2safe;
 
     double aL[kK][4], aE[kK][4];  // FP64 accumulators Fx, Fy, Fz, E of the LJ and El fields
 #pragma unroll
     for (int k = 0; k < kK; k++)
 #pragma unroll
         for (int c = 0; c < 4; c++) { aL[k][c] = 0; aE[k][c] = 0; }
-    unsigned near_mask = 0;
 
     for (int i0 = 0; i0 < A.natoms; i0 += kTile) {
         const int nt = min(kTile, A.natoms - i0);
@@ -144,6 +172,7 @@ __global__ __launch_bounds__(kThreads) void ff_build_f32(FFArgs A, const AtomRec
             r.lo = make_float4((float)(rx - hx), (float)(ry - hy), (float)(rz - hz), 0.f);
             r.lj = make_float4((float)(6.0 * a.c6), (float)(12.0 * a.c12), (float)a.c6, (float)a.c12);
             lds[j] = r;
+            ldsd[j] = make_double4(a.q, a.c6, a.c12, 0.0);
         }
         __syncthreads();
 
@@ -165,13 +194,21 @@ __global__ __launch_bounds__(kThreads) void ff_build_f32(FFArgs A, const AtomRec
                     const float dx = (px[k] - hi.x) - lo.x;
                     const float dy = (py[k] - hi.y) - lo.y;
                     const float dz = (pz[k] - hi.z) - lo.z;
+                    const float r2c = r2_f32(dx, dy, dz, A.r2safe);
+                    const bool nr = r2c < near2s;
+                    if (nr) {
+                        // pair inside the FP64 radius: exact brick-local offsets
+                        const double fz = (z0 + zl + kZLanes * k) - cz;
+                        const double qx = D[0] * fx + D[3] * fy + D[6] * fz;
+                        const double qy = D[1] * fx + D[4] * fy + D[7] * fz;
+                        const double qz = D[2] * fx + D[5] * fy + D[8] * fz;
+                        const double4 ad = ldsd[j];
+                        pair_fp64<LJ, NS, WE>(A, qx - ((double)hi.x + (double)lo.x), qy - ((double)hi.y + (double)lo.y),
+                                              qz - ((double)hi.z + (double)lo.z), ad.x, ad.y, ad.z, aL[k], aE[k]);
+                    }
                     float ir0 = 0.f, ir20 = 0.f;
                     if (LJ || Z0) {
-                        float r2 = r2_f32(dx, dy, dz, A.r2safe);
-                        const bool nr = r2 < near2s;
-                        near_mask |= nr ? (1u << k) : 0u;
-                        r2 = nr ? 1.0e30f : r2;
-                        ir0 = __builtin_amdgcn_rsqf(r2);
+                        ir0 = __builtin_amdgcn_rsqf(nr ? 1.0e30f : r2c);
                         ir20 = ir0 * ir0;
                     }
                     if (LJ) {
@@ -189,11 +226,8 @@ __global__ __launch_bounds__(kThreads) void ff_build_f32(FFArgs A, const AtomRec
                             ir = ir0; ir2 = ir20; dzs = dz;
                         } else {
                             dzs = dz + A.sdz[s_];
-                            float r2 = r2_f32(dx, dy, dzs, A.r2safe);
-                            const bool nr = r2 < near2s;
-                            near_mask |= nr ? (1u << k) : 0u;
-                            r2 = nr ? 1.0e30f : r2;
-                            ir = __builtin_amdgcn_rsqf(r2);
+                            const float r2 = r2_f32(dx, dy, dzs, A.r2safe);
+                            ir = __builtin_amdgcn_rsqf(nr ? 1.0e30f : r2);
                             ir2 = ir * ir;
                         }
                         const float e = (hi.w * A.sw[s_]) * ir;
@@ -212,48 +246,90 @@ __global__ __launch_bounds__(kThreads) void ff_build_f32(FFArgs A, const AtomRec
         }
     }
 
-    // FP64 pass over the pairs the FP32 pass skipped (same FP32 test decides)
-    if (near_mask) {
-        for (int i = 0; i < A.natoms; i++) {
-            const AtomRec a = atoms[i];
-            const double rx = a.x - ccx, ry = a.y - ccy, rz = a.z - ccz;
-            const float hx = (float)rx, hy = (float)ry, hz = (float)rz;
-            const float lx = (float)(rx - hx), ly = (float)(ry - hy), lz = (float)(rz - hz);
+    if (ix >= A.n[0] || iy >= A.n[1]) return;
+#pragma unroll
+    for (int k = 0; k < kK; k++) {
+        const int iz = z0 + zl + kZLanes * k;
+        if (iz >= A.n[2]) break;
+        double gx, gy, gz, e;
+        if (A.mode == 0) {
+            clamp_vec(aL[k][0], aL[k][1], aL[k][2], A.fmax);
+            clamp_vec(aE[k][0], aE[k][1], aE[k][2], A.fmax);
+            gx = aL[k][0] + A.probe_q * aE[k][0];
+            gy = aL[k][1] + A.probe_q * aE[k][1];
+            gz = aL[k][2] + A.probe_q * aE[k][2];
+            e = aL[k][3] + A.probe_q * aE[k][3];
+        } else {
+            gx = aL[k][0] + aE[k][0];
+            gy = aL[k][1] + aE[k][1];
+            gz = aL[k][2] + aE[k][2];
+            e = aL[k][3] + aE[k][3];
+        }
+        const long ig = ((long)ix * A.n[1] + iy) * A.n[2] + iz;
+        out[ig] = make_float4((float)gx, (float)gy, (float)gz, WE ? (float)e : 0.f);
+    }
+}
+
+// Exact variant: every pair in FP64 (pair_fp64) and a double4 grid, for
+// callers that need the grid to FP64 accuracy (the FIRE stop rule turns
+// grid perturbations of ~1e-7 into stopping-point changes of >1e-3 A on
+// ill-conditioned strokes).  Same brick mapping as ff_build_f32.
+template <bool LJ, int NS, bool WE>
+__global__ __launch_bounds__(kThreads) void ff_build_f64(FFArgs A, const AtomRec *__restrict__ atoms,
+                                                         double4 *__restrict__ out)
+{
+    __shared__ double4 lpos[kTile];  // brick-local x, y, z, q
+    __shared__ double2 llj[kTile];   // c6, c12
+
+    const int t = threadIdx.x;
+    const int col = t / kZLanes;
+    const int zl = t % kZLanes;
+    int b = blockIdx.x;
+    const int bz = b % A.nbz; b /= A.nbz;
+    const int by = b % A.nby;
+    const int bx = b / A.nby;
+    const int ix = bx * kColsX + (col % kColsX);
+    const int iy = by * kColsY + (col / kColsX);
+    const int z0 = bz * kZBrick;
+    const double cx = bx * kColsX + 0.5 * (kColsX - 1);
+    const double cy = by * kColsY + 0.5 * (kColsY - 1);
+    const double cz = z0 + 0.5 * (kZBrick - 1);
+    const double *D = A.dcell;
+    const double ccx = D[0] * cx + D[3] * cy + D[6] * cz;
+    const double ccy = D[1] * cx + D[4] * cy + D[7] * cz;
+    const double ccz = D[2] * cx + D[5] * cy + D[8] * cz;
+    const double fx = ix - cx, fy = iy - cy;
+    double px[kK], py[kK], pz[kK];
+#pragma unroll
+    for (int k = 0; k < kK; k++) {
+        const double fz = (z0 + zl + kZLanes * k) - cz;
+        px[k] = D[0] * fx + D[3] * fy + D[6] * fz;
+        py[k] = D[1] * fx + D[4] * fy + D[7] * fz;
+        pz[k] = D[2] * fx + D[5] * fy + D[8] * fz;
+    }
+    const int nk = min(kK, (A.n[2] - z0 + kZLanes - 1) / kZLanes);
+    double aL[kK][4], aE[kK][4];
+#pragma unroll
+    for (int k = 0; k < kK; k++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) { aL[k][c] = 0; aE[k][c] = 0; }
+
+    for (int i0 = 0; i0 < A.natoms; i0 += kTile) {
+        const int nt = min(kTile, A.natoms - i0);
+        __syncthreads();
+        for (int j = t; j < nt; j += kThreads) {
+            const AtomRec a = atoms[i0 + j];
+            lpos[j] = make_double4(a.x - ccx, a.y - ccy, a.z - ccz, a.q);
+            llj[j] = make_double2(a.c6, a.c12);
+        }
+        __syncthreads();
+        for (int j = 0; j < nt; j++) {
+            const double4 a = lpos[j];
+            const double2 c = llj[j];
 #pragma unroll
             for (int k = 0; k < kK; k++) {
-                if (!(near_mask & (1u << k))) continue;
-                const float dx = (px[k] - hx) - lx, dy = (py[k] - hy) - ly, dz = (pz[k] - hz) - lz;
-                const double fx = ix - cx, fy = iy - cy, fz = (z0 + zl + kZLanes * k) - cz;
-                const double ddx = (D[0] * fx + D[3] * fy + D[6] * fz) - rx;
-                const double ddy = (D[1] * fx + D[4] * fy + D[7] * fz) - ry;
-                const double ddz = (D[2] * fx + D[5] * fy + D[8] * fz) - rz;
-                const bool n0 = (LJ || Z0) && r2_f32(dx, dy, dz, A.r2safe) < near2s;
-                if (LJ && n0) {
-                    const double ir2 = 1.0 / (ddx * ddx + ddy * ddy + ddz * ddz + A.r2safe_d);
-                    const double ir6 = ir2 * ir2 * ir2;
-                    const double e6 = a.c6 * ir6, e12 = a.c12 * ir6 * ir6;
-                    const double s = (12 * e12 - 6 * e6) * ir2;
-                    aL[k][0] += ddx * s; aL[k][1] += ddy * s; aL[k][2] += ddz * s;
-                    if (WE) aL[k][3] += e12 - e6;
-                }
-#pragma unroll
-                for (int s_ = 0; s_ < NS; s_++) {
-                    bool nr;
-                    double ddzs;
-                    if (Z0 && s_ == 0) {
-                        nr = n0; ddzs = ddz;
-                    } else {
-                        nr = r2_f32(dx, dy, dz + A.sdz[s_], A.r2safe) < near2s;
-                        ddzs = ddz + (double)A.sdz[s_];
-                    }
-                    if (nr) {
-                        const double ir2 = 1.0 / (ddx * ddx + ddy * ddy + ddzs * ddzs + A.r2safe_d);
-                        const double e = sqrt(ir2) * (a.q * (double)A.sw[s_]);
-                        const double s = e * ir2;
-                        aE[k][0] += ddx * s; aE[k][1] += ddy * s; aE[k][2] += ddzs * s;
-                        if (WE) aE[k][3] += e;
-                    }
-                }
+                if (k >= nk) break;
+                pair_fp64<LJ, NS, WE>(A, px[k] - a.x, py[k] - a.y, pz[k] - a.z, a.w, c.x, c.y, aL[k], aE[k]);
             }
         }
     }
@@ -263,22 +339,21 @@ __global__ __launch_bounds__(kThreads) void ff_build_f32(FFArgs A, const AtomRec
     for (int k = 0; k < kK; k++) {
         const int iz = z0 + zl + kZLanes * k;
         if (iz >= A.n[2]) break;
-        double fx, fy, fz, e;
+        double gx, gy, gz, e;
         if (A.mode == 0) {
             clamp_vec(aL[k][0], aL[k][1], aL[k][2], A.fmax);
             clamp_vec(aE[k][0], aE[k][1], aE[k][2], A.fmax);
-            fx = aL[k][0] + A.probe_q * aE[k][0];
-            fy = aL[k][1] + A.probe_q * aE[k][1];
-            fz = aL[k][2] + A.probe_q * aE[k][2];
+            gx = aL[k][0] + A.probe_q * aE[k][0];
+            gy = aL[k][1] + A.probe_q * aE[k][1];
+            gz = aL[k][2] + A.probe_q * aE[k][2];
             e = aL[k][3] + A.probe_q * aE[k][3];
         } else {
-            fx = aL[k][0] + aE[k][0];
-            fy = aL[k][1] + aE[k][1];
-            fz = aL[k][2] + aE[k][2];
+            gx = aL[k][0] + aE[k][0];
+            gy = aL[k][1] + aE[k][1];
+            gz = aL[k][2] + aE[k][2];
             e = aL[k][3] + aE[k][3];
         }
-        const long ig = ((long)ix * A.n[1] + iy) * A.n[2] + iz;
-        out[ig] = make_float4((float)fx, (float)fy, (float)fz, WE ? (float)e : 0.f);
+        out[((long)ix * A.n[1] + iy) * A.n[2] + iz] = make_double4(gx, gy, gz, WE ? e : 0.0);
     }
 }
 
@@ -368,21 +443,24 @@ __global__ __launch_bounds__(256) void ff_accum_f64(F64Args A, const AtomRec *__
 }
 
 template <bool LJ, int NS, bool Z0, bool WE>
-void launch_t(const FFArgs &A, const AtomRec *atoms, float4 *out, hipStream_t st)
+void launch_t(const FFArgs &A, const AtomRec *atoms, void *out, bool f64, hipStream_t st)
 {
     dim3 grid((unsigned)A.nbx * A.nby * A.nbz), block(kThreads);
-    hipLaunchKernelGGL((ff_build_f32<LJ, NS, Z0, WE>), grid, block, 0, st, A, atoms, out);
+    if (f64)
+        hipLaunchKernelGGL((ff_build_f64<LJ, NS, WE>), grid, block, 0, st, A, atoms, (double4 *)out);
+    else
+        hipLaunchKernelGGL((ff_build_f32<LJ, NS, Z0, WE>), grid, block, 0, st, A, atoms, (float4 *)out);
 }
 
 template <bool LJ, bool WE>
-int launch_ns(int ns, bool z0, const FFArgs &A, const AtomRec *atoms, float4 *out, hipStream_t st)
+int launch_ns(int ns, bool z0, const FFArgs &A, const AtomRec *atoms, void *out, bool f64, hipStream_t st)
 {
     switch (ns) {
-    case 0: launch_t<LJ, 0, false, WE>(A, atoms, out, st); break;
-    case 1: z0 ? launch_t<LJ, 1, true, WE>(A, atoms, out, st) : launch_t<LJ, 1, false, WE>(A, atoms, out, st); break;
-    case 2: z0 ? launch_t<LJ, 2, true, WE>(A, atoms, out, st) : launch_t<LJ, 2, false, WE>(A, atoms, out, st); break;
-    case 3: z0 ? launch_t<LJ, 3, true, WE>(A, atoms, out, st) : launch_t<LJ, 3, false, WE>(A, atoms, out, st); break;
-    case 4: z0 ? launch_t<LJ, 4, true, WE>(A, atoms, out, st) : launch_t<LJ, 4, false, WE>(A, atoms, out, st); break;
+    case 0: launch_t<LJ, 0, false, WE>(A, atoms, out, f64, st); break;
+    case 1: z0 ? launch_t<LJ, 1, true, WE>(A, atoms, out, f64, st) : launch_t<LJ, 1, false, WE>(A, atoms, out, f64, st); break;
+    case 2: z0 ? launch_t<LJ, 2, true, WE>(A, atoms, out, f64, st) : launch_t<LJ, 2, false, WE>(A, atoms, out, f64, st); break;
+    case 3: z0 ? launch_t<LJ, 3, true, WE>(A, atoms, out, f64, st) : launch_t<LJ, 3, false, WE>(A, atoms, out, f64, st); break;
+    case 4: z0 ? launch_t<LJ, 4, true, WE>(A, atoms, out, f64, st) : launch_t<LJ, 4, false, WE>(A, atoms, out, f64, st); break;
     default: set_error("n_stencil must be 0..4"); return -2;
     }
     return 0;
@@ -397,8 +475,8 @@ int first_zero_index(const ppafm_ff_model &m)
 
 }  // namespace
 
-int launch_ff_build_f32(const GridGeom &g, const ppafm_ff_model &m, int natoms, const AtomRec *atoms_dev,
-                        float4 *out, hipStream_t st)
+int launch_ff_build(const GridGeom &g, const ppafm_ff_model &m, int natoms, const AtomRec *atoms_dev, void *out,
+                    bool f64, hipStream_t st)
 {
     PP_REQUIRE(g.n[0] > 0 && g.n[1] > 0 && g.n[2] > 0, "grid dimensions must be positive");
     PP_REQUIRE(m.n_stencil >= 0 && m.n_stencil <= 4, "n_stencil must be 0..4");
@@ -429,13 +507,20 @@ int launch_ff_build_f32(const GridGeom &g, const ppafm_ff_model &m, int natoms, 
     A.with_lj = m.with_lj;
     A.probe_q = m.probe_q;
     A.fmax = m.fmax;
+    // FP64 radius: at least 0.2 A beyond the farthest stencil offset, so no
+    // FP32 pair (LJ or shifted Coulomb point) comes closer than 0.2 A
+    double maxdz = 0;
+    for (int s = 0; s < ns; s++) maxdz = fmax(maxdz, fabs((double)A.sdz[s]));
+    const double rmin = kMinFp64Radius + maxdz;
+    const double rn = m.fp64_radius > rmin ? m.fp64_radius : rmin;
+    A.near2 = (float)(rn * rn);
     const bool we = m.with_energy != 0;
     if (m.with_lj) {
-        if (we) return launch_ns<true, true>(A.ns, z0, A, atoms_dev, out, st);
-        return launch_ns<true, false>(A.ns, z0, A, atoms_dev, out, st);
+        if (we) return launch_ns<true, true>(A.ns, z0, A, atoms_dev, out, f64, st);
+        return launch_ns<true, false>(A.ns, z0, A, atoms_dev, out, f64, st);
     }
-    if (we) return launch_ns<false, true>(A.ns, z0, A, atoms_dev, out, st);
-    return launch_ns<false, false>(A.ns, z0, A, atoms_dev, out, st);
+    if (we) return launch_ns<false, true>(A.ns, z0, A, atoms_dev, out, f64, st);
+    return launch_ns<false, false>(A.ns, z0, A, atoms_dev, out, f64, st);
 }
 
 int launch_ff_accum_f64(const GridGeom &g, int model, int natoms, const AtomRec *atoms_dev, double *F, double *E,

@@ -55,9 +55,9 @@ hipStream_t as_stream(void *s);
 // Launchers implemented in the .hip files.
 namespace pp {
 
-// FP32 device grid (z fastest, float4).
-int launch_ff_build_f32(const GridGeom &g, const ppafm_ff_model &m, int natoms, const AtomRec *atoms_dev,
-                        float4 *out, hipStream_t st);
+// Device grid (z fastest): float4 (f64 = false) or double4 (f64 = true).
+int launch_ff_build(const GridGeom &g, const ppafm_ff_model &m, int natoms, const AtomRec *atoms_dev, void *out,
+                    bool f64, hipStream_t st);
 // FP64 accumulate (+=) into double3 grid F (and E) for the host ABI.
 // model: 0 = LJ, 1..3 = Coulomb kind 0..2.
 int launch_ff_accum_f64(const GridGeom &g, int model, int natoms, const AtomRec *atoms_dev, double *F, double *E,
@@ -71,11 +71,12 @@ struct RelaxLaunch {
     const double *tips;  // optional explicit tip positions (nx,ny,nz,3)
     ppafm_relax_out out;
 };
-int launch_relax_f32grid(const RelaxLaunch &L, const float4 *grid, hipStream_t st);
+int launch_relax_dev(const RelaxLaunch &L, const void *grid, bool f64, hipStream_t st);
 int launch_relax_f64grid(const RelaxLaunch &L, const double *grid, double *pos, double *force, int *iters,
                          hipStream_t st);
 
 int launch_limit_vec_field(double *F, long npts, double fmax, hipStream_t st);
+int launch_copy(const void *src, void *dst, long n16, hipStream_t st);
 int launch_convolve_z(long ncol, int nz, int in_stride, int in_step, int in_offset, const void *in, int nw,
                       const double *w, double scale, void *out, int f64, hipStream_t st);
 

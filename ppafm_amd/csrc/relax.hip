@@ -117,6 +117,24 @@ struct GridF32 {
     }
 };
 
+// FP64 double4 grid of the exact device path.
+struct GridD4 {
+    const double4 *__restrict__ g;
+    __device__ __forceinline__ void sample(const int n[3], d3 gc, double out[4]) const
+    {
+        const Corners c = corners(n, gc);
+        double4 v[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) v[k] = g[c.idx[k]];
+        out[0] = v[0].x * c.w[0]; out[1] = v[0].y * c.w[0]; out[2] = v[0].z * c.w[0]; out[3] = v[0].w * c.w[0];
+#pragma unroll
+        for (int k = 1; k < 8; k++) {
+            out[0] += v[k].x * c.w[k]; out[1] += v[k].y * c.w[k];
+            out[2] += v[k].z * c.w[k]; out[3] += v[k].w * c.w[k];
+        }
+    }
+};
+
 // FP64 (nx,ny,nz,3) grid of the host ABI.
 struct GridF64 {
     const double *__restrict__ g;
@@ -366,7 +384,7 @@ int check(const RelaxLaunch &L)
 
 }  // namespace
 
-int launch_relax_f32grid(const RelaxLaunch &L, const float4 *grid, hipStream_t st)
+int launch_relax_dev(const RelaxLaunch &L, const void *grid, bool f64, hipStream_t st)
 {
     int e = check(L);
     if (e) return e;
@@ -375,7 +393,12 @@ int launch_relax_f32grid(const RelaxLaunch &L, const float4 *grid, hipStream_t s
     if (nt == 0 || A.nz == 0) return 0;
     const long nblk = ((nt + 7) / 8) * 8;
     PP_REQUIRE(nblk < (1L << 31), "scan too large");
-    hipLaunchKernelGGL((relax_kernel<GridF32, false>), dim3((unsigned)nblk), dim3(64), 0, st, A, GridF32{grid});
+    if (f64)
+        hipLaunchKernelGGL((relax_kernel<GridD4, false>), dim3((unsigned)nblk), dim3(64), 0, st, A,
+                           GridD4{(const double4 *)grid});
+    else
+        hipLaunchKernelGGL((relax_kernel<GridF32, false>), dim3((unsigned)nblk), dim3(64), 0, st, A,
+                           GridF32{(const float4 *)grid});
     return 0;
 }
 

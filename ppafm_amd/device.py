@@ -94,8 +94,9 @@ class Event:
 # ---------------------------------------------------------------- descriptors
 
 
-def grid_desc(n, cell):
+def grid_desc(n, cell, f64=False):
     g = _lib.Grid()
+    g.f64 = int(bool(f64))
     for k in range(3):
         g.n[k] = int(n[k])
     c = np.asarray(cell, dtype=np.float64).reshape(3, 3)
@@ -104,7 +105,10 @@ def grid_desc(n, cell):
     return g
 
 
-def ff_model(mode, with_lj=True, stencil=(), with_energy=False, coulomb_const=None, probe_q=0.0, fmax=10.0):
+FP64_RADIUS = 4.0  # [A] pairs closer than this are summed in FP64 by the grid kernel
+
+
+def ff_model(mode, with_lj=True, stencil=(), with_energy=False, coulomb_const=None, probe_q=0.0, fmax=10.0, fp64_radius=None):
     """stencil: sequence of (weight, dz) Coulomb evaluation points."""
     m = _lib.FFModel()
     m.mode = int(mode)
@@ -122,6 +126,7 @@ def ff_model(mode, with_lj=True, stencil=(), with_energy=False, coulomb_const=No
     m.coulomb_const = float(coulomb_const)
     m.probe_q = float(probe_q)
     m.fmax = float(fmax) if fmax is not None else 0.0
+    m.fp64_radius = float(FP64_RADIUS if fp64_radius is None else fp64_radius)
     return m
 
 
@@ -183,12 +188,12 @@ def pack_atoms(xyzq, cLJs, coulomb_const):
     return out[:n]
 
 
-def ff_build(n, cell, xyzq, cLJs, model, out=None, stream=None, atoms_dev=None):
-    """Build the FP32 float4 grid (nx,ny,nz,4) on the device."""
+def ff_build(n, cell, xyzq, cLJs, model, out=None, stream=None, atoms_dev=None, f64=False):
+    """Build the (nx,ny,nz,4) grid on the device: float32, or float64 when f64."""
     n = tuple(int(v) for v in n)
     if out is None:
-        out = DeviceArray(n + (4,), np.float32)
-    g = grid_desc(n, cell)
+        out = DeviceArray(n + (4,), np.float64 if f64 else np.float32)
+    g = grid_desc(n, cell, f64=out.dtype == np.float64)
     if atoms_dev is None:
         packed = pack_atoms(xyzq, cLJs, model.coulomb_const)
         atoms_dev = DeviceArray.from_host(packed if len(packed) else np.zeros((1, 6)))
@@ -216,7 +221,7 @@ def relax(n, cell, grid_dev, rp, tip, scan, tips_dev=None, pos=None, force=None,
     o.force = force.ptr if force is not None else None
     o.iters = iters.ptr if iters is not None else None
     o.counters = counters.ptr if counters is not None else None
-    g = grid_desc(n, cell)
+    g = grid_desc(n, cell, f64=grid_dev.dtype == np.float64)
     check(
         lib().ppafm_relax(ctypes.byref(g), grid_dev.ptr, ctypes.byref(rp), ctypes.byref(tip), ctypes.byref(scan), tips_dev.ptr if tips_dev is not None else None, ctypes.byref(o), stream),
         "relax",

@@ -9,8 +9,10 @@ stroke) and the df stack is convolveZ with get_simple_df_weight
 (relax.cl:571-608, AFMulator.py:226-229).  Interpolation is exact software
 trilinear, where the reference uses the hardware image filter.
 
-Not on this path (raise NotImplementedError): Hartree potentials, tip
-densities ``rho`` / ``rho_delta`` and the FDBM method.
+Not on this path (raise NotImplementedError): Hartree potentials and the
+FDBM method (``rho_sample`` / ``rho_delta``).  A tip density ``rho`` dict is
+kept for parameter I/O; like the reference, the point-charge method does not
+use it.
 """
 
 import os
@@ -22,6 +24,16 @@ from .. import common
 from .. import device as D
 
 VALID_SIZES = np.array([16, 32, 64, 128, 192, 256, 384, 512, 768, 1024, 1536, 2048])
+
+
+class _Scanner:
+    """Holds the relaxation settings the reference keeps on its
+    RelaxedScanner (ocl/relax.py:93-110)."""
+
+    def __init__(self, stiffness, relax_params):
+        self.stiffness = stiffness
+        self.relax_params = relax_params
+        self.zstep = 0.1
 
 
 class AFMulator:
@@ -55,8 +67,7 @@ class AFMulator:
         colorscale="gray",
         minimize_memory=False,
     ):
-        self.stiffness = np.array(tipStiffness, dtype=np.float32) / -common.eVA_Nm
-        self.relax_params = np.array(self.relaxParams, dtype=np.float32)
+        self.scanner = _Scanner(np.array(tipStiffness, dtype=np.float32) / -common.eVA_Nm, np.array(self.relaxParams, dtype=np.float32))
         self.iZPP = iZPP
         self.tipR0 = tipR0
         self.f0Cantilever = f0Cantilever
@@ -78,6 +89,7 @@ class AFMulator:
         self.setLvec(lvec, pixPerAngstrome)
         self.setRho(rho, sigma, B_pauli)
         self.setRhoDelta(rho_delta)
+        self.A_pauli = A_pauli
         self.setQs(Qs, QZs)
         self.typeParams = common.loadSpecies("atomtypes.ini")
 
@@ -121,11 +133,23 @@ class AFMulator:
         self.amplitude = self.dz * len(self.dfWeight)
         self._out = None
 
+    @property
+    def stiffness(self):
+        return self.scanner.stiffness
+
+    @property
+    def relax_params(self):
+        return self.scanner.relax_params
+
     def setRho(self, rho=None, sigma=0.71, B_pauli=1.0):
-        if rho is not None and len(rho) > 0:
-            raise NotImplementedError("tip charge densities (rho) are not on this path; use point charges Qs/QZs")
-        self._rho = None
-        self.rho = None
+        """Remember a multipole tip density dict (used by parameter I/O; the
+        point-charge force field does not use it, as in the reference)."""
+        if rho is not None and not isinstance(rho, dict):
+            raise NotImplementedError("grid tip densities are not on this path; pass a multipole dict or None")
+        self.sigma = sigma
+        self.B_pauli = B_pauli
+        self._rho = rho
+        self.rho = rho
 
     def setBPauli(self, B_pauli=1.0):
         self.B_pauli = B_pauli
@@ -142,7 +166,7 @@ class AFMulator:
         self.QZs = QZs
 
     def setStiffness(self, tipStiffness):
-        self.stiffness = np.array(tipStiffness, dtype=np.float32) / -common.eVA_Nm
+        self.scanner.stiffness = np.array(tipStiffness, dtype=np.float32) / -common.eVA_Nm
 
     # --------------------------------------------------------------- imaging
     def prepareFF(self, xyzs, Zs, qs, rho_sample=None, sample_lvec=None, rot=np.eye(3), rot_center=None, REAs=None):
@@ -177,7 +201,8 @@ class AFMulator:
         self._model = D.ff_model(1, with_lj=True, stencil=stencil, with_energy=True, fmax=None)
         n = tuple(int(v) for v in self.nDim[:3])
         if self._grid is None or self._nDim_alloc != n:
-            self._grid = D.DeviceArray(n + (4,), np.float32)
+            # double4 grid summed in FP64 (see ppafm_amd.scan.PlainScan for why)
+            self._grid = D.DeviceArray(n + (4,), np.float64)
             self._nDim_alloc = n
         cell = self.lvec[1:].astype(np.float32).astype(np.float64)
         D.ff_build(n, cell, xyzq, cLJs, self._model, out=self._grid)
@@ -228,7 +253,7 @@ class AFMulator:
     # ------------------------------------------------------------ accessors
     def downloadFF(self):
         """Force field (nx, ny, nz, 4) float32 (Fx, Fy, Fz, E)."""
-        return self._grid.download()
+        return self._grid.download().astype(np.float32)
 
     def downloadPaths(self):
         """Probe positions (nx, ny, nz, 3) in the molecule frame (relax.py:282-305)."""
@@ -245,7 +270,7 @@ class AFMulator:
     def load_params(self, file_path="./params.ini"):
         parameters, sample_lvec = _get_params(file_path)
         if parameters["tipStiffness"] is not None:
-            self.stiffness = np.array(parameters["tipStiffness"], dtype=np.float32) / -common.eVA_Nm
+            self.scanner.stiffness = np.array(parameters["tipStiffness"], dtype=np.float32) / -common.eVA_Nm
         self.iZPP = parameters["iZPP"]
         self.tipR0 = parameters["tipR0"]
         self.f0Cantilever = parameters["f0Cantilever"]
@@ -254,6 +279,7 @@ class AFMulator:
         self.A_pauli = parameters["A_pauli"]
         self.setScanWindow(parameters["scan_window"], parameters["scan_dim"], parameters["df_steps"])
         self.setLvec(parameters["lvec"], parameters["pixPerAngstrome"])
+        self.setRho(parameters["rho"], parameters["sigma"], parameters["B_pauli"])
         self.sample_lvec = sample_lvec
         self.colorscale = parameters["colorscale"]
 
@@ -267,6 +293,13 @@ class AFMulator:
         )
         with open(file_path, "w") as f:
             f.write(f"probeType {self.iZPP}\n")
+            if isinstance(self._rho, dict) and len(self._rho) > 0:
+                if len(self._rho) > 1:
+                    warnings.warn("More than one tip multipole type specified. Only writing the first one into params.ini.")
+                tip = list(self._rho)[0]
+                f.write(f"tip {tip}\n")
+                f.write(f"charge {self._rho[tip]}\n")
+                f.write(f"sigma {self.sigma}\n")
             f.write(f"stiffness {k[0]} {k[1]} {k[3]}\n")
             f.write(f"r0Probe {self.tipR0[0]} {self.tipR0[1]} {self.tipR0[2]}\n")
             f.write(f"PBC {(np.array(self.npbc) > 0).any()}\n")
@@ -333,7 +366,7 @@ def _get_params(file_path):
         "scan_dim": scan_dim,
         "scan_window": scan_window,
         "iZPP": iZPP,
-        "rho": None,
+        "rho": {parameters.tip: parameters.charge},
         "sigma": parameters.sigma,
         "A_pauli": parameters.Apauli,
         "B_pauli": parameters.Bpauli,

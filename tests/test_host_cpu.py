@@ -1,0 +1,142 @@
+"""Host-side checks that need no GPU: the C ABI library loads and exports
+every entry point its header declares, the Python mirrors of the reference
+API behave like the reference on host-only logic, and the synthetic input
+generator has the properties the benchmark relies on."""
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from ppafm_amd import _lib
+from ppafm_amd import common as C
+from ppafm_amd import molgen as M
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "ppafm_hip.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_ ]*?[\s\*]+([A-Za-z_][A-Za-z0-9_]*)\s*\(", text, flags=re.M)
+    return sorted(set(n for n in names if n not in ("if", "while", "return")))
+
+
+def test_library_exports_every_declared_symbol():
+    L = ctypes.CDLL(str(_lib.LIB_PATH))
+    names = header_functions()
+    assert len(names) >= 30
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    # and the Python binding declares each of them
+    bound = {n for n, _, _ in _lib.SIGNATURES}
+    assert set(names) <= bound, set(names) - bound
+
+
+def test_df_weights_reference_golden_vectors():
+    # ppafm tests/test_common.py:10-24
+    assert np.allclose(C.get_df_weight(0.1), [-5.0, 5.0])
+    assert np.allclose(C.get_df_weight(1.0, dz=0.2), [-0.35594622, -0.22193265, -0.05447093, 0.05447093, 0.22193265, 0.35594622])
+    assert np.allclose(C.get_simple_df_weight(n=5, dz=0.2), [-0.31362841, -0.37274317, 0, 0.37274317, 0.31362841])
+    # the benchmark's 8-step window: n = ceil(0.8/0.1) = 8 intervals, 9 taps
+    assert len(C.get_df_weight(0.8, 0.1)) == 9
+
+
+def test_lj_mixing_closed_form():
+    sp = C.loadSpecies()
+    cLJ = C.getAtomsLJ(8, np.array([6, 1]), sp)
+    R = 1.6612 + 1.9080
+    e = np.sqrt(0.009106 * 0.003729)
+    assert np.allclose(cLJ[0], [2 * e * R**6, e * R**12])
+    REA = C.getAtomsREA(8, np.array([6, 1]), sp)
+    assert np.allclose(C.REA2LJ(REA), cLJ)
+
+
+def test_pbc_images_order():
+    Zs, Rs, Qs = C.PBCAtoms(np.array([6, 8]), np.array([[0.0, 0, 0], [1, 1, 1]]), np.array([0.1, -0.1]), avec=[2, 0, 0], bvec=[0, 3, 0], na=1, nb=1)
+    assert len(Zs) == 18 and np.allclose(Rs[0], [-2, -3, 0]) and np.allclose(Rs[-1], [3, 4, 1])
+    z3, x3, q3, c3, r3 = C.PBCAtoms3D_np(np.array([6]), np.array([[0.5, 0.5, 0.5]]), np.array([0.2]), np.array([[1.0, 2.0]]), None, np.eye(3) * 4, npbc=(1, 1, 0))
+    assert len(z3) == 9 and np.allclose(x3[0], [0.5, 0.5, 0.5]) and np.allclose(x3[1], [-3.5, -3.5, 0.5])
+
+
+def test_params_defaults_and_ini():
+    p = C.PpafmParameters()
+    assert p.PBC is True and p.nPBC == [1, 1, 1] and p.probeType == "O" and p.r0Probe == [0.0, 0.0, 4.0]
+    q = C.PpafmParameters.from_file(os.path.join(GOLDEN, "test_params.ini"))
+    assert q.PBC is False and q.nPBC == [2, 3, 4] and q.probeType == "Xe" and q.tip == "p"
+    assert np.allclose(q.gridA, [5, 6, 7]) and np.allclose(q.scanStep, [0.1, 0.2, 0.3]) and q.vdWDampKind == 3
+    with pytest.raises(ValueError):
+        C.PpafmParameters().load_ini(["notAParameter 1"])
+
+
+def test_prepare_scan_grids():
+    p = C.PpafmParameters(scanMin=[0.0, 0.0, 5.0], scanMax=[1.0, 2.0, 6.0], scanStep=[0.5, 0.5, 0.1], r0Probe=[0.0, 0.0, 4.0])
+    x, y, z, lv = C.prepareScanGrids(parameters=p)
+    assert len(x) == 3 and len(y) == 5 and len(z) == 11
+    assert np.allclose(lv[0], [0, 0, 1.0])
+
+
+def test_afmulator_params_round_trip(tmp_path):
+    # mirrors ppafm tests/test_afmulator.py:10-56 (host-only part)
+    from ppafm_amd.ocl.AFMulator import AFMulator
+
+    a = AFMulator(pixPerAngstrome=15, lvec=np.array([[1, 1, 1], [5, 1, 0], [2, 5, 0], [0, 0, 8]]), scan_dim=(200, 200, 60),
+                  scan_window=((0.0, 0.0, 16.0), (19.9, 19.9, 22.0)), iZPP=8, df_steps=10, tipStiffness=(0.37, 0.37, 0.0, 20.0),
+                  rho={"dz2": -0.05}, sigma=0.8, A_pauli=15.0, B_pauli=1.0, tipR0=[0.0, 0.0, 4.0], npbc=(1, 1, 1),
+                  f0Cantilever=30000, kCantilever=2000)
+    path = str(tmp_path / "params_test.ini")
+    a.save_params(path)
+    b1 = AFMulator.from_params(path)
+    b2 = AFMulator()
+    b2.load_params(path)
+    for b in (b1, b2):
+        assert np.allclose(b.scan_dim, a.scan_dim) and np.allclose(b.scan_window, a.scan_window)
+        assert np.allclose(b.iZPP, a.iZPP) and np.allclose(b.df_steps, a.df_steps)
+        assert np.allclose(b.scanner.stiffness, a.scanner.stiffness) and np.allclose(b.tipR0, a.tipR0)
+        assert np.allclose(b.f0Cantilever, a.f0Cantilever) and np.allclose(b.kCantilever, a.kCantilever)
+        assert np.allclose(b.npbc, a.npbc) and np.allclose(b.lvec, a.lvec) and np.allclose(b.pixPerAngstrome, a.pixPerAngstrome)
+        assert b._rho == a._rho and np.allclose(b.sigma, a.sigma) and np.allclose(b.A_pauli, a.A_pauli) and np.allclose(b.B_pauli, a.B_pauli)
+
+
+def test_get_lvec_valid_sizes():
+    from ppafm_amd.ocl.AFMulator import get_lvec
+
+    lv = get_lvec(((2.0, 2.0, 7.0), (18.0, 18.0, 10.0)))
+    # 16 + 6 = 22 A -> 220 points -> 256; z: 3 + 10 = 13 A -> 130 -> 192
+    assert np.allclose(lv[1:], np.diag([25.6, 25.6, 19.2]))
+    assert np.allclose(lv[0], [10 - 12.8, 10 - 12.8, 8.5 - 9.6 - 3.0])
+
+
+def test_afmulator_bad_df_steps():
+    from ppafm_amd.ocl.AFMulator import AFMulator
+
+    with pytest.raises(ValueError):
+        AFMulator(scan_dim=(8, 8, 5), df_steps=6)
+
+
+def test_core_rejects_wrong_arrays():
+    from ppafm_amd import core
+
+    with pytest.raises(TypeError):
+        core.getLennardJonesFF(np.zeros((3, 3), np.float32), np.zeros((3, 2)))
+    with pytest.raises(TypeError):
+        core.relaxTipStrokes_omp(np.zeros((2, 2, 3)), np.zeros((2, 2, 3)), np.zeros((2, 2, 3)))
+
+
+def test_molecule_generator_properties():
+    a = M.config_molecule(0, 0, 36)
+    b = M.config_molecule(0, 0, 36)
+    assert all(np.array_equal(x, y) for x, y in zip(a, b))
+    xyz, Zs, qs = a
+    assert xyz.shape == (36, 3) and abs(qs.sum()) < 1e-12 and np.abs(qs).max() <= 0.6
+    assert np.abs(xyz[:, 2]).max() <= 0.3 and np.allclose(xyz[:, :2].mean(0), 0, atol=1e-12)
+    d = np.linalg.norm(xyz[:, None, :2] - xyz[None, :, :2], axis=-1)
+    np.fill_diagonal(d, 10)
+    assert np.allclose(d.min(axis=1), M.BOND, atol=1e-9)  # every atom has a bonded neighbour
+    assert set(np.unique(Zs)) <= {1, 6, 7, 8}
+    # batch config: atom count drawn from U[8,120]
+    counts = [len(M.config_molecule(2, i)[0]) for i in range(40)]
+    assert min(counts) >= 8 and max(counts) <= 120 and len(set(counts)) > 10

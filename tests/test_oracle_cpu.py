@@ -1,0 +1,272 @@
+"""Pins of the FP64 oracle (CPU only).
+
+Each oracle function is checked against something that does not retype its
+formula: the compiled reference C++ (golden fixtures from
+tests/golden/make_golden.py and, where built, oracle/_ref live), the
+reference's own golden df-weight vectors (tests/test_common.py:10-24), and
+closed forms / invariants: F = -grad E, the LJ minimum at R_ij with depth
+eps_ij, Coulomb's law, exact trilinear reproduction of affine fields and
+periodicity, the analytic tip equilibrium, 1-D force balance by bisection,
+and Giessibl's linear-force limit of the df weights."""
+
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+# ------------------------------------------------------------ golden fixtures
+
+
+def test_grid_golden_bit_exact():
+    g = np.load(os.path.join(GOLDEN, "grid.npz"))
+    shp = tuple(g["n"])
+    F = np.zeros(shp + (3,))
+    E = np.zeros(shp)
+    O.lj_ff(g["n"], g["cell"], g["Rs"], g["cLJs"], F=F, E=E)
+    assert np.array_equal(F, g["F_lj"]) and np.array_equal(E, g["E_lj"])
+    for kind in (0, 1, 2):
+        F[:] = 0
+        E[:] = 0
+        O.coulomb_ff(g["n"], g["cell"], g["Rs"], g["kQQs"], kind=kind, F=F, E=E)
+        assert np.array_equal(F, g[f"F_coul{kind}"]), kind
+        assert np.array_equal(E, g[f"E_coul{kind}"]), kind
+
+
+def _golden_tip(r):
+    return O.tip_params(float(r["tip_lRadial"]), float(r["tip_kRadial"]), r["tip_rPP0"], r["tip_kSpring"])
+
+
+@pytest.mark.parametrize("ps", [1, 0, -1])
+def test_relax_golden_bit_exact(ps):
+    r = np.load(os.path.join(GOLDEN, "relax.npz"))
+    rs, fs, it, tot = O.relax_strokes(r["n"], r["cell"], r["FF"], _golden_tip(r), r["tips"], probeStart=ps)
+    assert np.array_equal(rs, r[f"rs_ps{ps}"])
+    assert np.array_equal(fs, r[f"fs_ps{ps}"])
+
+
+def test_leapfrog_golden_bit_exact():
+    r = np.load(os.path.join(GOLDEN, "relax.npz"))
+    rs, fs, it, tot = O.relax_strokes(r["n"], r["cell"], r["FF"], _golden_tip(r), r["tips"], relax=(300, 1e-8, 0.1, 0.1, 1.1, 0.5, 0.99), relaxAlg=0)
+    assert np.array_equal(rs, r["rs_alg0"]) and np.array_equal(fs, r["fs_alg0"])
+
+
+def test_stroke_itrsum_golden():
+    r = np.load(os.path.join(GOLDEN, "relax.npz"))
+    tips = np.ascontiguousarray(r["tips"][2, 1])
+    rs = np.zeros_like(tips)
+    fs = np.zeros_like(tips)
+    n = np.ascontiguousarray(r["n"], dtype=np.int32)
+    tot = O.lib().orc_relax_stroke(
+        n.ctypes.data_as(O._ip), O._ptr(np.ascontiguousarray(r["cell"])), O._ptr(r["FF"]), O._ptr(_golden_tip(r)),
+        O._ptr(np.array(O.DEFAULT_RELAX, dtype=np.float64)), 1, 1, len(tips), O._ptr(tips), O._ptr(rs), O._ptr(fs), None, None,
+    )
+    assert tot == int(r["stroke_itrsum"])
+    assert np.array_equal(rs, r["stroke_rs"])
+
+
+@pytest.mark.skipif(not O.have_reference(), reason="oracle/_ref not built (reference sources absent)")
+def test_live_reference_random_case():
+    rng = np.random.default_rng(99)
+    n = np.array([9, 12, 10], dtype=np.int32)
+    cell = np.array([[0.9, 0.1, 0.0], [-0.1, 1.2, 0.05], [0.0, 0.0, 1.0]])
+    Rs = rng.uniform(0, 1, (7, 3))
+    cLJs = np.column_stack([rng.uniform(5, 30, 7), rng.uniform(1e3, 3e4, 7)])
+    ref = O.Reference()
+    F = np.zeros(tuple(n) + (3,))
+    ref.set_grid(n, cell, F)
+    ref.lj(Rs, cLJs)
+    assert np.array_equal(F, O.lj_ff(n, cell, Rs, cLJs))
+
+
+# ------------------------------------------------------------ closed forms
+
+
+def _single_atom_grid(c6, c12, kq, model="lj", n=(24, 24, 24), h=0.1, atom=(1.23, 1.07, 1.31)):
+    cell = np.diag([n[0] * h, n[1] * h, n[2] * h])
+    Rs = np.array([atom])
+    E = np.zeros(n)
+    if model == "lj":
+        F = O.lj_ff(n, cell, Rs, np.array([[c6, c12]]), E=E)
+    else:
+        F = O.coulomb_ff(n, cell, Rs, np.array([kq]), E=E)
+    x = np.arange(n[0]) * h
+    P = np.stack(np.meshgrid(x, np.arange(n[1]) * h, np.arange(n[2]) * h, indexing="ij"), axis=-1)
+    return F, E, P - np.array(atom), h
+
+
+def test_lj_force_is_minus_energy_gradient():
+    F, E, D, h = _single_atom_grid(23.6, 2.4e4, 0)
+    r = np.linalg.norm(D, axis=-1)
+    # central differences of E along x, where the point and its neighbours are > 2 A away
+    gx = -(E[2:, :, :] - E[:-2, :, :]) / (2 * h)
+    sel = r[1:-1] > 2.0
+    fx = F[1:-1, :, :, 0]
+    # O(h^2) truncation of the central difference: relative agreement ~1e-2 at 0.1 A
+    assert np.abs(gx[sel] - fx[sel]).max() <= 2e-2 * np.abs(fx[sel]).max()
+
+
+def test_lj_minimum_at_rij_with_depth_eps():
+    # c6 = 2 eps R^6, c12 = eps R^12 (common.py:851-857): E has its minimum -eps at r = R
+    eps, R = 0.0058, 3.5692
+    c6, c12 = 2 * eps * R**6, eps * R**12
+    n = np.array([1, 1, 1], dtype=np.int32)
+    for r, want_sign in ((R * 0.97, +1), (R * 1.03, -1)):
+        cell = np.eye(3)
+        F = np.zeros((1, 1, 1, 3))
+        E = np.zeros((1, 1, 1))
+        O.lj_ff(n, cell, np.array([[-r, 0.0, 0.0]]), np.array([[c6, c12]]), F=F, E=E)
+        assert np.sign(F[0, 0, 0, 0]) == want_sign  # repulsive inside R, attractive outside
+    F = np.zeros((1, 1, 1, 3))
+    E = np.zeros((1, 1, 1))
+    O.lj_ff(n, np.eye(3), np.array([[-R, 0.0, 0.0]]), np.array([[c6, c12]]), F=F, E=E)
+    # R2SAFE = 1e-8 moves the zero by ~1e-9 A: |F| ~ 4e-11 eV/A there
+    assert abs(F[0, 0, 0, 0]) < 1e-9 and abs(E[0, 0, 0] + eps) < 1e-12
+
+
+def test_coulomb_law():
+    F, E, D, h = _single_atom_grid(0, 0, 4.32, model="coul")
+    r = np.sqrt((D**2).sum(-1) + (1e-8))
+    sel = r > 0.5
+    assert np.allclose(E[sel] * r[sel], 4.32, rtol=1e-7)
+    Fn = np.linalg.norm(F, axis=-1)
+    assert np.allclose(Fn[sel] * r[sel] ** 2, 4.32, rtol=1e-6)
+    # repulsive for positive kqq: F parallel to d
+    assert (np.einsum("...i,...i->...", F, D)[sel] > 0).all()
+
+
+def test_limit_vec_field_keeps_direction():
+    rng = np.random.default_rng(1)
+    F = rng.normal(0, 10, (5, 4, 3, 3))
+    G = O.limit_vec_field(F.copy(), 10.0)
+    nF, nG = np.linalg.norm(F, axis=-1), np.linalg.norm(G, axis=-1)
+    assert np.allclose(np.minimum(nF, 10.0), nG, rtol=1e-14)
+    assert np.allclose(G / nG[..., None], F / nF[..., None], rtol=1e-13)
+
+
+def test_interpolation_exact_for_affine_fields_and_periodic():
+    rng = np.random.default_rng(5)
+    n = np.array([7, 6, 9], dtype=np.int32)
+    cell = np.array([[0.7, 0.0, 0.0], [0.1, 0.6, 0.0], [0.0, 0.05, 0.9]])
+    dcell, dinv = O.grid_shape(n, cell)
+    assert np.allclose(dcell @ dinv.T, np.eye(3), atol=1e-14)  # (dCell)(dCell^-T)^T = I
+    A = rng.normal(size=(3, 3))
+    b = rng.normal(size=3)
+    idx = np.stack(np.meshgrid(*[np.arange(k) for k in n], indexing="ij"), axis=-1)
+    P = idx @ dcell
+    F = np.ascontiguousarray(P @ A.T + b)
+    # inside the cell away from the wrapped last layer the interpolant is exact
+    g = rng.uniform(0, 1, (200, 3)) * (n - 1.001)
+    r = g @ dcell
+    got = O.interp(n, cell, F, r)
+    assert np.abs(got - (r @ A.T + b)).max() < 1e-12
+    # periodic: r and r + cell rows give the same value, negative coordinates too
+    r2 = rng.uniform(-3, 3, (200, 3))
+    base = O.interp(n, cell, F, r2)
+    for k in range(3):
+        assert np.abs(O.interp(n, cell, F, r2 + cell[k]) - base).max() < 1e-12
+        assert np.abs(O.interp(n, cell, F, r2 - 2 * cell[k]) - base).max() < 1e-12
+
+
+def test_zero_field_relaxes_to_tip_equilibrium():
+    n = np.array([10, 10, 10], dtype=np.int32)
+    cell = np.eye(3) * 2.0
+    F = np.zeros((10, 10, 10, 3))
+    kS = np.array([0.25, 0.25, 0.0]) / -16.0217662
+    tip = O.tip_params(3.0, 30 / -16.0217662, (0.2, -0.1, 0.0), kS)
+    rpp0 = np.array([0.2, -0.1, -np.sqrt(9 - 0.04 - 0.01)])
+    tips = np.zeros((2, 2, 5, 3))
+    tips[..., 0] = 0.3 + 0.2 * np.arange(2)[:, None, None]
+    tips[..., 1] = 0.4 + 0.3 * np.arange(2)[None, :, None]
+    tips[..., 2] = 5.0 - 0.1 * np.arange(5)[None, None, :]
+    rs, fs, it, tot = O.relax_strokes(n, cell, F, tip, tips, probeStart=1)
+    assert np.abs(rs - (tips + rpp0)).max() < 1e-12  # starts at the minimum: no motion
+    assert (it == 0).all()
+    # probe left behind (probeStart -1): FIRE must walk to the minimum, |F| < 1e-4
+    rs, fs, it, tot = O.relax_strokes(n, cell, F, tip, tips, probeStart=-1)
+    kmin = 0.25 / 16.0217662
+    assert np.abs(rs - (tips + rpp0)).max() < 1e-4 / kmin
+    assert (it[..., 1:] > 0).all() and (it < 1000).all()
+
+
+def test_on_axis_force_balance_by_bisection():
+    # one atom under the tip on a grid node column: the probe stays on the axis
+    # and stops where F_grid,z (piecewise linear in z) balances the springs
+    h = 0.1
+    n = np.array([20, 20, 60], dtype=np.int32)
+    cell = np.diag([2.0, 2.0, 6.0])
+    atom = np.array([[1.0, 1.0, 0.5]])
+    cLJ = np.array([[23.6, 2.4e4]])
+    F = O.limit_vec_field(O.lj_ff(n, cell, atom, cLJ), 10.0)
+    krad = 30 / -16.0217662
+    tip = O.tip_params(3.0, krad, (0.0, 0.0, 0.0), np.array([0.25, 0.25, 0.0]) / -16.0217662)
+    tips = np.zeros((1, 1, 1, 3))
+    tips[0, 0, 0] = (1.0, 1.0, 6.2)
+    rs, fs, it, tot = O.relax_strokes(n, cell, F, tip, tips)
+    col = F[10, 10, :, 2]
+
+    def fz(z):  # periodic piecewise-linear grid force along the column + radial spring
+        g = z / h
+        i = int(np.floor(g))
+        t = g - i
+        fg = col[i % 60] * (1 - t) + col[(i + 1) % 60] * t
+        d = z - 6.2
+        return fg + krad * (abs(d) - 3.0) * np.sign(d)
+
+    lo, hi = 2.0, 4.5
+    assert fz(lo) * fz(hi) < 0
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if fz(lo) * fz(mid) <= 0:
+            hi = mid
+        else:
+            lo = mid
+    z_star = 0.5 * (lo + hi)
+    assert abs(rs[0, 0, 0, 0] - 1.0) < 1e-12 and abs(rs[0, 0, 0, 1] - 1.0) < 1e-12
+    # |F| < 1e-4 at the stop; the local stiffness is > 1 eV/A^2 here
+    assert abs(rs[0, 0, 0, 2] - z_star) < 1e-4
+
+
+# ------------------------------------------------------------------- df
+
+
+def test_df_weights_reference_golden_vectors():
+    # ppafm tests/test_common.py:10-24
+    assert np.allclose(O.get_df_weight(0.1), [-5.0, 5.0])
+    assert np.allclose(O.get_df_weight(1.0, dz=0.2), [-0.35594622, -0.22193265, -0.05447093, 0.05447093, 0.22193265, 0.35594622])
+    assert np.allclose(O.get_simple_df_weight(n=5, dz=0.2), [-0.31362841, -0.37274317, 0, 0.37274317, 0.31362841])
+
+
+def test_df_weights_linear_force_limit():
+    # For F = c z the Giessibl integral is exact: df = -f0/(2 k0) dF/dz, i.e.
+    # the correlation gives -c/2 for every amplitude; a constant force gives 0.
+    for A, dz in ((0.8, 0.1), (1.0, 0.2), (0.35, 0.1)):
+        W = O.get_df_weight(A, dz)
+        assert abs(W.sum()) < 1e-12
+        z = np.arange(40) * dz
+        conv = np.convolve(1.7 * z, W, mode="valid")
+        assert np.allclose(conv, -0.85, atol=1e-12)
+
+
+def test_fz2df_matches_brute_force():
+    rng = np.random.default_rng(3)
+    Fz = rng.normal(size=(3, 4, 25))
+    W = O.get_df_weight(0.8, 0.1)
+    got = O.fz2df(Fz, 0.1, 1800.0, 30300.0, amplitude=0.8)
+    n = len(W)
+    want = np.zeros((3, 4, 25 - n + 1))
+    for j in range(25 - n + 1):
+        for k in range(n):
+            want[..., j] += Fz[..., j + k] * W[n - 1 - k]
+    assert np.allclose(got, want * 16.0217656 * 30300.0 / 1800.0, rtol=1e-13, atol=1e-13)
+
+
+def test_planted_sign_error_is_caught():
+    # a plausible mistake (dropping the reversal of the kernel) fails the
+    # reference golden vectors
+    w = O.get_df_weight(1.0, dz=0.2)[::-1]
+    assert not np.allclose(w, [-0.35594622, -0.22193265, -0.05447093, 0.05447093, 0.22193265, 0.35594622])
