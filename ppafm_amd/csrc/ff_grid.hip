@@ -270,11 +270,59 @@ __global__ __launch_bounds__(kThreads) void ff_build_f32(FFArgs A, const AtomRec
     }
 }
 
+__device__ __forceinline__ double rsq_nr(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const double h = 0.5 * x * y;
+        y = __fma_rn(y, __fma_rn(-h, y, 0.5), y);
+    }
+    return y;
+}
+
+// FP64 pair with reciprocal square roots (v_rsq_f64 + two Newton steps, a
+// few ulp) in place of IEEE divide and sqrt; the Coulomb point at zero z
+// offset shares the LJ pair's rsq.
+template <bool LJ, int NS, bool Z0, bool WE>
+__device__ __forceinline__ void pair_fp64_fast(const FFArgs &A, double dx, double dy, double dz, double q, double c6,
+                                               double c12, double aL[4], double aE[4])
+{
+    const double r2 = __fma_rn(dx, dx, __fma_rn(dy, dy, __fma_rn(dz, dz, A.r2safe_d)));
+    double ir0 = 0, ir20 = 0;
+    if (LJ || Z0) {
+        ir0 = rsq_nr(r2);
+        ir20 = ir0 * ir0;
+    }
+    if (LJ) {
+        const double ir6 = ir20 * ir20 * ir20;
+        const double e6 = c6 * ir6, e12 = c12 * ir6 * ir6;
+        const double s = __fma_rn(12.0, e12, -6.0 * e6) * ir20;
+        aL[0] = __fma_rn(dx, s, aL[0]); aL[1] = __fma_rn(dy, s, aL[1]); aL[2] = __fma_rn(dz, s, aL[2]);
+        if (WE) aL[3] += e12 - e6;
+    }
+#pragma unroll
+    for (int s_ = 0; s_ < NS; s_++) {
+        double ir, ir2, dzs;
+        if (Z0 && s_ == 0) {
+            ir = ir0; ir2 = ir20; dzs = dz;
+        } else {
+            dzs = dz + (double)A.sdz[s_];
+            ir = rsq_nr(__fma_rn(dx, dx, __fma_rn(dy, dy, __fma_rn(dzs, dzs, A.r2safe_d))));
+            ir2 = ir * ir;
+        }
+        const double e = ir * (q * (double)A.sw[s_]);
+        const double s = e * ir2;
+        aE[0] = __fma_rn(dx, s, aE[0]); aE[1] = __fma_rn(dy, s, aE[1]); aE[2] = __fma_rn(dzs, s, aE[2]);
+        if (WE) aE[3] += e;
+    }
+}
+
 // Exact variant: every pair in FP64 (pair_fp64) and a double4 grid, for
 // callers that need the grid to FP64 accuracy (the FIRE stop rule turns
 // grid perturbations of ~1e-7 into stopping-point changes of >1e-3 A on
 // ill-conditioned strokes).  Same brick mapping as ff_build_f32.
-template <bool LJ, int NS, bool WE>
+template <bool LJ, int NS, bool Z0, bool WE>
 __global__ __launch_bounds__(kThreads) void ff_build_f64(FFArgs A, const AtomRec *__restrict__ atoms,
                                                          double4 *__restrict__ out)
 {
@@ -329,7 +377,7 @@ __global__ __launch_bounds__(kThreads) void ff_build_f64(FFArgs A, const AtomRec
 #pragma unroll
             for (int k = 0; k < kK; k++) {
                 if (k >= nk) break;
-                pair_fp64<LJ, NS, WE>(A, px[k] - a.x, py[k] - a.y, pz[k] - a.z, a.w, c.x, c.y, aL[k], aE[k]);
+                pair_fp64_fast<LJ, NS, Z0, WE>(A, px[k] - a.x, py[k] - a.y, pz[k] - a.z, a.w, c.x, c.y, aL[k], aE[k]);
             }
         }
     }
@@ -447,7 +495,7 @@ void launch_t(const FFArgs &A, const AtomRec *atoms, void *out, bool f64, hipStr
 {
     dim3 grid((unsigned)A.nbx * A.nby * A.nbz), block(kThreads);
     if (f64)
-        hipLaunchKernelGGL((ff_build_f64<LJ, NS, WE>), grid, block, 0, st, A, atoms, (double4 *)out);
+        hipLaunchKernelGGL((ff_build_f64<LJ, NS, Z0, WE>), grid, block, 0, st, A, atoms, (double4 *)out);
     else
         hipLaunchKernelGGL((ff_build_f32<LJ, NS, Z0, WE>), grid, block, 0, st, A, atoms, (float4 *)out);
 }

@@ -72,6 +72,7 @@ struct RelaxLaunch {
     ppafm_relax_out out;
 };
 int launch_relax_dev(const RelaxLaunch &L, const void *grid, bool f64, hipStream_t st);
+int launch_relax_fast(const RelaxLaunch &L, const void *grid, bool f64, hipStream_t st);
 int launch_relax_f64grid(const RelaxLaunch &L, const double *grid, double *pos, double *force, int *iters,
                          hipStream_t st);
 

@@ -20,6 +20,8 @@
 // FP64 (host ABI path, bit-compatible with the CPU reference).
 // Blocks are remapped so that each XCD works on a contiguous band of tiles.
 
+#include <cstdlib>
+
 #include "pp_common.hpp"
 
 // No FMA contraction in this file: the FP64 path then rounds every operation
@@ -388,6 +390,11 @@ int launch_relax_dev(const RelaxLaunch &L, const void *grid, bool f64, hipStream
 {
     int e = check(L);
     if (e) return e;
+    static const bool generic = getenv("PPAFM_RELAX_GENERIC") != nullptr;
+    if (!generic) {
+        e = launch_relax_fast(L, grid, f64, st);
+        if (e <= 0) return e;
+    }
     RelaxArgs A = make_args(L);
     const long nt = (long)A.tiles_x * A.tiles_y;
     if (nt == 0 || A.nz == 0) return 0;
