@@ -16,8 +16,8 @@
  * (2) Device ABI (ppafm_* prefix).  Plain device pointers and sizes, explicit
  *     descriptors, an explicit HIP stream (NULL = default stream), an int
  *     status (0 = success; ppafm_last_error() describes a failure).  The grid
- *     is FP32 float4 (Fx,Fy,Fz,E) with z fastest; relaxation arithmetic is
- *     FP64.  These have no counterpart in the reference; they are what the
+ *     is float4 or double4 (Fx,Fy,Fz,E) with z fastest; relaxation arithmetic
+ *     is FP64.  These have no counterpart in the reference; they are what the
  *     Python mirror of ppafm.ocl.AFMulator and the batched path call.
  */
 #ifndef PPAFM_HIP_H
@@ -122,12 +122,12 @@ typedef struct {
 int ppafm_pack_atoms(int natoms, const double *atoms_xyzq, const double *cLJs, double coulomb_const,
                      ppafm_atom *out_host);
 
-/* Build the FP32 float4 grid from device-resident atoms (no host work, no
- * synchronisation: capturable in a HIP graph). */
+/* Build the float4 / double4 (grid->f64) grid from device-resident atoms (no
+ * host work, no synchronisation: capturable in a HIP graph). */
 int ppafm_ff_build_dev(const ppafm_grid *grid, const ppafm_ff_model *model, int natoms, const ppafm_atom *atoms_dev,
                        void *out_dev, void *stream);
 
-/* Build the FP32 float4 grid out[(ix*ny+iy)*nz+iz] on the device.
+/* Build the grid out[(ix*ny+iy)*nz+iz] (float4 or double4) on the device.
  * atoms_xyzq (natoms,4) and cLJs (natoms,2) are HOST float64 arrays
  * (positions relative to the grid origin).  Waits for earlier work on
  * the stream before it stages the atoms. */
@@ -198,6 +198,68 @@ int ppafm_limit_vec_field(double *F_dev, long npts, double fmax, void *stream);
  * known-size reference for HBM byte counters. */
 int ppafm_copy_kernel(const void *src_dev, void *dst_dev, unsigned long long bytes, void *stream);
 
+/* ---------------------------------------------------------------- batches
+ * No counterpart in the reference, which runs one molecule per call
+ * (AFMulator.__call__, AFMulator.py:142-179; relaxTipStrokes_omp,
+ * ProbeParticle.cpp:682-701).  A batch plan holds many molecules of uneven
+ * atom counts, grid sizes and scans and runs grid build -> FIRE relaxation
+ * -> df for all of them as one captured HIP graph: one grid-build launch
+ * over every brick of every grid, one persistent relaxation launch fed by
+ * per-XCD work queues, and one z-correlation launch per run of equal nz.
+ *
+ * Device layout: grids back to back (float4 or double4, z fastest), outputs
+ * back to back per molecule in stroke order: pos float32 (.., 3), force
+ * float32 (.., 4), iters int32, df float32 (ncol, nz - nw + 1). */
+typedef struct {
+    int n[3];         /* grid points along a, b, c */
+    double cell[9];   /* grid cell rows a, b, c; atoms and scan in the grid frame */
+    int natoms;       /* atoms of this molecule in the packed table */
+    ppafm_scan scan;  /* apex scan of this molecule */
+} ppafm_batch_item;
+
+/* Launch shape of the persistent relaxation: pixel tiles of
+ * (64 / tile_y) x tile_y columns per wave (tile_y in 1, 2, 4, 8, 16),
+ * waves_per_block in 1..4, blocks_per_cu persistent workgroups per CU. */
+typedef struct {
+    int tile_y;
+    int waves_per_block;
+    int blocks_per_cu;
+} ppafm_batch_tuning;
+
+typedef struct ppafm_batch ppafm_batch;
+
+/* model / rp / tip as for ppafm_ff_build_dev and ppafm_relax (rp->alg must
+ * be 1).  df: nw (<= 64) host weights correlated with force channel 2 of
+ * each column, times df_scale; nw = 0 skips df.  keep_iters: allocate the
+ * per-position iteration output.  tuning NULL = defaults.  stream NULL = the
+ * plan creates its own.  Returns NULL on error (ppafm_last_error). */
+ppafm_batch *ppafm_batch_create(int nitems, const ppafm_batch_item *items, const ppafm_ff_model *model,
+                                const ppafm_relax_params *rp, const ppafm_tip *tip, int grid_f64, int nw,
+                                const double *df_w, double df_scale, int keep_iters,
+                                const ppafm_batch_tuning *tuning, void *stream);
+/* Upload the packed atoms of all molecules (ppafm_pack_atoms records, item
+ * order, natoms of each item).  Synchronous. */
+int ppafm_batch_set_atoms(ppafm_batch *b, const ppafm_atom *atoms);
+/* Enqueue the whole batch (graph launch) on the plan's stream. */
+int ppafm_batch_run(ppafm_batch *b);
+/* Enqueue single stages without the graph: bit 0 grid, bit 1 relaxation,
+ * bit 2 df (for timing stages separately). */
+int ppafm_batch_run_stages(ppafm_batch *b, int stages);
+/* Autotuner: time every launch shape of the persistent relaxation on the
+ * first nsample molecules (grids built once), keep the fastest, re-capture
+ * the graph and return it in best (and its time in best_ms, may be NULL). */
+int ppafm_batch_tune(ppafm_batch *b, int nsample, int reps, ppafm_batch_tuning *best, double *best_ms);
+int ppafm_batch_set_tuning(ppafm_batch *b, const ppafm_batch_tuning *tuning);
+int ppafm_batch_get_tuning(const ppafm_batch *b, ppafm_batch_tuning *tuning);
+/* Device buffers: 0 grids, 1 pos, 2 force, 3 iters, 4 df, 5 counters
+ * (uint64[2]: force evaluations, positions), 6 packed atoms. */
+void *ppafm_batch_buffer(const ppafm_batch *b, int which);
+/* Per-item element offsets (nitems + 1 entries each, NULL to skip): grid
+ * points, positions, df values. */
+int ppafm_batch_offsets(const ppafm_batch *b, long long *grid_off, long long *pos_off, long long *df_off);
+void *ppafm_batch_stream(const ppafm_batch *b);
+void ppafm_batch_destroy(ppafm_batch *b);
+
 /* Memory helpers so Python callers need no other GPU runtime. */
 int ppafm_malloc(void **ptr, unsigned long long bytes);
 int ppafm_free(void *ptr);
@@ -206,6 +268,8 @@ int ppafm_memcpy_d2h(void *dst, const void *src, unsigned long long bytes, void 
 int ppafm_memset(void *dst, int value, unsigned long long bytes, void *stream);
 int ppafm_synchronize(void *stream);
 int ppafm_device_count(void);
+/* Name (gcnArchName) and compute-unit count of the current device. */
+int ppafm_device_info(char *name, int len, int *ncu);
 /* HIP events for timing (elapsed waits for ev1) and non-blocking streams of
  * default priority. */
 int ppafm_event_create(void **ev);

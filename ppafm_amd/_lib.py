@@ -73,6 +73,17 @@ class RelaxOut(ctypes.Structure):
     _fields_ = [("pos", c_void_p), ("pos_f32", c_int), ("force", c_void_p), ("iters", c_void_p), ("counters", c_void_p)]
 
 
+class BatchItem(ctypes.Structure):
+    _fields_ = [("n", c_int * 3), ("cell", c_double * 9), ("natoms", c_int), ("scan", Scan)]
+
+
+class BatchTuning(ctypes.Structure):
+    _fields_ = [("tile_y", c_int), ("waves_per_block", c_int), ("blocks_per_cu", c_int)]
+
+    def as_dict(self):
+        return {"tile_y": self.tile_y, "waves_per_block": self.waves_per_block, "blocks_per_cu": self.blocks_per_cu}
+
+
 # (name, restype, argtypes) of every exported symbol declared in the header
 SIGNATURES = [
     ("setGridN", None, [_ip]),
@@ -99,6 +110,22 @@ SIGNATURES = [
     ("ppafm_convolve_z", c_int, [c_long, c_int, c_int, c_int, c_int, c_void_p, c_int, _dp, c_double, c_void_p, c_int, c_void_p]),
     ("ppafm_limit_vec_field", c_int, [c_void_p, c_long, c_double, c_void_p]),
     ("ppafm_copy_kernel", c_int, [c_void_p, c_void_p, c_ull, c_void_p]),
+    (
+        "ppafm_batch_create",
+        c_void_p,
+        [c_int, ctypes.POINTER(BatchItem), ctypes.POINTER(FFModel), ctypes.POINTER(RelaxParams), ctypes.POINTER(Tip), c_int, c_int, _dp, c_double, c_int, ctypes.POINTER(BatchTuning), c_void_p],
+    ),
+    ("ppafm_batch_set_atoms", c_int, [c_void_p, c_void_p]),
+    ("ppafm_batch_run", c_int, [c_void_p]),
+    ("ppafm_batch_run_stages", c_int, [c_void_p, c_int]),
+    ("ppafm_batch_tune", c_int, [c_void_p, c_int, c_int, ctypes.POINTER(BatchTuning), _dp]),
+    ("ppafm_batch_set_tuning", c_int, [c_void_p, ctypes.POINTER(BatchTuning)]),
+    ("ppafm_batch_get_tuning", c_int, [c_void_p, ctypes.POINTER(BatchTuning)]),
+    ("ppafm_batch_buffer", c_void_p, [c_void_p, c_int]),
+    ("ppafm_batch_offsets", c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("ppafm_batch_stream", c_void_p, [c_void_p]),
+    ("ppafm_batch_destroy", None, [c_void_p]),
+    ("ppafm_device_info", c_int, [ctypes.c_char_p, c_int, _ip]),
     ("ppafm_malloc", c_int, [ctypes.POINTER(c_void_p), c_ull]),
     ("ppafm_free", c_int, [c_void_p]),
     ("ppafm_memcpy_h2d", c_int, [c_void_p, c_void_p, c_ull, c_void_p]),

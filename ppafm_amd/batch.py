@@ -1,0 +1,301 @@
+"""Batched relaxed scans: many molecules per call.
+
+A :class:`BatchScan` owns one batch plan of libppafm_hip.so
+(``ppafm_batch_*`` in include/ppafm_hip.h): every grid, relaxation output and
+df stack of the batch lives back to back in device memory, and one call of
+:meth:`BatchScan.run` replays a captured HIP graph that builds all grids in
+one launch, relaxes every pixel column of every molecule in one persistent
+launch and converts the force stacks to df.
+
+The reference has no batched entry point (one ``AFMulator.__call__`` per
+molecule, ppafm/ocl/AFMulator.py:142-179, and the ML generators loop over it,
+ppafm/ml/Generator.py:474-498).  Semantics per molecule are those of
+:class:`ppafm_amd.scan.PlainScan` (ProbeParticle, ``BatchScan.plain``) or of
+``AFMulator`` (OpenCL, ``AFMulator.batch``); a batch gives the same numbers
+as the per-molecule calls.
+
+Autotuning: :meth:`BatchScan.tune` times the persistent relaxation's launch
+shapes (pixel tile, waves per workgroup, workgroups per CU) on a sample of the
+batch and keeps the fastest; :func:`tuned` caches winners per device and
+problem class in a JSON file (``PPAFM_TUNING_CACHE`` or
+``~/.cache/ppafm_amd/tuning.json``).
+"""
+
+import ctypes
+import json
+import os
+import pathlib
+
+import numpy as np
+
+from . import _lib
+from . import common as PPU
+from . import device as D
+from ._lib import HipError, check, lib
+from .scan import plain_setup
+
+BUF_GRIDS, BUF_POS, BUF_FORCE, BUF_ITERS, BUF_DF, BUF_COUNTERS, BUF_ATOMS = range(7)
+
+
+def _scan_copy(dst, src):
+    dst.nx, dst.ny, dst.nz = src.nx, src.ny, src.nz
+    for k in range(3):
+        dst.p0[k], dst.da[k], dst.db[k], dst.dstep[k] = src.p0[k], src.da[k], src.db[k], src.dstep[k]
+
+
+def device_info():
+    name = ctypes.create_string_buffer(128)
+    ncu = ctypes.c_int()
+    check(lib().ppafm_device_info(name, 128, ctypes.byref(ncu)), "device_info")
+    return name.value.decode(errors="replace"), int(ncu.value)
+
+
+class BatchScan:
+    """A batch plan: items share the force-field model, relaxation parameters,
+    tip and df weights; each item has its own grid (n, cell), atom count and
+    scan.
+
+    items: sequence of dicts with ``n`` (3 ints), ``cell`` (3x3, rows a, b, c),
+    ``natoms`` and ``scan`` (a ``ppafm_scan`` descriptor in the grid frame).
+    tuning: None (defaults), a dict (tile_y, waves_per_block, blocks_per_cu)
+    or "auto" (cached winner for this device and problem class, tuned on the
+    first uploaded atoms and cached on a miss).
+    """
+
+    def __init__(self, items, model, rp, tip, df_weights=None, df_scale=1.0, grid_f64=True, keep_iters=False,
+                 tuning=None, stream=None, tune_sample=32):
+        if len(items) < 1:
+            raise ValueError("a batch needs at least one item")
+        self.nitems = len(items)
+        self.items = list(items)
+        self.model, self.rp, self.tip = model, rp, tip
+        self.grid_f64 = bool(grid_f64)
+        self.w = np.zeros(0) if df_weights is None else np.ascontiguousarray(df_weights, dtype=np.float64)
+        if len(self.w) > 64:
+            raise ValueError("at most 64 df weights")
+        arr = (_lib.BatchItem * self.nitems)()
+        for k, it in enumerate(self.items):
+            for a in range(3):
+                arr[k].n[a] = int(it["n"][a])
+            c = np.asarray(it["cell"], dtype=np.float64).reshape(9)
+            for a in range(9):
+                arr[k].cell[a] = float(c[a])
+            arr[k].natoms = int(it["natoms"])
+            _scan_copy(arr[k].scan, it["scan"])
+        self._arr = arr
+        tun = None
+        if isinstance(tuning, dict):
+            tun = _lib.BatchTuning(**tuning)
+        wp = self.w.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if len(self.w) else None
+        h = lib().ppafm_batch_create(self.nitems, arr, ctypes.byref(model), ctypes.byref(rp), ctypes.byref(tip),
+                                     int(self.grid_f64), len(self.w), wp, float(df_scale), int(bool(keep_iters)),
+                                     ctypes.byref(tun) if tun is not None else None, stream)
+        if not h:
+            raise HipError("ppafm_batch_create failed: " + lib().ppafm_last_error().decode(errors="replace"))
+        self._h = h
+        self._lib = lib()
+        self.stream = lib().ppafm_batch_stream(h)
+        n1 = self.nitems + 1
+        self.grid_off = np.zeros(n1, np.int64)
+        self.pos_off = np.zeros(n1, np.int64)
+        self.df_off = np.zeros(n1, np.int64)
+        check(lib().ppafm_batch_offsets(h, self.grid_off.ctypes.data, self.pos_off.ctypes.data, self.df_off.ctypes.data), "batch_offsets")
+        self.natoms = np.array([int(it["natoms"]) for it in self.items])
+        self.tune_sample = tune_sample
+        self._auto = tuning == "auto"
+
+    # ------------------------------------------------------------ plain mode
+    @classmethod
+    def plain(cls, lvecs, ns, scans, tipR0=(0.0, 0.0, 3.0), k_lat=0.25, k_rad=30.0, iZPP=8, probe_q=-0.1, tip="s",
+              amplitude=0.8, k0=1800.0, f0=30300.0, Fmax=10.0, relax=None, molecules=None, natoms=None,
+              grid_f64=True, keep_iters=False, tuning=None, stream=None, species=None):
+        """ProbeParticle-semantics batch (PlainScan per item): lvecs, ns and
+        scans are per-item grid lattices (4,3), grid sizes and scan dicts.
+        molecules: (xyzs, Zs, qs) per item (uploaded), or natoms per item and
+        set_molecules later."""
+        if molecules is not None:
+            natoms = [len(m[1]) for m in molecules]
+        if natoms is None:
+            raise ValueError("give molecules or natoms")
+        items = []
+        shared = None
+        for k, (lvec, n, scan) in enumerate(zip(lvecs, ns, scans)):
+            model, rp, tip_d, scan_d, w, df_scale = plain_setup(
+                lvec, scan, tipR0=tipR0, k_lat=k_lat, k_rad=k_rad, probe_q=probe_q, tip=tip, amplitude=amplitude,
+                k0=k0, f0=f0, Fmax=Fmax, relax=relax)
+            if shared is None:
+                shared = (model, rp, tip_d, w, df_scale)
+            elif len(w) != len(shared[3]) or not np.array_equal(w, shared[3]):
+                raise ValueError("all items of a batch need the same df weights (same dz)")
+            items.append(dict(n=n, cell=np.asarray(lvec)[1:], natoms=int(natoms[k]), scan=scan_d))
+        model, rp, tip_d, w, df_scale = shared
+        b = cls(items, model, rp, tip_d, df_weights=w, df_scale=df_scale, grid_f64=grid_f64, keep_iters=keep_iters,
+                tuning=tuning, stream=stream)
+        b.lvecs = [np.asarray(l, dtype=np.float64) for l in lvecs]
+        b.iZPP = iZPP
+        b.species = species or PPU.loadSpecies()
+        b.scans = [dict(s) for s in scans]
+        if molecules is not None:
+            b.set_molecules(molecules)
+        return b
+
+    def set_molecules(self, molecules):
+        """molecules: sequence of (xyzs, Zs, qs) in item order (plain mode;
+        the atom counts must match the plan's)."""
+        packs = []
+        for k, (xyzs, Zs, qs) in enumerate(molecules):
+            xyzs = np.asarray(xyzs, dtype=np.float64)
+            cLJs = PPU.getAtomsLJ(self.iZPP, Zs, self.species)
+            xyzq = np.concatenate([xyzs - self.lvecs[k][0][None, :], np.asarray(qs, dtype=np.float64)[:, None]], axis=1)
+            packs.append(D.pack_atoms(xyzq, cLJs, self.model.coulomb_const))
+        self.set_atoms(packs)
+
+    def set_atoms(self, packs):
+        """packs: per item (natoms, 6) ppafm_atom records (D.pack_atoms)."""
+        if len(packs) != self.nitems:
+            raise ValueError(f"expected {self.nitems} molecules, got {len(packs)}")
+        for k, p in enumerate(packs):
+            if len(p) != self.natoms[k]:
+                raise ValueError(f"item {k}: plan has {self.natoms[k]} atoms, got {len(p)}")
+        allp = np.ascontiguousarray(np.concatenate([np.asarray(p, dtype=np.float64).reshape(-1, 6) for p in packs], axis=0))
+        check(lib().ppafm_batch_set_atoms(self._h, allp.ctypes.data if len(allp) else None), "batch_set_atoms")
+        if self._auto:
+            self._auto = False
+            self.set_tuning(tuned(self, nsample=self.tune_sample))
+
+    # ------------------------------------------------------------ running
+    def run(self):
+        check(lib().ppafm_batch_run(self._h), "batch_run")
+
+    def run_stages(self, stages):
+        check(lib().ppafm_batch_run_stages(self._h, int(stages)), "batch_run_stages")
+
+    def synchronize(self):
+        D.synchronize(self.stream)
+
+    def tune(self, nsample=32, reps=2):
+        best = _lib.BatchTuning()
+        ms = ctypes.c_double()
+        check(lib().ppafm_batch_tune(self._h, int(nsample), int(reps), ctypes.byref(best), ctypes.byref(ms)), "batch_tune")
+        return best.as_dict(), float(ms.value)
+
+    @property
+    def tuning(self):
+        t = _lib.BatchTuning()
+        check(lib().ppafm_batch_get_tuning(self._h, ctypes.byref(t)), "batch_get_tuning")
+        return t.as_dict()
+
+    def set_tuning(self, tuning):
+        t = _lib.BatchTuning(**tuning)
+        check(lib().ppafm_batch_set_tuning(self._h, ctypes.byref(t)), "batch_set_tuning")
+
+    def problem_key(self):
+        """Problem class of the autotuner cache."""
+        it = self.items[0]
+        s = it["scan"]
+        return f"mode{self.rp.mode}-{'f64' if self.grid_f64 else 'f32'}-scan{s.nx}x{s.ny}x{s.nz}"
+
+    # ------------------------------------------------------------ outputs
+    def _download(self, which, off_elems, shape, dtype):
+        dtype = np.dtype(dtype)
+        out = np.empty(shape, dtype)
+        base = lib().ppafm_batch_buffer(self._h, which)
+        if base is None:
+            raise HipError("buffer not allocated")
+        check(lib().ppafm_memcpy_d2h(out.ctypes.data, base + int(off_elems), out.nbytes, self.stream), "download")
+        return out
+
+    def _scan_shape(self, k):
+        s = self.items[k]["scan"]
+        return s.nx, s.ny, s.nz
+
+    def grid(self, k):
+        n = tuple(int(v) for v in self.items[k]["n"])
+        esz = 32 if self.grid_f64 else 16
+        return self._download(BUF_GRIDS, self.grid_off[k] * esz, n + (4,), np.float64 if self.grid_f64 else np.float32)
+
+    def positions_raw(self, k):
+        """Relaxed positions (nx, ny, nz, 3) float32, grid frame, stroke order."""
+        return self._download(BUF_POS, self.pos_off[k] * 12, self._scan_shape(k) + (3,), np.float32)
+
+    def positions(self, k):
+        """Relaxed positions (nx, ny, nz, 3) in the molecule frame (plain mode)."""
+        return self.positions_raw(k).astype(np.float64) + self.lvecs[k][0][None, None, None, :]
+
+    def forces(self, k):
+        return self._download(BUF_FORCE, self.pos_off[k] * 16, self._scan_shape(k) + (4,), np.float32)
+
+    def iterations(self, k):
+        return self._download(BUF_ITERS, self.pos_off[k] * 4, self._scan_shape(k), np.int32)
+
+    def df_raw(self, k):
+        """df (nx, ny, nz - nw + 1) in stroke order (index 0 = top)."""
+        nx, ny, nz = self._scan_shape(k)
+        nzout = nz - len(self.w) + 1
+        return self._download(BUF_DF, self.df_off[k] * 4, (nx, ny, nzout), np.float32)
+
+    def df_host(self, k):
+        """df with z ascending, like PlainScan.df_host / Fz2df (plain mode)."""
+        return self.df_raw(k)[:, :, ::-1].copy()
+
+    def counts(self):
+        c = np.empty(2, np.uint64)
+        check(lib().ppafm_memcpy_d2h(c.ctypes.data, lib().ppafm_batch_buffer(self._h, BUF_COUNTERS), 16, self.stream), "download")
+        return int(c[0]), int(c[1])
+
+    def total_positions(self):
+        return int(self.pos_off[-1])
+
+    def total_grid_points(self):
+        return int(self.grid_off[-1])
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.ppafm_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------ tuning cache
+
+
+def cache_path():
+    p = os.environ.get("PPAFM_TUNING_CACHE")
+    return pathlib.Path(p) if p else pathlib.Path.home() / ".cache" / "ppafm_amd" / "tuning.json"
+
+
+def _load_cache(path):
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d if isinstance(d, dict) else {}
+    except (OSError, ValueError):
+        return {}
+
+
+def tuned(batch, nsample=32, reps=2, path=None):
+    """Cached autotuner winner for this device and the batch's problem class;
+    tunes on the batch (and stores the winner) on a miss."""
+    path = pathlib.Path(path) if path else cache_path()
+    name, ncu = device_info()
+    key = f"{name}-{ncu}cu-{batch.problem_key()}"
+    cache = _load_cache(path)
+    hit = cache.get(key)
+    if isinstance(hit, dict) and {"tile_y", "waves_per_block", "blocks_per_cu"} <= set(hit):
+        return {k: int(hit[k]) for k in ("tile_y", "waves_per_block", "blocks_per_cu")}
+    best, ms = batch.tune(nsample=nsample, reps=reps)
+    cache[key] = dict(best, ms=ms, nsample=int(min(nsample, batch.nitems)))
+    try:
+        path.parent.mkdir(parents=True, exist_ok=True)
+        tmp = path.with_suffix(".tmp")
+        with open(tmp, "w") as f:
+            json.dump(cache, f, indent=1, sort_keys=True)
+        os.replace(tmp, path)
+    except OSError:
+        pass
+    return best

@@ -485,6 +485,19 @@ int ppafm_device_count(void)
     return n;
 }
 
+int ppafm_device_info(char *name, int len, int *ncu)
+{
+    int dev = 0;
+    PP_TRY_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t p;
+    PP_TRY_HIP(hipGetDeviceProperties(&p, dev));
+    if (name && len > 0) {
+        std::snprintf(name, (size_t)len, "%s", p.gcnArchName);
+    }
+    if (ncu) *ncu = p.multiProcessorCount;
+    return 0;
+}
+
 const char *ppafm_last_error(void) { return get_error(); }
 
 void ppafm_clear_error(void) { set_error(""); }

@@ -62,7 +62,23 @@ __global__ __launch_bounds__(256) void copy_kernel(const float4 *__restrict__ sr
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) dst[i] = src[i];
 }
 
+// Fill n 32-bit words with a value: clears counters and queues inside a
+// captured graph, where memset nodes are avoided.
+__global__ __launch_bounds__(64) void fill_u32_kernel(unsigned *__restrict__ p, long n, unsigned v)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = v;
+}
+
 }  // namespace
+
+int launch_fill_u32(void *p, long n, unsigned v, hipStream_t st)
+{
+    PP_REQUIRE(n >= 0, "size must be >= 0");
+    if (n == 0) return 0;
+    const long nblk = std::min<long>((n + 63) / 64, 1024L);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)nblk), dim3(64), 0, st, (unsigned *)p, n, v);
+    return 0;
+}
 
 int launch_copy(const void *src, void *dst, long n16, hipStream_t st)
 {

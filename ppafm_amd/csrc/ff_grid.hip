@@ -114,16 +114,12 @@ __device__ __forceinline__ void pair_fp64(const FFArgs &A, double dx, double dy,
 // stencil point 0 sits at zero z offset and reuses the LJ pair's rsq.
 // WE: accumulate the energy channel.
 template <bool LJ, int NS, bool Z0, bool WE>
-__global__ __launch_bounds__(kThreads) void ff_build_f32(FFArgs A, const AtomRec *__restrict__ atoms,
-                                                         float4 *__restrict__ out)
+__device__ __forceinline__ void ff_brick_f32(const FFArgs &A, const AtomRec *__restrict__ atoms, float4 *__restrict__ out,
+                                             int b, AtomLds *lds, double4 *ldsd)
 {
-    __shared__ AtomLds lds[kTile];
-    __shared__ double4 ldsd[kTile];  // q, c6, c12 in FP64 for the FP64 pairs
-
     const int t = threadIdx.x;
     const int col = t / kZLanes;
     const int zl = t % kZLanes;
-    int b = blockIdx.x;
     const int bz = b % A.nbz; b /= A.nbz;
     const int by = b % A.nby;
     const int bx = b / A.nby;
@@ -323,16 +319,12 @@ __device__ __forceinline__ void pair_fp64_fast(const FFArgs &A, double dx, doubl
 // grid perturbations of ~1e-7 into stopping-point changes of >1e-3 A on
 // ill-conditioned strokes).  Same brick mapping as ff_build_f32.
 template <bool LJ, int NS, bool Z0, bool WE>
-__global__ __launch_bounds__(kThreads) void ff_build_f64(FFArgs A, const AtomRec *__restrict__ atoms,
-                                                         double4 *__restrict__ out)
+__device__ __forceinline__ void ff_brick_f64(const FFArgs &A, const AtomRec *__restrict__ atoms, double4 *__restrict__ out,
+                                             int b, double4 *lpos, double2 *llj)
 {
-    __shared__ double4 lpos[kTile];  // brick-local x, y, z, q
-    __shared__ double2 llj[kTile];   // c6, c12
-
     const int t = threadIdx.x;
     const int col = t / kZLanes;
     const int zl = t % kZLanes;
-    int b = blockIdx.x;
     const int bz = b % A.nbz; b /= A.nbz;
     const int by = b % A.nby;
     const int bx = b / A.nby;
@@ -403,6 +395,77 @@ __global__ __launch_bounds__(kThreads) void ff_build_f64(FFArgs A, const AtomRec
         }
         out[((long)ix * A.n[1] + iy) * A.n[2] + iz] = make_double4(gx, gy, gz, WE ? e : 0.0);
     }
+}
+
+template <bool LJ, int NS, bool Z0, bool WE>
+__global__ __launch_bounds__(kThreads) void ff_build_f32(FFArgs A, const AtomRec *__restrict__ atoms,
+                                                         float4 *__restrict__ out)
+{
+    __shared__ AtomLds lds[kTile];
+    __shared__ double4 ldsd[kTile];  // q, c6, c12 in FP64 for the FP64 pairs
+    ff_brick_f32<LJ, NS, Z0, WE>(A, atoms, out, blockIdx.x, lds, ldsd);
+}
+
+template <bool LJ, int NS, bool Z0, bool WE>
+__global__ __launch_bounds__(kThreads) void ff_build_f64(FFArgs A, const AtomRec *__restrict__ atoms,
+                                                         double4 *__restrict__ out)
+{
+    __shared__ double4 lpos[kTile];  // brick-local x, y, z, q
+    __shared__ double2 llj[kTile];   // c6, c12
+    ff_brick_f64<LJ, NS, Z0, WE>(A, atoms, out, blockIdx.x, lpos, llj);
+}
+
+// ---- batches: one launch over the bricks of every molecule.  Block b finds
+// its molecule by binary search over the brick prefix sums.
+__device__ __forceinline__ int find_item(const long long *__restrict__ prefix, int nitems, long long x)
+{
+    int lo = 0, hi = nitems;  // prefix[lo] <= x < prefix[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (prefix[mid] <= x) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ FFArgs batch_args(const FFArgs &A0, const BatchGridItem &it)
+{
+    FFArgs A = A0;
+    for (int k = 0; k < 3; k++) A.n[k] = it.n[k];
+    A.nbx = (it.n[0] + kColsX - 1) / kColsX;
+    A.nby = (it.n[1] + kColsY - 1) / kColsY;
+    A.nbz = (it.n[2] + kZBrick - 1) / kZBrick;
+    A.natoms = it.natoms;
+    for (int k = 0; k < 9; k++) A.dcell[k] = it.dcell[k];
+    return A;
+}
+
+template <bool LJ, int NS, bool Z0, bool WE>
+__global__ __launch_bounds__(kThreads) void ff_build_f32_batch(FFArgs A0, const BatchGridItem *__restrict__ items,
+                                                               const long long *__restrict__ prefix, int nitems,
+                                                               const AtomRec *__restrict__ atoms, float4 *__restrict__ grid)
+{
+    __shared__ AtomLds lds[kTile];
+    __shared__ double4 ldsd[kTile];
+    const long long bidx = blockIdx.x;
+    const int m = find_item(prefix, nitems, bidx);
+    const BatchGridItem it = items[m];
+    const FFArgs A = batch_args(A0, it);
+    ff_brick_f32<LJ, NS, Z0, WE>(A, atoms + it.atom_off, grid + it.grid_off, (int)(bidx - prefix[m]), lds, ldsd);
+}
+
+template <bool LJ, int NS, bool Z0, bool WE>
+__global__ __launch_bounds__(kThreads) void ff_build_f64_batch(FFArgs A0, const BatchGridItem *__restrict__ items,
+                                                               const long long *__restrict__ prefix, int nitems,
+                                                               const AtomRec *__restrict__ atoms, double4 *__restrict__ grid)
+{
+    __shared__ double4 lpos[kTile];
+    __shared__ double2 llj[kTile];
+    const long long bidx = blockIdx.x;
+    const int m = find_item(prefix, nitems, bidx);
+    const BatchGridItem it = items[m];
+    const FFArgs A = batch_args(A0, it);
+    ff_brick_f64<LJ, NS, Z0, WE>(A, atoms + it.atom_off, grid + it.grid_off, (int)(bidx - prefix[m]), lpos, llj);
 }
 
 // ------------------------------------------------------------------ FP64
@@ -490,10 +553,29 @@ __global__ __launch_bounds__(256) void ff_accum_f64(F64Args A, const AtomRec *__
     if (E) E[ig] += e;
 }
 
+// One launch: a single grid (batch == nullptr) or every grid of a batch.
+struct BatchLaunch {
+    const BatchGridItem *items;
+    const long long *prefix;
+    int nitems;
+    long long nbricks;
+};
+
 template <bool LJ, int NS, bool Z0, bool WE>
-void launch_t(const FFArgs &A, const AtomRec *atoms, void *out, bool f64, hipStream_t st)
+void launch_t(const FFArgs &A, const AtomRec *atoms, void *out, bool f64, const BatchLaunch *batch, hipStream_t st)
 {
-    dim3 grid((unsigned)A.nbx * A.nby * A.nbz), block(kThreads);
+    dim3 block(kThreads);
+    if (batch) {
+        dim3 grid((unsigned)batch->nbricks);
+        if (f64)
+            hipLaunchKernelGGL((ff_build_f64_batch<LJ, NS, Z0, WE>), grid, block, 0, st, A, batch->items, batch->prefix,
+                               batch->nitems, atoms, (double4 *)out);
+        else
+            hipLaunchKernelGGL((ff_build_f32_batch<LJ, NS, Z0, WE>), grid, block, 0, st, A, batch->items, batch->prefix,
+                               batch->nitems, atoms, (float4 *)out);
+        return;
+    }
+    dim3 grid((unsigned)A.nbx * A.nby * A.nbz);
     if (f64)
         hipLaunchKernelGGL((ff_build_f64<LJ, NS, Z0, WE>), grid, block, 0, st, A, atoms, (double4 *)out);
     else
@@ -501,14 +583,15 @@ void launch_t(const FFArgs &A, const AtomRec *atoms, void *out, bool f64, hipStr
 }
 
 template <bool LJ, bool WE>
-int launch_ns(int ns, bool z0, const FFArgs &A, const AtomRec *atoms, void *out, bool f64, hipStream_t st)
+int launch_ns(int ns, bool z0, const FFArgs &A, const AtomRec *atoms, void *out, bool f64, const BatchLaunch *b,
+              hipStream_t st)
 {
     switch (ns) {
-    case 0: launch_t<LJ, 0, false, WE>(A, atoms, out, f64, st); break;
-    case 1: z0 ? launch_t<LJ, 1, true, WE>(A, atoms, out, f64, st) : launch_t<LJ, 1, false, WE>(A, atoms, out, f64, st); break;
-    case 2: z0 ? launch_t<LJ, 2, true, WE>(A, atoms, out, f64, st) : launch_t<LJ, 2, false, WE>(A, atoms, out, f64, st); break;
-    case 3: z0 ? launch_t<LJ, 3, true, WE>(A, atoms, out, f64, st) : launch_t<LJ, 3, false, WE>(A, atoms, out, f64, st); break;
-    case 4: z0 ? launch_t<LJ, 4, true, WE>(A, atoms, out, f64, st) : launch_t<LJ, 4, false, WE>(A, atoms, out, f64, st); break;
+    case 0: launch_t<LJ, 0, false, WE>(A, atoms, out, f64, b, st); break;
+    case 1: z0 ? launch_t<LJ, 1, true, WE>(A, atoms, out, f64, b, st) : launch_t<LJ, 1, false, WE>(A, atoms, out, f64, b, st); break;
+    case 2: z0 ? launch_t<LJ, 2, true, WE>(A, atoms, out, f64, b, st) : launch_t<LJ, 2, false, WE>(A, atoms, out, f64, b, st); break;
+    case 3: z0 ? launch_t<LJ, 3, true, WE>(A, atoms, out, f64, b, st) : launch_t<LJ, 3, false, WE>(A, atoms, out, f64, b, st); break;
+    case 4: z0 ? launch_t<LJ, 4, true, WE>(A, atoms, out, f64, b, st) : launch_t<LJ, 4, false, WE>(A, atoms, out, f64, b, st); break;
     default: set_error("n_stencil must be 0..4"); return -2;
     }
     return 0;
@@ -522,6 +605,49 @@ int first_zero_index(const ppafm_ff_model &m)
 }
 
 }  // namespace
+
+// Model part of the kernel arguments (stencil compaction, R2SAFE, FP64
+// radius); returns whether stencil point 0 sits at zero z offset.
+static bool model_args(const ppafm_ff_model &m, FFArgs &A)
+{
+    // stencil points with zero weight are dropped; the first zero-offset point
+    // goes first so that it can share the LJ pair's rsq
+    const int zi = first_zero_index(m);
+    int ns = 0;
+    if (zi >= 0) { A.sw[ns] = (float)m.stencil_w[zi]; A.sdz[ns] = 0.f; ns++; }
+    for (int s = 0; s < m.n_stencil; s++)
+        if (m.stencil_w[s] != 0 && s != zi) {
+            A.sw[ns] = (float)m.stencil_w[s]; A.sdz[ns] = (float)m.stencil_dz[s]; ns++;
+        }
+    for (int s = ns; s < 4; s++) { A.sw[s] = 0.f; A.sdz[s] = 0.f; }
+    A.ns = ns;
+    A.mode = m.mode;
+    A.r2safe_d = m.mode == 0 ? (double)1.0e-8f : (double)1.0e-6f;  // Forces.h:12 / FF.cl:1
+    A.r2safe = (float)A.r2safe_d;
+    A.with_lj = m.with_lj;
+    A.probe_q = m.probe_q;
+    A.fmax = m.fmax;
+    // FP64 radius: at least 0.2 A beyond the farthest stencil offset, so no
+    // FP32 pair (LJ or shifted Coulomb point) comes closer than 0.2 A
+    double maxdz = 0;
+    for (int s = 0; s < ns; s++) maxdz = fmax(maxdz, fabs((double)A.sdz[s]));
+    const double rmin = kMinFp64Radius + maxdz;
+    const double rn = m.fp64_radius > rmin ? m.fp64_radius : rmin;
+    A.near2 = (float)(rn * rn);
+    return zi >= 0;
+}
+
+static int dispatch(const ppafm_ff_model &m, const FFArgs &A, bool z0, const AtomRec *atoms, void *out, bool f64,
+                    const BatchLaunch *b, hipStream_t st)
+{
+    const bool we = m.with_energy != 0;
+    if (m.with_lj) {
+        if (we) return launch_ns<true, true>(A.ns, z0, A, atoms, out, f64, b, st);
+        return launch_ns<true, false>(A.ns, z0, A, atoms, out, f64, b, st);
+    }
+    if (we) return launch_ns<false, true>(A.ns, z0, A, atoms, out, f64, b, st);
+    return launch_ns<false, false>(A.ns, z0, A, atoms, out, f64, b, st);
+}
 
 int launch_ff_build(const GridGeom &g, const ppafm_ff_model &m, int natoms, const AtomRec *atoms_dev, void *out,
                     bool f64, hipStream_t st)
@@ -537,38 +663,26 @@ int launch_ff_build(const GridGeom &g, const ppafm_ff_model &m, int natoms, cons
     PP_REQUIRE((long)A.nbx * A.nby * A.nbz < (1L << 31), "grid too large for one launch");
     A.natoms = natoms;
     for (int i = 0; i < 9; i++) A.dcell[i] = g.dcell[i];
-    // stencil points with zero weight are dropped; the first zero-offset point
-    // goes first so that it can share the LJ pair's rsq
-    const int zi = first_zero_index(m);
-    int ns = 0;
-    if (zi >= 0) { A.sw[ns] = (float)m.stencil_w[zi]; A.sdz[ns] = 0.f; ns++; }
-    for (int s = 0; s < m.n_stencil; s++)
-        if (m.stencil_w[s] != 0 && s != zi) {
-            A.sw[ns] = (float)m.stencil_w[s]; A.sdz[ns] = (float)m.stencil_dz[s]; ns++;
-        }
-    for (int s = ns; s < 4; s++) { A.sw[s] = 0.f; A.sdz[s] = 0.f; }
-    A.ns = ns;
-    const bool z0 = zi >= 0;
-    A.mode = m.mode;
-    A.r2safe_d = m.mode == 0 ? (double)1.0e-8f : (double)1.0e-6f;  // Forces.h:12 / FF.cl:1
-    A.r2safe = (float)A.r2safe_d;
-    A.with_lj = m.with_lj;
-    A.probe_q = m.probe_q;
-    A.fmax = m.fmax;
-    // FP64 radius: at least 0.2 A beyond the farthest stencil offset, so no
-    // FP32 pair (LJ or shifted Coulomb point) comes closer than 0.2 A
-    double maxdz = 0;
-    for (int s = 0; s < ns; s++) maxdz = fmax(maxdz, fabs((double)A.sdz[s]));
-    const double rmin = kMinFp64Radius + maxdz;
-    const double rn = m.fp64_radius > rmin ? m.fp64_radius : rmin;
-    A.near2 = (float)(rn * rn);
-    const bool we = m.with_energy != 0;
-    if (m.with_lj) {
-        if (we) return launch_ns<true, true>(A.ns, z0, A, atoms_dev, out, f64, st);
-        return launch_ns<true, false>(A.ns, z0, A, atoms_dev, out, f64, st);
-    }
-    if (we) return launch_ns<false, true>(A.ns, z0, A, atoms_dev, out, f64, st);
-    return launch_ns<false, false>(A.ns, z0, A, atoms_dev, out, f64, st);
+    const bool z0 = model_args(m, A);
+    return dispatch(m, A, z0, atoms_dev, out, f64, nullptr, st);
+}
+
+long long ff_bricks(const int n[3])
+{
+    return (long long)((n[0] + kColsX - 1) / kColsX) * ((n[1] + kColsY - 1) / kColsY) * ((n[2] + kZBrick - 1) / kZBrick);
+}
+
+int launch_ff_build_batch(const ppafm_ff_model &m, int nitems, const BatchGridItem *items_dev,
+                          const long long *prefix_dev, long long nbricks, const AtomRec *atoms_dev, void *grids,
+                          bool f64, hipStream_t st)
+{
+    PP_REQUIRE(m.n_stencil >= 0 && m.n_stencil <= 4, "n_stencil must be 0..4");
+    PP_REQUIRE(nbricks >= 0 && nbricks < (1LL << 31), "batch too large for one launch");
+    if (nbricks == 0) return 0;
+    FFArgs A = {};
+    const bool z0 = model_args(m, A);
+    BatchLaunch b{items_dev, prefix_dev, nitems, nbricks};
+    return dispatch(m, A, z0, atoms_dev, grids, f64, &b, st);
 }
 
 int launch_ff_accum_f64(const GridGeom &g, int model, int natoms, const AtomRec *atoms_dev, double *F, double *E,

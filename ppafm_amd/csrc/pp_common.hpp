@@ -50,6 +50,28 @@ using AtomRec = ppafm_atom;  // x, y, z (grid frame), q * coulomb constant, c6, 
 
 hipStream_t as_stream(void *s);
 
+// One molecule of a batched grid build: its grid and its slice of the packed
+// atom table.  Grids of a batch lie back to back in one allocation.
+struct BatchGridItem {
+    int n[3];
+    int natoms;
+    long long atom_off;  // first atom in the packed table
+    long long grid_off;  // first grid point (element index)
+    double dcell[9];
+};
+
+// One molecule of a batched relaxation: the grid it samples and its scan.
+struct BatchRelaxItem {
+    int n[3];
+    int nx, ny, nz;
+    int tiles_y;
+    int pad_;
+    long long grid_off;  // first grid point (element index)
+    long long out_off;   // first output position (stroke order, nx*ny*nz block)
+    double dinv[9];
+    double p0[3], da[3], db[3], dstep[3];
+};
+
 }  // namespace pp
 
 // Launchers implemented in the .hip files.
@@ -58,6 +80,12 @@ namespace pp {
 // Device grid (z fastest): float4 (f64 = false) or double4 (f64 = true).
 int launch_ff_build(const GridGeom &g, const ppafm_ff_model &m, int natoms, const AtomRec *atoms_dev, void *out,
                     bool f64, hipStream_t st);
+// Bricks of one grid (launch blocks of the grid kernels).
+long long ff_bricks(const int n[3]);
+// Every grid of a batch in one launch; prefix[k] = bricks of items < k.
+int launch_ff_build_batch(const ppafm_ff_model &m, int nitems, const BatchGridItem *items_dev,
+                          const long long *prefix_dev, long long nbricks, const AtomRec *atoms_dev, void *grids,
+                          bool f64, hipStream_t st);
 // FP64 accumulate (+=) into double3 grid F (and E) for the host ABI.
 // model: 0 = LJ, 1..3 = Coulomb kind 0..2.
 int launch_ff_accum_f64(const GridGeom &g, int model, int natoms, const AtomRec *atoms_dev, double *F, double *E,
@@ -76,8 +104,32 @@ int launch_relax_fast(const RelaxLaunch &L, const void *grid, bool f64, hipStrea
 int launch_relax_f64grid(const RelaxLaunch &L, const double *grid, double *pos, double *force, int *iters,
                          hipStream_t st);
 
+// Batched FIRE relaxation (persistent kernel, see relax_dev.hip).
+struct RelaxBatchLaunch {
+    ppafm_relax_params rp;
+    ppafm_tip tip;
+    const BatchRelaxItem *items;  // device
+    const long long *prefix;      // device, tiles of items < k
+    int nitems;
+    long long ntiles;
+    const void *grids;  // float4 or double4, back to back
+    bool f64;
+    unsigned *queue;  // 8 part heads, zero at launch
+    float *pos;
+    float4 *force;
+    int *iters;
+    unsigned long long *counters;
+    int tile_y;           // pixel tile: (64 / tile_y) x tile_y columns
+    int waves_per_block;  // 1..4
+    int nblocks;          // persistent workgroups
+};
+// Tiles of one scan for a given tile_y (also returns tiles along y).
+int relax_batch_tiles(int nx, int ny, int tile_y, int &tiles_y);
+int launch_relax_batch(const RelaxBatchLaunch &L, hipStream_t st);
+
 int launch_limit_vec_field(double *F, long npts, double fmax, hipStream_t st);
 int launch_copy(const void *src, void *dst, long n16, hipStream_t st);
+int launch_fill_u32(void *p, long n, unsigned v, hipStream_t st);
 int launch_convolve_z(long ncol, int nz, int in_stride, int in_step, int in_offset, const void *in, int nw,
                       const double *w, double scale, void *out, int f64, hipStream_t st);
 

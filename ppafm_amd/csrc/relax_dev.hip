@@ -2,9 +2,12 @@
 // citations there) with a leaner instruction stream, because each pixel
 // column is a serial chain of ~10^4 dependent force evaluations and the
 // per-iteration latency sets the kernel time:
-//   * FMA contraction on (this path is not bit-matched to the CPU);
 //   * 1/l and sqrt(vv/ff) from v_rsq_f64 plus two Newton steps instead of
 //     IEEE divide / square root sequences;
+//   * fused multiply-adds written out (contraction is off, so every kernel
+//     that shares these helpers rounds identically: a batch reproduces the
+//     single-scan results bit for bit; this path is not bit-matched to the
+//     CPU reference);
 //   * 32-bit grid indices, ProbeParticle or OpenCL semantics chosen at
 //     compile time, per-lane tip base so the loop touches few uniforms.
 // Lanes run independent state machines over (z step, iteration); an
@@ -14,7 +17,7 @@
 
 #include "pp_common.hpp"
 
-#pragma clang fp contract(fast)
+#pragma clang fp contract(off)
 
 namespace pp {
 
@@ -50,6 +53,15 @@ __device__ __forceinline__ double rsq_nr(double x)
     return y;
 }
 
+// grid index coordinates of a position (rows of dinv)
+__device__ __forceinline__ void to_grid(const DevArgs &A, double rx, double ry, double rz, double &gx, double &gy,
+                                        double &gz)
+{
+    gx = fma(rz, A.dinv[2], fma(ry, A.dinv[1], rx * A.dinv[0]));
+    gy = fma(rz, A.dinv[5], fma(ry, A.dinv[4], rx * A.dinv[3]));
+    gz = fma(rz, A.dinv[8], fma(ry, A.dinv[7], rx * A.dinv[6]));
+}
+
 // periodic cell index and weights along one axis (Grid.h:66-69 semantics)
 __device__ __forceinline__ void axis(double r, int n, int &lo, int &hi, double &w)
 {
@@ -64,6 +76,26 @@ __device__ __forceinline__ void axis(double r, int n, int &lo, int &hi, double &
     hi = (i + 1 == n) ? 0 : i + 1;
 }
 
+// trilinear sum of the 8 corners (order: x, y, z bits 2, 1, 0)
+template <typename T4>
+__device__ __forceinline__ void interp8(const T4 v[8], double tx, double ty, double tz, double out[4], bool want_e)
+{
+    const double mx = 1 - tx, my = 1 - ty, mz = 1 - tz;
+    const double w00 = mx * my, w01 = mx * ty, w10 = tx * my, w11 = tx * ty;
+    const double w[8] = {w00 * mz, w00 * tz, w01 * mz, w01 * tz, w10 * mz, w10 * tz, w11 * mz, w11 * tz};
+    out[0] = (double)v[0].x * w[0];
+    out[1] = (double)v[0].y * w[0];
+    out[2] = (double)v[0].z * w[0];
+    out[3] = want_e ? (double)v[0].w * w[0] : 0.0;
+#pragma unroll
+    for (int k = 1; k < 8; k++) {
+        out[0] = fma((double)v[k].x, w[k], out[0]);
+        out[1] = fma((double)v[k].y, w[k], out[1]);
+        out[2] = fma((double)v[k].z, w[k], out[2]);
+        if (want_e) out[3] = fma((double)v[k].w, w[k], out[3]);
+    }
+}
+
 template <typename T4>
 __device__ __forceinline__ void gather(const T4 *__restrict__ g, const DevArgs &A, double gx, double gy, double gz,
                                        double out[4], bool want_e)
@@ -73,21 +105,223 @@ __device__ __forceinline__ void gather(const T4 *__restrict__ g, const DevArgs &
     axis(gx, A.n0, ix, jx, tx);
     axis(gy, A.n1, iy, jy, ty);
     axis(gz, A.n2, iz, jz, tz);
-    const double mx = 1 - tx, my = 1 - ty, mz = 1 - tz;
     const unsigned c00 = ((unsigned)ix * A.n1 + iy) * A.n2, c01 = ((unsigned)ix * A.n1 + jy) * A.n2;
     const unsigned c10 = ((unsigned)jx * A.n1 + iy) * A.n2, c11 = ((unsigned)jx * A.n1 + jy) * A.n2;
-    const T4 v0 = g[c00 + iz], v1 = g[c00 + jz], v2 = g[c01 + iz], v3 = g[c01 + jz];
-    const T4 v4 = g[c10 + iz], v5 = g[c10 + jz], v6 = g[c11 + iz], v7 = g[c11 + jz];
-    const double w00 = mx * my, w01 = mx * ty, w10 = tx * my, w11 = tx * ty;
-    const double w[8] = {w00 * mz, w00 * tz, w01 * mz, w01 * tz, w10 * mz, w10 * tz, w11 * mz, w11 * tz};
-    const T4 v[8] = {v0, v1, v2, v3, v4, v5, v6, v7};
-    out[0] = out[1] = out[2] = out[3] = 0;
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-        out[0] += (double)v[k].x * w[k];
-        out[1] += (double)v[k].y * w[k];
-        out[2] += (double)v[k].z * w[k];
-        if (want_e) out[3] += (double)v[k].w * w[k];
+    const T4 v[8] = {g[c00 + iz], g[c00 + jz], g[c01 + iz], g[c01 + jz],
+                     g[c10 + iz], g[c10 + jz], g[c11 + iz], g[c11 + jz]};
+    interp8(v, tx, ty, tz, out, want_e);
+}
+
+// FIRE state of one probe
+struct Fire {
+    double rx, ry, rz, vx, vy, vz, dt, ac;
+};
+
+// Total force (grid force fe + tip springs, ProbeParticle.cpp:282-294 /
+// relax.cl:37-41 in the tip frame) and one FIRE move (ProbeParticle.cpp:
+// 142-174 / relax.cl:105-126).  Returns |F|^2 of the force it moved with.
+template <int MODE>
+__device__ __forceinline__ double fire_move(const DevArgs &A, const double fe[4], double tx, double ty, double tz,
+                                            Fire &S)
+{
+    double fx = fe[0], fy = fe[1], fz = fe[2];
+    const double dx = S.rx - tx, dy = S.ry - ty, dz = S.rz - tz;
+    if (MODE == 0) {
+        const double l2 = fma(dx, dx, fma(dy, dy, dz * dz));
+        const double il = rsq_nr(l2);
+        const double sr = A.krad * fma(l2, il, -A.lrad) * il;
+        fx = fx + fma(dx, sr, (dx - A.d0[0]) * A.ks[0]);
+        fy = fy + fma(dy, sr, (dy - A.d0[1]) * A.ks[1]);
+        fz = fz + fma(dz, sr, (dz - A.d0[2]) * A.ks[2]);
+    } else {
+        const double qx = fma(dz, A.rot[2], fma(dy, A.rot[1], dx * A.rot[0]));
+        const double qy = fma(dz, A.rot[5], fma(dy, A.rot[4], dx * A.rot[3]));
+        const double qz = fma(dz, A.rot[8], fma(dy, A.rot[7], dx * A.rot[6]));
+        const double l2 = fma(qx, qx, fma(qy, qy, qz * qz));
+        const double il = rsq_nr(l2);
+        const double sr = A.krad * fma(l2, il, -A.lrad) * il;
+        const double ax = fma(qx, sr, (qx - A.d0[0]) * A.ks[0]);
+        const double ay = fma(qy, sr, (qy - A.d0[1]) * A.ks[1]);
+        const double az = fma(qz, sr, (qz - A.d0[2]) * A.ks[2]) + A.surf_ff;
+        fx = fx + fma(A.rot[6], az, fma(A.rot[3], ay, A.rot[0] * ax));
+        fy = fy + fma(A.rot[7], az, fma(A.rot[4], ay, A.rot[1] * ax));
+        fz = fz + fma(A.rot[8], az, fma(A.rot[5], ay, A.rot[2] * ax));
+    }
+    const double ff = fma(fx, fx, fma(fy, fy, fz * fz));
+    const double vv = fma(S.vx, S.vx, fma(S.vy, S.vy, S.vz * S.vz));
+    const double vf = fma(S.vx, fx, fma(S.vy, fy, S.vz * fz));
+    if (vf < 0) {
+        S.vx = S.vy = S.vz = 0;
+        S.dt = MODE == 0 ? S.dt * A.fdec : fmax(A.dtmin, S.dt * A.fdec);
+        S.ac = A.a0;
+    } else {
+        const double cf = vv > 0 ? S.ac * (vv * rsq_nr(vv)) * rsq_nr(MODE == 0 ? ff : ff + A.f2safe) : 0.0;
+        const double cv = 1 - S.ac;
+        S.vx = fma(fx, cf, S.vx * cv);
+        S.vy = fma(fy, cf, S.vy * cv);
+        S.vz = fma(fz, cf, S.vz * cv);
+        S.dt = fmin(S.dt * A.finc, A.dtmax);
+        S.ac *= A.falpha;
+    }
+    S.vx = fma(fx, S.dt, S.vx);
+    S.vy = fma(fy, S.dt, S.vy);
+    S.vz = fma(fz, S.dt, S.vz);
+    S.rx = fma(S.vx, S.dt, S.rx);
+    S.ry = fma(S.vy, S.dt, S.ry);
+    S.rz = fma(S.vz, S.dt, S.rz);
+    return ff;
+}
+
+__device__ __forceinline__ void store_pos(const DevArgs &A, long o, double rx, double ry, double rz)
+{
+    if (!A.pos) return;
+    if (A.pos_f32) {
+        float *p = (float *)A.pos + 3 * o;
+        p[0] = (float)rx; p[1] = (float)ry; p[2] = (float)rz;
+    } else {
+        double *p = (double *)A.pos + 3 * o;
+        p[0] = rx; p[1] = ry; p[2] = rz;
+    }
+}
+
+// grid force rotated into the tip frame (relax.cl:446-452), .w = energy
+__device__ __forceinline__ float4 tip_frame_force(const DevArgs &A, const double fe[4])
+{
+    return make_float4((float)fma(fe[2], A.rot[2], fma(fe[1], A.rot[1], fe[0] * A.rot[0])),
+                       (float)fma(fe[2], A.rot[5], fma(fe[1], A.rot[4], fe[0] * A.rot[3])),
+                       (float)fma(fe[2], A.rot[8], fma(fe[1], A.rot[7], fe[0] * A.rot[6])), (float)fe[3]);
+}
+
+// Start of the stroke: tip step 0 and the probe at tip + rotated d0.
+struct Stroke {
+    double bx, by, bz;  // tip position of step 0
+    long col0;
+};
+
+__device__ __forceinline__ Stroke stroke_at(const DevArgs &A, int i, int j)
+{
+    Stroke c;
+    c.bx = A.p0[0] + i * A.da[0] + j * A.db[0];
+    c.by = A.p0[1] + i * A.da[1] + j * A.db[1];
+    c.bz = A.p0[2] + i * A.da[2] + j * A.db[2];
+    c.col0 = ((long)i * A.ny + j) * A.nz;
+    return c;
+}
+
+__device__ __forceinline__ void tip_pos(const DevArgs &A, const Stroke &c, int s, bool live, double &x, double &y,
+                                        double &z)
+{
+    if (A.tips && live) {
+        const double *p = A.tips + 3 * (c.col0 + s);
+        x = p[0]; y = p[1]; z = p[2];
+    } else {
+        x = c.bx + s * A.dstep[0]; y = c.by + s * A.dstep[1]; z = c.bz + s * A.dstep[2];
+    }
+}
+
+template <int MODE>
+__device__ __forceinline__ void probe_offset(const DevArgs &A, double &ox, double &oy, double &oz)
+{
+    if (MODE == 0) {
+        ox = A.d0[0]; oy = A.d0[1]; oz = A.d0[2];
+    } else {
+        ox = fma(A.rot[6], A.d0[2], fma(A.rot[3], A.d0[1], A.rot[0] * A.d0[0]));
+        oy = fma(A.rot[7], A.d0[2], fma(A.rot[4], A.d0[1], A.rot[1] * A.d0[0]));
+        oz = fma(A.rot[8], A.d0[2], fma(A.rot[5], A.d0[1], A.rot[2] * A.d0[0]));
+    }
+}
+
+// Next stroke step: tip moves to step s, the probe follows (probeStart
+// semantics of relaxTipStroke, ProbeParticle.cpp:644-656; mode 1 always
+// continues, relax.cl:434-436) and the FIRE velocity resets.
+template <int MODE>
+__device__ __forceinline__ void next_step(const DevArgs &A, const Stroke &c, int s, bool live, double &tx, double &ty,
+                                          double &tz, Fire &S)
+{
+    double nx_, ny_, nz_;
+    tip_pos(A, c, s, live, nx_, ny_, nz_);
+    if (MODE == 0 && A.probe_start == 0) {
+        S.rx = nx_ + A.d0[0]; S.ry = ny_ + A.d0[1]; S.rz = nz_ + A.d0[2];
+    } else if (MODE == 1 || A.probe_start == 1) {
+        S.rx += nx_ - tx; S.ry += ny_ - ty; S.rz += nz_ - tz;
+    }
+    tx = nx_; ty = ny_; tz = nz_;
+    S.vx = S.vy = S.vz = 0;
+    if (MODE == 0) {
+        S.dt = A.dtmax;
+        S.ac = A.a0;
+    }
+}
+
+// One pixel column (i, j): every z step of the stroke, FIRE to convergence,
+// outputs in stroke order.  Counts force evaluations and finished positions.
+template <typename T4, int MODE>
+__device__ __forceinline__ void relax_column(const DevArgs &A, const T4 *__restrict__ grid, int i, int j, bool live,
+                                             unsigned long long &nevals, unsigned long long &npos)
+{
+    const int nz = A.nz;
+    const Stroke c = stroke_at(A, i, j);
+    double tx = 0, ty = 0, tz = 0;
+    int s = live ? 0 : nz;
+    if (live) tip_pos(A, c, 0, live, tx, ty, tz);
+    double ox, oy, oz;  // equilibrium offset in the world frame
+    probe_offset<MODE>(A, ox, oy, oz);
+    Fire S{tx + ox, ty + oy, tz + oz, 0, 0, 0, A.dtmax, A.a0};
+    int it = 0, itr = 0;
+    bool sample = false;  // mode 0: next gather is the output sample
+
+    while (s < nz) {
+        double fe[4], gx, gy, gz;
+        to_grid(A, S.rx, S.ry, S.rz, gx, gy, gz);
+        gather(grid, A, gx, gy, gz, fe, MODE == 1);
+        nevals++;
+        bool done = false;
+        const long o = c.col0 + s;
+        if (MODE == 0 && sample) {
+            store_pos(A, o, S.rx, S.ry, S.rz);
+            if (A.force) A.force[o] = make_float4((float)fe[0], (float)fe[1], (float)fe[2], 0.f);
+            if (A.iters) A.iters[o] = itr;
+            done = true;
+        } else {
+            const double ff = fire_move<MODE>(A, fe, tx, ty, tz, S);
+            it++;
+            const bool conv = ff < A.conv_f2;
+            if (conv || it >= A.max_iters) {
+                if (MODE == 0) {
+                    itr = conv ? it - 1 : A.max_iters;
+                    sample = true;
+                } else {
+                    if (A.force) A.force[o] = tip_frame_force(A, fe);
+                    store_pos(A, o, S.rx, S.ry, S.rz);
+                    if (A.iters) A.iters[o] = it;
+                    done = true;
+                }
+            }
+        }
+        if (done) {
+            npos++;
+            s++;
+            if (s < nz) {
+                next_step<MODE>(A, c, s, live, tx, ty, tz, S);
+                it = 0;
+                sample = false;
+            }
+        }
+    }
+}
+
+// Wave sum of the per-lane counters, one atomic per wave.
+__device__ __forceinline__ void flush_counters(unsigned long long *counters, unsigned long long nevals,
+                                               unsigned long long npos, int lane)
+{
+    if (!counters) return;
+    for (int off = 32; off > 0; off >>= 1) {
+        nevals += __shfl_xor(nevals, off, 64);
+        npos += __shfl_xor(npos, off, 64);
+    }
+    if (lane == 0) {
+        atomicAdd(&counters[0], nevals);
+        atomicAdd(&counters[1], npos);
     }
 }
 
@@ -101,174 +335,90 @@ __global__ __launch_bounds__(64) void relax_dev_kernel(DevArgs A, const T4 *__re
     const int ti = tile / A.tiles_y, tj = tile % A.tiles_y;
     const int lane = threadIdx.x;
     const int i = ti * 8 + lane / 8, j = tj * 8 + lane % 8;
-    const bool live = i < A.nx && j < A.ny;
-    const int nz = A.nz;
-    const long col0 = ((long)i * A.ny + j) * nz;
-
-    // tip position of step s: base + s*dstep (or the explicit array)
-    double bx = A.p0[0] + i * A.da[0] + j * A.db[0];
-    double by = A.p0[1] + i * A.da[1] + j * A.db[1];
-    double bz = A.p0[2] + i * A.da[2] + j * A.db[2];
-    auto tip_at = [&](int s, double &x, double &y, double &z) {
-        if (A.tips) {
-            const double *p = A.tips + 3 * (col0 + s);
-            x = p[0]; y = p[1]; z = p[2];
-        } else {
-            x = bx + s * A.dstep[0]; y = by + s * A.dstep[1]; z = bz + s * A.dstep[2];
-        }
-    };
-
-    double tx = 0, ty = 0, tz = 0;
-    int s = live ? 0 : nz;
-    if (live) tip_at(0, tx, ty, tz);
-    double ox, oy, oz;  // equilibrium offset in the world frame
-    if (MODE == 0) {
-        ox = A.d0[0]; oy = A.d0[1]; oz = A.d0[2];
-    } else {
-        ox = A.rot[0] * A.d0[0] + A.rot[3] * A.d0[1] + A.rot[6] * A.d0[2];
-        oy = A.rot[1] * A.d0[0] + A.rot[4] * A.d0[1] + A.rot[7] * A.d0[2];
-        oz = A.rot[2] * A.d0[0] + A.rot[5] * A.d0[1] + A.rot[8] * A.d0[2];
-    }
-    double rx = tx + ox, ry = ty + oy, rz = tz + oz;
-    double vx = 0, vy = 0, vz = 0;
-    double dt = A.dtmax, ac = A.a0;
-    int it = 0, itr = 0;
-    bool sample = false;  // mode 0: next gather is the output sample
     unsigned long long nevals = 0, npos = 0;
+    relax_column<T4, MODE>(A, grid, i, j, i < A.nx && j < A.ny, nevals, npos);
+    flush_counters(A.counters, nevals, npos, lane);
+}
 
-    while (s < nz) {
-        double fe[4];
-        const double gx = rx * A.dinv[0] + ry * A.dinv[1] + rz * A.dinv[2];
-        const double gy = rx * A.dinv[3] + ry * A.dinv[4] + rz * A.dinv[5];
-        const double gz = rx * A.dinv[6] + ry * A.dinv[7] + rz * A.dinv[8];
-        gather(grid, A, gx, gy, gz, fe, MODE == 1);
-        nevals++;
-        bool done = false;
-        if (MODE == 0 && sample) {
-            const long o = col0 + s;
-            if (A.pos) {
-                if (A.pos_f32) {
-                    float *p = (float *)A.pos + 3 * o;
-                    p[0] = (float)rx; p[1] = (float)ry; p[2] = (float)rz;
-                } else {
-                    double *p = (double *)A.pos + 3 * o;
-                    p[0] = rx; p[1] = ry; p[2] = rz;
-                }
+// ---------------------------------------------------------------------------
+// Batches: a persistent kernel.  Each wave takes pixel tiles (64 columns of
+// one molecule, TY columns along y) from a work queue until none are left.
+// The tile range is split into 8 contiguous parts, one per XCD (workgroups
+// are dispatched to XCDs round robin, so blockIdx % 8 names the XCD): the
+// molecules of one part share that XCD's L2.  A wave that finds its part
+// empty moves on to the next part, which balances the tail.  queue[0..7]
+// are the part heads and must be zero at launch.
+struct BatchArgs {
+    int nitems;
+    long long ntiles;
+    unsigned *queue;
+    float *pos;  // (positions, 3) FP32
+    float4 *force;
+    int *iters;
+};
+
+__device__ __forceinline__ int find_batch_item(const long long *__restrict__ prefix, int nitems, long long x)
+{
+    int lo = 0, hi = nitems;  // prefix[lo] <= x < prefix[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (prefix[mid] <= x) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <typename T4, int MODE, int TY>
+__global__ __launch_bounds__(256) void relax_batch_kernel(DevArgs A0, BatchArgs B, const BatchRelaxItem *__restrict__ items,
+                                                          const long long *__restrict__ prefix,
+                                                          const T4 *__restrict__ grids)
+{
+    constexpr int TX = 64 / TY;
+    const int lane = threadIdx.x & 63;
+    const int xcd = blockIdx.x & 7;
+    const long long per = (B.ntiles + 7) / 8;
+    unsigned long long nevals = 0, npos = 0;
+    for (int q = 0; q < 8; q++) {
+        const int part = (xcd + q) & 7;
+        const long long lo = part * per;
+        const long long hi = min(B.ntiles, lo + per);
+        if (lo >= hi) continue;
+        while (true) {
+            unsigned t = 0;
+            if (lane == 0) t = atomicAdd(&B.queue[part], 1u);
+            t = __builtin_amdgcn_readfirstlane(__shfl(t, 0, 64));
+            const long long tile = lo + (long long)t;
+            if (tile >= hi) break;
+            const int m = find_batch_item(prefix, B.nitems, tile);
+            const BatchRelaxItem &it = items[m];
+            DevArgs A = A0;
+            A.n0 = it.n[0]; A.n1 = it.n[1]; A.n2 = it.n[2];
+            A.nx = it.nx; A.ny = it.ny; A.nz = it.nz;
+            for (int k = 0; k < 9; k++) A.dinv[k] = it.dinv[k];
+            for (int k = 0; k < 3; k++) {
+                A.p0[k] = it.p0[k]; A.da[k] = it.da[k]; A.db[k] = it.db[k]; A.dstep[k] = it.dstep[k];
             }
-            if (A.force) A.force[o] = make_float4((float)fe[0], (float)fe[1], (float)fe[2], 0.f);
-            if (A.iters) A.iters[o] = itr;
-            done = true;
-        } else {
-            double fx = fe[0], fy = fe[1], fz = fe[2];
-            const double dx = rx - tx, dy = ry - ty, dz = rz - tz;
-            if (MODE == 0) {
-                const double l2 = dx * dx + dy * dy + dz * dz;
-                const double il = rsq_nr(l2);
-                const double sr = A.krad * (l2 * il - A.lrad) * il;
-                fx += dx * sr + (dx - A.d0[0]) * A.ks[0];
-                fy += dy * sr + (dy - A.d0[1]) * A.ks[1];
-                fz += dz * sr + (dz - A.d0[2]) * A.ks[2];
-            } else {
-                const double qx = dx * A.rot[0] + dy * A.rot[1] + dz * A.rot[2];
-                const double qy = dx * A.rot[3] + dy * A.rot[4] + dz * A.rot[5];
-                const double qz = dx * A.rot[6] + dy * A.rot[7] + dz * A.rot[8];
-                const double l2 = qx * qx + qy * qy + qz * qz;
-                const double il = rsq_nr(l2);
-                const double sr = A.krad * (l2 * il - A.lrad) * il;
-                const double ax = (qx - A.d0[0]) * A.ks[0] + qx * sr;
-                const double ay = (qy - A.d0[1]) * A.ks[1] + qy * sr;
-                const double az = (qz - A.d0[2]) * A.ks[2] + qz * sr + A.surf_ff;
-                fx += A.rot[0] * ax + A.rot[3] * ay + A.rot[6] * az;
-                fy += A.rot[1] * ax + A.rot[4] * ay + A.rot[7] * az;
-                fz += A.rot[2] * ax + A.rot[5] * ay + A.rot[8] * az;
-            }
-            const double ff = fx * fx + fy * fy + fz * fz;
-            const double vv = vx * vx + vy * vy + vz * vz;
-            const double vf = vx * fx + vy * fy + vz * fz;
-            if (vf < 0) {
-                vx = vy = vz = 0;
-                dt = MODE == 0 ? dt * A.fdec : fmax(A.dtmin, dt * A.fdec);
-                ac = A.a0;
-            } else {
-                const double cf = vv > 0 ? ac * (vv * rsq_nr(vv)) * rsq_nr(MODE == 0 ? ff : ff + A.f2safe) : 0.0;
-                const double cv = 1 - ac;
-                vx = vx * cv + fx * cf;
-                vy = vy * cv + fy * cf;
-                vz = vz * cv + fz * cf;
-                dt = fmin(dt * A.finc, A.dtmax);
-                ac *= A.falpha;
-            }
-            vx += fx * dt; vy += fy * dt; vz += fz * dt;
-            rx += vx * dt; ry += vy * dt; rz += vz * dt;
-            it++;
-            const bool conv = ff < A.conv_f2;
-            if (conv || it >= A.max_iters) {
-                if (MODE == 0) {
-                    itr = conv ? it - 1 : A.max_iters;
-                    sample = true;
-                } else {
-                    const long o = col0 + s;
-                    if (A.force)
-                        A.force[o] = make_float4((float)(fe[0] * A.rot[0] + fe[1] * A.rot[1] + fe[2] * A.rot[2]),
-                                                 (float)(fe[0] * A.rot[3] + fe[1] * A.rot[4] + fe[2] * A.rot[5]),
-                                                 (float)(fe[0] * A.rot[6] + fe[1] * A.rot[7] + fe[2] * A.rot[8]),
-                                                 (float)fe[3]);
-                    if (A.pos) {
-                        if (A.pos_f32) {
-                            float *p = (float *)A.pos + 3 * o;
-                            p[0] = (float)rx; p[1] = (float)ry; p[2] = (float)rz;
-                        } else {
-                            double *p = (double *)A.pos + 3 * o;
-                            p[0] = rx; p[1] = ry; p[2] = rz;
-                        }
-                    }
-                    if (A.iters) A.iters[o] = it;
-                    done = true;
-                }
-            }
-        }
-        if (done) {
-            npos++;
-            s++;
-            if (s < nz) {
-                double nx_, ny_, nz_;
-                tip_at(s, nx_, ny_, nz_);
-                if (MODE == 0 && A.probe_start == 0) {
-                    rx = nx_ + A.d0[0]; ry = ny_ + A.d0[1]; rz = nz_ + A.d0[2];
-                } else if (MODE == 1 || A.probe_start == 1) {
-                    rx += nx_ - tx; ry += ny_ - ty; rz += nz_ - tz;
-                }
-                tx = nx_; ty = ny_; tz = nz_;
-                vx = vy = vz = 0;
-                if (MODE == 0) {
-                    dt = A.dtmax;
-                    ac = A.a0;
-                }
-                it = 0;
-                sample = false;
-            }
+            A.tips = nullptr;
+            A.pos = B.pos ? (void *)(B.pos + 3 * it.out_off) : nullptr;
+            A.pos_f32 = 1;
+            A.force = B.force ? B.force + it.out_off : nullptr;
+            A.iters = B.iters ? B.iters + it.out_off : nullptr;
+            const int lt = (int)(tile - prefix[m]);
+            const int ti = lt / it.tiles_y, tj = lt % it.tiles_y;
+            const int i = ti * TX + lane / TY, j = tj * TY + lane % TY;
+            relax_column<T4, MODE>(A, grids + it.grid_off, i, j, i < A.nx && j < A.ny, nevals, npos);
         }
     }
-
-    if (A.counters) {
-        for (int off = 32; off > 0; off >>= 1) {
-            nevals += __shfl_xor(nevals, off, 64);
-            npos += __shfl_xor(npos, off, 64);
-        }
-        if (lane == 0) {
-            atomicAdd(&A.counters[0], nevals);
-            atomicAdd(&A.counters[1], npos);
-        }
-    }
+    flush_counters(A0.counters, nevals, npos, lane);
 }
 
 // ---------------------------------------------------------------------------
 // LDS-staged variant.  The lanes of a wave relax z step s together; before
 // the step the wave reduces its probes' bounding box (grid index space),
-// copies that block of the grid (plus a margin) into LDS as three
-// structure-of-arrays planes, and every gather whose 8 corners fall inside
-// the block reads LDS; the others read global memory.  The per-lane
-// arithmetic is that of relax_dev_kernel, so the results are identical.
+// copies that block of the grid (plus a margin) into LDS as structure-of-
+// arrays planes, and every gather whose 8 corners fall inside the block
+// reads LDS; the others read global memory.  The per-lane arithmetic is the
+// shared helpers', so the results equal relax_dev_kernel's.
 constexpr int kMargin = 2;
 
 template <typename T>
@@ -309,49 +459,19 @@ __global__ __launch_bounds__(64) void relax_lds_kernel(DevArgs A, const T4 *__re
     const int i = ti * 8 + lane / 8, j = tj * 8 + lane % 8;
     const bool live = i < A.nx && j < A.ny;
     const int nz = A.nz;
-    const long col0 = ((long)i * A.ny + j) * nz;
-    const double bx = A.p0[0] + i * A.da[0] + j * A.db[0];
-    const double by = A.p0[1] + i * A.da[1] + j * A.db[1];
-    const double bz = A.p0[2] + i * A.da[2] + j * A.db[2];
-    auto tip_at = [&](int s, double &x, double &y, double &z) {
-        if (A.tips && live) {
-            const double *p = A.tips + 3 * (col0 + s);
-            x = p[0]; y = p[1]; z = p[2];
-        } else {
-            x = bx + s * A.dstep[0]; y = by + s * A.dstep[1]; z = bz + s * A.dstep[2];
-        }
-    };
+    const Stroke c = stroke_at(A, i, j);
     double tx, ty, tz;
-    tip_at(0, tx, ty, tz);
+    tip_pos(A, c, 0, live, tx, ty, tz);
     double ox, oy, oz;
-    if (MODE == 0) {
-        ox = A.d0[0]; oy = A.d0[1]; oz = A.d0[2];
-    } else {
-        ox = A.rot[0] * A.d0[0] + A.rot[3] * A.d0[1] + A.rot[6] * A.d0[2];
-        oy = A.rot[1] * A.d0[0] + A.rot[4] * A.d0[1] + A.rot[7] * A.d0[2];
-        oz = A.rot[2] * A.d0[0] + A.rot[5] * A.d0[1] + A.rot[8] * A.d0[2];
-    }
-    double rx = tx + ox, ry = ty + oy, rz = tz + oz;
-    double vx = 0, vy = 0, vz = 0, dt = A.dtmax, ac = A.a0;
+    probe_offset<MODE>(A, ox, oy, oz);
+    Fire S{tx + ox, ty + oy, tz + oz, 0, 0, 0, A.dtmax, A.a0};
     unsigned long long nevals = 0, npos = 0;
 
     for (int s = 0; s < nz; s++) {
-        if (s > 0) {
-            double nx_, ny_, nz_;
-            tip_at(s, nx_, ny_, nz_);
-            if (MODE == 0 && A.probe_start == 0) {
-                rx = nx_ + A.d0[0]; ry = ny_ + A.d0[1]; rz = nz_ + A.d0[2];
-            } else if (MODE == 1 || A.probe_start == 1) {
-                rx += nx_ - tx; ry += ny_ - ty; rz += nz_ - tz;
-            }
-            tx = nx_; ty = ny_; tz = nz_;
-            vx = vy = vz = 0;
-            if (MODE == 0) { dt = A.dtmax; ac = A.a0; }
-        }
+        if (s > 0) next_step<MODE>(A, c, s, live, tx, ty, tz, S);
         // ---- stage the block around the wave's probes
-        const double g0x = rx * A.dinv[0] + ry * A.dinv[1] + rz * A.dinv[2];
-        const double g0y = rx * A.dinv[3] + ry * A.dinv[4] + rz * A.dinv[5];
-        const double g0z = rx * A.dinv[6] + ry * A.dinv[7] + rz * A.dinv[8];
+        double g0x, g0y, g0z;
+        to_grid(A, S.rx, S.ry, S.rz, g0x, g0y, g0z);
         const float big = 3.0e38f;
         const float mnx = wave_min(live ? (float)g0x : big), mxx = wave_max(live ? (float)g0x : -big);
         const float mny = wave_min(live ? (float)g0y : big), mxy = wave_max(live ? (float)g0y : -big);
@@ -371,9 +491,9 @@ __global__ __launch_bounds__(64) void relax_lds_kernel(DevArgs A, const T4 *__re
         if (staged) {
             const int vol = ex * ey * ez;
             for (int q = lane; q < vol; q += 64) {
-                const int c = q % ez, t = q / ez;
+                const int cz = q % ez, t = q / ez;
                 const int bb = t % ey, a = t / ey;
-                const unsigned gi = ((unsigned)wrapi(lox + a, A.n0) * A.n1 + wrapi(loy + bb, A.n1)) * A.n2 + wrapi(loz + c, A.n2);
+                const unsigned gi = ((unsigned)wrapi(lox + a, A.n0) * A.n1 + wrapi(loy + bb, A.n1)) * A.n2 + wrapi(loz + cz, A.n2);
                 const T4 v = grid[gi];
                 LX[q] = v.x; LY[q] = v.y; LZ[q] = v.z;
                 if (MODE == 1) LW[q] = v.w;
@@ -384,81 +504,35 @@ __global__ __launch_bounds__(64) void relax_lds_kernel(DevArgs A, const T4 *__re
         // ---- relax this step, all lanes together
         int st = live ? 0 : 2;  // 0 = FIRE, 1 = output sample pending (mode 0), 2 = done
         int it = 0, itr = 0;
-        double ofx = 0, ofy = 0, ofz = 0, ofe = 0;
+        float4 outf = make_float4(0.f, 0.f, 0.f, 0.f);
         while (__ballot(st < 2) != 0) {
             if (st < 2) {
-                const double gx = rx * A.dinv[0] + ry * A.dinv[1] + rz * A.dinv[2];
-                const double gy = rx * A.dinv[3] + ry * A.dinv[4] + rz * A.dinv[5];
-                const double gz = rx * A.dinv[6] + ry * A.dinv[7] + rz * A.dinv[8];
+                double gx, gy, gz;
+                to_grid(A, S.rx, S.ry, S.rz, gx, gy, gz);
                 const double flx = floor(gx), fly = floor(gy), flz = floor(gz);
                 double fe[4];
                 // inside test in floating point (false for NaN), before any int conversion
                 if (staged && flx >= lox && flx + 1 < lox + ex && fly >= loy && fly + 1 < loy + ey && flz >= loz &&
                     flz + 1 < loz + ez) {
-                    const int a = (int)flx - lox, bb = (int)fly - loy, c = (int)flz - loz;
-                    const double wx = gx - flx, wy = gy - fly, wz = gz - flz;
-                    const double mx = 1 - wx, my = 1 - wy, mz = 1 - wz;
-                    const int q00 = (a * ey + bb) * ez + c, q01 = q00 + ez, q10 = q00 + ey * ez, q11 = q10 + ez;
+                    const int a = (int)flx - lox, bb = (int)fly - loy, cz = (int)flz - loz;
+                    const int q00 = (a * ey + bb) * ez + cz, q01 = q00 + ez, q10 = q00 + ey * ez, q11 = q10 + ez;
                     const int q[8] = {q00, q00 + 1, q01, q01 + 1, q10, q10 + 1, q11, q11 + 1};
-                    const double w00 = mx * my, w01 = mx * wy, w10 = wx * my, w11 = wx * wy;
-                    const double w[8] = {w00 * mz, w00 * wz, w01 * mz, w01 * wz, w10 * mz, w10 * wz, w11 * mz, w11 * wz};
-                    fe[0] = fe[1] = fe[2] = fe[3] = 0;
+                    T4 v[8];
 #pragma unroll
                     for (int k = 0; k < 8; k++) {
-                        fe[0] += (double)LX[q[k]] * w[k];
-                        fe[1] += (double)LY[q[k]] * w[k];
-                        fe[2] += (double)LZ[q[k]] * w[k];
-                        if (MODE == 1) fe[3] += (double)LW[q[k]] * w[k];
+                        v[k].x = LX[q[k]]; v[k].y = LY[q[k]]; v[k].z = LZ[q[k]];
+                        v[k].w = MODE == 1 ? LW[q[k]] : (T)0;
                     }
+                    interp8(v, gx - flx, gy - fly, gz - flz, fe, MODE == 1);
                 } else {
                     gather(grid, A, gx, gy, gz, fe, MODE == 1);
                 }
                 nevals++;
                 if (MODE == 0 && st == 1) {
-                    ofx = fe[0]; ofy = fe[1]; ofz = fe[2];
+                    outf = make_float4((float)fe[0], (float)fe[1], (float)fe[2], 0.f);
                     st = 2;
                 } else {
-                    double fx = fe[0], fy = fe[1], fz = fe[2];
-                    const double dx = rx - tx, dy = ry - ty, dz = rz - tz;
-                    if (MODE == 0) {
-                        const double l2 = dx * dx + dy * dy + dz * dz;
-                        const double il = rsq_nr(l2);
-                        const double sr = A.krad * (l2 * il - A.lrad) * il;
-                        fx += dx * sr + (dx - A.d0[0]) * A.ks[0];
-                        fy += dy * sr + (dy - A.d0[1]) * A.ks[1];
-                        fz += dz * sr + (dz - A.d0[2]) * A.ks[2];
-                    } else {
-                        const double qx = dx * A.rot[0] + dy * A.rot[1] + dz * A.rot[2];
-                        const double qy = dx * A.rot[3] + dy * A.rot[4] + dz * A.rot[5];
-                        const double qz = dx * A.rot[6] + dy * A.rot[7] + dz * A.rot[8];
-                        const double l2 = qx * qx + qy * qy + qz * qz;
-                        const double il = rsq_nr(l2);
-                        const double sr = A.krad * (l2 * il - A.lrad) * il;
-                        const double ax = (qx - A.d0[0]) * A.ks[0] + qx * sr;
-                        const double ay = (qy - A.d0[1]) * A.ks[1] + qy * sr;
-                        const double az = (qz - A.d0[2]) * A.ks[2] + qz * sr + A.surf_ff;
-                        fx += A.rot[0] * ax + A.rot[3] * ay + A.rot[6] * az;
-                        fy += A.rot[1] * ax + A.rot[4] * ay + A.rot[7] * az;
-                        fz += A.rot[2] * ax + A.rot[5] * ay + A.rot[8] * az;
-                    }
-                    const double ff = fx * fx + fy * fy + fz * fz;
-                    const double vv = vx * vx + vy * vy + vz * vz;
-                    const double vf = vx * fx + vy * fy + vz * fz;
-                    if (vf < 0) {
-                        vx = vy = vz = 0;
-                        dt = MODE == 0 ? dt * A.fdec : fmax(A.dtmin, dt * A.fdec);
-                        ac = A.a0;
-                    } else {
-                        const double cf = vv > 0 ? ac * (vv * rsq_nr(vv)) * rsq_nr(MODE == 0 ? ff : ff + A.f2safe) : 0.0;
-                        const double cv = 1 - ac;
-                        vx = vx * cv + fx * cf;
-                        vy = vy * cv + fy * cf;
-                        vz = vz * cv + fz * cf;
-                        dt = fmin(dt * A.finc, A.dtmax);
-                        ac *= A.falpha;
-                    }
-                    vx += fx * dt; vy += fy * dt; vz += fz * dt;
-                    rx += vx * dt; ry += vy * dt; rz += vz * dt;
+                    const double ff = fire_move<MODE>(A, fe, tx, ty, tz, S);
                     it++;
                     const bool conv = ff < A.conv_f2;
                     if (conv || it >= A.max_iters) {
@@ -466,10 +540,7 @@ __global__ __launch_bounds__(64) void relax_lds_kernel(DevArgs A, const T4 *__re
                             itr = conv ? it - 1 : A.max_iters;
                             st = 1;
                         } else {
-                            ofx = fe[0] * A.rot[0] + fe[1] * A.rot[1] + fe[2] * A.rot[2];
-                            ofy = fe[0] * A.rot[3] + fe[1] * A.rot[4] + fe[2] * A.rot[5];
-                            ofz = fe[0] * A.rot[6] + fe[1] * A.rot[7] + fe[2] * A.rot[8];
-                            ofe = fe[3];
+                            outf = tip_frame_force(A, fe);
                             itr = it;
                             st = 2;
                         }
@@ -478,33 +549,15 @@ __global__ __launch_bounds__(64) void relax_lds_kernel(DevArgs A, const T4 *__re
             }
         }
         if (live) {
-            const long o = col0 + s;
-            if (A.pos) {
-                if (A.pos_f32) {
-                    float *p = (float *)A.pos + 3 * o;
-                    p[0] = (float)rx; p[1] = (float)ry; p[2] = (float)rz;
-                } else {
-                    double *p = (double *)A.pos + 3 * o;
-                    p[0] = rx; p[1] = ry; p[2] = rz;
-                }
-            }
-            if (A.force) A.force[o] = make_float4((float)ofx, (float)ofy, (float)ofz, (float)ofe);
+            const long o = c.col0 + s;
+            store_pos(A, o, S.rx, S.ry, S.rz);
+            if (A.force) A.force[o] = outf;
             if (A.iters) A.iters[o] = itr;
             npos++;
         }
         __syncthreads();
     }
-
-    if (A.counters) {
-        for (int off = 32; off > 0; off >>= 1) {
-            nevals += __shfl_xor(nevals, off, 64);
-            npos += __shfl_xor(npos, off, 64);
-        }
-        if (lane == 0) {
-            atomicAdd(&A.counters[0], nevals);
-            atomicAdd(&A.counters[1], npos);
-        }
-    }
+    flush_counters(A.counters, nevals, npos, lane);
 }
 
 }  // namespace
@@ -570,6 +623,61 @@ int launch_relax_fast(const RelaxLaunch &L, const void *grid, bool f64, hipStrea
         else hipLaunchKernelGGL((relax_dev_kernel<float4, 1>), grd, blk, 0, st, A, (const float4 *)grid);
     }
     return 0;
+}
+
+static void fill_shared_args(DevArgs &A, const ppafm_relax_params &rp, const ppafm_tip &tip)
+{
+    A.max_iters = rp.max_iters;
+    A.probe_start = rp.probe_start;
+    for (int k = 0; k < 9; k++) A.rot[k] = tip.rot[k];
+    A.conv_f2 = rp.conv_f2; A.dtmax = rp.dt; A.a0 = rp.damping;
+    A.finc = rp.finc; A.fdec = rp.fdec; A.falpha = rp.falpha;
+    A.dtmin = rp.dtmin_frac * rp.dt; A.f2safe = rp.f2safe;
+    for (int k = 0; k < 3; k++) { A.ks[k] = tip.kspring[k]; A.d0[k] = tip.d0[k]; }
+    A.krad = tip.krad; A.lrad = tip.lrad; A.surf_ff = tip.surf_ff;
+}
+
+template <typename T4, int MODE>
+static int launch_batch_ty(int ty, dim3 grd, dim3 blk, hipStream_t st, const DevArgs &A, const BatchArgs &B,
+                           const BatchRelaxItem *items, const long long *prefix, const void *grids)
+{
+    switch (ty) {
+    case 1: hipLaunchKernelGGL((relax_batch_kernel<T4, MODE, 1>), grd, blk, 0, st, A, B, items, prefix, (const T4 *)grids); break;
+    case 2: hipLaunchKernelGGL((relax_batch_kernel<T4, MODE, 2>), grd, blk, 0, st, A, B, items, prefix, (const T4 *)grids); break;
+    case 4: hipLaunchKernelGGL((relax_batch_kernel<T4, MODE, 4>), grd, blk, 0, st, A, B, items, prefix, (const T4 *)grids); break;
+    case 8: hipLaunchKernelGGL((relax_batch_kernel<T4, MODE, 8>), grd, blk, 0, st, A, B, items, prefix, (const T4 *)grids); break;
+    case 16: hipLaunchKernelGGL((relax_batch_kernel<T4, MODE, 16>), grd, blk, 0, st, A, B, items, prefix, (const T4 *)grids); break;
+    default: set_error("tile_y must be 1, 2, 4, 8 or 16"); return -2;
+    }
+    return 0;
+}
+
+int relax_batch_tiles(int nx, int ny, int tile_y, int &tiles_y)
+{
+    const int tx = 64 / tile_y;
+    tiles_y = (ny + tile_y - 1) / tile_y;
+    return ((nx + tx - 1) / tx) * tiles_y;
+}
+
+int launch_relax_batch(const RelaxBatchLaunch &L, hipStream_t st)
+{
+    PP_REQUIRE(L.rp.alg == 1 && L.rp.max_iters >= 1, "batched relaxation needs FIRE (alg 1) and max_iters >= 1");
+    PP_REQUIRE(L.rp.mode == 0 || L.rp.mode == 1, "relax mode must be 0 or 1");
+    PP_REQUIRE(L.rp.probe_start >= -1 && L.rp.probe_start <= 1, "probeStart must be -1, 0 or 1");
+    PP_REQUIRE(L.waves_per_block >= 1 && L.waves_per_block <= 4, "waves_per_block must be 1..4");
+    PP_REQUIRE(L.nblocks >= 1, "nblocks must be >= 1");
+    DevArgs A = {};
+    fill_shared_args(A, L.rp, L.tip);
+    A.counters = L.counters;
+    BatchArgs B{L.nitems, L.ntiles, L.queue, L.pos, L.force, L.iters};
+    if (L.ntiles == 0) return 0;
+    dim3 grd((unsigned)L.nblocks), blk(64 * L.waves_per_block);
+    if (L.f64) {
+        if (L.rp.mode == 0) return launch_batch_ty<double4, 0>(L.tile_y, grd, blk, st, A, B, L.items, L.prefix, L.grids);
+        return launch_batch_ty<double4, 1>(L.tile_y, grd, blk, st, A, B, L.items, L.prefix, L.grids);
+    }
+    if (L.rp.mode == 0) return launch_batch_ty<float4, 0>(L.tile_y, grd, blk, st, A, B, L.items, L.prefix, L.grids);
+    return launch_batch_ty<float4, 1>(L.tile_y, grd, blk, st, A, B, L.items, L.prefix, L.grids);
 }
 
 }  // namespace pp

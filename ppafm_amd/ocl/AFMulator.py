@@ -169,8 +169,9 @@ class AFMulator:
         self.scanner.stiffness = np.array(tipStiffness, dtype=np.float32) / -common.eVA_Nm
 
     # --------------------------------------------------------------- imaging
-    def prepareFF(self, xyzs, Zs, qs, rho_sample=None, sample_lvec=None, rot=np.eye(3), rot_center=None, REAs=None):
-        self.check_scan_window()
+    def _prepare_atoms(self, xyzs, Zs, qs, rho_sample=None, sample_lvec=None, rot=np.eye(3), rot_center=None, REAs=None):
+        """Atoms of one molecule as the grid kernel takes them: (xyzq relative
+        to the grid origin, cLJs, charged)."""
         if rho_sample is not None:
             raise NotImplementedError("FDBM (rho_sample) is not on this path")
         if qs is not None and not isinstance(qs, np.ndarray) and not isinstance(qs, (list, tuple)):
@@ -196,9 +197,16 @@ class AFMulator:
         cLJs = np.asarray(cLJs, dtype=np.float32).astype(np.float64)
         lvec0 = self.lvec[0].astype(np.float32).astype(np.float64)
         xyzq = np.concatenate([xyzs - lvec0[None, :], qs[:, None]], axis=1)
-        charged = not np.allclose(qs, 0)
+        return xyzq, cLJs, not np.allclose(qs, 0)
+
+    def _ff_model(self, charged):
         stencil = list(zip(np.float32(self.Qs).tolist(), np.float32(self.QZs).tolist())) if charged else []
-        self._model = D.ff_model(1, with_lj=True, stencil=stencil, with_energy=True, fmax=None)
+        return D.ff_model(1, with_lj=True, stencil=stencil, with_energy=True, fmax=None)
+
+    def prepareFF(self, xyzs, Zs, qs, rho_sample=None, sample_lvec=None, rot=np.eye(3), rot_center=None, REAs=None):
+        self.check_scan_window()
+        xyzq, cLJs, charged = self._prepare_atoms(xyzs, Zs, qs, rho_sample, sample_lvec, rot, rot_center, REAs)
+        self._model = self._ff_model(charged)
         n = tuple(int(v) for v in self.nDim[:3])
         if self._grid is None or self._nDim_alloc != n:
             # double4 grid summed in FP64 (see ppafm_amd.scan.PlainScan for why)
@@ -206,6 +214,44 @@ class AFMulator:
             self._nDim_alloc = n
         cell = self.lvec[1:].astype(np.float32).astype(np.float64)
         D.ff_build(n, cell, xyzq, cLJs, self._model, out=self._grid)
+
+    def batch(self, molecules, tuning=None):
+        """Evaluate many molecules in one batch plan (one captured HIP graph).
+
+        molecules: sequence of (xyzs, Zs, qs) or dicts with those keys plus
+        the optional ``rot``, ``rot_center``, ``REAs`` of ``__call__``.
+        Returns the list of df images X, each as ``__call__`` returns it.
+        No counterpart in the reference (one __call__ per molecule,
+        AFMulator.py:142-179); the numbers equal the per-molecule calls."""
+        from ..batch import BatchScan
+
+        self.check_scan_window()
+        packs, charged = [], False
+        for m in molecules:
+            kw = dict(m) if isinstance(m, dict) else dict(zip(("xyzs", "Zs", "qs"), m))
+            xyzq, cLJs, ch = self._prepare_atoms(kw["xyzs"], kw["Zs"], kw.get("qs"), rot=kw.get("rot", np.eye(3)),
+                                                 rot_center=kw.get("rot_center"), REAs=kw.get("REAs"))
+            charged = charged or ch
+            packs.append((xyzq, cLJs))
+        model = self._ff_model(charged)
+        packs = [D.pack_atoms(xyzq, cLJs, model.coulomb_const) for xyzq, cLJs in packs]
+        self.prepareScanner()
+        n = tuple(int(v) for v in self.nDim[:3])
+        cell = self.lvec[1:].astype(np.float32).astype(np.float64)
+        key = (n, tuple(len(p) for p in packs), tuple(self.scan_dim), self.dfWeight.tobytes(), charged,
+               np.asarray(self.stiffness).tobytes(), np.asarray(self.relax_params).tobytes(), tuple(np.float32(self.tipR0)),
+               repr(tuning))
+        if getattr(self, "_batch_key", None) != key:
+            self._batch = None
+            items = [dict(n=n, cell=cell, natoms=len(p), scan=self._scan) for p in packs]
+            self._batch = BatchScan(items, model, self._rp, self._tip, df_weights=self.dfWeight.astype(np.float64),
+                                    df_scale=1.0, grid_f64=True, tuning=tuning)
+            self._batch_key = key
+        b = self._batch
+        b.set_atoms(packs)
+        b.run()
+        b.synchronize()
+        return [b.df_raw(k) for k in range(len(packs))]
 
     def prepareScanner(self):
         self.pos0 = np.array([0, 0, self.scan_window[1][2]]) - self.lvec[0]

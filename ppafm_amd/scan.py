@@ -18,6 +18,35 @@ from . import common as PPU
 from . import device as D
 
 
+def plain_setup(lvec, scan, tipR0=(0.0, 0.0, 3.0), k_lat=0.25, k_rad=30.0, probe_q=-0.1, tip="s", amplitude=0.8,
+                k0=1800.0, f0=30300.0, Fmax=10.0, relax=None):
+    """Descriptors of one ProbeParticle-semantics scan: (ff model, relax
+    params, tip, scan, df weights, df scale).  Positions are relative to the
+    grid origin lvec[0]."""
+    lvec = np.asarray(lvec, dtype=np.float64)
+    model = D.ff_model(0, with_lj=True, stencil=D.PROBE_STENCILS[tip], probe_q=probe_q, fmax=Fmax)
+    rp = D.relax_params(mode=0, **(relax or {}))
+    kS = np.array([k_lat, k_lat, 0.0]) / -PPU.eVA_Nm
+    tip_d = D.cpu_tip(tipR0[2], k_rad / -PPU.eVA_Nm, (tipR0[0], tipR0[1]), kS)
+    nx, ny = scan["scan_n"]
+    nz = scan["nz"]
+    x0, x1 = scan["scan_x"]
+    y0, y1 = scan["scan_y"]
+    o = lvec[0]
+    scan_d = D.scan_desc(
+        nx, ny, nz,
+        p0=(x0 - o[0], y0 - o[1], scan["z_top"] - o[2]),
+        da=((x1 - x0) / max(nx - 1, 1), 0, 0),
+        db=(0, (y1 - y0) / max(ny - 1, 1), 0),
+        dstep=(0, 0, -scan["dz"]),
+    )
+    # Fz2df: correlating the stroke-ordered (z descending) Fz with the
+    # weights as returned equals np.convolve on the ascending stack.
+    w = PPU.get_df_weight(amplitude, dz=scan["dz"])
+    df_scale = 16.0217656 * f0 / k0
+    return model, rp, tip_d, scan_d, w, df_scale
+
+
 class PlainScan:
     """Relaxed AFM scan of one molecule, ProbeParticle semantics, on the GPU.
 
@@ -43,23 +72,11 @@ class PlainScan:
         self.stream = stream
         self.iZPP = iZPP
         self.species = species or PPU.loadSpecies()
-        self.model = D.ff_model(0, with_lj=True, stencil=D.PROBE_STENCILS[tip], probe_q=probe_q, fmax=Fmax)
-        relax = relax or {}
-        self.rp = D.relax_params(mode=0, **relax)
-        kS = np.array([k_lat, k_lat, 0.0]) / -PPU.eVA_Nm
-        self.tip = D.cpu_tip(tipR0[2], k_rad / -PPU.eVA_Nm, (tipR0[0], tipR0[1]), kS)
+        self.model, self.rp, self.tip, self.scan_d, self.w, self.df_scale = plain_setup(
+            self.lvec, self.scan, tipR0=tipR0, k_lat=k_lat, k_rad=k_rad, probe_q=probe_q, tip=tip, amplitude=amplitude,
+            k0=k0, f0=f0, Fmax=Fmax, relax=relax)
         nx, ny = self.scan["scan_n"]
         nz = self.scan["nz"]
-        x0, x1 = self.scan["scan_x"]
-        y0, y1 = self.scan["scan_y"]
-        o = self.lvec[0]
-        self.scan_d = D.scan_desc(
-            nx, ny, nz,
-            p0=(x0 - o[0], y0 - o[1], self.scan["z_top"] - o[2]),
-            da=((x1 - x0) / max(nx - 1, 1), 0, 0),
-            db=(0, (y1 - y0) / max(ny - 1, 1), 0),
-            dstep=(0, 0, -self.scan["dz"]),
-        )
         self.grid_f64 = bool(grid_f64)
         self.grid = D.DeviceArray(self.n + (4,), np.float64 if self.grid_f64 else np.float32)
         self.pos = D.DeviceArray((nx, ny, nz, 3), np.float32)
@@ -67,10 +84,6 @@ class PlainScan:
         self.iters = D.DeviceArray((nx, ny, nz), np.int32)
         self.counters = D.DeviceArray((2,), np.uint64)
         self.counters.zero()
-        # Fz2df: correlating the stroke-ordered (z descending) Fz with the
-        # weights as returned equals np.convolve on the ascending stack.
-        self.w = PPU.get_df_weight(amplitude, dz=self.scan["dz"])
-        self.df_scale = 16.0217656 * f0 / k0
         self.nzout = nz - len(self.w) + 1
         self.df = D.DeviceArray((nx * ny, max(self.nzout, 1)), np.float32)
         self.atoms = None

@@ -1,0 +1,169 @@
+"""GPU tests of the batched path (one captured graph for many molecules).
+
+The batch runs the same per-brick and per-column device code as the
+single-molecule path, so every output of every item must equal the
+single-molecule PlainScan / AFMulator result bit for bit; one item is also
+checked against the FP64 oracle at the plain path's tolerances (positions
+1e-3 A, df 1e-3 Hz).  Items differ in atom count, grid size, scan size and
+number of z steps (two df groups)."""
+
+import numpy as np
+import pytest
+
+import oracle as O
+from cases import small_config, tip_params
+from ppafm_amd import batch as B
+from ppafm_amd import common as C
+from ppafm_amd import molgen as M
+from ppafm_amd._lib import HipError
+from ppafm_amd.scan import PlainScan
+
+pytestmark = pytest.mark.gpu
+
+# (grid n, scan (nx, ny), nz, natoms, seed)
+SPECS = [
+    ((64, 64, 80), (24, 24), 20, 20, 7),
+    ((48, 56, 72), (17, 9), 20, 8, 8),
+    ((72, 64, 64), (30, 21), 16, 33, 9),
+    ((64, 48, 80), (8, 40), 16, 11, 10),
+    ((56, 56, 88), (13, 13), 20, 47, 11),
+]
+
+
+def _cases():
+    out = []
+    for n, scan_n, nz, natoms, seed in SPECS:
+        cfg, _ = small_config(n=n, scan_n=scan_n, nz=nz, natoms=natoms, seed=seed)
+        out.append((cfg, M.config_lvec(cfg), M.molecule(natoms, seed)))
+    return out
+
+
+def _plain(cfg, lvec, mol, **kw):
+    s = PlainScan(lvec, cfg["n"], cfg, tipR0=M.TIP_R0, k_lat=M.K_LAT, k_rad=M.K_RAD, iZPP=M.PROBE_Z, probe_q=M.PROBE_Q,
+                  amplitude=M.DF_AMPLITUDE, **kw)
+    s.set_molecule(*mol)
+    s.run()
+    return s
+
+
+def _batch(cases, **kw):
+    return B.BatchScan.plain([c[1] for c in cases], [c[0]["n"] for c in cases], [c[0] for c in cases], tipR0=M.TIP_R0,
+                             k_lat=M.K_LAT, k_rad=M.K_RAD, iZPP=M.PROBE_Z, probe_q=M.PROBE_Q,
+                             amplitude=M.DF_AMPLITUDE, molecules=[c[2] for c in cases], keep_iters=True, **kw)
+
+
+@pytest.fixture(scope="module")
+def run():
+    print("case: plain batch of 5 molecules (8..47 atoms, 4 grid sizes, nz 20/16)", flush=True)
+    cases = _cases()
+    b = _batch(cases)
+    b.run()
+    b.synchronize()
+    singles = [_plain(*c) for c in cases]
+    return cases, b, singles
+
+
+def test_batch_equals_single_scans(run):
+    cases, b, singles = run
+    for k, s in enumerate(singles):
+        assert np.array_equal(b.grid(k), s.grid_host()), k
+        assert np.array_equal(b.positions_raw(k), s.pos.download()), k
+        assert np.array_equal(b.forces(k), s.forces()), k
+        assert np.array_equal(b.iterations(k), s.iterations()), k
+        assert np.array_equal(b.df_host(k), s.df_host()), k
+    ne, npos = b.counts()
+    assert npos == b.total_positions()
+    assert ne == sum(s.counts()[0] for s in singles)
+
+
+def test_batch_item_against_oracle(run):
+    cases, b, _ = run
+    k = 4
+    cfg, lvec, (xyz, Zs, qs) = cases[k]
+    cLJs = C.getAtomsLJ(M.PROBE_Z, Zs, C.loadSpecies())
+    Rs0 = xyz - lvec[0]
+    FL = O.limit_vec_field(O.lj_ff(cfg["n"], lvec[1:], Rs0, cLJs), 10.0)
+    FE = O.limit_vec_field(O.coulomb_ff(cfg["n"], lvec[1:], Rs0, qs * C.CoulombConst), 10.0)
+    FF = FL + M.PROBE_Q * FE
+    tips = M.scan_tips(cfg) - lvec[0]
+    rs, fs, it, _ = O.relax_strokes(cfg["n"], lvec[1:], FF, tip_params(), tips)
+    dr = np.abs(b.positions(k) - lvec[0] - rs).max()
+    df_ref = O.fz2df(fs[:, :, ::-1, 2], 0.1, M.K0, M.F0, amplitude=M.DF_AMPLITUDE)
+    ddf = np.abs(b.df_host(k) - df_ref).max()
+    print(f"batch item {k}: max |dr| {dr:.3g} A, max |ddf| {ddf:.3g} Hz")
+    assert dr < 1e-3 and ddf < 1e-3
+
+
+def test_graph_replays_and_tunings_agree(run):
+    cases, b, _ = run
+    ref = [(b.positions_raw(k), b.forces(k), b.df_raw(k)) for k in range(b.nitems)]
+    c0 = b.counts()
+    for tuning in ({"tile_y": 4, "waves_per_block": 2, "blocks_per_cu": 3}, {"tile_y": 16, "waves_per_block": 4, "blocks_per_cu": 1},
+                   {"tile_y": 1, "waves_per_block": 1, "blocks_per_cu": 2}, {"tile_y": 8, "waves_per_block": 1, "blocks_per_cu": 16}):
+        b.set_tuning(tuning)
+        assert b.tuning == tuning
+        for rep in range(2):
+            b.run()
+            b.synchronize()
+            # eager work between replays
+            b.run_stages(4)
+            b.synchronize()
+            assert b.counts() == c0, (tuning, rep)
+            for k in range(b.nitems):
+                p, f, d = ref[k]
+                assert np.array_equal(b.positions_raw(k), p), (tuning, rep, k)
+                assert np.array_equal(b.forces(k), f), (tuning, rep, k)
+                assert np.array_equal(b.df_raw(k), d), (tuning, rep, k)
+
+
+def test_autotuner_and_cache(run, tmp_path):
+    cases, b, _ = run
+    ref = [b.positions_raw(k) for k in range(b.nitems)]
+    best, ms = b.tune(nsample=3, reps=1)
+    assert best["tile_y"] in (2, 4, 8, 16) and 1 <= best["waves_per_block"] <= 4 and best["blocks_per_cu"] >= 1
+    assert ms > 0
+    assert b.tuning == best
+    b.run()
+    b.synchronize()
+    for k in range(b.nitems):
+        assert np.array_equal(b.positions_raw(k), ref[k])
+    path = tmp_path / "tuning.json"
+    t1 = B.tuned(b, nsample=2, reps=1, path=path)
+    assert path.exists()
+    t2 = B.tuned(b, nsample=2, reps=1, path=path)  # cache hit
+    assert t1 == t2
+
+
+def test_batch_errors(run):
+    cases, b, _ = run
+    with pytest.raises(ValueError):
+        b.set_atoms([np.zeros((3, 6))] * b.nitems)
+    with pytest.raises(HipError):
+        b.set_tuning({"tile_y": 3, "waves_per_block": 1, "blocks_per_cu": 1})
+    assert b.tuning["tile_y"] != 3
+
+
+def test_afmulator_batch_equals_calls():
+    from ppafm_amd.ocl.AFMulator import AFMulator
+
+    print("case: AFMulator.batch of 4 molecules vs __call__ per molecule", flush=True)
+    afm = AFMulator(scan_dim=(24, 20, 18), scan_window=((-2.0, -2.0, 7.0), (2.0, 2.0, 8.8)), df_steps=8)
+    mols = [M.molecule(n, 100 + n) for n in (8, 20, 35, 12)]
+    Xs = afm.batch(mols)
+    for m, X in zip(mols, Xs):
+        Xr = afm(*m)
+        assert X.shape == Xr.shape
+        assert np.array_equal(X, Xr)
+
+
+def test_batch_f32_grid_equals_single_scans():
+    print("case: plain batch, float4 grids, 3 molecules", flush=True)
+    cases = _cases()[:3]
+    b = _batch(cases, grid_f64=False)
+    b.run()
+    b.synchronize()
+    for k, c in enumerate(cases):
+        s = _plain(*c, grid_f64=False)
+        assert np.array_equal(b.grid(k), s.grid_host()), k
+        assert np.array_equal(b.positions_raw(k), s.pos.download()), k
+        assert np.array_equal(b.df_host(k), s.df_host()), k
