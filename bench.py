@@ -41,6 +41,7 @@ def parse():
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--grid", choices=["f64", "f32"], default="f64", help="grid storage of the measured plain path")
+    p.add_argument("--batch-items", type=int, default=512, help="molecules of the batched workload (0 skips it)")
     return p.parse_args()
 
 
@@ -118,6 +119,50 @@ def run_gpu(args, cfg, xyz, Zs, qs, lvec):
     return dict(scan=s, t_step=t_total, t_grid=t_grid, t_relax=t_relax, t_df=t_df, wall_step_ms=1e3 * wall / args.steps, n_eval=nev, n_pos=npos, mean_iters=float(iters.mean()))
 
 
+def run_batch(args):
+    """Batched workload: seeded molecules of U[8,120] atoms, each with its own
+    128x128x160 grid and 128x128x30 scan, run as one captured graph (grid
+    build, persistent FIRE relaxation, df) with the autotuned launch shape."""
+    from ppafm_amd import batch as B
+    from ppafm_amd import device as D
+    from ppafm_amd import molgen as M
+
+    cfg = M.CONFIGS[2]
+    nb = args.batch_items
+    mols = [M.config_molecule(2, k) for k in range(nb)]
+    lvec = M.config_lvec(cfg)
+    b = B.BatchScan.plain([lvec] * nb, [cfg["n"]] * nb, [cfg] * nb, tipR0=M.TIP_R0, k_lat=M.K_LAT, k_rad=M.K_RAD,
+                          iZPP=M.PROBE_Z, probe_q=M.PROBE_Q, amplitude=M.DF_AMPLITUDE, molecules=mols,
+                          grid_f64=args.grid == "f64", tuning="auto")
+    st = b.stream
+    for _ in range(max(1, args.warmup)):
+        b.run()
+    b.synchronize()
+    steps = max(1, min(args.steps, 3))
+    e0, e1 = D.Event(), D.Event()
+    e0.record(st)
+    for _ in range(steps):
+        b.run()
+    e1.record(st)
+    b.synchronize()
+    t_step = e0.elapsed_ms(e1) / steps
+    nev, npos = b.counts()
+    # stages one by one (same kernels, launched without the graph)
+    stage = {}
+    for name, mask in (("grid", 1), ("relax", 2), ("df", 4)):
+        e0.record(st)
+        b.run_stages(mask)
+        e1.record(st)
+        b.synchronize()
+        stage[name] = e0.elapsed_ms(e1)
+    natoms = np.array([len(m[1]) for m in mols])
+    npts = int(np.prod(cfg["n"]))
+    res = dict(t_step=t_step, stage=stage, n_eval=nev, n_pos=npos, natoms_total=int(natoms.sum()), npts=npts * nb,
+               tuning=b.tuning, nitems=nb)
+    b.close()
+    return res
+
+
 def run_cpu(args, cfg, xyz, Zs, qs, lvec, grid_host):
     """Reference C++ (or FP64 port) on a bounded sample of the workload."""
     os.environ.setdefault("OMP_PROC_BIND", "close")
@@ -134,52 +179,46 @@ def run_cpu(args, cfg, xyz, Zs, qs, lvec, grid_host):
     kq = np.ascontiguousarray(qs * C.CoulombConst)
     nx, ny, nz = cfg["n"]
     kind = "reference" if O.have_reference() else "port"
-    # grid: a z-slab of the same grid (same spacing), scaled by nz / kz
-    kz = 64
-    n_s = np.array([nx, ny, kz], dtype=np.int32)
-    cell_s = np.array(lvec[1:]) * np.array([1.0, 1.0, kz / nz])[:, None]
-    F = np.zeros((nx, ny, kz, 3))
+    # the whole workload, median of 3 runs per stage
+    reps = 3
+    n_full = np.array([nx, ny, nz], dtype=np.int32)
+    F = np.zeros((nx, ny, nz, 3))
+    FF = np.ascontiguousarray(grid_host[..., :3], dtype=np.float64)  # relax on the GPU's FP64 grid
+    tips = np.ascontiguousarray(M.scan_tips(cfg) - lvec[0])
+    tip = O.tip_params(M.TIP_R0[2], M.K_RAD / -C.eVA_Nm, (0.0, 0.0, 0.0), np.array([M.K_LAT, M.K_LAT, 0.0]) / -C.eVA_Nm)
+    t_grid_r, t_rel_r = [], []
     with quiet_stdout():
         if kind == "reference":
             ref = O.Reference()
-            ref.set_grid(n_s, cell_s, F)
-            t0 = time.perf_counter()
-            ref.lj(Rs0, cLJs)
-            ref.coulomb(Rs0, kq, 0)
-            t_grid_s = time.perf_counter() - t0
-        else:
-            t0 = time.perf_counter()
-            O.lj_ff(n_s, cell_s, Rs0, cLJs, F=F)
-            O.coulomb_ff(n_s, cell_s, Rs0, kq, F=F)
-            t_grid_s = time.perf_counter() - t0
-    t_grid = t_grid_s * nz / kz
-    # relaxation: every k-th pixel column on the full grid (the GPU's FP64 grid)
-    FF = np.ascontiguousarray(grid_host[..., :3], dtype=np.float64)
-    tips = M.scan_tips(cfg) - lvec[0]
+        for _ in range(reps):
+            F[:] = 0
+            if kind == "reference":
+                ref.set_grid(n_full, lvec[1:], F)
+                t0 = time.perf_counter()
+                ref.lj(Rs0, cLJs)
+                ref.coulomb(Rs0, kq, 0)
+                t_grid_r.append(time.perf_counter() - t0)
+                ref.set_grid(n_full, lvec[1:], FF)
+                ref.set_tip(M.TIP_R0[2], M.K_RAD / -C.eVA_Nm, np.zeros(3), np.array([M.K_LAT, M.K_LAT, 0.0]) / -C.eVA_Nm)
+                ref.set_relax(1000, 1e-8, 0.1, 0.1)
+                t0 = time.perf_counter()
+                ref.relax_strokes(tips)
+                t_rel_r.append(time.perf_counter() - t0)
+            else:
+                t0 = time.perf_counter()
+                O.lj_ff(n_full, lvec[1:], Rs0, cLJs, F=F)
+                O.coulomb_ff(n_full, lvec[1:], Rs0, kq, F=F)
+                t_grid_r.append(time.perf_counter() - t0)
+                t0 = time.perf_counter()
+                O.relax_strokes(cfg["n"], lvec[1:], FF, tip, tips, want_iters=False)
+                t_rel_r.append(time.perf_counter() - t0)
+    t_grid = float(np.median(t_grid_r))
+    t_relax = float(np.median(t_rel_r))
     sx, sy = cfg["scan_n"]
-    stride = 2
-    cols = [(i, j) for i in range(0, sx, stride) for j in range(0, sy, stride)]
-    sub = np.ascontiguousarray(np.stack([np.stack([tips[i, j] for j in range(0, sy, stride)]) for i in range(0, sx, stride)]))
-    tip = O.tip_params(M.TIP_R0[2], M.K_RAD / -C.eVA_Nm, (0.0, 0.0, 0.0), np.array([M.K_LAT, M.K_LAT, 0.0]) / -C.eVA_Nm)
-    with quiet_stdout():
-        if kind == "reference":
-            ref.set_grid(np.array(cfg["n"], dtype=np.int32), lvec[1:], FF)
-            ref.set_tip(M.TIP_R0[2], M.K_RAD / -C.eVA_Nm, np.zeros(3), np.array([M.K_LAT, M.K_LAT, 0.0]) / -C.eVA_Nm)
-            ref.set_relax(1000, 1e-8, 0.1, 0.1)
-            t0 = time.perf_counter()
-            ref.relax_strokes(sub)
-            t_rel_s = time.perf_counter() - t0
-        else:
-            t0 = time.perf_counter()
-            O.relax_strokes(cfg["n"], lvec[1:], FF, tip, sub, want_iters=False)
-            t_rel_s = time.perf_counter() - t0
-    frac = len(cols) / (sx * sy)
-    t_relax = t_rel_s / frac
     n_pos = sx * sy * cfg["nz"]
     value = n_pos / (t_grid + t_relax)
-    sample = (f"grid: {kz} of {nz} z-layers ({100 * kz / nz:.2f}%, LJ + Coulomb) scaled by {nz / kz:g}; "
-              f"relaxation: every {stride}th pixel in x and y ({len(cols)} of {sx * sy} columns, {100 * frac:.2f}%, all {cfg['nz']} z-steps) "
-              f"scaled by {1 / frac:g}; CPU {cpu_model()}; measured {t_grid_s:.2f} s + {t_rel_s:.2f} s")
+    sample = (f"whole workload (LJ + Coulomb grid of {nx * ny * nz} points, {n_pos} relaxed positions on the GPU-built FP64 grid), "
+              f"median of {reps} runs per stage: grid {t_grid:.2f} s, relaxation {t_relax:.2f} s; CPU {cpu_model()}, {cores} OpenMP threads")
     return dict(value=value, unit="relaxed tip positions/s", cores=cores, kind=kind, sample=sample, grid_pts_per_s=nx * ny * nz / t_grid, t_grid_s=t_grid, t_relax_s=t_relax)
 
 
@@ -257,6 +296,24 @@ def main():
             "note": "32 algorithmic FLOP per atom-point pair (LJ + Coulomb, force + energy)",
         },
     }
+    if args.batch_items > 0:
+        bt = run_batch(args)
+        pairs = bt["npts"] // bt["nitems"] * bt["natoms_total"]
+        out["batch"] = {
+            "workload": f"{bt['nitems']} seeded molecules (U[8,120] atoms), each a 128x128x160 grid at 0.1 A and a 128x128x30 scan, one captured graph",
+            "positions": bt["n_pos"],
+            "grid_points": bt["npts"],
+            "ms_per_step": bt["t_step"],
+            "positions_per_s": bt["n_pos"] / (bt["t_step"] * 1e-3),
+            "grid_pts_per_s": bt["npts"] / (bt["stage"]["grid"] * 1e-3),
+            "relax_positions_per_s": bt["n_pos"] / (bt["stage"]["relax"] * 1e-3),
+            "stage_ms": bt["stage"],
+            "force_evals": bt["n_eval"],
+            "grid_build_tflops": 32 * pairs / (bt["stage"]["grid"] * 1e-3) / 1e12,
+            "relax_algorithmic_gbs": (8 * 3 * corner_bytes * bt["n_eval"] + 24 * bt["n_pos"]) / (bt["stage"]["relax"] * 1e-3) / 1e9,
+            "tuning": bt["tuning"],
+            "speedup_vs_single_molecule_path": (bt["n_pos"] / bt["t_step"]) / (n_pos / g["t_step"]),
+        }
     if not args.no_cpu:
         grid_host = g["scan"].grid_host()
         c = run_cpu(args, cfg, xyz, Zs, qs, lvec, grid_host)

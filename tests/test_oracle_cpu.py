@@ -270,3 +270,160 @@ def test_planted_sign_error_is_caught():
     # reference golden vectors
     w = O.get_df_weight(1.0, dz=0.2)[::-1]
     assert not np.allclose(w, [-0.35594622, -0.22193265, -0.05447093, 0.05447093, 0.22193265, 0.35594622])
+
+
+# ------------------------------------------------- OpenCL-mode restatements
+#
+# ljc_qzs_ff, relax_tilted and convolve_z restate FF.cl / relax.cl, which
+# cannot run here.  They are pinned against the CPU oracle (itself bit-exact
+# with the compiled reference) through identities that hold between the two
+# semantics, and against closed-form equilibria.
+
+
+def _rot(ax, deg):
+    t = np.deg2rad(deg)
+    c, s = np.cos(t), np.sin(t)
+    if ax == "x":
+        return np.array([[1, 0, 0], [0, c, -s], [0, s, c]])
+    if ax == "y":
+        return np.array([[c, 0, s], [0, 1, 0], [-s, 0, c]])
+    return np.array([[c, -s, 0], [s, c, 0], [0, 0, 1]])
+
+
+def _far_mask(n, lvec, xyz, rmin):
+    idx = np.stack(np.meshgrid(*[np.arange(k) for k in n], indexing="ij"), axis=-1)
+    d = np.array([lvec[1] / n[0], lvec[2] / n[1], lvec[3] / n[2]])
+    P = lvec[0] + idx @ d
+    r = np.linalg.norm(P[..., None, :] - xyz[None, None, None, :, :], axis=-1).min(-1)
+    return r > rmin
+
+
+def _ocl_case():
+    rng = np.random.default_rng(21)
+    xyz = rng.uniform(-1.0, 1.0, (5, 3))
+    q = rng.uniform(-0.3, 0.3, 5)
+    cLJs = np.stack([rng.uniform(1, 3, 5), rng.uniform(50, 150, 5)], axis=1)
+    n = (12, 10, 14)
+    lvec = np.array([[-2.0, -1.8, -2.2], [4.0, 0.0, 0.0], [0.2, 3.6, 0.0], [0.0, 0.1, 4.4]])
+    return xyz, q, cLJs, n, lvec
+
+
+def test_ljc_lj_part_matches_cpu_lj():
+    xyz, q, cLJs, n, lvec = _ocl_case()
+    FE = O.ljc_qzs_ff(n, lvec, np.concatenate([xyz, 0 * q[:, None]], 1), cLJs, np.zeros(4), np.zeros(4))
+    F = np.zeros(n + (3,))
+    E = np.zeros(n)
+    O.lj_ff(n, lvec[1:], xyz - lvec[0], cLJs, F=F, E=E)
+    m = _far_mask(n, lvec, xyz, 1.0)
+    rel = np.linalg.norm(FE[..., :3] - F, axis=-1) / np.maximum(np.linalg.norm(F, axis=-1), 1e-6)
+    assert rel[m].max() < 3e-5  # R2SAFE 1e-6 vs 1e-8 only
+    # energy relative to the attractive term's size (E crosses zero)
+    E6 = np.zeros(n)
+    O.lj_ff(n, lvec[1:], xyz - lvec[0], cLJs * np.array([1.0, 0.0]), F=np.zeros(n + (3,)), E=E6)
+    assert (np.abs(FE[..., 3] - E) / (np.abs(E) + np.abs(E6)))[m].max() < 3e-5
+
+
+def test_ljc_charge_stencil_equals_shifted_cpu_coulomb():
+    xyz, q, cLJs, n, lvec = _ocl_case()
+    Qs, QZs = np.array([-10.0, 20.0, -10.0, 0.0]), np.array([0.1, 0.0, -0.1, 0.0])
+    FE = O.ljc_qzs_ff(n, lvec, np.concatenate([xyz, q[:, None]], 1), 0 * cLJs, Qs, QZs)
+    Fx = np.zeros(n + (3,))
+    Ex = np.zeros(n)
+    for Qk, dz in zip(Qs, QZs):
+        if Qk == 0:
+            continue
+        F = np.zeros(n + (3,))
+        E = np.zeros(n)
+        # the field at p + dz*z from atom R equals the field at p from R - dz*z
+        O.coulomb_ff(n, lvec[1:], xyz - lvec[0] - np.array([0, 0, dz]), q * 14.399645, kind=0, F=F, E=E)
+        Fx += Qk * (14.399644 / 14.399645) * F
+        Ex += Qk * (14.399644 / 14.399645) * E
+    m = _far_mask(n, lvec, xyz, 1.0)
+    scale = np.abs(Fx).max()
+    assert np.abs(FE[..., :3] - Fx)[m].max() < 3e-5 * scale
+    assert np.abs(FE[..., 3] - Ex)[m].max() < 3e-5 * np.abs(Ex).max()
+
+
+def _tilted(FE, dinv, start, nz, R, stiffness, dpos0, surf=0.0, step=-0.1):
+    tipRot = np.zeros((3, 4))
+    tipRot[:, :3] = R
+    tipRot[2, 3] = step
+    poss = np.array([[start[0], start[1], start[2], 0.0]])
+    return O.relax_tilted(FE, dinv, poss, nz, tipRot, np.asarray(stiffness, float), np.asarray(dpos0, float), (0.5, 0.1),
+                          surfFF=surf, max_iters=4000)
+
+
+def test_tilted_surface_force_equilibrium_on_tip_axis():
+    # no grid force, a constant force along the tip axis C: the probe stays on
+    # the axis where the radial spring balances it, |d| = L + surf / krad
+    FE = np.zeros((8, 8, 8, 4))
+    dinv = np.eye(3) * 10.0
+    R = _rot("y", 10.0) @ _rot("z", 25.0)
+    L, krad, klat, surf = 3.0, 30 / -16.0217662, 0.25 / -16.0217662, 0.02
+    tp0 = np.array([0.3, 0.2, 5.0])
+    FEs, paths, nev = _tilted(FE, dinv, tp0, 4, R, (klat, klat, 0.0, krad), (0.0, 0.0, -L, L), surf=surf)
+    C = R[2]
+    for s in range(4):
+        tp = tp0 + s * (-0.1) * C
+        expect = tp - (L + surf / krad) * C
+        assert np.abs(paths[0, s, :3] - expect).max() < 1e-3, s
+    assert nev[0, 0] > 1  # the first step has to move
+
+
+def test_tilted_constant_field_equilibrium_by_energy_gradient():
+    from scipy.optimize import fsolve
+
+    F0 = np.array([0.02, -0.015, 0.03])
+    FE = np.zeros((8, 8, 8, 4))
+    FE[..., :3] = F0
+    dinv = np.eye(3) * 10.0
+    R = _rot("x", 10.0) @ _rot("z", 20.0)
+    L, krad, klat = 3.0, 30 / -16.0217662, 1.0 / -16.0217662
+    k = np.array([klat, klat, 0.0])
+    d0 = np.array([0.1, -0.05, -np.sqrt(L**2 - 0.0125)])
+
+    def energy(dq):  # spring energy in the tip frame (k < 0 restores)
+        return -0.5 * np.sum(k * (dq - d0) ** 2) - 0.5 * krad * (np.linalg.norm(dq) - L) ** 2
+
+    def residual(dq, h=1e-6):
+        g = np.array([(energy(dq + h * e) - energy(dq - h * e)) / (2 * h) for e in np.eye(3)])
+        return R @ F0 - g
+
+    dq = fsolve(residual, d0, xtol=1e-13)
+    tp0 = np.array([0.2, 0.1, 5.0])
+    FEs, paths, nev = _tilted(FE, dinv, tp0, 3, R, (klat, klat, 0.0, krad), (d0[0], d0[1], d0[2], L))
+    for s in range(3):
+        tp = tp0 + s * (-0.1) * R[2]
+        assert np.abs(paths[0, s, :3] - (tp + R.T @ dq)).max() < 3e-3, s
+        # the stored force is the grid force in the tip frame
+        assert np.allclose(FEs[0, s, :3], R @ F0, atol=1e-12)
+
+
+def test_tilted_affine_field_sampling_layout():
+    # springs off, an affine grid force -K (r - r0) on a sheared cell: every
+    # stroke step ends at r0, which pins dinv and the x-fastest layout
+    n = np.array([9, 8, 10])
+    cell = np.array([[1.8, 0.0, 0.0], [0.3, 1.6, 0.0], [0.0, 0.2, 2.0]])
+    dcell = cell / n[:, None]
+    dinv = np.linalg.inv(dcell).T
+    K = np.diag([0.5, 0.4, 0.6])
+    r0 = np.array([0.9, 0.8, 1.0])
+    idx = np.stack(np.meshgrid(*[np.arange(v) for v in n], indexing="ij"), axis=-1)
+    P = idx @ dcell
+    FE = np.zeros(tuple(n) + (4,))
+    FE[..., :3] = -(P - r0) @ K.T
+    R = _rot("y", 10.0)
+    FEs, paths, nev = _tilted(FE, dinv, r0 + np.array([0.1, -0.1, 3.1]), 3, R, (0, 0, 0, 0), (0.0, 0.0, -3.0, 3.0), step=-0.05)
+    assert np.abs(paths[0, :, :3] - r0).max() < 5e-4
+
+
+def test_convolve_z_brute_force():
+    rng = np.random.default_rng(4)
+    F = rng.normal(size=(3, 4, 17))
+    w = rng.normal(size=6)
+    got = O.convolve_z(F, w)
+    ref = np.zeros((3, 4, 12))
+    for j in range(12):
+        for kk in range(6):
+            ref[..., j] += F[..., j + kk] * w[kk]
+    assert np.allclose(got, ref, atol=1e-13)
