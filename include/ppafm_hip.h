@@ -260,6 +260,34 @@ int ppafm_batch_offsets(const ppafm_batch *b, long long *grid_off, long long *po
 void *ppafm_batch_stream(const ppafm_batch *b);
 void ppafm_batch_destroy(ppafm_batch *b);
 
+/* ------------------------------------------------------- learned warm start
+ * No counterpart in the reference.  A 3D-conv network (3 -> 32 -> 32 -> 3
+ * channels, kernels 3x3x3, 3x3x1, 1x1x1, SiLU) maps the RMS-normalised
+ * total-force patch (5x5x3 grid nodes x 3 components, bf16 rows of 232
+ * values: index ((x*5 + y)*3 + z)*3 + c) around a probe's start to its
+ * relaxed displacement (in units of 0.1 A from the patch centre node).
+ * Matrix products on bf16 MFMA with FP32 accumulation; FP32 master weights
+ * (11,971 values: W1[32][81] b1[32] W2[32][288] b2[32] W3[3][32] b3[3]);
+ * Adam.  The model keeps scratch for minibatches of up to max_batch samples. */
+typedef struct ppafm_ws_model ppafm_ws_model;
+ppafm_ws_model *ppafm_ws_create(int max_batch);
+void ppafm_ws_destroy(ppafm_ws_model *m);
+int ppafm_ws_nparams(void);
+int ppafm_ws_set_params(ppafm_ws_model *m, const float *theta_host, int n);
+int ppafm_ws_get_params(const ppafm_ws_model *m, float *theta_host, int n);
+/* out_dev (n, 3) float32 = network(patches_dev (n, 232) bf16). */
+int ppafm_ws_forward(ppafm_ws_model *m, int n, const unsigned short *patches_dev, float *out_dev, void *stream);
+/* Mean squared error over n <= max_batch samples and its gradient (host,
+ * 11,971 values), without updating the model.  Synchronous. */
+int ppafm_ws_loss_grad(ppafm_ws_model *m, int n, const unsigned short *patches_dev, const float *targets_dev,
+                       float *grad_host, float *loss_host);
+/* Adam over consecutive minibatches of `batch` samples (a multiple of 64)
+ * of the n samples (cycling), `steps` minibatches (<= 0: one pass), cosine
+ * learning rate from lr0 to lr0 * lr_end_frac; per-step losses to
+ * losses_host (steps values, may be NULL).  Synchronous. */
+int ppafm_ws_train(ppafm_ws_model *m, long n, const unsigned short *patches_dev, const float *targets_dev, int batch,
+                   long steps, float lr0, float lr_end_frac, float *losses_host, void *stream);
+
 /* Memory helpers so Python callers need no other GPU runtime. */
 int ppafm_malloc(void **ptr, unsigned long long bytes);
 int ppafm_free(void *ptr);

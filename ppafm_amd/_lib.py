@@ -126,6 +126,14 @@ SIGNATURES = [
     ("ppafm_batch_stream", c_void_p, [c_void_p]),
     ("ppafm_batch_destroy", None, [c_void_p]),
     ("ppafm_device_info", c_int, [ctypes.c_char_p, c_int, _ip]),
+    ("ppafm_ws_create", c_void_p, [c_int]),
+    ("ppafm_ws_destroy", None, [c_void_p]),
+    ("ppafm_ws_nparams", c_int, []),
+    ("ppafm_ws_set_params", c_int, [c_void_p, c_void_p, c_int]),
+    ("ppafm_ws_get_params", c_int, [c_void_p, c_void_p, c_int]),
+    ("ppafm_ws_forward", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    ("ppafm_ws_loss_grad", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("ppafm_ws_train", c_int, [c_void_p, c_long, c_void_p, c_void_p, c_int, c_long, ctypes.c_float, ctypes.c_float, c_void_p, c_void_p]),
     ("ppafm_malloc", c_int, [ctypes.POINTER(c_void_p), c_ull]),
     ("ppafm_free", c_int, [c_void_p]),
     ("ppafm_memcpy_h2d", c_int, [c_void_p, c_void_p, c_ull, c_void_p]),

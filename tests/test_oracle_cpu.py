@@ -427,3 +427,49 @@ def test_convolve_z_brute_force():
         for kk in range(6):
             ref[..., j] += F[..., j + kk] * w[kk]
     assert np.allclose(got, ref, atol=1e-13)
+
+
+# ------------------------------------------------------- warm-start network
+
+
+def test_warmstart_backward_matches_finite_differences():
+    import warmstart as W
+
+    rng = np.random.default_rng(3)
+    theta = W.init(1) + rng.normal(0, 0.05, W.NPARAM)
+    P = rng.normal(size=(7, 5, 5, 3, 3))
+    T = rng.normal(size=(7, 3))
+    L, g = W.loss_grad(theta, P, T)
+    idx = rng.choice(W.NPARAM, 60, replace=False)
+    h = 1e-6
+    for i in idx:
+        e = np.zeros(W.NPARAM)
+        e[i] = h
+        fd = (W.loss_grad(theta + e, P, T)[0] - W.loss_grad(theta - e, P, T)[0]) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)), (i, fd, g[i])
+
+
+def test_warmstart_layers_are_valid_convolutions():
+    # conv1 output at (ox, oy) only sees patch cells ox..ox+2, oy..oy+2
+    import warmstart as W
+
+    rng = np.random.default_rng(4)
+    theta = W.init(2)
+    P = rng.normal(size=(1, 5, 5, 3, 3))
+    out, c = W.forward(theta, P)
+    P2 = P.copy()
+    P2[0, 4, 4] += 1.0  # only the (2, 2) window contains (4, 4)
+    _, c2 = W.forward(theta, P2)
+    d = np.abs(c2["Z1"] - c["Z1"]).max(-1)[0]
+    assert (d.reshape(3, 3) > 0).sum() == 1 and d[8] > 0
+
+
+def test_bf16_rounding():
+    import warmstart as W
+
+    x = np.array([1.0, 1.0 + 2**-8, 1.0 + 3 * 2**-9, -2.5, 3.0e-3])
+    r = W.bf16(x)
+    assert r[0] == 1.0 and r[1] == 1.0  # ties to even
+    assert r[2] == 1.0 + 2**-7
+    assert r[3] == -2.5
+    assert abs(r[4] - 3.0e-3) <= 3.0e-3 * 2**-8
