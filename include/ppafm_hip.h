@@ -227,6 +227,7 @@ typedef struct {
 } ppafm_batch_tuning;
 
 typedef struct ppafm_batch ppafm_batch;
+typedef struct ppafm_ws_model ppafm_ws_model;
 
 /* model / rp / tip as for ppafm_ff_build_dev and ppafm_relax (rp->alg must
  * be 1).  df: nw (<= 64) host weights correlated with force channel 2 of
@@ -242,8 +243,9 @@ ppafm_batch *ppafm_batch_create(int nitems, const ppafm_batch_item *items, const
 int ppafm_batch_set_atoms(ppafm_batch *b, const ppafm_atom *atoms);
 /* Enqueue the whole batch (graph launch) on the plan's stream. */
 int ppafm_batch_run(ppafm_batch *b);
-/* Enqueue single stages without the graph: bit 0 grid, bit 1 relaxation,
- * bit 2 df (for timing stages separately). */
+/* Enqueue single stages without the graph: bit 0 grid, bit 1 relaxation
+ * (with the warm-start predictions first when a model is set), bit 2 df,
+ * bit 3 predictions only, bit 4 FIRE only (for timing stages separately). */
 int ppafm_batch_run_stages(ppafm_batch *b, int stages);
 /* Autotuner: time every launch shape of the persistent relaxation on the
  * first nsample molecules (grids built once), keep the fastest, re-capture
@@ -252,24 +254,71 @@ int ppafm_batch_tune(ppafm_batch *b, int nsample, int reps, ppafm_batch_tuning *
 int ppafm_batch_set_tuning(ppafm_batch *b, const ppafm_batch_tuning *tuning);
 int ppafm_batch_get_tuning(const ppafm_batch *b, ppafm_batch_tuning *tuning);
 /* Device buffers: 0 grids, 1 pos, 2 force, 3 iters, 4 df, 5 counters
- * (uint64[2]: force evaluations, positions), 6 packed atoms. */
+ * (uint64[3]: force evaluations, positions, warm-start fallbacks), 6 packed
+ * atoms, 7 step-0 estimates of the network (pixel columns, 3) float32. */
 void *ppafm_batch_buffer(const ppafm_batch *b, int which);
 /* Per-item element offsets (nitems + 1 entries each, NULL to skip): grid
  * points, positions, df values. */
 int ppafm_batch_offsets(const ppafm_batch *b, long long *grid_off, long long *pos_off, long long *df_off);
 void *ppafm_batch_stream(const ppafm_batch *b);
 void ppafm_batch_destroy(ppafm_batch *b);
+/* Warm starts for the batch's relaxation.  No counterpart in the reference:
+ * relaxTipStroke (ProbeParticle.cpp:644-656) starts FIRE at every step from
+ * the continuation (the previous step's result moved with the tip).  With
+ * warm starts each step first runs Newton iterations on the interpolated
+ * field from that same continuation (step 0: from the network's estimate
+ * when a model is given, else from the plain start tip + probe offset):
+ * J dr = -F with the exact Jacobian of the trilinear interpolant plus the
+ * springs', each step at most newton_cap long and taken only where the
+ * Jacobian is stable (all eigenvalues with negative real part, so the
+ * iterates head for a minimum, not a saddle).  A step shorter than
+ * newton_tol ends them: FIRE starts there.  Otherwise (an unstable Jacobian,
+ * a step above the cap, newton_max iterations, newton_max0 at step 0) FIRE
+ * starts from the continuation, as without warm starts, and counters[2]
+ * counts the step.  While FIRE runs, every newton_retry of its iterations a
+ * Newton step is tried from its position; where the Jacobian is stable,
+ * Newton iterations take over as above and FIRE restarts at their zero
+ * (velocity zero), its iteration count running on toward max_iters.  FIRE
+ * ends every step by the plan's own stop rule (conv_f2, max_iters), and the
+ * next step continues from FIRE's result.
+ * Newton evaluations count in counters[0].  With a model, stage bit 3 (8)
+ * computes the step-0 estimates (buffer 7): net_passes network passes, each
+ * on the patch centred at the last estimate, then net_newton Newton steps.
+ * The model must outlive the plan's runs; the graph is re-captured. */
+typedef struct ppafm_warm_params {
+    int enable;        /* 0: plain FIRE, the other fields are ignored */
+    int net_passes;    /* network passes at step 0 (model given), 1..8 */
+    int net_newton;    /* Newton steps after the network, 0..4 */
+    int newton_max;    /* Newton iterations per step > 0, 0..64 (0: plain start) */
+    int newton_max0;   /* Newton iterations at step 0, 0..64 (0: the estimate or plain start as is) */
+    int newton_retry;  /* FIRE iterations between Newton tries from FIRE's position, 0..1000 (0: none) */
+    double newton_tol; /* A: a shorter Newton step means converged */
+    double newton_cap; /* A: longest Newton step */
+} ppafm_warm_params;
+int ppafm_batch_set_warmstart(ppafm_batch *b, const ppafm_warm_params *p, ppafm_ws_model *m);
+/* FIRE from the plan's relaxed positions (buffer 1) to |F|^2 < conv_f2
+ * (at most max_iters steps, state reset per position) into out_dev
+ * (positions, 3) float32; iters_dev optional.  Enqueued on the plan's stream. */
+int ppafm_batch_polish(ppafm_batch *b, double conv_f2, int max_iters, float *out_dev, int *iters_dev);
+/* Training pairs for the warm-start network at nsel selected positions
+ * (global position indices, device int64): patch rows (nsel, 232) bf16 around
+ * the continuation start built from the relaxed positions of the last run,
+ * moved by a pseudo-random offset of up to jitter A laterally (half that
+ * along z; seed selects the offsets), targets (nsel, 3) = (truth - patch
+ * centre node) / 0.1 A.  Enqueued on the plan's stream. */
+int ppafm_batch_ws_samples(ppafm_batch *b, const float *truth_dev, long long nsel, const long long *sel_dev,
+                           unsigned short *patches_dev, float *targets_dev, double jitter, unsigned seed);
 
 /* ------------------------------------------------------- learned warm start
  * No counterpart in the reference.  A 3D-conv network (3 -> 32 -> 32 -> 3
  * channels, kernels 3x3x3, 3x3x1, 1x1x1, SiLU) maps the RMS-normalised
  * total-force patch (5x5x3 grid nodes x 3 components, bf16 rows of 232
  * values: index ((x*5 + y)*3 + z)*3 + c) around a probe's start to its
- * relaxed displacement (in units of 0.1 A from the patch centre node).
+ * relaxed displacement (in units of 0.1 A from the patch centre node); each
+ * force component of the patch is scaled by its own RMS.
  * Matrix products on bf16 MFMA with FP32 accumulation; FP32 master weights
  * (11,971 values: W1[32][81] b1[32] W2[32][288] b2[32] W3[3][32] b3[3]);
  * Adam.  The model keeps scratch for minibatches of up to max_batch samples. */
-typedef struct ppafm_ws_model ppafm_ws_model;
 ppafm_ws_model *ppafm_ws_create(int max_batch);
 void ppafm_ws_destroy(ppafm_ws_model *m);
 int ppafm_ws_nparams(void);

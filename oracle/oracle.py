@@ -79,6 +79,13 @@ def lib():
             ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp, _ip,
         ]
         L.orc_convolve_z.argtypes = [ctypes.c_long, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]
+        L.orc_stable3.argtypes = [_dp]
+        L.orc_stable3.restype = ctypes.c_int
+        L.orc_force_jac.argtypes = [_ip, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp, _dp]
+        L.orc_relax_strokes_warm.argtypes = [ctypes.c_int] * 3 + [_ip, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int,
+                                                                  ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                                                  ctypes.c_double, _dp, _dp, _ip, _ip]
+        L.orc_relax_strokes_warm.restype = ctypes.c_long
         L.orc_num_threads.restype = ctypes.c_int
         _lib = L
     return _lib
@@ -165,6 +172,45 @@ def relax_strokes(n, cell, F, tip, rTips, relax=DEFAULT_RELAX, probeStart=1, rel
         int(probeStart), int(relaxAlg), _ptr(rTips), _ptr(rs), _ptr(fs), _ptr(iters, _ip), None,
     )
     return rs, fs, iters, int(tot)
+
+
+def stable3(J):
+    """1 when every eigenvalue of the 3x3 matrix J has a negative real part."""
+    return bool(lib().orc_stable3(_ptr(_f64(J, (3, 3)))))
+
+
+def force_jac(n, cell, F, tip, rtips, rs):
+    """Total probe force and its Jacobian dF/dr (trilinear field + springs)
+    at positions rs (m, 3) for tips rtips (m, 3): (m, 3), (m, 3, 3)."""
+    rtips = _f64(rtips, (-1, 3))
+    rs = _f64(rs, (-1, 3))
+    f = np.zeros_like(rs)
+    J = np.zeros((len(rs), 3, 3))
+    n = np.ascontiguousarray(n, dtype=np.int32)
+    lib().orc_force_jac(_ptr(n, _ip), _ptr(_f64(cell, (3, 3))), _ptr(_f64(F)), _ptr(_f64(tip)), len(rs), _ptr(rtips),
+                        _ptr(rs), _ptr(f), _ptr(J))
+    return f, J
+
+
+def relax_strokes_warm(n, cell, F, tip, rTips, starts0=None, newton_max=8, newton_max0=12, retry=8, tol=1e-4,
+                       cap=0.1, relax=DEFAULT_RELAX):
+    """Warm-start strokes (Newton from the continuation start, then FIRE
+    with a Newton try every ``retry`` iterations; probeStart 1) on
+    (nx,ny,nz,3) tip positions; starts0 (nx,ny,3) optional step-0 estimates.  Returns rs, fs, iters, nevals (per
+    position) and the number of fallback steps."""
+    rTips = _f64(rTips)
+    nx, ny, nz, _ = rTips.shape
+    rs = np.zeros_like(rTips)
+    fs = np.zeros_like(rTips)
+    iters = np.zeros((nx, ny, nz), dtype=np.int32)
+    nev = np.zeros((nx, ny, nz), dtype=np.int32)
+    n = np.ascontiguousarray(n, dtype=np.int32)
+    st = None if starts0 is None else _f64(starts0, (nx, ny, 3))
+    nfb = lib().orc_relax_strokes_warm(nx, ny, nz, _ptr(n, _ip), _ptr(_f64(cell, (3, 3))), _ptr(_f64(F)), _ptr(_f64(tip)),
+                                       _ptr(_f64(relax)), _ptr(rTips), None if st is None else _ptr(st), int(newton_max),
+                                       int(newton_max0), int(retry), float(tol), float(cap), _ptr(rs), _ptr(fs), _ptr(iters, _ip),
+                                       _ptr(nev, _ip))
+    return rs, fs, iters, nev, int(nfb)
 
 
 def relax_strokes_subset(cols, n, cell, F, tip, rTips, relax=DEFAULT_RELAX, probeStart=1, relaxAlg=1):

@@ -433,6 +433,265 @@ long orc_relax_strokes_subset(int ncols, const int *cols, int nx, int nz, const 
     return total;
 }
 
+/* --------------------------------------------- warm starts (Newton + FIRE)
+ * No counterpart in the reference: the warm-start method of the GPU batch
+ * path.  Every step first runs Newton iterations J dr = -F on the trilinear
+ * field from the continuation start (step 0: from a given estimate or the
+ * plain start), each step cut to at most `cap` and taken only where J is
+ * stable; a full step shorter than `tol` means converged and FIRE
+ * (relax_probe) starts there, otherwise it starts from the continuation
+ * start. */
+
+/* Force (as pp_force) and its Jacobian dF/dr at r: derivatives of the
+ * trilinear weights of the cell holding r (wrap_axis conventions), mapped by
+ * dinv, plus the radial spring's krad[(1 - L/l) I + (L/l^3) d d^T] and the
+ * lateral springs' diag(kspring). */
+static void force_jac(const orc_field *ff, const orc_tip *tip, v3 rtip, v3 r, double f[3], double J[3][3])
+{
+    const double g[3] = {v3_dot(r, (v3){ff->dinv[0], ff->dinv[1], ff->dinv[2]}),
+                         v3_dot(r, (v3){ff->dinv[3], ff->dinv[4], ff->dinv[5]}),
+                         v3_dot(r, (v3){ff->dinv[6], ff->dinv[7], ff->dinv[8]})};
+    int lo[3], hi[3];
+    double wl[3], wh[3];
+    for (int a = 0; a < 3; a++) wrap_axis(g[a], ff->n[a], &lo[a], &hi[a], &wl[a], &wh[a]);
+    const long ny = ff->n[1], nz = ff->n[2];
+    double dFg[3][3] = {{0}};  /* dF_i / dg_a */
+    for (int i = 0; i < 3; i++) f[i] = 0;
+    for (int bx = 0; bx < 2; bx++)
+        for (int by = 0; by < 2; by++)
+            for (int bz = 0; bz < 2; bz++) {
+                const int ix = bx ? hi[0] : lo[0], iy = by ? hi[1] : lo[1], iz = bz ? hi[2] : lo[2];
+                const double *c = ff->F + 3 * (((long)ix * ny + iy) * nz + iz);
+                const double w0 = bx ? wh[0] : wl[0], w1 = by ? wh[1] : wl[1], w2 = bz ? wh[2] : wl[2];
+                const double s0 = bx ? 1 : -1, s1 = by ? 1 : -1, s2 = bz ? 1 : -1;
+                for (int i = 0; i < 3; i++) {
+                    f[i] += c[i] * w0 * w1 * w2;
+                    dFg[i][0] += c[i] * s0 * w1 * w2;
+                    dFg[i][1] += c[i] * w0 * s1 * w2;
+                    dFg[i][2] += c[i] * w0 * w1 * s2;
+                }
+            }
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            J[i][j] = dFg[i][0] * ff->dinv[j] + dFg[i][1] * ff->dinv[3 + j] + dFg[i][2] * ff->dinv[6 + j];
+    const double d[3] = {r.x - rtip.x, r.y - rtip.y, r.z - rtip.z};
+    const double l = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    const double a1 = tip->krad * (1 - tip->lrad / l), a2 = tip->krad * tip->lrad / (l * l * l);
+    for (int i = 0; i < 3; i++) {
+        f[i] += d[i] * a1 + (d[i] - tip->rpp0[i]) * tip->kspring[i];
+        for (int j = 0; j < 3; j++) J[i][j] += a2 * d[i] * d[j] + (i == j ? a1 + tip->kspring[i] : 0);
+    }
+}
+
+/* 1 when every eigenvalue of J has a negative real part (Routh-Hurwitz on
+ * det(lambda I - J) = lambda^3 + c2 lambda^2 + c1 lambda + c0). */
+int orc_stable3(const double *Jf)
+{
+    const double (*J)[3] = (const double (*)[3])Jf;
+    const double tr = J[0][0] + J[1][1] + J[2][2];
+    const double m2 = J[0][0] * J[1][1] - J[0][1] * J[1][0] + J[0][0] * J[2][2] - J[0][2] * J[2][0] +
+                      J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                       J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                       J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+    const double c2 = -tr, c1 = m2, c0 = -det;
+    return c2 > 0 && c0 > 0 && c2 * c1 > c0;
+}
+
+/* Solve J x = b (Gaussian elimination, partial pivoting); 0 if singular. */
+static int solve3(double J[3][3], const double b[3], double x[3])
+{
+    double A[3][4];
+    for (int i = 0; i < 3; i++) {
+        for (int j = 0; j < 3; j++) A[i][j] = J[i][j];
+        A[i][3] = b[i];
+    }
+    for (int c = 0; c < 3; c++) {
+        int p = c;
+        for (int i = c + 1; i < 3; i++)
+            if (fabs(A[i][c]) > fabs(A[p][c])) p = i;
+        if (!(fabs(A[p][c]) > 1e-300)) return 0;
+        for (int j = 0; j < 4; j++) { double t = A[c][j]; A[c][j] = A[p][j]; A[p][j] = t; }
+        for (int i = c + 1; i < 3; i++) {
+            const double m = A[i][c] / A[c][c];
+            for (int j = c; j < 4; j++) A[i][j] -= m * A[c][j];
+        }
+    }
+    for (int i = 2; i >= 0; i--) {
+        double s = A[i][3];
+        for (int j = i + 1; j < 3; j++) s -= A[i][j] * x[j];
+        x[i] = s / A[i][i];
+    }
+    return 1;
+}
+
+/* One Newton iteration from *r, the step cut to at most cap (damped Newton):
+ * returns 1 and moves r when the step is finite; *st2 = squared length of
+ * the full step; *stable = stability of J at the old r. */
+static int newton_iter(const orc_field *ff, const orc_tip *tip, v3 rtip, v3 *r, double cap, int *stable,
+                       double *st2)
+{
+    double f[3], J[3][3], dr[3];
+    force_jac(ff, tip, rtip, *r, f, J);
+    *stable = orc_stable3(&J[0][0]);
+    const double b[3] = {-f[0], -f[1], -f[2]};
+    if (!solve3(J, b, dr)) return 0;
+    *st2 = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
+    if (!(*st2 < 1e6)) return 0;
+    const double sc = *st2 > cap * cap ? cap / sqrt(*st2) : 1.0;
+    r->x += dr[0] * sc; r->y += dr[1] * sc; r->z += dr[2] * sc;
+    return 1;
+}
+
+/* Force and Jacobian at npts points (tests: finite-difference pin). */
+void orc_force_jac(const int n[3], const double cell[9], const double *F, const double tip[8], int npts,
+                   const double *rtips, const double *rs, double *f_out, double *J_out)
+{
+    orc_field ff; orc_tip t;
+    make_field(&ff, n, cell, F); make_tip(&t, tip);
+    for (int i = 0; i < npts; i++) {
+        double J[3][3];
+        force_jac(&ff, &t, (v3){rtips[3 * i], rtips[3 * i + 1], rtips[3 * i + 2]},
+                  (v3){rs[3 * i], rs[3 * i + 1], rs[3 * i + 2]}, f_out + 3 * i, J);
+        for (int a = 0; a < 9; a++) J_out[9 * i + a] = J[a / 3][a % 3];
+    }
+}
+
+/* Newton iterations from *q (at most kmax, the first one already done by
+ * the caller when first_done): 1 when they converge (full step < tol, every
+ * iterate stable), *q then the converged point; counts evaluations. */
+static int newton_run(const orc_field *ff, const orc_tip *tip, v3 rtip, v3 *q, int kmax, double tol, double cap,
+                      int *ne)
+{
+    for (int k = 0; k < kmax; k++) {
+        int stable = 0;
+        double st2 = 0;
+        const int moved = newton_iter(ff, tip, rtip, q, cap, &stable, &st2);
+        (*ne)++;
+        if (moved && stable && st2 < tol * tol) return 1;
+        if (!(moved && stable)) return 0;
+    }
+    return 0;
+}
+
+/* FIRE (relaxProbe, ProbeParticle.cpp:297-313, FIREstate::move 142-174)
+ * from *r with a Newton try every `retry` iterations: one Newton step from
+ * the current position; where it is stable and finite, Newton iterations
+ * continue (newton_max in all, this step included) and on convergence FIRE
+ * restarts at the zero with v = 0 and fresh dt, alpha, its iteration count
+ * running on.  Returns the iteration index at which FIRE stopped. */
+static int fire_retry(const orc_field *ff, const orc_tip *tip, const orc_relax *rp, v3 rtip, v3 *r, int retry,
+                      int newton_max, double tol, double cap, int *ne)
+{
+    v3 v = {0, 0, 0};
+    const double dtmax = rp->dt, a0 = rp->damping;
+    double dt = dtmax, acoef = a0;
+    int it = 0, next_try = retry;
+    while (it < rp->max_iters) {
+        v3 f = pp_force(ff, tip, rtip, *r);
+        (*ne)++;
+        if (retry > 0 && it >= next_try) {
+            next_try = it + retry;
+            v3 q = *r;
+            int stable = 0;
+            double st2 = 0;
+            const int moved = newton_iter(ff, tip, rtip, &q, cap, &stable, &st2);
+            if (moved && stable) {
+                int conv = st2 < tol * tol;
+                if (!conv) conv = newton_run(ff, tip, rtip, &q, newton_max - 1, tol, cap, ne);
+                if (conv) {
+                    *r = q;
+                    v = (v3){0, 0, 0};
+                    dt = dtmax;
+                    acoef = a0;
+                }
+                continue;
+            }
+        }
+        double ff2 = v3_norm2(f), vv = v3_norm2(v), vf = v3_dot(f, v);
+        if (vf < 0) {
+            v.x = 0; v.y = 0; v.z = 0;
+            dt = dt * rp->fdec;
+            acoef = a0;
+        } else {
+            double cf = acoef * sqrt(vv / ff2);
+            double cv = 1 - acoef;
+            v.x *= cv; v.y *= cv; v.z *= cv;
+            v.x += f.x * cf; v.y += f.y * cf; v.z += f.z * cf;
+            dt = fmin(dt * rp->finc, dtmax);
+            acoef = acoef * rp->falpha;
+        }
+        v.x += f.x * dt; v.y += f.y * dt; v.z += f.z * dt;
+        r->x += v.x * dt; r->y += v.y * dt; r->z += v.z * dt;
+        it++;
+        if (ff2 < rp->conv_f2) return it - 1;
+    }
+    return rp->max_iters;
+}
+
+/* One stroke with warm starts (probeStart = 1 continuation, FIRE alg 1).
+ * start0 (optional) = step-0 estimate.  nevals[i] = Newton + FIRE
+ * evaluations of step i (the final sample not included); returns the
+ * number of steps whose opening Newton run fell back to the continuation
+ * start. */
+static int warm_stroke(const orc_field *ff, const orc_tip *tip, const orc_relax *rp, int nstep,
+                       const double *rtips, const double *start0, int newton_max, int newton_max0, int retry,
+                       double tol, double cap, double *rs, double *fs, int *iters, int *nevals)
+{
+    v3 rtip = {rtips[0], rtips[1], rtips[2]};
+    v3 rp_ = {rtip.x + tip->rpp0[0], rtip.y + tip->rpp0[1], rtip.z + tip->rpp0[2]};
+    int nfb = 0;
+    for (int i = 0; i < nstep; i++) {
+        v3 nt = {rtips[3 * i], rtips[3 * i + 1], rtips[3 * i + 2]};
+        v3 drp = {rp_.x - rtip.x, rp_.y - rtip.y, rp_.z - rtip.z};
+        rtip = nt;
+        rp_.x = rtip.x + drp.x; rp_.y = rtip.y + drp.y; rp_.z = rtip.z + drp.z;
+        int ne = 0;
+        const int kmax = i == 0 ? newton_max0 : newton_max;
+        v3 q = rp_;
+        if (i == 0 && start0) q = (v3){start0[0], start0[1], start0[2]};
+        if (kmax == 0) {
+            rp_ = q;
+        } else if (newton_run(ff, tip, rtip, &q, kmax, tol, cap, &ne)) {
+            rp_ = q;
+        } else {
+            nfb++;
+        }
+        const int itr = fire_retry(ff, tip, rp, rtip, &rp_, retry, newton_max, tol, cap, &ne);
+        v3 g = {v3_dot(rp_, (v3){ff->dinv[0], ff->dinv[1], ff->dinv[2]}),
+                v3_dot(rp_, (v3){ff->dinv[3], ff->dinv[4], ff->dinv[5]}),
+                v3_dot(rp_, (v3){ff->dinv[6], ff->dinv[7], ff->dinv[8]})};
+        v3 fsamp = interp_wrap(ff->F, ff->n, g);
+        rs[3 * i] = rp_.x; rs[3 * i + 1] = rp_.y; rs[3 * i + 2] = rp_.z;
+        fs[3 * i] = fsamp.x; fs[3 * i + 1] = fsamp.y; fs[3 * i + 2] = fsamp.z;
+        if (iters) iters[i] = itr;
+        if (nevals) nevals[i] = ne;
+    }
+    return nfb;
+}
+
+/* Warm-start strokes over (nx,ny,nz,3) C-order arrays, column offsets as
+ * orc_relax_strokes; starts0 (optional) = (nx*ny, 3) step-0 estimates in the
+ * same column order.  Returns the total number of fallback steps. */
+long orc_relax_strokes_warm(int nx, int ny, int nz, const int n[3], const double cell[9], const double *F,
+                            const double tip[8], const double relax[7], const double *rtips, const double *starts0,
+                            int newton_max, int newton_max0, int retry, double tol, double cap, double *rs,
+                            double *fs, int *iters, int *nevals)
+{
+    orc_field ff; orc_tip t; orc_relax r;
+    make_field(&ff, n, cell, F); make_tip(&t, tip); make_relax(&r, relax);
+    long total = 0;
+    #pragma omp parallel for collapse(2) schedule(dynamic, 16) reduction(+ : total)
+    for (int ix = 0; ix < nx; ix++)
+        for (int iy = 0; iy < ny; iy++) {
+            const long c = (long)ix + (long)iy * nx, off = c * nz;
+            total += warm_stroke(&ff, &t, &r, nz, rtips + 3 * off, starts0 ? starts0 + 3 * c : 0, newton_max,
+                                 newton_max0, retry, tol, cap, rs + 3 * off, fs + 3 * off, iters ? iters + off : 0,
+                                 nevals ? nevals + off : 0);
+        }
+    return total;
+}
+
 /* ------------------------------------------------------- OpenCL-mode grid */
 
 /* evalLJC_QZs_noPos (FF.cl:366-423) restated in FP64: LJ (getLJ, FF.cl:123-130)

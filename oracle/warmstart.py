@@ -117,6 +117,7 @@ def loss_grad(theta, P, T, bf=False):
 
 
 def normalise_patch(F):
-    """Total-force patch (S, 5, 5, 3, 3) -> RMS-normalised patch and scales."""
-    s = np.sqrt(np.mean(F.reshape(len(F), -1) ** 2, axis=1)) + 1e-30
-    return F / s[:, None, None, None, None], s
+    """Total-force patch (S, 5, 5, 3, 3) -> each component scaled by its RMS
+    over the patch, and the scales (S, 3)."""
+    s = np.sqrt(np.mean(F.reshape(len(F), -1, 3) ** 2, axis=1)) + 1e-30
+    return F / s[:, None, None, None, :], s

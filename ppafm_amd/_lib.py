@@ -84,6 +84,11 @@ class BatchTuning(ctypes.Structure):
         return {"tile_y": self.tile_y, "waves_per_block": self.waves_per_block, "blocks_per_cu": self.blocks_per_cu}
 
 
+class WarmParams(ctypes.Structure):
+    _fields_ = [("enable", c_int), ("net_passes", c_int), ("net_newton", c_int), ("newton_max", c_int),
+                ("newton_max0", c_int), ("newton_retry", c_int), ("newton_tol", c_double), ("newton_cap", c_double)]
+
+
 # (name, restype, argtypes) of every exported symbol declared in the header
 SIGNATURES = [
     ("setGridN", None, [_ip]),
@@ -126,6 +131,9 @@ SIGNATURES = [
     ("ppafm_batch_stream", c_void_p, [c_void_p]),
     ("ppafm_batch_destroy", None, [c_void_p]),
     ("ppafm_device_info", c_int, [ctypes.c_char_p, c_int, _ip]),
+    ("ppafm_batch_set_warmstart", c_int, [c_void_p, c_void_p, c_void_p]),
+    ("ppafm_batch_polish", c_int, [c_void_p, c_double, c_int, c_void_p, c_void_p]),
+    ("ppafm_batch_ws_samples", c_int, [c_void_p, c_void_p, ctypes.c_longlong, c_void_p, c_void_p, c_void_p, c_double, ctypes.c_uint]),
     ("ppafm_ws_create", c_void_p, [c_int]),
     ("ppafm_ws_destroy", None, [c_void_p]),
     ("ppafm_ws_nparams", c_int, []),

@@ -239,9 +239,70 @@ class BatchScan:
         return self.df_raw(k)[:, :, ::-1].copy()
 
     def counts(self):
-        c = np.empty(2, np.uint64)
-        check(lib().ppafm_memcpy_d2h(c.ctypes.data, lib().ppafm_batch_buffer(self._h, BUF_COUNTERS), 16, self.stream), "download")
-        return int(c[0]), int(c[1])
+        """(force evaluations, positions) of the last run."""
+        c = self.counters()
+        return c[0], c[1]
+
+    def counters(self):
+        """(force evaluations, positions, warm-start fallbacks) of the last run."""
+        c = np.empty(3, np.uint64)
+        check(lib().ppafm_memcpy_d2h(c.ctypes.data, lib().ppafm_batch_buffer(self._h, BUF_COUNTERS), 24, self.stream), "download")
+        return int(c[0]), int(c[1]), int(c[2])
+
+    # ------------------------------------------------------ learned warm start
+    def set_warmstart(self, net=None, enable=True, net_passes=1, net_newton=2, newton_max=8, newton_max0=None,
+                      newton_retry=8, newton_tol=1e-4, newton_cap=0.1):
+        """Warm starts (``enable`` False: plain FIRE).  Every step first runs
+        up to ``newton_max`` Newton iterations on the interpolated field from
+        the continuation start (step 0: ``newton_max0`` from the network's
+        estimate when ``net`` is given, after ``net_passes`` passes and
+        ``net_newton`` Newton steps; else from the plain start), each step at
+        most ``newton_cap`` A and only where the force Jacobian is stable; a
+        step below ``newton_tol`` A means converged and FIRE starts there,
+        otherwise FIRE starts from the continuation start (counters()[2]).
+        Every ``newton_retry`` FIRE iterations Newton is tried again from
+        FIRE's position.  FIRE ends every step by the plan's stop rule.
+        ``newton_max0`` defaults to 4 with a network, 12 without."""
+        if newton_max0 is None:
+            newton_max0 = 4 if net is not None else 12
+        p = _lib.WarmParams(int(bool(enable)), int(net_passes), int(net_newton), int(newton_max), int(newton_max0),
+                            int(newton_retry), float(newton_tol), float(newton_cap))
+        self._ws = net if enable else None
+        check(lib().ppafm_batch_set_warmstart(self._h, ctypes.byref(p), net._h if (net is not None and enable) else None),
+              "batch_set_warmstart")
+
+    def set_plain(self):
+        """Plain FIRE from the continuation start at every step."""
+        self.set_warmstart(None, enable=False)
+
+    def polish(self, conv_f2=1e-18, max_iters=4000):
+        """FIRE from the last run's relaxed positions to |F|^2 < conv_f2:
+        device arrays (positions, 3) float32 and the iteration counts."""
+        n = self.total_positions()
+        out = D.DeviceArray((n, 3), np.float32)
+        it = D.DeviceArray((n,), np.int32)
+        check(lib().ppafm_batch_polish(self._h, float(conv_f2), int(max_iters), out.ptr, it.ptr), "batch_polish")
+        return out, it
+
+    def ws_samples(self, truth, sel, jitter=0.0, seed=0):
+        """Training pairs at the global position indices ``sel`` (host int64
+        array): device arrays patches (n, 232) bf16 and targets (n, 3); the
+        patch centre is moved by up to ``jitter`` A (half along z)."""
+        sel = np.ascontiguousarray(sel, dtype=np.int64)
+        if len(sel) and (sel.min() < 0 or sel.max() >= self.total_positions()):
+            raise ValueError("sample index out of range")
+        sd = D.DeviceArray.from_host(sel if len(sel) else np.zeros(1, np.int64))
+        P = D.DeviceArray((max(len(sel), 1), 232), np.uint16)
+        T = D.DeviceArray((max(len(sel), 1), 3), np.float32)
+        check(lib().ppafm_batch_ws_samples(self._h, truth.ptr, len(sel), sd.ptr, P.ptr, T.ptr, float(jitter), int(seed) & 0xFFFFFFFF), "batch_ws_samples")
+        self.synchronize()
+        return P, T
+
+    def predictions(self, k):
+        """Step-0 estimates of the network (nx, ny, 3) of item k (grid frame)."""
+        nx, ny, nz = self._scan_shape(k)
+        col = sum(int(np.prod(self._scan_shape(q)[:2])) for q in range(k) if self._scan_shape(q)[2] > 0)
+        return self._download(7, col * 12, (nx, ny, 3), np.float32)
 
     def total_positions(self):
         return int(self.pos_off[-1])

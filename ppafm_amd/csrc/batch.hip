@@ -90,6 +90,14 @@ struct ppafm_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 
+    // warm starts (ppafm_batch_set_warmstart)
+    ppafm_warm_params warm{};
+    ppafm_ws_model *ws = nullptr;
+    float *pred = nullptr;           // step-0 estimates (pixel columns, 3)
+    long long *d_wprefix = nullptr;  // 64-column groups of items < k
+    long long nwaves = 0;
+    long long *d_pprefix = nullptr;  // positions of items < k
+
     unsigned long long *counters() const { return (unsigned long long *)(ctl + 8); }
 };
 
@@ -101,6 +109,7 @@ void relax_tables(const ppafm_batch &b, int n, int tile_y, std::vector<BatchRela
 {
     it.resize(n);
     prefix.assign(n + 1, 0);
+    long long ncol = 0;
     for (int k = 0; k < n; k++) {
         const ppafm_batch_item &s = b.items[k];
         BatchRelaxItem &r = it[k];
@@ -112,10 +121,13 @@ void relax_tables(const ppafm_batch &b, int n, int tile_y, std::vector<BatchRela
         if (r.nz <= 0) r.tiles_y = 1;
         r.grid_off = b.grid_off[k];
         r.out_off = b.pos_off[k];
+        r.col_off = (int)ncol;
+        if (r.nz > 0) ncol += (long long)r.nx * r.ny;
         for (int a = 0; a < 9; a++) r.dinv[a] = g.dinv[a];
         for (int a = 0; a < 3; a++) {
             r.p0[a] = s.scan.p0[a]; r.da[a] = s.scan.da[a]; r.db[a] = s.scan.db[a]; r.dstep[a] = s.scan.dstep[a];
         }
+        for (int a = 0; a < 9; a++) r.dcell[a] = g.dcell[a];
         prefix[k + 1] = prefix[k] + nt;
     }
 }
@@ -145,7 +157,45 @@ int enqueue_relax(ppafm_batch &b, const ppafm_batch_tuning &t, const BatchRelaxI
     L.tile_y = t.tile_y;
     L.waves_per_block = t.waves_per_block;
     L.nblocks = std::max(1, b.ncu * t.blocks_per_cu);
+    if (b.warm.enable) {
+        L.warm = true;
+        L.pred = b.ws ? b.pred : nullptr;
+        L.newton_max = b.warm.newton_max;
+        L.newton_max0 = b.warm.newton_max0;
+        L.newton_retry = b.warm.newton_retry;
+        L.newton_tol = b.warm.newton_tol;
+        L.newton_cap = b.warm.newton_cap;
+    }
     return launch_relax_batch(L, b.st);
+}
+
+RelaxBatchLaunch base_launch(ppafm_batch &b)
+{
+    RelaxBatchLaunch L;
+    L.rp = b.rp;
+    L.tip = b.tip;
+    L.items = b.d_ritems;
+    L.prefix = b.d_tprefix;
+    L.nitems = b.nitems;
+    L.ntiles = b.ntiles;
+    L.grids = b.grids;
+    L.f64 = b.f64;
+    L.queue = b.ctl;
+    L.pos = b.pos;
+    L.force = b.force;
+    L.iters = b.iters;
+    L.counters = b.counters();
+    L.tile_y = b.tuning.tile_y;
+    L.waves_per_block = b.tuning.waves_per_block;
+    L.nblocks = 1;
+    return L;
+}
+
+int enqueue_predict(ppafm_batch &b)
+{
+    if (!b.warm.enable || !b.ws) return 0;
+    RelaxBatchLaunch L = base_launch(b);
+    return launch_ws_predict(ws_view(b.ws), L, b.d_wprefix, b.nwaves, b.pred, b.warm.net_passes, b.warm.net_newton, b.st);
 }
 
 int enqueue_df(ppafm_batch &b)
@@ -159,18 +209,24 @@ int enqueue_df(ppafm_batch &b)
     return 0;
 }
 
+// stages: 1 grid, 2 relaxation (step-0 estimates first when warm starts
+// with a model are on), 4 df, 8 estimates only, 16 FIRE only.
 int enqueue_stages(ppafm_batch &b, int stages)
 {
     int e = 0;
-    if (stages & 2) {
-        e = launch_fill_u32(b.ctl, 12, 0u, b.st);
+    if (stages & (2 | 16)) {
+        e = launch_fill_u32(b.ctl, 14, 0u, b.st);
         if (e) return e;
     }
     if (stages & 1) {
         e = enqueue_grid(b, b.nitems, b.brick_prefix[b.nitems]);
         if (e) return e;
     }
-    if (stages & 2) {
+    if (stages & (2 | 8)) {
+        e = enqueue_predict(b);
+        if (e) return e;
+    }
+    if (stages & (2 | 16)) {
         e = enqueue_relax(b, b.tuning, b.d_ritems, b.d_tprefix, b.nitems, b.ntiles);
         if (e) return e;
     }
@@ -241,7 +297,8 @@ void free_plan(ppafm_batch *b)
 {
     drop_graph(*b);
     void *bufs[] = {b->grids, b->pos, b->force, b->iters, b->df, b->atoms, b->ctl, b->d_gitems,
-                    b->d_bprefix, b->d_ritems, b->d_tprefix, b->d_sitems, b->d_sprefix};
+                    b->d_bprefix, b->d_ritems, b->d_tprefix, b->d_sitems, b->d_sprefix, b->pred, b->d_wprefix,
+                    b->d_pprefix};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     if (b->own_stream && b->st) (void)hipStreamDestroy(b->st);
@@ -316,6 +373,8 @@ int build_plan(ppafm_batch *b, int nitems, const ppafm_batch_item *items)
     int e = upload(b->d_gitems, b->gitems, b->st);
     if (e) return e;
     e = upload(b->d_bprefix, b->brick_prefix, b->st);
+    if (e) return e;
+    e = upload(b->d_pprefix, b->pos_off, b->st);
     if (e) return e;
     return apply_tuning(*b);
 }
@@ -393,7 +452,7 @@ int ppafm_batch_run(ppafm_batch *b)
 int ppafm_batch_run_stages(ppafm_batch *b, int stages)
 {
     PP_REQUIRE(b, "batch is NULL");
-    PP_REQUIRE(stages >= 0 && stages <= 7, "stages is a mask of bits 0..2");
+    PP_REQUIRE(stages >= 0 && stages <= 31, "stages is a mask of bits 0..4");
     return enqueue_stages(*b, stages);
 }
 
@@ -402,6 +461,8 @@ int ppafm_batch_tune(ppafm_batch *b, int nsample, int reps, ppafm_batch_tuning *
     PP_REQUIRE(b, "batch is NULL");
     PP_REQUIRE(reps >= 1, "reps must be >= 1");
     const int ns = std::max(1, std::min(nsample, b->nitems));
+    const ppafm_warm_params warm_saved = b->warm;  // candidates are timed on plain FIRE
+    b->warm.enable = 0;
     // grids of the sample, once
     int e = launch_fill_u32(b->ctl, 12, 0u, b->st);
     if (e) return e;
@@ -451,6 +512,7 @@ int ppafm_batch_tune(ppafm_batch *b, int nsample, int reps, ppafm_batch_tuning *
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    b->warm = warm_saved;
     if (rc) {
         if (rc == -2 || rc == -1) return rc;
         return -1;
@@ -488,6 +550,7 @@ void *ppafm_batch_buffer(const ppafm_batch *b, int which)
     case 4: return b->df;
     case 5: return b->counters();
     case 6: return b->atoms;
+    case 7: return b->pred;
     default: return nullptr;
     }
 }
@@ -503,6 +566,59 @@ int ppafm_batch_offsets(const ppafm_batch *b, long long *grid_off, long long *po
 }
 
 void *ppafm_batch_stream(const ppafm_batch *b) { return b ? (void *)b->st : nullptr; }
+
+int ppafm_batch_set_warmstart(ppafm_batch *b, const ppafm_warm_params *p, ppafm_ws_model *m)
+{
+    PP_REQUIRE(b && p, "batch and parameters must not be NULL");
+    if (p->enable) {
+        PP_REQUIRE(p->newton_max >= 0 && p->newton_max <= 64, "newton_max must be 0..64");
+        PP_REQUIRE(p->newton_max0 >= 0 && p->newton_max0 <= 64, "newton_max0 must be 0..64");
+        PP_REQUIRE(p->newton_retry >= 0 && p->newton_retry <= 1000, "newton_retry must be 0..1000");
+        PP_REQUIRE(p->newton_tol > 0 && p->newton_tol < 1, "newton_tol must be in (0, 1) A");
+        PP_REQUIRE(p->newton_cap > 0 && p->newton_cap <= 1, "newton_cap must be in (0, 1] A");
+        PP_REQUIRE(!m || (p->net_passes >= 1 && p->net_passes <= 8), "net_passes must be 1..8");
+        PP_REQUIRE(!m || (p->net_newton >= 0 && p->net_newton <= 4), "net_newton must be 0..4");
+    }
+    PP_TRY_HIP(hipStreamSynchronize(b->st));
+    if (p->enable && m && !b->pred) {
+        std::vector<long long> wp(b->nitems + 1, 0);
+        long long ncol = 0;
+        for (int k = 0; k < b->nitems; k++) {
+            const ppafm_batch_item &it = b->items[k];
+            const long long nc = it.scan.nz > 0 ? (long long)it.scan.nx * it.scan.ny : 0;
+            wp[k + 1] = wp[k] + (nc + 63) / 64;
+            ncol += nc;
+        }
+        PP_REQUIRE(ncol < (1LL << 31), "too many pixel columns for warm starts");
+        PP_TRY_HIP(hipMalloc((void **)&b->pred, std::max<long long>(ncol, 1) * 12));
+        b->nwaves = wp[b->nitems];
+        int e = upload(b->d_wprefix, wp, b->st);
+        if (e) return e;
+        PP_TRY_HIP(hipStreamSynchronize(b->st));
+    }
+    b->warm = *p;
+    b->ws = p->enable ? m : nullptr;
+    return capture(*b);
+}
+
+int ppafm_batch_polish(ppafm_batch *b, double conv_f2, int max_iters, float *out_dev, int *iters_dev)
+{
+    PP_REQUIRE(b && out_dev, "batch and output must not be NULL");
+    PP_REQUIRE(max_iters >= 1, "max_iters must be >= 1");
+    RelaxBatchLaunch L = base_launch(*b);
+    L.rp.conv_f2 = conv_f2;
+    L.rp.max_iters = max_iters;
+    return launch_fire_points(L, b->d_pprefix, b->pos_off[b->nitems], b->pos, out_dev, iters_dev, b->st);
+}
+
+int ppafm_batch_ws_samples(ppafm_batch *b, const float *truth_dev, long long nsel, const long long *sel_dev,
+                           unsigned short *patches_dev, float *targets_dev, double jitter, unsigned seed)
+{
+    PP_REQUIRE(b && truth_dev && (nsel == 0 || (sel_dev && patches_dev && targets_dev)), "NULL argument");
+    RelaxBatchLaunch L = base_launch(*b);
+    return launch_ws_samples(L, b->d_pprefix, b->pos, truth_dev, nsel, sel_dev, patches_dev, targets_dev, (float)jitter,
+                             seed, b->st);
+}
 
 void ppafm_batch_destroy(ppafm_batch *b)
 {

@@ -65,11 +65,12 @@ struct BatchRelaxItem {
     int n[3];
     int nx, ny, nz;
     int tiles_y;
-    int pad_;
+    int col_off;         // first pixel column in the batch
     long long grid_off;  // first grid point (element index)
     long long out_off;   // first output position (stroke order, nx*ny*nz block)
     double dinv[9];
     double p0[3], da[3], db[3], dstep[3];
+    double dcell[9];  // grid step vectors (rows)
 };
 
 }  // namespace pp
@@ -122,10 +123,39 @@ struct RelaxBatchLaunch {
     int tile_y;           // pixel tile: (64 / tile_y) x tile_y columns
     int waves_per_block;  // 1..4
     int nblocks;          // persistent workgroups
+    // warm starts (relax_column_warm): on when `warm`; pred = step-0
+    // estimates per pixel column (3 floats) or nullptr
+    bool warm = false;
+    const float *pred = nullptr;
+    int newton_max = 0, newton_max0 = 0, newton_retry = 0;
+    double newton_tol = 0, newton_cap = 0;
 };
 // Tiles of one scan for a given tile_y (also returns tiles along y).
 int relax_batch_tiles(int nx, int ny, int tile_y, int &tiles_y);
 int launch_relax_batch(const RelaxBatchLaunch &L, hipStream_t st);
+// FIRE from the given starts, one independent position each (items, grids,
+// rp and tip of L; pos_prefix[k] = positions of items < k).
+int launch_fire_points(const RelaxBatchLaunch &L, const long long *pos_prefix, long long npos, const float *start,
+                       float *out, int *iters, hipStream_t st);
+
+// Learned warm start (warmstart.hip).
+struct WsNetView {
+    const float *theta;
+    const unsigned short *W1b, *W2b;
+};
+WsNetView ws_view(const ppafm_ws_model *m);
+// Step-0 estimates of every pixel column of a batch: pred (columns, 3),
+// item k's columns from items[k].col_off; wave_prefix[k] = 64-column groups
+// of items < k; reps network passes, then `newton` Newton steps on the
+// interpolated field.
+int launch_ws_predict(const WsNetView &net, const RelaxBatchLaunch &L, const long long *wave_prefix, long long nwaves,
+                      float *pred, int reps, int newton, hipStream_t st);
+// Training pairs at the selected positions of a relaxed batch: patch around
+// the continuation start (from pos) moved by a pseudo-random offset of up to
+// jitter A (half along z), target = truth - patch node (in 0.1 A).
+int launch_ws_samples(const RelaxBatchLaunch &L, const long long *pos_prefix, const float *pos, const float *truth,
+                      long long nsel, const long long *sel, unsigned short *patches, float *targets, float jitter,
+                      unsigned seed, hipStream_t st);
 
 int launch_limit_vec_field(double *F, long npts, double fmax, hipStream_t st);
 int launch_copy(const void *src, void *dst, long n16, hipStream_t st);

@@ -19,239 +19,11 @@
 
 #pragma clang fp contract(off)
 
+#include "relax_dev.hpp"
+
 namespace pp {
 
 namespace {
-
-struct DevArgs {
-    int n0, n1, n2;
-    int nx, ny, nz;
-    int tiles_y, ntiles;
-    int max_iters;
-    int probe_start;
-    double dinv[9];
-    double conv_f2, dtmax, a0, finc, fdec, falpha, dtmin, f2safe;
-    double ks[3], d0[3], krad, lrad, surf_ff;
-    double rot[9];
-    double p0[3], da[3], db[3], dstep[3];
-    const double *tips;
-    void *pos;
-    int pos_f32;
-    float4 *force;
-    int *iters;
-    unsigned long long *counters;
-};
-
-__device__ __forceinline__ double rsq_nr(double x)
-{
-    double y = __builtin_amdgcn_rsq(x);
-#pragma unroll
-    for (int k = 0; k < 2; k++) {
-        const double h = 0.5 * x * y;
-        y = fma(y, fma(-h, y, 0.5), y);
-    }
-    return y;
-}
-
-// grid index coordinates of a position (rows of dinv)
-__device__ __forceinline__ void to_grid(const DevArgs &A, double rx, double ry, double rz, double &gx, double &gy,
-                                        double &gz)
-{
-    gx = fma(rz, A.dinv[2], fma(ry, A.dinv[1], rx * A.dinv[0]));
-    gy = fma(rz, A.dinv[5], fma(ry, A.dinv[4], rx * A.dinv[3]));
-    gz = fma(rz, A.dinv[8], fma(ry, A.dinv[7], rx * A.dinv[6]));
-}
-
-// periodic cell index and weights along one axis (Grid.h:66-69 semantics)
-__device__ __forceinline__ void axis(double r, int n, int &lo, int &hi, double &w)
-{
-    double fl = floor(r);
-    w = r - fl;
-    int i = (int)fl;
-    if ((unsigned)i >= (unsigned)n) {
-        i %= n;
-        if (i < 0) i += n;
-    }
-    lo = i;
-    hi = (i + 1 == n) ? 0 : i + 1;
-}
-
-// trilinear sum of the 8 corners (order: x, y, z bits 2, 1, 0)
-template <typename T4>
-__device__ __forceinline__ void interp8(const T4 v[8], double tx, double ty, double tz, double out[4], bool want_e)
-{
-    const double mx = 1 - tx, my = 1 - ty, mz = 1 - tz;
-    const double w00 = mx * my, w01 = mx * ty, w10 = tx * my, w11 = tx * ty;
-    const double w[8] = {w00 * mz, w00 * tz, w01 * mz, w01 * tz, w10 * mz, w10 * tz, w11 * mz, w11 * tz};
-    out[0] = (double)v[0].x * w[0];
-    out[1] = (double)v[0].y * w[0];
-    out[2] = (double)v[0].z * w[0];
-    out[3] = want_e ? (double)v[0].w * w[0] : 0.0;
-#pragma unroll
-    for (int k = 1; k < 8; k++) {
-        out[0] = fma((double)v[k].x, w[k], out[0]);
-        out[1] = fma((double)v[k].y, w[k], out[1]);
-        out[2] = fma((double)v[k].z, w[k], out[2]);
-        if (want_e) out[3] = fma((double)v[k].w, w[k], out[3]);
-    }
-}
-
-template <typename T4>
-__device__ __forceinline__ void gather(const T4 *__restrict__ g, const DevArgs &A, double gx, double gy, double gz,
-                                       double out[4], bool want_e)
-{
-    int ix, jx, iy, jy, iz, jz;
-    double tx, ty, tz;
-    axis(gx, A.n0, ix, jx, tx);
-    axis(gy, A.n1, iy, jy, ty);
-    axis(gz, A.n2, iz, jz, tz);
-    const unsigned c00 = ((unsigned)ix * A.n1 + iy) * A.n2, c01 = ((unsigned)ix * A.n1 + jy) * A.n2;
-    const unsigned c10 = ((unsigned)jx * A.n1 + iy) * A.n2, c11 = ((unsigned)jx * A.n1 + jy) * A.n2;
-    const T4 v[8] = {g[c00 + iz], g[c00 + jz], g[c01 + iz], g[c01 + jz],
-                     g[c10 + iz], g[c10 + jz], g[c11 + iz], g[c11 + jz]};
-    interp8(v, tx, ty, tz, out, want_e);
-}
-
-// FIRE state of one probe
-struct Fire {
-    double rx, ry, rz, vx, vy, vz, dt, ac;
-};
-
-// Total force (grid force fe + tip springs, ProbeParticle.cpp:282-294 /
-// relax.cl:37-41 in the tip frame) and one FIRE move (ProbeParticle.cpp:
-// 142-174 / relax.cl:105-126).  Returns |F|^2 of the force it moved with.
-template <int MODE>
-__device__ __forceinline__ double fire_move(const DevArgs &A, const double fe[4], double tx, double ty, double tz,
-                                            Fire &S)
-{
-    double fx = fe[0], fy = fe[1], fz = fe[2];
-    const double dx = S.rx - tx, dy = S.ry - ty, dz = S.rz - tz;
-    if (MODE == 0) {
-        const double l2 = fma(dx, dx, fma(dy, dy, dz * dz));
-        const double il = rsq_nr(l2);
-        const double sr = A.krad * fma(l2, il, -A.lrad) * il;
-        fx = fx + fma(dx, sr, (dx - A.d0[0]) * A.ks[0]);
-        fy = fy + fma(dy, sr, (dy - A.d0[1]) * A.ks[1]);
-        fz = fz + fma(dz, sr, (dz - A.d0[2]) * A.ks[2]);
-    } else {
-        const double qx = fma(dz, A.rot[2], fma(dy, A.rot[1], dx * A.rot[0]));
-        const double qy = fma(dz, A.rot[5], fma(dy, A.rot[4], dx * A.rot[3]));
-        const double qz = fma(dz, A.rot[8], fma(dy, A.rot[7], dx * A.rot[6]));
-        const double l2 = fma(qx, qx, fma(qy, qy, qz * qz));
-        const double il = rsq_nr(l2);
-        const double sr = A.krad * fma(l2, il, -A.lrad) * il;
-        const double ax = fma(qx, sr, (qx - A.d0[0]) * A.ks[0]);
-        const double ay = fma(qy, sr, (qy - A.d0[1]) * A.ks[1]);
-        const double az = fma(qz, sr, (qz - A.d0[2]) * A.ks[2]) + A.surf_ff;
-        fx = fx + fma(A.rot[6], az, fma(A.rot[3], ay, A.rot[0] * ax));
-        fy = fy + fma(A.rot[7], az, fma(A.rot[4], ay, A.rot[1] * ax));
-        fz = fz + fma(A.rot[8], az, fma(A.rot[5], ay, A.rot[2] * ax));
-    }
-    const double ff = fma(fx, fx, fma(fy, fy, fz * fz));
-    const double vv = fma(S.vx, S.vx, fma(S.vy, S.vy, S.vz * S.vz));
-    const double vf = fma(S.vx, fx, fma(S.vy, fy, S.vz * fz));
-    if (vf < 0) {
-        S.vx = S.vy = S.vz = 0;
-        S.dt = MODE == 0 ? S.dt * A.fdec : fmax(A.dtmin, S.dt * A.fdec);
-        S.ac = A.a0;
-    } else {
-        const double cf = vv > 0 ? S.ac * (vv * rsq_nr(vv)) * rsq_nr(MODE == 0 ? ff : ff + A.f2safe) : 0.0;
-        const double cv = 1 - S.ac;
-        S.vx = fma(fx, cf, S.vx * cv);
-        S.vy = fma(fy, cf, S.vy * cv);
-        S.vz = fma(fz, cf, S.vz * cv);
-        S.dt = fmin(S.dt * A.finc, A.dtmax);
-        S.ac *= A.falpha;
-    }
-    S.vx = fma(fx, S.dt, S.vx);
-    S.vy = fma(fy, S.dt, S.vy);
-    S.vz = fma(fz, S.dt, S.vz);
-    S.rx = fma(S.vx, S.dt, S.rx);
-    S.ry = fma(S.vy, S.dt, S.ry);
-    S.rz = fma(S.vz, S.dt, S.rz);
-    return ff;
-}
-
-__device__ __forceinline__ void store_pos(const DevArgs &A, long o, double rx, double ry, double rz)
-{
-    if (!A.pos) return;
-    if (A.pos_f32) {
-        float *p = (float *)A.pos + 3 * o;
-        p[0] = (float)rx; p[1] = (float)ry; p[2] = (float)rz;
-    } else {
-        double *p = (double *)A.pos + 3 * o;
-        p[0] = rx; p[1] = ry; p[2] = rz;
-    }
-}
-
-// grid force rotated into the tip frame (relax.cl:446-452), .w = energy
-__device__ __forceinline__ float4 tip_frame_force(const DevArgs &A, const double fe[4])
-{
-    return make_float4((float)fma(fe[2], A.rot[2], fma(fe[1], A.rot[1], fe[0] * A.rot[0])),
-                       (float)fma(fe[2], A.rot[5], fma(fe[1], A.rot[4], fe[0] * A.rot[3])),
-                       (float)fma(fe[2], A.rot[8], fma(fe[1], A.rot[7], fe[0] * A.rot[6])), (float)fe[3]);
-}
-
-// Start of the stroke: tip step 0 and the probe at tip + rotated d0.
-struct Stroke {
-    double bx, by, bz;  // tip position of step 0
-    long col0;
-};
-
-__device__ __forceinline__ Stroke stroke_at(const DevArgs &A, int i, int j)
-{
-    Stroke c;
-    c.bx = A.p0[0] + i * A.da[0] + j * A.db[0];
-    c.by = A.p0[1] + i * A.da[1] + j * A.db[1];
-    c.bz = A.p0[2] + i * A.da[2] + j * A.db[2];
-    c.col0 = ((long)i * A.ny + j) * A.nz;
-    return c;
-}
-
-__device__ __forceinline__ void tip_pos(const DevArgs &A, const Stroke &c, int s, bool live, double &x, double &y,
-                                        double &z)
-{
-    if (A.tips && live) {
-        const double *p = A.tips + 3 * (c.col0 + s);
-        x = p[0]; y = p[1]; z = p[2];
-    } else {
-        x = c.bx + s * A.dstep[0]; y = c.by + s * A.dstep[1]; z = c.bz + s * A.dstep[2];
-    }
-}
-
-template <int MODE>
-__device__ __forceinline__ void probe_offset(const DevArgs &A, double &ox, double &oy, double &oz)
-{
-    if (MODE == 0) {
-        ox = A.d0[0]; oy = A.d0[1]; oz = A.d0[2];
-    } else {
-        ox = fma(A.rot[6], A.d0[2], fma(A.rot[3], A.d0[1], A.rot[0] * A.d0[0]));
-        oy = fma(A.rot[7], A.d0[2], fma(A.rot[4], A.d0[1], A.rot[1] * A.d0[0]));
-        oz = fma(A.rot[8], A.d0[2], fma(A.rot[5], A.d0[1], A.rot[2] * A.d0[0]));
-    }
-}
-
-// Next stroke step: tip moves to step s, the probe follows (probeStart
-// semantics of relaxTipStroke, ProbeParticle.cpp:644-656; mode 1 always
-// continues, relax.cl:434-436) and the FIRE velocity resets.
-template <int MODE>
-__device__ __forceinline__ void next_step(const DevArgs &A, const Stroke &c, int s, bool live, double &tx, double &ty,
-                                          double &tz, Fire &S)
-{
-    double nx_, ny_, nz_;
-    tip_pos(A, c, s, live, nx_, ny_, nz_);
-    if (MODE == 0 && A.probe_start == 0) {
-        S.rx = nx_ + A.d0[0]; S.ry = ny_ + A.d0[1]; S.rz = nz_ + A.d0[2];
-    } else if (MODE == 1 || A.probe_start == 1) {
-        S.rx += nx_ - tx; S.ry += ny_ - ty; S.rz += nz_ - tz;
-    }
-    tx = nx_; ty = ny_; tz = nz_;
-    S.vx = S.vy = S.vz = 0;
-    if (MODE == 0) {
-        S.dt = A.dtmax;
-        S.ac = A.a0;
-    }
-}
 
 // One pixel column (i, j): every z step of the stroke, FIRE to convergence,
 // outputs in stroke order.  Counts force evaluations and finished positions.
@@ -310,18 +82,143 @@ __device__ __forceinline__ void relax_column(const DevArgs &A, const T4 *__restr
     }
 }
 
+// The same column with warm starts.  Every step begins with Newton
+// iterations on the interpolated field (newton_from) from the continuation
+// start the reference uses (step 0: from the column's estimate A.pred, or
+// the plain start without one), each step cut to at most newton_cap.  When
+// they reach a stable zero (a full step shorter than sqrt(newton_tol2) with
+// a stable Jacobian at every iterate, at most newton_max of them, newton_max0
+// at step 0), FIRE runs from there; otherwise (an unstable Jacobian, a
+// non-finite step, no convergence) FIRE runs from the continuation start
+// itself, exactly as without warm starts (counted in nfb).  While FIRE runs,
+// every newton_retry of its iterations one Newton step is tried from its
+// current position with the corners FIRE just loaded; where the Jacobian is
+// stable Newton iterations take over as above, and on convergence FIRE
+// restarts there (velocity zero; mode 0 also resets dt and alpha), its
+// iteration count running on toward max_iters.  Either way FIRE ends the
+// step by the reference's stop rule, and the next step continues from
+// FIRE's result.  Newton iterates and FIRE moves share one 8-corner load per
+// loop pass.  Newton evaluations count in nevals.
+template <typename T4, int MODE>
+__device__ __forceinline__ void relax_column_warm(const DevArgs &A, const T4 *__restrict__ grid, int i, int j,
+                                                  bool live, unsigned long long &nevals, unsigned long long &npos,
+                                                  unsigned long long &nfb)
+{
+    const int nz = A.nz;
+    const Stroke c = stroke_at(A, i, j);
+    double tx = 0, ty = 0, tz = 0;
+    int s = live ? 0 : nz;
+    if (live) tip_pos(A, c, 0, live, tx, ty, tz);
+    double ox, oy, oz;
+    probe_offset<MODE>(A, ox, oy, oz);
+    Fire S{tx + ox, ty + oy, tz + oz, 0, 0, 0, A.dtmax, A.a0};
+    int it = 0, itr = 0, next_try = A.newton_retry;
+    bool sample = false;
+    // Newton phase: iterate q, iterations taken and allowed; `opening` = the
+    // step's first Newton run (its failure is a fallback)
+    double qx = S.rx, qy = S.ry, qz = S.rz;
+    if (A.pred && live) {
+        const float *p = A.pred + 3 * ((long)i * A.ny + j);
+        qx = (double)p[0]; qy = (double)p[1]; qz = (double)p[2];
+    }
+    int nk = 0, kmax = A.newton_max0;
+    bool newton = live && kmax > 0, opening = newton;
+    if (live && !newton && A.pred) { S.rx = qx; S.ry = qy; S.rz = qz; }  // the estimate as it is
+
+    while (s < nz) {
+        double gx, gy, gz, ux, uy, uz;
+        to_grid(A, newton ? qx : S.rx, newton ? qy : S.ry, newton ? qz : S.rz, gx, gy, gz);
+        T4 v[8];
+        corners(grid, A, gx, gy, gz, v, ux, uy, uz);
+        nevals++;
+        bool done = false;
+        const long o = c.col0 + s;
+        const bool retry = !newton && !(MODE == 0 && sample) && A.newton_retry > 0 && it >= next_try;
+        bool fire = !newton;
+        if (newton || retry) {
+            double px = newton ? qx : S.rx, py = newton ? qy : S.ry, pz = newton ? qz : S.rz;
+            bool stable;
+            double st2;
+            const bool ok = newton_from<T4, MODE>(A, v, ux, uy, uz, tx, ty, tz, px, py, pz, A.newton_cap, stable, st2) &&
+                            stable;
+            if (retry) next_try = it + A.newton_retry;
+            if (ok || newton) {
+                fire = false;
+                nk = newton ? nk + 1 : 1;
+                if (!newton) kmax = A.newton_max;
+                qx = px; qy = py; qz = pz;
+                if (ok && st2 < A.newton_tol2) {  // converged: FIRE from here
+                    S.rx = qx; S.ry = qy; S.rz = qz;
+                    S.vx = S.vy = S.vz = 0;
+                    if (MODE == 0) { S.dt = A.dtmax; S.ac = A.a0; }
+                    newton = false;
+                    opening = false;
+                } else if (!ok || nk >= kmax) {  // FIRE from where it was
+                    if (opening) nfb++;
+                    newton = false;
+                    opening = false;
+                } else {
+                    newton = true;
+                }
+            }
+        }
+        if (fire) {
+            double fe[4];
+            interp8(v, ux, uy, uz, fe, MODE == 1);
+            if (MODE == 0 && sample) {
+                store_pos(A, o, S.rx, S.ry, S.rz);
+                if (A.force) A.force[o] = make_float4((float)fe[0], (float)fe[1], (float)fe[2], 0.f);
+                if (A.iters) A.iters[o] = itr;
+                done = true;
+            } else {
+                const double ff = fire_move<MODE>(A, fe, tx, ty, tz, S);
+                it++;
+                const bool conv = ff < A.conv_f2;
+                if (conv || it >= A.max_iters) {
+                    if (MODE == 0) {
+                        itr = conv ? it - 1 : A.max_iters;
+                        sample = true;
+                    } else {
+                        if (A.force) A.force[o] = tip_frame_force(A, fe);
+                        store_pos(A, o, S.rx, S.ry, S.rz);
+                        if (A.iters) A.iters[o] = it;
+                        done = true;
+                    }
+                }
+            }
+        }
+        if (done) {
+            npos++;
+            s++;
+            if (s < nz) {
+                next_step<MODE>(A, c, s, live, tx, ty, tz, S);
+                it = 0;
+                next_try = A.newton_retry;
+                sample = false;
+                qx = S.rx; qy = S.ry; qz = S.rz;
+                nk = 0;
+                kmax = A.newton_max;
+                newton = opening = kmax > 0;
+            }
+        }
+    }
+}
+
 // Wave sum of the per-lane counters, one atomic per wave.
 __device__ __forceinline__ void flush_counters(unsigned long long *counters, unsigned long long nevals,
-                                               unsigned long long npos, int lane)
+                                               unsigned long long npos, int lane, unsigned long long nfb = 0,
+                                               bool with_fb = false)
 {
     if (!counters) return;
     for (int off = 32; off > 0; off >>= 1) {
         nevals += __shfl_xor(nevals, off, 64);
         npos += __shfl_xor(npos, off, 64);
+        nfb += __shfl_xor(nfb, off, 64);
     }
     if (lane == 0) {
         atomicAdd(&counters[0], nevals);
         atomicAdd(&counters[1], npos);
+        if (with_fb) atomicAdd(&counters[2], nfb);
     }
 }
 
@@ -355,6 +252,7 @@ struct BatchArgs {
     float *pos;  // (positions, 3) FP32
     float4 *force;
     int *iters;
+    const float *pred;  // step-0 estimates (pixel columns, 3), WARM kernels only
 };
 
 __device__ __forceinline__ int find_batch_item(const long long *__restrict__ prefix, int nitems, long long x)
@@ -368,7 +266,7 @@ __device__ __forceinline__ int find_batch_item(const long long *__restrict__ pre
     return lo;
 }
 
-template <typename T4, int MODE, int TY>
+template <typename T4, int MODE, int TY, bool WARM>
 __global__ __launch_bounds__(256) void relax_batch_kernel(DevArgs A0, BatchArgs B, const BatchRelaxItem *__restrict__ items,
                                                           const long long *__restrict__ prefix,
                                                           const T4 *__restrict__ grids)
@@ -377,7 +275,7 @@ __global__ __launch_bounds__(256) void relax_batch_kernel(DevArgs A0, BatchArgs 
     const int lane = threadIdx.x & 63;
     const int xcd = blockIdx.x & 7;
     const long long per = (B.ntiles + 7) / 8;
-    unsigned long long nevals = 0, npos = 0;
+    unsigned long long nevals = 0, npos = 0, nfb = 0;
     for (int q = 0; q < 8; q++) {
         const int part = (xcd + q) & 7;
         const long long lo = part * per;
@@ -403,13 +301,15 @@ __global__ __launch_bounds__(256) void relax_batch_kernel(DevArgs A0, BatchArgs 
             A.pos_f32 = 1;
             A.force = B.force ? B.force + it.out_off : nullptr;
             A.iters = B.iters ? B.iters + it.out_off : nullptr;
+            if (WARM) A.pred = B.pred ? B.pred + 3 * (long long)it.col_off : nullptr;
             const int lt = (int)(tile - prefix[m]);
             const int ti = lt / it.tiles_y, tj = lt % it.tiles_y;
             const int i = ti * TX + lane / TY, j = tj * TY + lane % TY;
-            relax_column<T4, MODE>(A, grids + it.grid_off, i, j, i < A.nx && j < A.ny, nevals, npos);
+            if (WARM) relax_column_warm<T4, MODE>(A, grids + it.grid_off, i, j, i < A.nx && j < A.ny, nevals, npos, nfb);
+            else relax_column<T4, MODE>(A, grids + it.grid_off, i, j, i < A.nx && j < A.ny, nevals, npos);
         }
     }
-    flush_counters(A0.counters, nevals, npos, lane);
+    flush_counters(A0.counters, nevals, npos, lane, nfb, WARM);
 }
 
 // ---------------------------------------------------------------------------
@@ -637,17 +537,90 @@ static void fill_shared_args(DevArgs &A, const ppafm_relax_params &rp, const ppa
     A.krad = tip.krad; A.lrad = tip.lrad; A.surf_ff = tip.surf_ff;
 }
 
-template <typename T4, int MODE>
+template <typename T4, int MODE, bool WARM>
 static int launch_batch_ty(int ty, dim3 grd, dim3 blk, hipStream_t st, const DevArgs &A, const BatchArgs &B,
                            const BatchRelaxItem *items, const long long *prefix, const void *grids)
 {
+    const T4 *g = (const T4 *)grids;
     switch (ty) {
-    case 1: hipLaunchKernelGGL((relax_batch_kernel<T4, MODE, 1>), grd, blk, 0, st, A, B, items, prefix, (const T4 *)grids); break;
-    case 2: hipLaunchKernelGGL((relax_batch_kernel<T4, MODE, 2>), grd, blk, 0, st, A, B, items, prefix, (const T4 *)grids); break;
-    case 4: hipLaunchKernelGGL((relax_batch_kernel<T4, MODE, 4>), grd, blk, 0, st, A, B, items, prefix, (const T4 *)grids); break;
-    case 8: hipLaunchKernelGGL((relax_batch_kernel<T4, MODE, 8>), grd, blk, 0, st, A, B, items, prefix, (const T4 *)grids); break;
-    case 16: hipLaunchKernelGGL((relax_batch_kernel<T4, MODE, 16>), grd, blk, 0, st, A, B, items, prefix, (const T4 *)grids); break;
+    case 1: hipLaunchKernelGGL((relax_batch_kernel<T4, MODE, 1, WARM>), grd, blk, 0, st, A, B, items, prefix, g); break;
+    case 2: hipLaunchKernelGGL((relax_batch_kernel<T4, MODE, 2, WARM>), grd, blk, 0, st, A, B, items, prefix, g); break;
+    case 4: hipLaunchKernelGGL((relax_batch_kernel<T4, MODE, 4, WARM>), grd, blk, 0, st, A, B, items, prefix, g); break;
+    case 8: hipLaunchKernelGGL((relax_batch_kernel<T4, MODE, 8, WARM>), grd, blk, 0, st, A, B, items, prefix, g); break;
+    case 16: hipLaunchKernelGGL((relax_batch_kernel<T4, MODE, 16, WARM>), grd, blk, 0, st, A, B, items, prefix, g); break;
     default: set_error("tile_y must be 1, 2, 4, 8 or 16"); return -2;
+    }
+    return 0;
+}
+
+template <typename T4, int MODE>
+static int launch_batch_w(bool warm, int ty, dim3 grd, dim3 blk, hipStream_t st, const DevArgs &A, const BatchArgs &B,
+                          const BatchRelaxItem *items, const long long *prefix, const void *grids)
+{
+    if (warm) return launch_batch_ty<T4, MODE, true>(ty, grd, blk, st, A, B, items, prefix, grids);
+    return launch_batch_ty<T4, MODE, false>(ty, grd, blk, st, A, B, items, prefix, grids);
+}
+
+// ---------------------------------------------------------------------------
+// Independent positions: FIRE from given starts (a state reset per position),
+// used to polish relaxed positions to a tight threshold.  Position g of the
+// batch belongs to the item with pos_prefix[m] <= g < pos_prefix[m + 1]; its
+// tip follows the item's scan.  One thread per position.
+template <typename T4, int MODE>
+__global__ __launch_bounds__(256) void fire_points_kernel(DevArgs A0, const BatchRelaxItem *__restrict__ items,
+                                                          const long long *__restrict__ pos_prefix, int nitems,
+                                                          long long npos, const T4 *__restrict__ grids,
+                                                          const float *__restrict__ start, float *__restrict__ out,
+                                                          int *__restrict__ iters)
+{
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= npos) return;
+    const int m = find_batch_item(pos_prefix, nitems, g);
+    const BatchRelaxItem &it = items[m];
+    DevArgs A = A0;
+    A.n0 = it.n[0]; A.n1 = it.n[1]; A.n2 = it.n[2];
+    A.nx = it.nx; A.ny = it.ny; A.nz = it.nz;
+    for (int k = 0; k < 9; k++) A.dinv[k] = it.dinv[k];
+    const long long o = g - pos_prefix[m];
+    const int s = (int)(o % it.nz);
+    const long long col = o / it.nz;
+    const int i = (int)(col / it.ny), j = (int)(col % it.ny);
+    const double tx = it.p0[0] + i * it.da[0] + j * it.db[0] + s * it.dstep[0];
+    const double ty = it.p0[1] + i * it.da[1] + j * it.db[1] + s * it.dstep[1];
+    const double tz = it.p0[2] + i * it.da[2] + j * it.db[2] + s * it.dstep[2];
+    const T4 *grid = grids + it.grid_off;
+    Fire S{(double)start[3 * g], (double)start[3 * g + 1], (double)start[3 * g + 2], 0, 0, 0, A.dtmax, A.a0};
+    int k = 0;
+    for (; k < A.max_iters; k++) {
+        double fe[4], gx, gy, gz;
+        to_grid(A, S.rx, S.ry, S.rz, gx, gy, gz);
+        gather(grid, A, gx, gy, gz, fe, false);
+        const double ff = fire_move<MODE>(A, fe, tx, ty, tz, S);
+        if (ff < A.conv_f2) break;
+    }
+    out[3 * g] = (float)S.rx;
+    out[3 * g + 1] = (float)S.ry;
+    out[3 * g + 2] = (float)S.rz;
+    if (iters) iters[g] = k;
+}
+
+int launch_fire_points(const RelaxBatchLaunch &L, const long long *pos_prefix, long long npos, const float *start,
+                       float *out, int *iters, hipStream_t st)
+{
+    PP_REQUIRE(L.rp.mode == 0 || L.rp.mode == 1, "relax mode must be 0 or 1");
+    PP_REQUIRE(L.rp.max_iters >= 1, "max_iters must be >= 1");
+    DevArgs A = {};
+    fill_shared_args(A, L.rp, L.tip);
+    if (npos <= 0) return 0;
+    const long long nblk = (npos + 255) / 256;
+    PP_REQUIRE(nblk < (1LL << 31), "too many positions");
+    dim3 grd((unsigned)nblk), blk(256);
+    if (L.f64) {
+        if (L.rp.mode == 0) hipLaunchKernelGGL((fire_points_kernel<double4, 0>), grd, blk, 0, st, A, L.items, pos_prefix, L.nitems, npos, (const double4 *)L.grids, start, out, iters);
+        else hipLaunchKernelGGL((fire_points_kernel<double4, 1>), grd, blk, 0, st, A, L.items, pos_prefix, L.nitems, npos, (const double4 *)L.grids, start, out, iters);
+    } else {
+        if (L.rp.mode == 0) hipLaunchKernelGGL((fire_points_kernel<float4, 0>), grd, blk, 0, st, A, L.items, pos_prefix, L.nitems, npos, (const float4 *)L.grids, start, out, iters);
+        else hipLaunchKernelGGL((fire_points_kernel<float4, 1>), grd, blk, 0, st, A, L.items, pos_prefix, L.nitems, npos, (const float4 *)L.grids, start, out, iters);
     }
     return 0;
 }
@@ -669,15 +642,27 @@ int launch_relax_batch(const RelaxBatchLaunch &L, hipStream_t st)
     DevArgs A = {};
     fill_shared_args(A, L.rp, L.tip);
     A.counters = L.counters;
-    BatchArgs B{L.nitems, L.ntiles, L.queue, L.pos, L.force, L.iters};
+    const bool warm = L.warm;
+    if (warm) {
+        PP_REQUIRE(L.newton_max >= 0 && L.newton_max <= 64 && L.newton_max0 >= 0 && L.newton_max0 <= 64,
+                   "Newton iteration limits must be 0..64");
+        PP_REQUIRE(L.newton_tol > 0 && L.newton_cap > 0, "Newton tolerance and step cap must be positive");
+        PP_REQUIRE(L.newton_retry >= 0, "newton_retry must be >= 0");
+        A.newton_max = L.newton_max;
+        A.newton_max0 = L.newton_max0;
+        A.newton_tol2 = L.newton_tol * L.newton_tol;
+        A.newton_cap = L.newton_cap;
+        A.newton_retry = L.newton_retry;
+    }
+    BatchArgs B{L.nitems, L.ntiles, L.queue, L.pos, L.force, L.iters, warm ? L.pred : nullptr};
     if (L.ntiles == 0) return 0;
     dim3 grd((unsigned)L.nblocks), blk(64 * L.waves_per_block);
     if (L.f64) {
-        if (L.rp.mode == 0) return launch_batch_ty<double4, 0>(L.tile_y, grd, blk, st, A, B, L.items, L.prefix, L.grids);
-        return launch_batch_ty<double4, 1>(L.tile_y, grd, blk, st, A, B, L.items, L.prefix, L.grids);
+        if (L.rp.mode == 0) return launch_batch_w<double4, 0>(warm, L.tile_y, grd, blk, st, A, B, L.items, L.prefix, L.grids);
+        return launch_batch_w<double4, 1>(warm, L.tile_y, grd, blk, st, A, B, L.items, L.prefix, L.grids);
     }
-    if (L.rp.mode == 0) return launch_batch_ty<float4, 0>(L.tile_y, grd, blk, st, A, B, L.items, L.prefix, L.grids);
-    return launch_batch_ty<float4, 1>(L.tile_y, grd, blk, st, A, B, L.items, L.prefix, L.grids);
+    if (L.rp.mode == 0) return launch_batch_w<float4, 0>(warm, L.tile_y, grd, blk, st, A, B, L.items, L.prefix, L.grids);
+    return launch_batch_w<float4, 1>(warm, L.tile_y, grd, blk, st, A, B, L.items, L.prefix, L.grids);
 }
 
 }  // namespace pp

@@ -32,6 +32,7 @@
 #include <vector>
 
 #include "pp_common.hpp"
+#include "relax_dev.hpp"
 
 namespace pp {
 
@@ -46,19 +47,18 @@ constexpr int kOffW1 = 0, kOffB1 = kN1 * kK1, kOffW2 = kOffB1 + kN1, kOffB2 = kO
 constexpr int kOffW3 = kOffB2 + kN2, kOffB3 = kOffW3 + kN3 * kN2, kNParam = kOffB3 + kN3;
 static_assert(kNParam == 11971, "parameter count");
 
-__device__ __forceinline__ unsigned short f2bf(float x)
-{
-    unsigned u = __float_as_uint(x);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (unsigned short)(u >> 16);
-}
+// float -> bf16 bits, round to nearest even (v_cvt_pk_bf16_f32 on gfx950)
+__device__ __forceinline__ unsigned short f2bf(float x) { return __builtin_bit_cast(unsigned short, (__bf16)x); }
 __device__ __forceinline__ float bf2f(unsigned short b) { return __uint_as_float((unsigned)b << 16); }
 __device__ __forceinline__ float bfr(float x) { return bf2f(f2bf(x)); }
 
-__device__ __forceinline__ float silu(float z) { return z / (1.f + __expf(-z)); }
+// sigmoid with the hardware reciprocal (about 1 ulp; the network runs in
+// bf16 anyway)
+__device__ __forceinline__ float sigm(float z) { return __builtin_amdgcn_rcpf(1.f + __expf(-z)); }
+__device__ __forceinline__ float silu(float z) { return z * sigm(z); }
 __device__ __forceinline__ float dsilu(float z)
 {
-    const float s = 1.f / (1.f + __expf(-z));
+    const float s = sigm(z);
     return s * (1.f + z * (1.f - s));
 }
 
@@ -80,16 +80,15 @@ __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c)
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-// offset of conv1 tap k = ((dx*3 + dy)*3 + dz)*3 + c inside a patch row,
-// relative to the window origin; -1 for the padding k >= 81
-__constant__ short kTapOff[kK1p] = {
-#define T3(dx, dy) \
-    (short)(((dx)*5 + (dy)) * 9 + 0), (short)(((dx)*5 + (dy)) * 9 + 1), (short)(((dx)*5 + (dy)) * 9 + 2), \
-    (short)(((dx)*5 + (dy)) * 9 + 3), (short)(((dx)*5 + (dy)) * 9 + 4), (short)(((dx)*5 + (dy)) * 9 + 5), \
-    (short)(((dx)*5 + (dy)) * 9 + 6), (short)(((dx)*5 + (dy)) * 9 + 7), (short)(((dx)*5 + (dy)) * 9 + 8)
-    T3(0, 0), T3(0, 1), T3(0, 2), T3(1, 0), T3(1, 1), T3(1, 2), T3(2, 0), T3(2, 1), T3(2, 2),
-#undef T3
-    -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
+// Offset of conv1 tap k = ((dx*3 + dy)*3 + dz)*3 + c inside a patch row,
+// relative to the window origin; -1 for the padding k >= 81.  Computed, not
+// looked up: the index differs across lanes.
+__device__ __forceinline__ int tap_off(int k)
+{
+    const int blk = k / 9, r9 = k - 9 * blk;  // blk = dx*3 + dy
+    const int dx = blk / 3, dy = blk - 3 * dx;
+    return k < kK1 ? (dx * 5 + dy) * 9 + r9 : -1;
+}
 
 struct Net {
     const float *theta;
@@ -126,7 +125,7 @@ __global__ __launch_bounds__(64) void ws_fwd1_kernel(int S, const unsigned short
             unsigned short v[8];
 #pragma unroll
             for (int e = 0; e < 8; e++) {
-                const int off = kTapOff[k0 + e];
+                const int off = tap_off(k0 + e);
                 v[e] = off >= 0 ? P[(long)s * kPatchStride + poff + off] : (unsigned short)0;
                 if (A1T) A1T[(long)(k0 + e) * (9L * S) + (long)s * 9 + p] = v[e];
             }
@@ -404,6 +403,449 @@ __global__ __launch_bounds__(256) void ws_adam_kernel(float *__restrict__ theta,
     }
 }
 
+
+// ------------------------------------------------------------ use in FIRE
+
+constexpr double kWsScale = 0.1;  // network output unit [A]
+
+__device__ __forceinline__ int wrap_idx(int i, int n)
+{
+    i %= n;
+    return i < 0 ? i + n : i;
+}
+
+__device__ __forceinline__ int search_prefix(const long long *__restrict__ prefix, int nitems, long long x)
+{
+    int lo = 0, hi = nitems;  // prefix[lo] <= x < prefix[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (prefix[mid] <= x) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ void item_args(DevArgs &A, const BatchRelaxItem &it)
+{
+    A.n0 = it.n[0]; A.n1 = it.n[1]; A.n2 = it.n[2];
+    A.nx = it.nx; A.ny = it.ny; A.nz = it.nz;
+    for (int k = 0; k < 9; k++) A.dinv[k] = it.dinv[k];
+}
+
+// Patch of one probe start: total force (grid force + tip springs) at the
+// 5x5x3 grid nodes around the node nearest to (sx, sy, sz), index
+// ((x*5 + y)*3 + z)*3 + c, each component scaled by its own RMS over the
+// patch (which keeps the zero set of every component, and with it the
+// minimum, while the soft lateral components keep bf16 precision next to the
+// stiff one), as bf16 (rounded once before and once after the scaling).
+// The patch feeds bf16 matrix products, so it is computed in FP32 (node
+// offsets from the tip are a few A); the 15 nodes of each x slab are loaded
+// before any is used.  Returns the centre node position (FP64).
+template <typename T4, int MODE>
+__device__ __forceinline__ void ws_patch(const DevArgs &A, const T4 *__restrict__ grid, const double *dcell, double tx,
+                                         double ty, double tz, double sx, double sy, double sz,
+                                         unsigned short *row, double &nx_, double &ny_, double &nz_)
+{
+    double gx, gy, gz;
+    to_grid(A, sx, sy, sz, gx, gy, gz);
+    const int ix = (int)floor(gx + 0.5), iy = (int)floor(gy + 0.5), iz = (int)floor(gz + 0.5);
+    nx_ = ix * dcell[0] + iy * dcell[3] + iz * dcell[6];
+    ny_ = ix * dcell[1] + iy * dcell[4] + iz * dcell[7];
+    nz_ = ix * dcell[2] + iy * dcell[5] + iz * dcell[8];
+    // node offsets from the tip, FP32
+    const float ox = (float)(nx_ - tx), oy = (float)(ny_ - ty), oz = (float)(nz_ - tz);
+    float dc[9];
+    for (int k = 0; k < 9; k++) dc[k] = (float)dcell[k];
+    float R[9], ks[3], d0[3];
+    for (int k = 0; k < 9; k++) R[k] = (float)A.rot[k];
+    for (int k = 0; k < 3; k++) { ks[k] = (float)A.ks[k]; d0[k] = (float)A.d0[k]; }
+    const float krad = (float)A.krad, lrad = (float)A.lrad, surf = MODE == 1 ? (float)A.surf_ff : 0.f;
+    const int jz[3] = {wrap_idx(iz - 1, A.n2), wrap_idx(iz, A.n2), wrap_idx(iz + 1, A.n2)};
+    float sumsq[3] = {0.f, 0.f, 0.f};
+    for (int dx = -2; dx <= 2; dx++) {
+        const int jx = wrap_idx(ix + dx, A.n0);
+        float vx[15], vy[15], vz[15];
+#pragma unroll
+        for (int q = 0; q < 15; q++) {
+            const int dy = q / 3 - 2;
+            const unsigned idx = ((unsigned)jx * A.n1 + wrap_idx(iy + dy, A.n1)) * A.n2 + jz[q % 3];
+            const T4 v = grid[idx];
+            vx[q] = (float)v.x; vy[q] = (float)v.y; vz[q] = (float)v.z;
+        }
+#pragma unroll
+        for (int q = 0; q < 15; q++) {
+            const int dy = q / 3 - 2, dz = q % 3 - 1;
+            // probe - tip at this node, then the springs (as total_force)
+            const float ex = ox + dx * dc[0] + dy * dc[3] + dz * dc[6];
+            const float ey = oy + dx * dc[1] + dy * dc[4] + dz * dc[7];
+            const float ez = oz + dx * dc[2] + dy * dc[5] + dz * dc[8];
+            const float qx = ex * R[0] + ey * R[1] + ez * R[2];
+            const float qy = ex * R[3] + ey * R[4] + ez * R[5];
+            const float qz = ex * R[6] + ey * R[7] + ez * R[8];
+            const float il = rsqrtf(qx * qx + qy * qy + qz * qz);
+            const float sr = krad * (1.f - lrad * il);
+            const float ax = (qx - d0[0]) * ks[0] + qx * sr;
+            const float ay = (qy - d0[1]) * ks[1] + qy * sr;
+            const float az = (qz - d0[2]) * ks[2] + qz * sr + surf;
+            const float f3[3] = {vx[q] + R[0] * ax + R[3] * ay + R[6] * az, vy[q] + R[1] * ax + R[4] * ay + R[7] * az,
+                                 vz[q] + R[2] * ax + R[5] * ay + R[8] * az};
+            const int e = (((dx + 2) * 5 + (dy + 2)) * 3 + (dz + 1)) * 3;
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                sumsq[c] += f3[c] * f3[c];
+                row[e + c] = f2bf(f3[c]);
+            }
+        }
+    }
+    float inv[3];
+    for (int c = 0; c < 3; c++) {
+        const float rms = sqrtf(sumsq[c] * (3.f / kPatch));
+        inv[c] = rms > 0.f ? 1.f / rms : 0.f;
+    }
+    for (int e = 0; e < kPatch; e++) row[e] = f2bf(bf2f(row[e]) * inv[e % 3]);
+    for (int e = kPatch; e < kPatchStride; e++) row[e] = 0;
+}
+
+template <typename T4, int MODE>
+__global__ __launch_bounds__(256) void ws_samples_kernel(DevArgs A0, const BatchRelaxItem *__restrict__ items,
+                                                         const long long *__restrict__ pos_prefix, int nitems,
+                                                         const T4 *__restrict__ grids, const float *__restrict__ pos,
+                                                         const float *__restrict__ truth, long long nsel,
+                                                         const long long *__restrict__ sel,
+                                                         unsigned short *__restrict__ patches,
+                                                         float *__restrict__ targets, float jitter, unsigned seed)
+{
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nsel) return;
+    const long long g = sel[q];
+    const int m = search_prefix(pos_prefix, nitems, g);
+    const BatchRelaxItem &it = items[m];
+    DevArgs A = A0;
+    item_args(A, it);
+    const long long o = g - pos_prefix[m];
+    const int s = (int)(o % it.nz);
+    const long long col = o / it.nz;
+    const int i = (int)(col / it.ny), j = (int)(col % it.ny);
+    const double bx = it.p0[0] + i * it.da[0] + j * it.db[0];
+    const double by = it.p0[1] + i * it.da[1] + j * it.db[1];
+    const double bz = it.p0[2] + i * it.da[2] + j * it.db[2];
+    const double tx = bx + s * it.dstep[0], ty = by + s * it.dstep[1], tz = bz + s * it.dstep[2];
+    double sx, sy, sz;
+    if (s == 0) {
+        double ox, oy, oz;
+        probe_offset<MODE>(A, ox, oy, oz);
+        sx = tx + ox; sy = ty + oy; sz = tz + oz;
+    } else {
+        sx = (double)pos[3 * (g - 1)] + it.dstep[0];
+        sy = (double)pos[3 * (g - 1) + 1] + it.dstep[1];
+        sz = (double)pos[3 * (g - 1) + 2] + it.dstep[2];
+    }
+    if (jitter > 0.f) {
+        // uniform offset in [-jitter, jitter) laterally and half that along z,
+        // a function of the position index and the seed only
+        unsigned h = ((unsigned)g ^ (unsigned)(g >> 32) * 0x85ebca6bu) * 0x9E3779B1u ^ seed;
+        float u[3];
+        for (int k = 0; k < 3; k++) {
+            h ^= h >> 16; h *= 0x7feb352du; h ^= h >> 15; h *= 0x846ca68bu; h ^= h >> 16;
+            u[k] = (float)(h >> 8) * (1.f / 16777216.f) * 2.f - 1.f;
+            h += 0x6a09e667u;
+        }
+        sx += jitter * u[0]; sy += jitter * u[1]; sz += 0.5f * jitter * u[2];
+    }
+    double nx_, ny_, nz_;
+    ws_patch<T4, MODE>(A, grids + it.grid_off, it.dcell, tx, ty, tz, sx, sy, sz, patches + q * kPatchStride, nx_, ny_, nz_);
+    targets[3 * q] = (float)(((double)truth[3 * g] - nx_) / kWsScale);
+    targets[3 * q + 1] = (float)(((double)truth[3 * g + 1] - ny_) / kWsScale);
+    targets[3 * q + 2] = (float)(((double)truth[3 * g + 2] - nz_) / kWsScale);
+}
+
+// Step-0 estimates: one block of 256 threads per 64 consecutive pixel
+// columns of one item (wave_prefix[k] = such groups of items < k); output
+// pred[col_off + column].  Per network pass:
+//   wave 0, lane l     nearest grid node of column l's estimate (first pass:
+//                      the plain start, tip + probe offset);
+//   all 4 waves        the 75 patch nodes of every column, split 4 ways,
+//                      into LDS (bf16), per-component RMS scaling;
+//   wave w             layers 1-2 on the matrix cores for columns 16w..16w+15;
+//   wave 0, lane l     layer 3: the new estimate.
+// Then `newton` Newton steps on the interpolated field (wave 0).
+template <typename T4, int MODE>
+__global__ __launch_bounds__(256) void ws_predict_kernel(DevArgs A0, Net net, const BatchRelaxItem *__restrict__ items,
+                                                         const long long *__restrict__ wave_prefix, int nitems,
+                                                         const T4 *__restrict__ grids, float *__restrict__ pred, int reps,
+                                                         int newton)
+{
+    // patch rows of 226 bf16 (113 dwords, odd: consecutive rows start in
+    // different LDS banks)
+    constexpr int kRow = 226;
+    __shared__ __align__(16) unsigned short Pl[64 * kRow];
+    __shared__ __align__(16) unsigned short H1l[64 * 40];
+    __shared__ __align__(16) unsigned short W1l[kN1 * kK1p];
+    __shared__ __align__(16) unsigned short W2l[kN2 * kK2];
+    __shared__ float W3l[kN3 * kN2];  // bf16-rounded
+    __shared__ float red[4][64][3];
+    __shared__ int nodes[64][3];
+    __shared__ float toff[64][3];  // centre node - tip, FP32
+    float *H2l = (float *)Pl;      // the patches are dead once layer 1 is done
+    static_assert(64 * 33 * 4 <= 64 * kRow * 2, "H2 fits in the patch buffer");
+    const int t = threadIdx.x;
+    const int w = t >> 6, l = t & 63;
+    const long long grp = blockIdx.x;
+    const int m = search_prefix(wave_prefix, nitems, grp);
+    const BatchRelaxItem &it = items[m];
+    DevArgs A = A0;
+    item_args(A, it);
+    const long long ncol = (long long)it.nx * it.ny;
+    const long long lc = (grp - wave_prefix[m]) * 64 + l;
+    const bool live = lc < ncol;
+    const int i = live ? (int)(lc / it.ny) : 0, j = live ? (int)(lc % it.ny) : 0;
+    const double bx = it.p0[0] + i * it.da[0] + j * it.db[0];
+    const double by = it.p0[1] + i * it.da[1] + j * it.db[1];
+    const double bz = it.p0[2] + i * it.da[2] + j * it.db[2];
+    double ox, oy, oz;
+    probe_offset<MODE>(A, ox, oy, oz);
+    const T4 *grid = grids + it.grid_off;
+    const float *W3 = net.theta + kOffW3;
+    float dc[9], R[9], ks[3], d0[3];
+    for (int k = 0; k < 9; k++) { dc[k] = (float)it.dcell[k]; R[k] = (float)A.rot[k]; }
+    for (int k = 0; k < 3; k++) { ks[k] = (float)A.ks[k]; d0[k] = (float)A.d0[k]; }
+    const float krad = (float)A.krad, lrad = (float)A.lrad, surf = MODE == 1 ? (float)A.surf_ff : 0.f;
+    // patch nodes of this thread: q in [q0, q1) of the 75 (x, y, z) nodes
+    const int q0 = (75 * w) / 4, q1 = (75 * (w + 1)) / 4;
+    for (int k = t; k < kN1 * kK1p / 8; k += 256) ((uint4 *)W1l)[k] = ((const uint4 *)net.W1b)[k];
+    for (int k = t; k < kN2 * kK2 / 8; k += 256) ((uint4 *)W2l)[k] = ((const uint4 *)net.W2b)[k];
+    for (int k = t; k < kN3 * kN2; k += 256) W3l[k] = bfr(W3[k]);
+    // node deltas of this thread's patch nodes, packed (dx+2) | (dy+2) << 4 | (dz+1) << 8
+    int nd[19];
+#pragma unroll
+    for (int k = 0; k < 19; k++) {
+        const int q = q0 + k;
+        nd[k] = (q / 15) | (((q / 3) % 5) << 4) | ((q % 3) << 8);
+    }
+    // conv1 tap offsets of this lane's A fragments (k0 = 32q + 8(l/16) + e)
+    short toffs[3][8];
+    for (int q = 0; q < 3; q++)
+        for (int e = 0; e < 8; e++) toffs[q][e] = (short)tap_off(32 * q + 8 * (l >> 4) + e);
+    __syncthreads();
+    // step 0 of every column: the plain start, then the network `reps` times,
+    // each pass on the patch centred at the last estimate
+    double rx = 0, ry = 0, rz = 0;  // wave 0: the estimate
+    double nx_ = 0, ny_ = 0, nz_ = 0;
+    const double tx = bx, ty = by, tz = bz;
+    if (w == 0) { rx = tx + ox; ry = ty + oy; rz = tz + oz; }
+    {
+        for (int rep = 0; rep < reps; rep++) {
+            // ---- centre node of every column (wave 0)
+            if (w == 0) {
+                double gx, gy, gz;
+                to_grid(A, rx, ry, rz, gx, gy, gz);
+                const int ix = (int)floor(gx + 0.5), iy = (int)floor(gy + 0.5), iz = (int)floor(gz + 0.5);
+                nx_ = ix * it.dcell[0] + iy * it.dcell[3] + iz * it.dcell[6];
+                ny_ = ix * it.dcell[1] + iy * it.dcell[4] + iz * it.dcell[7];
+                nz_ = ix * it.dcell[2] + iy * it.dcell[5] + iz * it.dcell[8];
+                nodes[l][0] = wrap_idx(ix, A.n0); nodes[l][1] = wrap_idx(iy, A.n1); nodes[l][2] = wrap_idx(iz, A.n2);
+                toff[l][0] = (float)(nx_ - tx); toff[l][1] = (float)(ny_ - ty); toff[l][2] = (float)(nz_ - tz);
+            }
+            __syncthreads();
+            // ---- patch nodes, split over the 4 waves
+            {
+                const int ix = nodes[l][0], iy = nodes[l][1], iz = nodes[l][2];  // wrapped into the grid
+                const float ex0 = toff[l][0], ey0 = toff[l][1], ez0 = toff[l][2];
+                unsigned short *row = Pl + l * kRow;
+                float ss[3] = {0.f, 0.f, 0.f};
+                // loads in chunks of 7 nodes, each chunk issued before use
+                for (int k0 = 0; k0 < 19; k0 += 7) {
+                    float vx[7], vy[7], vz[7];
+#pragma unroll
+                    for (int k = 0; k < 7; k++) {
+                        const int q = q0 + k0 + k;
+                        if (k0 + k < 19 && q < q1) {
+                            const int dx = (nd[k0 + k] & 15) - 2, dy = ((nd[k0 + k] >> 4) & 15) - 2, dz = (nd[k0 + k] >> 8) - 1;
+                            int jx = ix + dx, jy = iy + dy, jz = iz + dz;  // |d| <= 2 < n: one wrap at most
+                            jx += jx < 0 ? A.n0 : (jx >= A.n0 ? -A.n0 : 0);
+                            jy += jy < 0 ? A.n1 : (jy >= A.n1 ? -A.n1 : 0);
+                            jz += jz < 0 ? A.n2 : (jz >= A.n2 ? -A.n2 : 0);
+                            const T4 *pv = grid + ((unsigned)jx * A.n1 + jy) * A.n2 + jz;
+                            vx[k] = (float)pv->x; vy[k] = (float)pv->y; vz[k] = (float)pv->z;
+                        }
+                    }
+#pragma unroll
+                    for (int k = 0; k < 7; k++) {
+                        const int q = q0 + k0 + k;
+                        if (k0 + k < 19 && q < q1) {
+                            const float dx = (float)((nd[k0 + k] & 15) - 2), dy = (float)(((nd[k0 + k] >> 4) & 15) - 2);
+                            const float dz = (float)((nd[k0 + k] >> 8) - 1);
+                            const float ex = ex0 + dx * dc[0] + dy * dc[3] + dz * dc[6];
+                            const float ey = ey0 + dx * dc[1] + dy * dc[4] + dz * dc[7];
+                            const float ez = ez0 + dx * dc[2] + dy * dc[5] + dz * dc[8];
+                            const float qx = ex * R[0] + ey * R[1] + ez * R[2];
+                            const float qy = ex * R[3] + ey * R[4] + ez * R[5];
+                            const float qz = ex * R[6] + ey * R[7] + ez * R[8];
+                            const float il = rsqrtf(qx * qx + qy * qy + qz * qz);
+                            const float sr = krad * (1.f - lrad * il);
+                            const float ax = (qx - d0[0]) * ks[0] + qx * sr;
+                            const float ay = (qy - d0[1]) * ks[1] + qy * sr;
+                            const float az = (qz - d0[2]) * ks[2] + qz * sr + surf;
+                            const float f3[3] = {vx[k] + R[0] * ax + R[3] * ay + R[6] * az,
+                                                 vy[k] + R[1] * ax + R[4] * ay + R[7] * az,
+                                                 vz[k] + R[2] * ax + R[5] * ay + R[8] * az};
+#pragma unroll
+                            for (int c = 0; c < 3; c++) {
+                                ss[c] += f3[c] * f3[c];
+                                row[3 * q + c] = f2bf(f3[c]);
+                            }
+                        }
+                    }
+                }
+                for (int c = 0; c < 3; c++) red[w][l][c] = ss[c];
+            }
+            __syncthreads();
+            {
+                float inv[3];
+                for (int c = 0; c < 3; c++) {
+                    const float rms = sqrtf((red[0][l][c] + red[1][l][c] + red[2][l][c] + red[3][l][c]) * (3.f / kPatch));
+                    inv[c] = rms > 0.f ? 1.f / rms : 0.f;
+                }
+                unsigned short *row = Pl + l * kRow;
+                if (!live) inv[0] = inv[1] = inv[2] = 0.f;
+                for (int q = q0; q < q1; q++)
+#pragma unroll
+                    for (int c = 0; c < 3; c++) row[3 * q + c] = f2bf(bf2f(row[3 * q + c]) * inv[c]);
+            }
+            __syncthreads();
+            // ---- layers 1 and 2: wave w takes columns 16w .. 16w+15
+            f32x4 acc2[2] = {f32x4{0.f, 0.f, 0.f, 0.f}, f32x4{0.f, 0.f, 0.f, 0.f}};
+            for (int p = 0; p < 9; p++) {
+                const int poff = ((p / 3) * 5 + (p % 3)) * 9;
+                f32x4 acc1[2] = {f32x4{0.f, 0.f, 0.f, 0.f}, f32x4{0.f, 0.f, 0.f, 0.f}};
+                const unsigned short *arow = Pl + (16 * w + (l & 15)) * kRow + poff;
+#pragma unroll
+                for (int q = 0; q < 3; q++) {
+                    const int k0 = 32 * q + 8 * (l >> 4);
+                    unsigned short v[8];
+#pragma unroll
+                    for (int e = 0; e < 8; e++) {
+                        const int off = toffs[q][e];
+                        v[e] = off >= 0 ? arow[off] : (unsigned short)0;
+                    }
+                    const bf16x8 a = pack8(v);
+#pragma unroll
+                    for (int tt = 0; tt < 2; tt++)
+                        acc1[tt] = mfma(a, ld8(W1l + (16 * tt + (l & 15)) * kK1p + k0), acc1[tt]);
+                }
+#pragma unroll
+                for (int tt = 0; tt < 2; tt++) {
+                    const int jj = 16 * tt + (l & 15);
+                    const float bj = net.theta[kOffB1 + jj];
+#pragma unroll
+                    for (int ii = 0; ii < 4; ii++)
+                        H1l[(16 * w + 4 * (l >> 4) + ii) * 40 + jj] = f2bf(silu(acc1[tt][ii] + bj));
+                }
+                __syncthreads();
+                const bf16x8 a2 = ld8(H1l + (16 * w + (l & 15)) * 40 + 8 * (l >> 4));
+#pragma unroll
+                for (int tt = 0; tt < 2; tt++)
+                    acc2[tt] = mfma(a2, ld8(W2l + (16 * tt + (l & 15)) * kK2 + p * 32 + 8 * (l >> 4)), acc2[tt]);
+                __syncthreads();
+            }
+#pragma unroll
+            for (int tt = 0; tt < 2; tt++) {
+                const int jj = 16 * tt + (l & 15);
+                const float bj = net.theta[kOffB2 + jj];
+#pragma unroll
+                for (int ii = 0; ii < 4; ii++) H2l[(16 * w + 4 * (l >> 4) + ii) * 33 + jj] = silu(acc2[tt][ii] + bj);
+            }
+            __syncthreads();
+            // ---- layer 3 and the new estimate (wave 0)
+            if (w == 0) {
+                float o3[kN3];
+#pragma unroll
+                for (int mm = 0; mm < kN3; mm++) {
+                    float a = net.theta[kOffB3 + mm];
+                    for (int jj = 0; jj < kN2; jj++) a = fmaf(bfr(H2l[l * 33 + jj]), W3l[mm * kN2 + jj], a);
+                    o3[mm] = a;
+                }
+                rx = nx_ + kWsScale * (double)o3[0];
+                ry = ny_ + kWsScale * (double)o3[1];
+                rz = nz_ + kWsScale * (double)o3[2];
+            }
+            __syncthreads();
+        }
+        if (w == 0 && live) {
+            // no Newton step from a point of unstable curvature: it would
+            // head for a saddle; FIRE descends from the estimate instead
+            for (int k = 0; k < newton; k++) {
+                const double px = rx, py = ry, pz = rz;
+                bool stable;
+                const bool moved = newton_step<T4, MODE>(A, grid, tx, ty, tz, rx, ry, rz, 0.1, stable);
+                if (!stable) { rx = px; ry = py; rz = pz; break; }
+                if (!moved) break;
+            }
+            float *pp = pred + 3 * ((long long)it.col_off + lc);
+            pp[0] = (float)rx; pp[1] = (float)ry; pp[2] = (float)rz;
+        }
+    }
+}
+
+static void shared_args(DevArgs &A, const ppafm_relax_params &rp, const ppafm_tip &tip)
+{
+    A = DevArgs{};
+    A.max_iters = rp.max_iters;
+    A.probe_start = rp.probe_start;
+    for (int k = 0; k < 9; k++) A.rot[k] = tip.rot[k];
+    A.conv_f2 = rp.conv_f2; A.dtmax = rp.dt; A.a0 = rp.damping;
+    A.finc = rp.finc; A.fdec = rp.fdec; A.falpha = rp.falpha;
+    A.dtmin = rp.dtmin_frac * rp.dt; A.f2safe = rp.f2safe;
+    for (int k = 0; k < 3; k++) { A.ks[k] = tip.kspring[k]; A.d0[k] = tip.d0[k]; }
+    A.krad = tip.krad; A.lrad = tip.lrad; A.surf_ff = tip.surf_ff;
+}
+
+}  // namespace
+
+int launch_ws_predict(const WsNetView &nv, const RelaxBatchLaunch &L, const long long *wave_prefix, long long nwaves,
+                      float *pred, int reps, int newton, hipStream_t st)
+{
+    PP_REQUIRE(reps >= 1 && reps <= 8, "network passes must be 1..8");
+    PP_REQUIRE(newton >= 0 && newton <= 4, "Newton steps must be 0..4");
+    PP_REQUIRE(L.rp.mode == 0 || L.rp.mode == 1, "relax mode must be 0 or 1");
+    PP_REQUIRE(nwaves >= 0 && nwaves < (1LL << 31), "too many columns");
+    if (nwaves == 0) return 0;
+    DevArgs A;
+    shared_args(A, L.rp, L.tip);
+    const Net net{nv.theta, nv.W1b, nv.W2b, nullptr};
+    dim3 grd((unsigned)nwaves), blk(256);
+    if (L.f64) {
+        if (L.rp.mode == 0) hipLaunchKernelGGL((ws_predict_kernel<double4, 0>), grd, blk, 0, st, A, net, L.items, wave_prefix, L.nitems, (const double4 *)L.grids, pred, reps, newton);
+        else hipLaunchKernelGGL((ws_predict_kernel<double4, 1>), grd, blk, 0, st, A, net, L.items, wave_prefix, L.nitems, (const double4 *)L.grids, pred, reps, newton);
+    } else {
+        if (L.rp.mode == 0) hipLaunchKernelGGL((ws_predict_kernel<float4, 0>), grd, blk, 0, st, A, net, L.items, wave_prefix, L.nitems, (const float4 *)L.grids, pred, reps, newton);
+        else hipLaunchKernelGGL((ws_predict_kernel<float4, 1>), grd, blk, 0, st, A, net, L.items, wave_prefix, L.nitems, (const float4 *)L.grids, pred, reps, newton);
+    }
+    return 0;
+}
+
+int launch_ws_samples(const RelaxBatchLaunch &L, const long long *pos_prefix, const float *pos, const float *truth,
+                      long long nsel, const long long *sel, unsigned short *patches, float *targets, float jitter,
+                      unsigned seed, hipStream_t st)
+{
+    PP_REQUIRE(L.rp.mode == 0 || L.rp.mode == 1, "relax mode must be 0 or 1");
+    if (nsel <= 0) return 0;
+    DevArgs A;
+    shared_args(A, L.rp, L.tip);
+    const long long nblk = (nsel + 255) / 256;
+    PP_REQUIRE(nblk < (1LL << 31), "too many samples");
+    dim3 grd((unsigned)nblk), blk(256);
+    if (L.f64) {
+        if (L.rp.mode == 0) hipLaunchKernelGGL((ws_samples_kernel<double4, 0>), grd, blk, 0, st, A, L.items, pos_prefix, L.nitems, (const double4 *)L.grids, pos, truth, nsel, sel, patches, targets, jitter, seed);
+        else hipLaunchKernelGGL((ws_samples_kernel<double4, 1>), grd, blk, 0, st, A, L.items, pos_prefix, L.nitems, (const double4 *)L.grids, pos, truth, nsel, sel, patches, targets, jitter, seed);
+    } else {
+        if (L.rp.mode == 0) hipLaunchKernelGGL((ws_samples_kernel<float4, 0>), grd, blk, 0, st, A, L.items, pos_prefix, L.nitems, (const float4 *)L.grids, pos, truth, nsel, sel, patches, targets, jitter, seed);
+        else hipLaunchKernelGGL((ws_samples_kernel<float4, 1>), grd, blk, 0, st, A, L.items, pos_prefix, L.nitems, (const float4 *)L.grids, pos, truth, nsel, sel, patches, targets, jitter, seed);
+    }
+    return 0;
+}
+
+namespace {
 }  // namespace
 
 // --------------------------------------------------------------- model object
@@ -500,6 +942,10 @@ using namespace pp;
 struct ppafm_ws_model {
     WsModel m;
 };
+
+namespace pp {
+WsNetView ws_view(const ppafm_ws_model *w) { return WsNetView{w->m.theta, w->m.W1b, w->m.W2b}; }
+}  // namespace pp
 
 extern "C" {
 

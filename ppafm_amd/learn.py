@@ -1,0 +1,157 @@
+"""Training and evaluation of the learned warm start on batched scans.
+
+The network (ppafm_amd.warmstart.WarmStartNet) learns, from FIRE runs of the
+method itself, where a probe relaxes at the first step of its stroke, where
+FIRE has no continuation to start from: a batch is relaxed with plain FIRE,
+its positions are polished to a tight threshold on the GPU (the minima), and
+training pairs are cut at step 0 of random pixel columns: the total-force
+patch around the plain start (moved by a small pseudo-random offset), and
+the polished minimum relative to the patch centre.  Training runs on the GPU
+kernels of the network.  With warm starts on (BatchScan.set_warmstart), step
+0 starts Newton iterations from the network's estimate and every later step
+from the continuation start; FIRE then finishes each step to the reference's
+threshold, and falls back to the plain start where Newton does not converge
+to a stable minimum.
+"""
+
+import numpy as np
+
+from . import device as D
+from ._lib import check, lib
+from .warmstart import WarmStartNet
+
+POLISH_CONV_F2 = 1e-18
+POLISH_MAX_ITERS = 4000
+
+MAX_TARGET = 8.0  # 0.8 A in units of 0.1 A: beyond it the pair is a polish outlier
+JITTER = 0.1  # A: training patches are centred up to this far from the plain start
+
+
+def first_steps(batch, items=None):
+    """Global position indices of step 0 of every pixel column of ``items``
+    (default: all)."""
+    ks = range(batch.nitems) if items is None else items
+    out = []
+    for k in ks:
+        nx, ny, nz = batch._scan_shape(k)
+        if nz > 0:
+            out.append(batch.pos_off[k] + np.arange(nx * ny, dtype=np.int64) * nz)
+    return np.concatenate(out) if out else np.zeros(0, np.int64)
+
+
+def training_set(batch, nsamples, seed=0, items=None, conv_f2=POLISH_CONV_F2, max_iters=POLISH_MAX_ITERS,
+                 max_target=MAX_TARGET, jitter=JITTER):
+    """Patches and targets (device arrays, shuffled) at step 0 of the pixel
+    columns of ``items`` (default: all), from the last run of a plain-FIRE
+    batch, and the polished minima of every position (device).
+
+    Each column gives one pair per round, the patch centre moved by up to
+    ``jitter`` A by a hash of (position, round seed); rounds repeat until
+    ``nsamples`` pairs are collected.  Pairs whose target exceeds
+    ``max_target`` (a polish that ended far away) are dropped."""
+    truth, _ = batch.polish(conv_f2=conv_f2, max_iters=max_iters)
+    batch.synchronize()
+    rng = np.random.default_rng(seed)
+    pool = first_steps(batch, items)
+    P_all, T_all, n = [], [], 0
+    r = 0
+    while n < nsamples and len(pool):
+        sel = rng.permutation(pool)[: int(nsamples) - n]
+        P, T = batch.ws_samples(truth, sel, jitter=jitter, seed=seed * 7919 + r)
+        Ph, Th = P.download(), T.download()
+        keep = np.abs(Th).max(axis=1) <= max_target
+        P_all.append(Ph[keep]); T_all.append(Th[keep])
+        n += int(keep.sum())
+        r += 1
+        del P, T
+        if r > 1000:
+            break
+    Ph = np.concatenate(P_all)[: int(nsamples)]
+    Th = np.concatenate(T_all)[: int(nsamples)]
+    perm = rng.permutation(len(Ph))
+    return D.DeviceArray.from_host(Ph[perm]), D.DeviceArray.from_host(Th[perm]), truth
+
+
+def train(net, P, T, epochs=1.0, batch=8192, lr=3e-3, lr_end_frac=0.02):
+    n = int(P.shape[0]) // batch * batch
+    steps = max(1, int(round(epochs * n / batch)))
+    return net.train(P, T, n=n, batch=batch, steps=steps, lr=lr, lr_end_frac=lr_end_frac)
+
+
+def positions_all(batch):
+    """All relaxed positions of the last run, (total, 3) float32."""
+    out = np.empty((batch.total_positions(), 3), np.float32)
+    check(lib().ppafm_memcpy_d2h(out.ctypes.data, lib().ppafm_batch_buffer(batch._h, 1), out.nbytes, batch.stream), "download")
+    return out
+
+
+def _stats(d):
+    return dict(mean=float(d.mean()), p50=float(np.median(d)), p90=float(np.percentile(d, 90)),
+                p99=float(np.percentile(d, 99)), max=float(d.max()))
+
+
+def timed_run(batch, reps=3):
+    """Mean wall time (ms) of ``reps`` graph runs after one warm-up run."""
+    batch.run()
+    e0, e1 = D.Event(), D.Event()
+    e0.record(batch.stream)
+    for _ in range(reps):
+        batch.run()
+    e1.record(batch.stream)
+    batch.synchronize()
+    return e0.elapsed_ms(e1) / reps
+
+
+def evaluate(batch, truth, net=None, enable=True, items=None, reps=0, **warm):
+    """Run the batch with the given warm-start setting (``enable`` False:
+    plain FIRE) and return its counters, the distances of its results to the
+    polished minima ``truth`` (positions of ``items``, default all), the
+    distances of the step-0 estimates to the minima (with a network), and
+    the mean graph time over ``reps`` runs (0: untimed)."""
+    batch.set_warmstart(net, enable=enable, **warm)
+    ms = timed_run(batch, reps) if reps else None
+    batch.run()
+    batch.synchronize()
+    ev, npos, nfb = batch.counters()
+    r = positions_all(batch)
+    rt = truth.download()
+    sl = slice(None)
+    if items is not None:
+        sl = np.concatenate([np.arange(batch.pos_off[k], batch.pos_off[k + 1]) for k in items])
+    out = dict(evals=ev, positions=npos, fallbacks=nfb, vs_min=_stats(np.linalg.norm(r[sl] - rt[sl], axis=1)), ms=ms,
+               positions_array=r)
+    if net is not None and enable:
+        ks = range(batch.nitems) if items is None else items
+        d = []
+        for k in ks:
+            nx, ny, nz = batch._scan_shape(k)
+            if nz == 0:
+                continue
+            e = batch.predictions(k).reshape(-1, 3)
+            t0 = rt[batch.pos_off[k] + np.arange(nx * ny) * nz]
+            d.append(np.linalg.norm(e - t0, axis=1))
+        out["estimate_vs_min"] = _stats(np.concatenate(d))
+    return out
+
+
+def compare(batch, net, truth=None, items=None, reps=0, **warm):
+    """Plain FIRE, warm starts without the network (Newton from the plain
+    start at step 0) and warm starts with it, on the same batch."""
+    batch.set_plain()
+    batch.run()
+    batch.synchronize()
+    if truth is None:
+        truth, _ = batch.polish()
+        batch.synchronize()
+    res = {"plain": evaluate(batch, truth, None, enable=False, items=items, reps=reps),
+           "newton": evaluate(batch, truth, None, items=items, reps=reps, **{k: v for k, v in warm.items() if k != "newton_max0"}),
+           "learned": evaluate(batch, truth, net, items=items, reps=reps, **warm)}
+    rp = res["plain"]["positions_array"]
+    for k in ("newton", "learned"):
+        rw = res[k]["positions_array"]
+        sl = slice(None)
+        if items is not None:
+            sl = np.concatenate([np.arange(batch.pos_off[q], batch.pos_off[q + 1]) for q in items])
+        res[k]["vs_plain"] = _stats(np.linalg.norm(rw[sl] - rp[sl], axis=1))
+    batch.set_plain()
+    return res

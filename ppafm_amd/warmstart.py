@@ -2,8 +2,9 @@
 reference): the network object of libppafm_hip.so (``ppafm_ws_*`` in
 include/ppafm_hip.h).
 
-The network maps the RMS-normalised total-force patch around a probe's start
-position (5x5x3 grid nodes x 3 force components, bf16) to the relaxed probe
+The network maps the total-force patch around a probe's start position (5x5x3
+grid nodes x 3 force components, each component scaled by its RMS over the
+patch, bf16) to the relaxed probe
 position as a displacement from the patch's centre node in units of
 ``SCALE`` = 0.1 A.  Forward, backward and Adam run on the GPU in hand-written
 kernels (bf16 MFMA, FP32 accumulation, FP32 master weights).

@@ -1,0 +1,149 @@
+"""GPU tests of warm-started relaxation on a small batch: the step-0
+network trained on the method's own FIRE runs, Newton from the
+continuation at every later step, FIRE finishing every step.
+
+What is checked:
+  * training lowers the loss;
+  * the GPU warm path equals the FP64 oracle of the same method
+    (oracle/pp_oracle.c orc_relax_strokes_warm, fed the GPU's step-0
+    estimates) within 1e-3 A, with and without the network;
+  * warm results sit at the minima of the interpolated field (tightly
+    polished plain FIRE) while plain FIRE stops within its stop-rule scatter
+    of them, and both end the same number of positions by the stop rule;
+  * what the trained network gains: its step-0 estimates against an
+    untrained network's, and the force evaluations of plain FIRE, warm
+    starts without the network and with it;
+  * runs are deterministic (graph replays give identical positions).
+"""
+
+import numpy as np
+import pytest
+
+import oracle as O
+from cases import small_config, tip_params
+from ppafm_amd import batch as B
+from ppafm_amd import common as C
+from ppafm_amd import learn as L
+from ppafm_amd import molgen as M
+from ppafm_amd.warmstart import WarmStartNet
+
+pytestmark = pytest.mark.gpu
+
+NITEMS, NTRAIN = 12, 10
+
+
+@pytest.fixture(scope="module")
+def setup():
+    print("case: 12 molecules (64x64x80 grids, 32x32x20 scans), train on step 0 of 10, evaluate on all", flush=True)
+    cases = []
+    for k in range(NITEMS):
+        cfg, _ = small_config(n=(64, 64, 80), scan_n=(32, 32), nz=20, natoms=8 + 7 * k, seed=100 + k)
+        cases.append((cfg, M.config_lvec(cfg), M.molecule(cfg["natoms"], 100 + k)))
+    b = B.BatchScan.plain([c[1] for c in cases], [c[0]["n"] for c in cases], [c[0] for c in cases], tipR0=M.TIP_R0,
+                          k_lat=M.K_LAT, k_rad=M.K_RAD, iZPP=M.PROBE_Z, probe_q=M.PROBE_Q, amplitude=M.DF_AMPLITUDE,
+                          molecules=[c[2] for c in cases], keep_iters=True)
+    b.run()
+    b.synchronize()
+    P, T, truth = L.training_set(b, 60_000, seed=1, items=range(NTRAIN))
+    net = WarmStartNet(max_batch=2048, seed=3)
+    losses = L.train(net, P, T, epochs=20, batch=2048, lr=3e-3)
+    yield cases, b, net, losses, truth
+    b.set_plain()
+
+
+def _oracle_field(case):
+    cfg, lvec, (xyz, Zs, qs) = case
+    cLJs = C.getAtomsLJ(M.PROBE_Z, Zs, C.loadSpecies())
+    Rs0 = xyz - lvec[0]
+    FL = O.limit_vec_field(O.lj_ff(cfg["n"], lvec[1:], Rs0, cLJs), 10.0)
+    FE = O.limit_vec_field(O.coulomb_ff(cfg["n"], lvec[1:], Rs0, qs * C.CoulombConst), 10.0)
+    return FL + M.PROBE_Q * FE, M.scan_tips(cfg) - lvec[0]
+
+
+def test_training_lowers_loss(setup):
+    cases, b, net, losses, truth = setup
+    print(f"loss: first {losses[:10].mean():.4f} last {losses[-10:].mean():.5f} ({len(losses)} steps)")
+    assert np.isfinite(losses).all()
+    assert losses[-10:].mean() < 0.1 * losses[:10].mean()
+
+
+@pytest.mark.parametrize("with_net", [True, False])
+def test_warm_path_matches_fp64_oracle(setup, with_net):
+    cases, b, net, losses, truth = setup
+    for k in (0, NITEMS - 1):  # a training item and the held-out one
+        print(f"case: item {k}, network {with_net}", flush=True)
+        FF, tips = _oracle_field(cases[k])
+        b.set_warmstart(net if with_net else None)
+        b.run()
+        b.synchronize()
+        rg = b.positions_raw(k).astype(np.float64)
+        starts0 = b.predictions(k).astype(np.float64) if with_net else None
+        rw, _, _, _, nfb = O.relax_strokes_warm(cases[k][0]["n"], cases[k][1][1:], FF, tip_params(), tips, starts0=starts0,
+                                                newton_max0=4 if with_net else 12)
+        d = np.linalg.norm(rg - rw, axis=-1)
+        print(f"item {k}: GPU warm vs FP64 oracle max {d.max():.2e} p99 {np.percentile(d, 99):.2e}; oracle fallbacks {nfb}")
+        assert d.max() < 1e-3
+    b.set_plain()
+
+
+def test_warm_results_are_minima(setup):
+    cases, b, net, losses, truth = setup
+    rt = truth.download()
+    b.set_plain()
+    b.run()
+    b.synchronize()
+    rp = L.positions_all(b)
+    it_p = np.concatenate([b.iterations(k).ravel() for k in range(NITEMS)])
+    b.set_warmstart(net)
+    b.run()
+    b.synchronize()
+    rw = L.positions_all(b)
+    it_w = np.concatenate([b.iterations(k).ravel() for k in range(NITEMS)])
+    b.set_plain()
+    dp = np.linalg.norm(rp - rt, axis=1)
+    dw = np.linalg.norm(rw - rt, axis=1)
+    print(f"to the polished minima: plain p50 {np.median(dp):.2e} p99 {np.percentile(dp, 99):.2e}; "
+          f"warm p50 {np.median(dw):.2e} p99 {np.percentile(dw, 99):.2e} max {dw.max():.2e}")
+    print(f"stopped by the threshold: warm {np.mean(it_w < 1000):.6f} plain {np.mean(it_p < 1000):.6f}")
+    assert np.mean(it_w < 1000) >= np.mean(it_p < 1000)
+    assert np.percentile(dw, 99) < 1e-3 and np.median(dw) < 1e-5
+    assert np.percentile(dw, 99) < np.percentile(dp, 99)
+    # where warm and plain differ, the difference is plain FIRE's stop scatter
+    m = np.linalg.norm(rw - rp, axis=1) > 0.02
+    print(f"|warm - plain| > 0.02 A at {m.mean():.4f} of the positions")
+    assert m.mean() < 0.01
+
+
+def test_network_gain(setup):
+    cases, b, net, losses, truth = setup
+    r = L.compare(b, net, truth=truth)
+    untrained = WarmStartNet(max_batch=2048, seed=3)
+    ru = L.evaluate(b, truth, untrained)
+    b.set_plain()
+    for k in ("plain", "newton", "learned"):
+        print(k, {q: v for q, v in r[k].items() if q != "positions_array"})
+    print("untrained network: estimate_vs_min", ru["estimate_vs_min"], "evals", ru["evals"])
+    e_p, e_n, e_l = r["plain"]["evals"], r["newton"]["evals"], r["learned"]["evals"]
+    print(f"force evaluations per position: plain {e_p / r['plain']['positions']:.1f}, Newton-only warm start "
+          f"{e_n / r['plain']['positions']:.2f}, learned {e_l / r['plain']['positions']:.2f}")
+    assert e_l < 0.25 * e_p
+    # the trained network's step-0 estimates beat an untrained one's and save
+    # Newton iterations at step 0
+    assert r["learned"]["estimate_vs_min"]["p50"] < 0.3 * ru["estimate_vs_min"]["p50"]
+    assert e_l < e_n
+
+
+def test_warm_runs_are_deterministic(setup):
+    cases, b, net, losses, truth = setup
+    b.set_warmstart(net)
+    b.run()
+    b.synchronize()
+    w1 = L.positions_all(b)
+    c1 = b.counters()
+    b.run()
+    b.synchronize()
+    w2 = L.positions_all(b)
+    c2 = b.counters()
+    b.set_plain()
+    assert np.array_equal(w1, w2)
+    assert c1 == c2 and c1[1] == w1.shape[0]

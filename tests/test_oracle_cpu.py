@@ -473,3 +473,131 @@ def test_bf16_rounding():
     assert r[2] == 1.0 + 2**-7
     assert r[3] == -2.5
     assert abs(r[4] - 3.0e-3) <= 3.0e-3 * 2**-8
+
+
+# ------------------------------------------------- warm starts (Newton + FIRE)
+
+
+def _bump_field(n=(32, 32, 40), h=0.25, A=0.2, sx=1.0, sz=2.0, c=(4.0, 4.0, 2.0)):
+    """F = -grad E on grid nodes for a repulsive Gaussian bump E =
+    A exp(-((x-cx)^2 + (y-cy)^2)/sx^2 - (z-cz)^2/sz^2): on its axis the lateral
+    curvature is -2A/sx^2 (times the z factor), far beyond the lateral spring,
+    so the on-axis equilibrium of a probe above it is a saddle."""
+    g = np.stack(np.meshgrid(*[np.arange(k) * h for k in n], indexing="ij"), -1)
+    d = g - np.array(c)
+    E = A * np.exp(-(d[..., 0] ** 2 + d[..., 1] ** 2) / sx ** 2 - d[..., 2] ** 2 / sz ** 2)
+    F = np.stack([2 * d[..., 0] / sx ** 2 * E, 2 * d[..., 1] / sx ** 2 * E, 2 * d[..., 2] / sz ** 2 * E], -1)
+    cell = np.diag([n[0] * h, n[1] * h, n[2] * h])
+    return np.ascontiguousarray(F), cell
+
+
+def _tip():
+    from cases import tip_params
+
+    return tip_params()
+
+
+def test_warm_jacobian_matches_finite_differences():
+    from cases import case
+
+    cs = case(0, small=True)
+    n, cell, F = cs["cfg"]["n"], cs["lvec"][1:], cs["FF"]
+    rng = np.random.default_rng(5)
+    h = 0.1
+    # points inside cells (fractions 0.2..0.8) so that +-eps stays in one cell
+    nodes = rng.integers(10, 50, size=(40, 3))
+    r = (nodes + rng.uniform(0.2, 0.8, size=(40, 3))) * h
+    tips = r + np.array([0.0, 0.0, 3.0]) + rng.normal(0, 0.3, size=(40, 3))
+    f, J = O.force_jac(n, cell, F, _tip(), tips, r)
+    eps = 1e-6
+    Jfd = np.zeros_like(J)
+    for a in range(3):
+        dr = np.zeros(3)
+        dr[a] = eps
+        fp, _ = O.force_jac(n, cell, F, _tip(), tips, r + dr)
+        fm, _ = O.force_jac(n, cell, F, _tip(), tips, r - dr)
+        Jfd[:, :, a] = (fp - fm) / (2 * eps)
+    scale = np.abs(J).max(axis=(1, 2), keepdims=True)
+    assert np.abs(J - Jfd).max() / scale.min() < 1e-6
+    # the force part is the relaxation's own total force (pp_force): a zero
+    # field with the probe at tip + rPP0 gives F = 0
+    z = np.zeros_like(F)
+    f0, _ = O.force_jac(n, cell, z, _tip(), tips, tips + np.array([0.0, 0.0, -3.0]))
+    assert np.abs(f0).max() < 1e-12
+
+
+def test_stable3_matches_eigenvalues():
+    rng = np.random.default_rng(1)
+    hits = 0
+    for _ in range(4000):
+        J = rng.normal(size=(3, 3)) - rng.uniform(0, 2) * np.eye(3)
+        re = np.linalg.eigvals(J).real.max()
+        if abs(re) < 1e-6:
+            continue
+        assert O.stable3(J) == (re < 0)
+        hits += re < 0
+    assert 200 < hits < 3800  # both outcomes exercised
+
+
+def test_warm_without_newton_is_plain_fire():
+    from cases import case
+    from ppafm_amd import molgen as M
+
+    cs = case(0, small=True)
+    n, cell, F = cs["cfg"]["n"], cs["lvec"][1:], cs["FF"]
+    tips = (M.scan_tips(cs["cfg"]) - cs["lvec"][0])[:6, :6]
+    rs, fs, it, _ = O.relax_strokes(n, cell, F, _tip(), tips)
+    rw, fw, iw, nev, nfb = O.relax_strokes_warm(n, cell, F, _tip(), tips, newton_max=0, newton_max0=0, retry=0)
+    assert np.array_equal(rs, rw) and np.array_equal(fs, fw) and np.array_equal(it, iw) and nfb == 0
+
+
+def test_warm_results_are_stable_minima():
+    from cases import case
+    from ppafm_amd import molgen as M
+
+    cs = case(0, small=True)
+    n, cell, F = cs["cfg"]["n"], cs["lvec"][1:], cs["FF"]
+    tips = (M.scan_tips(cs["cfg"]) - cs["lvec"][0])[::3, ::3]
+    rs, _, it, _ = O.relax_strokes(n, cell, F, _tip(), tips)
+    rw, _, iw, nev, nfb = O.relax_strokes_warm(n, cell, F, _tip(), tips)
+    npos = rw.size // 3
+    # Newton falls back to FIRE only where the continuation start is already
+    # in a non-convex region (deep in the approach of this rough case)
+    assert nfb < 0.1 * npos
+    f, J = O.force_jac(n, cell, F, _tip(), tips.reshape(-1, 3), rw.reshape(-1, 3))
+    fn = np.linalg.norm(f, axis=1)
+    conv = fn < 1e-6  # the positions Newton finished
+    assert conv.mean() > 0.9
+    assert all(O.stable3(Jk) for Jk in J[conv])
+    # the sample after FIRE's stop (|F|^2 < 1e-8 before its last move)
+    assert np.percentile(fn, 99) < 1e-3
+    # plain FIRE stops within its stop-rule scatter of the same minima
+    d = np.linalg.norm(rw - rs, axis=-1)
+    assert np.percentile(d, 99) < 0.03
+    # with several times fewer force evaluations (FIRE: iterations + 1 each)
+    assert nev.sum() < 0.3 * (it.sum() + npos)
+
+
+def test_warm_rejects_saddle_on_a_repulsive_bump():
+    F, cell = _bump_field()
+    n = F.shape[:3]
+    tip = _tip()
+    # strokes just off the bump's axis: the lateral instability pushes the
+    # probe sideways; Newton from the near-axis start heads for the on-axis
+    # saddle, which the stability test rejects
+    nz = 12
+    tz = 2.0 + 3.0 + 2.0 - 0.1 * np.arange(nz)
+    off = [0.002, 0.01, 0.05]
+    tips = np.zeros((len(off), 1, nz, 3))
+    for a, dx in enumerate(off):
+        tips[a, 0, :, 0] = 4.0 + dx
+        tips[a, 0, :, 1] = 4.0
+        tips[a, 0, :, 2] = tz
+    rs, _, _, _ = O.relax_strokes(n, cell, F, tip, tips)
+    rw, _, _, _, nfb = O.relax_strokes_warm(n, cell, F, tip, tips, newton_max0=0)
+    f, J = O.force_jac(n, cell, F, tip, tips[0, 0, :1], np.array([[4.0, 4.0, rs[0, 0, 0, 2]]]))
+    assert not O.stable3(J[0])  # the on-axis point is a saddle
+    assert nfb > 0
+    # the probe leaves the axis as plain FIRE does, and ends where FIRE ends
+    assert (np.abs(rw[..., 0] - 4.0)[:, :, -1] > 0.1).all()
+    assert np.abs(rw - rs).max() < 0.03
