@@ -42,6 +42,9 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--grid", choices=["f64", "f32"], default="f64", help="grid storage of the measured plain path")
     p.add_argument("--batch-items", type=int, default=512, help="molecules of the batched workload (0 skips it)")
+    p.add_argument("--learned-samples", type=int, default=1_000_000,
+                   help="training pairs of the warm-start network on the batch (0 skips the learned comparison)")
+    p.add_argument("--learned-epochs", type=float, default=8.0)
     return p.parse_args()
 
 
@@ -159,8 +162,77 @@ def run_batch(args):
     npts = int(np.prod(cfg["n"]))
     res = dict(t_step=t_step, stage=stage, n_eval=nev, n_pos=npos, natoms_total=int(natoms.sum()), npts=npts * nb,
                tuning=b.tuning, nitems=nb)
+    if args.learned_samples > 0:
+        res["learned"] = run_learned(args, b)
     b.close()
     return res
+
+
+def run_learned(args, b):
+    """Learned warm start on the same batch: the network trained on step 0
+    of 95% of the molecules (the last 5% held out for the accuracy
+    figures), then plain FIRE, warm starts without the network (Newton from
+    the plain start at step 0) and with it, each as the whole captured
+    graph (grid build, relaxation, df) timed over 3 runs."""
+    from ppafm_amd import device as D
+    from ppafm_amd import learn as L
+    from ppafm_amd.warmstart import NPARAM, WarmStartNet
+
+    nb = b.nitems
+    held = list(range(nb - max(1, nb // 20), nb))
+    t0 = time.time()
+    b.set_plain()
+    b.run()
+    b.synchronize()
+    P, T, truth = L.training_set(b, args.learned_samples, seed=1, items=range(held[0]))
+    t_data = time.time() - t0
+    print(f"[bench] learned: {P.shape[0]} training pairs in {t_data:.1f} s", file=sys.stderr, flush=True)
+    net = WarmStartNet(max_batch=8192, seed=3)
+    t0 = time.time()
+    losses = L.train(net, P, T, epochs=args.learned_epochs, batch=8192, lr=3e-3)
+    t_train = time.time() - t0
+    nsteps = len(losses)
+    print(f"[bench] learned: {nsteps} training steps in {t_train:.1f} s, loss {losses[:5].mean():.3f} -> {losses[-5:].mean():.4f}",
+          file=sys.stderr, flush=True)
+    r = L.compare(b, net, truth=truth, items=held, reps=3)
+    # stage times of the learned run: step-0 estimates, warm relaxation
+    b.set_warmstart(net)
+    st = b.stream
+    e0, e1 = D.Event(), D.Event()
+    stage = {}
+    for name, mask in (("grid", 1), ("estimates", 8), ("relax", 16), ("df", 4)):
+        b.run_stages(mask)
+        e0.record(st)
+        b.run_stages(mask)
+        e1.record(st)
+        b.synchronize()
+        stage[name] = e0.elapsed_ms(e1)
+    b.set_plain()
+    npos = r["plain"]["positions"]
+    ncols = sum(int(np.prod(b._scan_shape(k)[:2])) for k in range(nb))
+    flop_col = 2 * (9 * 81 * 32 + 288 * 32 + 32 * 3)  # one network pass per column
+    out = {"training": {"pairs": int(P.shape[0]), "held_out_molecules": len(held), "data_s": t_data, "train_s": t_train,
+                        "steps": nsteps, "batch": 8192, "epochs": args.learned_epochs,
+                        "samples_per_s": nsteps * 8192 / t_train, "loss_first": float(losses[:5].mean()),
+                        "loss_last": float(losses[-5:].mean()), "parameters": NPARAM}}
+    for k in ("plain", "newton", "learned"):
+        q = r[k]
+        out[k] = {"ms_per_step": q["ms"], "positions_per_s": npos / (q["ms"] * 1e-3),
+                  "force_evals_per_position": q["evals"] / npos, "fallback_steps": q["fallbacks"],
+                  "held_out_vs_min_A": q["vs_min"]}
+        if "vs_plain" in q:
+            out[k]["held_out_vs_plain_A"] = q["vs_plain"]
+        if "estimate_vs_min" in q:
+            out[k]["held_out_step0_estimate_vs_min_A"] = q["estimate_vs_min"]
+    out["learned"]["stage_ms"] = stage
+    out["learned"]["estimates_mfma"] = {"flop_per_column": flop_col, "columns": ncols,
+                                        "achieved_tflops": flop_col * ncols / (stage["estimates"] * 1e-3) / 1e12,
+                                        "peak_tflops": 2500.0,
+                                        "note": "bf16 MFMA, dense peak; the stage is bound by its patch gathers"}
+    out["speedup_learned_vs_plain"] = r["plain"]["ms"] / r["learned"]["ms"]
+    out["speedup_newton_only_vs_plain"] = r["plain"]["ms"] / r["newton"]["ms"]
+    out["speedup_learned_vs_newton_only"] = r["newton"]["ms"] / r["learned"]["ms"]
+    return out
 
 
 def run_cpu(args, cfg, xyz, Zs, qs, lvec, grid_host):
@@ -314,6 +386,9 @@ def main():
             "tuning": bt["tuning"],
             "speedup_vs_single_molecule_path": (bt["n_pos"] / bt["t_step"]) / (n_pos / g["t_step"]),
         }
+        if "learned" in bt:
+            out["learned"] = dict(workload="the batch above: step-0 network + Newton warm starts, FIRE to the same threshold",
+                                  **bt["learned"])
     if not args.no_cpu:
         grid_host = g["scan"].grid_host()
         c = run_cpu(args, cfg, xyz, Zs, qs, lvec, grid_host)

@@ -85,6 +85,26 @@ def positions_all(batch):
     return out
 
 
+def _rows(batch, ptr, items):
+    """Rows (positions, 3) float32 of the items' positions from a device
+    array at ``ptr`` laid out like the batch's positions."""
+    if items is None:
+        items = range(batch.nitems)
+    parts = []
+    for k in items:
+        lo, hi = int(batch.pos_off[k]), int(batch.pos_off[k + 1])
+        a = np.empty((hi - lo, 3), np.float32)
+        if hi > lo:
+            check(lib().ppafm_memcpy_d2h(a.ctypes.data, ptr + 12 * lo, a.nbytes, batch.stream), "download")
+        parts.append(a)
+    return np.concatenate(parts) if parts else np.zeros((0, 3), np.float32)
+
+
+def positions_of(batch, items=None):
+    """Relaxed positions of the last run for ``items`` (default all)."""
+    return _rows(batch, lib().ppafm_batch_buffer(batch._h, 1), items)
+
+
 def _stats(d):
     return dict(mean=float(d.mean()), p50=float(np.median(d)), p90=float(np.percentile(d, 90)),
                 p99=float(np.percentile(d, 99)), max=float(d.max()))
@@ -105,20 +125,18 @@ def timed_run(batch, reps=3):
 def evaluate(batch, truth, net=None, enable=True, items=None, reps=0, **warm):
     """Run the batch with the given warm-start setting (``enable`` False:
     plain FIRE) and return its counters, the distances of its results to the
-    polished minima ``truth`` (positions of ``items``, default all), the
-    distances of the step-0 estimates to the minima (with a network), and
-    the mean graph time over ``reps`` runs (0: untimed)."""
+    polished minima ``truth`` (a device array of all positions; statistics
+    over the positions of ``items``, default all), the distances of the
+    step-0 estimates to the minima (with a network), the positions of
+    ``items`` and the mean graph time over ``reps`` runs (0: untimed)."""
     batch.set_warmstart(net, enable=enable, **warm)
     ms = timed_run(batch, reps) if reps else None
     batch.run()
     batch.synchronize()
     ev, npos, nfb = batch.counters()
-    r = positions_all(batch)
-    rt = truth.download()
-    sl = slice(None)
-    if items is not None:
-        sl = np.concatenate([np.arange(batch.pos_off[k], batch.pos_off[k + 1]) for k in items])
-    out = dict(evals=ev, positions=npos, fallbacks=nfb, vs_min=_stats(np.linalg.norm(r[sl] - rt[sl], axis=1)), ms=ms,
+    r = positions_of(batch, items)
+    rt = _rows(batch, truth.ptr, items)
+    out = dict(evals=ev, positions=npos, fallbacks=nfb, vs_min=_stats(np.linalg.norm(r - rt, axis=1)), ms=ms,
                positions_array=r)
     if net is not None and enable:
         ks = range(batch.nitems) if items is None else items
@@ -128,7 +146,7 @@ def evaluate(batch, truth, net=None, enable=True, items=None, reps=0, **warm):
             if nz == 0:
                 continue
             e = batch.predictions(k).reshape(-1, 3)
-            t0 = rt[batch.pos_off[k] + np.arange(nx * ny) * nz]
+            t0 = _rows(batch, truth.ptr, [k])[::nz]
             d.append(np.linalg.norm(e - t0, axis=1))
         out["estimate_vs_min"] = _stats(np.concatenate(d))
     return out
@@ -148,10 +166,6 @@ def compare(batch, net, truth=None, items=None, reps=0, **warm):
            "learned": evaluate(batch, truth, net, items=items, reps=reps, **warm)}
     rp = res["plain"]["positions_array"]
     for k in ("newton", "learned"):
-        rw = res[k]["positions_array"]
-        sl = slice(None)
-        if items is not None:
-            sl = np.concatenate([np.arange(batch.pos_off[q], batch.pos_off[q + 1]) for q in items])
-        res[k]["vs_plain"] = _stats(np.linalg.norm(rw[sl] - rp[sl], axis=1))
+        res[k]["vs_plain"] = _stats(np.linalg.norm(res[k]["positions_array"] - rp, axis=1))
     batch.set_plain()
     return res
