@@ -194,7 +194,7 @@ def run_learned(args, b):
     nsteps = len(losses)
     print(f"[bench] learned: {nsteps} training steps in {t_train:.1f} s, loss {losses[:5].mean():.3f} -> {losses[-5:].mean():.4f}",
           file=sys.stderr, flush=True)
-    r = L.compare(b, net, truth=truth, items=held, reps=3)
+    r = L.compare(b, net, truth=truth, items=held, reps=3, tune=True)
     # stage times of the learned run: step-0 estimates, warm relaxation
     b.set_warmstart(net)
     st = b.stream
@@ -217,7 +217,7 @@ def run_learned(args, b):
                         "loss_last": float(losses[-5:].mean()), "parameters": NPARAM}}
     for k in ("plain", "newton", "learned"):
         q = r[k]
-        out[k] = {"ms_per_step": q["ms"], "positions_per_s": npos / (q["ms"] * 1e-3),
+        out[k] = {"ms_per_step": q["ms"], "positions_per_s": npos / (q["ms"] * 1e-3), "tuning": q["tuning"],
                   "force_evals_per_position": q["evals"] / npos, "fallback_steps": q["fallbacks"],
                   "held_out_vs_min_A": q["vs_min"]}
         if "vs_plain" in q:

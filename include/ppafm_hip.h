@@ -244,12 +244,14 @@ int ppafm_batch_set_atoms(ppafm_batch *b, const ppafm_atom *atoms);
 /* Enqueue the whole batch (graph launch) on the plan's stream. */
 int ppafm_batch_run(ppafm_batch *b);
 /* Enqueue single stages without the graph: bit 0 grid, bit 1 relaxation
- * (with the warm-start predictions first when a model is set), bit 2 df,
- * bit 3 predictions only, bit 4 FIRE only (for timing stages separately). */
+ * (with the step-0 estimates first when warm starts with a model are on),
+ * bit 2 df, bit 3 estimates only, bit 4 relaxation only (for timing stages
+ * separately). */
 int ppafm_batch_run_stages(ppafm_batch *b, int stages);
-/* Autotuner: time every launch shape of the persistent relaxation on the
- * first nsample molecules (grids built once), keep the fastest, re-capture
- * the graph and return it in best (and its time in best_ms, may be NULL). */
+/* Autotuner: time every launch shape of the persistent relaxation as the
+ * plan runs it (warm starts included) on the first nsample molecules (grids
+ * and step-0 estimates computed once), keep the fastest, re-capture the
+ * graph and return it in best (and its time in best_ms, may be NULL). */
 int ppafm_batch_tune(ppafm_batch *b, int nsample, int reps, ppafm_batch_tuning *best, double *best_ms);
 int ppafm_batch_set_tuning(ppafm_batch *b, const ppafm_batch_tuning *tuning);
 int ppafm_batch_get_tuning(const ppafm_batch *b, ppafm_batch_tuning *tuning);

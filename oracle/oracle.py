@@ -192,7 +192,7 @@ def force_jac(n, cell, F, tip, rtips, rs):
     return f, J
 
 
-def relax_strokes_warm(n, cell, F, tip, rTips, starts0=None, newton_max=8, newton_max0=12, retry=8, tol=1e-4,
+def relax_strokes_warm(n, cell, F, tip, rTips, starts0=None, newton_max=8, newton_max0=12, retry=8, tol=1e-3,
                        cap=0.1, relax=DEFAULT_RELAX):
     """Warm-start strokes (Newton from the continuation start, then FIRE
     with a Newton try every ``retry`` iterations; probeStart 1) on

@@ -579,9 +579,12 @@ static int newton_run(const orc_field *ff, const orc_tip *tip, v3 rtip, v3 *q, i
  * the current position; where it is stable and finite, Newton iterations
  * continue (newton_max in all, this step included) and on convergence FIRE
  * restarts at the zero with v = 0 and fresh dt, alpha, its iteration count
- * running on.  Returns the iteration index at which FIRE stopped. */
-static int fire_retry(const orc_field *ff, const orc_tip *tip, const orc_relax *rp, v3 rtip, v3 *r, int retry,
-                      int newton_max, double tol, double cap, int *ne)
+ * running on.  `check`: *r is a Newton zero; when the first evaluation there
+ * already meets the stop rule the step ends at *r without a move (returns
+ * 0), as after any later convergence of Newton.  Returns the iteration index
+ * at which FIRE stopped. */
+static int fire_retry(const orc_field *ff, const orc_tip *tip, const orc_relax *rp, v3 rtip, v3 *r, int check,
+                      int retry, int newton_max, double tol, double cap, int *ne)
 {
     v3 v = {0, 0, 0};
     const double dtmax = rp->dt, a0 = rp->damping;
@@ -590,6 +593,10 @@ static int fire_retry(const orc_field *ff, const orc_tip *tip, const orc_relax *
     while (it < rp->max_iters) {
         v3 f = pp_force(ff, tip, rtip, *r);
         (*ne)++;
+        if (check) {
+            check = 0;
+            if (v3_norm2(f) < rp->conv_f2) return it;
+        }
         if (retry > 0 && it >= next_try) {
             next_try = it + retry;
             v3 q = *r;
@@ -604,6 +611,7 @@ static int fire_retry(const orc_field *ff, const orc_tip *tip, const orc_relax *
                     v = (v3){0, 0, 0};
                     dt = dtmax;
                     acoef = a0;
+                    check = 1;
                 }
                 continue;
             }
@@ -646,7 +654,7 @@ static int warm_stroke(const orc_field *ff, const orc_tip *tip, const orc_relax 
         v3 drp = {rp_.x - rtip.x, rp_.y - rtip.y, rp_.z - rtip.z};
         rtip = nt;
         rp_.x = rtip.x + drp.x; rp_.y = rtip.y + drp.y; rp_.z = rtip.z + drp.z;
-        int ne = 0;
+        int ne = 0, check = 0;
         const int kmax = i == 0 ? newton_max0 : newton_max;
         v3 q = rp_;
         if (i == 0 && start0) q = (v3){start0[0], start0[1], start0[2]};
@@ -654,10 +662,11 @@ static int warm_stroke(const orc_field *ff, const orc_tip *tip, const orc_relax 
             rp_ = q;
         } else if (newton_run(ff, tip, rtip, &q, kmax, tol, cap, &ne)) {
             rp_ = q;
+            check = 1;
         } else {
             nfb++;
         }
-        const int itr = fire_retry(ff, tip, rp, rtip, &rp_, retry, newton_max, tol, cap, &ne);
+        const int itr = fire_retry(ff, tip, rp, rtip, &rp_, check, retry, newton_max, tol, cap, &ne);
         v3 g = {v3_dot(rp_, (v3){ff->dinv[0], ff->dinv[1], ff->dinv[2]}),
                 v3_dot(rp_, (v3){ff->dinv[3], ff->dinv[4], ff->dinv[5]}),
                 v3_dot(rp_, (v3){ff->dinv[6], ff->dinv[7], ff->dinv[8]})};

@@ -103,6 +103,7 @@ class BatchScan:
         self.natoms = np.array([int(it["natoms"]) for it in self.items])
         self.tune_sample = tune_sample
         self._auto = tuning == "auto"
+        self._warm_key = ""
 
     # ------------------------------------------------------------ plain mode
     @classmethod
@@ -193,7 +194,7 @@ class BatchScan:
         """Problem class of the autotuner cache."""
         it = self.items[0]
         s = it["scan"]
-        return f"mode{self.rp.mode}-{'f64' if self.grid_f64 else 'f32'}-scan{s.nx}x{s.ny}x{s.nz}"
+        return f"mode{self.rp.mode}-{'f64' if self.grid_f64 else 'f32'}-scan{s.nx}x{s.ny}x{s.nz}{self._warm_key}"
 
     # ------------------------------------------------------------ outputs
     def _download(self, which, off_elems, shape, dtype):
@@ -251,7 +252,7 @@ class BatchScan:
 
     # ------------------------------------------------------ learned warm start
     def set_warmstart(self, net=None, enable=True, net_passes=1, net_newton=2, newton_max=8, newton_max0=None,
-                      newton_retry=8, newton_tol=1e-4, newton_cap=0.1):
+                      newton_retry=8, newton_tol=1e-3, newton_cap=0.1):
         """Warm starts (``enable`` False: plain FIRE).  Every step first runs
         up to ``newton_max`` Newton iterations on the interpolated field from
         the continuation start (step 0: ``newton_max0`` from the network's
@@ -270,6 +271,7 @@ class BatchScan:
         self._ws = net if enable else None
         check(lib().ppafm_batch_set_warmstart(self._h, ctypes.byref(p), net._h if (net is not None and enable) else None),
               "batch_set_warmstart")
+        self._warm_key = "" if not enable else ("-warm-net" if net is not None else "-warm")
 
     def set_plain(self):
         """Plain FIRE from the continuation start at every step."""

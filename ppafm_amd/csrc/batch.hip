@@ -95,6 +95,7 @@ struct ppafm_batch {
     ppafm_ws_model *ws = nullptr;
     float *pred = nullptr;           // step-0 estimates (pixel columns, 3)
     long long *d_wprefix = nullptr;  // 64-column groups of items < k
+    std::vector<long long> wprefix;  // host copy
     long long nwaves = 0;
     long long *d_pprefix = nullptr;  // positions of items < k
 
@@ -191,11 +192,14 @@ RelaxBatchLaunch base_launch(ppafm_batch &b)
     return L;
 }
 
-int enqueue_predict(ppafm_batch &b)
+// Step-0 estimates of the first nitems items (all: b.nitems).
+int enqueue_predict(ppafm_batch &b, int nitems = -1)
 {
     if (!b.warm.enable || !b.ws) return 0;
     RelaxBatchLaunch L = base_launch(b);
-    return launch_ws_predict(ws_view(b.ws), L, b.d_wprefix, b.nwaves, b.pred, b.warm.net_passes, b.warm.net_newton, b.st);
+    if (nitems >= 0) L.nitems = nitems;
+    return launch_ws_predict(ws_view(b.ws), L, b.d_wprefix, b.wprefix[L.nitems], b.pred, b.warm.net_passes,
+                             b.warm.net_newton, b.st);
 }
 
 int enqueue_df(ppafm_batch &b)
@@ -461,12 +465,13 @@ int ppafm_batch_tune(ppafm_batch *b, int nsample, int reps, ppafm_batch_tuning *
     PP_REQUIRE(b, "batch is NULL");
     PP_REQUIRE(reps >= 1, "reps must be >= 1");
     const int ns = std::max(1, std::min(nsample, b->nitems));
-    const ppafm_warm_params warm_saved = b->warm;  // candidates are timed on plain FIRE
-    b->warm.enable = 0;
-    // grids of the sample, once
+    // candidates are timed on the relaxation the plan runs (warm starts
+    // included); grids and step-0 estimates of the sample, once
     int e = launch_fill_u32(b->ctl, 12, 0u, b->st);
     if (e) return e;
     e = enqueue_grid(*b, ns, b->brick_prefix[ns]);
+    if (e) return e;
+    e = enqueue_predict(*b, ns);
     if (e) return e;
     static const int tys[] = {2, 4, 8, 16};
     static const int wpbs[] = {1, 2, 4};
@@ -512,7 +517,6 @@ int ppafm_batch_tune(ppafm_batch *b, int nsample, int reps, ppafm_batch_tuning *
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    b->warm = warm_saved;
     if (rc) {
         if (rc == -2 || rc == -1) return rc;
         return -1;
@@ -592,6 +596,7 @@ int ppafm_batch_set_warmstart(ppafm_batch *b, const ppafm_warm_params *p, ppafm_
         PP_REQUIRE(ncol < (1LL << 31), "too many pixel columns for warm starts");
         PP_TRY_HIP(hipMalloc((void **)&b->pred, std::max<long long>(ncol, 1) * 12));
         b->nwaves = wp[b->nitems];
+        b->wprefix = wp;
         int e = upload(b->d_wprefix, wp, b->st);
         if (e) return e;
         PP_TRY_HIP(hipStreamSynchronize(b->st));

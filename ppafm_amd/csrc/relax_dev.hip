@@ -95,9 +95,11 @@ __device__ __forceinline__ void relax_column(const DevArgs &A, const T4 *__restr
 // current position with the corners FIRE just loaded; where the Jacobian is
 // stable Newton iterations take over as above, and on convergence FIRE
 // restarts there (velocity zero; mode 0 also resets dt and alpha), its
-// iteration count running on toward max_iters.  Either way FIRE ends the
-// step by the reference's stop rule, and the next step continues from
-// FIRE's result.  Newton iterates and FIRE moves share one 8-corner load per
+// iteration count running on toward max_iters.  When FIRE's first
+// evaluation at a Newton zero already meets the stop rule, the step ends at
+// that zero without a move (output force: the grid force there).  Either way
+// FIRE ends the step by the reference's stop rule, and the next step
+// continues from FIRE's result.  Newton iterates and FIRE moves share one 8-corner load per
 // loop pass.  Newton evaluations count in nevals.
 template <typename T4, int MODE>
 __device__ __forceinline__ void relax_column_warm(const DevArgs &A, const T4 *__restrict__ grid, int i, int j,
@@ -119,10 +121,13 @@ __device__ __forceinline__ void relax_column_warm(const DevArgs &A, const T4 *__
     double qx = S.rx, qy = S.ry, qz = S.rz;
     if (A.pred && live) {
         const float *p = A.pred + 3 * ((long)i * A.ny + j);
-        qx = (double)p[0]; qy = (double)p[1]; qz = (double)p[2];
+        if (isfinite(p[0]) && isfinite(p[1]) && isfinite(p[2])) {  // else the plain start
+            qx = (double)p[0]; qy = (double)p[1]; qz = (double)p[2];
+        }
     }
     int nk = 0, kmax = A.newton_max0;
     bool newton = live && kmax > 0, opening = newton;
+    bool check = false;  // S.r is a Newton zero not yet evaluated by FIRE
     if (live && !newton && A.pred) { S.rx = qx; S.ry = qy; S.rz = qz; }  // the estimate as it is
 
     while (s < nz) {
@@ -135,12 +140,14 @@ __device__ __forceinline__ void relax_column_warm(const DevArgs &A, const T4 *__
         const long o = c.col0 + s;
         const bool retry = !newton && !(MODE == 0 && sample) && A.newton_retry > 0 && it >= next_try;
         bool fire = !newton;
+        double fe[4];
+        interp8(v, ux, uy, uz, fe, MODE == 1);
         if (newton || retry) {
             double px = newton ? qx : S.rx, py = newton ? qy : S.ry, pz = newton ? qz : S.rz;
             bool stable;
             double st2;
-            const bool ok = newton_from<T4, MODE>(A, v, ux, uy, uz, tx, ty, tz, px, py, pz, A.newton_cap, stable, st2) &&
-                            stable;
+            const bool ok =
+                newton_from<T4, MODE>(A, v, ux, uy, uz, fe, tx, ty, tz, px, py, pz, A.newton_cap, stable, st2) && stable;
             if (retry) next_try = it + A.newton_retry;
             if (ok || newton) {
                 fire = false;
@@ -153,6 +160,7 @@ __device__ __forceinline__ void relax_column_warm(const DevArgs &A, const T4 *__
                     if (MODE == 0) { S.dt = A.dtmax; S.ac = A.a0; }
                     newton = false;
                     opening = false;
+                    check = true;
                 } else if (!ok || nk >= kmax) {  // FIRE from where it was
                     if (opening) nfb++;
                     newton = false;
@@ -163,26 +171,38 @@ __device__ __forceinline__ void relax_column_warm(const DevArgs &A, const T4 *__
             }
         }
         if (fire) {
-            double fe[4];
-            interp8(v, ux, uy, uz, fe, MODE == 1);
             if (MODE == 0 && sample) {
                 store_pos(A, o, S.rx, S.ry, S.rz);
                 if (A.force) A.force[o] = make_float4((float)fe[0], (float)fe[1], (float)fe[2], 0.f);
                 if (A.iters) A.iters[o] = itr;
                 done = true;
             } else {
-                const double ff = fire_move<MODE>(A, fe, tx, ty, tz, S);
-                it++;
-                const bool conv = ff < A.conv_f2;
-                if (conv || it >= A.max_iters) {
-                    if (MODE == 0) {
-                        itr = conv ? it - 1 : A.max_iters;
-                        sample = true;
-                    } else {
-                        if (A.force) A.force[o] = tip_frame_force(A, fe);
-                        store_pos(A, o, S.rx, S.ry, S.rz);
-                        if (A.iters) A.iters[o] = it;
-                        done = true;
+                double fx, fy, fz;
+                total_force<MODE>(A, fe, tx, ty, tz, S.rx, S.ry, S.rz, fx, fy, fz);
+                const double f2 = fma(fx, fx, fma(fy, fy, fz * fz));
+                const bool settled = check && f2 < A.conv_f2;
+                check = false;
+                if (settled) {  // Newton's zero meets the stop rule: the step ends here, no move
+                    store_pos(A, o, S.rx, S.ry, S.rz);
+                    if (A.force)
+                        A.force[o] = MODE == 0 ? make_float4((float)fe[0], (float)fe[1], (float)fe[2], 0.f)
+                                               : tip_frame_force(A, fe);
+                    if (A.iters) A.iters[o] = it;
+                    done = true;
+                } else {
+                    const double ff = fire_update<MODE>(A, fx, fy, fz, S);
+                    it++;
+                    const bool conv = ff < A.conv_f2;
+                    if (conv || it >= A.max_iters) {
+                        if (MODE == 0) {
+                            itr = conv ? it - 1 : A.max_iters;
+                            sample = true;
+                        } else {
+                            if (A.force) A.force[o] = tip_frame_force(A, fe);
+                            store_pos(A, o, S.rx, S.ry, S.rz);
+                            if (A.iters) A.iters[o] = it;
+                            done = true;
+                        }
                     }
                 }
             }
@@ -199,6 +219,7 @@ __device__ __forceinline__ void relax_column_warm(const DevArgs &A, const T4 *__
                 nk = 0;
                 kmax = A.newton_max;
                 newton = opening = kmax > 0;
+                check = false;
             }
         }
     }

@@ -196,31 +196,29 @@ __device__ __forceinline__ double fire_move(const DevArgs &A, const double fe[4]
 // One Newton step toward the zero of the total force on the trilinear grid
 // field: J dr = -F with the exact Jacobian of the interpolant in the current
 // cell (derivatives of the trilinear weights mapped by dinv) plus the
-// springs' Jacobian, from the cell's 8 corners v and the fractional position
-// (ux, uy, uz) of r in it.  A step longer than max_step is cut to that
-// length (damped Newton).  Returns false, leaving r unchanged, when the step
-// is not finite; st2 = squared length of the full step.
-// `stable` tells whether the Jacobian at r has all eigenvalues in the left
-// half plane (Routh-Hurwitz on its characteristic polynomial
-// l^3 + a2 l^2 + a1 l + a0): only then is a zero of the force there a minimum
-// that damped dynamics settle in, not a saddle or a maximum such as the point
-// straight above a repulsive atom.
+// springs' Jacobian, from the cell's 8 corners v, the fractional position
+// (ux, uy, uz) of r in it and the grid force fe there (interp8), all FP64.
+// A step longer than max_step is cut to that length (damped Newton).
+// Returns false, leaving r unchanged, when the step is not finite; st2 =
+// squared length of the full step.  `stable` tells whether the Jacobian at r
+// has all eigenvalues in the left half plane (Routh-Hurwitz on its
+// characteristic polynomial l^3 + a2 l^2 + a1 l + a0): only then is a zero of
+// the force there a minimum that damped dynamics settle in, not a saddle or a
+// maximum such as the point straight above a repulsive atom.
 template <typename T4, int MODE>
 __device__ __forceinline__ bool newton_from(const DevArgs &A, const T4 v[8], double ux, double uy, double uz,
-                                            double tx, double ty, double tz, double &rx, double &ry, double &rz,
-                                            double max_step, bool &stable, double &st2)
+                                            const double fe[4], double tx, double ty, double tz, double &rx,
+                                            double &ry, double &rz, double max_step, bool &stable, double &st2)
 {
-    double F[3] = {0, 0, 0}, dT[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};  // dT[i][a] = dF_i / du_a
+    double dT[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};  // dT[i][a] = dF_i / du_a
 #pragma unroll
     for (int k = 0; k < 8; k++) {
         const int bx = k >> 2, by = (k >> 1) & 1, bz = k & 1;
         const double wx = bx ? ux : 1 - ux, wy = by ? uy : 1 - uy, wz = bz ? uz : 1 - uz;
-        const double sx = bx ? 1.0 : -1.0, sy = by ? 1.0 : -1.0, sz = bz ? 1.0 : -1.0;
-        const double w = wx * wy * wz, d0 = sx * wy * wz, d1 = wx * sy * wz, d2 = wx * wy * sz;
+        const double d0 = (bx ? 1.0 : -1.0) * wy * wz, d1 = wx * (by ? 1.0 : -1.0) * wz, d2 = wx * wy * (bz ? 1.0 : -1.0);
         const double c[3] = {(double)v[k].x, (double)v[k].y, (double)v[k].z};
 #pragma unroll
         for (int i = 0; i < 3; i++) {
-            F[i] = fma(c[i], w, F[i]);
             dT[i][0] = fma(c[i], d0, dT[i][0]);
             dT[i][1] = fma(c[i], d1, dT[i][1]);
             dT[i][2] = fma(c[i], d2, dT[i][2]);
@@ -231,11 +229,10 @@ __device__ __forceinline__ bool newton_from(const DevArgs &A, const T4 v[8], dou
     for (int i = 0; i < 3; i++)
 #pragma unroll
         for (int j = 0; j < 3; j++) J[i][j] = fma(dT[i][2], A.dinv[6 + j], fma(dT[i][1], A.dinv[3 + j], dT[i][0] * A.dinv[j]));
-    // springs in the tip frame q = R d: a(q) = ks (q - d0) + krad (|q| - L) q / |q|,
-    // force R^T a, Jacobian R^T (da/dq) R
-    const double fe[4] = {F[0], F[1], F[2], 0.0};
     double fx, fy, fz;
     total_force<MODE>(A, fe, tx, ty, tz, rx, ry, rz, fx, fy, fz);
+    // springs in the tip frame q = R d: a(q) = ks (q - d0) + krad (|q| - L) q / |q|,
+    // force R^T a, Jacobian R^T (da/dq) R
     const double dx = rx - tx, dy = ry - ty, dz = rz - tz;
     const double *R = A.rot;
     const double q[3] = {MODE == 0 ? dx : fma(dz, R[2], fma(dy, R[1], dx * R[0])),
@@ -272,10 +269,10 @@ __device__ __forceinline__ bool newton_from(const DevArgs &A, const T4 v[8], dou
     const double C02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
     const double det = J[0][0] * C00 + J[0][1] * C01 + J[0][2] * C02;
     {
-        const double a2 = -(J[0][0] + J[1][1] + J[2][2]);
-        const double a1 = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) + (J[0][0] * J[2][2] - J[0][2] * J[2][0]) + C00;
-        const double a0 = -det;
-        stable = a2 > 0 && a0 > 0 && a2 * a1 > a0;
+        const double b2 = -(J[0][0] + J[1][1] + J[2][2]);
+        const double b1 = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) + (J[0][0] * J[2][2] - J[0][2] * J[2][0]) + C00;
+        const double b0 = -det;
+        stable = b2 > 0 && b0 > 0 && b2 * b1 > b0;
     }
     if (!(fabs(det) > 1e-300)) return false;
     const double id = 1.0 / det;
@@ -296,11 +293,12 @@ __device__ __forceinline__ bool newton_step(const DevArgs &A, const T4 *__restri
                                             double tz, double &rx, double &ry, double &rz, double max_step,
                                             bool &stable)
 {
-    double gx, gy, gz, ux, uy, uz, st2;
+    double gx, gy, gz, ux, uy, uz, st2, fe[4];
     to_grid(A, rx, ry, rz, gx, gy, gz);
     T4 v[8];
     corners(g, A, gx, gy, gz, v, ux, uy, uz);
-    return newton_from<T4, MODE>(A, v, ux, uy, uz, tx, ty, tz, rx, ry, rz, max_step, stable, st2);
+    interp8(v, ux, uy, uz, fe, false);
+    return newton_from<T4, MODE>(A, v, ux, uy, uz, fe, tx, ty, tz, rx, ry, rz, max_step, stable, st2);
 }
 
 __device__ __forceinline__ void store_pos(const DevArgs &A, long o, double rx, double ry, double rz)

@@ -774,13 +774,14 @@ __global__ __launch_bounds__(256) void ws_predict_kernel(DevArgs A0, Net net, co
         if (w == 0 && live) {
             // no Newton step from a point of unstable curvature: it would
             // head for a saddle; FIRE descends from the estimate instead
-            for (int k = 0; k < newton; k++) {
+            for (int k = 0; k < newton && isfinite(rx + ry + rz); k++) {
                 const double px = rx, py = ry, pz = rz;
                 bool stable;
                 const bool moved = newton_step<T4, MODE>(A, grid, tx, ty, tz, rx, ry, rz, 0.1, stable);
                 if (!stable) { rx = px; ry = py; rz = pz; break; }
                 if (!moved) break;
             }
+            if (!(isfinite(rx) && isfinite(ry) && isfinite(rz))) { rx = tx + ox; ry = ty + oy; rz = tz + oz; }
             float *pp = pred + 3 * ((long long)it.col_off + lc);
             pp[0] = (float)rx; pp[1] = (float)ry; pp[2] = (float)rz;
         }

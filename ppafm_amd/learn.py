@@ -122,14 +122,20 @@ def timed_run(batch, reps=3):
     return e0.elapsed_ms(e1) / reps
 
 
-def evaluate(batch, truth, net=None, enable=True, items=None, reps=0, **warm):
+def evaluate(batch, truth, net=None, enable=True, items=None, reps=0, tune=False, **warm):
     """Run the batch with the given warm-start setting (``enable`` False:
     plain FIRE) and return its counters, the distances of its results to the
     polished minima ``truth`` (a device array of all positions; statistics
     over the positions of ``items``, default all), the distances of the
     step-0 estimates to the minima (with a network), the positions of
-    ``items`` and the mean graph time over ``reps`` runs (0: untimed)."""
+    ``items`` and the mean graph time over ``reps`` runs (0: untimed).
+    ``tune``: launch shape from the autotuner cache for this setting (tuned
+    on a miss)."""
     batch.set_warmstart(net, enable=enable, **warm)
+    if tune:
+        from .batch import tuned
+
+        batch.set_tuning(tuned(batch))
     ms = timed_run(batch, reps) if reps else None
     batch.run()
     batch.synchronize()
@@ -137,7 +143,7 @@ def evaluate(batch, truth, net=None, enable=True, items=None, reps=0, **warm):
     r = positions_of(batch, items)
     rt = _rows(batch, truth.ptr, items)
     out = dict(evals=ev, positions=npos, fallbacks=nfb, vs_min=_stats(np.linalg.norm(r - rt, axis=1)), ms=ms,
-               positions_array=r)
+               positions_array=r, tuning=batch.tuning)
     if net is not None and enable:
         ks = range(batch.nitems) if items is None else items
         d = []
@@ -152,18 +158,20 @@ def evaluate(batch, truth, net=None, enable=True, items=None, reps=0, **warm):
     return out
 
 
-def compare(batch, net, truth=None, items=None, reps=0, **warm):
+def compare(batch, net, truth=None, items=None, reps=0, tune=False, **warm):
     """Plain FIRE, warm starts without the network (Newton from the plain
-    start at step 0) and warm starts with it, on the same batch."""
+    start at step 0) and warm starts with it, on the same batch (``tune``:
+    each with its own autotuned launch shape)."""
     batch.set_plain()
     batch.run()
     batch.synchronize()
     if truth is None:
         truth, _ = batch.polish()
         batch.synchronize()
-    res = {"plain": evaluate(batch, truth, None, enable=False, items=items, reps=reps),
-           "newton": evaluate(batch, truth, None, items=items, reps=reps, **{k: v for k, v in warm.items() if k != "newton_max0"}),
-           "learned": evaluate(batch, truth, net, items=items, reps=reps, **warm)}
+    res = {"plain": evaluate(batch, truth, None, enable=False, items=items, reps=reps, tune=tune),
+           "newton": evaluate(batch, truth, None, items=items, reps=reps, tune=tune,
+                              **{k: v for k, v in warm.items() if k != "newton_max0"}),
+           "learned": evaluate(batch, truth, net, items=items, reps=reps, tune=tune, **warm)}
     rp = res["plain"]["positions_array"]
     for k in ("newton", "learned"):
         res[k]["vs_plain"] = _stats(np.linalg.norm(res[k]["positions_array"] - rp, axis=1))

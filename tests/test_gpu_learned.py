@@ -69,21 +69,38 @@ def test_training_lowers_loss(setup):
 
 @pytest.mark.parametrize("with_net", [True, False])
 def test_warm_path_matches_fp64_oracle(setup, with_net):
+    """The FP64 oracle of the warm method reproduces every position within
+    1e-3 A, on the GPU-built grid (downloaded: the relaxation alone) and on
+    its own FP64 grid (the whole path; the grids differ by ~2e-8 relative in
+    summation order).  Item 5 has bistable points where plain FIRE's own
+    result flips under such differences (printed beside: the oracle's plain
+    FIRE moves by 1.6e-2 A under 1e-15 relative grid noise there); the warm
+    method converges to the minima and does not."""
     cases, b, net, losses, truth = setup
-    for k in (0, NITEMS - 1):  # a training item and the held-out one
+    for k in (0, 5, NITEMS - 1):  # training items (5: a bistable case) and the held-out one
         print(f"case: item {k}, network {with_net}", flush=True)
         FF, tips = _oracle_field(cases[k])
+        cfg, lvec = cases[k][0], cases[k][1]
+        Fg = np.ascontiguousarray(b.grid(k)[..., :3], dtype=np.float64)
         b.set_warmstart(net if with_net else None)
         b.run()
         b.synchronize()
         rg = b.positions_raw(k).astype(np.float64)
         starts0 = b.predictions(k).astype(np.float64) if with_net else None
-        rw, _, _, _, nfb = O.relax_strokes_warm(cases[k][0]["n"], cases[k][1][1:], FF, tip_params(), tips, starts0=starts0,
-                                                newton_max0=4 if with_net else 12)
-        d = np.linalg.norm(rg - rw, axis=-1)
-        print(f"item {k}: GPU warm vs FP64 oracle max {d.max():.2e} p99 {np.percentile(d, 99):.2e}; oracle fallbacks {nfb}")
-        assert d.max() < 1e-3
-    b.set_plain()
+        kw = dict(starts0=starts0, newton_max0=4 if with_net else 12)
+        rw_g = O.relax_strokes_warm(cfg["n"], lvec[1:], Fg, tip_params(), tips, **kw)[0]
+        rw_o = O.relax_strokes_warm(cfg["n"], lvec[1:], FF, tip_params(), tips, **kw)[0]
+        dg = np.linalg.norm(rg - rw_g, axis=-1)
+        do = np.linalg.norm(rg - rw_o, axis=-1)
+        b.set_plain()
+        b.run()
+        b.synchronize()
+        rp = b.positions_raw(k).astype(np.float64)
+        dpo = np.linalg.norm(rp - O.relax_strokes(cfg["n"], lvec[1:], FF, tip_params(), tips)[0], axis=-1)
+        print(f"item {k}: warm vs oracle on the GPU grid max {dg.max():.2e}; on the oracle grid max {do.max():.2e} "
+              f"p99.9 {np.percentile(do, 99.9):.2e} frac>1e-3 {np.mean(do > 1e-3):.5f} "
+              f"(plain FIRE there: max {dpo.max():.2e} frac>1e-3 {np.mean(dpo > 1e-3):.5f})")
+        assert dg.max() < 1e-3 and do.max() < 1e-3
 
 
 def test_warm_results_are_minima(setup):
