@@ -45,6 +45,9 @@ def parse():
     p.add_argument("--learned-samples", type=int, default=1_000_000,
                    help="training pairs of the warm-start network on the batch (0 skips the learned comparison)")
     p.add_argument("--learned-epochs", type=float, default=8.0)
+    p.add_argument("--c4-instances", type=int, default=16,
+                   help="instances of the large tilted case (Au slab + molecule, 400x400x300 grid; 0 skips it)")
+    p.add_argument("--c4-grid", choices=["f64", "f32"], default="f64", help="grid storage of the large tilted case")
     return p.parse_args()
 
 
@@ -235,6 +238,62 @@ def run_learned(args, b):
     return out
 
 
+def run_c4(args):
+    """Large tilted case: per instance an Au(111) slab with a seeded
+    60-atom molecule and 3x3 xy images (17,928 atoms), the OpenCL LJ +
+    point-charge grid on 400x400x300 points, a 400x400x60 scan with the tip
+    tilted 10 degrees (RelaxedScanner.run_relaxStrokesTilted semantics), df.
+    All instances in one batch plan (one captured graph); one timed run
+    after one warm-up run."""
+    from ppafm_amd import batch as B
+    from ppafm_amd import device as D
+    from ppafm_amd import molgen as M
+
+    cfg = M.CONFIGS[4]
+    ni = args.c4_instances
+    systems = [M.c4_system(k) for k in range(ni)]
+    lvec = M.config_lvec(cfg)
+    t = np.deg2rad(cfg["tilt_deg"])
+    rot = np.array([[np.cos(t), 0, np.sin(t)], [0, 1, 0], [-np.sin(t), 0, np.cos(t)]])
+    b = B.BatchScan.tilted([lvec] * ni, [cfg["n"]] * ni, rot=rot, pos0s=[(0.0, 0.0, cfg["z_top"])] * ni,
+                           start=(cfg["scan_x"][0], cfg["scan_y"][0]), end=(cfg["scan_x"][1], cfg["scan_y"][1]),
+                           scan_n=cfg["scan_n"], nz=cfg["nz"], zstep=cfg["dz"],
+                           molecules=[(x, z, q) for x, z, q, _ in systems], tuning="auto", tune_sample=2,
+                           grid_f64=args.c4_grid == "f64")
+    st = b.stream
+    b.run()
+    b.synchronize()
+    e0, e1 = D.Event(), D.Event()
+    e0.record(st)
+    b.run()
+    e1.record(st)
+    b.synchronize()
+    t_step = e0.elapsed_ms(e1)
+    nev, npos = b.counts()
+    stage = {}
+    for name, mask in (("grid", 1), ("relax", 2), ("df", 4)):
+        e0.record(st)
+        b.run_stages(mask)
+        e1.record(st)
+        b.synchronize()
+        stage[name] = e0.elapsed_ms(e1)
+    natoms = len(systems[0][0])
+    npts = int(np.prod(cfg["n"]))
+    pairs = npts * natoms * ni
+    tuning = b.tuning
+    b.close()
+    return {
+        "workload": f"{ni} instances: Au(111) slab (3 layers, 1932 atoms) + 60-atom molecule, 3x3 images ({natoms} atoms), "
+                    "OpenCL LJ+charge grid 400x400x300 at 0.1 A, 400x400x60 scan tilted 10 deg, df",
+        "instances": ni, "atoms_per_instance": natoms, "grid_points": npts * ni, "positions": npos,
+        "grid_storage": args.c4_grid,
+        "ms_per_step": t_step, "positions_per_s": npos / (t_step * 1e-3), "stage_ms": stage,
+        "grid_pairs_per_s": pairs / (stage["grid"] * 1e-3),
+        "grid_build_tflops": 32 * pairs / (stage["grid"] * 1e-3) / 1e12,
+        "force_evals_per_position": nev / npos, "tuning": tuning,
+    }
+
+
 def run_cpu(args, cfg, xyz, Zs, qs, lvec, grid_host):
     """Reference C++ (or FP64 port) on a bounded sample of the workload."""
     os.environ.setdefault("OMP_PROC_BIND", "close")
@@ -389,6 +448,9 @@ def main():
         if "learned" in bt:
             out["learned"] = dict(workload="the batch above: step-0 network + Newton warm starts, FIRE to the same threshold",
                                   **bt["learned"])
+    if args.c4_instances > 0:
+        print(f"[bench] c4: {args.c4_instances} instances", file=sys.stderr, flush=True)
+        out["c4"] = run_c4(args)
     if not args.no_cpu:
         grid_host = g["scan"].grid_host()
         c = run_cpu(args, cfg, xyz, Zs, qs, lvec, grid_host)

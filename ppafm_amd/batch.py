@@ -140,13 +140,71 @@ class BatchScan:
             b.set_molecules(molecules)
         return b
 
+    # ------------------------------------------------------------ tilted mode
+    @classmethod
+    def tilted(cls, lvecs, ns, rot=np.eye(3), pos0s=None, start=(-5.0, -5.0), end=(5.0, 5.0), scan_n=(32, 32), nz=20,
+               zstep=0.1, tipR0=(0.0, 0.0, 3.0), k_lat=0.25, k_rad=30.0, relax_params=None, Qs=(-0.1, 0.0, 0.0, 0.0),
+               QZs=(0.0, 0.0, 0.0, 0.0), iZPP=8, df_steps=10, k0=1800.0, f0=30300.0, surf_ff=0.0, molecules=None,
+               natoms=None, grid_f64=True, keep_iters=False, tuning=None, stream=None, species=None, tune_sample=32):
+        """Batch of tilted scans with the OpenCL semantics of
+        RelaxedScanner.run_relaxStrokesTilted (relax.cl:391-464) on the
+        evalLJC_QZs_noPos grid (FF.cl:366-423): per item a grid lattice (4,3)
+        and size, and pos0 (scan origin in world coordinates; default the
+        grid origin + (0, 0, 12)); shared tip frame ``rot`` (rows A, B, C),
+        scan window start/end along A and B, nz steps of zstep along -C,
+        tip stiffness (k_lat, k_lat, 0, k_rad) N/m, the Qs/QZs charge stencil
+        and df from get_simple_df_weight(df_steps, dz=zstep) as AFMulator
+        (AFMulator.py:125-133).  LJ parameters come from the REA table
+        (getAtomsREA with alphaFac -1, REA2LJ), as AFMulator takes them."""
+        from .ocl.relax import DEFAULT_relax_params, rotTip
+
+        if molecules is not None:
+            natoms = [len(m[1]) for m in molecules]
+        if natoms is None:
+            raise ValueError("give molecules or natoms")
+        rot = np.asarray(rot, dtype=np.float64)
+        dTip, tipRot, dpos0Tip, _ = rotTip(rot, zstep, (tipR0[0], tipR0[1], tipR0[2]))
+        st = np.array([k_lat, k_lat, 0.0, k_rad], dtype=np.float32) / -PPU.eVA_Nm
+        tip_d = D.tip_desc(np.asarray(dpos0Tip[:3], np.float64), st[:3].astype(np.float64), float(st[3]),
+                           float(np.float32(dpos0Tip[3])), rot=np.asarray(tipRot, np.float64)[:, :3], surf_ff=float(surf_ff))
+        rp_ = np.asarray(DEFAULT_relax_params if relax_params is None else relax_params, dtype=np.float32)
+        rp = D.relax_params(mode=1, alg=1, max_iters=128, conv_f2=float(np.float32(1e-8)), dt=float(rp_[0]),
+                            damping=float(rp_[1]), dtmin_frac=float(np.float32(0.1)), f2safe=float(np.float32(1e-8)))
+        model = D.ff_model(1, with_lj=True, stencil=list(zip(np.float32(Qs).tolist(), np.float32(QZs).tolist())),
+                           with_energy=True, fmax=None)
+        w = PPU.get_simple_df_weight(df_steps, dz=zstep) * PPU.eVA_Nm * f0 / k0
+        a = np.asarray(tipRot, np.float64)[0, :3]
+        b_ = np.asarray(tipRot, np.float64)[1, :3]
+        nx, ny = scan_n
+        items = []
+        for k, (lvec, n) in enumerate(zip(lvecs, ns)):
+            lvec = np.asarray(lvec, dtype=np.float64)
+            p = np.asarray(pos0s[k] if pos0s is not None else lvec[0] + np.array([0.0, 0.0, 12.0]), np.float64) - lvec[0]
+            scan_d = D.scan_desc(nx, ny, nz, p0=p + start[0] * a + start[1] * b_,
+                                 da=a * ((end[0] - start[0]) / max(nx - 1, 1)), db=b_ * ((end[1] - start[1]) / max(ny - 1, 1)),
+                                 dstep=np.asarray(dTip[:3], np.float64))
+            items.append(dict(n=n, cell=lvec[1:], natoms=int(natoms[k]), scan=scan_d))
+        b = cls(items, model, rp, tip_d, df_weights=w, df_scale=1.0, grid_f64=grid_f64, keep_iters=keep_iters,
+                tuning=tuning, stream=stream, tune_sample=tune_sample)
+        b.lvecs = [np.asarray(l, dtype=np.float64) for l in lvecs]
+        b.iZPP = iZPP
+        b.species = species or PPU.loadSpecies()
+        b._rea = True
+        if molecules is not None:
+            b.set_molecules(molecules)
+        return b
+
     def set_molecules(self, molecules):
-        """molecules: sequence of (xyzs, Zs, qs) in item order (plain mode;
-        the atom counts must match the plan's)."""
+        """molecules: sequence of (xyzs, Zs, qs) in item order (the atom
+        counts must match the plan's).  LJ parameters: the CPU table
+        (getAtomsLJ) for plain batches, the REA table for tilted ones."""
         packs = []
         for k, (xyzs, Zs, qs) in enumerate(molecules):
             xyzs = np.asarray(xyzs, dtype=np.float64)
-            cLJs = PPU.getAtomsLJ(self.iZPP, Zs, self.species)
+            if getattr(self, "_rea", False):
+                cLJs = PPU.REA2LJ(PPU.getAtomsREA(self.iZPP, Zs, self.species, alphaFac=-1.0))
+            else:
+                cLJs = PPU.getAtomsLJ(self.iZPP, Zs, self.species)
             xyzq = np.concatenate([xyzs - self.lvecs[k][0][None, :], np.asarray(qs, dtype=np.float64)[:, None]], axis=1)
             packs.append(D.pack_atoms(xyzq, cLJs, self.model.coulomb_const))
         self.set_atoms(packs)

@@ -90,6 +90,8 @@ struct ppafm_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 
+    bool any_uncharged = false;  // some uploaded atom has q = 0 (grid kernel variant)
+
     // warm starts (ppafm_batch_set_warmstart)
     ppafm_warm_params warm{};
     ppafm_ws_model *ws = nullptr;
@@ -135,7 +137,8 @@ void relax_tables(const ppafm_batch &b, int n, int tile_y, std::vector<BatchRela
 
 int enqueue_grid(ppafm_batch &b, int nitems, long long nbricks)
 {
-    return launch_ff_build_batch(b.model, nitems, b.d_gitems, b.d_bprefix, nbricks, b.atoms, b.grids, b.f64, b.st);
+    return launch_ff_build_batch(b.model, nitems, b.d_gitems, b.d_bprefix, nbricks, b.atoms, b.grids, b.f64,
+                                 b.any_uncharged, b.st);
 }
 
 int enqueue_relax(ppafm_batch &b, const ppafm_batch_tuning &t, const BatchRelaxItem *items, const long long *prefix,
@@ -443,6 +446,14 @@ int ppafm_batch_set_atoms(ppafm_batch *b, const ppafm_atom *atoms)
     PP_REQUIRE(atoms || n == 0, "atoms must not be NULL");
     if (n > 0) PP_TRY_HIP(hipMemcpyAsync(b->atoms, atoms, n * sizeof(AtomRec), hipMemcpyHostToDevice, b->st));
     PP_TRY_HIP(hipStreamSynchronize(b->st));
+    // the grid kernel skips uncharged atoms' Coulomb terms (exact zeros) when
+    // there are any; a change of that re-captures the graph
+    bool any = false;
+    for (long long k = 0; k < n && !any; k++) any = atoms[k].q == 0.0;
+    if (any != b->any_uncharged) {
+        b->any_uncharged = any;
+        if (b->exec) return capture(*b);
+    }
     return 0;
 }
 

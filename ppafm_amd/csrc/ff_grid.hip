@@ -113,7 +113,7 @@ __device__ __forceinline__ void pair_fp64(const FFArgs &A, double dx, double dy,
 // LJ: include the LJ term.  NS: number of Coulomb stencil points.  Z0:
 // stencil point 0 sits at zero z offset and reuses the LJ pair's rsq.
 // WE: accumulate the energy channel.
-template <bool LJ, int NS, bool Z0, bool WE>
+template <bool LJ, int NS, bool Z0, bool WE, bool QS = false>
 __device__ __forceinline__ void ff_brick_f32(const FFArgs &A, const AtomRec *__restrict__ atoms, float4 *__restrict__ out,
                                              int b, AtomLds *lds, double4 *ldsd)
 {
@@ -215,6 +215,9 @@ __device__ __forceinline__ void ff_brick_f32(const FFArgs &A, const AtomRec *__r
                         sL[k][2] = __fmaf_rn(dz, s, sL[k][2]);
                         if (WE) sL[k][3] = __fmaf_rn(__fmaf_rn(lj.w, ir6, -lj.z), ir6, sL[k][3]);
                     }
+                    // QS: uniform over the wave (one atom for all lanes); an
+                    // uncharged atom's Coulomb terms are exact zeros
+                    if (!QS || hi.w != 0.f)
 #pragma unroll
                     for (int s_ = 0; s_ < NS; s_++) {
                         float ir, ir2, dzs;
@@ -280,7 +283,7 @@ __device__ __forceinline__ double rsq_nr(double x)
 // FP64 pair with reciprocal square roots (v_rsq_f64 + two Newton steps, a
 // few ulp) in place of IEEE divide and sqrt; the Coulomb point at zero z
 // offset shares the LJ pair's rsq.
-template <bool LJ, int NS, bool Z0, bool WE>
+template <bool LJ, int NS, bool Z0, bool WE, bool QS = false>
 __device__ __forceinline__ void pair_fp64_fast(const FFArgs &A, double dx, double dy, double dz, double q, double c6,
                                                double c12, double aL[4], double aE[4])
 {
@@ -297,6 +300,10 @@ __device__ __forceinline__ void pair_fp64_fast(const FFArgs &A, double dx, doubl
         aL[0] = __fma_rn(dx, s, aL[0]); aL[1] = __fma_rn(dy, s, aL[1]); aL[2] = __fma_rn(dz, s, aL[2]);
         if (WE) aL[3] += e12 - e6;
     }
+    // QS (batches holding uncharged atoms): every lane of the brick reads the
+    // same atom, so this branch is uniform; an uncharged atom's Coulomb terms
+    // are exact zeros and are skipped
+    if (!QS || q != 0.0)
 #pragma unroll
     for (int s_ = 0; s_ < NS; s_++) {
         double ir, ir2, dzs;
@@ -318,7 +325,7 @@ __device__ __forceinline__ void pair_fp64_fast(const FFArgs &A, double dx, doubl
 // callers that need the grid to FP64 accuracy (the FIRE stop rule turns
 // grid perturbations of ~1e-7 into stopping-point changes of >1e-3 A on
 // ill-conditioned strokes).  Same brick mapping as ff_build_f32.
-template <bool LJ, int NS, bool Z0, bool WE>
+template <bool LJ, int NS, bool Z0, bool WE, bool QS = false>
 __device__ __forceinline__ void ff_brick_f64(const FFArgs &A, const AtomRec *__restrict__ atoms, double4 *__restrict__ out,
                                              int b, double4 *lpos, double2 *llj)
 {
@@ -369,7 +376,7 @@ __device__ __forceinline__ void ff_brick_f64(const FFArgs &A, const AtomRec *__r
 #pragma unroll
             for (int k = 0; k < kK; k++) {
                 if (k >= nk) break;
-                pair_fp64_fast<LJ, NS, Z0, WE>(A, px[k] - a.x, py[k] - a.y, pz[k] - a.z, a.w, c.x, c.y, aL[k], aE[k]);
+                pair_fp64_fast<LJ, NS, Z0, WE, QS>(A, px[k] - a.x, py[k] - a.y, pz[k] - a.z, a.w, c.x, c.y, aL[k], aE[k]);
             }
         }
     }
@@ -440,7 +447,7 @@ __device__ __forceinline__ FFArgs batch_args(const FFArgs &A0, const BatchGridIt
     return A;
 }
 
-template <bool LJ, int NS, bool Z0, bool WE>
+template <bool LJ, int NS, bool Z0, bool WE, bool QS>
 __global__ __launch_bounds__(kThreads) void ff_build_f32_batch(FFArgs A0, const BatchGridItem *__restrict__ items,
                                                                const long long *__restrict__ prefix, int nitems,
                                                                const AtomRec *__restrict__ atoms, float4 *__restrict__ grid)
@@ -451,10 +458,10 @@ __global__ __launch_bounds__(kThreads) void ff_build_f32_batch(FFArgs A0, const 
     const int m = find_item(prefix, nitems, bidx);
     const BatchGridItem it = items[m];
     const FFArgs A = batch_args(A0, it);
-    ff_brick_f32<LJ, NS, Z0, WE>(A, atoms + it.atom_off, grid + it.grid_off, (int)(bidx - prefix[m]), lds, ldsd);
+    ff_brick_f32<LJ, NS, Z0, WE, QS>(A, atoms + it.atom_off, grid + it.grid_off, (int)(bidx - prefix[m]), lds, ldsd);
 }
 
-template <bool LJ, int NS, bool Z0, bool WE>
+template <bool LJ, int NS, bool Z0, bool WE, bool QS>
 __global__ __launch_bounds__(kThreads) void ff_build_f64_batch(FFArgs A0, const BatchGridItem *__restrict__ items,
                                                                const long long *__restrict__ prefix, int nitems,
                                                                const AtomRec *__restrict__ atoms, double4 *__restrict__ grid)
@@ -465,7 +472,7 @@ __global__ __launch_bounds__(kThreads) void ff_build_f64_batch(FFArgs A0, const 
     const int m = find_item(prefix, nitems, bidx);
     const BatchGridItem it = items[m];
     const FFArgs A = batch_args(A0, it);
-    ff_brick_f64<LJ, NS, Z0, WE>(A, atoms + it.atom_off, grid + it.grid_off, (int)(bidx - prefix[m]), lpos, llj);
+    ff_brick_f64<LJ, NS, Z0, WE, QS>(A, atoms + it.atom_off, grid + it.grid_off, (int)(bidx - prefix[m]), lpos, llj);
 }
 
 // ------------------------------------------------------------------ FP64
@@ -559,20 +566,28 @@ struct BatchLaunch {
     const long long *prefix;
     int nitems;
     long long nbricks;
+    bool qskip;  // some atoms are uncharged: skip their Coulomb terms
 };
+
+template <bool LJ, int NS, bool Z0, bool WE, bool QS>
+void launch_batch_t(const FFArgs &A, const AtomRec *atoms, void *out, bool f64, const BatchLaunch *batch, hipStream_t st)
+{
+    dim3 block(kThreads), grid((unsigned)batch->nbricks);
+    if (f64)
+        hipLaunchKernelGGL((ff_build_f64_batch<LJ, NS, Z0, WE, QS>), grid, block, 0, st, A, batch->items, batch->prefix,
+                           batch->nitems, atoms, (double4 *)out);
+    else
+        hipLaunchKernelGGL((ff_build_f32_batch<LJ, NS, Z0, WE, QS>), grid, block, 0, st, A, batch->items, batch->prefix,
+                           batch->nitems, atoms, (float4 *)out);
+}
 
 template <bool LJ, int NS, bool Z0, bool WE>
 void launch_t(const FFArgs &A, const AtomRec *atoms, void *out, bool f64, const BatchLaunch *batch, hipStream_t st)
 {
     dim3 block(kThreads);
     if (batch) {
-        dim3 grid((unsigned)batch->nbricks);
-        if (f64)
-            hipLaunchKernelGGL((ff_build_f64_batch<LJ, NS, Z0, WE>), grid, block, 0, st, A, batch->items, batch->prefix,
-                               batch->nitems, atoms, (double4 *)out);
-        else
-            hipLaunchKernelGGL((ff_build_f32_batch<LJ, NS, Z0, WE>), grid, block, 0, st, A, batch->items, batch->prefix,
-                               batch->nitems, atoms, (float4 *)out);
+        if (LJ && NS > 0 && batch->qskip) launch_batch_t<LJ, NS, Z0, WE, (LJ && (NS > 0))>(A, atoms, out, f64, batch, st);
+        else launch_batch_t<LJ, NS, Z0, WE, false>(A, atoms, out, f64, batch, st);
         return;
     }
     dim3 grid((unsigned)A.nbx * A.nby * A.nbz);
@@ -674,14 +689,14 @@ long long ff_bricks(const int n[3])
 
 int launch_ff_build_batch(const ppafm_ff_model &m, int nitems, const BatchGridItem *items_dev,
                           const long long *prefix_dev, long long nbricks, const AtomRec *atoms_dev, void *grids,
-                          bool f64, hipStream_t st)
+                          bool f64, bool qskip, hipStream_t st)
 {
     PP_REQUIRE(m.n_stencil >= 0 && m.n_stencil <= 4, "n_stencil must be 0..4");
     PP_REQUIRE(nbricks >= 0 && nbricks < (1LL << 31), "batch too large for one launch");
     if (nbricks == 0) return 0;
     FFArgs A = {};
     const bool z0 = model_args(m, A);
-    BatchLaunch b{items_dev, prefix_dev, nitems, nbricks};
+    BatchLaunch b{items_dev, prefix_dev, nitems, nbricks, qskip};
     return dispatch(m, A, z0, atoms_dev, grids, f64, &b, st);
 }
 

@@ -86,7 +86,7 @@ long long ff_bricks(const int n[3]);
 // Every grid of a batch in one launch; prefix[k] = bricks of items < k.
 int launch_ff_build_batch(const ppafm_ff_model &m, int nitems, const BatchGridItem *items_dev,
                           const long long *prefix_dev, long long nbricks, const AtomRec *atoms_dev, void *grids,
-                          bool f64, hipStream_t st);
+                          bool f64, bool qskip, hipStream_t st);
 // FP64 accumulate (+=) into double3 grid F (and E) for the host ABI.
 // model: 0 = LJ, 1..3 = Coulomb kind 0..2.
 int launch_ff_accum_f64(const GridGeom &g, int model, int natoms, const AtomRec *atoms_dev, double *F, double *E,

@@ -120,7 +120,63 @@ CONFIGS = {
             scan_n=(256, 256), scan_x=(-10.0, 10.0), scan_y=(-10.0, 10.0), z_top=12.0, nz=60, dz=0.1),
     2: dict(natoms=None, origin=(-6.4, -6.4, -6.0), cell=(12.8, 12.8, 16.0), n=(128, 128, 160),
             scan_n=(128, 128), scan_x=(-4.8, 4.8), scan_y=(-4.8, 4.8), z_top=9.0, nz=30, dz=0.1),
+    # c4: Au(111) slab + molecule (c4_system), tip tilted by tilt_deg about y
+    4: dict(natoms=None, origin=(-20.0, -20.0, -10.0), cell=(40.0, 40.0, 30.0), n=(400, 400, 300),
+            scan_n=(400, 400), scan_x=(-18.0, 18.0), scan_y=(-18.0, 18.0), z_top=12.0, nz=60, dz=0.1,
+            tilt_deg=10.0, instances=16),
 }
+
+
+# ------------------------------------------------- large tilted case (c4)
+
+AU_Z = 79
+FCC_A = 4.08  # A, Au lattice constant
+C4_CELLS = (23, 14)  # rectangular fcc(111) cells along x, y (2 atoms per layer each)
+C4_LAYERS = 3
+C4_TOP = -3.0  # z of the slab's top layer
+C4_MOLECULE_ATOMS = 60
+
+
+def fcc111_slab(m=C4_CELLS[0], n=C4_CELLS[1], layers=C4_LAYERS, a=FCC_A, z_top=C4_TOP):
+    """fcc(111) slab on the rectangular cell (d, d*sqrt(3)) of the close-packed
+    layer, d = a/sqrt(2) the nearest-neighbour distance; ABC stacking, layer
+    spacing a/sqrt(3), top layer at z_top, centred on x = y = 0.  Returns
+    positions (N, 3) and the supercell's lateral vectors (2, 3)."""
+    d = a / math.sqrt(2.0)
+    h = d * math.sqrt(3.0)
+    dz = a / math.sqrt(3.0)
+    base = np.array([[0.0, 0.0], [0.5 * d, 0.5 * h]])
+    shift = np.array([0.5 * d, h / 6.0])  # A -> B -> C site offsets of the close-packed stacking
+    i, j = np.meshgrid(np.arange(m), np.arange(n), indexing="ij")
+    cells = np.stack([i.ravel() * d, j.ravel() * h], 1)
+    out = []
+    for layer in range(layers):
+        xy = (cells[:, None, :] + base[None] + (layers - 1 - layer) * shift).reshape(-1, 2)
+        z = np.full((len(xy), 1), z_top - layer * dz)
+        out.append(np.concatenate([xy, z], 1))
+    xyz = np.concatenate(out)
+    L = np.array([m * d, n * h])
+    xyz[:, :2] = np.mod(xyz[:, :2] + 0.5 * L, L) - 0.5 * L  # wrap into the cell centred on the origin
+    return xyz, np.array([[L[0], 0.0, 0.0], [0.0, L[1], 0.0]])
+
+
+def c4_system(instance=0, images=1):
+    """The c4 system of one instance: the Au(111) slab plus a seeded 60-atom
+    molecule from the config-0 generator in the z = 0 plane (seed 4e6 +
+    instance), with (2*images+1)^2 xy images of the slab supercell.
+    Returns xyz (N, 3), Zs, qs (slab neutral) and the supercell vectors."""
+    slab, sv = fcc111_slab()
+    mol, mZ, mq = config_molecule(4, instance, C4_MOLECULE_ATOMS)
+    xyz = np.concatenate([slab, mol])
+    Zs = np.concatenate([np.full(len(slab), AU_Z, np.int32), mZ])
+    qs = np.concatenate([np.zeros(len(slab)), mq])
+    out_x, out_z, out_q = [], [], []
+    for a in range(-images, images + 1):
+        for b in range(-images, images + 1):
+            out_x.append(xyz + a * sv[0] + b * sv[1])
+            out_z.append(Zs)
+            out_q.append(qs)
+    return np.concatenate(out_x), np.concatenate(out_z), np.concatenate(out_q), sv
 
 
 def config_lvec(cfg):

@@ -167,3 +167,28 @@ def test_batch_f32_grid_equals_single_scans():
         assert np.array_equal(b.grid(k), s.grid_host()), k
         assert np.array_equal(b.positions_raw(k), s.pos.download()), k
         assert np.array_equal(b.df_host(k), s.df_host()), k
+
+
+@pytest.mark.parametrize("f64", [True, False])
+def test_batch_with_uncharged_atoms_equals_single_scans(f64):
+    """A batch holding uncharged atoms runs the grid kernel variant that skips
+    their Coulomb terms (exact zeros); the grids and every output still equal
+    the single-molecule path's (which evaluates them) bit for bit, on double4
+    and float4 grids."""
+    print(f"case: 2 molecules with every third charge zero, grid_f64={f64}", flush=True)
+    cases = []
+    for n, scan_n, nz, natoms, seed in SPECS[:2]:
+        cfg, _ = small_config(n=n, scan_n=scan_n, nz=nz, natoms=natoms, seed=seed)
+        xyz, Zs, qs = M.molecule(natoms, seed)
+        qs = qs.copy()
+        qs[::3] = 0.0
+        cases.append((cfg, M.config_lvec(cfg), (xyz, Zs, qs)))
+    b = _batch(cases, grid_f64=f64)
+    b.run()
+    b.synchronize()
+    for k, c in enumerate(cases):
+        s = _plain(*c, grid_f64=f64)
+        assert np.array_equal(b.grid(k), s.grid_host()), k
+        assert np.array_equal(b.positions_raw(k), s.pos.download()), k
+        assert np.array_equal(b.df_host(k), s.df_host()), k
+    b.close()

@@ -1,0 +1,104 @@
+"""GPU parity of the large tilted case (c4): an Au(111) slab (3 layers, 1932
+atoms) with a 60-atom molecule, 3x3 xy images (17,928 atoms), the OpenCL
+LJ + point-charge grid on 400x400x300 points at 0.1 A, and tilted strokes
+(10 degrees about y, RelaxedScanner.run_relaxStrokesTilted semantics).
+
+The full grid is built on the GPU (one batch item).  Checked against the
+FP64 oracle:
+  * two 32x32x40 windows of the grid (at the molecule and near a corner of
+    the scan), 1e-5 relative with a 1e-3 eV/A floor;
+  * two 12x12x60 tilted scans (over the molecule and at the window edge,
+    where the tilted apex comes closest to the slab) relaxed by the oracle
+    on the GPU-built grid: paths within 1e-3 A, tip-frame forces within
+    1e-5 relative to the largest, df from the oracle's forces within
+    1e-3 Hz."""
+
+import numpy as np
+import pytest
+
+import oracle as O
+from cases import grid_rel_err
+from ppafm_amd import batch as B
+from ppafm_amd import common as C
+from ppafm_amd import molgen as M
+
+pytestmark = pytest.mark.gpu
+
+SCANS = {"centre": ((-1.5, -1.5), (1.5, 1.5)), "edge": ((-18.0, -1.5), (-15.0, 1.5))}
+
+
+def _rot_y(deg):
+    t = np.deg2rad(deg)
+    return np.array([[np.cos(t), 0, np.sin(t)], [0, 1, 0], [-np.sin(t), 0, np.cos(t)]])
+
+
+@pytest.fixture(scope="module")
+def c4():
+    cfg = M.CONFIGS[4]
+    print("case: c4 instance 0 (17,928 atoms), 400x400x300 grid, 10-degree tilt, scans 12x12x60", flush=True)
+    xyz, Zs, qs, _ = M.c4_system(0)
+    lvec = M.config_lvec(cfg)
+    out = {}
+    for name, (s0, s1) in SCANS.items():
+        b = B.BatchScan.tilted([lvec], [cfg["n"]], rot=_rot_y(cfg["tilt_deg"]), pos0s=[(0.0, 0.0, cfg["z_top"])],
+                               start=s0, end=s1, scan_n=(12, 12), nz=cfg["nz"], zstep=cfg["dz"],
+                               molecules=[(xyz, Zs, qs)])
+        b.run()
+        b.synchronize()
+        out[name] = b
+    return cfg, lvec, xyz, Zs, qs, out
+
+
+def test_c4_grid_windows_match_oracle(c4):
+    cfg, lvec, xyz, Zs, qs, bs = c4
+    G = bs["centre"].grid(0)
+    REAs = C.getAtomsREA(8, Zs, C.loadSpecies(), alphaFac=-1.0)
+    cLJs = C.REA2LJ(REAs)
+    n = np.array(cfg["n"])
+    h = np.array(cfg["cell"]) / n
+    for lo in ((184, 184, 90), (16, 184, 140)):
+        lo = np.array(lo)
+        sub = np.array([32, 32, 40])
+        lv = np.array([np.asarray(cfg["origin"]) + lo * h, [sub[0] * h[0], 0, 0], [0, sub[1] * h[1], 0], [0, 0, sub[2] * h[2]]])
+        FE = O.ljc_qzs_ff(sub, lv, np.concatenate([xyz, qs[:, None]], 1), cLJs, np.array([-0.1, 0, 0, 0.0]), np.zeros(4))
+        Gs = G[lo[0]:lo[0] + sub[0], lo[1]:lo[1] + sub[1], lo[2]:lo[2] + sub[2]]
+        e = grid_rel_err(Gs, FE)
+        print(f"window at {tuple(lo)}: max rel err {e.max():.2e}, |F| max {np.linalg.norm(FE[..., :3], axis=-1).max():.2e} eV/A")
+        assert e.max() < 1e-5
+
+
+@pytest.mark.parametrize("name", list(SCANS))
+def test_c4_tilted_scan_matches_oracle(c4, name):
+    cfg, lvec, xyz, Zs, qs, bs = c4
+    b = bs[name]
+    it = b.items[0]
+    s = it["scan"]
+    nx, ny, nz = s.nx, s.ny, s.nz
+    i, j = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    poss = np.zeros((nx, ny, 4))
+    for c in range(3):
+        poss[..., c] = s.p0[c] + i * s.da[c] + j * s.db[c]
+    cell = np.asarray(it["cell"], dtype=np.float64)
+    dinv = np.linalg.inv(cell / np.array(cfg["n"])[:, None]).T
+    FE = b.grid(0)
+    tip = b.tip
+    tipRot = np.zeros((3, 4))
+    tipRot[:, :3] = np.array(tip.rot).reshape(3, 3)
+    tipRot[2, 3] = -cfg["dz"]
+    st = np.array([tip.kspring[0], tip.kspring[1], tip.kspring[2], tip.krad])
+    dpos0 = np.array([tip.d0[0], tip.d0[1], tip.d0[2], tip.lrad])
+    FEs, paths, nev = O.relax_tilted(FE, dinv, poss.reshape(-1, 4), nz, tipRot, st, dpos0,
+                                     np.array([b.rp.dt, b.rp.damping]), max_iters=128,
+                                     conv_f2=float(np.float32(1e-8)), f2safe=float(np.float32(1e-8)))
+    gp = b.positions_raw(0).reshape(-1, nz, 3)
+    gf = b.forces(0).reshape(-1, nz, 4)
+    dp = np.linalg.norm(gp - paths[..., :3], axis=-1)
+    dff = np.abs(gf - FEs).max()
+    df_o = O.convolve_z(FEs[..., 2].reshape(-1, nz), b.w)
+    ddf = np.abs(b.df_raw(0).reshape(len(df_o), -1) - df_o).max()
+    zmin = (paths[..., 2] + lvec[0][2]).min()
+    print(f"{name}: paths max {dp.max():.2e} A, forces max {dff:.2e} eV/A, df max {ddf:.2e} Hz; "
+          f"lowest probe z {zmin:.2f} A, mean evaluations {nev.mean():.1f}")
+    assert dp.max() < 1e-3
+    assert dff < 1e-5 * max(1.0, np.abs(FEs).max())
+    assert ddf < 1e-3
