@@ -382,6 +382,9 @@ def main():
     alg_bytes = 8 * 3 * corner_bytes * g["n_eval"] + 24 * n_pos
     achieved = alg_bytes / (g["t_relax"] * 1e-3) / 1e9
     traffic = traffic_from_profiles()
+    from ppafm_amd import device as D
+
+    peaks = {k: D.measure_peak(k) for k in D.PEAK_KINDS}
     out = {
         "metric": METRIC,
         "value": value,
@@ -418,14 +421,18 @@ def main():
             "traffic": traffic,
             "algorithmic_bytes_per_launch": alg_bytes,
             "bytes_model": f"8 corners x 3 x {corner_bytes} B per force evaluation + 24 B per relaxed position",
+            "achieved_fp32_corner_units": (96 * g["n_eval"] + 24 * n_pos) / (g["t_relax"] * 1e-3) / 1e9,
+            "fp32_corner_units_note": "96 B per force evaluation (8 corners x 3 FP32) + 24 B per position, as for an FP32 grid",
         },
         "grid_build": {
             "bound": "valu",
             "pairs": npts * len(xyz),
             "achieved_tflops": 32 * npts * len(xyz) / (g["t_grid"] * 1e-3) / 1e12,
             "peak_tflops": 78.6 if args.grid == "f64" else 157.3,
-            "note": "32 algorithmic FLOP per atom-point pair (LJ + Coulomb, force + energy)",
+            "peak_measured_tflops": peaks["fp64_fma"] if args.grid == "f64" else peaks["fp32_fma"],
+            "note": "32 algorithmic FLOP per atom-point pair (LJ + Coulomb, force + energy); FP64 pairs on the double4 grid",
         },
+        "peaks_measured_tflops": peaks,
     }
     if args.batch_items > 0:
         bt = run_batch(args)
@@ -448,6 +455,7 @@ def main():
         if "learned" in bt:
             out["learned"] = dict(workload="the batch above: step-0 network + Newton warm starts, FIRE to the same threshold",
                                   **bt["learned"])
+            out["learned"]["learned"]["estimates_mfma"]["peak_measured_tflops"] = peaks["bf16_mfma"]
     if args.c4_instances > 0:
         print(f"[bench] c4: {args.c4_instances} instances", file=sys.stderr, flush=True)
         out["c4"] = run_c4(args)

@@ -349,6 +349,11 @@ int ppafm_synchronize(void *stream);
 int ppafm_device_count(void);
 /* Name (gcnArchName) and compute-unit count of the current device. */
 int ppafm_device_info(char *name, int len, int *ncu);
+/* Measured peak rate of the current device in TFLOP/s, for the rooflines:
+ * kind 0 FP32 vector FMA (unpacked, as every kernel here is built), 1 FP64
+ * vector FMA, 2 bf16 MFMA (v_mfma_f32_16x16x32_bf16); iters sets the run
+ * length.  No counterpart in the reference. */
+int ppafm_measure_peak(int kind, int iters, double *tflops);
 /* HIP events for timing (elapsed waits for ev1) and non-blocking streams of
  * default priority. */
 int ppafm_event_create(void **ev);

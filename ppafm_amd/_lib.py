@@ -132,6 +132,7 @@ SIGNATURES = [
     ("ppafm_batch_destroy", None, [c_void_p]),
     ("ppafm_device_info", c_int, [ctypes.c_char_p, c_int, _ip]),
     ("ppafm_batch_set_warmstart", c_int, [c_void_p, c_void_p, c_void_p]),
+    ("ppafm_measure_peak", c_int, [c_int, c_int, ctypes.POINTER(c_double)]),
     ("ppafm_batch_polish", c_int, [c_void_p, c_double, c_int, c_void_p, c_void_p]),
     ("ppafm_batch_ws_samples", c_int, [c_void_p, c_void_p, ctypes.c_longlong, c_void_p, c_void_p, c_void_p, c_double, ctypes.c_uint]),
     ("ppafm_ws_create", c_void_p, [c_int]),

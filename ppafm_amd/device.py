@@ -63,6 +63,18 @@ def synchronize(stream=None):
     check(lib().ppafm_synchronize(stream), "synchronize")
 
 
+PEAK_KINDS = {"fp32_fma": (0, 500_000), "fp64_fma": (1, 500_000), "bf16_mfma": (2, 250_000)}
+
+
+def measure_peak(kind):
+    """Measured peak TFLOP/s of the current device for ``kind`` in
+    PEAK_KINDS (microbenchmark kernels of the library)."""
+    k, iters = PEAK_KINDS[kind]
+    v = ctypes.c_double()
+    check(lib().ppafm_measure_peak(k, iters, ctypes.byref(v)), "measure_peak")
+    return float(v.value)
+
+
 def device_count():
     return lib().ppafm_device_count()
 
