@@ -167,8 +167,75 @@ def run_batch(args):
                tuning=b.tuning, nitems=nb)
     if args.learned_samples > 0:
         res["learned"] = run_learned(args, b)
+    if not args.no_cpu:
+        res["cpu"] = run_cpu_batch(args, b, mols, cfg)
     b.close()
     return res
+
+
+def _cpu_env():
+    """The oracle module and the OpenMP threads it runs with (the
+    environment is set at the start of main(), before any OpenMP runtime
+    starts: once a bound parallel region has run, the main thread's
+    affinity no longer shows the cores)."""
+    if os.path.join(ROOT, "oracle") not in sys.path:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    return O, O.num_threads()
+
+
+def run_cpu_batch(args, b, mols, cfg, nsub=16):
+    """Reference C++ on the first ``nsub`` molecules of the batch (grid build,
+    then the strokes on the GPU-built FP64 grid), one run each; the batch is
+    a seeded sample of one distribution, so the full batch scales by
+    positions."""
+    O, cores = _cpu_env()
+    from ppafm_amd import common as C
+    from ppafm_amd import molgen as M
+
+    kind = "reference" if O.have_reference() else "port"
+    lvec = M.config_lvec(cfg)
+    n = np.array(cfg["n"], dtype=np.int32)
+    tips = np.ascontiguousarray(M.scan_tips(cfg) - lvec[0])
+    tip = O.tip_params(M.TIP_R0[2], M.K_RAD / -C.eVA_Nm, (0.0, 0.0, 0.0), np.array([M.K_LAT, M.K_LAT, 0.0]) / -C.eVA_Nm)
+    t_grid = t_relax = 0.0
+    nsub = min(nsub, b.nitems)
+    with quiet_stdout():
+        ref = O.Reference() if kind == "reference" else None
+        for k in range(nsub):
+            xyz, Zs, qs = mols[k]
+            cLJs = C.getAtomsLJ(M.PROBE_Z, Zs, C.loadSpecies())
+            Rs0 = np.ascontiguousarray(xyz - lvec[0])
+            kq = np.ascontiguousarray(qs * C.CoulombConst)
+            F = np.zeros(tuple(n) + (3,))
+            FF = np.ascontiguousarray(b.grid(k)[..., :3], dtype=np.float64)
+            if ref is not None:
+                ref.set_grid(n, lvec[1:], F)
+                t0 = time.perf_counter()
+                ref.lj(Rs0, cLJs)
+                ref.coulomb(Rs0, kq, 0)
+                t_grid += time.perf_counter() - t0
+                ref.set_grid(n, lvec[1:], FF)
+                ref.set_tip(M.TIP_R0[2], M.K_RAD / -C.eVA_Nm, np.zeros(3), np.array([M.K_LAT, M.K_LAT, 0.0]) / -C.eVA_Nm)
+                ref.set_relax(1000, 1e-8, 0.1, 0.1)
+                t0 = time.perf_counter()
+                ref.relax_strokes(tips)
+                t_relax += time.perf_counter() - t0
+            else:
+                t0 = time.perf_counter()
+                O.lj_ff(n, lvec[1:], Rs0, cLJs, F=F)
+                O.coulomb_ff(n, lvec[1:], Rs0, kq, F=F)
+                t_grid += time.perf_counter() - t0
+                t0 = time.perf_counter()
+                O.relax_strokes(n, lvec[1:], FF, tip, tips, want_iters=False)
+                t_relax += time.perf_counter() - t0
+    npos = nsub * int(np.prod(cfg["scan_n"])) * cfg["nz"]
+    scale = b.nitems / nsub
+    return dict(value=npos / (t_grid + t_relax), unit="relaxed tip positions/s", cores=cores, kind=kind,
+                sample=f"first {nsub} of {b.nitems} molecules, one run each: grid {t_grid:.2f} s, relaxation {t_relax:.2f} s; "
+                       f"CPU {cpu_model()}, {cores} OpenMP threads",
+                full_batch_s_extrapolated=(t_grid + t_relax) * scale)
 
 
 def run_learned(args, b):
@@ -281,6 +348,7 @@ def run_c4(args):
     npts = int(np.prod(cfg["n"]))
     pairs = npts * natoms * ni
     tuning = b.tuning
+    cpu = None if args.no_cpu else run_cpu_c4(b, systems[0], cfg, lvec)
     b.close()
     return {
         "workload": f"{ni} instances: Au(111) slab (3 layers, 1932 atoms) + 60-atom molecule, 3x3 images ({natoms} atoms), "
@@ -291,16 +359,59 @@ def run_c4(args):
         "grid_pairs_per_s": pairs / (stage["grid"] * 1e-3),
         "grid_build_tflops": 32 * pairs / (stage["grid"] * 1e-3) / 1e12,
         "force_evals_per_position": nev / npos, "tuning": tuning,
+        "cpu_baseline": cpu,
+        "speedup_vs_cpu": (npos / (t_step * 1e-3)) / cpu["value"] if cpu else None,
     }
+
+
+def run_cpu_c4(b, system, cfg, lvec, planes=(60, 120, 180, 240), stride=10):
+    """FP64 port (the reference C++ has neither the OpenCL grid nor tilted
+    strokes) on a sample of c4 instance 0: the grid on 4 z-planes of
+    400x400 points and the tilted strokes of every stride-th pixel column
+    along x and y (1 %) on the GPU-built grid, scaled linearly to the whole
+    instance and to all instances."""
+    O, cores = _cpu_env()
+    from ppafm_amd import common as C
+
+    xyz, Zs, qs, _ = system
+    cLJs = C.REA2LJ(C.getAtomsREA(8, Zs, C.loadSpecies(), alphaFac=-1.0))
+    atoms = np.concatenate([xyz, qs[:, None]], 1)
+    n = np.array(cfg["n"])
+    h = np.array(cfg["cell"]) / n
+    t0 = time.perf_counter()
+    for iz in planes:
+        lv = np.array([np.asarray(cfg["origin"]) + np.array([0, 0, iz * h[2]]), [cfg["cell"][0], 0, 0],
+                       [0, cfg["cell"][1], 0], [0, 0, h[2]]])
+        O.ljc_qzs_ff((n[0], n[1], 1), lv, atoms, cLJs, np.array([-0.1, 0, 0, 0.0]), np.zeros(4))
+    t_grid = (time.perf_counter() - t0) * n[2] / len(planes)
+    it = b.items[0]
+    s = it["scan"]
+    ii, jj = np.meshgrid(np.arange(0, s.nx, stride), np.arange(0, s.ny, stride), indexing="ij")
+    poss = np.zeros(ii.shape + (4,))
+    for c in range(3):
+        poss[..., c] = s.p0[c] + ii * s.da[c] + jj * s.db[c]
+    cell = np.asarray(it["cell"], dtype=np.float64)
+    dinv = np.linalg.inv(cell / n[:, None]).T
+    tip = b.tip
+    tipRot = np.zeros((3, 4))
+    tipRot[:, :3] = np.array(tip.rot).reshape(3, 3)
+    tipRot[2, 3] = -cfg["dz"]
+    FE = b.grid(0)
+    t0 = time.perf_counter()
+    O.relax_tilted(FE, dinv, poss.reshape(-1, 4), s.nz, tipRot, np.array([tip.kspring[0], tip.kspring[1], tip.kspring[2], tip.krad]),
+                   np.array([tip.d0[0], tip.d0[1], tip.d0[2], tip.lrad]), np.array([b.rp.dt, b.rp.damping]), max_iters=128,
+                   conv_f2=float(np.float32(1e-8)), f2safe=float(np.float32(1e-8)))
+    t_rel = (time.perf_counter() - t0) * (s.nx * s.ny) / ii.size
+    npos = s.nx * s.ny * s.nz
+    return dict(value=npos / (t_grid + t_rel), unit="relaxed tip positions/s", cores=cores, kind="port",
+                sample=f"instance 0: grid on {len(planes)} of {n[2]} z-planes, tilted strokes of {ii.size} of {s.nx * s.ny} columns, "
+                       f"scaled linearly: grid {t_grid:.1f} s, relaxation {t_rel:.1f} s per instance; CPU {cpu_model()}, {cores} OpenMP threads",
+                instance_s_extrapolated=t_grid + t_rel)
 
 
 def run_cpu(args, cfg, xyz, Zs, qs, lvec, grid_host):
     """Reference C++ (or FP64 port) on a bounded sample of the workload."""
-    os.environ.setdefault("OMP_PROC_BIND", "close")
-    cores = cpu_cores()
-    os.environ["OMP_NUM_THREADS"] = str(cores)
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
+    O, cores = _cpu_env()
 
     from ppafm_amd import common as C
     from ppafm_amd import molgen as M
@@ -366,6 +477,9 @@ def traffic_from_profiles():
 
 def main():
     args = parse()
+    # OpenMP threads of the CPU legs: every core this process may use
+    os.environ["OMP_NUM_THREADS"] = str(cpu_cores())
+    os.environ.setdefault("OMP_PROC_BIND", "close")
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).", file=sys.stderr)
         sys.exit(2)
@@ -452,6 +566,9 @@ def main():
             "tuning": bt["tuning"],
             "speedup_vs_single_molecule_path": (bt["n_pos"] / bt["t_step"]) / (n_pos / g["t_step"]),
         }
+        if "cpu" in bt:
+            out["batch"]["cpu_baseline"] = bt["cpu"]
+            out["batch"]["speedup_vs_cpu"] = out["batch"]["positions_per_s"] / bt["cpu"]["value"]
         if "learned" in bt:
             out["learned"] = dict(workload="the batch above: step-0 network + Newton warm starts, FIRE to the same threshold",
                                   **bt["learned"])

@@ -285,12 +285,12 @@ void ppafm_batch_destroy(ppafm_batch *b);
  * next step continues from FIRE's result.
  * Newton evaluations count in counters[0].  With a model, stage bit 3 (8)
  * computes the step-0 estimates (buffer 7): net_passes network passes, each
- * on the patch centred at the last estimate, then net_newton Newton steps.
+ * on the patch centred at the last estimate; step 0's Newton run starts
+ * there.
  * The model must outlive the plan's runs; the graph is re-captured. */
 typedef struct ppafm_warm_params {
     int enable;        /* 0: plain FIRE, the other fields are ignored */
     int net_passes;    /* network passes at step 0 (model given), 1..8 */
-    int net_newton;    /* Newton steps after the network, 0..4 */
     int newton_max;    /* Newton iterations per step > 0, 0..64 (0: plain start) */
     int newton_max0;   /* Newton iterations at step 0, 0..64 (0: the estimate or plain start as is) */
     int newton_retry;  /* FIRE iterations between Newton tries from FIRE's position, 0..1000 (0: none) */

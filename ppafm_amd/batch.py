@@ -309,23 +309,23 @@ class BatchScan:
         return int(c[0]), int(c[1]), int(c[2])
 
     # ------------------------------------------------------ learned warm start
-    def set_warmstart(self, net=None, enable=True, net_passes=1, net_newton=2, newton_max=8, newton_max0=None,
-                      newton_retry=8, newton_tol=1e-3, newton_cap=0.1):
+    def set_warmstart(self, net=None, enable=True, net_passes=1, newton_max=8, newton_max0=None, newton_retry=8,
+                      newton_tol=1e-3, newton_cap=0.1):
         """Warm starts (``enable`` False: plain FIRE).  Every step first runs
         up to ``newton_max`` Newton iterations on the interpolated field from
         the continuation start (step 0: ``newton_max0`` from the network's
-        estimate when ``net`` is given, after ``net_passes`` passes and
-        ``net_newton`` Newton steps; else from the plain start), each step at
-        most ``newton_cap`` A and only where the force Jacobian is stable; a
-        step below ``newton_tol`` A means converged and FIRE starts there,
-        otherwise FIRE starts from the continuation start (counters()[2]).
-        Every ``newton_retry`` FIRE iterations Newton is tried again from
-        FIRE's position.  FIRE ends every step by the plan's stop rule.
-        ``newton_max0`` defaults to 4 with a network, 12 without."""
+        estimate after ``net_passes`` passes when ``net`` is given, else from
+        the plain start), each step at most ``newton_cap`` A and only where
+        the force Jacobian is stable; a step below ``newton_tol`` A means
+        converged and FIRE starts there, otherwise FIRE starts from the
+        continuation start (counters()[2]).  Every ``newton_retry`` FIRE
+        iterations Newton is tried again from FIRE's position.  FIRE ends
+        every step by the plan's stop rule.  ``newton_max0`` defaults to 6
+        with a network, 12 without."""
         if newton_max0 is None:
-            newton_max0 = 4 if net is not None else 12
-        p = _lib.WarmParams(int(bool(enable)), int(net_passes), int(net_newton), int(newton_max), int(newton_max0),
-                            int(newton_retry), float(newton_tol), float(newton_cap))
+            newton_max0 = 6 if net is not None else 12
+        p = _lib.WarmParams(int(bool(enable)), int(net_passes), int(newton_max), int(newton_max0), int(newton_retry),
+                            float(newton_tol), float(newton_cap))
         self._ws = net if enable else None
         check(lib().ppafm_batch_set_warmstart(self._h, ctypes.byref(p), net._h if (net is not None and enable) else None),
               "batch_set_warmstart")

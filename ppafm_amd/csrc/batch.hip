@@ -201,8 +201,7 @@ int enqueue_predict(ppafm_batch &b, int nitems = -1)
     if (!b.warm.enable || !b.ws) return 0;
     RelaxBatchLaunch L = base_launch(b);
     if (nitems >= 0) L.nitems = nitems;
-    return launch_ws_predict(ws_view(b.ws), L, b.d_wprefix, b.wprefix[L.nitems], b.pred, b.warm.net_passes,
-                             b.warm.net_newton, b.st);
+    return launch_ws_predict(ws_view(b.ws), L, b.d_wprefix, b.wprefix[L.nitems], b.pred, b.warm.net_passes, b.st);
 }
 
 int enqueue_df(ppafm_batch &b)
@@ -592,7 +591,6 @@ int ppafm_batch_set_warmstart(ppafm_batch *b, const ppafm_warm_params *p, ppafm_
         PP_REQUIRE(p->newton_tol > 0 && p->newton_tol < 1, "newton_tol must be in (0, 1) A");
         PP_REQUIRE(p->newton_cap > 0 && p->newton_cap <= 1, "newton_cap must be in (0, 1] A");
         PP_REQUIRE(!m || (p->net_passes >= 1 && p->net_passes <= 8), "net_passes must be 1..8");
-        PP_REQUIRE(!m || (p->net_newton >= 0 && p->net_newton <= 4), "net_newton must be 0..4");
     }
     PP_TRY_HIP(hipStreamSynchronize(b->st));
     if (p->enable && m && !b->pred) {

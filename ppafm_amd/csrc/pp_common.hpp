@@ -146,10 +146,9 @@ struct WsNetView {
 WsNetView ws_view(const ppafm_ws_model *m);
 // Step-0 estimates of every pixel column of a batch: pred (columns, 3),
 // item k's columns from items[k].col_off; wave_prefix[k] = 64-column groups
-// of items < k; reps network passes, then `newton` Newton steps on the
-// interpolated field.
+// of items < k (nwaves in all); reps network passes.
 int launch_ws_predict(const WsNetView &net, const RelaxBatchLaunch &L, const long long *wave_prefix, long long nwaves,
-                      float *pred, int reps, int newton, hipStream_t st);
+                      float *pred, int reps, hipStream_t st);
 // Training pairs at the selected positions of a relaxed batch: patch around
 // the continuation start (from pos) moved by a pseudo-random offset of up to
 // jitter A (half along z), target = truth - patch node (in 0.1 A).

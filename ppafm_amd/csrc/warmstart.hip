@@ -559,59 +559,50 @@ __global__ __launch_bounds__(256) void ws_samples_kernel(DevArgs A0, const Batch
     targets[3 * q + 2] = (float)(((double)truth[3 * g + 2] - nz_) / kWsScale);
 }
 
-// Step-0 estimates: one block of 256 threads per 64 consecutive pixel
-// columns of one item (wave_prefix[k] = such groups of items < k); output
-// pred[col_off + column].  Per network pass:
+// Step-0 estimates, persistent: each block of 256 threads stages the
+// weights once and then takes groups of 64 consecutive pixel columns of one
+// item (wave_prefix[k] = such groups of items < k; ngroups in all) until none
+// are left; output pred[col_off + column].  Per group and network pass:
 //   wave 0, lane l     nearest grid node of column l's estimate (first pass:
 //                      the plain start, tip + probe offset);
 //   all 4 waves        the 75 patch nodes of every column, split 4 ways,
 //                      into LDS (bf16), per-component RMS scaling;
-//   wave w             layers 1-2 on the matrix cores for columns 16w..16w+15;
-//   wave 0, lane l     layer 3: the new estimate.
-// Then `newton` Newton steps on the interpolated field (wave 0).
+//   wave w             layers 1-3 for columns 16w..16w+15 (layers 1-2 on the
+//                      matrix cores); only this wave reads what it writes,
+//                      so a wave-scope fence orders it, not a block barrier;
+// four block barriers per pass.  The relaxation's step-0 Newton run refines
+// the estimate.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <typename T4, int MODE>
 __global__ __launch_bounds__(256) void ws_predict_kernel(DevArgs A0, Net net, const BatchRelaxItem *__restrict__ items,
                                                          const long long *__restrict__ wave_prefix, int nitems,
-                                                         const T4 *__restrict__ grids, float *__restrict__ pred, int reps,
-                                                         int newton)
+                                                         long long ngroups, const T4 *__restrict__ grids,
+                                                         float *__restrict__ pred, int reps)
 {
     // patch rows of 226 bf16 (113 dwords, odd: consecutive rows start in
     // different LDS banks)
     constexpr int kRow = 226;
     __shared__ __align__(16) unsigned short Pl[64 * kRow];
-    __shared__ __align__(16) unsigned short H1l[64 * 40];
+    __shared__ __align__(16) unsigned short H1l[64 * 40];  // rows 16w.. belong to wave w
+    __shared__ float H2l[64 * 33];                         // rows 16w.. belong to wave w
     __shared__ __align__(16) unsigned short W1l[kN1 * kK1p];
     __shared__ __align__(16) unsigned short W2l[kN2 * kK2];
     __shared__ float W3l[kN3 * kN2];  // bf16-rounded
     __shared__ float red[4][64][3];
     __shared__ int nodes[64][3];
-    __shared__ float toff[64][3];  // centre node - tip, FP32
-    float *H2l = (float *)Pl;      // the patches are dead once layer 1 is done
-    static_assert(64 * 33 * 4 <= 64 * kRow * 2, "H2 fits in the patch buffer");
+    __shared__ float toff[64][3];     // centre node - tip, FP32
+    __shared__ double cnode[64][3];   // centre node position
+    __shared__ double est[64][3];     // the estimates
     const int t = threadIdx.x;
     const int w = t >> 6, l = t & 63;
-    const long long grp = blockIdx.x;
-    const int m = search_prefix(wave_prefix, nitems, grp);
-    const BatchRelaxItem &it = items[m];
-    DevArgs A = A0;
-    item_args(A, it);
-    const long long ncol = (long long)it.nx * it.ny;
-    const long long lc = (grp - wave_prefix[m]) * 64 + l;
-    const bool live = lc < ncol;
-    const int i = live ? (int)(lc / it.ny) : 0, j = live ? (int)(lc % it.ny) : 0;
-    const double bx = it.p0[0] + i * it.da[0] + j * it.db[0];
-    const double by = it.p0[1] + i * it.da[1] + j * it.db[1];
-    const double bz = it.p0[2] + i * it.da[2] + j * it.db[2];
-    double ox, oy, oz;
-    probe_offset<MODE>(A, ox, oy, oz);
-    const T4 *grid = grids + it.grid_off;
     const float *W3 = net.theta + kOffW3;
-    float dc[9], R[9], ks[3], d0[3];
-    for (int k = 0; k < 9; k++) { dc[k] = (float)it.dcell[k]; R[k] = (float)A.rot[k]; }
-    for (int k = 0; k < 3; k++) { ks[k] = (float)A.ks[k]; d0[k] = (float)A.d0[k]; }
-    const float krad = (float)A.krad, lrad = (float)A.lrad, surf = MODE == 1 ? (float)A.surf_ff : 0.f;
-    // patch nodes of this thread: q in [q0, q1) of the 75 (x, y, z) nodes
-    const int q0 = (75 * w) / 4, q1 = (75 * (w + 1)) / 4;
+    const int q0 = (75 * w) / 4, q1 = (75 * (w + 1)) / 4;  // this wave's patch nodes
     for (int k = t; k < kN1 * kK1p / 8; k += 256) ((uint4 *)W1l)[k] = ((const uint4 *)net.W1b)[k];
     for (int k = t; k < kN2 * kK2 / 8; k += 256) ((uint4 *)W2l)[k] = ((const uint4 *)net.W2b)[k];
     for (int k = t; k < kN3 * kN2; k += 256) W3l[k] = bfr(W3[k]);
@@ -626,27 +617,42 @@ __global__ __launch_bounds__(256) void ws_predict_kernel(DevArgs A0, Net net, co
     short toffs[3][8];
     for (int q = 0; q < 3; q++)
         for (int e = 0; e < 8; e++) toffs[q][e] = (short)tap_off(32 * q + 8 * (l >> 4) + e);
-    __syncthreads();
-    // step 0 of every column: the plain start, then the network `reps` times,
-    // each pass on the patch centred at the last estimate
-    double rx = 0, ry = 0, rz = 0;  // wave 0: the estimate
-    double nx_ = 0, ny_ = 0, nz_ = 0;
-    const double tx = bx, ty = by, tz = bz;
-    if (w == 0) { rx = tx + ox; ry = ty + oy; rz = tz + oz; }
-    {
+    double ox, oy, oz;
+    probe_offset<MODE>(A0, ox, oy, oz);
+
+    for (long long grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+        const int m = search_prefix(wave_prefix, nitems, grp);
+        const BatchRelaxItem &it = items[m];
+        DevArgs A = A0;
+        item_args(A, it);
+        const long long ncol = (long long)it.nx * it.ny;
+        const long long lc = (grp - wave_prefix[m]) * 64 + l;  // column l of the group
+        const bool live = lc < ncol;
+        const int i = live ? (int)(lc / it.ny) : 0, j = live ? (int)(lc % it.ny) : 0;
+        const double tx = it.p0[0] + i * it.da[0] + j * it.db[0];
+        const double ty = it.p0[1] + i * it.da[1] + j * it.db[1];
+        const double tz = it.p0[2] + i * it.da[2] + j * it.db[2];
+        const T4 *grid = grids + it.grid_off;
+        float dc[9], R[9], ks[3], d0[3];
+        for (int k = 0; k < 9; k++) { dc[k] = (float)it.dcell[k]; R[k] = (float)A.rot[k]; }
+        for (int k = 0; k < 3; k++) { ks[k] = (float)A.ks[k]; d0[k] = (float)A.d0[k]; }
+        const float krad = (float)A.krad, lrad = (float)A.lrad, surf = MODE == 1 ? (float)A.surf_ff : 0.f;
+        __syncthreads();  // the previous group is done with every LDS buffer
+        if (w == 0) { est[l][0] = tx + ox; est[l][1] = ty + oy; est[l][2] = tz + oz; }
         for (int rep = 0; rep < reps; rep++) {
             // ---- centre node of every column (wave 0)
             if (w == 0) {
                 double gx, gy, gz;
-                to_grid(A, rx, ry, rz, gx, gy, gz);
+                to_grid(A, est[l][0], est[l][1], est[l][2], gx, gy, gz);
                 const int ix = (int)floor(gx + 0.5), iy = (int)floor(gy + 0.5), iz = (int)floor(gz + 0.5);
-                nx_ = ix * it.dcell[0] + iy * it.dcell[3] + iz * it.dcell[6];
-                ny_ = ix * it.dcell[1] + iy * it.dcell[4] + iz * it.dcell[7];
-                nz_ = ix * it.dcell[2] + iy * it.dcell[5] + iz * it.dcell[8];
+                const double nx_ = ix * it.dcell[0] + iy * it.dcell[3] + iz * it.dcell[6];
+                const double ny_ = ix * it.dcell[1] + iy * it.dcell[4] + iz * it.dcell[7];
+                const double nz_ = ix * it.dcell[2] + iy * it.dcell[5] + iz * it.dcell[8];
+                cnode[l][0] = nx_; cnode[l][1] = ny_; cnode[l][2] = nz_;
                 nodes[l][0] = wrap_idx(ix, A.n0); nodes[l][1] = wrap_idx(iy, A.n1); nodes[l][2] = wrap_idx(iz, A.n2);
                 toff[l][0] = (float)(nx_ - tx); toff[l][1] = (float)(ny_ - ty); toff[l][2] = (float)(nz_ - tz);
             }
-            __syncthreads();
+            __syncthreads();  // (1) nodes
             // ---- patch nodes, split over the 4 waves
             {
                 const int ix = nodes[l][0], iy = nodes[l][1], iz = nodes[l][2];  // wrapped into the grid
@@ -699,7 +705,7 @@ __global__ __launch_bounds__(256) void ws_predict_kernel(DevArgs A0, Net net, co
                 }
                 for (int c = 0; c < 3; c++) red[w][l][c] = ss[c];
             }
-            __syncthreads();
+            __syncthreads();  // (2) patch rows and partial sums
             {
                 float inv[3];
                 for (int c = 0; c < 3; c++) {
@@ -712,7 +718,7 @@ __global__ __launch_bounds__(256) void ws_predict_kernel(DevArgs A0, Net net, co
 #pragma unroll
                     for (int c = 0; c < 3; c++) row[3 * q + c] = f2bf(bf2f(row[3 * q + c]) * inv[c]);
             }
-            __syncthreads();
+            __syncthreads();  // (3) normalised rows
             // ---- layers 1 and 2: wave w takes columns 16w .. 16w+15
             f32x4 acc2[2] = {f32x4{0.f, 0.f, 0.f, 0.f}, f32x4{0.f, 0.f, 0.f, 0.f}};
             for (int p = 0; p < 9; p++) {
@@ -733,6 +739,7 @@ __global__ __launch_bounds__(256) void ws_predict_kernel(DevArgs A0, Net net, co
                     for (int tt = 0; tt < 2; tt++)
                         acc1[tt] = mfma(a, ld8(W1l + (16 * tt + (l & 15)) * kK1p + k0), acc1[tt]);
                 }
+                wave_sync();  // this wave's reads of H1l (previous p) are done
 #pragma unroll
                 for (int tt = 0; tt < 2; tt++) {
                     const int jj = 16 * tt + (l & 15);
@@ -741,12 +748,11 @@ __global__ __launch_bounds__(256) void ws_predict_kernel(DevArgs A0, Net net, co
                     for (int ii = 0; ii < 4; ii++)
                         H1l[(16 * w + 4 * (l >> 4) + ii) * 40 + jj] = f2bf(silu(acc1[tt][ii] + bj));
                 }
-                __syncthreads();
+                wave_sync();  // H1l rows of this wave written
                 const bf16x8 a2 = ld8(H1l + (16 * w + (l & 15)) * 40 + 8 * (l >> 4));
 #pragma unroll
                 for (int tt = 0; tt < 2; tt++)
                     acc2[tt] = mfma(a2, ld8(W2l + (16 * tt + (l & 15)) * kK2 + p * 32 + 8 * (l >> 4)), acc2[tt]);
-                __syncthreads();
             }
 #pragma unroll
             for (int tt = 0; tt < 2; tt++) {
@@ -755,32 +761,18 @@ __global__ __launch_bounds__(256) void ws_predict_kernel(DevArgs A0, Net net, co
 #pragma unroll
                 for (int ii = 0; ii < 4; ii++) H2l[(16 * w + 4 * (l >> 4) + ii) * 33 + jj] = silu(acc2[tt][ii] + bj);
             }
-            __syncthreads();
-            // ---- layer 3 and the new estimate (wave 0)
-            if (w == 0) {
-                float o3[kN3];
-#pragma unroll
-                for (int mm = 0; mm < kN3; mm++) {
-                    float a = net.theta[kOffB3 + mm];
-                    for (int jj = 0; jj < kN2; jj++) a = fmaf(bfr(H2l[l * 33 + jj]), W3l[mm * kN2 + jj], a);
-                    o3[mm] = a;
-                }
-                rx = nx_ + kWsScale * (double)o3[0];
-                ry = ny_ + kWsScale * (double)o3[1];
-                rz = nz_ + kWsScale * (double)o3[2];
+            wave_sync();  // H2l rows of this wave written
+            // ---- layer 3: lane l < 48 of wave w gives output l / 16 of column 16w + l % 16
+            if (l < 48) {
+                const int c = 16 * w + (l & 15), mm = l >> 4;
+                float a = net.theta[kOffB3 + mm];
+                for (int jj = 0; jj < kN2; jj++) a = fmaf(bfr(H2l[c * 33 + jj]), W3l[mm * kN2 + jj], a);
+                est[c][mm] = cnode[c][mm] + kWsScale * (double)a;
             }
-            __syncthreads();
+            __syncthreads();  // (4) estimates
         }
         if (w == 0 && live) {
-            // no Newton step from a point of unstable curvature: it would
-            // head for a saddle; FIRE descends from the estimate instead
-            for (int k = 0; k < newton && isfinite(rx + ry + rz); k++) {
-                const double px = rx, py = ry, pz = rz;
-                bool stable;
-                const bool moved = newton_step<T4, MODE>(A, grid, tx, ty, tz, rx, ry, rz, 0.1, stable);
-                if (!stable) { rx = px; ry = py; rz = pz; break; }
-                if (!moved) break;
-            }
+            double rx = est[l][0], ry = est[l][1], rz = est[l][2];
             if (!(isfinite(rx) && isfinite(ry) && isfinite(rz))) { rx = tx + ox; ry = ty + oy; rz = tz + oz; }
             float *pp = pred + 3 * ((long long)it.col_off + lc);
             pp[0] = (float)rx; pp[1] = (float)ry; pp[2] = (float)rz;
@@ -804,23 +796,26 @@ static void shared_args(DevArgs &A, const ppafm_relax_params &rp, const ppafm_ti
 }  // namespace
 
 int launch_ws_predict(const WsNetView &nv, const RelaxBatchLaunch &L, const long long *wave_prefix, long long nwaves,
-                      float *pred, int reps, int newton, hipStream_t st)
+                      float *pred, int reps, hipStream_t st)
 {
     PP_REQUIRE(reps >= 1 && reps <= 8, "network passes must be 1..8");
-    PP_REQUIRE(newton >= 0 && newton <= 4, "Newton steps must be 0..4");
     PP_REQUIRE(L.rp.mode == 0 || L.rp.mode == 1, "relax mode must be 0 or 1");
-    PP_REQUIRE(nwaves >= 0 && nwaves < (1LL << 31), "too many columns");
+    PP_REQUIRE(nwaves >= 0, "negative column count");
     if (nwaves == 0) return 0;
     DevArgs A;
     shared_args(A, L.rp, L.tip);
     const Net net{nv.theta, nv.W1b, nv.W2b, nullptr};
-    dim3 grd((unsigned)nwaves), blk(256);
+    int dev = 0, ncu = 0;
+    PP_TRY_HIP(hipGetDevice(&dev));
+    PP_TRY_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const long long nblk = std::min<long long>(nwaves, 2LL * std::max(ncu, 1));  // persistent: 2 blocks per CU
+    dim3 grd((unsigned)nblk), blk(256);
     if (L.f64) {
-        if (L.rp.mode == 0) hipLaunchKernelGGL((ws_predict_kernel<double4, 0>), grd, blk, 0, st, A, net, L.items, wave_prefix, L.nitems, (const double4 *)L.grids, pred, reps, newton);
-        else hipLaunchKernelGGL((ws_predict_kernel<double4, 1>), grd, blk, 0, st, A, net, L.items, wave_prefix, L.nitems, (const double4 *)L.grids, pred, reps, newton);
+        if (L.rp.mode == 0) hipLaunchKernelGGL((ws_predict_kernel<double4, 0>), grd, blk, 0, st, A, net, L.items, wave_prefix, L.nitems, nwaves, (const double4 *)L.grids, pred, reps);
+        else hipLaunchKernelGGL((ws_predict_kernel<double4, 1>), grd, blk, 0, st, A, net, L.items, wave_prefix, L.nitems, nwaves, (const double4 *)L.grids, pred, reps);
     } else {
-        if (L.rp.mode == 0) hipLaunchKernelGGL((ws_predict_kernel<float4, 0>), grd, blk, 0, st, A, net, L.items, wave_prefix, L.nitems, (const float4 *)L.grids, pred, reps, newton);
-        else hipLaunchKernelGGL((ws_predict_kernel<float4, 1>), grd, blk, 0, st, A, net, L.items, wave_prefix, L.nitems, (const float4 *)L.grids, pred, reps, newton);
+        if (L.rp.mode == 0) hipLaunchKernelGGL((ws_predict_kernel<float4, 0>), grd, blk, 0, st, A, net, L.items, wave_prefix, L.nitems, nwaves, (const float4 *)L.grids, pred, reps);
+        else hipLaunchKernelGGL((ws_predict_kernel<float4, 1>), grd, blk, 0, st, A, net, L.items, wave_prefix, L.nitems, nwaves, (const float4 *)L.grids, pred, reps);
     }
     return 0;
 }

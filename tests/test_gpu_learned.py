@@ -87,7 +87,7 @@ def test_warm_path_matches_fp64_oracle(setup, with_net):
         b.synchronize()
         rg = b.positions_raw(k).astype(np.float64)
         starts0 = b.predictions(k).astype(np.float64) if with_net else None
-        kw = dict(starts0=starts0, newton_max0=4 if with_net else 12)
+        kw = dict(starts0=starts0, newton_max0=6 if with_net else 12)
         rw_g = O.relax_strokes_warm(cfg["n"], lvec[1:], Fg, tip_params(), tips, **kw)[0]
         rw_o = O.relax_strokes_warm(cfg["n"], lvec[1:], FF, tip_params(), tips, **kw)[0]
         dg = np.linalg.norm(rg - rw_g, axis=-1)
