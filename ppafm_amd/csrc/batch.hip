@@ -96,7 +96,7 @@ struct ppafm_batch {
     ppafm_warm_params warm{};
     ppafm_ws_model *ws = nullptr;
     float *pred = nullptr;           // step-0 estimates (pixel columns, 3)
-    long long *d_wprefix = nullptr;  // 64-column groups of items < k
+    long long *d_wprefix = nullptr;  // 8x8-column tiles of items < k (step-0 estimates)
     std::vector<long long> wprefix;  // host copy
     long long nwaves = 0;
     long long *d_pprefix = nullptr;  // positions of items < k
@@ -599,7 +599,8 @@ int ppafm_batch_set_warmstart(ppafm_batch *b, const ppafm_warm_params *p, ppafm_
         for (int k = 0; k < b->nitems; k++) {
             const ppafm_batch_item &it = b->items[k];
             const long long nc = it.scan.nz > 0 ? (long long)it.scan.nx * it.scan.ny : 0;
-            wp[k + 1] = wp[k] + (nc + 63) / 64;
+            const long long tiles = nc > 0 ? (long long)((it.scan.nx + 7) / 8) * ((it.scan.ny + 7) / 8) : 0;
+            wp[k + 1] = wp[k] + tiles;  // 8x8 pixel tiles of the step-0 estimates
             ncol += nc;
         }
         PP_REQUIRE(ncol < (1LL << 31), "too many pixel columns for warm starts");

@@ -145,7 +145,7 @@ struct WsNetView {
 };
 WsNetView ws_view(const ppafm_ws_model *m);
 // Step-0 estimates of every pixel column of a batch: pred (columns, 3),
-// item k's columns from items[k].col_off; wave_prefix[k] = 64-column groups
+// item k's columns from items[k].col_off; wave_prefix[k] = 8x8-column tiles
 // of items < k (nwaves in all); reps network passes.
 int launch_ws_predict(const WsNetView &net, const RelaxBatchLaunch &L, const long long *wave_prefix, long long nwaves,
                       float *pred, int reps, hipStream_t st);

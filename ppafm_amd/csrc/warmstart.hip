@@ -560,24 +560,40 @@ __global__ __launch_bounds__(256) void ws_samples_kernel(DevArgs A0, const Batch
 }
 
 // Step-0 estimates, persistent: each block of 256 threads stages the
-// weights once and then takes groups of 64 consecutive pixel columns of one
-// item (wave_prefix[k] = such groups of items < k; ngroups in all) until none
-// are left; output pred[col_off + column].  Per group and network pass:
+// weights once and then takes groups of 8x8 pixel columns (a tile of one
+// item; wave_prefix[k] = tiles of items < k, ngroups in all) until none are
+// left; output pred[col_off + i*ny + j].  Per tile and network pass:
 //   wave 0, lane l     nearest grid node of column l's estimate (first pass:
-//                      the plain start, tip + probe offset);
+//                      the plain start, tip + probe offset), and the box of
+//                      grid nodes that holds the tile's 64 patches;
+//   all 4 waves        that box, loaded once into LDS (coalesced along z;
+//                      direct gathers when it is larger than kBoxCap);
 //   all 4 waves        the 75 patch nodes of every column, split 4 ways,
-//                      into LDS (bf16), per-component RMS scaling;
+//                      from the box into LDS rows (bf16), per-component RMS
+//                      scaling;
 //   wave w             layers 1-3 for columns 16w..16w+15 (layers 1-2 on the
 //                      matrix cores); only this wave reads what it writes,
-//                      so a wave-scope fence orders it, not a block barrier;
-// four block barriers per pass.  The relaxation's step-0 Newton run refines
-// the estimate.
+//                      so a wave-scope fence orders it, not a block barrier.
+// The relaxation's step-0 Newton run refines the estimate.
 __device__ __forceinline__ void wave_sync()
 {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
+
+__device__ __forceinline__ int wave_min_i(int v)
+{
+    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_max_i(int v)
+{
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+constexpr int kBoxCap = 1024;  // grid nodes of a staged box (x, y, z floats)
 
 template <typename T4, int MODE>
 __global__ __launch_bounds__(256) void ws_predict_kernel(DevArgs A0, Net net, const BatchRelaxItem *__restrict__ items,
@@ -589,8 +605,14 @@ __global__ __launch_bounds__(256) void ws_predict_kernel(DevArgs A0, Net net, co
     // different LDS banks)
     constexpr int kRow = 226;
     __shared__ __align__(16) unsigned short Pl[64 * kRow];
-    __shared__ __align__(16) unsigned short H1l[64 * 40];  // rows 16w.. belong to wave w
-    __shared__ float H2l[64 * 33];                         // rows 16w.. belong to wave w
+    // the staged box (3 planes of kBoxCap floats) is dead once the patch rows
+    // are built; layers 1-2 then reuse its memory for H1l and H2l
+    constexpr int kUnion = 3 * kBoxCap * 4 > 64 * 40 * 2 + 64 * 33 * 4 ? 3 * kBoxCap * 4 : 64 * 40 * 2 + 64 * 33 * 4;
+    __shared__ __align__(16) unsigned char U[kUnion];
+    unsigned short *H1l = (unsigned short *)U;            // rows 16w.. belong to wave w
+    float *H2l = (float *)(U + 64 * 40 * 2);              // rows 16w.. belong to wave w
+    float *Bx = (float *)U, *By = Bx + kBoxCap, *Bz = By + kBoxCap;
+    __shared__ int box[6];                                // lo x, y, z; extent x, y, z (0: not staged)
     __shared__ __align__(16) unsigned short W1l[kN1 * kK1p];
     __shared__ __align__(16) unsigned short W2l[kN2 * kK2];
     __shared__ float W3l[kN3 * kN2];  // bf16-rounded
@@ -620,15 +642,24 @@ __global__ __launch_bounds__(256) void ws_predict_kernel(DevArgs A0, Net net, co
     double ox, oy, oz;
     probe_offset<MODE>(A0, ox, oy, oz);
 
-    for (long long grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
-        const int m = search_prefix(wave_prefix, nitems, grp);
+    // a contiguous range of tiles per block: consecutive tiles mostly share
+    // their item, which is then looked up once
+    const long long per = (ngroups + gridDim.x - 1) / gridDim.x;
+    const long long g0 = (long long)blockIdx.x * per, g1 = min(ngroups, g0 + per);
+    int m = g0 < g1 ? search_prefix(wave_prefix, nitems, g0) : 0;
+    long long mend = g0 < g1 ? wave_prefix[m + 1] : 0;
+    for (long long grp = g0; grp < g1; grp++) {
+        while (grp >= mend) {  // next item with tiles (items without any have equal prefixes)
+            m++;
+            mend = wave_prefix[m + 1];
+        }
         const BatchRelaxItem &it = items[m];
         DevArgs A = A0;
         item_args(A, it);
-        const long long ncol = (long long)it.nx * it.ny;
-        const long long lc = (grp - wave_prefix[m]) * 64 + l;  // column l of the group
-        const bool live = lc < ncol;
-        const int i = live ? (int)(lc / it.ny) : 0, j = live ? (int)(lc % it.ny) : 0;
+        const int tiles_y = (it.ny + 7) / 8;
+        const int tile = (int)(grp - wave_prefix[m]);
+        const int i = (tile / tiles_y) * 8 + (l >> 3), j = (tile % tiles_y) * 8 + (l & 7);  // column l of the tile
+        const bool live = i < it.nx && j < it.ny;
         const double tx = it.p0[0] + i * it.da[0] + j * it.db[0];
         const double ty = it.p0[1] + i * it.da[1] + j * it.db[1];
         const double tz = it.p0[2] + i * it.da[2] + j * it.db[2];
@@ -651,11 +682,41 @@ __global__ __launch_bounds__(256) void ws_predict_kernel(DevArgs A0, Net net, co
                 cnode[l][0] = nx_; cnode[l][1] = ny_; cnode[l][2] = nz_;
                 nodes[l][0] = wrap_idx(ix, A.n0); nodes[l][1] = wrap_idx(iy, A.n1); nodes[l][2] = wrap_idx(iz, A.n2);
                 toff[l][0] = (float)(nx_ - tx); toff[l][1] = (float)(ny_ - ty); toff[l][2] = (float)(nz_ - tz);
+                // box of the live columns' patches (unwrapped node indices)
+                const int big = 1 << 30;
+                const int lx = wave_min_i(live ? ix - 2 : big), hx = wave_max_i(live ? ix + 2 : -big);
+                const int ly = wave_min_i(live ? iy - 2 : big), hy = wave_max_i(live ? iy + 2 : -big);
+                const int lz = wave_min_i(live ? iz - 1 : big), hz = wave_max_i(live ? iz + 1 : -big);
+                if (l == 0) {
+                    const long ex = (long)hx - lx + 1, ey = (long)hy - ly + 1, ez = (long)hz - lz + 1;
+                    const bool fits = ex > 0 && ey > 0 && ez > 0 && ex <= A.n0 && ey <= A.n1 && ez <= A.n2 &&
+                                      ex * ey * ez <= kBoxCap;
+                    box[0] = lx; box[1] = ly; box[2] = lz;
+                    box[3] = fits ? (int)ex : 0; box[4] = fits ? (int)ey : 0; box[5] = fits ? (int)ez : 0;
+                }
             }
-            __syncthreads();  // (1) nodes
+            __syncthreads();  // (1) nodes and the box
+            const int bx0 = box[0], by0 = box[1], bz0 = box[2], ex = box[3], ey = box[4], ez = box[5];
+            const bool staged = ex > 0;
+            if (staged) {
+                const int vol = ex * ey * ez;
+                for (int q = t; q < vol; q += 256) {
+                    const int cz = q % ez, r = q / ez;
+                    const int cy = r % ey, cx = r / ey;
+                    const T4 v = grid[((unsigned)wrap_idx(bx0 + cx, A.n0) * A.n1 + wrap_idx(by0 + cy, A.n1)) * A.n2 +
+                                      wrap_idx(bz0 + cz, A.n2)];
+                    Bx[q] = (float)v.x; By[q] = (float)v.y; Bz[q] = (float)v.z;
+                }
+                __syncthreads();  // (1b) the box
+            }
             // ---- patch nodes, split over the 4 waves
             {
                 const int ix = nodes[l][0], iy = nodes[l][1], iz = nodes[l][2];  // wrapped into the grid
+                // this column's patch origin inside the box (the box's lower
+                // corner wraps to the same node as the column's, modulo n)
+                const int ox_ = staged ? wrap_idx(ix - 2 - bx0, A.n0) : 0;
+                const int oy_ = staged ? wrap_idx(iy - 2 - by0, A.n1) : 0;
+                const int oz_ = staged ? wrap_idx(iz - 1 - bz0, A.n2) : 0;
                 const float ex0 = toff[l][0], ey0 = toff[l][1], ez0 = toff[l][2];
                 unsigned short *row = Pl + l * kRow;
                 float ss[3] = {0.f, 0.f, 0.f};
@@ -667,12 +728,19 @@ __global__ __launch_bounds__(256) void ws_predict_kernel(DevArgs A0, Net net, co
                         const int q = q0 + k0 + k;
                         if (k0 + k < 19 && q < q1) {
                             const int dx = (nd[k0 + k] & 15) - 2, dy = ((nd[k0 + k] >> 4) & 15) - 2, dz = (nd[k0 + k] >> 8) - 1;
-                            int jx = ix + dx, jy = iy + dy, jz = iz + dz;  // |d| <= 2 < n: one wrap at most
-                            jx += jx < 0 ? A.n0 : (jx >= A.n0 ? -A.n0 : 0);
-                            jy += jy < 0 ? A.n1 : (jy >= A.n1 ? -A.n1 : 0);
-                            jz += jz < 0 ? A.n2 : (jz >= A.n2 ? -A.n2 : 0);
-                            const T4 *pv = grid + ((unsigned)jx * A.n1 + jy) * A.n2 + jz;
-                            vx[k] = (float)pv->x; vy[k] = (float)pv->y; vz[k] = (float)pv->z;
+                            if (!live) {
+                                vx[k] = vy[k] = vz[k] = 0.f;  // a column off the scan (edge tile)
+                            } else if (staged) {
+                                const int bq = ((ox_ + dx + 2) * ey + (oy_ + dy + 2)) * ez + (oz_ + dz + 1);
+                                vx[k] = Bx[bq]; vy[k] = By[bq]; vz[k] = Bz[bq];
+                            } else {
+                                int jx = ix + dx, jy = iy + dy, jz = iz + dz;  // |d| <= 2 < n: one wrap at most
+                                jx += jx < 0 ? A.n0 : (jx >= A.n0 ? -A.n0 : 0);
+                                jy += jy < 0 ? A.n1 : (jy >= A.n1 ? -A.n1 : 0);
+                                jz += jz < 0 ? A.n2 : (jz >= A.n2 ? -A.n2 : 0);
+                                const T4 *pv = grid + ((unsigned)jx * A.n1 + jy) * A.n2 + jz;
+                                vx[k] = (float)pv->x; vy[k] = (float)pv->y; vz[k] = (float)pv->z;
+                            }
                         }
                     }
 #pragma unroll
@@ -774,7 +842,7 @@ __global__ __launch_bounds__(256) void ws_predict_kernel(DevArgs A0, Net net, co
         if (w == 0 && live) {
             double rx = est[l][0], ry = est[l][1], rz = est[l][2];
             if (!(isfinite(rx) && isfinite(ry) && isfinite(rz))) { rx = tx + ox; ry = ty + oy; rz = tz + oz; }
-            float *pp = pred + 3 * ((long long)it.col_off + lc);
+            float *pp = pred + 3 * ((long long)it.col_off + (long long)i * it.ny + j);
             pp[0] = (float)rx; pp[1] = (float)ry; pp[2] = (float)rz;
         }
     }
