@@ -125,6 +125,63 @@ def run_gpu(args, cfg, xyz, Zs, qs, lvec):
     return dict(scan=s, t_step=t_total, t_grid=t_grid, t_relax=t_relax, t_df=t_df, wall_step_ms=1e3 * wall / args.steps, n_eval=nev, n_pos=npos, mean_iters=float(iters.mean()))
 
 
+def run_stiffness(s, reps=3):
+    """stiffnessMatrix (which = 3, ddisp 0.05) at every relaxed position of
+    the scan, on its grid; HIP events around the launches."""
+    from ppafm_amd import device as D
+
+    outs = D.stiffness(s.n, s.lvec[1:], s.grid, s.tip, s.scan_d, s.pos, which=3, ddisp=0.05)
+    D.synchronize()
+    e0, e1 = D.Event(), D.Event()
+    e0.record()
+    for _ in range(reps):
+        D.stiffness(s.n, s.lvec[1:], s.grid, s.tip, s.scan_d, s.pos, which=3, ddisp=0.05, out=outs)
+    e1.record()
+    D.synchronize()
+    ms = e0.elapsed_ms(e1) / reps
+    evals = outs[0].download().reshape(-1, 3)
+    return dict(ms=ms, npos=len(evals), n_unstable=int((evals[:, 0] <= 0).sum()), n_nan=int(np.isnan(evals).any(axis=1).sum()),
+                positions=s.pos.download().astype(np.float64).reshape(-1, 3), evals=evals)
+
+
+def run_cpu_stiffness(cfg, lvec, grid_host, st, ncols=None):
+    """The reference's stiffnessMatrix (serial loop, ProbeParticle.cpp:703-750)
+    at the GPU's relaxed positions and on its FP64 grid: the whole scan
+    (about a second) or its first ncols pixel columns."""
+    O, threads = _cpu_env()
+    from ppafm_amd import common as C
+    from ppafm_amd import molgen as M
+
+    nz = cfg["nz"]
+    ntot = cfg["scan_n"][0] * cfg["scan_n"][1]
+    ncols = ntot if ncols is None else min(ncols, ntot)
+    tips = np.ascontiguousarray((M.scan_tips(cfg) - lvec[0]).reshape(-1, 3)[: ncols * nz])
+    rpps = np.ascontiguousarray(st["positions"][: ncols * nz])
+    FF = np.ascontiguousarray(grid_host[..., :3], dtype=np.float64)
+    kS = np.array([M.K_LAT, M.K_LAT, 0.0]) / -C.eVA_Nm
+    kind = "reference" if O.have_reference() else "port"
+    with quiet_stdout():
+        if kind == "reference":
+            ref = O.Reference()
+            ref.set_grid(np.array(cfg["n"], dtype=np.int32), lvec[1:], FF)
+            ref.set_tip(M.TIP_R0[2], M.K_RAD / -C.eVA_Nm, np.zeros(3), kS)
+            t0 = time.perf_counter()
+            ev, _ = ref.stiffness(tips, rpps, which=3, ddisp=0.05)
+            t = time.perf_counter() - t0
+        else:
+            tip = O.tip_params(M.TIP_R0[2], M.K_RAD / -C.eVA_Nm, (0.0, 0.0, 0.0), kS)
+            t0 = time.perf_counter()
+            ev, _, _ = O.stiffness(cfg["n"], lvec[1:], FF, tip, tips, rpps, which=3, ddisp=0.05)
+            t = time.perf_counter() - t0
+    m = len(tips)
+    dev = float(np.abs(ev - st["evals"][:m]).max() / np.abs(ev).max())
+    cores = 1 if kind == "reference" else threads
+    return dict(value=m / t, unit="positions/s", cores=cores, kind=kind,
+                sample=f"{ncols} of {ntot} pixel columns ({m} positions, {t:.2f} s); "
+                       f"the reference's loop is serial; CPU {cpu_model()}",
+                max_rel_eigenvalue_diff_vs_gpu=dev)
+
+
 def run_batch(args):
     """Batched workload: seeded molecules of U[8,120] atoms, each with its own
     128x128x160 grid and 128x128x30 scan, run as one captured graph (grid
@@ -576,8 +633,23 @@ def main():
     if args.c4_instances > 0:
         print(f"[bench] c4: {args.c4_instances} instances", file=sys.stderr, flush=True)
         out["c4"] = run_c4(args)
+    st = run_stiffness(g["scan"])
+    stiff_bytes = 6 * 8 * 4 * corner_bytes + 12 + 96
+    out["stiffness"] = {
+        "workload": "stiffnessMatrix (which 3, ddisp 0.05 A) at every relaxed position of the scan above, on its grid",
+        "positions": st["npos"],
+        "ms": st["ms"],
+        "positions_per_s": st["npos"] / (st["ms"] * 1e-3),
+        "algorithmic_gbs": stiff_bytes * st["npos"] / (st["ms"] * 1e-3) / 1e9,
+        "bytes_model": f"6 force evaluations x 8 corners x {4 * corner_bytes} B + 12 B position in + 96 B eigenvalues and eigenvectors out (FP64)",
+        "positions_with_non_positive_smallest_eigenvalue": st["n_unstable"],
+        "positions_with_nan": st["n_nan"],
+    }
     if not args.no_cpu:
         grid_host = g["scan"].grid_host()
+        cs = run_cpu_stiffness(cfg, lvec, grid_host, st)
+        out["stiffness"]["cpu_baseline"] = cs
+        out["stiffness"]["speedup_vs_cpu"] = out["stiffness"]["positions_per_s"] / cs["value"]
         c = run_cpu(args, cfg, xyz, Zs, qs, lvec, grid_host)
         out["cpu_baseline"] = {"value": c["value"], "unit": c["unit"], "cores": c["cores"], "kind": c["kind"], "sample": c["sample"], "grid_pts_per_s": c["grid_pts_per_s"]}
         out["speedup_vs_cpu"] = value / c["value"]

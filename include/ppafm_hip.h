@@ -71,6 +71,14 @@ int relaxTipStroke(int probeStart, int relaxAlg, int nstep, double *rTips, doubl
  * returns 0 like the reference (ppafm_last_itrsum() gives the sum). */
 int relaxTipStrokes_omp(int nx, int ny, int probeStart, int relaxAlg, int nstep, double *rTips, double *rs,
                         double *fs, ppafm_spline_params *splineParams);
+/* ProbeParticle.cpp:703  dynamical matrix of the probe at n relaxed positions
+ * rPPs for tip positions rTips (n,3): rows -(F(r+h e_k) - F(r-h e_k))/2h with
+ * h = ddisp, made symmetric; eigenvals (n,3) ascending; evec1/2/3 (n,3) the
+ * unit eigenvectors of the smallest/middle/largest eigenvalue, written for
+ * which >= 1/2/3 (may be NULL otherwise).  A NaN eigenvalue is reported
+ * (stderr, ppafm_last_error) where the reference exits the process. */
+void stiffnessMatrix(double ddisp, int which, int n, double *rTips, double *rPPs, double *eigenvals, double *evec1,
+                     double *evec2, double *evec3, ppafm_spline_params *splineParams);
 
 /* ------------------------------------------------------------------------
  * (2) Device ABI
@@ -181,6 +189,16 @@ typedef struct {
 
 int ppafm_relax(const ppafm_grid *grid, const void *grid_dev, const ppafm_relax_params *rp, const ppafm_tip *tip,
                 const ppafm_scan *scan, const double *tips_dev, const ppafm_relax_out *out, void *stream);
+
+/* stiffnessMatrix (ProbeParticle.cpp:703-750) on a device grid for the
+ * positions of a scan: pos_dev (nx,ny,nz,3) float32 (pos_f32) or float64 in
+ * stroke order -- the relaxation's pos output -- with tip positions from
+ * `scan` or tips_dev as for ppafm_relax, mode-0 tip (tip->rot unused).
+ * evals_dev (npos,3) float64 ascending; evec1/2/3_dev (npos,3) float64 for
+ * which >= 1/2/3.  NaN positions give NaN outputs.  No synchronisation. */
+int ppafm_stiffness(const ppafm_grid *grid, const void *grid_dev, const ppafm_tip *tip, const ppafm_scan *scan,
+                    const double *tips_dev, const void *pos_dev, int pos_f32, double ddisp, int which,
+                    double *evals_dev, double *evec1_dev, double *evec2_dev, double *evec3_dev, void *stream);
 
 /* Valid-mode correlation along z for ncol columns (convolveZ, relax.cl:571-608;
  * with reversed weights it is np.convolve(..., 'valid') of common.py:429):

@@ -86,6 +86,9 @@ def lib():
                                                                   ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                                                   ctypes.c_double, _dp, _dp, _ip, _ip]
         L.orc_relax_strokes_warm.restype = ctypes.c_long
+        L.orc_stiffness.argtypes = [_ip, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int, ctypes.c_long, _dp, _dp, _dp,
+                                    _dp, _dp, _dp, _dp]
+        L.orc_stiffness.restype = ctypes.c_long
         L.orc_num_threads.restype = ctypes.c_int
         _lib = L
     return _lib
@@ -190,6 +193,24 @@ def force_jac(n, cell, F, tip, rtips, rs):
     lib().orc_force_jac(_ptr(n, _ip), _ptr(_f64(cell, (3, 3))), _ptr(_f64(F)), _ptr(_f64(tip)), len(rs), _ptr(rtips),
                         _ptr(rs), _ptr(f), _ptr(J))
     return f, J
+
+
+def stiffness(n, cell, F, tip, rTips, rPPs, which=0, ddisp=0.05, want_dynmat=False):
+    """stiffnessMatrix (ProbeParticle.cpp:703-750) at positions rPPs (m,3)
+    for tips rTips (m,3): eigenvalues (m,3) ascending, the list of three
+    eigenvector arrays (the first ``which`` filled, others None), the number
+    of NaN points and, with want_dynmat, the symmetric matrices (m,3,3)."""
+    rTips = _f64(rTips, (-1, 3))
+    rPPs = _f64(rPPs, (-1, 3))
+    m = len(rTips)
+    ev = np.zeros((m, 3))
+    vecs = [np.zeros((m, 3)) if k < which else None for k in range(3)]
+    D = np.zeros((m, 3, 3)) if want_dynmat else None
+    n = np.ascontiguousarray(n, dtype=np.int32)
+    nnan = lib().orc_stiffness(_ptr(n, _ip), _ptr(_f64(cell, (3, 3))), _ptr(_f64(F)), _ptr(_f64(tip)), float(ddisp),
+                               int(which), m, _ptr(rTips), _ptr(rPPs), _ptr(ev), *[_ptr(v) for v in vecs], _ptr(D))
+    out = (ev, vecs, int(nnan))
+    return out + (D,) if want_dynmat else out
 
 
 def relax_strokes_warm(n, cell, F, tip, rTips, starts0=None, newton_max=8, newton_max0=12, retry=8, tol=1e-3,
@@ -357,6 +378,8 @@ class Reference:
         L.relaxTipStroke.restype = ctypes.c_int
         L.relaxTipStrokes_omp.argtypes = [ctypes.c_int] * 5 + [_dp, _dp, _dp, ctypes.c_void_p]
         L.relaxTipStrokes_omp.restype = ctypes.c_int
+        L.stiffnessMatrix.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_int] + [_dp] * 6 + [ctypes.c_void_p]
+        L.stiffnessMatrix.restype = None
         self.L = L
         self._keep = []
 
@@ -404,6 +427,17 @@ class Reference:
         fs = np.zeros_like(rTips)
         itr = self.L.relaxTipStroke(int(probeStart), int(relaxAlg), len(rTips), _ptr(rTips), _ptr(rs), _ptr(fs), None)
         return rs, fs, itr
+
+
+    def stiffness(self, rTips, rPPs, which=0, ddisp=0.05):
+        """stiffnessMatrix on the current grid and tip (rTips, rPPs (m,3))."""
+        rTips = _f64(rTips, (-1, 3))
+        rPPs = _f64(rPPs, (-1, 3))
+        m = len(rTips)
+        ev = np.zeros((m, 3))
+        vecs = [np.zeros((m, 3)) if k < which else None for k in range(3)]
+        self.L.stiffnessMatrix(float(ddisp), int(which), m, _ptr(rTips), _ptr(rPPs), _ptr(ev), *[_ptr(v) for v in vecs], None)
+        return ev, vecs
 
 
 def have_reference():

@@ -20,6 +20,7 @@
  *   - OpenCL-mode grid  evalLJC_QZs_noPos (FP64, x-fastest)   ocl/cl/FF.cl:115-130, 366-423
  *   - OpenCL-mode FIRE  relaxStrokesTilted, update_FIRE       ocl/cl/relax.cl:37-41, 90-126, 391-464
  *   - z convolution     convolveZ (valid-mode correlation)    ocl/cl/relax.cl:571-608
+ *   - stiffness       stiffnessMatrix, Mat3::eigenvals/vec   ProbeParticle.cpp:703-750, Mat3.h:459-500
  *
  * The operation order of every floating-point expression on the CPU-mode
  * path follows the reference so that, built with gcc -O2 (no FMA
@@ -31,7 +32,7 @@
  * committed golden fixtures, closed forms (single atom, affine fields,
  * harmonic tip minimum, 1-D force balance by bisection), F = -grad E.
  * The OpenCL-mode functions are pinned only by the mathematics (the
- * reference OpenCL path cannot run here); see DESIGN.md.
+ * reference OpenCL path cannot run here).
  */
 #include <math.h>
 #include <stdlib.h>
@@ -699,6 +700,106 @@ long orc_relax_strokes_warm(int nx, int ny, int nz, const int n[3], const double
                                  nevals ? nevals + off : 0);
         }
     return total;
+}
+
+/* ------------------------------------------------------------- stiffness */
+
+/* Mat3::eigenvals (Mat3.h:459-482) of a symmetric matrix m (row-major 3x3):
+ * the characteristic cubic's coefficients scaled by the largest element,
+ * then its three real roots in trigonometric form.  The reference writes
+ * 1/3 and sqrt(3) as the decimals 0.33333333333 and 1.73205080757; they are
+ * kept so that this rounds like the compiled reference.  Unsorted. */
+static void ref_eigenvals(const double m[9], double ev[3])
+{
+    const double third = 0.33333333333, sqrt3 = 1.73205080757;
+    double amax = m[0];
+    for (int k = 1; k < 9; k++)
+        if (m[k] > amax) amax = m[k];
+    const double xx = m[0], xy = m[1], xz = m[2], yy = m[4], yz = m[5], zz = m[8];
+    double det = xx * yy * zz + 2 * xy * xz * yz - xx * yz * yz - yy * xz * xz - zz * xy * xy;
+    double minors = xx * yy - xy * xy + xx * zz - xz * xz + yy * zz - yz * yz;
+    double tr = xx + yy + zz;
+    const double amax2 = amax * amax;
+    tr /= amax;
+    minors /= amax2;
+    det /= (amax2 * amax);
+    const double t3 = tr * third;
+    double p3 = (minors - tr * t3) * third;
+    if (p3 > 0.0) p3 = 0.0;
+    const double h = 0.5 * (det + t3 * (2.0 * t3 * t3 - minors));
+    double disc = h * h + p3 * p3 * p3;
+    if (disc > 0.0) disc = 0.0;
+    const double mag = sqrt(-p3);
+    const double ang = atan2(sqrt(-disc), h) * third;
+    const double cs = cos(ang), sn = sin(ang);
+    ev[0] = amax * (t3 + 2.0 * mag * cs);
+    ev[1] = amax * (t3 - mag * (cs + sqrt3 * sn));
+    ev[2] = amax * (t3 - mag * (cs - sqrt3 * sn));
+}
+
+/* Mat3::eigenvec (Mat3.h:484-500): the longest of the three cross products
+ * of the rows of (m - lam I), normalised (NaN when all three vanish, as in
+ * the reference). */
+static void ref_eigenvec(const double m[9], double lam, double out[3])
+{
+    const v3 r0 = {m[0] - lam, m[1], m[2]}, r1 = {m[3], m[4] - lam, m[5]}, r2 = {m[6], m[7], m[8] - lam};
+    const v3 c[3] = {{r0.y * r1.z - r0.z * r1.y, r0.z * r1.x - r0.x * r1.z, r0.x * r1.y - r0.y * r1.x},
+                     {r0.y * r2.z - r0.z * r2.y, r0.z * r2.x - r0.x * r2.z, r0.x * r2.y - r0.y * r2.x},
+                     {r1.y * r2.z - r1.z * r2.y, r1.z * r2.x - r1.x * r2.z, r1.x * r2.y - r1.y * r2.x}};
+    const double d[3] = {v3_dot(c[0], c[0]), v3_dot(c[1], c[1]), v3_dot(c[2], c[2])};
+    int k = 0;
+    double dm = d[0];
+    if (d[1] > dm) { dm = d[1]; k = 1; }
+    if (d[2] > dm) k = 2;
+    const double s = 1 / sqrt(d[k]);
+    out[0] = c[k].x * s; out[1] = c[k].y * s; out[2] = c[k].z * s;
+}
+
+/* stiffnessMatrix (ProbeParticle.cpp:703-750): for each point, rows of the
+ * dynamical matrix from central differences of getPPforce under +-ddisp
+ * displacements along x, y, z (applied to and undone on r in place, as the
+ * reference does), symmetrised; eigenvalues sorted ascending into evals
+ * (n,3); for which >= 1/2/3 the eigenvectors of the smallest/middle/largest
+ * into ev1/ev2/ev3 (n,3).  dynmat (n,9), optional, receives the symmetric
+ * matrix.  Returns the number of points with a NaN eigenvalue (where the
+ * reference exits). */
+long orc_stiffness(const int n[3], const double cell[9], const double *F, const double tip[8], double ddisp,
+                   int which, long np, const double *rtips, const double *rpps, double *evals, double *ev1,
+                   double *ev2, double *ev3, double *dynmat)
+{
+    orc_field ff; orc_tip t;
+    make_field(&ff, n, cell, F); make_tip(&t, tip);
+    long nnan = 0;
+    #pragma omp parallel for schedule(static) reduction(+ : nnan)
+    for (long i = 0; i < np; i++) {
+        const v3 rtip = {rtips[3 * i], rtips[3 * i + 1], rtips[3 * i + 2]};
+        v3 r = {rpps[3 * i], rpps[3 * i + 1], rpps[3 * i + 2]};
+        double m[9];
+        v3 f1, f2;
+        const double sc = -0.5 / ddisp;
+        r.x -= ddisp; f1 = pp_force(&ff, &t, rtip, r); r.x += 2 * ddisp; f2 = pp_force(&ff, &t, rtip, r); r.x -= ddisp;
+        m[0] = (f2.x - f1.x) * sc; m[1] = (f2.y - f1.y) * sc; m[2] = (f2.z - f1.z) * sc;
+        r.y -= ddisp; f1 = pp_force(&ff, &t, rtip, r); r.y += 2 * ddisp; f2 = pp_force(&ff, &t, rtip, r); r.y -= ddisp;
+        m[3] = (f2.x - f1.x) * sc; m[4] = (f2.y - f1.y) * sc; m[5] = (f2.z - f1.z) * sc;
+        r.z -= ddisp; f1 = pp_force(&ff, &t, rtip, r); r.z += 2 * ddisp; f2 = pp_force(&ff, &t, rtip, r); r.z -= ddisp;
+        m[6] = (f2.x - f1.x) * sc; m[7] = (f2.y - f1.y) * sc; m[8] = (f2.z - f1.z) * sc;
+        double s;
+        s = 0.5 * (m[1] + m[3]); m[1] = s; m[3] = s;
+        s = 0.5 * (m[5] + m[7]); m[5] = s; m[7] = s;
+        s = 0.5 * (m[6] + m[2]); m[6] = s; m[2] = s;
+        double e[3];
+        ref_eigenvals(m, e);
+        if (isnan(e[0] * e[1] * e[2])) nnan++;
+        if (e[0] > e[1]) { s = e[0]; e[0] = e[1]; e[1] = s; }
+        if (e[1] > e[2]) { s = e[1]; e[1] = e[2]; e[2] = s; }
+        if (e[0] > e[1]) { s = e[0]; e[0] = e[1]; e[1] = s; }
+        evals[3 * i] = e[0]; evals[3 * i + 1] = e[1]; evals[3 * i + 2] = e[2];
+        if (which > 0) ref_eigenvec(m, e[0], ev1 + 3 * i);
+        if (which > 1) ref_eigenvec(m, e[1], ev2 + 3 * i);
+        if (which > 2) ref_eigenvec(m, e[2], ev3 + 3 * i);
+        if (dynmat) memcpy(dynmat + 9 * i, m, sizeof m);
+    }
+    return nnan;
 }
 
 /* ------------------------------------------------------- OpenCL-mode grid */

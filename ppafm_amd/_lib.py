@@ -104,6 +104,7 @@ SIGNATURES = [
     ("getCoulombFF", None, [c_int, _dp, _dp, c_int]),
     ("relaxTipStroke", c_int, [c_int, c_int, c_int, _dp, _dp, _dp, ctypes.POINTER(SplineParams)]),
     ("relaxTipStrokes_omp", c_int, [c_int, c_int, c_int, c_int, c_int, _dp, _dp, _dp, ctypes.POINTER(SplineParams)]),
+    ("stiffnessMatrix", None, [c_double, c_int, c_int, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.POINTER(SplineParams)]),
     ("ppafm_pack_atoms", c_int, [c_int, _dp, _dp, c_double, c_void_p]),
     ("ppafm_ff_build_dev", c_int, [ctypes.POINTER(Grid), ctypes.POINTER(FFModel), c_int, c_void_p, c_void_p, c_void_p]),
     ("ppafm_ff_build", c_int, [ctypes.POINTER(Grid), ctypes.POINTER(FFModel), c_int, _dp, _dp, c_void_p, c_void_p]),
@@ -111,6 +112,11 @@ SIGNATURES = [
         "ppafm_relax",
         c_int,
         [ctypes.POINTER(Grid), c_void_p, ctypes.POINTER(RelaxParams), ctypes.POINTER(Tip), ctypes.POINTER(Scan), c_void_p, ctypes.POINTER(RelaxOut), c_void_p],
+    ),
+    (
+        "ppafm_stiffness",
+        c_int,
+        [ctypes.POINTER(Grid), c_void_p, ctypes.POINTER(Tip), ctypes.POINTER(Scan), c_void_p, c_void_p, c_int, c_double, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
     ),
     ("ppafm_convolve_z", c_int, [c_long, c_int, c_int, c_int, c_int, c_void_p, c_int, _dp, c_double, c_void_p, c_int, c_void_p]),
     ("ppafm_limit_vec_field", c_int, [c_void_p, c_long, c_double, c_void_p]),

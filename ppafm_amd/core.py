@@ -137,6 +137,26 @@ def relaxTipStrokes_omp(rTips, rs, fs, probeStart=1, relaxAlg=1, tip_spline=None
     return r
 
 
+def stiffnessMatrix(rTips, rPPs, which=0, ddisp=0.05, tip_spline=None):
+    """Eigenvalues (n,3), ascending, of the probe's dynamical matrix at the
+    relaxed positions rPPs (n,3) for tips rTips (n,3), and a list of three
+    (n,3) eigenvector arrays of which the first ``which`` are filled (the
+    rest None), as ppafm/core.py:332-345 (ProbeParticle.cpp:703-750)."""
+    _c64(rTips, 2, "rTips"), _c64(rPPs, 2, "rPPs")
+    n = len(rTips)
+    if rPPs.shape != rTips.shape or rTips.shape[1:] != (3,):
+        raise ValueError(f"rTips and rPPs must both be (n,3), got {rTips.shape} and {rPPs.shape}")
+    eigenvals = np.zeros((n, 3))
+    evecs = [None, None, None]
+    for i in range(which):
+        evecs[i] = np.zeros((n, 3))
+    lib().ppafm_clear_error()
+    lib().stiffnessMatrix(float(ddisp), int(which), n, _lib.dptr(rTips), _lib.dptr(rPPs), _lib.dptr(eigenvals),
+                          _lib.dptr(evecs[0]), _lib.dptr(evecs[1]), _lib.dptr(evecs[2]), tip_spline)
+    _check_last("stiffnessMatrix")
+    return eigenvals, evecs
+
+
 def last_itrsum():
     """Sum of FIRE iteration indices of the last relaxTipStrokes_omp call."""
     return int(lib().ppafm_last_itrsum())

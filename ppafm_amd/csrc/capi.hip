@@ -78,7 +78,7 @@ struct RefState {
     double conv_f2 = 1.0e-8, dt = 0.1, damping = 0.1;
     double finc = 1.1, fdec = 0.5, falpha = 0.99;
     long long last_itrsum = 0;
-    DevBuf dF, dE, dAtoms, dTips, dPos, dFs, dIters;
+    DevBuf dF, dE, dAtoms, dTips, dPos, dFs, dIters, dEv;
 };
 
 RefState &ref()
@@ -174,6 +174,62 @@ int ref_relax(int nx, int ny, int probeStart, int relaxAlg, int nstep, const dou
     for (size_t k = 0; k < npos; k++) sum += it[k];
     *itrsum = sum;
     S.last_itrsum = sum;
+    return 0;
+}
+
+// stiffnessMatrix (ProbeParticle.cpp:703-750) on the process-wide grid and
+// tip: rTips, rPPs (n,3) host float64 in, eigenvalues (n,3) ascending and
+// the first `which` eigenvectors (n,3) out.  The reference prints and exits
+// the process on a NaN eigenvalue (ProbeParticle.cpp:735); this reports it
+// as an error after writing every output.
+int ref_stiffness(double ddisp, int which, int n, const double *rTips, const double *rPPs, double *evals,
+                  double *ev1, double *ev2, double *ev3, const ppafm_spline_params *spline)
+{
+    RefState &S = ref();
+    std::lock_guard<std::mutex> lk(S.mu);
+    int e = ref_grid_ready(S);
+    if (e) return e;
+    PP_REQUIRE(spline == nullptr, "tip splines are not supported (pass None for the harmonic radial spring)");
+    PP_REQUIRE(S.F != nullptr, "setFF_Fpointer has not been called");
+    PP_REQUIRE(n >= 0, "n must be >= 0");
+    if (n == 0) return 0;
+    PP_REQUIRE(rTips && rPPs && evals, "null rTips, rPPs or eigenvals");
+    const int nv = which < 0 ? 0 : (which > 3 ? 3 : which);
+    double *hv[3] = {ev1, ev2, ev3};
+    for (int k = 0; k < nv; k++) PP_REQUIRE(hv[k], "an eigenvector output requested by `which` is NULL");
+    const size_t npts = (size_t)S.n[0] * S.n[1] * S.n[2];
+    const size_t b3 = (size_t)n * 3 * sizeof(double);
+    if ((e = S.dF.ensure(npts * 3 * sizeof(double)))) return e;
+    if ((e = S.dTips.ensure(b3))) return e;
+    if ((e = S.dPos.ensure(b3))) return e;
+    if ((e = S.dFs.ensure(b3))) return e;
+    if ((e = S.dEv.ensure(b3 * 3))) return e;
+    PP_TRY_HIP(hipMemcpy(S.dF.p, S.F, npts * 3 * sizeof(double), hipMemcpyHostToDevice));
+    PP_TRY_HIP(hipMemcpy(S.dTips.p, rTips, b3, hipMemcpyHostToDevice));
+    PP_TRY_HIP(hipMemcpy(S.dPos.p, rPPs, b3, hipMemcpyHostToDevice));
+    RelaxLaunch L;
+    L.g = make_geom(S.n, S.cell);
+    L.rp = ppafm_relax_params{0, 1, S.max_iters, 1, S.conv_f2, S.dt, S.damping, S.finc, S.fdec, S.falpha, 0.0, 0.0};
+    for (int k = 0; k < 9; k++) L.tip.rot[k] = (k % 4 == 0) ? 1.0 : 0.0;
+    for (int k = 0; k < 3; k++) { L.tip.d0[k] = S.rpp0[k]; L.tip.kspring[k] = S.kspring[k]; }
+    L.tip.krad = S.krad; L.tip.lrad = S.lrad; L.tip.surf_ff = 0;
+    L.scan = ppafm_scan{1, 1, n, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    L.tips = (const double *)S.dTips.p;
+    L.out = ppafm_relax_out{nullptr, 0, nullptr, nullptr, nullptr};
+    double *dv = (double *)S.dEv.p;
+    if ((e = launch_stiffness(L, S.dF.p, 2, S.dPos.p, 0, ddisp, nv, (double *)S.dFs.p, dv, dv + 3 * (size_t)n,
+                              dv + 6 * (size_t)n, nullptr)))
+        return e;
+    PP_TRY_HIP(hipGetLastError());
+    PP_TRY_HIP(hipMemcpy(evals, S.dFs.p, b3, hipMemcpyDeviceToHost));
+    for (int k = 0; k < nv; k++) PP_TRY_HIP(hipMemcpy(hv[k], dv + 3 * (size_t)k * n, b3, hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; i++)
+        if (std::isnan(evals[3 * i]) || std::isnan(evals[3 * i + 1]) || std::isnan(evals[3 * i + 2])) {
+            char msg[160];
+            snprintf(msg, sizeof msg, "stiffnessMatrix: NaN eigenvalues at point %d (first of them)", i);
+            set_error(msg);
+            return -1;
+        }
     return 0;
 }
 
@@ -297,6 +353,13 @@ int relaxTipStrokes_omp(int nx, int ny, int probeStart, int relaxAlg, int nstep,
     return rc ? -1 : 0;
 }
 
+void stiffnessMatrix(double ddisp, int which, int n, double *rTips, double *rPPs, double *eigenvals, double *evec1,
+                     double *evec2, double *evec3, ppafm_spline_params *splineParams)
+{
+    report(ref_stiffness(ddisp, which, n, rTips, rPPs, eigenvals, evec1, evec2, evec3, splineParams),
+           "stiffnessMatrix");
+}
+
 long long ppafm_last_itrsum(void) { return ref().last_itrsum; }
 
 // ------------------------------------------------------------------- device
@@ -358,6 +421,25 @@ int ppafm_relax(const ppafm_grid *grid, const void *grid_dev, const ppafm_relax_
     L.tips = tips_dev;
     L.out = *out;
     int e = launch_relax_dev(L, grid_dev, grid->f64 != 0, as_stream(stream));
+    if (e) return e;
+    PP_TRY_HIP(hipGetLastError());
+    return 0;
+}
+
+int ppafm_stiffness(const ppafm_grid *grid, const void *grid_dev, const ppafm_tip *tip, const ppafm_scan *scan,
+                    const double *tips_dev, const void *pos_dev, int pos_f32, double ddisp, int which,
+                    double *evals_dev, double *evec1_dev, double *evec2_dev, double *evec3_dev, void *stream)
+{
+    PP_REQUIRE(grid && grid_dev && tip && scan && pos_dev && evals_dev, "null argument");
+    RelaxLaunch L;
+    L.g = make_geom(grid->n, grid->cell);
+    L.rp = ppafm_relax_params{0, 1, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0};
+    L.tip = *tip;
+    L.scan = *scan;
+    L.tips = tips_dev;
+    L.out = ppafm_relax_out{nullptr, 0, nullptr, nullptr, nullptr};
+    int e = launch_stiffness(L, grid_dev, grid->f64 ? 1 : 0, pos_dev, pos_f32, ddisp, which, evals_dev, evec1_dev,
+                             evec2_dev, evec3_dev, as_stream(stream));
     if (e) return e;
     PP_TRY_HIP(hipGetLastError());
     return 0;

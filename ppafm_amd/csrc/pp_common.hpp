@@ -104,6 +104,11 @@ int launch_relax_dev(const RelaxLaunch &L, const void *grid, bool f64, hipStream
 int launch_relax_fast(const RelaxLaunch &L, const void *grid, bool f64, hipStream_t st);
 int launch_relax_f64grid(const RelaxLaunch &L, const double *grid, double *pos, double *force, int *iters,
                          hipStream_t st);
+// Dynamical matrix and its eigen-decomposition at L.scan's positions (pos,
+// float32 or float64 (npos,3) in stroke order), mode-0 tip.  grid_kind:
+// 0 float4, 1 double4, 2 double (npts,3) (the host ABI's grid).
+int launch_stiffness(const RelaxLaunch &L, const void *grid, int grid_kind, const void *pos, int pos_f32,
+                     double ddisp, int which, double *evals, double *ev1, double *ev2, double *ev3, hipStream_t st);
 
 // Batched FIRE relaxation (persistent kernel, see relax_dev.hip).
 struct RelaxBatchLaunch {

@@ -20,6 +20,7 @@
 // FP64 (host ABI path, bit-compatible with the CPU reference).
 // Blocks are remapped so that each XCD works on a contiguous band of tiles.
 
+#include <cmath>
 #include <cstdlib>
 
 #include "pp_common.hpp"
@@ -349,6 +350,156 @@ __global__ __launch_bounds__(64) void relax_kernel(RelaxArgs A, Grid G)
     }
 }
 
+// ------------------------------------------------------------- stiffness
+//
+// stiffnessMatrix (ProbeParticle.cpp:703-750): at each relaxed position r
+// the rows of the dynamical matrix are central differences of the total
+// probe force (getPPforce, 282-294) under displacements of +-ddisp along x,
+// y, z, D_k = -(F(r + h e_k) - F(r - h e_k)) / 2h, made symmetric; its
+// eigenvalues are written in ascending order and, on request, the unit
+// eigenvectors of the smallest, middle and largest.  The displacement is
+// applied and undone on r in place, in the reference's order, so the
+// displaced positions round as there.  One lane per position; the six force
+// evaluations are six independent 8-corner gathers.
+
+// getPPforce with the harmonic radial spring, in the reference's order.
+template <class Grid>
+__device__ __forceinline__ d3 pp_force_m0(const RelaxArgs &A, const Grid &G, d3 rtip, d3 r)
+{
+    const d3 gc{dot3(r, d3{A.dinv[0], A.dinv[1], A.dinv[2]}), dot3(r, d3{A.dinv[3], A.dinv[4], A.dinv[5]}),
+                dot3(r, d3{A.dinv[6], A.dinv[7], A.dinv[8]})};
+    double fe[4];
+    G.sample(A.n, gc, fe);
+    d3 f{fe[0], fe[1], fe[2]};
+    const d3 dp{r.x - rtip.x, r.y - rtip.y, r.z - rtip.z};
+    const double l = sqrt(dot3(dp, dp));
+    const double sr = A.krad * (l - A.lrad) / l;
+    f.x += dp.x * sr; f.y += dp.y * sr; f.z += dp.z * sr;
+    const d3 dl{dp.x - A.d0[0], dp.y - A.d0[1], dp.z - A.d0[2]};
+    f.x += dl.x * A.ks[0]; f.y += dl.y * A.ks[1]; f.z += dl.z * A.ks[2];
+    return f;
+}
+
+// Eigenvalues of the symmetric matrix [[xx,xy,xz],[xy,yy,yz],[xz,yz,zz]],
+// ascending, by the trigonometric solution of the characteristic cubic
+// (O. K. Smith, Comm. ACM 4(4):168, 1961): with q = tr/3,
+// p = sqrt(|A - qI|_F^2 / 6) and B = (A - qI)/p, det(B)/2 = cos(3 phi) and
+// the roots are q + 2p cos(phi + 2k pi/3).  p comes from a sum of squares,
+// so it does not cancel when eigenvalues nearly coincide.  NaN inputs give
+// NaN eigenvalues.
+__device__ __forceinline__ void sym3_eigenvalues(double xx, double yy, double zz, double xy, double xz, double yz,
+                                                 double ev[3])
+{
+    const double q = (xx + yy + zz) / 3.0;
+    const double dx = xx - q, dy = yy - q, dz = zz - q;
+    const double p2 = dx * dx + dy * dy + dz * dz + 2.0 * (xy * xy + xz * xz + yz * yz);
+    if (p2 == 0.0) {
+        ev[0] = ev[1] = ev[2] = q;
+        return;
+    }
+    const double p = sqrt(p2 / 6.0), ip = 1.0 / p;
+    const double bxx = dx * ip, byy = dy * ip, bzz = dz * ip, bxy = xy * ip, bxz = xz * ip, byz = yz * ip;
+    double r = 0.5 * (bxx * (byy * bzz - byz * byz) - bxy * (bxy * bzz - byz * bxz) + bxz * (bxy * byz - byy * bxz));
+    r = r > 1.0 ? 1.0 : (r < -1.0 ? -1.0 : r);  // rounding only; NaN passes through
+    const double phi = acos(r) / 3.0;
+    const double two_pi_3 = 2.0943951023931954923;  // 2 pi / 3
+    double hi = q + 2.0 * p * cos(phi);
+    double lo = q + 2.0 * p * cos(phi + two_pi_3);
+    double mid = 3.0 * q - hi - lo;
+    // the construction orders them; the swaps only guard rounding ties
+    double t;
+    if (lo > mid) { t = lo; lo = mid; mid = t; }
+    if (mid > hi) { t = mid; mid = hi; hi = t; }
+    if (lo > mid) { t = lo; lo = mid; mid = t; }
+    ev[0] = lo; ev[1] = mid; ev[2] = hi;
+}
+
+// Unit eigenvector for eigenvalue lam: the longest of the three pairwise
+// cross products of the rows of (A - lam I), which are orthogonal to its null
+// space (the selection and sign rule of Mat3::eigenvec, Mat3.h:484-500, so
+// the vectors come out with the reference's signs).  A fully degenerate
+// matrix (all three products zero) gives the unit axis `axis`.
+__device__ __forceinline__ d3 sym3_eigenvector(double xx, double yy, double zz, double xy, double xz, double yz,
+                                               double lam, int axis)
+{
+    const d3 r0{xx - lam, xy, xz}, r1{xy, yy - lam, yz}, r2{xz, yz, zz - lam};
+    const d3 c01{r0.y * r1.z - r0.z * r1.y, r0.z * r1.x - r0.x * r1.z, r0.x * r1.y - r0.y * r1.x};
+    const d3 c02{r0.y * r2.z - r0.z * r2.y, r0.z * r2.x - r0.x * r2.z, r0.x * r2.y - r0.y * r2.x};
+    const d3 c12{r1.y * r2.z - r1.z * r2.y, r1.z * r2.x - r1.x * r2.z, r1.x * r2.y - r1.y * r2.x};
+    const double d0 = dot3(c01, c01), d1 = dot3(c02, c02), d2 = dot3(c12, c12);
+    d3 c = c01;
+    double dm = d0;
+    if (d1 > dm) { dm = d1; c = c02; }
+    if (d2 > dm) { dm = d2; c = c12; }
+    if (dm == 0.0) return d3{axis == 0 ? 1.0 : 0.0, axis == 1 ? 1.0 : 0.0, axis == 2 ? 1.0 : 0.0};
+    const double s = 1.0 / sqrt(dm);
+    return d3{c.x * s, c.y * s, c.z * s};
+}
+
+struct StiffOut {
+    const void *pos;
+    int pos_f32;
+    double ddisp;
+    int which;
+    double *evals, *ev1, *ev2, *ev3;
+};
+
+template <class Grid>
+__global__ __launch_bounds__(256) void stiffness_kernel(RelaxArgs A, Grid G, StiffOut O)
+{
+    const long o = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long np = (long)A.nx * A.ny * A.nz;
+    if (o >= np) return;
+    const int s = (int)(o % A.nz);
+    const long col = o / A.nz;
+    const int j = (int)(col % A.ny), i = (int)(col / A.ny);
+    const d3 rtip = tip_pos(A, i, j, s);
+    d3 r;
+    if (O.pos_f32) {
+        const float *p = (const float *)O.pos + 3 * o;
+        r = d3{p[0], p[1], p[2]};
+    } else {
+        const double *p = (const double *)O.pos + 3 * o;
+        r = d3{p[0], p[1], p[2]};
+    }
+    const double h = O.ddisp, h2 = 2 * O.ddisp, sc = -0.5 / O.ddisp;
+    // displaced coordinates as the reference's in-place updates round them:
+    // x - h, then + 2h, then - h back (x1), before y is displaced
+    const double xm = r.x - h, xp = xm + h2, x1 = xp - h;
+    const double ym = r.y - h, yp = ym + h2, y1 = yp - h;
+    const double zm = r.z - h, zp = zm + h2;
+    d3 a, b, c;
+    // one axis per trip: its two gathers in flight together, registers
+    // reused across trips (k is uniform, so the branches are scalar)
+#pragma unroll 1
+    for (int k = 0; k < 3; k++) {
+        d3 lo, hi;
+        if (k == 0) { lo = d3{xm, r.y, r.z}; hi = d3{xp, r.y, r.z}; }
+        else if (k == 1) { lo = d3{x1, ym, r.z}; hi = d3{x1, yp, r.z}; }
+        else { lo = d3{x1, y1, zm}; hi = d3{x1, y1, zp}; }
+        const d3 f1 = pp_force_m0(A, G, rtip, lo), f2 = pp_force_m0(A, G, rtip, hi);
+        const d3 row{(f2.x - f1.x) * sc, (f2.y - f1.y) * sc, (f2.z - f1.z) * sc};
+        if (k == 0) a = row;
+        else if (k == 1) b = row;
+        else c = row;
+    }
+    // rows a, b, c are the responses to x, y, z displacements; symmetric part
+    const double xy = 0.5 * (a.y + b.x), yz = 0.5 * (b.z + c.y), xz = 0.5 * (c.x + a.z);
+    double ev[3];
+    sym3_eigenvalues(a.x, b.y, c.z, xy, xz, yz, ev);
+    double *e = O.evals + 3 * o;
+    e[0] = ev[0]; e[1] = ev[1]; e[2] = ev[2];
+    double *outs[3] = {O.ev1, O.ev2, O.ev3};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        if (O.which > k) {
+            const d3 v = sym3_eigenvector(a.x, b.y, c.z, xy, xz, yz, ev[k], k);
+            double *w = outs[k] + 3 * o;
+            w[0] = v.x; w[1] = v.y; w[2] = v.z;
+        }
+    }
+}
+
 RelaxArgs make_args(const RelaxLaunch &L)
 {
     RelaxArgs A;
@@ -422,6 +573,35 @@ int launch_relax_f64grid(const RelaxLaunch &L, const double *grid, double *pos, 
     const long nblk = ((nt + 7) / 8) * 8;
     PP_REQUIRE(nblk < (1L << 31), "scan too large");
     hipLaunchKernelGGL((relax_kernel<GridF64, true>), dim3((unsigned)nblk), dim3(64), 0, st, A, GridF64{grid});
+    return 0;
+}
+
+int launch_stiffness(const RelaxLaunch &L, const void *grid, int grid_kind, const void *pos, int pos_f32,
+                     double ddisp, int which, double *evals, double *ev1, double *ev2, double *ev3, hipStream_t st)
+{
+    PP_REQUIRE(L.scan.nx >= 0 && L.scan.ny >= 0 && L.scan.nz >= 0, "scan sizes must be >= 0");
+    PP_REQUIRE(L.g.n[0] > 0 && L.g.n[1] > 0 && L.g.n[2] > 0, "grid dimensions must be positive");
+    PP_REQUIRE(grid_kind >= 0 && grid_kind <= 2, "grid_kind must be 0 (float4), 1 (double4) or 2 (double3)");
+    PP_REQUIRE(std::isfinite(ddisp) && ddisp != 0.0, "ddisp must be finite and non-zero");
+    const long np = (long)L.scan.nx * L.scan.ny * L.scan.nz;
+    if (np == 0) return 0;
+    PP_REQUIRE(grid && pos && evals, "null grid, position or eigenvalue pointer");
+    PP_REQUIRE(which < 1 || ev1, "which >= 1 needs the first eigenvector output");
+    PP_REQUIRE(which < 2 || ev2, "which >= 2 needs the second eigenvector output");
+    PP_REQUIRE(which < 3 || ev3, "which >= 3 needs the third eigenvector output");
+    RelaxArgs A = make_args(L);
+    const StiffOut O{pos, pos_f32, ddisp, which, evals, ev1, ev2, ev3};
+    const long nblk = (np + 255) / 256;
+    PP_REQUIRE(nblk < (1L << 31), "too many positions");
+    if (grid_kind == 0)
+        hipLaunchKernelGGL((stiffness_kernel<GridF32>), dim3((unsigned)nblk), dim3(256), 0, st, A,
+                           GridF32{(const float4 *)grid}, O);
+    else if (grid_kind == 1)
+        hipLaunchKernelGGL((stiffness_kernel<GridD4>), dim3((unsigned)nblk), dim3(256), 0, st, A,
+                           GridD4{(const double4 *)grid}, O);
+    else
+        hipLaunchKernelGGL((stiffness_kernel<GridF64>), dim3((unsigned)nblk), dim3(256), 0, st, A,
+                           GridF64{(const double *)grid}, O);
     return 0;
 }
 

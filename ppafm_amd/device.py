@@ -241,6 +241,32 @@ def relax(n, cell, grid_dev, rp, tip, scan, tips_dev=None, pos=None, force=None,
     return pos, force, iters
 
 
+def stiffness(n, cell, grid_dev, tip, scan, pos, which=0, ddisp=0.05, tips_dev=None, out=None, stream=None):
+    """stiffnessMatrix on a device grid at the relaxed positions ``pos`` (a
+    DeviceArray (nx,ny,nz,3), float32 or float64, as relax() writes it):
+    returns DeviceArrays eigenvalues (nx,ny,nz,3) ascending and a list of
+    three eigenvector arrays, the first ``which`` allocated (others None).
+    ``out`` = (evals, evecs) from an earlier call is reused instead."""
+    shp = (scan.nx, scan.ny, scan.nz)
+    if tuple(pos.shape) != shp + (3,) or pos.dtype not in (np.float32, np.float64):
+        raise ValueError(f"pos must be a float32/float64 DeviceArray of shape {shp + (3,)}, got {pos.shape} {pos.dtype}")
+    if out is None:
+        evals = DeviceArray(shp + (3,), np.float64)
+        evecs = [DeviceArray(shp + (3,), np.float64) if k < which else None for k in range(3)]
+    else:
+        evals, evecs = out
+        if tuple(evals.shape) != shp + (3,) or any((v is None) != (k >= which) for k, v in enumerate(evecs)):
+            raise ValueError("out does not match the scan shape and `which`")
+    g = grid_desc(n, cell, f64=grid_dev.dtype == np.float64)
+    check(
+        lib().ppafm_stiffness(ctypes.byref(g), grid_dev.ptr, ctypes.byref(tip), ctypes.byref(scan),
+                              tips_dev.ptr if tips_dev is not None else None, pos.ptr, int(pos.dtype == np.float32),
+                              float(ddisp), int(which), evals.ptr, *[v.ptr if v is not None else None for v in evecs], stream),
+        "stiffness",
+    )
+    return evals, evecs
+
+
 def convolve_z_dev(in_dev, ncol, nz, w, scale=1.0, in_stride=None, in_step=1, in_offset=0, out=None, f64=False, stream=None):
     w = np.ascontiguousarray(w, dtype=np.float64)
     nzout = nz - len(w) + 1

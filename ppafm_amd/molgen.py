@@ -198,8 +198,8 @@ def scan_tips(cfg):
     return t
 
 
-# Probe and tip (decisions recorded in DESIGN.md): O probe, monopole charge
-# -0.1 e, tipR0 = (0, 0, 3) A, k_xy = 0.25 N/m, k_r = 30 N/m.
+# Probe and tip (recorded choices where ppafm's CPU and OpenCL defaults
+# differ): O probe, monopole charge -0.1 e, tipR0 = (0, 0, 3) A, k_xy = 0.25 N/m, k_r = 30 N/m.
 PROBE_Z = 8
 PROBE_Q = -0.1
 TIP_R0 = (0.0, 0.0, 3.0)

@@ -122,6 +122,14 @@ class PlainScan:
         self.relax()
         self.to_df()
 
+    def stiffness(self, which=0, ddisp=0.05):
+        """Eigenvalues (nx,ny,nz,3) of the probe's dynamical matrix at the
+        relaxed positions and the first ``which`` eigenvector arrays
+        (core.stiffnessMatrix / ProbeParticle.cpp:703-750), on the GPU grid."""
+        ev, vecs = D.stiffness(self.n, self.lvec[1:], self.grid, self.tip, self.scan_d, self.pos, which=which,
+                               ddisp=ddisp, stream=self.stream)
+        return ev.download(), [v.download() if v is not None else None for v in vecs]
+
     def reset_counters(self):
         self.counters.zero(self.stream)
 

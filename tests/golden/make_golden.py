@@ -12,6 +12,10 @@ ABI on small seeded inputs and stores inputs and outputs as .npz data:
              of 6 tip positions, for probeStart 1/0/-1 (FIRE) and the damped
              leap-frog (relaxAlg 0), plus relaxTipStroke's iteration sum
              (ProbeParticle.cpp:633-701)
+  stiff.npz  stiffnessMatrix on that grid at the relaxed positions of the
+             probeStart-1 strokes (and at positions displaced from them by
+             up to 0.3 A), which = 3, ddisp 0.05 and 0.02
+             (ProbeParticle.cpp:703-750)
 
 Usage: python tests/golden/make_golden.py   (needs oracle/_ref built)
 """
@@ -84,10 +88,29 @@ def relax_case(ref):
     return out
 
 
+def stiff_case(ref, rc):
+    rng = np.random.default_rng(20240603)
+    ref.set_grid(rc["n"], rc["cell"], rc["FF"])
+    ref.set_tip(rc["tip_lRadial"], rc["tip_kRadial"], rc["tip_rPP0"], rc["tip_kSpring"])
+    tips = rc["tips"].reshape(-1, 3)
+    rpps = np.concatenate([rc["rs_ps1"].reshape(-1, 3), rc["rs_ps1"].reshape(-1, 3) + rng.uniform(-0.3, 0.3, (len(tips), 3))])
+    tips = np.concatenate([tips, tips])
+    out = dict(tips=tips, rpps=rpps)
+    for dd in (0.05, 0.02):
+        ev, vecs = ref.stiffness(tips, rpps, which=3, ddisp=dd)
+        tag = f"{int(round(dd * 100)):02d}"
+        out[f"evals_{tag}"] = ev
+        for k in range(3):
+            out[f"evec{k + 1}_{tag}"] = vecs[k]
+    return out
+
+
 def main():
     ref = O.Reference()
     np.savez_compressed(os.path.join(HERE, "grid.npz"), **grid_case(ref))
-    np.savez_compressed(os.path.join(HERE, "relax.npz"), **relax_case(ref))
+    rc = relax_case(ref)
+    np.savez_compressed(os.path.join(HERE, "relax.npz"), **rc)
+    np.savez_compressed(os.path.join(HERE, "stiff.npz"), **stiff_case(ref, rc))
     print("wrote", os.listdir(HERE))
 
 

@@ -601,3 +601,72 @@ def test_warm_rejects_saddle_on_a_repulsive_bump():
     # the probe leaves the axis as plain FIRE does, and ends where FIRE ends
     assert (np.abs(rw[..., 0] - 4.0)[:, :, -1] > 0.1).all()
     assert np.abs(rw - rs).max() < 0.03
+
+
+# ------------------------------------------------------------ stiffness
+
+
+@pytest.mark.parametrize("tag", ["05", "02"])
+def test_stiffness_golden_bit_exact(tag):
+    r = np.load(os.path.join(GOLDEN, "relax.npz"))
+    s = np.load(os.path.join(GOLDEN, "stiff.npz"))
+    ev, vecs, nnan = O.stiffness(r["n"], r["cell"], r["FF"], _golden_tip(r), s["tips"], s["rpps"], which=3,
+                                 ddisp=int(tag) / 100)
+    assert nnan == 0
+    assert np.array_equal(ev, s[f"evals_{tag}"])
+    for k in range(3):
+        assert np.array_equal(vecs[k], s[f"evec{k + 1}_{tag}"]), k
+
+
+@pytest.mark.skipif(not O.have_reference(), reason="oracle/_ref not built (reference sources absent)")
+def test_stiffness_live_reference_random_case():
+    rng = np.random.default_rng(7)
+    n = np.array([14, 12, 30], dtype=np.int32)
+    cell = np.array([[1.4, 0.0, 0.0], [0.2, 1.2, 0.0], [0.0, 0.0, 3.0]])
+    Rs = np.column_stack([rng.uniform(0.2, 1.4, 4), rng.uniform(0.2, 1.2, 4), np.full(4, 0.4)])
+    cLJs = np.tile([23.6, 2.4e4], (4, 1)) * rng.uniform(0.8, 1.2, (4, 1))
+    F = O.limit_vec_field(O.lj_ff(n, cell, Rs, cLJs), 10.0)
+    tip = O.tip_params(1.2, 25 / -O.eVA_Nm, (0.1, 0.0, 0.0), np.array([0.4, 0.3, 0.0]) / -O.eVA_Nm)
+    tips = np.column_stack([rng.uniform(0, 1.5, 40), rng.uniform(0, 1.3, 40), rng.uniform(2.3, 2.9, 40)])
+    rpps = tips + np.array([0.1, 0.0, -1.2]) + rng.uniform(-0.2, 0.2, (40, 3))
+    ref = O.Reference()
+    ref.set_grid(n, cell, F)
+    ref.set_tip(1.2, 25 / -O.eVA_Nm, np.array([0.1, 0.0, 0.0]), np.array([0.4, 0.3, 0.0]) / -O.eVA_Nm)
+    for which in (0, 2, 3):
+        evr, vr = ref.stiffness(tips, rpps, which=which, ddisp=0.03)
+        ev, vecs, nnan = O.stiffness(n, cell, F, tip, tips, rpps, which=which, ddisp=0.03)
+        assert np.array_equal(ev, evr)
+        for k in range(which):
+            assert np.array_equal(vecs[k], vr[k]), (which, k)
+
+
+def _affine_grid(M, F0, n=(20, 20, 20), h=0.1):
+    cell = np.diag([n[0] * h, n[1] * h, n[2] * h])
+    idx = np.stack(np.meshgrid(*[np.arange(k) * h for k in n], indexing="ij"), axis=-1)
+    return cell, np.ascontiguousarray(F0 + idx @ M.T)
+
+
+def test_stiffness_affine_field_closed_form():
+    """Field F0 + M r (interpolated exactly away from the wrap) with a linear
+    tip (kRadial = 0, springs kS): the dynamical matrix is exactly
+    -(M + M^T)/2 - diag(kS) whatever ddisp, so its eigen-decomposition is
+    pinned by numpy's symmetric eigensolver."""
+    rng = np.random.default_rng(11)
+    M = rng.normal(0, 0.5, (3, 3))
+    F0 = rng.normal(0, 0.2, 3)
+    cell, F = _affine_grid(M, F0)
+    ks = np.array([-0.4, -0.25, -0.1])
+    tip = O.tip_params(1.0, 0.0, (0.05, -0.02, -0.9), ks)
+    rpps = rng.uniform(0.6, 1.3, (25, 3))
+    tips = rpps + rng.uniform(-0.5, 0.5, (25, 3))
+    S = -(M + M.T) / 2 - np.diag(ks)
+    w, V = np.linalg.eigh(S)
+    for dd in (0.05, 0.01):
+        ev, vecs, nnan, D = O.stiffness((20, 20, 20), cell, F, tip, tips, rpps, which=3, ddisp=dd, want_dynmat=True)
+        assert nnan == 0
+        assert np.abs(D - S).max() < 1e-12
+        # the reference's decimal 1/3 and sqrt(3) cost ~1e-11 relative
+        assert np.abs(ev - w[None, :]).max() < 1e-9 * np.abs(w).max()
+        for k in range(3):
+            assert np.abs(np.abs(vecs[k] @ V[:, k]) - 1).max() < 1e-9
+            assert np.abs(np.linalg.norm(vecs[k], axis=1) - 1).max() < 1e-12
