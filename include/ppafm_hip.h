@@ -31,8 +31,11 @@ extern "C" {
  * (1) Reference-compatible host ABI
  * ---------------------------------------------------------------------- */
 
-/* Tip radial spline (ProbeParticle.cpp:58-62).  Only NULL (harmonic radial
- * spring) is supported; a non-NULL spline makes the relax calls fail. */
+/* Tip radial spline (TIP::SplineParams, ProbeParticle.cpp:58-62): rff_n
+ * knots rff_xs and (rff_n,2) values and slopes rff_ydys of the radial force;
+ * NULL selects the harmonic radial spring of setTip.  The relaxation and
+ * stiffness calls evaluate it as forceRSpline (ProbeParticle.cpp:211-224),
+ * extrapolating the last interval beyond the last knot. */
 typedef struct {
     int rff_n;
     double *rff_xs;
@@ -79,6 +82,58 @@ int relaxTipStrokes_omp(int nx, int ny, int probeStart, int relaxAlg, int nstep,
  * (stderr, ppafm_last_error) where the reference exits the process. */
 void stiffnessMatrix(double ddisp, int which, int n, double *rTips, double *rPPs, double *eigenvals, double *evec1,
                      double *evec2, double *evec3, ppafm_spline_params *splineParams);
+
+/* Further force-field grids on the process-wide grid, += like
+ * getLennardJonesFF (coefficients per atom, positions relative to the grid
+ * origin):
+ *   ProbeParticle.cpp:515  vdW -6 c6 / (r^8 + ADamp c6) r, ADamp 180 (cLJs: c6, c12).
+ *                          Its energy is 0, as in the reference. */
+void getVdWFF(int natoms, double *Rs, double *cLJs);
+/* ProbeParticle.cpp:577  vdW with damping function kind 1..4 (R2, R4,
+ * invR4, invR8) of (r/R)^2, REs (R, E0); ADamp > 0 replaces the kind's
+ * constant (0.5, 0.5, 0.03, 0.01) for this and later calls; other kinds add
+ * nothing. */
+void getVdWFF_RE(int natoms, double *Rs, double *REs, int kind, double ADamp);
+/* ProbeParticle.cpp:523  DFT-D3 dispersion, coefficients (c6, c8, R0^6,
+ * R0^8) per atom from computeD3Coeffs; pairs beyond 20 A add nothing. */
+void getDFTD3FF(int natoms, double *Rs, double *d3_coeffs);
+/* ProbeParticle.cpp:531  Morse E0 (e^{2a(r-R0)} - 2 e^{a(r-R0)}), REs (R0, E0). */
+void getMorseFF(int natoms, double *Rs, double *REs, double alpha);
+/* ProbeParticle.cpp:601, 610, 619  densities into the E grid only (the F
+ * grid is left alone): amp exp(-(r^2/s)^2/2), amp exp(-r/s),
+ * amp (1 - r^2/R^2)^2 inside R; cRAs (amp, width). */
+void getGaussDensity(int natoms, double *Rs, double *cRAs);
+void getSlaterDensity(int natoms, double *Rs, double *cRAs);
+void getDensityR4spline(int natoms, double *Rs, double *cRAs);
+/* ProbeParticle.cpp:379  LJ force and energy at npoints points (npoints,3)
+ * -> FEs (npoints,4), separation taken atom minus point as there. */
+void getInPoints_LJ(int npoints, double *points, double *FEs, int natoms, double *Rs, double *cLJs);
+/* ProbeParticle.cpp:556  one pair term at separations (r,0,0) -> Es, Fs (x
+ * force); kind -3 LJ in (R, E), -2 LJ (c6, c12), -1 bare r^-6, 0 vdW with
+ * constant damping, 1..4 damping functions as getVdWFF_RE (ADamp > 0
+ * replaces the kind's constant). */
+void evalRadialFF(int n, double *rs, double *coefs, double *Es, double *Fs, int kind, double ADamp);
+/* ProbeParticle.cpp:397  DFT-D3 coefficients per atom from the caller's
+ * tables (r_cov[94], ref_cn[94*5], ref_c6[94*94*5*5], r4r2[94], k[3],
+ * params {s6, s8, a1, a2}); r_cut is accepted and unused, as there.
+ * Reference coordination slots < 0 of the probe get weight 0 (the reference
+ * leaves them uninitialised). */
+void computeD3Coeffs(int natoms, const double *rs, const int *elems, const double *r_cov, const double *r_cut,
+                     const double *ref_cn, const double *ref_c6, const double *r4r2, const double *k,
+                     const double *params, int elem_pp, double *d3_coeffs);
+/* ProbeParticle.cpp:752, 765  cubic Hermite spline (n knots, ydys (n,2)
+ * values and slopes) sampled at m points; uniform knots x0 + i dx, or knots
+ * xs with the reference's carried-over interval search.  Samples beyond the
+ * last interval extrapolate it (the reference reads past the table). */
+void subsample_uniform_spline(double x0, double dx, int n, double *ydys, int m, double *xs, double *ys);
+void subsample_nonuniform_spline(int n, double *xs, double *ydys, int m, double *xs_, double *ys_);
+/* ProbeParticle.cpp:786  radial tip force at r0 + k dr (k < n) about R:
+ * type 1 the spline, type 2 the harmonic spring of setTip; fs (n,3). */
+void test_force(int type, int n, double *r0, double *dr, double *R, double *fs, ppafm_spline_params *splineParams);
+/* ProbeParticle.cpp:805  eigenvalues of a 3x3 matrix in Mat3::eigenvals'
+ * order (largest, smallest, middle for a positive largest element) and an
+ * eigenvector of each: evs (4,3). */
+void test_eigen3x3(double *mat, double *evs);
 
 /* ------------------------------------------------------------------------
  * (2) Device ABI

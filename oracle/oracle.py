@@ -89,6 +89,13 @@ def lib():
         L.orc_stiffness.argtypes = [_ip, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int, ctypes.c_long, _dp, _dp, _dp,
                                     _dp, _dp, _dp, _dp]
         L.orc_stiffness.restype = ctypes.c_long
+        L.orc_model_ff.argtypes = [ctypes.c_int, ctypes.c_double, _ip, _dp, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, _dp]
+        L.orc_radial.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp, _dp, _dp, _dp]
+        L.orc_relax_strokes_spline.argtypes = [ctypes.c_int] * 3 + [_ip, _dp, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
+        L.orc_relax_strokes_spline.restype = ctypes.c_long
+        L.orc_stiffness_spline.argtypes = [_ip, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_int,
+                                           ctypes.c_long, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_stiffness_spline.restype = ctypes.c_long
         L.orc_num_threads.restype = ctypes.c_int
         _lib = L
     return _lib
@@ -195,7 +202,146 @@ def force_jac(n, cell, F, tip, rtips, rs):
     return f, J
 
 
-def stiffness(n, cell, F, tip, rTips, rPPs, which=0, ddisp=0.05, want_dynmat=False):
+# Oracle model ids (pp_oracle.c pair_model) of the reference's grid builders
+# and evalRadialFF kinds.
+MODEL = dict(vdw=4, vdw_r2=5, vdw_r4=6, vdw_invr4=7, vdw_invr8=8, d3=9, morse=10, gauss=11, slater=12, r4=13,
+             lj_re=14, inv_r6=15, lj=0)
+RADIAL_KIND = {-3: "lj_re", -2: "lj", -1: "inv_r6", 0: "vdw", 1: "vdw_r2", 2: "vdw_r4", 3: "vdw_invr4", 4: "vdw_invr8"}
+ADAMP_DEFAULT = {"vdw": 180.0, "vdw_r2": 0.5, "vdw_r4": 0.5, "vdw_invr4": 0.03, "vdw_invr8": 0.01}  # ProbeParticle.cpp:48
+
+
+def model_ff(name, n, cell, Rs, coefs, par=0.0, F=None, E=None):
+    """Accumulate one of the further grid models (getVdWFF, getVdWFF_RE,
+    getDFTD3FF, getMorseFF, the densities; ProbeParticle.cpp:515-631) into F
+    (nx,ny,nz,3) and E (nx,ny,nz); Rs relative to the grid origin."""
+    n = np.ascontiguousarray(n, dtype=np.int32)
+    Rs = _f64(Rs, (-1, 3))
+    coefs = _f64(coefs)
+    coefs = coefs.reshape(len(Rs), -1)
+    if F is None:
+        F = np.zeros((n[0], n[1], n[2], 3))
+    lib().orc_model_ff(MODEL[name], float(par), _ptr(n, _ip), _ptr(_f64(cell, (3, 3))), len(Rs), _ptr(Rs), _ptr(coefs),
+                       coefs.shape[1], _ptr(F), _ptr(E))
+    return F
+
+
+def radial(kind, rs, coefs, adamp=None):
+    """evalRadialFF (ProbeParticle.cpp:556-575): Es, Fs at (r,0,0); adamp None
+    = the kind's default constant."""
+    name = RADIAL_KIND[kind]
+    par = ADAMP_DEFAULT.get(name, 0.0) if adamp is None else adamp
+    rs = _f64(rs)
+    c = np.zeros(4)
+    cf = _f64(coefs).ravel()
+    c[: len(cf)] = cf
+    Es, Fs = np.zeros(len(rs)), np.zeros(len(rs))
+    lib().orc_radial(MODEL[name], float(par), len(rs), _ptr(rs), _ptr(c), _ptr(Es), _ptr(Fs))
+    return Es, Fs
+
+
+def relax_strokes_spline(n, cell, F, tip, rTips, spline, relax=DEFAULT_RELAX):
+    """relaxTipStrokes_omp (probeStart 1, FIRE) with the radial tip force of
+    spline = (xs, ydys) (forceRSpline, ProbeParticle.cpp:211-224)."""
+    rTips = _f64(rTips)
+    nx, ny, nz, _ = rTips.shape
+    rs, fs = np.zeros_like(rTips), np.zeros_like(rTips)
+    xs, ydys = _f64(spline[0]), _f64(spline[1])
+    n = np.ascontiguousarray(n, dtype=np.int32)
+    lib().orc_relax_strokes_spline(nx, ny, nz, _ptr(n, _ip), _ptr(_f64(cell, (3, 3))), _ptr(_f64(F)), _ptr(_f64(tip)),
+                                   _ptr(_f64(relax)), len(xs), _ptr(xs), _ptr(ydys), _ptr(rTips), _ptr(rs), _ptr(fs))
+    return rs, fs
+
+
+def hermite(x, y0, y1, dy0, dy1):
+    """Spline_Hermite::val (spline_hermite.h:45-51)."""
+    y01 = y0 - y1
+    return y0 + x * (dy0 + x * (-3 * y01 - 2 * dy0 - dy1 + x * (2 * y01 + dy0 + dy1)))
+
+
+def find_index(i, di, x, xs):
+    """Spline_Hermite::find_index (spline_hermite.h:159-168)."""
+    while di > 1:
+        di >>= 1
+        if xs[i + di] < x:
+            i += di
+    return i
+
+
+def spline_uniform(x0, dx, ydys, xs):
+    """subsample_uniform_spline (ProbeParticle.cpp:752-763); samples beyond
+    the last interval extrapolate it."""
+    n = len(ydys)
+    ys = np.zeros(len(xs))
+    for j, x in enumerate(xs):
+        u = (x - x0) * (1 / dx)
+        i = min(max(int(u), 0), n - 2)
+        u -= i
+        ys[j] = hermite(u, ydys[i, 0], ydys[i + 1, 0], ydys[i, 1] * dx, ydys[i + 1, 1] * dx)
+    return ys
+
+
+def spline_nonuniform(kx, ydys, xs):
+    """subsample_nonuniform_spline (ProbeParticle.cpp:765-784): the interval
+    found for one sample is kept while later samples stay below its end."""
+    n = len(kx)
+    ys = np.zeros(len(xs))
+    i, x0, x1, dx, denom = 0, 0.0, -1e300, 1.0, 1.0
+    for j, x in enumerate(xs):
+        if x > x1:
+            i = min(find_index(i, n - i, x, kx), n - 2)
+            x0, x1 = kx[i], kx[i + 1]
+            dx = x1 - x0
+            denom = 1 / dx
+        u = (x - x0) * denom
+        ys[j] = hermite(u, ydys[i, 0], ydys[i + 1, 0], ydys[i, 1] * dx, ydys[i + 1, 1] * dx)
+    return ys
+
+
+def d3_coeffs(Rs, iZs, tables, params, iZPP):
+    """computeD3Coeffs (ProbeParticle.cpp:397-506) in numpy: coordination
+    numbers, Gaussian reference weights (kept in FP32 as there), C6, C8 and
+    the damping radii; the probe's unused reference slots get weight 0."""
+    Rs = _f64(Rs, (-1, 3))
+    k1, k2, k3 = tables["K"][0], tables["K"][1], -tables["K"][2]
+    k12 = -k1 * k2
+    s6, s8, a1, a2 = params
+    rc, rcn, c6t, r4r2 = tables["R_COV"], np.asarray(tables["REF_CN"]).reshape(94, 5), np.asarray(tables["REF_C6"]).reshape(94, 94, 5, 5), tables["R4R2"]
+    pp = iZPP - 1
+    lpp = np.array([np.exp(k3 * c * c) if c >= 0.0 else 0.0 for c in rcn[pp]])
+    out = np.zeros((len(Rs), 4))
+    for ia in range(len(Rs)):
+        el = iZs[ia] - 1
+        cn = 0.0
+        for j in range(len(Rs)):
+            if j == ia:
+                continue
+            d = np.sqrt(np.sum((Rs[j] - Rs[ia]) ** 2))
+            cn += 1.0 / (1.0 + np.exp(k12 * (rc[iZs[j] - 1] + rc[el]) / d + k1))
+        L = np.zeros((5, 5), dtype=np.float32)
+        norm = 0.0
+        na = 0
+        while na < 5 and rcn[el, na] >= 0.0:
+            la = np.exp(k3 * (rcn[el, na] - cn) ** 2)
+            for b in range(5):
+                norm += la * lpp[b]
+                L[na, b] = la * lpp[b]
+            na += 1
+        if norm == 0 and na > 0:
+            for b in range(5):
+                norm += lpp[b]
+                L[na - 1, b] = lpp[b]
+        c6 = 0.0
+        for a in range(na):
+            for b in range(5):
+                c6 += float(L[a, b]) * c6t[el, pp, a, b]
+        c6 /= norm
+        qq = 3 * r4r2[el] * r4r2[pp]
+        r0 = a1 * np.sqrt(qq) + a2
+        out[ia] = (c6 * s6, qq * c6 * s8, r0 ** 6, r0 ** 8)
+    return out
+
+
+def stiffness(n, cell, F, tip, rTips, rPPs, which=0, ddisp=0.05, want_dynmat=False, spline=None):
     """stiffnessMatrix (ProbeParticle.cpp:703-750) at positions rPPs (m,3)
     for tips rTips (m,3): eigenvalues (m,3) ascending, the list of three
     eigenvector arrays (the first ``which`` filled, others None), the number
@@ -207,8 +353,10 @@ def stiffness(n, cell, F, tip, rTips, rPPs, which=0, ddisp=0.05, want_dynmat=Fal
     vecs = [np.zeros((m, 3)) if k < which else None for k in range(3)]
     D = np.zeros((m, 3, 3)) if want_dynmat else None
     n = np.ascontiguousarray(n, dtype=np.int32)
-    nnan = lib().orc_stiffness(_ptr(n, _ip), _ptr(_f64(cell, (3, 3))), _ptr(_f64(F)), _ptr(_f64(tip)), float(ddisp),
-                               int(which), m, _ptr(rTips), _ptr(rPPs), _ptr(ev), *[_ptr(v) for v in vecs], _ptr(D))
+    sxs, sydys = (None, None) if spline is None else (_f64(spline[0]), _f64(spline[1]))
+    nnan = lib().orc_stiffness_spline(_ptr(n, _ip), _ptr(_f64(cell, (3, 3))), _ptr(_f64(F)), _ptr(_f64(tip)),
+                                      0 if spline is None else len(sxs), _ptr(sxs), _ptr(sydys), float(ddisp),
+                                      int(which), m, _ptr(rTips), _ptr(rPPs), _ptr(ev), *[_ptr(v) for v in vecs], _ptr(D))
     out = (ev, vecs, int(nnan))
     return out + (D,) if want_dynmat else out
 
@@ -380,6 +528,17 @@ class Reference:
         L.relaxTipStrokes_omp.restype = ctypes.c_int
         L.stiffnessMatrix.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_int] + [_dp] * 6 + [ctypes.c_void_p]
         L.stiffnessMatrix.restype = None
+        for name in ("getVdWFF", "getDFTD3FF", "getGaussDensity", "getSlaterDensity", "getDensityR4spline"):
+            getattr(L, name).argtypes = [ctypes.c_int, _dp, _dp]
+        L.getVdWFF_RE.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int, ctypes.c_double]
+        L.getMorseFF.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_double]
+        L.getInPoints_LJ.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, _dp]
+        L.evalRadialFF.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_double]
+        L.computeD3Coeffs.argtypes = [ctypes.c_int, _dp, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int, _dp]
+        L.subsample_uniform_spline.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp]
+        L.subsample_nonuniform_spline.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, _dp]
+        L.test_force.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_void_p]
+        L.test_eigen3x3.argtypes = [_dp, _dp]
         self.L = L
         self._keep = []
 
@@ -429,15 +588,86 @@ class Reference:
         return rs, fs, itr
 
 
-    def stiffness(self, rTips, rPPs, which=0, ddisp=0.05):
+    def stiffness(self, rTips, rPPs, which=0, ddisp=0.05, spline=None):
         """stiffnessMatrix on the current grid and tip (rTips, rPPs (m,3))."""
         rTips = _f64(rTips, (-1, 3))
         rPPs = _f64(rPPs, (-1, 3))
         m = len(rTips)
         ev = np.zeros((m, 3))
         vecs = [np.zeros((m, 3)) if k < which else None for k in range(3)]
-        self.L.stiffnessMatrix(float(ddisp), int(which), m, _ptr(rTips), _ptr(rPPs), _ptr(ev), *[_ptr(v) for v in vecs], None)
+        self.L.stiffnessMatrix(float(ddisp), int(which), m, _ptr(rTips), _ptr(rPPs), _ptr(ev), *[_ptr(v) for v in vecs],
+                               self._spline(spline))
         return ev, vecs
+
+    def _spline(self, spline):
+        """TIP::SplineParams (ProbeParticle.cpp:58-62) for (xs, ydys) or None."""
+        if spline is None:
+            return None
+        xs, ydys = _f64(spline[0]), _f64(spline[1])
+
+        class SP(ctypes.Structure):
+            _fields_ = [("n", ctypes.c_int), ("xs", _dp), ("ydys", _dp)]
+
+        sp = SP(len(xs), _ptr(xs), _ptr(ydys))
+        self._keep_spline = (xs, ydys, sp)
+        return ctypes.cast(ctypes.pointer(sp), ctypes.c_void_p)
+
+    def relax_strokes_spline(self, rTips, spline, probeStart=1, relaxAlg=1):
+        rTips = _f64(rTips)
+        nx, ny, nz, _ = rTips.shape
+        rs = np.zeros_like(rTips)
+        fs = np.zeros_like(rTips)
+        self.L.relaxTipStrokes_omp(nx, ny, int(probeStart), int(relaxAlg), nz, _ptr(rTips), _ptr(rs), _ptr(fs), self._spline(spline))
+        return rs, fs
+
+    def grid_model(self, name, Rs, coefs, *extra):
+        """One of the grid builders that take (natoms, Rs, coefs, ...)."""
+        Rs = _f64(Rs, (-1, 3))
+        coefs = _f64(coefs)
+        getattr(self.L, name)(len(Rs), _ptr(Rs), _ptr(coefs), *extra)
+
+    def points_lj(self, ps, Rs, cLJs):
+        ps, Rs, cLJs = _f64(ps, (-1, 3)), _f64(Rs, (-1, 3)), _f64(cLJs, (-1, 2))
+        FEs = np.zeros((len(ps), 4))
+        self.L.getInPoints_LJ(len(ps), _ptr(ps), _ptr(FEs), len(Rs), _ptr(Rs), _ptr(cLJs))
+        return FEs
+
+    def radial(self, rs, coefs, kind, ADamp=-1.0):
+        rs, coefs = _f64(rs), _f64(coefs)
+        Es, Fs = np.zeros(len(rs)), np.zeros(len(rs))
+        self.L.evalRadialFF(len(rs), _ptr(rs), _ptr(coefs), _ptr(Es), _ptr(Fs), int(kind), float(ADamp))
+        return Es, Fs
+
+    def d3_coeffs(self, Rs, iZs, tables, params, iZPP):
+        Rs = _f64(Rs, (-1, 3))
+        iZs = np.ascontiguousarray(iZs, dtype=np.int32)
+        t = {k: _f64(v).ravel() for k, v in tables.items()}
+        out = np.zeros(4 * len(Rs))
+        self.L.computeD3Coeffs(len(Rs), _ptr(Rs), _ptr(iZs, _ip), _ptr(t["R_COV"]), _ptr(t["R0"]), _ptr(t["REF_CN"]),
+                               _ptr(t["REF_C6"]), _ptr(t["R4R2"]), _ptr(t["K"]), _ptr(_f64(params)), int(iZPP), _ptr(out))
+        return out.reshape(-1, 4)
+
+    def spline_uniform(self, x0, dx, ydys, xs):
+        ydys, xs = _f64(ydys), _f64(xs)
+        ys = np.zeros(len(xs))
+        self.L.subsample_uniform_spline(float(x0), float(dx), len(ydys), _ptr(ydys), len(xs), _ptr(xs), _ptr(ys))
+        return ys
+
+    def spline_nonuniform(self, kx, ydys, xs):
+        kx, ydys, xs = _f64(kx), _f64(ydys), _f64(xs)
+        ys = np.zeros(len(xs))
+        self.L.subsample_nonuniform_spline(len(kx), _ptr(kx), _ptr(ydys), len(xs), _ptr(xs), _ptr(ys))
+        return ys
+
+    def test_force(self, typ, n, r0, dr, R, spline=None):
+        fs = np.zeros((n, 3))
+        self.L.test_force(int(typ), n, _ptr(_f64(r0)), _ptr(_f64(dr)), _ptr(_f64(R)), _ptr(fs), self._spline(spline))
+        return fs
+
+    def eigen3x3(self, mat):
+        evs = np.zeros((4, 3))
+        self.L.test_eigen3x3(_ptr(_f64(mat)), _ptr(evs))
+        return evs
 
 
 def have_reference():

@@ -246,7 +246,39 @@ typedef struct {
     double lrad, krad;
     double rpp0[3];
     double kspring[3];
+    /* radial force spline (TIP::SplineParams); sn = 0: harmonic spring */
+    int sn;
+    const double *sxs, *sydys;
 } orc_tip;
+
+/* Spline_Hermite::find_index / ::val (spline_hermite.h:45-51, 159-168) and
+ * forceRSpline (ProbeParticle.cpp:211-224): radial force / distance. */
+static inline int find_index(int i, int di, double x, const double *xs)
+{
+    while (di > 1) {
+        di >>= 1;
+        int j = i + di;
+        if (xs[j] < x) i = j;
+    }
+    return i;
+}
+
+static inline double hermite(double x, double y0, double y1, double dy0, double dy1)
+{
+    double y01 = y0 - y1;
+    return y0 + x * (dy0 + x * (-3 * y01 - 2 * dy0 - dy1 + x * (2 * y01 + dy0 + dy1)));
+}
+
+static inline double spline_scale(const orc_tip *t, double x)
+{
+    int i = find_index(0, t->sn, x, t->sxs);
+    if (i > t->sn - 2) i = t->sn - 2;  /* beyond the last knot: extrapolate */
+    double x0 = t->sxs[i], x1 = t->sxs[i + 1];
+    double dx = x1 - x0, denom = 1 / dx, u = (x - x0) * denom;
+    i = i << 1;
+    double fr = hermite(u, t->sydys[i], t->sydys[i + 2], t->sydys[i + 1] * dx, t->sydys[i + 3] * dx);
+    return fr / x;
+}
 
 /* RELAX + FIRE namespaces (ProbeParticle.cpp:74-114). */
 typedef struct {
@@ -270,7 +302,7 @@ static inline v3 pp_force(const orc_field *ff, const orc_tip *tip, v3 rtip, v3 r
     v3 d = {r.x - rtip.x, r.y - rtip.y, r.z - rtip.z};
     v3 f = interp_wrap(ff->F, ff->n, g);
     double l = sqrt(v3_norm2(d));
-    double s = tip->krad * (l - tip->lrad) / l;
+    double s = tip->sn ? spline_scale(tip, l) : tip->krad * (l - tip->lrad) / l;
     f.x += d.x * s; f.y += d.y * s; f.z += d.z * s;
     d.x -= tip->rpp0[0]; d.y -= tip->rpp0[1]; d.z -= tip->rpp0[2];
     f.x += d.x * tip->kspring[0]; f.y += d.y * tip->kspring[1]; f.z += d.z * tip->kspring[2];
@@ -372,6 +404,7 @@ static void make_tip(orc_tip *t, const double tip[8])
     t->lrad = tip[0]; t->krad = tip[1];
     t->rpp0[0] = tip[2]; t->rpp0[1] = tip[3]; t->rpp0[2] = tip[4];
     t->kspring[0] = tip[5]; t->kspring[1] = tip[6]; t->kspring[2] = tip[7];
+    t->sn = 0; t->sxs = 0; t->sydys = 0;
     if (fabs(t->krad) > 1e-8)
         t->rpp0[2] = -sqrt(t->lrad * t->lrad - t->rpp0[0] * t->rpp0[0] - t->rpp0[1] * t->rpp0[1]);
 }
@@ -411,6 +444,24 @@ long orc_relax_strokes(int nx, int ny, int nz, const int n[3], const double cell
             long off = ((long)ix + (long)iy * nx) * nz;
             total += relax_stroke(&ff, &t, &r, probe_start, alg, nz, rtips + 3 * off, rs + 3 * off,
                                   fs + 3 * off, iters ? iters + off : 0, nevals ? nevals + off : 0);
+        }
+    return total;
+}
+
+/* relaxTipStrokes_omp with a radial tip spline of sn knots (sxs, sydys). */
+long orc_relax_strokes_spline(int nx, int ny, int nz, const int n[3], const double cell[9], const double *F,
+                              const double tip[8], const double relax[7], int sn, const double *sxs,
+                              const double *sydys, const double *rtips, double *rs, double *fs)
+{
+    orc_field ff; orc_tip t; orc_relax r;
+    make_field(&ff, n, cell, F); make_tip(&t, tip); make_relax(&r, relax);
+    t.sn = sn; t.sxs = sxs; t.sydys = sydys;
+    long total = 0;
+    #pragma omp parallel for collapse(2) schedule(static) reduction(+ : total)
+    for (int ix = 0; ix < nx; ix++)
+        for (int iy = 0; iy < ny; iy++) {
+            long off = ((long)ix + (long)iy * nx) * nz;
+            total += relax_stroke(&ff, &t, &r, 1, 1, nz, rtips + 3 * off, rs + 3 * off, fs + 3 * off, 0, 0);
         }
     return total;
 }
@@ -702,6 +753,148 @@ long orc_relax_strokes_warm(int nx, int ny, int nz, const int n[3], const double
     return total;
 }
 
+/* ----------------------------------------------------- further pair models */
+
+/* Pair terms of Forces.h:108-232 and ProbeParticle.cpp:227-256, each in the
+ * reference's operation order.  model: 4 vdW with constant damping (c6;
+ * par = ADamp), 5..8 vdW with damping function R2/R4/invR4/invR8 (R, E0;
+ * par = ADamp), 9 DFT-D3 (c6, c8, R0^6, R0^8), 10 Morse (R0, E0; par =
+ * alpha), 11/12/13 Gauss/Slater/R4 densities (amp, width; energy only),
+ * 14 LJ in (R0, E0), 15 bare r^-6 (c6). */
+static inline double damp_fn(int kind, double u2, double *dd)
+{
+    if (kind == 1) {
+        if (u2 > 1) { *dd = 0; return 0; }
+        *dd = -2;
+        return 1 - u2;
+    }
+    if (kind == 2) {
+        if (u2 > 1) { *dd = 0; return 0; }
+        double e = 1 - u2;
+        *dd = -4 * e;
+        return e * e;
+    }
+    double ir2 = 1 / (u2 + ORC_R2SAFE);
+    if (kind == 3) {
+        double ir4 = ir2 * ir2;
+        *dd = -4 * ir4 * ir2;
+        return ir4;
+    }
+    double ir8 = ir2 * ir2;
+    ir8 *= ir8;
+    *dd = -8 * ir8 * ir2;
+    return ir8;
+}
+
+static double pair_model(int model, v3 d, v3 *f, const double *c, double par)
+{
+    double r2 = v3_norm2(d), s, e;
+    switch (model) {
+    case 4:
+        r2 *= r2;
+        r2 *= r2;
+        s = -6 * c[0] / (r2 + par * c[0]);
+        f->x += d.x * s; f->y += d.y * s; f->z += d.z * s;
+        return 0;
+    case 5: case 6: case 7: case 8: {
+        double ir2 = 1. / (c[0] * c[0]), u2 = r2 * ir2, u4 = u2 * u2, dd;
+        double D = damp_fn(model - 4, u2, &dd);
+        double ee = 1. / (u4 * u2 + D * par);
+        e = -2 * c[1] * ee;
+        s = e * ee * (6 * u4 + dd * par) * ir2;
+        f->x += d.x * s; f->y += d.y * s; f->z += d.z * s;
+        return e;
+    }
+    case 9: {
+        if (r2 > 400.0) return 0;
+        double r4 = r2 * r2, r6 = r4 * r2, r8 = r6 * r2;
+        double d6 = 1.0 / (r6 + c[2]), d8 = 1.0 / (r8 + c[3]);
+        double e6 = -c[0] * d6, e8 = -c[1] * d8;
+        s = e6 * d6 * 6 * r4 + e8 * d8 * 8 * r6;
+        f->x += d.x * s; f->y += d.y * s; f->z += d.z * s;
+        return e6 + e8;
+    }
+    case 10: {
+        double r = sqrt(r2 + ORC_R2SAFE);
+        double ex = exp(par * (r - c[0]));
+        e = c[1] * (ex * ex - 2 * ex);
+        s = c[1] * 2 * par * (ex - ex * ex) / r;
+        f->x += d.x * s; f->y += d.y * s; f->z += d.z * s;
+        return e;
+    }
+    case 11: {
+        double t = r2 * (1 / c[1]);
+        return c[0] * exp(-0.5 * (t * t));
+    }
+    case 12:
+        return c[0] * exp(-(1 / c[1]) * sqrt(r2));
+    case 13: {
+        double s2 = c[1] * c[1];
+        if (r2 > s2) return 0;
+        r2 /= s2;
+        double rf = 1 - r2;
+        return c[0] * rf * rf;
+    }
+    case 14: {
+        double ir2 = 1 / (r2 + ORC_R2SAFE), u2 = c[0] * c[0] * ir2, u6 = u2 * u2 * u2;
+        double e6 = c[1] * u6, e12 = e6 * u6;
+        s = 12 * (e12 - e6) * ir2;
+        f->x += d.x * s; f->y += d.y * s; f->z += d.z * s;
+        return e12 - 2 * e6;
+    }
+    case 15: {
+        double ir2 = 1 / r2, ir4 = ir2 * ir2;
+        e = -c[0] * ir4 * ir2;
+        s = e * 6 * ir2;
+        f->x += d.x * s; f->y += d.y * s; f->z += d.z * s;
+        return e;
+    }
+    }
+    return 0;
+}
+
+/* evalCell<model> over the grid (ProbeParticle.cpp:262-278, 515-631):
+ * += into F (unless a density) and E where non-NULL; coefs (natoms,ncoef). */
+void orc_model_ff(int model, double par, const int n[3], const double cell[9], int natoms, const double *Rs,
+                  const double *coefs, int ncoef, double *F, double *E)
+{
+    double dcell[9], dinv[9];
+    orc_grid_shape(n, cell, dcell, dinv);
+    const int nx = n[0], ny = n[1], nz = n[2];
+    const int dens = model >= 11 && model <= 13;
+    #pragma omp parallel for collapse(3) schedule(static)
+    for (int ia = 0; ia < nx; ia++)
+        for (int ib = 0; ib < ny; ib++)
+            for (int ic = 0; ic < nz; ic++) {
+                v3 p;
+                p.x = dcell[0] * ia + dcell[3] * ib + dcell[6] * ic;
+                p.y = dcell[1] * ia + dcell[4] * ib + dcell[7] * ic;
+                p.z = dcell[2] * ia + dcell[5] * ib + dcell[8] * ic;
+                v3 f = {0, 0, 0};
+                double e = 0;
+                for (int i = 0; i < natoms; i++) {
+                    v3 d = {p.x - Rs[3 * i], p.y - Rs[3 * i + 1], p.z - Rs[3 * i + 2]};
+                    e += pair_model(model, d, &f, coefs + (long)i * ncoef, par);
+                }
+                long ig = ((long)ia * ny + ib) * nz + ic;
+                if (F && !dens) { F[3 * ig] += f.x; F[3 * ig + 1] += f.y; F[3 * ig + 2] += f.z; }
+                if (E) E[ig] += e;
+            }
+}
+
+/* evalRadialFF (ProbeParticle.cpp:556-575) for one model at separations
+ * (r,0,0): Es, Fs (x force). */
+void orc_radial(int model, double par, int n, const double *rs, const double *coefs, double *Es, double *Fs)
+{
+    for (int i = 0; i < n; i++) {
+        v3 f = {0, 0, 0};
+        v3 d = {rs[i], 0, 0};
+        double e = model == 0 ? pair_lj(d, &f, coefs[0], coefs[1]) : pair_model(model, d, &f, coefs, par);
+        Es[i] = e;
+        Fs[i] = f.x;
+    }
+}
+
 /* ------------------------------------------------------------- stiffness */
 
 /* Mat3::eigenvals (Mat3.h:459-482) of a symmetric matrix m (row-major 3x3):
@@ -763,12 +956,14 @@ static void ref_eigenvec(const double m[9], double lam, double out[3])
  * into ev1/ev2/ev3 (n,3).  dynmat (n,9), optional, receives the symmetric
  * matrix.  Returns the number of points with a NaN eigenvalue (where the
  * reference exits). */
-long orc_stiffness(const int n[3], const double cell[9], const double *F, const double tip[8], double ddisp,
-                   int which, long np, const double *rtips, const double *rpps, double *evals, double *ev1,
-                   double *ev2, double *ev3, double *dynmat)
+long orc_stiffness_spline(const int n[3], const double cell[9], const double *F, const double tip[8], int sn,
+                          const double *sxs, const double *sydys, double ddisp, int which, long np,
+                          const double *rtips, const double *rpps, double *evals, double *ev1, double *ev2,
+                          double *ev3, double *dynmat)
 {
     orc_field ff; orc_tip t;
     make_field(&ff, n, cell, F); make_tip(&t, tip);
+    t.sn = sn; t.sxs = sxs; t.sydys = sydys;
     long nnan = 0;
     #pragma omp parallel for schedule(static) reduction(+ : nnan)
     for (long i = 0; i < np; i++) {
@@ -800,6 +995,14 @@ long orc_stiffness(const int n[3], const double cell[9], const double *F, const 
         if (dynmat) memcpy(dynmat + 9 * i, m, sizeof m);
     }
     return nnan;
+}
+
+long orc_stiffness(const int n[3], const double cell[9], const double *F, const double tip[8], double ddisp,
+                   int which, long np, const double *rtips, const double *rpps, double *evals, double *ev1,
+                   double *ev2, double *ev3, double *dynmat)
+{
+    return orc_stiffness_spline(n, cell, F, tip, 0, 0, 0, ddisp, which, np, rtips, rpps, evals, ev1, ev2, ev3,
+                                dynmat);
 }
 
 /* ------------------------------------------------------- OpenCL-mode grid */

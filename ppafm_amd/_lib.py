@@ -105,6 +105,20 @@ SIGNATURES = [
     ("relaxTipStroke", c_int, [c_int, c_int, c_int, _dp, _dp, _dp, ctypes.POINTER(SplineParams)]),
     ("relaxTipStrokes_omp", c_int, [c_int, c_int, c_int, c_int, c_int, _dp, _dp, _dp, ctypes.POINTER(SplineParams)]),
     ("stiffnessMatrix", None, [c_double, c_int, c_int, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.POINTER(SplineParams)]),
+    ("getVdWFF", None, [c_int, _dp, _dp]),
+    ("getVdWFF_RE", None, [c_int, _dp, _dp, c_int, c_double]),
+    ("getDFTD3FF", None, [c_int, _dp, _dp]),
+    ("getMorseFF", None, [c_int, _dp, _dp, c_double]),
+    ("getGaussDensity", None, [c_int, _dp, _dp]),
+    ("getSlaterDensity", None, [c_int, _dp, _dp]),
+    ("getDensityR4spline", None, [c_int, _dp, _dp]),
+    ("getInPoints_LJ", None, [c_int, _dp, _dp, c_int, _dp, _dp]),
+    ("evalRadialFF", None, [c_int, _dp, _dp, _dp, _dp, c_int, c_double]),
+    ("computeD3Coeffs", None, [c_int, _dp, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _dp, c_int, _dp]),
+    ("subsample_uniform_spline", None, [c_double, c_double, c_int, _dp, c_int, _dp, _dp]),
+    ("subsample_nonuniform_spline", None, [c_int, _dp, _dp, c_int, _dp, _dp]),
+    ("test_force", None, [c_int, c_int, _dp, _dp, _dp, _dp, ctypes.POINTER(SplineParams)]),
+    ("test_eigen3x3", None, [_dp, _dp]),
     ("ppafm_pack_atoms", c_int, [c_int, _dp, _dp, c_double, c_void_p]),
     ("ppafm_ff_build_dev", c_int, [ctypes.POINTER(Grid), ctypes.POINTER(FFModel), c_int, c_void_p, c_void_p, c_void_p]),
     ("ppafm_ff_build", c_int, [ctypes.POINTER(Grid), ctypes.POINTER(FFModel), c_int, _dp, _dp, c_void_p, c_void_p]),

@@ -78,7 +78,11 @@ struct RefState {
     double conv_f2 = 1.0e-8, dt = 0.1, damping = 0.1;
     double finc = 1.1, fdec = 0.5, falpha = 0.99;
     long long last_itrsum = 0;
-    DevBuf dF, dE, dAtoms, dTips, dPos, dFs, dIters, dEv;
+    // damping constants of the vdW models (ProbeParticle.cpp:48); a positive
+    // ADamp passed to getVdWFF_RE / evalRadialFF replaces the model's value
+    // for later calls too, as in the reference
+    double adamp_const = 180.0, adamp_r2 = 0.5, adamp_r4 = 0.5, adamp_invr4 = 0.03, adamp_invr8 = 0.01;
+    DevBuf dF, dE, dAtoms, dTips, dPos, dFs, dIters, dEv, dCoef, dSpline, dAux;
 };
 
 RefState &ref()
@@ -94,29 +98,33 @@ int ref_grid_ready(RefState &S)
     return 0;
 }
 
-int ref_accum(int model, int natoms, const double *Rs, const double *coefs)
+// evalCell<model> over the process-wide grid (ProbeParticle.cpp:262-278):
+// += into the F and E grids that are set (E only for the densities, which
+// the reference computes with the force grid switched off).
+int ref_accum(int model, double par, int natoms, const double *Rs, const double *coefs, int ncoef,
+              bool energy_only = false)
 {
     RefState &S = ref();
     std::lock_guard<std::mutex> lk(S.mu);
     int e = ref_grid_ready(S);
     if (e) return e;
     PP_REQUIRE(natoms >= 0, "natoms must be >= 0");
-    if (!S.F && !S.E) return 0;
+    PP_REQUIRE(natoms == 0 || (Rs && coefs), "null atom positions or coefficients");
+    double *hF = energy_only ? nullptr : S.F;
+    if (!hF && !S.E) return 0;
     const GridGeom g = make_geom(S.n, S.cell);
     const size_t npts = (size_t)S.n[0] * S.n[1] * S.n[2];
-    std::vector<AtomRec> h(natoms);
-    for (int i = 0; i < natoms; i++) {
-        AtomRec a{Rs[3 * i], Rs[3 * i + 1], Rs[3 * i + 2], 0, 0, 0};
-        if (model == 0) { a.c6 = coefs[2 * i]; a.c12 = coefs[2 * i + 1]; }
-        else a.q = coefs[i];
-        h[i] = a;
+    const size_t na = natoms > 0 ? natoms : 1;
+    if ((e = S.dAtoms.ensure(na * 3 * sizeof(double)))) return e;
+    if ((e = S.dCoef.ensure(na * (ncoef > 0 ? ncoef : 1) * sizeof(double)))) return e;
+    if (natoms) {
+        PP_TRY_HIP(hipMemcpy(S.dAtoms.p, Rs, (size_t)natoms * 3 * sizeof(double), hipMemcpyHostToDevice));
+        PP_TRY_HIP(hipMemcpy(S.dCoef.p, coefs, (size_t)natoms * ncoef * sizeof(double), hipMemcpyHostToDevice));
     }
-    if ((e = S.dAtoms.ensure(sizeof(AtomRec) * (natoms > 0 ? natoms : 1)))) return e;
-    if (natoms) PP_TRY_HIP(hipMemcpy(S.dAtoms.p, h.data(), sizeof(AtomRec) * natoms, hipMemcpyHostToDevice));
     double *dF = nullptr, *dE = nullptr;
-    if (S.F) {
+    if (hF) {
         if ((e = S.dF.ensure(npts * 3 * sizeof(double)))) return e;
-        PP_TRY_HIP(hipMemcpy(S.dF.p, S.F, npts * 3 * sizeof(double), hipMemcpyHostToDevice));
+        PP_TRY_HIP(hipMemcpy(S.dF.p, hF, npts * 3 * sizeof(double), hipMemcpyHostToDevice));
         dF = (double *)S.dF.p;
     }
     if (S.E) {
@@ -124,10 +132,29 @@ int ref_accum(int model, int natoms, const double *Rs, const double *coefs)
         PP_TRY_HIP(hipMemcpy(S.dE.p, S.E, npts * sizeof(double), hipMemcpyHostToDevice));
         dE = (double *)S.dE.p;
     }
-    if ((e = launch_ff_accum_f64(g, model, natoms, (const AtomRec *)S.dAtoms.p, dF, dE, nullptr))) return e;
+    if ((e = launch_ff_models(g, model, par, natoms, (const double *)S.dAtoms.p, (const double *)S.dCoef.p, ncoef,
+                              dF, dE, nullptr)))
+        return e;
     PP_TRY_HIP(hipGetLastError());
-    if (S.F) PP_TRY_HIP(hipMemcpy(S.F, S.dF.p, npts * 3 * sizeof(double), hipMemcpyDeviceToHost));
+    if (hF) PP_TRY_HIP(hipMemcpy(hF, S.dF.p, npts * 3 * sizeof(double), hipMemcpyDeviceToHost));
     if (S.E) PP_TRY_HIP(hipMemcpy(S.E, S.dE.p, npts * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// Upload a tip spline (TIP::SplineParams: n knots xs, values and slopes
+// ydys) into buf; fills L's spline fields.  NULL leaves the harmonic spring.
+int upload_spline(DevBuf &buf, const ppafm_spline_params *sp, RelaxLaunch &L)
+{
+    if (!sp) return 0;
+    PP_REQUIRE(sp->rff_n >= 2 && sp->rff_xs && sp->rff_ydys, "tip spline: need n >= 2 knots and non-NULL xs, ydys");
+    int e = buf.ensure((size_t)sp->rff_n * 3 * sizeof(double));
+    if (e) return e;
+    double *d = (double *)buf.p;
+    PP_TRY_HIP(hipMemcpy(d, sp->rff_xs, (size_t)sp->rff_n * sizeof(double), hipMemcpyHostToDevice));
+    PP_TRY_HIP(hipMemcpy(d + sp->rff_n, sp->rff_ydys, (size_t)sp->rff_n * 2 * sizeof(double), hipMemcpyHostToDevice));
+    L.spline_n = sp->rff_n;
+    L.spline_xs = d;
+    L.spline_ydys = d + sp->rff_n;
     return 0;
 }
 
@@ -138,7 +165,6 @@ int ref_relax(int nx, int ny, int probeStart, int relaxAlg, int nstep, const dou
     std::lock_guard<std::mutex> lk(S.mu);
     int e = ref_grid_ready(S);
     if (e) return e;
-    PP_REQUIRE(spline == nullptr, "tip splines are not supported (pass None for the harmonic radial spring)");
     PP_REQUIRE(S.F != nullptr, "setFF_Fpointer has not been called");
     PP_REQUIRE(nx >= 0 && ny >= 0 && nstep >= 0, "negative scan size");
     const size_t npts = (size_t)S.n[0] * S.n[1] * S.n[2];
@@ -162,6 +188,7 @@ int ref_relax(int nx, int ny, int probeStart, int relaxAlg, int nstep, const dou
     L.scan = ppafm_scan{nx, ny, nstep, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     L.tips = (const double *)S.dTips.p;
     L.out = ppafm_relax_out{nullptr, 0, nullptr, nullptr, nullptr};
+    if ((e = upload_spline(S.dSpline, spline, L))) return e;
     if ((e = launch_relax_f64grid(L, (const double *)S.dF.p, (double *)S.dPos.p, (double *)S.dFs.p,
                                   (int *)S.dIters.p, nullptr)))
         return e;
@@ -189,7 +216,6 @@ int ref_stiffness(double ddisp, int which, int n, const double *rTips, const dou
     std::lock_guard<std::mutex> lk(S.mu);
     int e = ref_grid_ready(S);
     if (e) return e;
-    PP_REQUIRE(spline == nullptr, "tip splines are not supported (pass None for the harmonic radial spring)");
     PP_REQUIRE(S.F != nullptr, "setFF_Fpointer has not been called");
     PP_REQUIRE(n >= 0, "n must be >= 0");
     if (n == 0) return 0;
@@ -217,6 +243,7 @@ int ref_stiffness(double ddisp, int which, int n, const double *rTips, const dou
     L.tips = (const double *)S.dTips.p;
     L.out = ppafm_relax_out{nullptr, 0, nullptr, nullptr, nullptr};
     double *dv = (double *)S.dEv.p;
+    if ((e = upload_spline(S.dSpline, spline, L))) return e;
     if ((e = launch_stiffness(L, S.dF.p, 2, S.dPos.p, 0, ddisp, nv, (double *)S.dFs.p, dv, dv + 3 * (size_t)n,
                               dv + 6 * (size_t)n, nullptr)))
         return e;
@@ -231,6 +258,181 @@ int ref_stiffness(double ddisp, int which, int n, const double *rTips, const dou
             return -1;
         }
     return 0;
+}
+
+// Small host-array entry points: upload, one kernel, download.
+struct Staged {
+    std::vector<DevBuf> bufs;
+    explicit Staged(int n) : bufs(n) {}
+    ~Staged()
+    {
+        for (auto &b : bufs)
+            if (b.p) (void)hipFree(b.p);
+    }
+    // device copy of a host array (bytes may be 0)
+    int put(int k, const void *h, size_t bytes)
+    {
+        int e = bufs[k].ensure(bytes > 0 ? bytes : 8);
+        if (e) return e;
+        if (bytes) PP_TRY_HIP(hipMemcpy(bufs[k].p, h, bytes, hipMemcpyHostToDevice));
+        return 0;
+    }
+    int alloc(int k, size_t bytes) { return bufs[k].ensure(bytes > 0 ? bytes : 8); }
+    int get(int k, void *h, size_t bytes)
+    {
+        if (bytes) PP_TRY_HIP(hipMemcpy(h, bufs[k].p, bytes, hipMemcpyDeviceToHost));
+        return 0;
+    }
+    template <class T> T *p(int k) { return (T *)bufs[k].p; }
+};
+
+#define PP_E(x)                                                                                  \
+    do {                                                                                         \
+        int _r = (x);                                                                            \
+        if (_r) return _r;                                                                       \
+    } while (0)
+
+int host_points_lj(int npoints, const double *pts, double *fes, int natoms, const double *Rs, const double *cljs)
+{
+    PP_REQUIRE(npoints >= 0 && natoms >= 0, "negative count");
+    if (npoints == 0) return 0;
+    PP_REQUIRE(pts && fes && (natoms == 0 || (Rs && cljs)), "null argument");
+    Staged S(4);
+    PP_E(S.put(0, pts, (size_t)npoints * 3 * sizeof(double)));
+    PP_E(S.put(1, Rs, (size_t)natoms * 3 * sizeof(double)));
+    PP_E(S.put(2, cljs, (size_t)natoms * 2 * sizeof(double)));
+    PP_E(S.alloc(3, (size_t)npoints * 4 * sizeof(double)));
+    PP_E(launch_lj_points(npoints, S.p<double>(0), natoms, S.p<double>(1), S.p<double>(2), S.p<double>(3), nullptr));
+    PP_TRY_HIP(hipGetLastError());
+    return S.get(3, fes, (size_t)npoints * 4 * sizeof(double));
+}
+
+// evalRadialFF kinds: -3 LJ in (R, E), -2 LJ (c6, c12), -1 bare r^-6, 0 vdW
+// with constant damping, 1..4 damping functions; others give zeros.
+int host_radial(int n, const double *rs, const double *coefs, double *es, double *fs, int kind, double adamp)
+{
+    PP_REQUIRE(n >= 0, "n must be >= 0");
+    if (n == 0) return 0;
+    PP_REQUIRE(rs && es && fs, "null argument");
+    RefState &R = ref();
+    int model = -1;
+    double par = 0;
+    {
+        std::lock_guard<std::mutex> lk(R.mu);
+        double *slot[5] = {&R.adamp_const, &R.adamp_r2, &R.adamp_r4, &R.adamp_invr4, &R.adamp_invr8};
+        if (kind >= 0 && kind <= 4) {
+            if (adamp > 0) *slot[kind] = adamp;
+            par = *slot[kind];
+            model = kind == 0 ? (int)kModelVdW : (int)kModelVdWR2 + kind - 1;
+        } else if (kind == -1) {
+            model = kModelInvR6;
+        } else if (kind == -2) {
+            model = kModelLJ;
+        } else if (kind == -3) {
+            model = kModelLJRE;
+        }
+    }
+    // the reference reads coefs[0] for the one-coefficient kinds and
+    // coefs[0..1] for the others, nothing for unknown kinds
+    const int nc = model < 0 ? 0 : ((kind == -1 || kind == 0) ? 1 : 2);
+    PP_REQUIRE(nc == 0 || coefs, "null coefs");
+    const double c0 = nc > 0 ? coefs[0] : 0.0, c1 = nc > 1 ? coefs[1] : 0.0;
+    Staged S(3);
+    PP_E(S.put(0, rs, (size_t)n * sizeof(double)));
+    PP_E(S.alloc(1, (size_t)n * sizeof(double)));
+    PP_E(S.alloc(2, (size_t)n * sizeof(double)));
+    PP_E(launch_radial(n, S.p<double>(0), model, c0, c1, par, S.p<double>(1), S.p<double>(2), nullptr));
+    PP_TRY_HIP(hipGetLastError());
+    PP_E(S.get(1, es, (size_t)n * sizeof(double)));
+    return S.get(2, fs, (size_t)n * sizeof(double));
+}
+
+int host_d3_coeffs(int natoms, const double *rs, const int *elems, const double *r_cov, const double *ref_cn,
+                   const double *ref_c6, const double *r4r2, const double *k, const double *params, int elem_pp,
+                   double *out)
+{
+    constexpr int kElem = 94, kRef = 5;  // ProbeParticle.cpp:28-29
+    PP_REQUIRE(natoms >= 0, "natoms must be >= 0");
+    if (natoms == 0) return 0;
+    PP_REQUIRE(rs && elems && r_cov && ref_cn && ref_c6 && r4r2 && k && params && out, "null argument");
+    PP_REQUIRE(elem_pp >= 1 && elem_pp <= kElem, "probe element out of range 1..94");
+    for (int i = 0; i < natoms; i++)
+        PP_REQUIRE(elems[i] >= 1 && elems[i] <= kElem, "atom element out of range 1..94");
+    Staged S(7);
+    PP_E(S.put(0, rs, (size_t)natoms * 3 * sizeof(double)));
+    PP_E(S.put(1, elems, (size_t)natoms * sizeof(int)));
+    PP_E(S.put(2, r_cov, kElem * sizeof(double)));
+    PP_E(S.put(3, ref_cn, kElem * kRef * sizeof(double)));
+    PP_E(S.put(4, ref_c6, (size_t)kElem * kElem * kRef * kRef * sizeof(double)));
+    PP_E(S.put(5, r4r2, kElem * sizeof(double)));
+    PP_E(S.alloc(6, (size_t)natoms * 4 * sizeof(double)));
+    PP_E(launch_d3_coeffs(natoms, S.p<double>(0), S.p<int>(1), S.p<double>(2), S.p<double>(3), S.p<double>(4),
+                          S.p<double>(5), k, params, elem_pp, S.p<double>(6), nullptr));
+    PP_TRY_HIP(hipGetLastError());
+    return S.get(6, out, (size_t)natoms * 4 * sizeof(double));
+}
+
+int host_spline(bool uniform, double x0, double dx, int n, const double *kx, const double *ydys, int m,
+                const double *xs, double *ys)
+{
+    PP_REQUIRE(n >= 2, "a spline needs at least 2 knots");
+    PP_REQUIRE(m >= 0, "m must be >= 0");
+    if (m == 0) return 0;
+    PP_REQUIRE(ydys && xs && ys && (uniform || kx), "null argument");
+    Staged S(4);
+    PP_E(S.put(0, ydys, (size_t)n * 2 * sizeof(double)));
+    PP_E(S.put(1, xs, (size_t)m * sizeof(double)));
+    PP_E(S.alloc(2, (size_t)m * sizeof(double)));
+    if (uniform) {
+        PP_E(launch_spline_uniform(x0, dx, n, S.p<double>(0), m, S.p<double>(1), S.p<double>(2), nullptr));
+    } else {
+        PP_E(S.put(3, kx, (size_t)n * sizeof(double)));
+        PP_E(launch_spline_nonuniform(n, S.p<double>(3), S.p<double>(0), m, S.p<double>(1), S.p<double>(2), nullptr));
+    }
+    PP_TRY_HIP(hipGetLastError());
+    return S.get(2, ys, (size_t)m * sizeof(double));
+}
+
+int host_test_force(int type, int n, const double *r0, const double *dr, const double *R, double *fs,
+                    const ppafm_spline_params *sp)
+{
+    PP_REQUIRE(n >= 0, "n must be >= 0");
+    if (n == 0) return 0;
+    PP_REQUIRE(r0 && dr && R && fs, "null argument");
+    RefState &RS = ref();
+    double krad, lrad;
+    {
+        std::lock_guard<std::mutex> lk(RS.mu);
+        krad = RS.krad;
+        lrad = RS.lrad;
+    }
+    Staged S(2);
+    int sn = 0;
+    if (type == 1) {
+        PP_REQUIRE(sp && sp->rff_n >= 2 && sp->rff_xs && sp->rff_ydys, "test_force type 1 needs a tip spline");
+        sn = sp->rff_n;
+        PP_E(S.alloc(0, (size_t)sn * 3 * sizeof(double)));
+        PP_TRY_HIP(hipMemcpy(S.p<double>(0), sp->rff_xs, (size_t)sn * sizeof(double), hipMemcpyHostToDevice));
+        PP_TRY_HIP(hipMemcpy(S.p<double>(0) + sn, sp->rff_ydys, (size_t)sn * 2 * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+        PP_E(S.alloc(0, 8));
+    }
+    PP_E(S.alloc(1, (size_t)n * 3 * sizeof(double)));
+    PP_E(launch_test_force(type, n, r0, dr, R, krad, lrad, sn, S.p<double>(0), S.p<double>(0) + sn, S.p<double>(1),
+                           nullptr));
+    PP_TRY_HIP(hipGetLastError());
+    return S.get(1, fs, (size_t)n * 3 * sizeof(double));
+}
+
+int host_eigen3x3(const double *mat, double *evs)
+{
+    PP_REQUIRE(mat && evs, "null argument");
+    Staged S(2);
+    PP_E(S.put(0, mat, 9 * sizeof(double)));
+    PP_E(S.alloc(1, 12 * sizeof(double)));
+    PP_E(launch_eigen3x3(S.p<double>(0), S.p<double>(1), nullptr));
+    PP_TRY_HIP(hipGetLastError());
+    return S.get(1, evs, 12 * sizeof(double));
 }
 
 // The reference ABI returns void: failures are reported on stderr (the
@@ -323,7 +525,10 @@ void setFIRE(double finc, double fdec, double falpha)
     S.falpha = falpha;
 }
 
-void getLennardJonesFF(int natoms, double *Rs, double *cLJs) { report(ref_accum(0, natoms, Rs, cLJs), "getLennardJonesFF"); }
+void getLennardJonesFF(int natoms, double *Rs, double *cLJs)
+{
+    report(ref_accum(kModelLJ, 0.0, natoms, Rs, cLJs, 2), "getLennardJonesFF");
+}
 
 void getCoulombFF(int natoms, double *Rs, double *kQQs, int kind)
 {
@@ -332,8 +537,88 @@ void getCoulombFF(int natoms, double *Rs, double *kQQs, int kind)
         report(-2, "getCoulombFF");
         return;
     }
-    report(ref_accum(1 + kind, natoms, Rs, kQQs), "getCoulombFF");
+    report(ref_accum(kModelCoulS + kind, 0.0, natoms, Rs, kQQs, 1), "getCoulombFF");
 }
+
+void getVdWFF(int natoms, double *Rs, double *cLJs)
+{
+    report(ref_accum(kModelVdW, ref().adamp_const, natoms, Rs, cLJs, 2), "getVdWFF");
+}
+
+// Kind 1..4 selects the damping function (R2, R4, invR4, invR8); other kinds
+// add nothing, as in the reference.
+void getVdWFF_RE(int natoms, double *Rs, double *REs, int kind, double ADamp)
+{
+    RefState &S = ref();
+    double *slot[5] = {nullptr, &S.adamp_r2, &S.adamp_r4, &S.adamp_invr4, &S.adamp_invr8};
+    if (kind < 1 || kind > 4) return;
+    {
+        std::lock_guard<std::mutex> lk(S.mu);
+        if (ADamp > 0) *slot[kind] = ADamp;
+    }
+    report(ref_accum(kModelVdWR2 + kind - 1, *slot[kind], natoms, Rs, REs, 2), "getVdWFF_RE");
+}
+
+void getDFTD3FF(int natoms, double *Rs, double *d3_coeffs)
+{
+    report(ref_accum(kModelD3, 0.0, natoms, Rs, d3_coeffs, 4), "getDFTD3FF");
+}
+
+void getMorseFF(int natoms, double *Rs, double *REs, double alpha)
+{
+    report(ref_accum(kModelMorse, alpha, natoms, Rs, REs, 2), "getMorseFF");
+}
+
+void getGaussDensity(int natoms, double *Rs, double *cRAs)
+{
+    report(ref_accum(kModelGauss, 0.0, natoms, Rs, cRAs, 2, true), "getGaussDensity");
+}
+
+void getSlaterDensity(int natoms, double *Rs, double *cRAs)
+{
+    report(ref_accum(kModelSlater, 0.0, natoms, Rs, cRAs, 2, true), "getSlaterDensity");
+}
+
+void getDensityR4spline(int natoms, double *Rs, double *cRAs)
+{
+    report(ref_accum(kModelR4, 0.0, natoms, Rs, cRAs, 2, true), "getDensityR4spline");
+}
+
+void getInPoints_LJ(int npoints, double *points, double *FEs, int natoms, double *Rs, double *cLJs)
+{
+    report(host_points_lj(npoints, points, FEs, natoms, Rs, cLJs), "getInPoints_LJ");
+}
+
+void evalRadialFF(int n, double *rs, double *coefs, double *Es, double *Fs, int kind, double ADamp)
+{
+    report(host_radial(n, rs, coefs, Es, Fs, kind, ADamp), "evalRadialFF");
+}
+
+void computeD3Coeffs(int natoms, const double *rs, const int *elems, const double *r_cov, const double *r_cut,
+                     const double *ref_cn, const double *ref_c6, const double *r4r2, const double *k,
+                     const double *params, int elem_pp, double *d3_coeffs)
+{
+    (void)r_cut;  // passed by ppafm/core.py but not used by the reference either
+    report(host_d3_coeffs(natoms, rs, elems, r_cov, ref_cn, ref_c6, r4r2, k, params, elem_pp, d3_coeffs),
+           "computeD3Coeffs");
+}
+
+void subsample_uniform_spline(double x0, double dx, int n, double *ydys, int m, double *xs, double *ys)
+{
+    report(host_spline(true, x0, dx, n, nullptr, ydys, m, xs, ys), "subsample_uniform_spline");
+}
+
+void subsample_nonuniform_spline(int n, double *xs, double *ydys, int m, double *xs_, double *ys_)
+{
+    report(host_spline(false, 0.0, 0.0, n, xs, ydys, m, xs_, ys_), "subsample_nonuniform_spline");
+}
+
+void test_force(int type, int n, double *r0, double *dr, double *R, double *fs, ppafm_spline_params *splineParams)
+{
+    report(host_test_force(type, n, r0, dr, R, fs, splineParams), "test_force");
+}
+
+void test_eigen3x3(double *mat, double *evs) { report(host_eigen3x3(mat, evs), "test_eigen3x3"); }
 
 int relaxTipStroke(int probeStart, int relaxAlg, int nstep, double *rTips, double *rs, double *fs,
                    ppafm_spline_params *splineParams)

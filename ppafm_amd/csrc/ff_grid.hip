@@ -24,8 +24,8 @@
 //     (io.py:766-778) and combines FF = clamp(LJ) + q clamp(El)
 //     (HighLevel.py:143-145) before the single FP32 store.
 //
-// Kernel ff_accum_f64 (host ABI): FP64 pair math, += into the caller's
-// float64 (nx,ny,nz,3) grid, one point per thread.
+// The host ABI's FP64 accumulation into the caller's grids is in
+// ff_models.hip.
 
 #include "pp_common.hpp"
 
@@ -473,91 +473,6 @@ __global__ __launch_bounds__(kThreads) void ff_build_f64_batch(FFArgs A0, const 
     ff_brick_f64<LJ, NS, Z0, WE, QS>(A, atoms + it.atom_off, grid + it.grid_off, (int)(bidx - prefix[m]), lpos, llj);
 }
 
-// ------------------------------------------------------------------ FP64
-
-struct F64Args {
-    int n[3];
-    long npts;
-    int natoms;
-    int model;  // 0 LJ, 1..3 Coulomb kind 0..2
-    double dcell[9];
-};
-
-constexpr int kTile64 = 256;
-
-__device__ __forceinline__ double coul64(double dx, double dy, double dz, double kqq, double &fx, double &fy,
-                                         double &fz, double r2safe)
-{
-    double ir2 = 1.0 / ((dx * dx + dy * dy + dz * dz) + r2safe);
-    double ir = sqrt(ir2);
-    double e = ir * kqq;
-    double s = e * ir2;
-    fx += dx * s; fy += dy * s; fz += dz * s;
-    return e;
-}
-
-__global__ __launch_bounds__(256) void ff_accum_f64(F64Args A, const AtomRec *__restrict__ atoms,
-                                                    double *__restrict__ F, double *__restrict__ E)
-{
-    __shared__ double4 sx[kTile64];
-    __shared__ double2 sc[kTile64];
-    const double r2safe = (double)1.0e-8f;  // Forces.h:12
-    const long ig = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = ig < A.npts;
-    const long igc = live ? ig : 0;
-    const int ic = (int)(igc % A.n[2]);
-    const long r = igc / A.n[2];
-    const int ib = (int)(r % A.n[1]);
-    const int ia = (int)(r / A.n[1]);
-    const double *D = A.dcell;
-    // iterateGrid3D_omp order: pos0 + a*ia + b*ib + c*ic
-    const double px = D[0] * ia + D[3] * ib + D[6] * ic;
-    const double py = D[1] * ia + D[4] * ib + D[7] * ic;
-    const double pz = D[2] * ia + D[5] * ib + D[8] * ic;
-    double fx = 0, fy = 0, fz = 0, e = 0;
-    for (int i0 = 0; i0 < A.natoms; i0 += kTile64) {
-        const int nt = min(kTile64, A.natoms - i0);
-        __syncthreads();
-        for (int j = threadIdx.x; j < nt; j += blockDim.x) {
-            const AtomRec a = atoms[i0 + j];
-            sx[j] = make_double4(a.x, a.y, a.z, a.q);
-            sc[j] = make_double2(a.c6, a.c12);
-        }
-        __syncthreads();
-        for (int j = 0; j < nt; j++) {
-            const double4 a = sx[j];
-            const double dx = px - a.x, dy = py - a.y, dz = pz - a.z;
-            if (A.model == 0) {
-                const double2 c = sc[j];
-                double ir2 = 1.0 / ((dx * dx + dy * dy + dz * dz) + r2safe);
-                double ir6 = ir2 * ir2 * ir2;
-                double e6 = c.x * ir6, e12 = c.y * ir6 * ir6;
-                double s = (12 * e12 - 6 * e6) * ir2;
-                fx += dx * s; fy += dy * s; fz += dz * s;
-                e += e12 - e6;
-            } else if (A.model == 1) {
-                e += coul64(dx, dy, dz, a.w, fx, fy, fz, r2safe);
-            } else if (A.model == 2) {
-                double gx = 0, gy = 0, gz = 0, ee = 0;
-                ee += coul64(dx, dy, dz - 0.1, -a.w, gx, gy, gz, r2safe);
-                ee += coul64(dx, dy, (dz - 0.1) + 2 * 0.1, a.w, gx, gy, gz, r2safe);
-                fx += gx * 10.0; fy += gy * 10.0; fz += gz * 10.0;
-                e += ee * 10.0;
-            } else {
-                double gx = 0, gy = 0, gz = 0, ee = 0;
-                ee += coul64(dx, dy, dz, -2 * a.w, gx, gy, gz, r2safe);
-                ee += coul64(dx, dy, dz - 0.1, a.w, gx, gy, gz, r2safe);
-                ee += coul64(dx, dy, (dz - 0.1) + 2 * 0.1, a.w, gx, gy, gz, r2safe);
-                fx += gx * 100.0; fy += gy * 100.0; fz += gz * 100.0;
-                e += ee * 100.0;
-            }
-        }
-    }
-    if (!live) return;
-    if (F) { F[3 * ig] += fx; F[3 * ig + 1] += fy; F[3 * ig + 2] += fz; }
-    if (E) E[ig] += e;
-}
-
 // One launch: a single grid (batch == nullptr) or every grid of a batch.
 struct BatchLaunch {
     const BatchGridItem *items;
@@ -696,23 +611,6 @@ int launch_ff_build_batch(const ppafm_ff_model &m, int nitems, const BatchGridIt
     const bool z0 = model_args(m, A);
     BatchLaunch b{items_dev, prefix_dev, nitems, nbricks, qskip};
     return dispatch(m, A, z0, atoms_dev, grids, f64, &b, st);
-}
-
-int launch_ff_accum_f64(const GridGeom &g, int model, int natoms, const AtomRec *atoms_dev, double *F, double *E,
-                        hipStream_t st)
-{
-    PP_REQUIRE(model >= 0 && model <= 3, "model must be 0 (LJ) or 1..3 (Coulomb kind 0..2)");
-    F64Args A;
-    for (int i = 0; i < 3; i++) A.n[i] = g.n[i];
-    A.npts = (long)g.n[0] * g.n[1] * g.n[2];
-    A.natoms = natoms;
-    A.model = model;
-    for (int i = 0; i < 9; i++) A.dcell[i] = g.dcell[i];
-    const long nblk = (A.npts + 255) / 256;
-    PP_REQUIRE(nblk < (1L << 31), "grid too large for one launch");
-    if (A.npts == 0) return 0;
-    hipLaunchKernelGGL(ff_accum_f64, dim3((unsigned)nblk), dim3(256), 0, st, A, atoms_dev, F, E);
-    return 0;
 }
 
 }  // namespace pp

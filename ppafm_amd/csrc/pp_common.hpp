@@ -87,10 +87,50 @@ long long ff_bricks(const int n[3]);
 int launch_ff_build_batch(const ppafm_ff_model &m, int nitems, const BatchGridItem *items_dev,
                           const long long *prefix_dev, long long nbricks, const AtomRec *atoms_dev, void *grids,
                           bool f64, bool qskip, hipStream_t st);
-// FP64 accumulate (+=) into double3 grid F (and E) for the host ABI.
-// model: 0 = LJ, 1..3 = Coulomb kind 0..2.
-int launch_ff_accum_f64(const GridGeom &g, int model, int natoms, const AtomRec *atoms_dev, double *F, double *E,
-                        hipStream_t st);
+// Pair models of the reference C ABI (ff_models.hip).  Coefficients per
+// atom: LJ (c6, c12); Coulomb (kqq); vdW (c6), parameter ADamp; vdW with a
+// damping function (R, E0), parameter ADamp; D3 (c6, c8, R0^6, R0^8); Morse
+// (R0, E0), parameter alpha; densities (amplitude, width); LJ in (R0, E0);
+// bare r^-6 (c6).  The three densities write energy only.
+enum FFModelF64 {
+    kModelLJ = 0,
+    kModelCoulS,
+    kModelCoulPz,
+    kModelCoulDz2,
+    kModelVdW,
+    kModelVdWR2,
+    kModelVdWR4,
+    kModelVdWInvR4,
+    kModelVdWInvR8,
+    kModelD3,
+    kModelMorse,
+    kModelGauss,
+    kModelSlater,
+    kModelR4,
+    kModelLJRE,
+    kModelInvR6,
+    kModelCount
+};
+// FP64 accumulate (+=) into the (npts,3) grid F and (npts) E, either may be
+// NULL; pos (natoms,3) relative to the grid origin, coefs (natoms,ncoef).
+int launch_ff_models(const GridGeom &g, int model, double par, int natoms, const double *pos, const double *coefs,
+                     int ncoef, double *F, double *E, hipStream_t st);
+// LJ at npoints points pts (3 each) from atoms pos/cljs -> fes (npoints,4).
+int launch_lj_points(long npoints, const double *pts, int natoms, const double *pos, const double *cljs, double *fes,
+                     hipStream_t st);
+// One pair model at separations (r,0,0): es = energy, fs = x force.
+int launch_radial(long n, const double *rs, int model, double c0, double c1, double par, double *es, double *fs,
+                  hipStream_t st);
+// DFT-D3 coefficients (c6 s6, c8 s8, R0^6, R0^8) per atom.
+int launch_d3_coeffs(int natoms, const double *pos, const int *elems, const double *r_cov, const double *ref_cn,
+                     const double *ref_c6, const double *r4r2, const double k[3], const double params[4], int elem_pp,
+                     double *out, hipStream_t st);
+int launch_spline_uniform(double x0, double dx, int n, const double *ydys, long m, const double *xs, double *ys,
+                          hipStream_t st);
+int launch_spline_nonuniform(int n, const double *kx, const double *ydys, long m, const double *xs, double *ys,
+                             hipStream_t st);
+int launch_test_force(int type, int n, const double r0[3], const double dr[3], const double R[3], double krad,
+                      double lrad, int sn, const double *sxs, const double *sydys, double *fs, hipStream_t st);
 
 struct RelaxLaunch {
     GridGeom g;
@@ -99,6 +139,9 @@ struct RelaxLaunch {
     ppafm_scan scan;
     const double *tips;  // optional explicit tip positions (nx,ny,nz,3)
     ppafm_relax_out out;
+    // mode-0 radial tip force from a spline (n knots, device xs and ydys)
+    int spline_n = 0;
+    const double *spline_xs = nullptr, *spline_ydys = nullptr;
 };
 int launch_relax_dev(const RelaxLaunch &L, const void *grid, bool f64, hipStream_t st);
 int launch_relax_fast(const RelaxLaunch &L, const void *grid, bool f64, hipStream_t st);
@@ -109,6 +152,9 @@ int launch_relax_f64grid(const RelaxLaunch &L, const double *grid, double *pos, 
 // 0 float4, 1 double4, 2 double (npts,3) (the host ABI's grid).
 int launch_stiffness(const RelaxLaunch &L, const void *grid, int grid_kind, const void *pos, int pos_f32,
                      double ddisp, int which, double *evals, double *ev1, double *ev2, double *ev3, hipStream_t st);
+// test_eigen3x3: eigenvalues (reference order) and eigenvectors of one
+// device 3x3 matrix -> out[12].
+int launch_eigen3x3(const double *m, double *out, hipStream_t st);
 
 // Batched FIRE relaxation (persistent kernel, see relax_dev.hip).
 struct RelaxBatchLaunch {

@@ -24,6 +24,7 @@
 #include <cstdlib>
 
 #include "pp_common.hpp"
+#include "tip_spline.hpp"
 
 // No FMA contraction in this file: the FP64 path then rounds every operation
 // like the CPU reference (x86-64 baseline code has no FMA either).
@@ -48,6 +49,10 @@ struct RelaxArgs {
     void *force;
     int *iters;
     unsigned long long *counters;
+    // radial tip force from a spline (forceRSpline) instead of the harmonic
+    // spring when spline_n > 0 (host ABI only)
+    int spline_n;
+    const double *spline_xs, *spline_ydys;
 };
 
 struct d3 {
@@ -245,9 +250,10 @@ __global__ __launch_bounds__(64) void relax_kernel(RelaxArgs A, Grid G)
             d3 f{fe[0], fe[1], fe[2]};
             const d3 dp{r.x - rtip.x, r.y - rtip.y, r.z - rtip.z};
             if (m0) {
-                // getPPforce: radial spring then lateral spring
+                // getPPforce: radial spring (or spline) then lateral spring
                 const double l = sqrt(dot3(dp, dp));
-                const double sr = A.krad * (l - A.lrad) / l;
+                const double sr = A.spline_n ? tip_spline_force_scale(l, A.spline_n, A.spline_xs, A.spline_ydys)
+                                             : A.krad * (l - A.lrad) / l;
                 f.x += dp.x * sr; f.y += dp.y * sr; f.z += dp.z * sr;
                 const d3 dl{dp.x - A.d0[0], dp.y - A.d0[1], dp.z - A.d0[2]};
                 f.x += dl.x * A.ks[0]; f.y += dl.y * A.ks[1]; f.z += dl.z * A.ks[2];
@@ -362,7 +368,8 @@ __global__ __launch_bounds__(64) void relax_kernel(RelaxArgs A, Grid G)
 // displaced positions round as there.  One lane per position; the six force
 // evaluations are six independent 8-corner gathers.
 
-// getPPforce with the harmonic radial spring, in the reference's order.
+// getPPforce with the harmonic radial spring or the tip spline, in the
+// reference's order.
 template <class Grid>
 __device__ __forceinline__ d3 pp_force_m0(const RelaxArgs &A, const Grid &G, d3 rtip, d3 r)
 {
@@ -373,7 +380,8 @@ __device__ __forceinline__ d3 pp_force_m0(const RelaxArgs &A, const Grid &G, d3 
     d3 f{fe[0], fe[1], fe[2]};
     const d3 dp{r.x - rtip.x, r.y - rtip.y, r.z - rtip.z};
     const double l = sqrt(dot3(dp, dp));
-    const double sr = A.krad * (l - A.lrad) / l;
+    const double sr = A.spline_n ? tip_spline_force_scale(l, A.spline_n, A.spline_xs, A.spline_ydys)
+                                 : A.krad * (l - A.lrad) / l;
     f.x += dp.x * sr; f.y += dp.y * sr; f.z += dp.z * sr;
     const d3 dl{dp.x - A.d0[0], dp.y - A.d0[1], dp.z - A.d0[2]};
     f.x += dl.x * A.ks[0]; f.y += dl.y * A.ks[1]; f.z += dl.z * A.ks[2];
@@ -414,26 +422,26 @@ __device__ __forceinline__ void sym3_eigenvalues(double xx, double yy, double zz
     ev[0] = lo; ev[1] = mid; ev[2] = hi;
 }
 
-// Unit eigenvector for eigenvalue lam: the longest of the three pairwise
-// cross products of the rows of (A - lam I), which are orthogonal to its null
-// space (the selection and sign rule of Mat3::eigenvec, Mat3.h:484-500, so
-// the vectors come out with the reference's signs).  A fully degenerate
-// matrix (all three products zero) gives the unit axis `axis`.
-__device__ __forceinline__ d3 sym3_eigenvector(double xx, double yy, double zz, double xy, double xz, double yz,
-                                               double lam, int axis)
+// Unit eigenvector for eigenvalue lam of the matrix with rows a, b, c: the
+// longest of the three pairwise cross products of the rows of (A - lam I),
+// which are orthogonal to its null space (the selection and sign rule of
+// Mat3::eigenvec, Mat3.h:484-500, so the vectors come out with the
+// reference's signs).  A fully degenerate matrix (all three products zero)
+// gives the unit axis `axis`.
+__device__ __forceinline__ d3 eigenvector3(d3 a, d3 b, d3 c, double lam, int axis)
 {
-    const d3 r0{xx - lam, xy, xz}, r1{xy, yy - lam, yz}, r2{xz, yz, zz - lam};
+    const d3 r0{a.x - lam, a.y, a.z}, r1{b.x, b.y - lam, b.z}, r2{c.x, c.y, c.z - lam};
     const d3 c01{r0.y * r1.z - r0.z * r1.y, r0.z * r1.x - r0.x * r1.z, r0.x * r1.y - r0.y * r1.x};
     const d3 c02{r0.y * r2.z - r0.z * r2.y, r0.z * r2.x - r0.x * r2.z, r0.x * r2.y - r0.y * r2.x};
     const d3 c12{r1.y * r2.z - r1.z * r2.y, r1.z * r2.x - r1.x * r2.z, r1.x * r2.y - r1.y * r2.x};
     const double d0 = dot3(c01, c01), d1 = dot3(c02, c02), d2 = dot3(c12, c12);
-    d3 c = c01;
+    d3 v = c01;
     double dm = d0;
-    if (d1 > dm) { dm = d1; c = c02; }
-    if (d2 > dm) { dm = d2; c = c12; }
+    if (d1 > dm) { dm = d1; v = c02; }
+    if (d2 > dm) { dm = d2; v = c12; }
     if (dm == 0.0) return d3{axis == 0 ? 1.0 : 0.0, axis == 1 ? 1.0 : 0.0, axis == 2 ? 1.0 : 0.0};
     const double s = 1.0 / sqrt(dm);
-    return d3{c.x * s, c.y * s, c.z * s};
+    return d3{v.x * s, v.y * s, v.z * s};
 }
 
 struct StiffOut {
@@ -493,10 +501,35 @@ __global__ __launch_bounds__(256) void stiffness_kernel(RelaxArgs A, Grid G, Sti
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         if (O.which > k) {
-            const d3 v = sym3_eigenvector(a.x, b.y, c.z, xy, xz, yz, ev[k], k);
+            const d3 v = eigenvector3(d3{a.x, xy, xz}, d3{xy, b.y, yz}, d3{xz, yz, c.z}, ev[k], k);
             double *w = outs[k] + 3 * o;
             w[0] = v.x; w[1] = v.y; w[2] = v.z;
         }
+    }
+}
+
+// test_eigen3x3 (ProbeParticle.cpp:805-815): the eigenvalues of the
+// symmetric matrix on and above the diagonal of m (row-major) in the order
+// of Mat3::eigenvals (Mat3.h:459-482) -- largest, smallest, middle when the
+// largest element of m is positive, the reverse of the first two when it is
+// negative, NaN when it is zero -- then the eigenvector of each from the
+// full rows of m.  out: 3 eigenvalues, then 3 vectors.
+__global__ __launch_bounds__(64) void eigen3x3_kernel(const double *__restrict__ m, double *__restrict__ out)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double amax = m[0];
+    for (int k = 1; k < 9; k++) amax = m[k] > amax ? m[k] : amax;
+    double ev[3];
+    sym3_eigenvalues(m[0], m[4], m[8], m[1], m[2], m[5], ev);
+    double o[3];
+    if (amax > 0) { o[0] = ev[2]; o[1] = ev[0]; o[2] = ev[1]; }
+    else if (amax < 0) { o[0] = ev[0]; o[1] = ev[2]; o[2] = ev[1]; }
+    else { o[0] = o[1] = o[2] = __builtin_nan(""); }
+    const d3 a{m[0], m[1], m[2]}, b{m[3], m[4], m[5]}, c{m[6], m[7], m[8]};
+    for (int k = 0; k < 3; k++) {
+        out[k] = o[k];
+        const d3 v = eigenvector3(a, b, c, o[k], k);
+        out[3 + 3 * k] = v.x; out[4 + 3 * k] = v.y; out[5 + 3 * k] = v.z;
     }
 }
 
@@ -519,6 +552,7 @@ RelaxArgs make_args(const RelaxLaunch &L)
     A.tips = L.tips;
     A.pos = L.out.pos; A.pos_f32 = L.out.pos_f32; A.force = L.out.force; A.iters = L.out.iters;
     A.counters = L.out.counters;
+    A.spline_n = L.spline_n; A.spline_xs = L.spline_xs; A.spline_ydys = L.spline_ydys;
     return A;
 }
 
@@ -532,6 +566,9 @@ int check(const RelaxLaunch &L)
     PP_REQUIRE(L.rp.max_iters >= 0 && L.rp.max_iters <= 1000000, "max_iters out of range");
     PP_REQUIRE(L.rp.mode == 0 || L.rp.max_iters >= 1, "mode 1 needs max_iters >= 1");
     PP_REQUIRE(L.rp.probe_start >= -1 && L.rp.probe_start <= 1, "probeStart must be -1, 0 or 1");
+    PP_REQUIRE(L.spline_n == 0 || (L.spline_n >= 2 && L.spline_xs && L.spline_ydys),
+               "a tip spline needs at least 2 knots");
+    PP_REQUIRE(L.spline_n == 0 || L.rp.mode == 0, "tip splines apply to mode 0");
     return 0;
 }
 
@@ -541,6 +578,7 @@ int launch_relax_dev(const RelaxLaunch &L, const void *grid, bool f64, hipStream
 {
     int e = check(L);
     if (e) return e;
+    PP_REQUIRE(L.spline_n == 0, "tip splines are supported on the host ABI's FP64 grid only");
     static const bool generic = getenv("PPAFM_RELAX_GENERIC") != nullptr;
     if (!generic) {
         e = launch_relax_fast(L, grid, f64, st);
@@ -583,6 +621,8 @@ int launch_stiffness(const RelaxLaunch &L, const void *grid, int grid_kind, cons
     PP_REQUIRE(L.g.n[0] > 0 && L.g.n[1] > 0 && L.g.n[2] > 0, "grid dimensions must be positive");
     PP_REQUIRE(grid_kind >= 0 && grid_kind <= 2, "grid_kind must be 0 (float4), 1 (double4) or 2 (double3)");
     PP_REQUIRE(std::isfinite(ddisp) && ddisp != 0.0, "ddisp must be finite and non-zero");
+    PP_REQUIRE(L.spline_n == 0 || (L.spline_n >= 2 && L.spline_xs && L.spline_ydys),
+               "a tip spline needs at least 2 knots");
     const long np = (long)L.scan.nx * L.scan.ny * L.scan.nz;
     if (np == 0) return 0;
     PP_REQUIRE(grid && pos && evals, "null grid, position or eigenvalue pointer");
@@ -602,6 +642,16 @@ int launch_stiffness(const RelaxLaunch &L, const void *grid, int grid_kind, cons
     else
         hipLaunchKernelGGL((stiffness_kernel<GridF64>), dim3((unsigned)nblk), dim3(256), 0, st, A,
                            GridF64{(const double *)grid}, O);
+    return 0;
+}
+
+}  // namespace pp
+
+namespace pp {
+
+int launch_eigen3x3(const double *m, double *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(eigen3x3_kernel, dim3(1), dim3(64), 0, st, m, out);
     return 0;
 }
 

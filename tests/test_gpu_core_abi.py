@@ -104,7 +104,8 @@ def test_relax_single_stroke_returns_itrsum():
     assert np.abs(rs - rr).max() < 1e-9
 
 
-def test_spline_tip_rejected():
+def test_malformed_tip_spline_rejected():
+    """A spline with fewer than 2 knots (here an empty SplineParams) is an error."""
     import ctypes
 
     from ppafm_amd import _lib

@@ -36,6 +36,18 @@ def test_library_exports_every_declared_symbol():
     assert set(names) <= bound, set(names) - bound
 
 
+def test_library_exports_every_symbol_ppafm_core_binds():
+    """ppafm/core.py sets argtypes on each of these when it is imported, so a
+    library missing one cannot stand in for ProbeParticle_lib.so
+    (tests/golden/core_symbols.txt lists them)."""
+    L = ctypes.CDLL(str(_lib.LIB_PATH))
+    path = os.path.join(ROOT, "tests", "golden", "core_symbols.txt")
+    names = [ln.strip() for ln in open(path) if ln.strip() and not ln.startswith("#")]
+    assert len(names) >= 28
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
 def test_df_weights_reference_golden_vectors():
     # ppafm tests/test_common.py:10-24
     assert np.allclose(C.get_df_weight(0.1), [-5.0, 5.0])

@@ -670,3 +670,158 @@ def test_stiffness_affine_field_closed_form():
         for k in range(3):
             assert np.abs(np.abs(vecs[k] @ V[:, k]) - 1).max() < 1e-9
             assert np.abs(np.linalg.norm(vecs[k], axis=1) - 1).max() < 1e-12
+
+
+# ------------------------------------------------- further force-field models
+
+
+def _models():
+    return np.load(os.path.join(GOLDEN, "models.npz")), np.load(os.path.join(GOLDEN, "grid.npz"))
+
+
+GRID_MODELS = [("vdw", "vdw", "cLJs", 180.0), ("vdwre1", "vdw_r2", "REs", 0.5), ("vdwre2", "vdw_r4", "REs", 0.5),
+               ("vdwre3", "vdw_invr4", "REs", 0.03), ("vdwre4", "vdw_invr8", "REs", 0.01), ("morse", "morse", "REs", -1.6),
+               ("d3", "d3", "d3_coeffs", 0.0), ("gauss", "gauss", "cRAs", 0.0), ("slater", "slater", "cRAs", 0.0),
+               ("r4", "r4", "cRAs", 0.0),
+               # after evalRadialFF replaced the damping constants (they persist)
+               ("vdwre1_after", "vdw_r2", "REs", 0.7), ("vdwre2_after", "vdw_r4", "REs", 0.9),
+               ("vdwre3_after", "vdw_invr4", "REs", 0.05), ("vdwre4_after", "vdw_invr8", "REs", 0.02),
+               ("vdw_after", "vdw", "cLJs", 90.0)]
+
+
+@pytest.mark.parametrize("tag,name,coef,par", GRID_MODELS)
+def test_model_grids_golden_bit_exact(tag, name, coef, par):
+    m, g = _models()
+    coefs = g["cLJs"] if coef == "cLJs" else m[coef]
+    F = np.zeros(tuple(g["n"]) + (3,))
+    E = np.zeros(tuple(g["n"]))
+    O.model_ff(name, g["n"], g["cell"], g["Rs"], coefs, par, F=F, E=E)
+    assert np.array_equal(F, m["F_" + tag]) and np.array_equal(E, m["E_" + tag])
+
+
+def test_radial_golden_bit_exact():
+    m, _ = _models()
+    cf = {-3: (3.4, 0.004), -2: (23.6, 2.4e4), -1: (23.6,), 0: (23.6,), 1: (3.4, 0.004), 2: (3.4, 0.004), 3: (3.4, 0.004), 4: (3.4, 0.004)}
+    for kind in range(-3, 5):
+        E, F = O.radial(kind, m["radial_rs"], cf[kind])
+        assert np.array_equal(E, m[f"radial_E{kind}"]) and np.array_equal(F, m[f"radial_F{kind}"]), kind
+    for kind, ad in ((0, 90.0), (1, 0.7), (2, 0.9), (3, 0.05), (4, 0.02)):
+        E, F = O.radial(kind, m["radial_rs"], cf[kind], adamp=ad)
+        assert np.array_equal(E, m[f"radialc_E{kind}"]) and np.array_equal(F, m[f"radialc_F{kind}"]), kind
+
+
+def test_d3_coeffs_golden():
+    import sys
+
+    sys.path.insert(0, GOLDEN)
+    import make_golden as MG
+
+    m, _ = _models()
+    c = O.d3_coeffs(m["d3_Rs"], m["d3_iZs"], MG.d3_tables(), m["d3_params"], 8)
+    assert np.abs(c - m["d3_coeffs"]).max() <= 1e-14 * np.abs(m["d3_coeffs"]).max()
+
+
+def test_splines_golden_and_cubic_reproduction():
+    m, _ = _models()
+    assert np.array_equal(O.spline_uniform(1.0, 0.2, m["spline_ydys_u"], m["spline_xs_u"]), m["spline_u"])
+    assert np.array_equal(O.spline_nonuniform(m["spline_kx"], m["spline_ydys"], m["spline_xs_n"]), m["spline_n"])
+    # a cubic Hermite interpolant reproduces any cubic from its values and slopes
+    p = np.poly1d([0.3, -1.1, 0.7, 2.0])
+    kx = np.array([0.2, 0.5, 1.1, 1.4, 2.3, 3.0])
+    ydys = np.column_stack([p(kx), p.deriv()(kx)])
+    xs = np.linspace(0.25, 2.95, 37)
+    assert np.abs(O.spline_nonuniform(kx, ydys, xs) - p(xs)).max() < 1e-12
+    ku = 0.4 + 0.3 * np.arange(9)
+    assert np.abs(O.spline_uniform(0.4, 0.3, np.column_stack([p(ku), p.deriv()(ku)]), xs + 0.3) - p(xs + 0.3)).max() < 1e-12
+
+
+def test_spline_tip_relax_and_stiffness_golden():
+    m, _ = _models()
+    r = np.load(os.path.join(GOLDEN, "relax.npz"))
+    sp = (m["tipspline_xs"], m["tipspline_ydys"])
+    rs, fs = O.relax_strokes_spline(r["n"], r["cell"], r["FF"], _golden_tip(r), r["tips"], sp)
+    assert np.array_equal(rs, m["spline_rs"]) and np.array_equal(fs, m["spline_fs"])
+    ev, vecs, nnan = O.stiffness(r["n"], r["cell"], r["FF"], _golden_tip(r), r["tips"].reshape(-1, 3), rs.reshape(-1, 3),
+                                 which=3, spline=sp)
+    assert np.array_equal(ev, m["spline_evals"])
+    for k in range(3):
+        assert np.array_equal(vecs[k], m[f"spline_evec{k + 1}"])
+
+
+def test_linear_spline_tip_equals_harmonic_spring():
+    """A spline of k (x - L) (values and slopes) is the harmonic radial spring:
+    the relaxation agrees with the spring's to rounding."""
+    r = np.load(os.path.join(GOLDEN, "relax.npz"))
+    tip = _golden_tip(r)
+    xs = np.linspace(0.3, 3.0, 9)
+    kR, L = float(r["tip_kRadial"]), float(r["tip_lRadial"])
+    sp = (xs, np.column_stack([kR * (xs - L), np.full_like(xs, kR)]))
+    rs, fs = O.relax_strokes_spline(r["n"], r["cell"], r["FF"], tip, r["tips"], sp)
+    assert np.abs(rs - r["rs_ps1"]).max() < 1e-6
+
+
+@pytest.mark.parametrize("name,coef", [("morse", (3.1, 0.006)), ("vdw_r2", (3.1, 0.006)), ("vdw_r4", (3.1, 0.006)),
+                                       ("vdw_invr4", (3.1, 0.006)), ("vdw_invr8", (3.1, 0.006)), ("d3", (20.0, 300.0, 900.0, 2.0e4)),
+                                       ("lj_re", (3.1, 0.006)), ("inv_r6", (12.0,))])
+def test_radial_force_is_minus_energy_gradient(name, coef):
+    kind = {v: k for k, v in O.RADIAL_KIND.items()}.get(name)
+    rs = np.linspace(1.2, 7.0, 300)
+    h = 1e-5
+    if kind is None:  # d3 / morse: through the grid builder on a line of points
+        par = -1.6 if name == "morse" else 0.0
+        E = lambda x: _line_energy(name, x, coef, par)  # noqa: E731
+        F = _line_force(name, rs, coef, par)
+    else:
+        E = lambda x: O.radial(kind, x, coef)[0]  # noqa: E731
+        F = O.radial(kind, rs, coef)[1]
+    g = -(E(rs + h) - E(rs - h)) / (2 * h)
+    assert np.abs(g - F).max() <= 1e-6 * max(np.abs(F).max(), 1e-8)
+
+
+def _line(name, xs, coef, par):
+    # one atom at the origin of a 1-point-wide grid whose points lie at x
+    out = []
+    for x in np.atleast_1d(xs):
+        F = np.zeros((1, 1, 1, 3))
+        E = np.zeros((1, 1, 1))
+        O.model_ff(name, (1, 1, 1), np.eye(3) * 50, np.array([[-x, 0.0, 0.0]]), np.array([coef]), par, F=F, E=E)
+        out.append((E[0, 0, 0], F[0, 0, 0, 0]))
+    return np.array(out)
+
+
+def _line_energy(name, xs, coef, par):
+    return _line(name, xs, coef, par)[:, 0]
+
+
+def _line_force(name, xs, coef, par):
+    return _line(name, xs, coef, par)[:, 1]
+
+
+def test_morse_minimum_and_lj_re_identity():
+    # Morse: minimum -E0 at r = R0 (the 1e-8 softening shifts it by ~1e-9)
+    r = np.array([3.1])
+    e = _line_energy("morse", r, (3.1, 0.006), -1.6)
+    assert abs(e[0] + 0.006) < 1e-12
+    # LJ in (R, E) equals LJ with c6 = 2 E R^6, c12 = E R^12
+    rs = np.linspace(2.0, 6.0, 50)
+    R, E0 = 3.1, 0.006
+    e1, f1 = O.radial(-3, rs, (R, E0))
+    e2, f2 = O.radial(-2, rs, (2 * E0 * R ** 6, E0 * R ** 12))
+    assert np.abs(e1 - e2).max() < 1e-13 and np.abs(f1 - f2).max() < 1e-13
+
+
+@pytest.mark.skipif(not O.have_reference(), reason="oracle/_ref not built (reference sources absent)")
+def test_model_grids_live_reference_random_case():
+    rng = np.random.default_rng(5)
+    n = np.array([7, 8, 9], dtype=np.int32)
+    cell = np.array([[1.0, 0.1, 0.0], [0.0, 1.1, 0.1], [0.05, 0.0, 1.2]])
+    Rs = rng.uniform(-0.5, 1.5, (6, 3))
+    REs = np.column_stack([rng.uniform(2.5, 3.5, 6), rng.uniform(0.01, 0.1, 6)])
+    ref = O.Reference()
+    for name, ref_name, extra, par in (("vdw_r4", "getVdWFF_RE", (2, 0.8), 0.8), ("morse", "getMorseFF", (-1.3,), -1.3)):
+        F, E = np.zeros(tuple(n) + (3,)), np.zeros(tuple(n))
+        ref.set_grid(n, cell, F, E)
+        ref.grid_model(ref_name, Rs, REs, *extra)
+        F2, E2 = np.zeros_like(F), np.zeros_like(E)
+        O.model_ff(name, n, cell, Rs, REs, par, F=F2, E=E2)
+        assert np.array_equal(F, F2) and np.array_equal(E, E2), name
