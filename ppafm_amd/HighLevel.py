@@ -118,9 +118,10 @@ def _parse_atoms(atoms, elem_dict_unused, parameters, lvec):
 
 
 def computeLJ(geomFile, speciesFile=None, geometry_format=None, save_format=None, computeVpot=False, Fmax=Fmax_DEFAULT, Vmax=Vmax_DEFAULT, ffModel="LJ", parameters=None):
-    """LJ force field of a geometry file (HighLevel.py:260-307), LJ model only."""
-    if ffModel != "LJ":
-        raise NotImplementedError(f"ffModel {ffModel!r}: only 'LJ' is on this path")
+    """LJ, Morse or vdW force field of a geometry file (HighLevel.py:260-307):
+    ffModel "Morse" uses getMorseFF with parameters.aMorse; "vdW" uses
+    getVdWFF for parameters.vdWDampKind 0 and getVdWFF_RE with that kind
+    otherwise; anything else is LJ."""
     if save_format is not None:
         raise NotImplementedError("saving grids to files is not on this path")
     FFparams = PPU.loadSpecies(speciesFile)
@@ -129,8 +130,15 @@ def computeLJ(geomFile, speciesFile=None, geometry_format=None, save_format=None
     iPP = FFparams.by_symbol.get(parameters.probeType) if not str(parameters.probeType).isdigit() else int(parameters.probeType)
     FF, V = setFF(None, computeVpot, lvec=lvec, parameters=parameters)
     Rs0 = shift_positions(Rs, -lvec[0])
-    cLJs = PPU.getAtomsLJ(iPP, iZs, FFparams)
-    core.getLennardJonesFF(Rs0, cLJs)
+    if ffModel == "Morse":
+        core.getMorseFF(Rs0, PPU.getAtomsRE(iPP, iZs, FFparams), parameters=parameters)
+    elif ffModel == "vdW":
+        if parameters.vdWDampKind == 0:
+            core.getVdWFF(Rs0, PPU.getAtomsLJ(iPP, iZs, FFparams))
+        else:
+            core.getVdWFF_RE(Rs0, PPU.getAtomsRE(iPP, iZs, FFparams), kind=parameters.vdWDampKind)
+    else:
+        core.getLennardJonesFF(Rs0, PPU.getAtomsLJ(iPP, iZs, FFparams))
     if Fmax is not None:
         io.limit_vec_field(FF, Fmax=Fmax)
     if (Vmax is not None) and computeVpot:

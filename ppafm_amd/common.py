@@ -87,6 +87,19 @@ def getAtomsLJ(iZprobe, iZs, FFparams):
     return cLJs
 
 
+def getAtomsRE(iZprobe, iZs, FFparams):
+    """(R_pp + R_i, sqrt(eps_pp eps_i)) per atom, the (R, E0) coefficients of
+    the Morse and damped-vdW grids (common.py:899-905)."""
+    n = len(iZs)
+    REs = np.zeros((n, 2))
+    rpp, epp = FFparams[iZprobe - 1][0], FFparams[iZprobe - 1][1]
+    for i in range(n):
+        REs[i, 0] = rpp + FFparams[iZs[i] - 1][0]
+        REs[i, 1] = epp * FFparams[iZs[i] - 1][1]
+    REs[:, 1] = np.sqrt(REs[:, 1])
+    return REs
+
+
 def REA2LJ(cREAs, cLJs=None):
     if cLJs is None:
         cLJs = np.zeros((len(cREAs), 2))

@@ -97,6 +97,9 @@ def test_points_and_radial_then_persistent_damping():
     F, E = _grid(g)
     core.getVdWFF(Rs, np.ascontiguousarray(g["cLJs"]))
     _check("vdw_after", F, E, m)
+    # back to the reference's defaults for whatever runs next in this process
+    for kind, ad in ((0, 180.0), (1, 0.5), (2, 0.5), (3, 0.03), (4, 0.01)):
+        core.evalRadialFF(rs[:1].copy(), cf[kind], kind=kind, ADamp=ad)
 
 
 def test_many_atoms_span_several_tiles():
