@@ -439,6 +439,26 @@ const char *ppafm_last_error(void);
 /* Clear the calling thread's last error (the reference-compatible void
  * entry points report failures only through ppafm_last_error and stderr). */
 void ppafm_clear_error(void);
+/* ------------------------------------------------------------------------
+ * (3) Grid files (host code, multithreaded; nthreads <= 0 = all cores)
+ * ---------------------------------------------------------------------- */
+
+/* The data block of an XSF grid as ppafm/io.py:575-599 (saveXSFData) writes
+ * it, appended to (append != 0) or replacing fname: the (nx,ny,nz) grid
+ * with one periodic plane repeated along each axis, x fastest, one "%10.5e"
+ * value per line, then "   END_DATAGRID_3D" and "END_BLOCK_DATAGRID_3D".
+ * Element (ix,iy,iz) is data[ix*sx + iy*sy + iz*sz] (host float64). */
+int ppafm_xsf_write_data(const char *fname, int append, const double *data, int nx, int ny, int nz, long long sx,
+                         long long sy, long long sz, int nthreads);
+/* The same from channel `channel` (0..3) of a device grid (nx,ny,nz) of
+ * float4 (f64 = 0) or double4 (f64 = 1) values, copied to the host first. */
+int ppafm_xsf_write_data_dev(const char *fname, int append, const void *grid_dev, int f64, int channel, int nx, int ny,
+                             int nz, int nthreads);
+/* GridUtils.cpp:32 ReadNumsUpTo_C: skip noline lines (as fgets with a
+ * 5000-byte buffer skips them), then read count whitespace-separated
+ * numbers into out; fewer numbers or a non-number field is an error. */
+int ppafm_read_numbers(const char *fname, int noline, long long count, double *out, int nthreads);
+
 /* Sum of iteration indices of the last relaxTipStrokes_omp call. */
 long long ppafm_last_itrsum(void);
 

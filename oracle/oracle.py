@@ -674,5 +674,52 @@ def have_reference():
     return REF_PATH.exists()
 
 
+REF_GU_PATH = HERE / "_ref" / "GridUtils_lib.so"
+
+
+def have_reference_gridutils():
+    return REF_GU_PATH.exists()
+
+
+def reference_read_numbers(fname, dims, noline):
+    """The reference's ReadNumsUpTo_C (GridUtils.cpp:32-71) from oracle/_ref:
+    prod(dims) numbers after noline lines.  It prints progress to stdout."""
+    L = ctypes.CDLL(str(REF_GU_PATH))
+    L.ReadNumsUpTo_C.argtypes = [ctypes.c_char_p, _dp, _ip, ctypes.c_int]
+    L.ReadNumsUpTo_C.restype = ctypes.c_int
+    dims = np.ascontiguousarray(dims, dtype=np.int32)
+    out = np.zeros(int(np.prod(dims.astype(np.int64))))
+    L.ReadNumsUpTo_C(str(fname).encode(), _ptr(out), _ptr(dims, _ip), int(noline))
+    return out
+
+
 if __name__ == "__main__":  # pragma: no cover
     print("oracle threads:", num_threads(), "ref:", have_reference(), os.getpid())
+
+
+# ------------------------------------------------------------- XSF grid text
+
+
+def xsf_data_text(data_xyz):
+    """The data block saveXSFData writes (io.py:575-599), restated with
+    Python's own %-formatting: the (nx,ny,nz) grid extended by its first
+    plane along each axis, x fastest, one "%10.5e" value per line, and the
+    two END lines."""
+    a = np.asarray(data_xyz, dtype=np.float64)
+    ext = np.concatenate([a, a[:1]], axis=0)
+    ext = np.concatenate([ext, ext[:, :1]], axis=1)
+    ext = np.concatenate([ext, ext[:, :, :1]], axis=2)
+    flat = ext.transpose(2, 1, 0).ravel()
+    return "".join("%10.5e\n" % v for v in flat) + "   END_DATAGRID_3D\nEND_BLOCK_DATAGRID_3D\n"
+
+
+def read_numbers(text, noline, count):
+    """ReadNumsUpTo_C (GridUtils.cpp:32-71) on a string: skip noline lines
+    (fgets with a 5000-byte buffer: at most 4999 characters each), then the
+    next count whitespace-separated fields as floats."""
+    pos = 0
+    for _ in range(noline):
+        nl = text.find("\n", pos, pos + 4999)
+        pos = nl + 1 if nl >= 0 else min(len(text), pos + 4999)
+    fields = text[pos:].split()
+    return np.array([float(v) for v in fields[:count]])

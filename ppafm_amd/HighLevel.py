@@ -121,9 +121,8 @@ def computeLJ(geomFile, speciesFile=None, geometry_format=None, save_format=None
     """LJ, Morse or vdW force field of a geometry file (HighLevel.py:260-307):
     ffModel "Morse" uses getMorseFF with parameters.aMorse; "vdW" uses
     getVdWFF for parameters.vdWDampKind 0 and getVdWFF_RE with that kind
-    otherwise; anything else is LJ."""
-    if save_format is not None:
-        raise NotImplementedError("saving grids to files is not on this path")
+    otherwise; anything else is LJ.  With save_format ("xsf" or "npy") the
+    grids are written as FF<model> (and E<model>) like the reference."""
     FFparams = PPU.loadSpecies(speciesFile)
     atoms, nDim, lvec = io.loadGeometry(geomFile, format=geometry_format, parameters=parameters)
     iZs, Rs, Qs = _parse_atoms(atoms, None, parameters, lvec)
@@ -143,13 +142,23 @@ def computeLJ(geomFile, speciesFile=None, geometry_format=None, save_format=None
         io.limit_vec_field(FF, Fmax=Fmax)
     if (Vmax is not None) and computeVpot:
         V[V > Vmax] = Vmax
+    if save_format is not None:
+        _save_fields("FF" + ffModel, "E" + ffModel, FF, V if computeVpot else None, atoms, lvec, save_format)
     return FF, V, nDim, lvec
 
 
+def _save_fields(fname, ename, FF, V, atoms, lvec, save_format):
+    # HighLevel.py:301-305, 381-385: the header carries the cell and atoms
+    head = io.primcoords2Xsf(np.asarray(atoms[0], dtype=np.int32), [atoms[1], atoms[2], atoms[3]], lvec)
+    info = (atoms[:4], lvec)
+    io.save_vec_field(fname, FF, lvec, data_format=save_format, head=head, atomic_info=info)
+    if V is not None:
+        io.save_scal_field(ename, V, lvec, data_format=save_format, head=head, atomic_info=info)
+
+
 def computeELFF_pointCharge(geomFile, geometry_format=None, tip="s", save_format=None, computeVpot=False, Fmax=Fmax_DEFAULT, Vmax=Vmax_DEFAULT, parameters=None):
-    """Unit-probe-charge Coulomb field of a geometry file (HighLevel.py:353-387)."""
-    if save_format is not None:
-        raise NotImplementedError("saving grids to files is not on this path")
+    """Unit-probe-charge Coulomb field of a geometry file (HighLevel.py:353-387);
+    with save_format the grids are written as FFel (and Vel)."""
     tipKind = {"s": 0, "pz": 1, "dz2": 2}[tip]
     atoms, nDim, lvec = io.loadGeometry(geomFile, format=geometry_format, parameters=parameters)
     _, Rs, Qs = _parse_atoms(atoms, None, parameters, lvec)
@@ -160,6 +169,8 @@ def computeELFF_pointCharge(geomFile, geometry_format=None, tip="s", save_format
         io.limit_vec_field(FF, Fmax=Fmax)
     if (Vmax is not None) and computeVpot:
         V[V > Vmax] = Vmax
+    if save_format is not None:
+        _save_fields("FFel", "Vel", FF, V if computeVpot else None, atoms, lvec, save_format)
     return FF, V, nDim, lvec
 
 

@@ -176,6 +176,9 @@ SIGNATURES = [
     ("ppafm_event_destroy", c_int, [c_void_p]),
     ("ppafm_stream_create", c_int, [ctypes.POINTER(c_void_p)]),
     ("ppafm_stream_destroy", c_int, [c_void_p]),
+    ("ppafm_xsf_write_data", c_int, [ctypes.c_char_p, c_int, c_void_p, c_int, c_int, c_int, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong, c_int]),
+    ("ppafm_xsf_write_data_dev", c_int, [ctypes.c_char_p, c_int, c_void_p, c_int, c_int, c_int, c_int, c_int, c_int]),
+    ("ppafm_read_numbers", c_int, [ctypes.c_char_p, c_int, ctypes.c_longlong, c_void_p, c_int]),
     ("ppafm_last_error", ctypes.c_char_p, []),
     ("ppafm_clear_error", None, []),
     ("ppafm_last_itrsum", ctypes.c_longlong, []),

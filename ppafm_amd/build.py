@@ -1,8 +1,9 @@
 """Build libppafm_hip.so for gfx950 in-tree (``python -m ppafm_amd.build``).
 
-Each .hip file is compiled with hipcc into an object under ppafm_amd/_build/,
-then linked into ppafm_amd/libppafm_hip.so.  Packed FP32 instructions are
-disabled (-packed-fp32-ops) for every device compile.
+Each .hip file is compiled with hipcc, each host-only .cpp file with g++,
+into an object under ppafm_amd/_build/; all are linked into
+ppafm_amd/libppafm_hip.so.  Packed FP32 instructions are disabled
+(-packed-fp32-ops) for every device compile.
 """
 
 import concurrent.futures
@@ -19,6 +20,7 @@ LIB = PKG / "libppafm_hip.so"
 ARCH = os.environ.get("PPAFM_HIP_ARCH", "gfx950")
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+CXX = shutil.which("g++") or "g++"
 
 COMMON_FLAGS = [
     f"--offload-arch={ARCH}",
@@ -34,17 +36,20 @@ COMMON_FLAGS = [
 
 
 def sources():
-    return sorted(CSRC.glob("*.hip"))
+    return sorted(CSRC.glob("*.hip")) + sorted(CSRC.glob("*.cpp"))
 
 
 def _compile(src, verbose):
     obj = BUILD / (src.stem + ".o")
-    cmd = [HIPCC, *COMMON_FLAGS, "-c", str(src), "-o", str(obj)]
+    if src.suffix == ".cpp":
+        cmd = [CXX, "-O3", "-std=c++17", "-fPIC", "-pthread", "-Wall", "-c", str(src), "-o", str(obj)]
+    else:
+        cmd = [HIPCC, *COMMON_FLAGS, "-c", str(src), "-o", str(obj)]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError(f"hipcc failed on {src.name}:\n{r.stdout}\n{r.stderr}")
+        raise RuntimeError(f"{cmd[0]} failed on {src.name}:\n{r.stdout}\n{r.stderr}")
     return obj
 
 
@@ -55,7 +60,7 @@ def build(verbose=False, jobs=None):
     with concurrent.futures.ThreadPoolExecutor(jobs) as ex:
         objs = list(ex.map(lambda s: _compile(s, verbose), srcs))
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", *map(str, objs), "-o", str(tmp)]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -730,6 +730,26 @@ int ppafm_stiffness(const ppafm_grid *grid, const void *grid_dev, const ppafm_ti
     return 0;
 }
 
+int ppafm_xsf_write_data_dev(const char *fname, int append, const void *grid_dev, int f64, int channel, int nx, int ny,
+                             int nz, int nthreads)
+{
+    PP_REQUIRE(grid_dev && fname, "null argument");
+    PP_REQUIRE(channel >= 0 && channel < 4, "channel must be 0..3");
+    PP_REQUIRE(nx > 0 && ny > 0 && nz > 0, "empty grid");
+    const size_t npts = (size_t)nx * ny * nz;
+    std::vector<double> h(npts);
+    if (f64) {
+        std::vector<double> g(npts * 4);
+        PP_TRY_HIP(hipMemcpy(g.data(), grid_dev, npts * 4 * sizeof(double), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < npts; i++) h[i] = g[4 * i + channel];
+    } else {
+        std::vector<float> g(npts * 4);
+        PP_TRY_HIP(hipMemcpy(g.data(), grid_dev, npts * 4 * sizeof(float), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < npts; i++) h[i] = g[4 * i + channel];
+    }
+    return ppafm_xsf_write_data(fname, append, h.data(), nx, ny, nz, (long long)ny * nz, nz, 1, nthreads);
+}
+
 int ppafm_convolve_z(long ncol, int nz, int in_stride, int in_step, int in_offset, const void *in_dev, int nw,
                      const double *w, double scale, void *out_dev, int f64, void *stream)
 {

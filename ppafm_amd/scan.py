@@ -122,6 +122,40 @@ class PlainScan:
         self.relax()
         self.to_df()
 
+    def save_grid(self, fname, data_format="xsf", head=None, nthreads=0):
+        """Write the device grid's force as a vector field the reference's
+        tools read (io.save_vec_field layout): fname_x/_y/_z.xsf formatted by
+        the library straight from the device copy, or fname.npz."""
+        from . import io
+
+        head = io.XSF_HEAD_DEFAULT if head is None else head
+        if data_format == "xsf":
+            from ._lib import check, lib
+
+            for k, c in enumerate("xyz"):
+                path = f"{fname}_{c}.xsf"
+                io.write_xsf_header(path, head, self.n, self.lvec)
+                check(lib().ppafm_xsf_write_data_dev(path.encode(), 1, self.grid.ptr, int(self.grid_f64), k, *self.n,
+                                                     int(nthreads)), "save_grid")
+        elif data_format == "npy":
+            io.save_vec_field(fname, np.ascontiguousarray(self.grid_host()[..., :3], dtype=np.float64), self.lvec,
+                              data_format="npy")
+        else:
+            raise ValueError(f"unknown data_format {data_format!r}")
+
+    def set_grid(self, FF, E=None):
+        """Use a force field from elsewhere -- (nx,ny,nz,3), e.g. from
+        io.load_vec_field -- as the device grid instead of building one
+        (run the relax() and to_df() stages after it, not run())."""
+        FF = np.asarray(FF)
+        if FF.shape != self.n + (3,):
+            raise ValueError(f"FF must have shape {self.n + (3,)}, got {FF.shape}")
+        g = np.zeros(self.n + (4,), dtype=np.float64 if self.grid_f64 else np.float32)
+        g[..., :3] = FF
+        if E is not None:
+            g[..., 3] = E
+        self.grid.upload(g)
+
     def stiffness(self, which=0, ddisp=0.05):
         """Eigenvalues (nx,ny,nz,3) of the probe's dynamical matrix at the
         relaxed positions and the first ``which`` eigenvector arrays

@@ -1,0 +1,164 @@
+"""Grid files (CPU, host code of the library): XSF writing and reading,
+npz and cube, and the vector/scalar field wrappers (ppafm/io.py:531-880,
+GridUtils.cpp:32-71).
+
+The XSF data block the library formats must be byte-identical to the one
+Python's %-formatting produces for the same values (the reference writes
+it that way, io.py:594-597), for awkward values too: exact decimal ties
+(round half to even), subnormals, three-digit exponents, signed zero, NaN
+and infinities, float32 input and strided views.  Reading must give the
+same doubles as Python's float() on each field after the header lines."""
+
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from ppafm_amd import io
+
+AWKWARD = np.array([0.0, -0.0, 1.0, -1.0, 1234565.0, 1234575.0, 0.1, 1e-310, -5e-324, 1.7976931348623157e308, 1e100,
+                    -1e-100, 9.999995e-5, 123456.5, np.nan, np.inf, -np.inf, 2.0 ** -20, 3.0e-7, 6.02214076e23])
+
+
+def _data(shape, seed=0, awkward=True):
+    rng = np.random.default_rng(seed)
+    a = rng.normal(size=shape) * 10.0 ** rng.integers(-8, 8, size=shape)
+    if awkward:
+        a.flat[: len(AWKWARD)] = AWKWARD
+    return a
+
+
+def _expected_file(data_xyz, lvec, head=io.XSF_HEAD_DEFAULT):
+    nx, ny, nz = data_xyz.shape
+    lines = "".join(head) + f"{nx + 1} {ny + 1} {nz + 1}\n" + "".join(" ".join(str(v) for v in row) + "\n" for row in lvec)
+    return lines + O.xsf_data_text(data_xyz)
+
+
+@pytest.mark.parametrize("shape,nthreads", [((3, 4, 5), 1), ((7, 6, 9), 0), ((40, 33, 29), 3), ((1, 1, 1), 0)])
+def test_xsf_bytes_equal_python_formatting(tmp_path, shape, nthreads):
+    a = _data(shape, seed=sum(shape))
+    lvec = np.array([[0.0, -1.5, 2.0], [4.0, 0.0, 0.0], [0.3, 3.6, 0.0], [0.0, 0.0, 5.0]])
+    fn = str(tmp_path / "g.xsf")
+    io.saveXSFData(fn, a, lvec, nthreads=nthreads)
+    assert open(fn).read() == _expected_file(a, lvec)
+
+
+def test_xsf_views_float32_and_zyx_order(tmp_path):
+    a = _data((6, 5, 7, 3), seed=4)
+    lvec = np.diag([1.0, 2.0, 3.0])
+    lvec = np.vstack([np.zeros(3), lvec])
+    fn = str(tmp_path / "v.xsf")
+    io.saveXSFData(fn, a[..., 1], lvec)  # strided view of one component
+    assert open(fn).read() == _expected_file(np.ascontiguousarray(a[..., 1]), lvec)
+    f32 = a[..., 2].astype(np.float32)
+    io.saveXSFData(fn, f32, lvec)
+    assert open(fn).read() == _expected_file(f32.astype(np.float64), lvec)
+    zyx = np.ascontiguousarray(a[..., 0].transpose(2, 1, 0))
+    io.saveXSFData(fn, zyx, lvec, data_is_xyz_order=False)
+    assert open(fn).read() == _expected_file(a[..., 0], lvec)
+    # default cell from the grid shape and steps (io.py:559-560)
+    io.saveXSFData(fn, a[..., 0], dd=[0.1, 0.2, 0.3])
+    assert open(fn).read() == _expected_file(a[..., 0], io._orthoLvec((7, 5, 6), [0.1, 0.2, 0.3]))
+
+
+def test_xsf_round_trip_and_reader_parity(tmp_path):
+    a = _data((9, 8, 11), seed=7, awkward=False)
+    lvec = np.array([[1.0, 2.0, 3.0], [4.5, 0.0, 0.0], [0.0, 4.0, 0.0], [0.0, 0.2, 5.5]])
+    fn = str(tmp_path / "r.xsf")
+    io.saveXSFData(fn, a, lvec, head=io.primcoords2Xsf([6, 8], [[0.1, 0.2], [1.0, 1.1], [2.0, 2.2]], lvec))
+    FF, lv, nDim, head = io.loadXSFData(fn)
+    assert FF.shape == a.shape and FF.flags.c_contiguous
+    assert tuple(nDim) == a.shape
+    assert np.array_equal(lv, lvec)
+    # values are what "%10.5e" kept
+    assert np.array_equal(FF, np.vectorize(lambda v: float("%10.5e" % v))(a))
+    Zs, Rs = io.getFromHead_PRIMCOORD(head)
+    assert list(Zs) == [6, 8] and np.allclose(Rs, [[0.1, 1.0, 2.0], [0.2, 1.1, 2.2]])
+    zyx = io.loadXSFData(fn, xyz_order=False)[0]
+    assert np.array_equal(zyx, FF.transpose(2, 1, 0))
+    # the native reader against Python's float() on the same fields
+    text = open(fn).read()
+    nhead = head.__len__()
+    n = int(np.prod(np.array(a.shape) + 1))
+    for nth in (1, 4, 0):
+        got = io.readNumsUpTo(fn, np.array(a.shape) + 1, nhead + 5, nthreads=nth)
+        assert np.array_equal(got, O.read_numbers(text, nhead + 5, n))
+
+
+def test_reader_formats_and_errors(tmp_path):
+    fn = str(tmp_path / "n.txt")
+    body = "header line\n" + "x" * 6000 + "\n" + " 1.5  -2e-3\t+3.25E+02\n\n4 nan inf -inf 1e-400 1e400\n5.0e+00END\n"
+    open(fn, "w").write(body)
+    # an overlong line counts as two fgets lines
+    got = io.readNumsUpTo(fn, (9, 1, 1), 3)
+    ref = O.read_numbers(body, 3, 9)
+    assert np.array_equal(got, ref, equal_nan=True)
+    with pytest.raises(Exception):
+        io.readNumsUpTo(fn, (12, 1, 1), 3)  # fewer fields than asked for
+    with pytest.raises(Exception):
+        io.readNumsUpTo(fn, (10, 1, 1), 3)  # "5.0e+00END" is not a number
+
+
+def test_vec_and_scal_fields_xsf_npy(tmp_path):
+    a = _data((5, 6, 7, 3), seed=9, awkward=False)
+    lvec = np.array([[0.0, 0.0, 0.0], [5.0, 0.0, 0.0], [0.0, 6.0, 0.0], [0.0, 0.0, 7.0]])
+    base = str(tmp_path / "FF")
+    io.save_vec_field(base, a, lvec)
+    for c in "xyz":
+        assert os.path.exists(f"{base}_{c}.xsf")
+    b, lv, nDim, head = io.load_vec_field(base)
+    assert b.shape == a.shape and np.array_equal(lv, lvec)
+    assert np.array_equal(b, np.vectorize(lambda v: float("%10.5e" % v))(a))
+    info = (np.arange(8.0).reshape(4, 2), lvec * 2)
+    io.save_vec_field(base, a, lvec, data_format="npy", atomic_info=info)
+    b, lv, nDim, inf2 = io.load_vec_field(base, data_format="npy")
+    assert np.array_equal(b, a) and np.array_equal(inf2[0], info[0]) and tuple(nDim) == a.shape
+    io.save_scal_field(base + "E", a[..., 0], lvec)
+    e, lv, nDim, _ = io.load_scal_field(base + "E")
+    assert np.allclose(e, a[..., 0], rtol=1e-5, atol=0)
+    io.save_scal_field(base + "E", a[..., 0], lvec, data_format="npy")
+    e, lv, nDim, _ = io.load_scal_field(base + "E", data_format="npy")
+    assert np.array_equal(e, a[..., 0])
+    with pytest.raises(ValueError):
+        io.load_vec_field(base, data_format="cube")
+
+
+def test_cube_reader(tmp_path):
+    """A cube file (2 atoms, 3x4x5 voxels, z fastest) read in bohr/Hartree and
+    returned in Angstrom/eV, transposed to (nz,ny,nx) by default."""
+    rng = np.random.default_rng(2)
+    v = rng.normal(size=(3, 4, 5))
+    fn = str(tmp_path / "d.cube")
+    with open(fn, "w") as f:
+        f.write("comment 1\ncomment 2\n")
+        f.write("    2   -1.000000   0.500000   0.000000\n")
+        f.write("    3    0.400000   0.000000   0.000000\n")
+        f.write("    4    0.000000   0.300000   0.000000\n")
+        f.write("    5    0.000000   0.000000   0.200000\n")
+        f.write("    6    6.000000    0.000000   0.000000   0.000000\n")
+        f.write("    1    1.000000    1.000000   0.000000   0.000000\n")
+        vals = v.ravel()
+        for i in range(0, len(vals), 6):
+            f.write(" ".join("%13.5E" % x for x in vals[i:i + 6]) + "\n")
+    FF, lvec, nDim, head = io.loadCUBE(fn, xyz_order=True)
+    assert FF.shape == (3, 4, 5)
+    assert np.allclose(FF, np.vectorize(lambda x: float("%13.5E" % x))(v) * io.Hartree2eV)
+    assert np.allclose(lvec[0], np.array([-1.0, 0.5, 0.0]) * io.bohrRadius2angstroem)
+    assert np.allclose(lvec[1], [3 * 0.4 * io.bohrRadius2angstroem, 0, 0])
+    FFt, _, nDim2, _ = io.loadCUBE(fn)
+    assert FFt.shape == (5, 4, 3) and list(nDim2) == [5, 4, 3]
+
+
+@pytest.mark.skipif(not O.have_reference_gridutils(), reason="oracle/_ref not built (reference sources absent)")
+def test_reader_bit_equal_to_reference_readnums(tmp_path):
+    """Against the reference's own ReadNumsUpTo_C compiled in oracle/_ref."""
+    a = _data((13, 9, 10), seed=21)
+    a[np.isnan(a)] = 1.0  # keep to values both parse identically by construction
+    fn = str(tmp_path / "ref.xsf")
+    io.saveXSFData(fn, a, np.vstack([np.zeros(3), np.eye(3) * 2.0]))
+    nhead = len(io.loadXSFData(fn)[3])
+    dims = np.array([11, 10, 14])[::-1].copy()
+    ours = io.readNumsUpTo(fn, dims, nhead + 5)
+    ref = O.reference_read_numbers(fn, dims, nhead + 5)
+    assert np.array_equal(ours, ref)
