@@ -314,6 +314,11 @@ ppafm_batch *ppafm_batch_create(int nitems, const ppafm_batch_item *items, const
 /* Upload the packed atoms of all molecules (ppafm_pack_atoms records, item
  * order, natoms of each item).  Synchronous. */
 int ppafm_batch_set_atoms(ppafm_batch *b, const ppafm_atom *atoms);
+/* The same with new atom counts per item (natoms[nitems]): the molecules of
+ * a plan can change between runs while grids and scans stay, as in a
+ * training-data generator.  Grows the atom buffer (and captures the graph
+ * again) when the new total exceeds it.  Synchronous. */
+int ppafm_batch_set_atoms_n(ppafm_batch *b, const int *natoms, const ppafm_atom *atoms);
 /* Enqueue the whole batch (graph launch) on the plan's stream. */
 int ppafm_batch_run(ppafm_batch *b);
 /* Enqueue single stages without the graph: bit 0 grid, bit 1 relaxation

@@ -141,6 +141,7 @@ SIGNATURES = [
         [c_int, ctypes.POINTER(BatchItem), ctypes.POINTER(FFModel), ctypes.POINTER(RelaxParams), ctypes.POINTER(Tip), c_int, c_int, _dp, c_double, c_int, ctypes.POINTER(BatchTuning), c_void_p],
     ),
     ("ppafm_batch_set_atoms", c_int, [c_void_p, c_void_p]),
+    ("ppafm_batch_set_atoms_n", c_int, [c_void_p, c_void_p, c_void_p]),
     ("ppafm_batch_run", c_int, [c_void_p]),
     ("ppafm_batch_run_stages", c_int, [c_void_p, c_int]),
     ("ppafm_batch_tune", c_int, [c_void_p, c_int, c_int, ctypes.POINTER(BatchTuning), _dp]),

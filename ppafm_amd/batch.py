@@ -195,8 +195,8 @@ class BatchScan:
         return b
 
     def set_molecules(self, molecules):
-        """molecules: sequence of (xyzs, Zs, qs) in item order (the atom
-        counts must match the plan's).  LJ parameters: the CPU table
+        """molecules: sequence of (xyzs, Zs, qs) in item order (atom counts
+        may differ from the plan's, see set_atoms).  LJ parameters: the CPU table
         (getAtomsLJ) for plain batches, the REA table for tilted ones."""
         packs = []
         for k, (xyzs, Zs, qs) in enumerate(molecules):
@@ -210,14 +210,21 @@ class BatchScan:
         self.set_atoms(packs)
 
     def set_atoms(self, packs):
-        """packs: per item (natoms, 6) ppafm_atom records (D.pack_atoms)."""
+        """packs: per item (natoms, 6) ppafm_atom records (D.pack_atoms).  The
+        atom counts may differ from the plan's: the plan keeps its grids and
+        scans and takes the new molecules."""
         if len(packs) != self.nitems:
             raise ValueError(f"expected {self.nitems} molecules, got {len(packs)}")
-        for k, p in enumerate(packs):
-            if len(p) != self.natoms[k]:
-                raise ValueError(f"item {k}: plan has {self.natoms[k]} atoms, got {len(p)}")
+        counts = np.array([len(np.asarray(p).reshape(-1, 6)) for p in packs], dtype=np.int32)
         allp = np.ascontiguousarray(np.concatenate([np.asarray(p, dtype=np.float64).reshape(-1, 6) for p in packs], axis=0))
-        check(lib().ppafm_batch_set_atoms(self._h, allp.ctypes.data if len(allp) else None), "batch_set_atoms")
+        ptr = allp.ctypes.data if len(allp) else None
+        if np.array_equal(counts, self.natoms):
+            check(lib().ppafm_batch_set_atoms(self._h, ptr), "batch_set_atoms")
+        else:
+            check(lib().ppafm_batch_set_atoms_n(self._h, counts.ctypes.data, ptr), "batch_set_atoms_n")
+            self.natoms = counts.astype(self.natoms.dtype)
+            for k, it in enumerate(self.items):
+                it["natoms"] = int(counts[k])
         if self._auto:
             self._auto = False
             self.set_tuning(tuned(self, nsample=self.tune_sample))

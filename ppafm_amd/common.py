@@ -28,10 +28,23 @@ CoulombConst = 14.399645  # [eV A / e^2]
 # (lines 1, 6, 7, 8, 47, 79).  Any other element: pass a species file.
 _BUILTIN_SPECIES = [
     (1.4870, 0.000681, 1.80, 1, "H"),
+    (2.0800, 0.003729, 1.80, 5, "B"),
     (1.9080, 0.003729, 1.80, 6, "C"),
     (1.7800, 0.007372, 1.80, 7, "N"),
     (1.6612, 0.009106, 1.80, 8, "O"),
+    (1.7500, 0.002645, 1.80, 9, "F"),
+    (2.7700, 0.010000, 1.80, 11, "Na"),
+    (1.9000, 0.025490, 1.80, 14, "Si"),
+    (2.1000, 0.008673, 1.80, 15, "P"),
+    (2.0000, 0.010841, 1.80, 16, "S"),
+    (1.9480, 0.011491, 1.80, 17, "Cl"),
+    (2.2300, 0.010000, 1.80, 29, "Cu"),
+    (2.2400, 0.010000, 1.80, 30, "Zn"),
+    (2.2200, 0.013876, 1.80, 35, "Br"),
     (2.3700, 0.010000, 1.80, 47, "Ag"),
+    (2.3500, 0.017345, 1.80, 53, "I"),
+    (2.1815, 0.024344, 1.80, 54, "Xe"),
+    (2.3700, 0.010000, 1.80, 78, "Pt"),
     (2.4100, 0.010000, 1.80, 79, "Au"),
 ]
 
@@ -237,6 +250,21 @@ def Fz2df_tilt(F, d, k0, f0, amplitude=1.0, units=16.0217656):
     for k in range(3):
         out = out + device.convolve_z(np.ascontiguousarray(F[..., k]), W, scale=1.0) * d[k]
     return out * units * f0 / k0
+
+
+def maxAlongDir(atoms, hdir):
+    """Index and projection of the atom furthest along hdir (common.py:500-503)."""
+    proj = np.asarray(atoms)[:, :3] @ np.asarray(hdir, dtype=np.float64)
+    i = int(np.argmax(proj))
+    return i, proj[i]
+
+
+def maxAlongDirEntropy(atoms, hdir, beta=1.0):
+    """maxAlongDir plus sum_k exp(beta (p_k - p_max)) over the projections p
+    (common.py:506-510): how many atoms share the top along hdir."""
+    proj = np.asarray(atoms)[:, :3] @ np.asarray(hdir, dtype=np.float64)
+    i = int(np.argmax(proj))
+    return i, proj[i], np.sum(np.exp(beta * (proj - proj[i])))
 
 
 # ------------------------------------------------------------------ parameters

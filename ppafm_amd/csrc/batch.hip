@@ -77,6 +77,7 @@ struct ppafm_batch {
     int *iters = nullptr;
     float *df = nullptr;
     AtomRec *atoms = nullptr;
+    long long atom_cap = 0;  // atom records the atoms buffer holds
     unsigned *ctl = nullptr;  // 8 queue heads (u32) then 2 counters (u64)
     BatchGridItem *d_gitems = nullptr;
     long long *d_bprefix = nullptr;
@@ -372,7 +373,8 @@ int build_plan(ppafm_batch *b, int nitems, const ppafm_batch_item *items)
     PP_TRY_HIP(hipMalloc((void **)&b->force, std::max<long long>(npos, 1) * 16));
     if (b->keep_iters) PP_TRY_HIP(hipMalloc((void **)&b->iters, std::max<long long>(npos, 1) * 4));
     PP_TRY_HIP(hipMalloc((void **)&b->df, std::max<long long>(b->df_off[nitems], 1) * 4));
-    PP_TRY_HIP(hipMalloc((void **)&b->atoms, std::max<long long>(b->atom_off[nitems], 1) * sizeof(AtomRec)));
+    b->atom_cap = std::max<long long>(b->atom_off[nitems], 1);
+    PP_TRY_HIP(hipMalloc((void **)&b->atoms, b->atom_cap * sizeof(AtomRec)));
     PP_TRY_HIP(hipMalloc((void **)&b->ctl, 64));
     PP_TRY_HIP(hipMemsetAsync(b->atoms, 0, std::max<long long>(b->atom_off[nitems], 1) * sizeof(AtomRec), b->st));
     PP_TRY_HIP(hipMemsetAsync(b->ctl, 0, 64, b->st));
@@ -453,6 +455,49 @@ int ppafm_batch_set_atoms(ppafm_batch *b, const ppafm_atom *atoms)
         b->any_uncharged = any;
         if (b->exec) return capture(*b);
     }
+    return 0;
+}
+
+int ppafm_batch_set_atoms_n(ppafm_batch *b, const int *natoms, const ppafm_atom *atoms)
+{
+    PP_REQUIRE(b && natoms, "batch or natoms is NULL");
+    long long total = 0;
+    for (int k = 0; k < b->nitems; k++) {
+        PP_REQUIRE(natoms[k] >= 0, "natoms must be >= 0");
+        total += natoms[k];
+    }
+    PP_REQUIRE(atoms || total == 0, "atoms must not be NULL");
+    PP_TRY_HIP(hipStreamSynchronize(b->st));
+    bool recapture = false;
+    if (total > b->atom_cap) {
+        // a larger buffer: the graph holds the old address, so capture again
+        const long long cap = total + total / 4;
+        AtomRec *p = nullptr;
+        PP_TRY_HIP(hipMalloc((void **)&p, cap * sizeof(AtomRec)));
+        if (b->atoms) PP_TRY_HIP(hipFree(b->atoms));
+        b->atoms = p;
+        b->atom_cap = cap;
+        recapture = true;
+    }
+    for (int k = 0; k < b->nitems; k++) {
+        b->items[k].natoms = natoms[k];
+        b->atom_off[k + 1] = b->atom_off[k] + natoms[k];
+        b->gitems[k].natoms = natoms[k];
+        b->gitems[k].atom_off = b->atom_off[k];
+    }
+    // the grid kernel reads each item's atom count and offset from this
+    // table when it runs
+    int e = upload(b->d_gitems, b->gitems, b->st);
+    if (e) return e;
+    if (total > 0) PP_TRY_HIP(hipMemcpyAsync(b->atoms, atoms, total * sizeof(AtomRec), hipMemcpyHostToDevice, b->st));
+    PP_TRY_HIP(hipStreamSynchronize(b->st));
+    bool any = false;
+    for (long long k = 0; k < total && !any; k++) any = atoms[k].q == 0.0;
+    if (any != b->any_uncharged) {
+        b->any_uncharged = any;
+        recapture = true;
+    }
+    if (recapture && b->exec) return capture(*b);
     return 0;
 }
 

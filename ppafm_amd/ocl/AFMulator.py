@@ -223,35 +223,69 @@ class AFMulator:
         Returns the list of df images X, each as ``__call__`` returns it.
         No counterpart in the reference (one __call__ per molecule,
         AFMulator.py:142-179); the numbers equal the per-molecule calls."""
-        from ..batch import BatchScan
-
-        self.check_scan_window()
-        packs, charged = [], False
+        reqs = []
         for m in molecules:
             kw = dict(m) if isinstance(m, dict) else dict(zip(("xyzs", "Zs", "qs"), m))
-            xyzq, cLJs, ch = self._prepare_atoms(kw["xyzs"], kw["Zs"], kw.get("qs"), rot=kw.get("rot", np.eye(3)),
-                                                 rot_center=kw.get("rot_center"), REAs=kw.get("REAs"))
-            charged = charged or ch
-            packs.append((xyzq, cLJs))
-        model = self._ff_model(charged)
-        packs = [D.pack_atoms(xyzq, cLJs, model.coulomb_const) for xyzq, cLJs in packs]
+            reqs.append(self.request(**kw))
+        return self.run_requests(reqs, tuning=tuning)
+
+    def request(self, xyzs, Zs, qs=None, rho_sample=None, sample_lvec=None, rot=np.eye(3), rot_center=None, REAs=None,
+                X=None, plot_to_dir=None):
+        """Everything one ``eval`` with the current settings would compute,
+        captured for a later batched evaluation (run_requests): the molecule's
+        atoms in the grid frame, the grid, the scan, the tip, the relaxation
+        and the df weights.  Later changes to the settings do not affect it.
+        Takes the arguments of ``eval``; X (an output array) is not used."""
+        if plot_to_dir:
+            raise NotImplementedError("plotting is not on this path")
+        self.check_scan_window()
+        xyzq, cLJs, charged = self._prepare_atoms(xyzs, Zs, qs, rho_sample, sample_lvec, rot, rot_center, REAs)
         self.prepareScanner()
         n = tuple(int(v) for v in self.nDim[:3])
         cell = self.lvec[1:].astype(np.float32).astype(np.float64)
-        key = (n, tuple(len(p) for p in packs), tuple(self.scan_dim), self.dfWeight.tobytes(), charged,
-               np.asarray(self.stiffness).tobytes(), np.asarray(self.relax_params).tobytes(), tuple(np.float32(self.tipR0)),
-               repr(tuning))
-        if getattr(self, "_batch_key", None) != key:
-            self._batch = None
-            items = [dict(n=n, cell=cell, natoms=len(p), scan=self._scan) for p in packs]
-            self._batch = BatchScan(items, model, self._rp, self._tip, df_weights=self.dfWeight.astype(np.float64),
-                                    df_scale=1.0, grid_f64=True, tuning=tuning)
-            self._batch_key = key
-        b = self._batch
-        b.set_atoms(packs)
-        b.run()
-        b.synchronize()
-        return [b.df_raw(k) for k in range(len(packs))]
+        return dict(xyzq=xyzq, cLJs=cLJs, charged=charged, Qs=tuple(np.float32(self.Qs).tolist()),
+                    QZs=tuple(np.float32(self.QZs).tolist()), n=n, cell=cell, scan=self._scan, tip=self._tip, rp=self._rp,
+                    dfw=self.dfWeight.astype(np.float64), out_shape=(self.scan_dim[0], self.scan_dim[1],
+                                                                     self.scan_dim[2] - len(self.dfWeight) + 1))
+
+    _PLAN_CACHE = 4  # batch plans kept per AFMulator (grids and graphs stay allocated)
+
+    def run_requests(self, reqs, tuning=None):
+        """Evaluate requests from ``request`` in as few batch plans as their
+        settings allow: requests with the same tip charges, tip, relaxation and
+        df weights share a plan (one captured graph), whatever their grids,
+        scans and molecules.  Plans are cached and reused when the grids and
+        scans recur (the molecules may change).  Returns the df images in
+        request order."""
+        from ..batch import BatchScan
+
+        groups = {}
+        for i, r in enumerate(reqs):
+            key = (r["Qs"], r["QZs"], bytes(r["tip"]), bytes(r["rp"]), r["dfw"].tobytes())
+            groups.setdefault(key, []).append(i)
+        out = [None] * len(reqs)
+        cache = self.__dict__.setdefault("_plans", {})
+        for key, idx in groups.items():
+            charged = any(reqs[i]["charged"] for i in idx)
+            r0 = reqs[idx[0]]
+            stencil = list(zip(r0["Qs"], r0["QZs"])) if charged else []
+            model = D.ff_model(1, with_lj=True, stencil=stencil, with_energy=True, fmax=None)
+            packs = [D.pack_atoms(reqs[i]["xyzq"], reqs[i]["cLJs"], model.coulomb_const) for i in idx]
+            geo = tuple((reqs[i]["n"], reqs[i]["cell"].tobytes(), bytes(reqs[i]["scan"])) for i in idx)
+            pkey = (key, charged, geo, repr(tuning))
+            b = cache.get(pkey)
+            if b is None:
+                items = [dict(n=reqs[i]["n"], cell=reqs[i]["cell"], natoms=len(p), scan=reqs[i]["scan"]) for i, p in zip(idx, packs)]
+                b = BatchScan(items, model, r0["rp"], r0["tip"], df_weights=r0["dfw"], df_scale=1.0, grid_f64=True, tuning=tuning)
+                if len(cache) >= self._PLAN_CACHE:
+                    cache.pop(next(iter(cache)))
+                cache[pkey] = b
+            b.set_atoms(packs)
+            b.run()
+            b.synchronize()
+            for k, i in enumerate(idx):
+                out[i] = b.df_raw(k)
+        return out
 
     def prepareScanner(self):
         self.pos0 = np.array([0, 0, self.scan_window[1][2]]) - self.lvec[0]

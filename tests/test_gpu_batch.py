@@ -137,10 +137,29 @@ def test_autotuner_and_cache(run, tmp_path):
 def test_batch_errors(run):
     cases, b, _ = run
     with pytest.raises(ValueError):
-        b.set_atoms([np.zeros((3, 6))] * b.nitems)
+        b.set_atoms([np.zeros((3, 6))] * (b.nitems + 1))
     with pytest.raises(HipError):
         b.set_tuning({"tile_y": 3, "waves_per_block": 1, "blocks_per_cu": 1})
     assert b.tuning["tile_y"] != 3
+
+
+def test_new_molecules_with_other_atom_counts():
+    """A plan takes molecules of other sizes (fewer, then more atoms than it
+    was built for, which grows its atom buffer and captures the graph again)
+    and gives the single-molecule results for them."""
+    print("case: plain batch of 5, then two sets of other molecules (3..90 atoms)", flush=True)
+    cases = _cases()
+    b = _batch(cases)
+    for seed0, sizes in ((300, (3, 30, 12, 5, 9)), (400, (90, 60, 75, 40, 88))):
+        mols = [M.molecule(na, seed0 + k) for k, na in enumerate(sizes)]
+        b.set_molecules(mols)
+        b.run()
+        b.synchronize()
+        for k, (c, m) in enumerate(zip(cases, mols)):
+            s = _plain(c[0], c[1], m)
+            assert np.array_equal(b.grid(k), s.grid_host()), (seed0, k)
+            assert np.array_equal(b.positions_raw(k), s.pos.download()), (seed0, k)
+            assert np.array_equal(b.df_host(k), s.df_host()), (seed0, k)
 
 
 def test_afmulator_batch_equals_calls():
