@@ -47,6 +47,8 @@ def parse():
     p.add_argument("--learned-epochs", type=float, default=8.0)
     p.add_argument("--c4-instances", type=int, default=16,
                    help="instances of the large tilted case (Au slab + molecule, 400x400x300 grid; 0 skips it)")
+    p.add_argument("--generator-molecules", type=int, default=64,
+                   help="molecules of the generator comparison, per call vs batched (0 skips it)")
     p.add_argument("--c4-grid", choices=["f64", "f32"], default="f64", help="grid storage of the large tilted case")
     return p.parse_args()
 
@@ -362,6 +364,44 @@ def run_learned(args, b):
     return out
 
 
+def run_generator(args, batch=32):
+    """AFM images as the ML generators make them: AFMulator with the
+    reference's defaults (128x128x30 scan over a 16x16 A window, CO-like tip
+    charges) for molecules of the batched configuration (U[8,120] atoms),
+    evaluated one call per molecule (the reference generators' loop,
+    ml/Generator.py:96-113) and as batches of requests
+    (AFMulator.request/run_requests, which ppafm_amd.ml.Generator uses)."""
+    from ppafm_amd import molgen as M
+    from ppafm_amd.ocl.AFMulator import AFMulator
+
+    n = args.generator_molecules
+    mols = []
+    for k in range(n + batch):
+        xyz, Zs, qs = M.config_molecule(2, k)
+        mols.append((xyz + np.array([10.0, 10.0, 0.0]), Zs, qs))
+    warm, mols = mols[:batch], mols[batch:]
+    afm = AFMulator()
+    afm(*warm[0])
+    afm.run_requests([afm.request(*m) for m in warm])  # builds and captures the plan
+    t0 = time.perf_counter()
+    per_call = [afm(*m) for m in mols]
+    t_call = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    batched = []
+    for i in range(0, n, batch):
+        batched += afm.run_requests([afm.request(*m) for m in mols[i:i + batch]])
+    t_batch = time.perf_counter() - t0
+    same = all(np.array_equal(a, b) for a, b in zip(per_call, batched))
+    return {
+        "workload": f"{n} molecules of the batched configuration (U[8,120] atoms), AFMulator defaults: grid {tuple(int(v) for v in afm.nDim[:3])}, "
+                    f"scan {tuple(afm.scan_dim)}, df_steps {afm.df_steps}; batches of {batch}",
+        "images_per_s_per_call": n / t_call,
+        "images_per_s_batched": n / t_batch,
+        "speedup": t_call / t_batch,
+        "identical_images": bool(same),
+    }
+
+
 def run_c4(args):
     """Large tilted case: per instance an Au(111) slab with a seeded
     60-atom molecule and 3x3 xy images (17,928 atoms), the OpenCL LJ +
@@ -630,6 +670,9 @@ def main():
             out["learned"] = dict(workload="the batch above: step-0 network + Newton warm starts, FIRE to the same threshold",
                                   **bt["learned"])
             out["learned"]["learned"]["estimates_mfma"]["peak_measured_tflops"] = peaks["bf16_mfma"]
+    if args.generator_molecules > 0:
+        print(f"[bench] generator: {args.generator_molecules} molecules", file=sys.stderr, flush=True)
+        out["generator"] = run_generator(args)
     if args.c4_instances > 0:
         print(f"[bench] c4: {args.c4_instances} instances", file=sys.stderr, flush=True)
         out["c4"] = run_c4(args)
