@@ -24,30 +24,31 @@ def shift_positions(R, s):
     return Rs
 
 
-def relaxedScan3D(xTips, yTips, zTips, trj=None, bF3d=False):
-    """Column by column (relaxTipStroke), honouring a tilted trajectory ``trj``
-    (HighLevel.py:63-101)."""
-    ntips = len(zTips)
-    rTips = np.zeros((ntips, 3))
-    rs = np.zeros((ntips, 3))
-    fs = np.zeros((ntips, 3))
-    nx, ny = len(xTips), len(yTips)
-    fzs = np.zeros((nx, ny, ntips, 3)) if bF3d else np.zeros((ntips, ny, nx))
-    PPpos = np.zeros((nx, ny, ntips, 3))
+def relaxedScan3D(xTips, yTips, zTips, trj=None, bF3d=False, tip_spline=None):
+    """Relaxed scan along a tip trajectory ``trj`` (ntips,3), default straight
+    down zTips (HighLevel.py:63-101): column (ix, iy) is the stroke
+    trj + (x, y, 0).  The reference relaxes the columns one relaxTipStroke
+    call at a time; here all of them go to the GPU in one
+    relaxTipStrokes_omp call with explicit tip positions, which relaxes each
+    column as the same stroke.  Returns (fzs, PPpos); fzs is (ntips, ny, nx)
+    Fz, or (nx, ny, ntips, 3) forces with bF3d."""
+    ntips, nx, ny = len(zTips), len(xTips), len(yTips)
     if trj is None:
         trj = np.zeros((ntips, 3))
-        trj[:, 2] = zTips[::-1]
-    for ix, x in enumerate(xTips):
-        for iy, y in enumerate(yTips):
-            rTips[:, 0] = trj[:, 0] + x
-            rTips[:, 1] = trj[:, 1] + y
-            rTips[:, 2] = trj[:, 2]
-            core.relaxTipStroke(rTips, rs, fs)
-            if bF3d:
-                fzs[ix, iy, :, :] = fs[::-1, :]
-            else:
-                fzs[:, iy, ix] = fs[::-1, 2]
-            PPpos[ix, iy, :, :] = rs[::-1, :]
+        trj[:, 2] = np.asarray(zTips)[::-1]
+    trj = np.asarray(trj, dtype=np.float64)
+    rTips = np.empty((nx, ny, ntips, 3))
+    rTips[..., 0] = trj[None, None, :, 0] + np.asarray(xTips, dtype=np.float64)[:, None, None]
+    rTips[..., 1] = trj[None, None, :, 1] + np.asarray(yTips, dtype=np.float64)[None, :, None]
+    rTips[..., 2] = trj[None, None, :, 2]
+    rs = np.zeros_like(rTips)
+    fs = np.zeros_like(rTips)
+    core.relaxTipStrokes_omp(rTips, rs, fs, tip_spline=tip_spline)
+    PPpos = rs[:, :, ::-1, :].copy()
+    if bF3d:
+        fzs = fs[:, :, ::-1, :].copy()
+    else:
+        fzs = np.ascontiguousarray(fs[:, :, ::-1, 2].transpose(2, 1, 0))
     return fzs, PPpos
 
 

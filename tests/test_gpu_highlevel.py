@@ -67,3 +67,41 @@ def test_computeLJ_models_match_oracle(tmp_path, model, damp):
     print(f"{model}/{damp}: max rel dF {rf:.3g}, dE {re:.3g}")
     assert rf < 1e-13
     assert re < 1e-13 or np.abs(Eo).max() == 0
+
+
+def test_relaxedScan3D_tilted_trajectory_matches_oracle():
+    """relaxedScan3D along trjByDir (HighLevel.py:36-41, 63-101): column
+    (ix, iy) is the stroke trj + (x, y, 0); vs the oracle relaxing the same
+    explicit strokes.  Untilted, it equals relaxedScan3D_omp."""
+    from cases import case, tip_params
+
+    from ppafm_amd import core
+
+    print("case: small molecule grid 64x64x80, 9x7 columns of 12 steps tilted by (0.02, -0.01, -0.1)", flush=True)
+    k = case(0, small=True)
+    F = np.ascontiguousarray(k["FF"])
+    core.setGridN(k["cfg"]["n"])
+    core.setGridCell(k["lvec"])
+    core.setFF_Fpointer(F)
+    tp = tip_params()
+    core.setTip(lRadial=tp[0], kRadial=tp[1], rPP0=np.zeros(3), kSpring=tp[5:8].copy())
+    core.setRelax(1000, 1e-4, 0.1, 0.1)
+    core.setFIRE(1.1, 0.5, 0.99)
+    o = k["lvec"][0]
+    xs = np.linspace(-0.8, 0.8, 9) - o[0]
+    ys = np.linspace(-0.6, 0.6, 7) - o[1]
+    zs = np.linspace(6.0, 7.1, 12) - o[2]
+    trj = PPH.trjByDir(len(zs), d=[0.02, -0.01, -0.1], p0=[0.0, 0.0, zs[0]])
+    fzs, pos = PPH.relaxedScan3D(xs, ys, zs, trj=trj, bF3d=True)
+    tips = np.zeros((9, 7, 12, 3))
+    tips[..., 0] = trj[None, None, :, 0] + xs[:, None, None]
+    tips[..., 1] = trj[None, None, :, 1] + ys[None, :, None]
+    tips[..., 2] = trj[None, None, :, 2]
+    rr, fr, it, tot = O.relax_strokes(k["cfg"]["n"], k["lvec"][1:], F, tp, tips)
+    d = np.abs(pos - rr[:, :, ::-1, :]).max()
+    df = np.abs(fzs - fr[:, :, ::-1, :]).max()
+    print(f"tilted: max |dr| {d:.3g} A, max |dF| {df:.3g} eV/A")
+    assert d < 1e-9 and df < 1e-9
+    f1, p1 = PPH.relaxedScan3D(xs, ys, zs)
+    f2, p2 = PPH.relaxedScan3D_omp(xs, ys, zs)
+    assert np.array_equal(p1, p2) and np.array_equal(f1, f2.transpose(2, 1, 0))
