@@ -467,6 +467,111 @@ int ppafm_read_numbers(const char *fname, int noline, long long count, double *o
 /* Sum of iteration indices of the last relaxTipStrokes_omp call. */
 long long ppafm_last_itrsum(void);
 
+/* ------------------------------------------------------------------------
+ * (4) Grids of the Hartree and full-density-based force fields
+ *     (ppafm/ocl/field.py DataGrid family, FFTCrossCorrelation,
+ *     ForceField_LJC.calc_force_hartree / calc_force_fdbm; kernels of
+ *     ppafm/ocl/cl/FF.cl).  Device ABI conventions as in (2).
+ *
+ * A grid is n[0]*n[1]*n[2] float64 values, C order (z fastest), with a
+ * lattice lvec[12] = 4 rows (origin, a, b, c); its step vectors are a/n[0],
+ * b/n[1], c/n[2].  The reference keeps these grids in FP32; here every
+ * value and every coordinate is FP64.
+ * ---------------------------------------------------------------------- */
+
+/* DataGrid.interp_at (field.py:528-575), kernel interp_at (FF.cl:1010-1046):
+ * out[i,j,k] = periodic trilinear interpolation of `in` at the point
+ * rot_center + rot (p - rot_center), p = out origin + i a' + j b' + k c'
+ * (a', b', c' the out step vectors; rot row-major 3x3). */
+int ppafm_grid_interp_at(const double in_lvec[12], const int in_n[3], const double *in_dev, const double out_lvec[12],
+                         const int out_n[3], double *out_dev, const double rot[9], const double rot_center[3],
+                         void *stream);
+/* TipDensity.interp_at (field.py:607-660), kernel interp_tip_at
+ * (FF.cl:1049-1093): the tip sits on the origin of both grids (their
+ * origins are ignored); out indices past half way wrap to negative
+ * offsets; points whose coordinates inv(A) p (A = rows a, b, c of in_lvec)
+ * exceed 1/2 in magnitude are 0, others interpolate `in` periodically. */
+int ppafm_grid_interp_tip_at(const double in_lvec[12], const int in_n[3], const double *in_dev,
+                             const double out_lvec[12], const int out_n[3], double *out_dev, void *stream);
+/* DataGrid.grad (field.py:471-526), kernel grad (FF.cl:920-961): periodic
+ * centred differences along the three axes with steps step[3],
+ * out[p] = scale * (dV/dx, dV/dy, dV/dz, V) as double4; out in C order
+ * (order_c != 0) or Fortran order (x fastest); accumulate != 0 adds to out
+ * instead of overwriting it (the force field of calc_force_hartree is
+ * this gradient plus the Lennard-Jones grid). */
+int ppafm_grid_grad(const int n[3], const double step[3], const double *in_dev, const double scale[4], int order_c,
+                    int accumulate, double *out_dev, void *stream);
+/* ForceField_LJC.run_gradPotentialGrid (field.py:1225-1317), kernel
+ * gradPotentialGrid (FF.cl:964-1007): the field -grad V on the grid
+ * (lvec, n) by centred differences of step h between trilinear
+ * interpolations of the potential grid pot at p -/+ h along x, y, z;
+ * field_dev (n, double4) = (Ex, Ey, Ez, V(p)), C order. */
+int ppafm_grid_grad_potential(const double pot_lvec[12], const int pot_n[3], const double *pot_dev,
+                              const double lvec[12], const int n[3], double h, double *field_dev, void *stream);
+/* The electrostatic part of evalLJC_Hartree (FF.cl:860-917): for every
+ * point p of the grid (lvec, n), grid4 (n, double4) += sum over q with
+ * Qs[q] != 0 of Qs[q] * (periodic trilinear interpolation of field_dev
+ * (n, double4, same grid) at p + (0, 0, QZs[q])). */
+int ppafm_grid_add_tip_charges(const double lvec[12], const int n[3], const double *field_dev, const double Qs[4],
+                               const double QZs[4], double *grid4_dev, void *stream);
+/* normalizeSumReduce + sumSingleGroup (FF.cl:1112-1176) as used by
+ * DataGrid._get_normalization_factor (field.py:447-469): sums[0] = sum of
+ * all n values, sums[1] = sum of the positive ones (host array; the call
+ * waits for the stream).  The reduction order is fixed for a given n. */
+int ppafm_grid_sums(long long n, const double *in_dev, double sums[2], void *stream);
+/* Kernel power (FF.cl:1096-1107): out = scale * max(0, in)^p. */
+int ppafm_grid_power(long long n, const double *in_dev, double *out_dev, double p, double scale, void *stream);
+/* Kernel addMult (FF.cl:1213-1230): out = in1 + A * in2. */
+int ppafm_grid_add_mult(long long n, const double *in1_dev, const double *in2_dev, double A, double *out_dev,
+                        void *stream);
+/* Kernels clamp_hard / clamp_soft (FF.cl:1179-1211): hard clips to
+ * [lo, hi]; soft (soft != 0) applies the sigmoid transitions of width
+ * `width` at each finite bound. */
+int ppafm_grid_clamp(long long n, const double *in_dev, double *out_dev, double lo, double hi, int soft, double width,
+                     void *stream);
+/* MultipoleTipDensity._make_tip_density (field.py:671-690) with
+ * getProbeDensity (fieldFFT.py:99-118) on the device: coordinates
+ * X = cx[i], Y = cy[j], Z = cz[k] (host arrays, the wrapped linspace
+ * coordinates along each lattice vector), r = mat (X, Y, Z) with mat
+ * row-major 3x3 (the transposed normalised lattice), Gaussian
+ * exp(-|r|^2 / (2 sigma^2)) normalised by its grid sum times
+ * norm[0] * norm[1] * norm[2] * norm[3] (|det mat| and the three steps,
+ * multiplied in that order), times sum_k coefs[k] Y_k(r / sigma) with the
+ * harmonic of kinds[k] (0 s, 1 px, 2 py, 3 pz, 4 dz2, 5 dx2, 6 dy2, 7 dxy,
+ * 8 dxz, 9 dyz, other values contribute 0) tilted by `tilt` in the zx
+ * plane; nkinds < 0 keeps the normalised Gaussian.  nkinds <= 16. */
+int ppafm_tip_density_multipole(const int n[3], const double *cx, const double *cy, const double *cz,
+                                const double mat[9], double sigma, double tilt, int nkinds, const int *kinds,
+                                const double *coefs, const double norm[4], double *out_dev, void *stream);
+
+/* FFTCrossCorrelation (field.py:707-843): circular cross-correlation of
+ * grids of shape n with a fixed tip density, by real-to-complex FP64 FFTs:
+ * E = scale * cell_vol * IFFT(conj(FFT(rho)) FFT(in)) with the normalised
+ * inverse, i.e. E[r] = scale * cell_vol * sum_s rho[s] in[s + r]
+ * (indices periodic).  create makes the plans and buffers; set_rho
+ * transforms rho (_set_rho, field.py:800-802; cell_vol is its grid cell
+ * volume); correlate leaves in_dev unchanged and may write E_dev = in_dev.
+ * Calls on one object are serialised by the caller. */
+typedef struct ppafm_fft_corr ppafm_fft_corr;
+int ppafm_fft_corr_create(const int n[3], ppafm_fft_corr **out);
+int ppafm_fft_corr_set_rho(ppafm_fft_corr *c, const double *rho_dev, double cell_vol, void *stream);
+int ppafm_fft_corr_correlate(ppafm_fft_corr *c, const double *in_dev, double scale, double *E_dev, void *stream);
+int ppafm_fft_corr_destroy(ppafm_fft_corr *c);
+
+/* addvdW (FF.cl:548-665) onto a double4 grid (C order, points of `grid`
+ * relative to its origin; grid->f64 ignored): damp_method -1 bare
+ * -C6/(r^2 + 1e-6)^3, 0 constant damping (force only), 1..4 the R2, R4,
+ * invR4, invR8 damping functions of u^2 = r^2/R^2.  pos_dev (natoms,3)
+ * relative to the grid origin; coefs_dev (natoms,2): (c6, c12) for
+ * damp_method <= 0, (R, E0) otherwise. */
+int ppafm_grid_add_vdw(const ppafm_grid *grid, int damp_method, int natoms, const double *pos_dev,
+                       const double *coefs_dev, double *grid4_dev, void *stream);
+/* addDFTD3_BJ (FF.cl:790-857) onto a double4 grid: coefs_dev (natoms,4) =
+ * (c6 s6, c8 s8, R0^6, R0^8) as computeD3Coeffs returns them; pairs beyond
+ * r^2 > 400 are skipped. */
+int ppafm_grid_add_d3(const ppafm_grid *grid, int natoms, const double *pos_dev, const double *coefs_dev,
+                      double *grid4_dev, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

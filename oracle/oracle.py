@@ -205,7 +205,7 @@ def force_jac(n, cell, F, tip, rtips, rs):
 # Oracle model ids (pp_oracle.c pair_model) of the reference's grid builders
 # and evalRadialFF kinds.
 MODEL = dict(vdw=4, vdw_r2=5, vdw_r4=6, vdw_invr4=7, vdw_invr8=8, d3=9, morse=10, gauss=11, slater=12, r4=13,
-             lj_re=14, inv_r6=15, lj=0)
+             lj_re=14, inv_r6=15, lj=0, vdw_nodamp_ocl=16, vdw_invr4_ocl=17, vdw_invr8_ocl=18)
 RADIAL_KIND = {-3: "lj_re", -2: "lj", -1: "inv_r6", 0: "vdw", 1: "vdw_r2", 2: "vdw_r4", 3: "vdw_invr4", 4: "vdw_invr8"}
 ADAMP_DEFAULT = {"vdw": 180.0, "vdw_r2": 0.5, "vdw_r4": 0.5, "vdw_invr4": 0.03, "vdw_invr8": 0.01}  # ProbeParticle.cpp:48
 

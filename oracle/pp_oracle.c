@@ -760,8 +760,11 @@ long orc_relax_strokes_warm(int nx, int ny, int nz, const int n[3], const double
  * par = ADamp), 5..8 vdW with damping function R2/R4/invR4/invR8 (R, E0;
  * par = ADamp), 9 DFT-D3 (c6, c8, R0^6, R0^8), 10 Morse (R0, E0; par =
  * alpha), 11/12/13 Gauss/Slater/R4 densities (amp, width; energy only),
- * 14 LJ in (R0, E0), 15 bare r^-6 (c6). */
-static inline double damp_fn(int kind, double u2, double *dd)
+ * 14 LJ in (R0, E0), 15 bare r^-6 (c6).  The vdW of the OpenCL force
+ * field (addvdW, FF.cl:548-665) writes R2SAFE as 1e-6f: 16 no damping (c6),
+ * 17/18 invR4/invR8 damping with that constant (models 5/6 serve its R2/R4). */
+#define ORC_R2SAFE_OCL ((double)1.0e-6f)
+static inline double damp_fn(int kind, double u2, double *dd, double r2safe)
 {
     if (kind == 1) {
         if (u2 > 1) { *dd = 0; return 0; }
@@ -774,7 +777,7 @@ static inline double damp_fn(int kind, double u2, double *dd)
         *dd = -4 * e;
         return e * e;
     }
-    double ir2 = 1 / (u2 + ORC_R2SAFE);
+    double ir2 = 1 / (u2 + r2safe);
     if (kind == 3) {
         double ir4 = ir2 * ir2;
         *dd = -4 * ir4 * ir2;
@@ -796,9 +799,9 @@ static double pair_model(int model, v3 d, v3 *f, const double *c, double par)
         s = -6 * c[0] / (r2 + par * c[0]);
         f->x += d.x * s; f->y += d.y * s; f->z += d.z * s;
         return 0;
-    case 5: case 6: case 7: case 8: {
+    case 5: case 6: case 7: case 8: case 17: case 18: {
         double ir2 = 1. / (c[0] * c[0]), u2 = r2 * ir2, u4 = u2 * u2, dd;
-        double D = damp_fn(model - 4, u2, &dd);
+        double D = model < 17 ? damp_fn(model - 4, u2, &dd, ORC_R2SAFE) : damp_fn(model - 14, u2, &dd, ORC_R2SAFE_OCL);
         double ee = 1. / (u4 * u2 + D * par);
         e = -2 * c[1] * ee;
         s = e * ee * (6 * u4 + dd * par) * ir2;
@@ -846,6 +849,13 @@ static double pair_model(int model, v3 d, v3 *f, const double *c, double par)
         double ir2 = 1 / r2, ir4 = ir2 * ir2;
         e = -c[0] * ir4 * ir2;
         s = e * 6 * ir2;
+        f->x += d.x * s; f->y += d.y * s; f->z += d.z * s;
+        return e;
+    }
+    case 16: {
+        double ir2 = 1.0 / (r2 + ORC_R2SAFE_OCL), ir6 = ir2 * ir2 * ir2;
+        e = -c[0] * ir6;
+        s = 6.0 * e * ir2;
         f->x += d.x * s; f->y += d.y * s; f->z += d.z * s;
         return e;
     }

@@ -183,6 +183,23 @@ SIGNATURES = [
     ("ppafm_last_error", ctypes.c_char_p, []),
     ("ppafm_clear_error", None, []),
     ("ppafm_last_itrsum", ctypes.c_longlong, []),
+    # (4) Hartree / density-based force-field grids
+    ("ppafm_grid_interp_at", c_int, [_dp, _ip, c_void_p, _dp, _ip, c_void_p, _dp, _dp, c_void_p]),
+    ("ppafm_grid_interp_tip_at", c_int, [_dp, _ip, c_void_p, _dp, _ip, c_void_p, c_void_p]),
+    ("ppafm_grid_grad", c_int, [_ip, _dp, c_void_p, _dp, c_int, c_int, c_void_p, c_void_p]),
+    ("ppafm_grid_grad_potential", c_int, [_dp, _ip, c_void_p, _dp, _ip, c_double, c_void_p, c_void_p]),
+    ("ppafm_grid_add_tip_charges", c_int, [_dp, _ip, c_void_p, _dp, _dp, c_void_p, c_void_p]),
+    ("ppafm_grid_sums", c_int, [ctypes.c_longlong, c_void_p, _dp, c_void_p]),
+    ("ppafm_grid_power", c_int, [ctypes.c_longlong, c_void_p, c_void_p, c_double, c_double, c_void_p]),
+    ("ppafm_grid_add_mult", c_int, [ctypes.c_longlong, c_void_p, c_void_p, c_double, c_void_p, c_void_p]),
+    ("ppafm_grid_clamp", c_int, [ctypes.c_longlong, c_void_p, c_void_p, c_double, c_double, c_int, c_double, c_void_p]),
+    ("ppafm_tip_density_multipole", c_int, [_ip, _dp, _dp, _dp, _dp, c_double, c_double, c_int, _ip, _dp, _dp, c_void_p, c_void_p]),
+    ("ppafm_fft_corr_create", c_int, [_ip, ctypes.POINTER(c_void_p)]),
+    ("ppafm_fft_corr_set_rho", c_int, [c_void_p, c_void_p, c_double, c_void_p]),
+    ("ppafm_fft_corr_correlate", c_int, [c_void_p, c_void_p, c_double, c_void_p, c_void_p]),
+    ("ppafm_fft_corr_destroy", c_int, [c_void_p]),
+    ("ppafm_grid_add_vdw", c_int, [ctypes.POINTER(Grid), c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("ppafm_grid_add_d3", c_int, [ctypes.POINTER(Grid), c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
 ]
 
 _lib = None

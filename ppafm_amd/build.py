@@ -21,6 +21,8 @@ ARCH = os.environ.get("PPAFM_HIP_ARCH", "gfx950")
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 CXX = shutil.which("g++") or "g++"
+# hipFFT (the FFT cross-correlations of the Hartree force field)
+ROCM_LIB = pathlib.Path(os.environ.get("ROCM_PATH", "/opt/rocm")) / "lib"
 
 COMMON_FLAGS = [
     f"--offload-arch={ARCH}",
@@ -60,7 +62,8 @@ def build(verbose=False, jobs=None):
     with concurrent.futures.ThreadPoolExecutor(jobs) as ex:
         objs = list(ex.map(lambda s: _compile(s, verbose), srcs))
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", *map(str, objs), "-o", str(tmp)]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", *map(str, objs), "-o", str(tmp),
+           f"-L{ROCM_LIB}", "-lhipfft", f"-Wl,-rpath,{ROCM_LIB}"]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

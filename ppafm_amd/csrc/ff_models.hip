@@ -13,6 +13,8 @@
 //   Coulomb s/pz/dz2   addAtomCoulomb + stencils Forces.h:237-244,
 //                                                ProbeParticle.cpp:178-209
 //   densities          addAtom_Gauss/Slater/splineR4 ProbeParticle.cpp:227-256
+//   OpenCL vdW         addvdW without damping and with the invR4 / invR8
+//                      damping (FF.cl:548-665: R2SAFE 1e-6f there)
 // and the entry points built on them:
 //   grids      getLennardJonesFF, getVdWFF, getVdWFF_RE, getDFTD3FF,
 //              getMorseFF, getCoulombFF, get{Gauss,Slater}Density,
@@ -89,7 +91,7 @@ __device__ __forceinline__ double pair_vdw_const(v3d d, v3d &f, double c6, doubl
 
 // Damping functions of u2 = r^2 / R^2 and their derivatives d/du2
 // (R2_func, R4_func, invR4_func, invR8_func, Forces.h:150-199).
-__device__ __forceinline__ double damp_fn(int kind, double u2, double &dd)
+__device__ __forceinline__ double damp_fn(int kind, double u2, double &dd, double r2safe = kR2Safe)
 {
     if (kind == 1) {
         if (u2 > 1) { dd = 0; return 0; }
@@ -102,7 +104,7 @@ __device__ __forceinline__ double damp_fn(int kind, double u2, double &dd)
         dd = -4 * e;
         return e * e;
     }
-    const double ir2 = 1 / (u2 + kR2Safe);
+    const double ir2 = 1 / (u2 + r2safe);
     if (kind == 3) {
         const double ir4 = ir2 * ir2;
         dd = -4 * ir4 * ir2;
@@ -114,18 +116,32 @@ __device__ __forceinline__ double damp_fn(int kind, double u2, double &dd)
     return ir8;
 }
 
-__device__ __forceinline__ double pair_vdw_damp(int kind, v3d d, v3d &f, double r, double e0, double adamp)
+__device__ __forceinline__ double pair_vdw_damp(int kind, v3d d, v3d &f, double r, double e0, double adamp,
+                                               double r2safe = kR2Safe)
 {
     const double r2 = norm2(d);
     const double ir2 = 1. / (r * r);
     const double u2 = r2 * ir2;
     const double u4 = u2 * u2;
     double dd;
-    const double dfn = damp_fn(kind, u2, dd);
+    const double dfn = damp_fn(kind, u2, dd, r2safe);
     const double e = 1. / (u4 * u2 + dfn * adamp);
     const double en = -2 * e0 * e;
     add_mul(f, d, en * e * (6 * u4 + dd * adamp) * ir2);
     return en;
+}
+
+// addvdW of the OpenCL force field (FF.cl:548-665) without damping: the
+// R2SAFE there is the float literal 1e-6f.
+constexpr double kR2SafeOcl = (double)1.0e-6f;
+
+__device__ __forceinline__ double pair_vdw_nodamp_ocl(v3d d, v3d &f, double c6)
+{
+    const double ir2 = 1.0 / (norm2(d) + kR2SafeOcl);
+    const double ir6 = ir2 * ir2 * ir2;
+    const double e = -c6 * ir6;
+    add_mul(f, d, 6.0 * e * ir2);
+    return e;
 }
 
 __device__ __forceinline__ double pair_d3(v3d d, v3d &f, double c6, double c8, double r06, double r08)
@@ -224,6 +240,9 @@ __device__ __forceinline__ double pair_model(int m, v3d d, v3d &f, const double 
     case kModelR4: return dens_r4(d, c[0], c[1]);
     case kModelLJRE: return pair_lj_re(d, f, c[0], c[1]);
     case kModelInvR6: return pair_inv_r6(d, f, c[0]);
+    case kModelVdWNoDampOcl: return pair_vdw_nodamp_ocl(d, f, c[0]);
+    case kModelVdWInvR4Ocl: return pair_vdw_damp(3, d, f, c[0], c[1], par, kR2SafeOcl);
+    case kModelVdWInvR8Ocl: return pair_vdw_damp(4, d, f, c[0], c[1], par, kR2SafeOcl);
     default: return 0;
     }
 }
@@ -234,6 +253,7 @@ struct AccumArgs {
     int n[3];
     long npts;
     int natoms, ncoef, model;
+    int fstride, estride;  // doubles between consecutive points of F and E
     double par;
     double dcell[9];
 };
@@ -279,8 +299,11 @@ __global__ __launch_bounds__(256) void ff_accum_models(AccumArgs A, const double
         }
     }
     if (!live) return;
-    if (F) { F[3 * ig] += f.x; F[3 * ig + 1] += f.y; F[3 * ig + 2] += f.z; }
-    if (E) E[ig] += e;
+    if (F) {
+        double *fp = F + ig * A.fstride;
+        fp[0] += f.x; fp[1] += f.y; fp[2] += f.z;
+    }
+    if (E) E[ig * A.estride] += e;
 }
 
 // getInPoints_LJ (ProbeParticle.cpp:379-395): LJ at listed points, with the
@@ -487,6 +510,13 @@ __global__ __launch_bounds__(64) void test_force_kernel(TestForceArgs A, double 
 int launch_ff_models(const GridGeom &g, int model, double par, int natoms, const double *pos, const double *coefs,
                      int ncoef, double *F, double *E, hipStream_t st)
 {
+    return launch_ff_models_strided(g, model, par, natoms, pos, coefs, ncoef, F, 3, E, 1, st);
+}
+
+int launch_ff_models_strided(const GridGeom &g, int model, double par, int natoms, const double *pos,
+                             const double *coefs, int ncoef, double *F, int fstride, double *E, int estride,
+                             hipStream_t st)
+{
     PP_REQUIRE(model >= 0 && model < kModelCount, "unknown force-field model");
     PP_REQUIRE(ncoef >= 0 && ncoef <= 4, "at most 4 coefficients per atom");
     AccumArgs A;
@@ -495,6 +525,8 @@ int launch_ff_models(const GridGeom &g, int model, double par, int natoms, const
     A.natoms = natoms;
     A.ncoef = ncoef;
     A.model = model;
+    A.fstride = fstride;
+    A.estride = estride;
     A.par = par;
     for (int i = 0; i < 9; i++) A.dcell[i] = g.dcell[i];
     if (A.npts == 0 || (!F && !E)) return 0;

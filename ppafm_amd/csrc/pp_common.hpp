@@ -91,7 +91,8 @@ int launch_ff_build_batch(const ppafm_ff_model &m, int nitems, const BatchGridIt
 // atom: LJ (c6, c12); Coulomb (kqq); vdW (c6), parameter ADamp; vdW with a
 // damping function (R, E0), parameter ADamp; D3 (c6, c8, R0^6, R0^8); Morse
 // (R0, E0), parameter alpha; densities (amplitude, width); LJ in (R0, E0);
-// bare r^-6 (c6).  The three densities write energy only.
+// bare r^-6 (c6); the OpenCL vdW variants as their CPU counterparts.  The
+// three densities write energy only.
 enum FFModelF64 {
     kModelLJ = 0,
     kModelCoulS,
@@ -109,12 +110,20 @@ enum FFModelF64 {
     kModelR4,
     kModelLJRE,
     kModelInvR6,
+    kModelVdWNoDampOcl,
+    kModelVdWInvR4Ocl,
+    kModelVdWInvR8Ocl,
     kModelCount
 };
 // FP64 accumulate (+=) into the (npts,3) grid F and (npts) E, either may be
 // NULL; pos (natoms,3) relative to the grid origin, coefs (natoms,ncoef).
 int launch_ff_models(const GridGeom &g, int model, double par, int natoms, const double *pos, const double *coefs,
                      int ncoef, double *F, double *E, hipStream_t st);
+// The same with F and E `fstride` and `estride` doubles apart per point
+// (4 and 4 for the force and energy channels of a double4 grid).
+int launch_ff_models_strided(const GridGeom &g, int model, double par, int natoms, const double *pos,
+                             const double *coefs, int ncoef, double *F, int fstride, double *E, int estride,
+                             hipStream_t st);
 // LJ at npoints points pts (3 each) from atoms pos/cljs -> fes (npoints,4).
 int launch_lj_points(long npoints, const double *pts, int natoms, const double *pos, const double *cljs, double *fes,
                      hipStream_t st);

@@ -258,6 +258,56 @@ def loadCUBE(fname, xyz_order=False):
     return FF, lvec, nDim, head
 
 
+def loadXSFGeom(fname):
+    """Atoms and grid cell of an XSF file (io.py:229-268): ([Zs, xs, ys, zs,
+    qs], nDim, lvec) with the atoms listed under PRIMCOORD (charges 0), nDim
+    the grid shape without the repeated periodic planes and lvec the four
+    rows after it.  A file without PRIMCOORD is an error (ValueError or
+    IndexError, as in the reference)."""
+    e, x, y, z, q = [], [], [], [], []
+    with open(fname) as f:
+        for _ in range(10000):
+            if "PRIMCOORD" in f.readline():
+                break
+        n = int(f.readline().split()[0])
+        for _ in range(n):
+            ws = f.readline().split()
+            e.append(int(ws[0]))
+            x.append(float(ws[1]))
+            y.append(float(ws[2]))
+            z.append(float(ws[3]))
+            q.append(0)
+        for _ in range(10000):
+            line = f.readline()
+            if "BEGIN_DATAGRID_3D" in line or "DATAGRID_3D_DENSITY" in line:
+                break
+        ws = f.readline().split()
+        nDim = [int(ws[0]), int(ws[1]), int(ws[2])]
+        lvec = [[float(v) for v in f.readline().split()[:3]] for _ in range(4)]
+    return [e, x, y, z, q], (nDim[0] - 1, nDim[1] - 1, nDim[2] - 1), lvec
+
+
+def loadAtomsCUBE(fname):
+    """Atoms of a Gaussian cube file (io.py:291-320): [Zs, xs, ys, zs, qs] in
+    angstrom (bohr * 0.5291772109217), charges 0."""
+    b2a = 0.5291772109217
+    e, x, y, z, q = [], [], [], [], []
+    with open(fname) as f:
+        f.readline()
+        f.readline()
+        natoms = int(f.readline().split()[0])
+        for _ in range(3):
+            f.readline()
+        for _ in range(natoms):
+            ws = f.readline().split()
+            x.append(float(ws[2]) * b2a)
+            y.append(float(ws[3]) * b2a)
+            z.append(float(ws[4]) * b2a)
+            e.append(int(ws[0]))
+            q.append(0.0)
+    return [e, x, y, z, q]
+
+
 def saveNpy(fname, data, lvec, atomic_info):
     """Scalar grid, its cell and the source atoms as ``fname``.npz (io.py:716-726)."""
     np.savez(fname + ".npz", data=data, lvec=lvec, atoms=atomic_info[0], lvec0=atomic_info[1])

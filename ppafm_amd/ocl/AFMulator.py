@@ -9,10 +9,15 @@ stroke) and the df stack is convolveZ with get_simple_df_weight
 (relax.cl:571-608, AFMulator.py:226-229).  Interpolation is exact software
 trilinear, where the reference uses the hardware image filter.
 
-Not on this path (raise NotImplementedError): Hartree potentials and the
-FDBM method (``rho_sample`` / ``rho_delta``).  A tip density ``rho`` dict is
-kept for parameter I/O; like the reference, the point-charge method does not
-use it.
+The Hartree method (``qs`` a HartreePotential: LJ plus the potential
+cross-correlated with the tip density set by setRho, or with the tip charges
+Qs/QZs when no density is set, field.py:1446-1504 and 1168-1317) and the
+full-density-based method (``rho_sample`` with setRhoDelta,
+field.py:1506-1606) build their grids with the kernels of ppafm_amd.ocl.field
+(FP64 FFT cross-correlations on hipFFT).  Tip densities are built on first
+use, on the force-field grid of the time setRho / setRhoDelta was called.
+The DFT-D3 vdW of the density-based method needs the D3 tables, which the
+caller registers with ppafm_amd.ocl.field.set_d3_tables.
 """
 
 import os
@@ -22,6 +27,7 @@ import numpy as np
 
 from .. import common
 from .. import device as D
+from . import field as FFcl
 
 VALID_SIZES = np.array([16, 32, 64, 128, 192, 256, 384, 512, 768, 1024, 1536, 2048])
 
@@ -34,6 +40,98 @@ class _Scanner:
         self.stiffness = stiffness
         self.relax_params = relax_params
         self.zstep = 0.1
+
+
+class _ForceField:
+    """The tip-density state of ForceField_LJC (field.py:845-1000) that
+    AFMulator and the ML generators read and assign: rho / fft_corr (tip
+    density of the Hartree electrostatics and the Pauli term) and
+    rho_delta / fft_corr_delta (the density-based electrostatics).  A density
+    given to set_rho / set_rho_delta is built when first read."""
+
+    def __init__(self):
+        self._spec = {}
+        self._rho = self._fft_corr = self._rho_delta = self._fft_corr_delta = None
+        self.iZPP = None
+
+    def setPP(self, Z_pp):
+        self.iZPP = int(Z_pp)
+
+    def set_rho(self, rho, lvec, n, sigma, B):
+        self._rho = self._fft_corr = None
+        self._spec.pop("rho", None)
+        if rho is not None:
+            self._spec["rho"] = (rho, np.array(lvec), tuple(int(v) for v in n[:3]), sigma, B)
+
+    def set_rho_delta(self, rho_delta, lvec, n):
+        self._rho_delta = self._fft_corr_delta = None
+        self._spec.pop("rho_delta", None)
+        if rho_delta is not None:
+            self._spec["rho_delta"] = (rho_delta, np.array(lvec), tuple(int(v) for v in n[:3]), None, 1.0)
+
+    def _build(self, which):
+        spec = self._spec.pop(which, None)
+        if spec is None:
+            return
+        rho, lvec, n, sigma, B = spec
+        if isinstance(rho, dict):
+            rho = FFcl.MultipoleTipDensity(lvec[1:], n, sigma=sigma, multipole=rho)
+        if not np.allclose(B, 1.0):
+            rho = rho.power_positive(p=B, in_place=False)
+        # prepareBuffers (field.py:961-986): onto the force-field grid
+        if not (np.allclose(rho.lvec, lvec) and np.allclose(rho.shape, n)):
+            rho = rho.interp_at(lvec, n)
+        fft = FFcl.FFTCrossCorrelation(rho)
+        if which == "rho":
+            self._rho, self._fft_corr = rho, fft
+        else:
+            self._rho_delta, self._fft_corr_delta = rho, fft
+
+    def has_rho(self):
+        return self._rho is not None or "rho" in self._spec
+
+    def has_rho_delta(self):
+        return self._rho_delta is not None or "rho_delta" in self._spec
+
+    @property
+    def rho(self):
+        self._build("rho")
+        return self._rho
+
+    @rho.setter
+    def rho(self, v):
+        self._spec.pop("rho", None)
+        self._rho = v
+
+    @property
+    def fft_corr(self):
+        self._build("rho")
+        return self._fft_corr
+
+    @fft_corr.setter
+    def fft_corr(self, v):
+        self._spec.pop("rho", None)
+        self._fft_corr = v
+
+    @property
+    def rho_delta(self):
+        self._build("rho_delta")
+        return self._rho_delta
+
+    @rho_delta.setter
+    def rho_delta(self, v):
+        self._spec.pop("rho_delta", None)
+        self._rho_delta = v
+
+    @property
+    def fft_corr_delta(self):
+        self._build("rho_delta")
+        return self._fft_corr_delta
+
+    @fft_corr_delta.setter
+    def fft_corr_delta(self, v):
+        self._spec.pop("rho_delta", None)
+        self._fft_corr_delta = v
 
 
 class AFMulator:
@@ -85,6 +183,10 @@ class AFMulator:
         self._grid = None
         self._nDim_alloc = None
         self._out = None
+        self.forcefield = _ForceField()
+        self._rho = None
+        self.rho_delta = None
+        self._old_nDim = np.zeros(3, dtype=np.int64)
         self.setScanWindow(scan_window, scan_dim, df_steps)
         self.setLvec(lvec, pixPerAngstrome)
         self.setRho(rho, sigma, B_pauli)
@@ -116,7 +218,14 @@ class AFMulator:
             self.lvec = np.array(lvec, dtype=np.float64)
         else:
             self.lvec = get_lvec(self.scan_window, tipR0=self.tipR0, pixPerAngstrome=self.pixPerAngstrome)
+        if hasattr(self, "nDim"):
+            self._old_nDim = np.array(self.nDim[:3])
         self.nDim = common.genFFSampling(self.lvec, pixPerAngstrome=self.pixPerAngstrome)
+
+    def _lvec_ff(self):
+        """The force-field lattice as the device grid uses it (FP32 values,
+        like the reference's ForceField_LJC.lvec0 / lvec)."""
+        return self.lvec.astype(np.float32).astype(np.float64)
 
     def setScanWindow(self, scan_window=None, scan_dim=None, df_steps=None):
         if scan_window is not None:
@@ -142,22 +251,35 @@ class AFMulator:
         return self.scanner.relax_params
 
     def setRho(self, rho=None, sigma=0.71, B_pauli=1.0):
-        """Remember a multipole tip density dict (used by parameter I/O; the
-        point-charge force field does not use it, as in the reference)."""
-        if rho is not None and not isinstance(rho, dict):
-            raise NotImplementedError("grid tip densities are not on this path; pass a multipole dict or None")
-        self.sigma = sigma
-        self.B_pauli = B_pauli
-        self._rho = rho
-        self.rho = rho
+        """Set the tip charge density (AFMulator.py:243-284): a multipole dict
+        (MultipoleTipDensity on the force-field grid with width sigma), a
+        TipDensity, or None to remove it.  With B_pauli != 1 the density is
+        raised to that power (FDBM Pauli term)."""
+        if rho is not None:
+            if not isinstance(rho, (dict, FFcl.TipDensity)):
+                raise ValueError(f"rho should of type `TipDensity`, but got `{type(rho)}`")
+            self.sigma = sigma
+            self.B_pauli = B_pauli
+            self._rho = rho
+            self.forcefield.set_rho(rho, self._lvec_ff(), self.nDim, sigma, B_pauli)
+        else:
+            self._rho = None
+            self.forcefield.set_rho(None, None, None, None, None)
 
     def setBPauli(self, B_pauli=1.0):
-        self.B_pauli = B_pauli
+        """Set the Pauli repulsion exponent of the FDBM (AFMulator.py:286-288)."""
+        self.setRho(self._rho, sigma=self.sigma, B_pauli=B_pauli)
 
     def setRhoDelta(self, rho_delta=None):
+        """Set the tip electron delta density of the FDBM electrostatics
+        (AFMulator.py:290-308); None removes it."""
+        self.rho_delta = rho_delta
         if rho_delta is not None:
-            raise NotImplementedError("rho_delta (FDBM) is not on this path")
-        self.rho_delta = None
+            if not isinstance(rho_delta, FFcl.TipDensity):
+                raise ValueError(f"rho_delta should of type `TipDensity`, but got `{type(rho_delta)}`")
+            self.forcefield.set_rho_delta(rho_delta, self._lvec_ff(), self.nDim)
+        else:
+            self.forcefield.set_rho_delta(None, None, None)
 
     def setQs(self, Qs, QZs):
         if (len(Qs) != 4) or (len(QZs) != 4):
@@ -169,13 +291,35 @@ class AFMulator:
         self.scanner.stiffness = np.array(tipStiffness, dtype=np.float32) / -common.eVA_Nm
 
     # --------------------------------------------------------------- imaging
-    def _prepare_atoms(self, xyzs, Zs, qs, rho_sample=None, sample_lvec=None, rot=np.eye(3), rot_center=None, REAs=None):
-        """Atoms of one molecule as the grid kernel takes them: (xyzq relative
-        to the grid origin, cLJs, charged)."""
+    def _method(self, qs, rho_sample, sample_lvec):
+        """The force-field method of one call and its Hartree potential
+        (AFMulator.py:364-396): 'point-charge', 'hartree' or 'fdbm'."""
         if rho_sample is not None:
-            raise NotImplementedError("FDBM (rho_sample) is not on this path")
-        if qs is not None and not isinstance(qs, np.ndarray) and not isinstance(qs, (list, tuple)):
-            raise NotImplementedError("Hartree potentials are not on this path; pass point charges")
+            if not isinstance(rho_sample, FFcl.ElectronDensity):
+                raise ValueError(f"rho_sample should of type `ElectronDensity`, but got `{type(rho_sample)}`")
+            if not isinstance(qs, FFcl.HartreePotential):
+                raise ValueError(f"qs should be HartreePotential when rho_sample is not None, but got type `{type(qs)}`.")
+            if self.rho_delta is None:
+                raise ValueError("rho_delta should be set when rho_sample is not None.")
+        pot = None
+        if isinstance(qs, FFcl.HartreePotential):
+            pot = qs
+            self.sample_lvec = np.asarray(sample_lvec if sample_lvec is not None else pot.lvec[1:], dtype=np.float64)
+            assert self.sample_lvec.shape == (3, 3), f"sample_lvec has shape {self.sample_lvec.shape}, but should have shape (3, 3)"
+        elif isinstance(qs, FFcl.DataGrid):
+            raise ValueError(f"qs should be point charges or a HartreePotential, but got `{type(qs)}`")
+        elif sample_lvec is not None:
+            self.sample_lvec = np.asarray(sample_lvec, dtype=np.float64)
+        method = "fdbm" if rho_sample is not None else "hartree" if pot is not None else "point-charge"
+        if method != "fdbm" and not np.allclose(self.B_pauli, 1.0):
+            warnings.warn(f"Not using FDBM, but tip density exponent is {self.B_pauli}! This is probably not what you want to do!")
+        return method, pot
+
+    def _prepare_atoms(self, xyzs, Zs, qs, rho_sample=None, sample_lvec=None, rot=np.eye(3), rot_center=None, REAs=None, full=False):
+        """Atoms of one molecule as the grid kernel takes them: (xyzq relative
+        to the grid origin, cLJs, charged); with full also (REAs, Zs, rot_center)
+        after the periodic images.  qs: point charges or None (the callers
+        replace a Hartree potential by zeros, AFMulator.py:374-377)."""
         xyzs = np.asarray(xyzs, dtype=np.float64)
         qs = np.zeros(len(Zs)) if qs is None else np.asarray(qs, dtype=np.float64)
         if sample_lvec is not None:
@@ -197,6 +341,9 @@ class AFMulator:
         cLJs = np.asarray(cLJs, dtype=np.float32).astype(np.float64)
         lvec0 = self.lvec[0].astype(np.float32).astype(np.float64)
         xyzq = np.concatenate([xyzs - lvec0[None, :], qs[:, None]], axis=1)
+        if full:
+            REAs = np.asarray(REAs, dtype=np.float32).astype(np.float64)
+            return xyzq, cLJs, not np.allclose(qs, 0), REAs, np.asarray(Zs), rot_center
         return xyzq, cLJs, not np.allclose(qs, 0)
 
     def _ff_model(self, charged):
@@ -205,15 +352,53 @@ class AFMulator:
 
     def prepareFF(self, xyzs, Zs, qs, rho_sample=None, sample_lvec=None, rot=np.eye(3), rot_center=None, REAs=None):
         self.check_scan_window()
-        xyzq, cLJs, charged = self._prepare_atoms(xyzs, Zs, qs, rho_sample, sample_lvec, rot, rot_center, REAs)
-        self._model = self._ff_model(charged)
         n = tuple(int(v) for v in self.nDim[:3])
+        # the grid size changed: rebuild the tip densities (AFMulator.py:346-355)
+        if (np.array(self._old_nDim[:3]) != np.array(n)).any():
+            if self._rho is not None:
+                self.setRho(self._rho, self.sigma, self.B_pauli)
+                self.setRhoDelta(self.rho_delta)
+            self._old_nDim = np.array(n)
+        method, pot = self._method(qs, rho_sample, sample_lvec)
+        if pot is not None:
+            qs = None
+        xyzq, cLJs, charged, REAs, Zs_pbc, rot_center = self._prepare_atoms(xyzs, Zs, qs, rho_sample, sample_lvec, rot, rot_center, REAs, full=True)
         if self._grid is None or self._nDim_alloc != n:
             # double4 grid summed in FP64 (see ppafm_amd.scan.PlainScan for why)
             self._grid = D.DeviceArray(n + (4,), np.float64)
             self._nDim_alloc = n
-        cell = self.lvec[1:].astype(np.float32).astype(np.float64)
-        D.ff_build(n, cell, xyzq, cLJs, self._model, out=self._grid)
+        lvec_ff = self._lvec_ff()
+        cell = lvec_ff[1:]
+        if method == "point-charge":
+            self._model = self._ff_model(charged)
+            D.ff_build(n, cell, xyzq, cLJs, self._model, out=self._grid)
+        elif method == "hartree":
+            self._model = self._ff_model(False)
+            D.ff_build(n, cell, xyzq, cLJs, self._model, out=self._grid)
+            if not self.forcefield.has_rho():
+                if not np.allclose(rot, np.eye(3)):
+                    raise NotImplementedError("Force field calculation with rotation for Hartree potential with point charges tip density is not implemented.")
+                Qs = np.float32(self.Qs).astype(np.float64)
+                QZs = np.float32(self.QZs).astype(np.float64)
+                FFcl.calc_force_hartree_tip_charges(self._grid, lvec_ff, n, pot, Qs, QZs)
+            else:
+                FFcl.calc_force_hartree(self._grid, lvec_ff, n, pot, self.forcefield.fft_corr, rot=np.linalg.inv(rot), rot_center=rot_center)
+        else:
+            self.forcefield.setPP(self.iZPP)
+            if not self.forcefield.has_rho():
+                raise ValueError("the tip density (setRho) should be set for the FDBM")
+            FFcl.calc_force_fdbm(self._grid, lvec_ff, n, pot, rho_sample, self.forcefield.fft_corr, self.forcefield.fft_corr_delta,
+                                 A=self.A_pauli, B=self.B_pauli, rot=np.linalg.inv(rot), rot_center=rot_center)
+            xyz_rel = xyzq[:, :3]
+            if self.fdbm_vdw_type == "D3":
+                coeffs = FFcl.d3_coeffs(xyz_rel, Zs_pbc, self.iZPP, self.d3_params)
+                FFcl.add_d3(self._grid, n, cell, xyz_rel, coeffs)
+            elif self.fdbm_vdw_type == "LJ":
+                coefs = cLJs if self.lj_vdw_damp <= 0 else REAs[:, :2]
+                FFcl.add_vdw(self._grid, n, cell, xyz_rel, coefs, self.lj_vdw_damp)
+            else:
+                raise ValueError(f"Invalid vdw type `{self.fdbm_vdw_type}`")
+        self._method_last = method
 
     def batch(self, molecules, tuning=None):
         """Evaluate many molecules in one batch plan (one captured HIP graph).
@@ -238,6 +423,9 @@ class AFMulator:
         Takes the arguments of ``eval``; X (an output array) is not used."""
         if plot_to_dir:
             raise NotImplementedError("plotting is not on this path")
+        if rho_sample is not None or isinstance(qs, FFcl.DataGrid):
+            # Hartree / density-based grids: evaluated now, one call each
+            return dict(image=self.eval(xyzs, Zs, qs, rho_sample, sample_lvec, rot, rot_center, REAs).copy())
         self.check_scan_window()
         xyzq, cLJs, charged = self._prepare_atoms(xyzs, Zs, qs, rho_sample, sample_lvec, rot, rot_center, REAs)
         self.prepareScanner()
@@ -260,10 +448,13 @@ class AFMulator:
         from ..batch import BatchScan
 
         groups = {}
+        out = [None] * len(reqs)
         for i, r in enumerate(reqs):
+            if "image" in r:
+                out[i] = r["image"]
+                continue
             key = (r["Qs"], r["QZs"], bytes(r["tip"]), bytes(r["rp"]), r["dfw"].tobytes())
             groups.setdefault(key, []).append(i)
-        out = [None] * len(reqs)
         cache = self.__dict__.setdefault("_plans", {})
         for key, idx in groups.items():
             charged = any(reqs[i]["charged"] for i in idx)
