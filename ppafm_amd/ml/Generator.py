@@ -18,8 +18,10 @@ Differences that follow from deferring the evaluation:
     sample's values (changes to ``self.Xs`` in place are kept, as there);
     settings it changes take effect from the next batch, not the next
     sample.
-  * Only the point-charge simulation types ("lj", "lj+pc") are on this path;
-    "lj+hartree", "fdbm" and tip densities raise NotImplementedError.
+  * The Hartree and density-based types ("lj+hartree", "fdbm") run one
+    AFMulator call per sample and tip when the request is recorded, on the
+    GPU grids of ppafm_amd.ocl.field; each tip density keeps its own FFT
+    correlator.
 AuxMaps are not part of this package: ``aux_maps`` takes any callables with
 the reference's AuxMap call signatures.
 """
@@ -31,6 +33,7 @@ import numpy as np
 
 from .. import common as PPU
 from .. import io
+from ..ocl import field as FFcl
 
 
 def _xyzq(xyzs, qs):
@@ -193,18 +196,15 @@ class GeneratorAFMtrainer:
         self.aux_maps = aux_maps
         self.sample_generator = sample_generator
         self.sim_type = sim_type.lower()
-        if self.sim_type in ("lj+hartree", "fdbm"):
-            raise NotImplementedError(f"simulation type {sim_type!r}: only 'LJ' and 'LJ+PC' are on this path")
-        if self.sim_type not in ("lj", "lj+pc"):
+        if self.sim_type not in ("lj", "lj+pc", "lj+hartree", "fdbm"):
             raise ValueError(f"Unknown simulation type `{self.sim_type}`")
-        if rhos is not None or rho_deltas is not None:
-            raise NotImplementedError("tip densities (rhos, rho_deltas) are not on this path")
         self.batch_size = batch_size
         self.distAbove = distAbove
         self.distAboveActive = distAbove
         self.iZPPs = iZPPs
         self.ignore_elements = ignore_elements
-        self.density_cutoff = None
+        self.density_cutoff = density_cutoff if self.sim_type == "fdbm" else None
+        self._prepare_tip_buffers(rhos, rho_deltas, self.density_cutoff)
         if Qs is None or QZs is None:
             if self.sim_type == "lj+pc":
                 raise ValueError("Both Qs and QZs should be specified when using point-charge electrostatics.")
@@ -220,6 +220,34 @@ class GeneratorAFMtrainer:
         self.z_size = self.scan_dim[2] - self.df_steps + 1
         self.timings = {}
 
+    def _prepare_tip_buffers(self, rhos=None, rho_deltas=None, density_cutoff=None):
+        """One tip density (and FDBM delta density) with its FFT correlator
+        per tip (Generator.py:389-418).  Each tip keeps its own correlator
+        (the reference's shares one when the grids have the same shape)."""
+        if rhos is None:
+            self.rhos_tip = self.rhos_tip_delta = self.ffts_tip = self.ffts_tip_delta = [None] * len(self.iZPPs)
+            return
+        if len(rhos) != len(self.iZPPs):
+            raise ValueError(f"The length of rhos ({len(rhos)}) does not match the length of iZPPs ({len(self.iZPPs)})")
+        if rho_deltas is not None and len(rhos) != len(rho_deltas):
+            raise ValueError(f"The length of rhos ({len(rhos)}) does not match the length of rho_deltas ({len(rho_deltas)})")
+        self.rhos_tip, self.ffts_tip, self._rhos_original = [], [], []
+        for rho in rhos:
+            if density_cutoff is not None:
+                rho = rho.clamp(maximum=density_cutoff, in_place=False)
+            self.afmulator.setRho(rho, sigma=self.afmulator.sigma, B_pauli=self.afmulator.B_pauli)
+            self.rhos_tip.append(self.afmulator.forcefield.rho)
+            self.ffts_tip.append(self.afmulator.forcefield.fft_corr)
+            self._rhos_original.append(rho)
+        if rho_deltas is not None:
+            self.rhos_tip_delta, self.ffts_tip_delta = [], []
+            for rho_delta in rho_deltas:
+                self.afmulator.setRhoDelta(rho_delta)
+                self.rhos_tip_delta.append(self.afmulator.forcefield.rho_delta)
+                self.ffts_tip_delta.append(self.afmulator.forcefield.fft_corr_delta)
+        else:
+            self.rhos_tip_delta = self.ffts_tip_delta = [None] * len(self.iZPPs)
+
     def __iter__(self):
         self.sample_dict = {}
         self.sample_iterator = iter(self.sample_generator)
@@ -232,6 +260,9 @@ class GeneratorAFMtrainer:
         return int(np.floor(len(self.sample_generator) / self.batch_size))
 
     def __next__(self):
+        """Point-charge samples are recorded and evaluated after the loop in
+        batch plans; Hartree and density-based samples are evaluated when
+        recorded (one AFMulator call each, as in the reference)."""
         if self.iteration_done:
             raise StopIteration
         self.on_batch_start()
@@ -247,16 +278,23 @@ class GeneratorAFMtrainer:
             rot = self.sample_dict["rot"]
             xyzs = self.sample_dict["xyzs"]
             Zs = self.sample_dict["Zs"]
-            qs = self.sample_dict["qs"]
+            pot = self.sample_dict["qs"] if isinstance(self.sample_dict["qs"], FFcl.HartreePotential) else None
+            qs = np.zeros(len(Zs)) if pot is not None else self.sample_dict["qs"]
             centre = xyzs.mean(axis=0)
             self.xyzs_rot = np.dot(xyzs - centre, rot.T) + centre
             mols.append(np.concatenate([self.xyzs_rot, qs[:, None], Zs[:, None]], axis=1))
             self.handle_positions()
             self.on_sample_start()
             sws_, reqs_ = [], []
-            for iZPP, Q, Qz in zip(self.iZPPs, self.Qs, self.QZs):
+            for iZPP, rho_tip, rho_tip_delta, fft_tip, fft_tip_delta, Q, Qz in zip(
+                self.iZPPs, self.rhos_tip, self.rhos_tip_delta, self.ffts_tip, self.ffts_tip_delta, self.Qs, self.QZs
+            ):
                 self.afmulator.iZPP = iZPP
                 self.afmulator.setQs(Q, Qz)
+                self.afmulator.forcefield.rho = rho_tip
+                self.afmulator.forcefield.fft_corr = fft_tip
+                self.afmulator.forcefield.rho_delta = rho_tip_delta
+                self.afmulator.forcefield.fft_corr_delta = fft_tip_delta
                 self.sample_dict["REAs"] = PPU.getAtomsREA(self.afmulator.iZPP, self.sample_dict["Zs"], self.afmulator.typeParams,
                                                            alphaFac=-1.0)
                 self.handle_distance()
@@ -266,8 +304,8 @@ class GeneratorAFMtrainer:
                 kw = {k: v for k, v in self.sample_dict.items() if k != "rho_sample" or v is not None}
                 reqs_.append(self.afmulator.request(**kw))
                 sws_.append(np.array(self.scan_window))
-            xyzqs = _xyzq(self.sample_dict["xyzs"], self.sample_dict["qs"])
-            Ys.append([aux_map(xyzqs, self.sample_dict["Zs"], None, self.sample_dict["rot"]) for aux_map in self.aux_maps])
+            xyzqs = _xyzq(self.sample_dict["xyzs"], qs)
+            Ys.append([aux_map(xyzqs, self.sample_dict["Zs"], pot, self.sample_dict["rot"]) for aux_map in self.aux_maps])
             sws.append(sws_)
             reqs.append(reqs_)
         if not mols:
@@ -281,16 +319,28 @@ class GeneratorAFMtrainer:
         return Xs, np.array(Ys), mols, np.array(sws)
 
     def _load_next_sample(self):
+        """The next sample dict, checked for the simulation type
+        (Generator.py:536-565)."""
         d = next(self.sample_iterator)
         if self.sim_type == "lj":
             d["qs"] = np.zeros((len(d["Zs"]),), dtype=np.float32)
+            d["rho_sample"] = None
         elif ("qs" not in d) or (d["qs"] is None):
             raise ValueError("Sample dict does not contain qs, which is required for a simulation with electrostatics")
-        elif not isinstance(d["qs"], np.ndarray):
-            raise ValueError("qs in sample dict is not a numpy array, which is required for a LJ+PC simulation.")
-        d["rho_sample"] = None
+        elif self.sim_type == "lj+pc":
+            if not isinstance(d["qs"], np.ndarray):
+                raise ValueError("qs in sample dict is not a numpy array, which is required for a LJ+PC simulation.")
+            d["rho_sample"] = None
+        elif self.sim_type == "lj+hartree":
+            if not isinstance(d["qs"], FFcl.HartreePotential):
+                raise ValueError("qs in sample dict is not a HartreePotential, which is required for a LJ+Hartree simulation.")
+            d["rho_sample"] = None
+        elif ("rho_sample" not in d) or (d["rho_sample"] is None):
+            raise ValueError("Sample dict does not contain rho_sample, which is required for an FDBM simulation.")
         if "rot" not in d:
             d["rot"] = np.eye(3)
+        if self.density_cutoff is not None:
+            d["rho_sample"] = d["rho_sample"].clamp(maximum=self.density_cutoff, in_place=False)
         return d
 
     def handle_positions(self):
@@ -322,7 +372,17 @@ class GeneratorAFMtrainer:
         self.scan_window = ((sw[0][0], sw[0][1], z_min), (sw[1][0], sw[1][1], z_min + self.scan_size[2]))
 
     def set_fdbm_parameters(self, A_pauli, B_pauli):
-        """FDBM only; does nothing for the point-charge types (Generator.py:617-635)."""
+        """Pauli prefactor and exponent of an FDBM simulation: the tip
+        densities are raised to the new power (Generator.py:617-635).  Does
+        nothing for the other types."""
+        if self.sim_type != "fdbm":
+            return
+        self.afmulator.A_pauli = A_pauli
+        self.rhos_tip, self.ffts_tip = [], []
+        for rho in self._rhos_original:
+            self.afmulator.setRho(rho, sigma=self.afmulator.sigma, B_pauli=B_pauli)
+            self.rhos_tip.append(self.afmulator.forcefield.rho)
+            self.ffts_tip.append(self.afmulator.forcefield.fft_corr)
 
     def randomize_df_steps(self, minimum=4, maximum=20):
         """Random df steps with the output depth and z step kept (Generator.py:637-657)."""
