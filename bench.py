@@ -50,6 +50,7 @@ def parse():
     p.add_argument("--generator-molecules", type=int, default=64,
                    help="molecules of the generator comparison, per call vs batched (0 skips it)")
     p.add_argument("--c4-grid", choices=["f64", "f32"], default="f64", help="grid storage of the large tilted case")
+    p.add_argument("--no-hartree", action="store_true", help="skip the LJ+Hartree AFMulator leg")
     return p.parse_args()
 
 
@@ -402,6 +403,101 @@ def run_generator(args, batch=32):
     }
 
 
+def _plane_wave_potential(lv, n, seed=3):
+    """A smooth periodic synthetic Hartree potential (V): a few plane waves
+    of the sample cell."""
+    rng = np.random.default_rng(seed)
+    f = [np.arange(m) / m for m in n]
+    V = np.zeros(n)
+    for _ in range(6):
+        k = rng.integers(-3, 4, 3)
+        a, ph = rng.uniform(0.2, 1.0), rng.uniform(0, 2 * np.pi)
+        V += a * np.cos(2 * np.pi * (k[0] * f[0][:, None, None] + k[1] * f[1][None, :, None] + k[2] * f[2][None, None, :]) + ph)
+    return V
+
+
+def run_hartree(args, reps=3):
+    """AFMulator with the LJ+Hartree method at the reference's defaults
+    (force-field grid 256x256x192, 128x128x30 scan, dz2 tip density): one
+    molecule of the batched configuration, a synthetic Hartree potential on
+    a 24x24x20 A cell (200x200x166 points), rotated by 0.3 rad.  Reports
+    images/s of whole AFMulator calls and the stage times of the force
+    field: the Lennard-Jones grid, the potential interpolated onto the
+    force-field grid, the FFT cross-correlation with the tip density and
+    the gradient added to the grid."""
+    from ppafm_amd import device as D
+    from ppafm_amd import molgen as M
+    from ppafm_amd.ocl import field as FFcl
+    from ppafm_amd.ocl.AFMulator import AFMulator
+
+    xyz, Zs, qs = M.config_molecule(2, 7)
+    xyz = xyz + np.array([10.0, 10.0, 0.0])
+    lv = np.array([[-2.0, -2.0, -6.0], [24.0, 0, 0], [0, 24.0, 0], [0, 0, 20.0]])
+    pot = FFcl.HartreePotential(_plane_wave_potential(lv, (200, 200, 166)), lv)
+    a = 0.3
+    rot = np.array([[np.cos(a), -np.sin(a), 0], [np.sin(a), np.cos(a), 0], [0, 0, 1.0]])
+    afm = AFMulator(rho={"dz2": -0.1})
+    afm(xyz, Zs, pot, rot=rot)  # tip density, FFT plans, buffers
+    D.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        X = afm(xyz, Zs, pot, rot=rot)
+    t_eval = (time.perf_counter() - t0) / reps
+    # the stages of prepareFF, each bracketed by synchronisation
+    n = tuple(int(v) for v in afm.nDim[:3])
+    lvec_ff = afm._lvec_ff()
+    xyzq, cLJs, _, _, _, rc = afm._prepare_atoms(xyz, Zs, None, rot=rot, full=True)
+    stage = {}
+
+    def timed(name, fn):
+        # HIP events on the stream the stage's kernels run on (the default one)
+        D.synchronize()
+        e0, e1 = D.Event(), D.Event()
+        e0.record()
+        for _ in range(reps):
+            r = fn()
+        e1.record()
+        D.synchronize()
+        stage[name] = e0.elapsed_ms(e1) / reps
+        return r
+
+    ws = afm._ws
+    timed("lj_grid", lambda: D.ff_build(n, lvec_ff[1:], xyzq, cLJs, afm._ff_model(False), out=afm._grid))
+    pot_i = timed("interp", lambda: pot.interp_at(lvec_ff, n, array_out=ws.buf("a"), rot=np.linalg.inv(rot), rot_center=rc))
+    fft = afm.forcefield.fft_corr
+    E = timed("correlate", lambda: fft.correlate(pot_i, E=ws.grid("b", lvec_ff)))
+    timed("grad_add", lambda: E.grad(scale=[-1.0, -1.0, -1.0, 1.0], array_out=afm._grid, order="C", accumulate=True))
+    npts = int(np.prod(n))
+    out = {
+        "workload": f"AFMulator LJ+Hartree, dz2 tip density: grid {n}, scan {tuple(afm.scan_dim)}, df_steps {afm.df_steps}; "
+                    f"{len(Zs)}-atom molecule, synthetic potential 200x200x166, rotated 0.3 rad",
+        "images_per_s": 1.0 / t_eval,
+        "ms_per_image": t_eval * 1e3,
+        "stage_ms": stage,
+        "grid_points": npts,
+        "interp_gbs": 16 * npts / (stage["interp"] * 1e-3) / 1e9,
+        "grad_add_gbs": 72 * npts / (stage["grad_add"] * 1e-3) / 1e9,
+        "bytes_model": "interp: 8 B value in + 8 B out per point; grad_add: 8 B in + 32 B read + 32 B written per point (double4 grid)",
+        "image_finite": bool(np.isfinite(X).all()),
+    }
+    if not args.no_cpu:
+        _cpu_env()
+        import hartree_oracle as H
+
+        Eh = E.array
+        rho = afm.forcefield.rho.array
+        t = time.perf_counter()
+        Ec = H.correlate_fft(rho, pot_i.array, afm.forcefield.rho.cell_vol)
+        G = H.grad(Ec, np.diag(H.steps(lvec_ff, n)), (-1, -1, -1, 1))
+        t_cpu = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": 1.0 / t_cpu, "unit": "correlate+grad per s", "cores": 1, "kind": "port",
+                               "sample": f"numpy FFT cross-correlation and gradient of the same {n} grids (the oracle)"}
+        out["correlate_grad_speedup_vs_cpu"] = t_cpu / ((stage["correlate"] + stage["grad_add"]) * 1e-3)
+        out["correlate_max_rel_diff_vs_cpu"] = float(np.abs(Eh - Ec).max() / np.abs(Ec).max())
+        del G
+    return out
+
+
 def run_c4(args):
     """Large tilted case: per instance an Au(111) slab with a seeded
     60-atom molecule and 3x3 xy images (17,928 atoms), the OpenCL LJ +
@@ -676,6 +772,9 @@ def main():
     if args.c4_instances > 0:
         print(f"[bench] c4: {args.c4_instances} instances", file=sys.stderr, flush=True)
         out["c4"] = run_c4(args)
+    if not args.no_hartree:
+        print("[bench] hartree", file=sys.stderr, flush=True)
+        out["hartree"] = run_hartree(args)
     st = run_stiffness(g["scan"])
     stiff_bytes = 6 * 8 * 4 * corner_bytes + 12 + 96
     out["stiffness"] = {

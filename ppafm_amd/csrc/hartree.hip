@@ -42,6 +42,7 @@ struct Lat {
     double o[3];
     double T[9];
     int n[3];
+    double inv_n[3];
 };
 
 // The grid points of the output side of a kernel: origin and step vectors.
@@ -51,10 +52,16 @@ struct Pts {
     int n[3];
 };
 
-__device__ __forceinline__ long pmod(long x, long n)
+// f mod n for an integral double f (a floor): the quotient from one
+// multiply, one correction step, then a clamp that keeps any input (huge,
+// infinite or NaN coordinates) inside the grid.  No integer division.
+__device__ __forceinline__ int wrap_index(double f, int n, double inv_n)
 {
-    const long r = x % n;
-    return r < 0 ? r + n : r;
+    const double q = floor(f * inv_n);
+    int i = (int)(f - q * n);
+    if (i >= n) i -= n;
+    if (i < 0) i += n;
+    return min(max(i, 0), n - 1);
 }
 
 // fract() of the OpenCL kernels: x - floor(x), kept below 1.
@@ -65,7 +72,7 @@ __device__ __forceinline__ double fract_d(double x, double &fl)
 }
 
 struct Corners {
-    long i[8];
+    int i[8];
     double dx, dy, dz;
 };
 
@@ -82,18 +89,20 @@ __device__ __forceinline__ Corners corners(const Lat &L, double px, double py, d
     c.dx = fract_d(u, fu);
     c.dy = fract_d(v, fv);
     c.dz = fract_d(w, fw);
-    const long nx = L.n[0], ny = L.n[1], nz = L.n[2];
-    const long i0 = pmod((long)fu, nx), j0 = pmod((long)fv, ny), k0 = pmod((long)fw, nz);
-    const long i1 = (i0 + 1) % nx, j1 = (j0 + 1) % ny, k1 = (k0 + 1) % nz;
-    const long nyz = ny * nz;
-    c.i[0] = i0 * nyz + j0 * nz + k0;
-    c.i[1] = i0 * nyz + j0 * nz + k1;
-    c.i[2] = i0 * nyz + j1 * nz + k0;
-    c.i[3] = i0 * nyz + j1 * nz + k1;
-    c.i[4] = i1 * nyz + j0 * nz + k0;
-    c.i[5] = i1 * nyz + j0 * nz + k1;
-    c.i[6] = i1 * nyz + j1 * nz + k0;
-    c.i[7] = i1 * nyz + j1 * nz + k1;
+    const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
+    const int i0 = wrap_index(fu, nx, L.inv_n[0]), j0 = wrap_index(fv, ny, L.inv_n[1]), k0 = wrap_index(fw, nz, L.inv_n[2]);
+    const int i1 = i0 + 1 == nx ? 0 : i0 + 1, j1 = j0 + 1 == ny ? 0 : j0 + 1, k1 = k0 + 1 == nz ? 0 : k0 + 1;
+    // grids hold fewer than 2^31 points (checked on the host)
+    const int a0 = i0 * ny, a1 = i1 * ny;
+    const int r00 = (a0 + j0) * nz, r01 = (a0 + j1) * nz, r10 = (a1 + j0) * nz, r11 = (a1 + j1) * nz;
+    c.i[0] = r00 + k0;
+    c.i[1] = r00 + k1;
+    c.i[2] = r01 + k0;
+    c.i[3] = r01 + k1;
+    c.i[4] = r10 + k0;
+    c.i[5] = r10 + k1;
+    c.i[6] = r11 + k0;
+    c.i[7] = r11 + k1;
     return c;
 }
 
@@ -139,12 +148,16 @@ __device__ __forceinline__ double4 trilinear4(const double4 *__restrict__ buf, c
     return s;
 }
 
+// Grid indices of a C-order point number (32-bit unsigned arithmetic:
+// grids hold fewer than 2^31 points).
 __device__ __forceinline__ void split_c(long ind, const int n[3], int &i, int &j, int &k)
 {
-    k = (int)(ind % n[2]);
-    const long r = ind / n[2];
-    j = (int)(r % n[1]);
-    i = (int)(r / n[1]);
+    const unsigned u = (unsigned)ind;
+    const unsigned r = u / (unsigned)n[2];
+    k = (int)(u - r * (unsigned)n[2]);
+    const unsigned q = r / (unsigned)n[1];
+    j = (int)(r - q * (unsigned)n[1]);
+    i = (int)q;
 }
 
 // origin + i a' + j b' + k c', summed in that order.
@@ -456,7 +469,11 @@ __global__ __launch_bounds__(kBlock) void copy_kernel(const double *__restrict__
 
 // ----------------------------------------------------------------- helpers
 
-bool valid_n(const int n[3]) { return n && n[0] > 0 && n[1] > 0 && n[2] > 0; }
+// Positive sizes and fewer than 2^31 points (the kernels index in 32 bits).
+bool valid_n(const int n[3])
+{
+    return n && n[0] > 0 && n[1] > 0 && n[2] > 0 && (long long)n[0] * n[1] * n[2] < (1LL << 31);
+}
 
 unsigned blocks_for(long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
@@ -496,6 +513,7 @@ int make_lat(const double lvec[12], const int n[3], bool zero_origin, Lat &L)
     for (int c = 0; c < 3; c++) {
         L.o[c] = zero_origin ? 0.0 : lvec[c];
         L.n[c] = n[c];
+        L.inv_n[c] = 1.0 / n[c];
     }
     return 0;
 }
@@ -544,7 +562,7 @@ int ppafm_grid_interp_at(const double in_lvec[12], const int in_n[3], const doub
                          void *stream)
 {
     PP_REQUIRE(in_lvec && out_lvec && in_dev && out_dev && rot && rot_center, "null argument");
-    PP_REQUIRE(valid_n(in_n) && valid_n(out_n), "grid sizes must be positive");
+    PP_REQUIRE(valid_n(in_n) && valid_n(out_n), "grid sizes must be positive with fewer than 2^31 points");
     Lat L;
     Pts P;
     Rot R;
@@ -563,7 +581,7 @@ int ppafm_grid_interp_tip_at(const double in_lvec[12], const int in_n[3], const 
                              const double out_lvec[12], const int out_n[3], double *out_dev, void *stream)
 {
     PP_REQUIRE(in_lvec && out_lvec && in_dev && out_dev, "null argument");
-    PP_REQUIRE(valid_n(in_n) && valid_n(out_n), "grid sizes must be positive");
+    PP_REQUIRE(valid_n(in_n) && valid_n(out_n), "grid sizes must be positive with fewer than 2^31 points");
     Lat L;
     Pts P;
     Inv3 V;
@@ -581,7 +599,7 @@ int ppafm_grid_grad(const int n[3], const double step[3], const double *in_dev, 
                     int accumulate, double *out_dev, void *stream)
 {
     PP_REQUIRE(step && in_dev && scale && out_dev, "null argument");
-    PP_REQUIRE(valid_n(n), "grid sizes must be positive");
+    PP_REQUIRE(valid_n(n), "grid sizes must be positive with fewer than 2^31 points");
     GradArgs A;
     for (int i = 0; i < 3; i++) {
         A.n[i] = n[i];
@@ -601,7 +619,7 @@ int ppafm_grid_grad_potential(const double pot_lvec[12], const int pot_n[3], con
                               const double lvec[12], const int n[3], double h, double *field_dev, void *stream)
 {
     PP_REQUIRE(pot_lvec && pot_dev && lvec && field_dev, "null argument");
-    PP_REQUIRE(valid_n(pot_n) && valid_n(n), "grid sizes must be positive");
+    PP_REQUIRE(valid_n(pot_n) && valid_n(n), "grid sizes must be positive with fewer than 2^31 points");
     PP_REQUIRE(h > 0, "finite-difference step must be positive");
     Lat L;
     Pts P;
@@ -618,7 +636,7 @@ int ppafm_grid_add_tip_charges(const double lvec[12], const int n[3], const doub
                                const double QZs[4], double *grid4_dev, void *stream)
 {
     PP_REQUIRE(lvec && field_dev && Qs && QZs && grid4_dev, "null argument");
-    PP_REQUIRE(valid_n(n), "grid sizes must be positive");
+    PP_REQUIRE(valid_n(n), "grid sizes must be positive with fewer than 2^31 points");
     PP_REQUIRE(field_dev != grid4_dev, "the field and the output grid must be different arrays");
     Lat L;
     Pts P;
@@ -700,7 +718,7 @@ int ppafm_tip_density_multipole(const int n[3], const double *cx, const double *
                                 const double *coefs, const double norm[4], double *out_dev, void *stream)
 {
     PP_REQUIRE(cx && cy && cz && mat && norm && out_dev, "null argument");
-    PP_REQUIRE(valid_n(n), "grid sizes must be positive");
+    PP_REQUIRE(valid_n(n), "grid sizes must be positive with fewer than 2^31 points");
     PP_REQUIRE(nkinds <= kMaxKinds, "at most 16 multipole terms");
     PP_REQUIRE(nkinds <= 0 || (kinds && coefs), "null multipole arrays");
     PP_REQUIRE(sigma > 0, "sigma must be positive");
@@ -751,7 +769,7 @@ int ppafm_fft_corr_create(const int n[3], ppafm_fft_corr **out)
 {
     PP_REQUIRE(out, "null argument");
     *out = nullptr;
-    PP_REQUIRE(valid_n(n), "grid sizes must be positive");
+    PP_REQUIRE(valid_n(n), "grid sizes must be positive with fewer than 2^31 points");
     auto *c = new ppafm_fft_corr();
     for (int i = 0; i < 3; i++) c->n[i] = n[i];
     c->npts = grid_points(n);
@@ -833,7 +851,7 @@ int ppafm_grid_add_vdw(const ppafm_grid *grid, int damp_method, int natoms, cons
     PP_REQUIRE(grid && grid4_dev, "null argument");
     PP_REQUIRE(damp_method >= -1 && damp_method <= 4, "damp_method must be -1..4");
     PP_REQUIRE(natoms >= 0 && (natoms == 0 || (pos_dev && coefs_dev)), "null atom arrays");
-    PP_REQUIRE(valid_n(grid->n), "grid sizes must be positive");
+    PP_REQUIRE(valid_n(grid->n), "grid sizes must be positive with fewer than 2^31 points");
     // ADamp_* of FF.cl:17-21 (float literals)
     static const int models[6] = {kModelVdWNoDampOcl, kModelVdW, kModelVdWR2, kModelVdWR4, kModelVdWInvR4Ocl,
                                   kModelVdWInvR8Ocl};
@@ -853,7 +871,7 @@ int ppafm_grid_add_d3(const ppafm_grid *grid, int natoms, const double *pos_dev,
 {
     PP_REQUIRE(grid && grid4_dev, "null argument");
     PP_REQUIRE(natoms >= 0 && (natoms == 0 || (pos_dev && coefs_dev)), "null atom arrays");
-    PP_REQUIRE(valid_n(grid->n), "grid sizes must be positive");
+    PP_REQUIRE(valid_n(grid->n), "grid sizes must be positive with fewer than 2^31 points");
     if (natoms == 0) return 0;
     const GridGeom g = make_geom(grid->n, grid->cell);
     int e = launch_ff_models_strided(g, kModelD3, 0.0, natoms, pos_dev, coefs_dev, 4, grid4_dev, 4, grid4_dev + 3, 4,

@@ -367,6 +367,7 @@ class AFMulator:
             # double4 grid summed in FP64 (see ppafm_amd.scan.PlainScan for why)
             self._grid = D.DeviceArray(n + (4,), np.float64)
             self._nDim_alloc = n
+            self._ws = FFcl.Workspace(n)
         lvec_ff = self._lvec_ff()
         cell = lvec_ff[1:]
         if method == "point-charge":
@@ -380,15 +381,16 @@ class AFMulator:
                     raise NotImplementedError("Force field calculation with rotation for Hartree potential with point charges tip density is not implemented.")
                 Qs = np.float32(self.Qs).astype(np.float64)
                 QZs = np.float32(self.QZs).astype(np.float64)
-                FFcl.calc_force_hartree_tip_charges(self._grid, lvec_ff, n, pot, Qs, QZs)
+                FFcl.calc_force_hartree_tip_charges(self._grid, lvec_ff, n, pot, Qs, QZs, ws=self._ws)
             else:
-                FFcl.calc_force_hartree(self._grid, lvec_ff, n, pot, self.forcefield.fft_corr, rot=np.linalg.inv(rot), rot_center=rot_center)
+                FFcl.calc_force_hartree(self._grid, lvec_ff, n, pot, self.forcefield.fft_corr, rot=np.linalg.inv(rot), rot_center=rot_center,
+                                        ws=self._ws)
         else:
             self.forcefield.setPP(self.iZPP)
             if not self.forcefield.has_rho():
                 raise ValueError("the tip density (setRho) should be set for the FDBM")
             FFcl.calc_force_fdbm(self._grid, lvec_ff, n, pot, rho_sample, self.forcefield.fft_corr, self.forcefield.fft_corr_delta,
-                                 A=self.A_pauli, B=self.B_pauli, rot=np.linalg.inv(rot), rot_center=rot_center)
+                                 A=self.A_pauli, B=self.B_pauli, rot=np.linalg.inv(rot), rot_center=rot_center, ws=self._ws)
             xyz_rel = xyzq[:, :3]
             if self.fdbm_vdw_type == "D3":
                 coeffs = FFcl.d3_coeffs(xyz_rel, Zs_pbc, self.iZPP, self.d3_params)

@@ -420,14 +420,36 @@ def _check_rectangular(lvec_ff):
         )
 
 
-def calc_force_hartree(grid4, lvec_ff, n, pot, fft_corr, rot=np.eye(3), rot_center=np.zeros(3)):
+class Workspace:
+    """Device scratch grids of one force-field shape, allocated on first
+    use and reused by every later call (no counterpart in the reference,
+    which allocates per call)."""
+
+    def __init__(self, n):
+        self.n = tuple(int(v) for v in n[:3])
+        self._b = {}
+
+    def buf(self, key, comps=1):
+        shape = self.n + ((comps,) if comps > 1 else ())
+        b = self._b.get(key)
+        if b is None or b.shape != shape:
+            b = self._b[key] = D.DeviceArray(shape, np.float64)
+        return b
+
+    def grid(self, key, lvec, cls=None):
+        return (cls or DataGrid)(self.buf(key), lvec, shape=self.n)
+
+
+def calc_force_hartree(grid4, lvec_ff, n, pot, fft_corr, rot=np.eye(3), rot_center=np.zeros(3), ws=None):
     """ForceField_LJC.calc_force_hartree (field.py:1446-1504) on a double4
     force-field grid (n, 4) that already holds the Lennard-Jones force and
     energy: the potential interpolated onto the force-field grid (points
     rotated by rot about rot_center), cross-correlated with the tip
-    density, and its gradient (-dE/dx, -dE/dy, -dE/dz, E) added."""
+    density, and its gradient (-dE/dx, -dE/dy, -dE/dz, E) added.  ws: a
+    Workspace of shape n for the intermediate grid."""
     _check_rectangular(lvec_ff)
-    pot_i = pot.interp_at(lvec_ff, n, rot=rot, rot_center=rot_center)
+    ws = ws or Workspace(n)
+    pot_i = pot.interp_at(lvec_ff, n, array_out=ws.buf("a"), rot=rot, rot_center=rot_center)
     E = fft_corr.correlate(pot_i, E=pot_i)
     E.grad(scale=[-1.0, -1.0, -1.0, 1.0], array_out=grid4, order="C", accumulate=True)
     return grid4
@@ -444,14 +466,15 @@ def _matching_grid(pot, lvec_ff, n):
     )
 
 
-def calc_force_hartree_tip_charges(grid4, lvec_ff, n, pot, Qs, QZs, h=None):
+def calc_force_hartree_tip_charges(grid4, lvec_ff, n, pot, Qs, QZs, h=None, ws=None):
     """The Hartree method with point-charge tips: run_gradPotentialGrid
     (field.py:1225-1317) then the charge terms of run_evalLJC_Hartree
     (field.py:1168-1223) added to a double4 grid holding the Lennard-Jones
     force.  Returns the field grid (n, 4)."""
     h = h or DEFAULT_FD_STEP
     n = tuple(int(v) for v in n[:3])
-    field = D.DeviceArray(n + (4,), np.float64)
+    ws = ws or Workspace(n)
+    field = ws.buf("field", 4)
     if _matching_grid(pot, lvec_ff, n):
         pot.grad(scale=[-1.0, -1.0, -1.0, 1.0], array_out=field, order="C")
     else:
@@ -462,7 +485,8 @@ def calc_force_hartree_tip_charges(grid4, lvec_ff, n, pot, Qs, QZs, h=None):
     return field
 
 
-def calc_force_fdbm(grid4, lvec_ff, n, pot, rho_sample, fft_corr, fft_corr_delta, A=18.0, B=1.0, rot=np.eye(3), rot_center=np.zeros(3)):
+def calc_force_fdbm(grid4, lvec_ff, n, pot, rho_sample, fft_corr, fft_corr_delta, A=18.0, B=1.0, rot=np.eye(3), rot_center=np.zeros(3),
+                    ws=None):
     """The density part of ForceField_LJC.calc_force_fdbm (field.py:1506-1606)
     written to a double4 grid (n, 4): electrostatics of the potential with
     the tip delta density (scale -1), Pauli repulsion A * (sample density^B
@@ -470,15 +494,21 @@ def calc_force_fdbm(grid4, lvec_ff, n, pot, rho_sample, fft_corr, fft_corr_delta
     The vdW term is added by the caller (add_vdw / add_d3)."""
     _check_rectangular(lvec_ff)
     lvec_ff = np.asarray(lvec_ff)
+    ws = ws or Workspace(n)
     pot_same = np.allclose(lvec_ff, pot.lvec) and np.allclose(pot.shape, n[:3])
     rs_same = np.allclose(lvec_ff, rho_sample.lvec) and np.allclose(rho_sample.shape, n[:3])
-    pot_i = pot if pot_same else pot.interp_at(lvec_ff, n, rot=rot, rot_center=rot_center)
-    rs = rho_sample if rs_same else rho_sample.interp_at(lvec_ff, n, rot=rot, rot_center=rot_center)
-    E_es = fft_corr_delta.correlate(pot_i, scale=-1.0)
+    pot_i = pot if pot_same else pot.interp_at(lvec_ff, n, array_out=ws.buf("a"), rot=rot, rot_center=rot_center)
+    E = fft_corr_delta.correlate(pot_i, E=ws.grid("b", lvec_ff), scale=-1.0)
+    rs = rho_sample if rs_same else rho_sample.interp_at(lvec_ff, n, array_out=ws.buf("a"), rot=rot, rot_center=rot_center)
     if not np.allclose(B, 1.0):
-        rs = rs.power_positive(p=B, in_place=False)
-    E_pauli = fft_corr.correlate(rs)
-    E = E_es.add_mult(E_pauli, scale=A, in_place=True)
+        # power_positive(p=B, in_place=False) into scratch
+        scale = rs._get_normalization_factor()
+        assert scale > 0, "Normalizing scaling factor should be positive."
+        out = ws.grid("c", lvec_ff)
+        check(lib().ppafm_grid_power(rs.npts, rs.dev.ptr, out.dev.ptr, float(B), float(scale**B), None), "power_positive")
+        rs = out
+    E_pauli = fft_corr.correlate(rs, E=ws.grid("c", lvec_ff))  # may alias its input
+    E = E.add_mult(E_pauli, scale=A, in_place=True)
     E.grad(scale=[-1.0, -1.0, -1.0, 1.0], array_out=grid4, order="C")
     return grid4
 
