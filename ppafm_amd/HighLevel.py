@@ -85,11 +85,13 @@ def setFF(FF=None, computeVpot=False, n=None, lvec=None, parameters=None):
             raise ValueError("FF dimensions not set !!")
         FF = np.zeros((gridN[0], gridN[1], gridN[2], 3))
     core.setGridN(gridN)
+    scalar = False
     if len(FF.shape) == 4 and FF.shape[-1] == 3:
         core.setFF_Fpointer(FF)
     elif len(FF.shape) == 3 or FF.shape[-1] == 1:
         core.setFF_Epointer(FF)
         computeVpot = False
+        scalar = True
     else:
         raise ValueError("setFF: Array dimensions wrong for both vector and array field !!")
     if computeVpot:
@@ -97,7 +99,8 @@ def setFF(FF=None, computeVpot=False, n=None, lvec=None, parameters=None):
         core.setFF_Epointer(V)
     else:
         V = None
-        core.deleteFF_Epointer()
+        if not scalar:
+            core.deleteFF_Epointer()  # no stale energy grid from an earlier call
     if (lvec is None) and (parameters is not None):
         lvec = np.array([parameters.gridA, parameters.gridB, parameters.gridC], dtype=np.float64)
     lvec = np.array(lvec, dtype=np.float64)
@@ -208,3 +211,43 @@ def perform_relaxation(lvec, FFLJ, FFel=None, FFpauli=None, FFboltz=None, FFkpfm
         PPdisp = None
     core.deleteFF_Fpointer()
     return fzs, PPpos, PPdisp, lvecScan
+
+
+# --------------------------------------------------------- core densities
+
+
+def loadValenceElectronDict():
+    """Valence electrons per atomic number (HighLevel.py:414-432).  The
+    reference executes a local valelec_dict.py when one exists; this package
+    runs no file and returns its built-in table (common.VALENCE_ELECTRONS):
+    pass valElDict to override it."""
+    return dict(PPU.VALENCE_ELECTRONS)
+
+
+def _getAtomsWhichTouchPBCcell(Rs, elems, nDim, lvec, Rcut, bSaveDebug=False, fname=None):
+    """Periodic images of the atoms Rs (3, N) within Rcut of the cell
+    (HighLevel.py:435-441): (positions (M, 3), their elements)."""
+    inds, Rs_ = PPU.findPBCAtoms3D_cutoff(Rs, np.array(lvec[1:]), Rcut=Rcut)
+    elems = [elems[i] for i in inds]
+    return Rs_.transpose().copy(), elems
+
+
+def subtractCoreDensities(rho, lvec, elems=None, Rs=None, fname=None, valElDict=None, Rcore=0.7, bSaveDebugDens=False, bSaveDebugGeom=True,
+                          head=io.XSF_HEAD_DEFAULT, parameters=None):
+    """Subtract the valence charge of each atom, spread as the normalised
+    (1 - r^2/Rcore^2)^2 density, from the density grid rho (float64
+    (nx,ny,nz), in place) (HighLevel.py:452-478).  The projection runs on
+    the GPU (getDensityR4spline)."""
+    if fname is not None:
+        # getAtomsWhichTouchPBCcell (HighLevel.py:444-449) on an xyz geometry
+        atoms, nDim_g, lvec_g = io.loadGeometry(fname, parameters=parameters)
+        Rs, elems = _getAtomsWhichTouchPBCcell(np.array(atoms[1:4]), np.array(atoms[0]), nDim_g, lvec_g, Rcore)
+    if valElDict is None:
+        valElDict = loadValenceElectronDict()
+    cRAs = np.array([(-valElDict[elem], Rcore) for elem in elems], dtype=np.float64).reshape(-1, 2)
+    setFF(rho, lvec=lvec, parameters=parameters)
+    core.getDensityR4spline(shift_positions(Rs, -np.asarray(lvec)[0]), cRAs.copy())
+    if bSaveDebugDens:
+        io.saveXSFData("rho_subCoreChg.xsf", rho, lvec, head=head)
+    return rho
+

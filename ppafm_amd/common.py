@@ -168,6 +168,63 @@ def PBCAtoms3D_np(Zs, Rs, Qs, cLJs, REAs, lvec, npbc=(1, 1, 1)):
     return Zs_, xyzs_, qs_, cLJs_, REAs_
 
 
+def findPBCAtoms3D_cutoff(Rs, lvec, Rcut=1.0):
+    """Periodic images (cells -1, 0, 1 along each lattice vector) of the
+    atoms Rs (3, N) that lie in the cell lvec (3, 3) enlarged by Rcut on
+    every side (common.py:667-706): returns (indices of the source atoms,
+    image positions (3, M)), images in the order ia, ib, ic ascending."""
+    lvec = np.asarray(lvec, dtype=np.float64)
+    Rs = np.asarray(Rs, dtype=np.float64)
+    inv = np.linalg.inv(lvec)
+    a, b, c = np.dot(inv, Rs)
+    mA, mB, mC = (np.sqrt(np.dot(inv[k], inv[k])) * Rcut for k in range(3))
+    inds, out = [], []
+    for ia in (-1, 0, 1):
+        in_a = (a > (-mA - ia)) & (a < (mA + 1 - ia))
+        for ib in (-1, 0, 1):
+            in_ab = in_a & (b > (-mB - ib)) & (b < (mB + 1 - ib))
+            for ic in (-1, 0, 1):
+                sel = np.nonzero(in_ab & (c > (-mC - ic)) & (c < (mC + 1 - ic)))[0]
+                if len(sel) == 0:
+                    continue
+                shift = ia * lvec[0] + ib * lvec[1] + ic * lvec[2]
+                inds.append(sel)
+                out.append(Rs[:, sel] + shift[:, None])
+    if not inds:
+        return np.zeros(0, dtype=np.int64), np.zeros((3, 0))
+    return np.concatenate(inds), np.hstack(out)
+
+
+# Valence electrons per element for the core-density subtraction: the s and
+# p electrons of the outer shell for main-group elements, plus the d
+# electrons for the transition metals (groups 3-12); numbers depend on the
+# pseudopotentials of the DFT code, so callers may pass their own table.
+def _valence_table():
+    table = {}
+    period_starts = [1, 3, 11, 19, 37, 55, 87]
+    for z in range(1, 87):
+        p = max(k for k, z0 in enumerate(period_starts) if z0 <= z)
+        pos = z - period_starts[p] + 1  # position in the period
+        if p == 0:
+            table[z] = float(pos)  # H 1, He 2
+        elif p <= 2:
+            table[z] = float(pos)  # Li..Ne, Na..Ar: 1..8
+        elif p <= 4:
+            table[z] = float(pos if pos <= 12 else pos - 10)  # groups 1-12, then 3..8
+        else:
+            if pos <= 2:
+                table[z] = float(pos)
+            elif pos <= 17:
+                table[z] = 3.0  # La and the lanthanides
+            else:
+                g = pos - 14  # groups 4..18 after the lanthanides
+                table[z] = float(g if g <= 12 else g - 10)
+    return table
+
+
+VALENCE_ELECTRONS = _valence_table()
+
+
 def prepareScanGrids(parameters):
     """Tip apex axes and the scan lattice vectors (common.py:922-946)."""
     scanMin, scanMax, scanStep, r0Probe = parameters.scanMin, parameters.scanMax, parameters.scanStep, parameters.r0Probe

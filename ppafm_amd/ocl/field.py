@@ -12,8 +12,7 @@ Same names, arguments and errors as the reference, with these differences:
 device arrays are ``ppafm_amd.device.DeviceArray`` in FP64 (the reference
 keeps pyopencl FP32 buffers), so grids computed on the device come back as
 float64 host arrays; grids built from host arrays are rounded to float32 on
-construction exactly as the reference does.  ``TipDensity.subCores`` (core
-densities, HighLevel.subtractCoreDensities) is not provided.
+construction exactly as the reference does.
 """
 
 import os
@@ -289,7 +288,17 @@ class TipDensity(DataGrid):
     """Tip electron density. Units should be in e/A^3."""
 
     def subCores(self, xyzs, Zs, Rcore=0.7, valElDict=None):
-        raise NotImplementedError("core-density subtraction (HighLevel.subtractCoreDensities) is not part of this package")
+        """A new TipDensity with the core densities subtracted
+        (field.py:586-605): each atom of xyzs (N,3) / Zs, and its periodic
+        images within 1 A of the cell, removes its valence electrons
+        (valElDict, default HighLevel.loadValenceElectronDict) spread over
+        radius Rcore (HighLevel.subtractCoreDensities, on the GPU)."""
+        from .. import HighLevel
+
+        array = np.array(self.array, dtype=np.float64, order="C")
+        Rs, elems = HighLevel._getAtomsWhichTouchPBCcell(np.asarray(xyzs, dtype=np.float64).T, Zs, self.shape, self.lvec, 1.0, False)
+        HighLevel.subtractCoreDensities(array, self.lvec, elems=elems, Rs=Rs, valElDict=valElDict, Rcore=Rcore, bSaveDebugGeom=False)
+        return TipDensity(array, self.lvec, ctx=self.ctx)
 
     def interp_at(self, lvec_new, shape_new, array_out=None, local_size=(32,), queue=None):
         """Interpolate onto a new grid with the tip on the origin of both

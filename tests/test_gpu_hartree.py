@@ -388,3 +388,27 @@ def test_datagrid_files_round_trip(tmp_path):
     Fv = io.load_vec_field(str(tmp_path / "E"))[0]
     w = io.load_scal_field(str(tmp_path / "E_w"))[0]
     assert np.allclose(Fv, gr.array[..., :3], rtol=1e-5, atol=1e-6) and np.allclose(w, gr.array[..., 3], rtol=1e-5, atol=1e-6)
+
+
+def test_tip_density_subcores():
+    print("case: TipDensity.subCores, 48^3 grid of a 6 A cell, C and O inside, H near a face (periodic image)", flush=True)
+    lv = _lv([0, 0, 0], np.diag([6.0, 6.0, 6.0]))
+    n = (48, 48, 48)
+    base = _sample_grid(np.array([[3.0, 3.0, 3.5]]), [1.0], lv, n, 1.0, 0.5)
+    t = FFcl.TipDensity(base, lv)
+    xyzs = np.array([[3.0, 3.0, 3.0], [3.0, 3.0, 4.1], [0.3, 3.0, 3.0]])
+    Zs = np.array([6, 8, 1])
+    out = t.subCores(xyzs, Zs, Rcore=0.7)
+    assert isinstance(out, FFcl.TipDensity) and out.array.dtype == np.float32
+    # oracle: the normalised R4 density of every periodic image within 1 A of the cell
+    inds, R_img = C.findPBCAtoms3D_cutoff(xyzs.T, lv[1:], Rcut=1.0)
+    val = {1: 1.0, 6: 4.0, 8: 6.0}
+    amps = np.array([[-val[int(Zs[i])] / ((np.pi * 32) / 105 * 0.7**3), 0.7] for i in inds])
+    E = t.array.astype(np.float64).copy()
+    O.model_ff("r4", n, lv[1:], R_img.T - lv[0], amps, F=np.zeros(n + (3,)), E=E)
+    err = np.abs(out.array - E.astype(np.float32)).max() / np.abs(E).max()
+    dV = abs(np.linalg.det(lv[1:])) / np.prod(n)
+    lost = (out.array.astype(np.float64) - t.array.astype(np.float64)).sum() * dV
+    print(f"subCores max rel err vs oracle {err:.3g}; electrons removed {-lost:.6f} (valence 11); images {len(inds)}")
+    assert err < 1e-6 and len(inds) == 4
+    assert abs(lost + 11.0) < 11.0 * 1e-3

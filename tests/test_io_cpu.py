@@ -162,3 +162,19 @@ def test_reader_bit_equal_to_reference_readnums(tmp_path):
     ours = io.readNumsUpTo(fn, dims, nhead + 5)
     ref = O.reference_read_numbers(fn, dims, nhead + 5)
     assert np.array_equal(ours, ref)
+
+
+def test_pbc_images_within_cutoff_and_valence_table():
+    from ppafm_amd import common as C
+
+    cell = np.diag([6.0, 5.0, 4.0])
+    Rs = np.array([[3.0, 2.5, 2.0], [0.5, 2.5, 2.0], [0.2, 0.3, 3.9]]).T
+    inds, R = C.findPBCAtoms3D_cutoff(Rs, cell, Rcut=1.0)
+    # the centre atom once; the one near x = 0 also at x + 6; the corner one in 8 cells
+    assert sorted(inds.tolist()) == [0] + [1] * 2 + [2] * 8
+    for i, r in zip(inds, R.T):
+        d = r - Rs[:, i]
+        assert np.allclose(np.round(d / np.diag(cell)), d / np.diag(cell))
+        assert np.all(r > -1.0) and np.all(r < np.diag(cell) + 1.0)
+    V = C.VALENCE_ELECTRONS
+    assert [V[z] for z in (1, 2, 6, 7, 8, 9, 14, 17, 29, 30, 35, 47, 54, 79)] == [1, 2, 4, 5, 6, 7, 4, 7, 11, 12, 7, 11, 8, 11]
