@@ -412,3 +412,25 @@ def test_tip_density_subcores():
     print(f"subCores max rel err vs oracle {err:.3g}; electrons removed {-lost:.6f} (valence 11); images {len(inds)}")
     assert err < 1e-6 and len(inds) == 4
     assert abs(lost + 11.0) < 11.0 * 1e-3
+
+
+def test_afmulator_hartree_periodic_images_and_skew_grid():
+    print("case: AFMulator LJ+Hartree with the default npbc (1,1,0): atoms replicated in the potential's cell; skew FF grid rejected", flush=True)
+    xyz, Zs, qs = _mol()
+    pot, lv_pot = _hartree_pot(xyz, qs)
+    afm = _afm(rho={"dz2": -0.1})
+    assert tuple(afm.npbc) == (1, 1, 0)
+    X = afm(xyz, Zs, pot)
+    assert np.allclose(afm.sample_lvec, lv_pot[1:])
+    FE, lvec, _, _, _, _ = _oracle_lj(afm, xyz, Zs, pbc_lvec=lv_pot[1:])
+    n = tuple(int(v) for v in afm.nDim[:3])
+    pot_i = H.interp_at(pot.array.astype(np.float64), lv_pot, lvec, n)
+    tip = H.interp_tip_at(H.tip_density(lvec[1:], n, sigma=afm.sigma, multipole={"dz2": -0.1}), _lv([0, 0, 0], lvec[1:]), lvec, n)
+    E = H.correlate_fft(tip, pot_i, abs(np.linalg.det(H.steps(lvec, n))))
+    G_ref = FE + H.grad(E, np.diag(H.steps(lvec, n)), (-1, -1, -1, 1))
+    _check_afm(afm, G_ref, lvec, X, "LJ+Hartree (periodic images)")
+    skew = afm.lvec.copy()
+    skew[2, 0] = 0.5
+    afm2 = _afm(rho={"dz2": -0.1}, lvec=skew, npbc=(0, 0, 0))
+    with pytest.raises(NotImplementedError):
+        afm2(xyz, Zs, pot)
