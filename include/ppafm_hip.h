@@ -4,7 +4,7 @@
  * FIRE relaxation of the probe through the interpolated grid, and the z
  * convolution that turns the force stack into a frequency shift.
  *
- * Two groups of entry points.
+ * Four groups of entry points.
  *
  * (1) Reference-compatible host ABI.  Same names, argument lists and
  *     semantics as the ctypes bindings of ppafm/core.py:33-324 over
@@ -19,6 +19,12 @@
  *     is float4 or double4 (Fx,Fy,Fz,E) with z fastest; relaxation arithmetic
  *     is FP64.  These have no counterpart in the reference; they are what the
  *     Python mirror of ppafm.ocl.AFMulator and the batched path call.
+ *
+ * (3) Grid files: the XSF writer and number reader of ppafm/io.py and
+ *     GridUtils.cpp, native and multithreaded (host code).
+ *
+ * (4) Grids of the Hartree and full-density-based force fields (the
+ *     ppafm/ocl/field.py operations), device ABI conventions as in (2).
  */
 #ifndef PPAFM_HIP_H
 #define PPAFM_HIP_H
@@ -476,7 +482,8 @@ long long ppafm_last_itrsum(void);
  * A grid is n[0]*n[1]*n[2] float64 values, C order (z fastest), with a
  * lattice lvec[12] = 4 rows (origin, a, b, c); its step vectors are a/n[0],
  * b/n[1], c/n[2].  The reference keeps these grids in FP32; here every
- * value and every coordinate is FP64.
+ * value and every coordinate is FP64.  Grids hold fewer than 2^31 points;
+ * input and output grids of one call are different arrays unless stated. 
  * ---------------------------------------------------------------------- */
 
 /* DataGrid.interp_at (field.py:528-575), kernel interp_at (FF.cl:1010-1046):

@@ -562,6 +562,7 @@ int ppafm_grid_interp_at(const double in_lvec[12], const int in_n[3], const doub
                          void *stream)
 {
     PP_REQUIRE(in_lvec && out_lvec && in_dev && out_dev && rot && rot_center, "null argument");
+    PP_REQUIRE((const void *)in_dev != (const void *)out_dev, "input and output grids must be different arrays");
     PP_REQUIRE(valid_n(in_n) && valid_n(out_n), "grid sizes must be positive with fewer than 2^31 points");
     Lat L;
     Pts P;
@@ -581,6 +582,7 @@ int ppafm_grid_interp_tip_at(const double in_lvec[12], const int in_n[3], const 
                              const double out_lvec[12], const int out_n[3], double *out_dev, void *stream)
 {
     PP_REQUIRE(in_lvec && out_lvec && in_dev && out_dev, "null argument");
+    PP_REQUIRE((const void *)in_dev != (const void *)out_dev, "input and output grids must be different arrays");
     PP_REQUIRE(valid_n(in_n) && valid_n(out_n), "grid sizes must be positive with fewer than 2^31 points");
     Lat L;
     Pts P;
@@ -599,6 +601,7 @@ int ppafm_grid_grad(const int n[3], const double step[3], const double *in_dev, 
                     int accumulate, double *out_dev, void *stream)
 {
     PP_REQUIRE(step && in_dev && scale && out_dev, "null argument");
+    PP_REQUIRE((const void *)in_dev != (const void *)out_dev, "input and output grids must be different arrays");
     PP_REQUIRE(valid_n(n), "grid sizes must be positive with fewer than 2^31 points");
     GradArgs A;
     for (int i = 0; i < 3; i++) {
@@ -619,6 +622,7 @@ int ppafm_grid_grad_potential(const double pot_lvec[12], const int pot_n[3], con
                               const double lvec[12], const int n[3], double h, double *field_dev, void *stream)
 {
     PP_REQUIRE(pot_lvec && pot_dev && lvec && field_dev, "null argument");
+    PP_REQUIRE((const void *)pot_dev != (const void *)field_dev, "input and output grids must be different arrays");
     PP_REQUIRE(valid_n(pot_n) && valid_n(n), "grid sizes must be positive with fewer than 2^31 points");
     PP_REQUIRE(h > 0, "finite-difference step must be positive");
     Lat L;
