@@ -325,19 +325,30 @@ def run_learned(args, b):
     print(f"[bench] learned: {nsteps} training steps in {t_train:.1f} s, loss {losses[:5].mean():.3f} -> {losses[-5:].mean():.4f}",
           file=sys.stderr, flush=True)
     r = L.compare(b, net, truth=truth, items=held, reps=3, tune=True)
-    # stage times of the learned run: step-0 estimates, warm relaxation
-    b.set_warmstart(net)
+    # stage times of each setting with its tuned launch shape
     st = b.stream
     e0, e1 = D.Event(), D.Event()
-    stage = {}
-    for name, mask in (("grid", 1), ("estimates", 8), ("relax", 16), ("df", 4)):
-        b.run_stages(mask)
-        e0.record(st)
-        b.run_stages(mask)
-        e1.record(st)
-        b.synchronize()
-        stage[name] = e0.elapsed_ms(e1)
+
+    def stages(names):
+        out_ms = {}
+        for name, mask in names:
+            b.run_stages(mask)
+            e0.record(st)
+            b.run_stages(mask)
+            e1.record(st)
+            b.synchronize()
+            out_ms[name] = e0.elapsed_ms(e1)
+        return out_ms
+
+    b.set_warmstart(net)
+    b.set_tuning(r["learned"]["tuning"])
+    stage = stages((("grid", 1), ("estimates", 8), ("relax", 16), ("df", 4)))
+    b.set_warmstart(None)
+    b.set_tuning(r["newton"]["tuning"])
+    stage_newton = stages((("relax", 16),))
     b.set_plain()
+    b.set_tuning(r["plain"]["tuning"])
+    stage_plain = stages((("relax", 16),))
     npos = r["plain"]["positions"]
     ncols = sum(int(np.prod(b._scan_shape(k)[:2])) for k in range(nb))
     flop_col = 2 * (9 * 81 * 32 + 288 * 32 + 32 * 3)  # one network pass per column
@@ -355,6 +366,8 @@ def run_learned(args, b):
         if "estimate_vs_min" in q:
             out[k]["held_out_step0_estimate_vs_min_A"] = q["estimate_vs_min"]
     out["learned"]["stage_ms"] = stage
+    out["newton"]["stage_ms"] = stage_newton
+    out["plain"]["stage_ms"] = stage_plain
     out["learned"]["estimates_mfma"] = {"flop_per_column": flop_col, "columns": ncols,
                                         "achieved_tflops": flop_col * ncols / (stage["estimates"] * 1e-3) / 1e12,
                                         "peak_tflops": 2500.0,
@@ -362,6 +375,9 @@ def run_learned(args, b):
     out["speedup_learned_vs_plain"] = r["plain"]["ms"] / r["learned"]["ms"]
     out["speedup_newton_only_vs_plain"] = r["plain"]["ms"] / r["newton"]["ms"]
     out["speedup_learned_vs_newton_only"] = r["newton"]["ms"] / r["learned"]["ms"]
+    # the relaxation alone (the grid build and df are the same work in every setting)
+    out["speedup_learned_vs_plain_relaxation"] = stage_plain["relax"] / (stage["estimates"] + stage["relax"])
+    out["speedup_newton_only_vs_plain_relaxation"] = stage_plain["relax"] / stage_newton["relax"]
     return out
 
 

@@ -41,11 +41,12 @@ __device__ __forceinline__ void relax_column(const DevArgs &A, const T4 *__restr
     Fire S{tx + ox, ty + oy, tz + oz, 0, 0, 0, A.dtmax, A.a0};
     int it = 0, itr = 0;
     bool sample = false;  // mode 0: next gather is the output sample
+    CellCache<T4> cache;
 
     while (s < nz) {
         double fe[4], gx, gy, gz;
         to_grid(A, S.rx, S.ry, S.rz, gx, gy, gz);
-        gather(grid, A, gx, gy, gz, fe, MODE == 1);
+        gather_cached(grid, A, gx, gy, gz, fe, MODE == 1, cache);
         nevals++;
         bool done = false;
         const long o = c.col0 + s;

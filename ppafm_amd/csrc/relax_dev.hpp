@@ -108,6 +108,46 @@ __device__ __forceinline__ void corners(const T4 *__restrict__ g, const DevArgs 
     v[4] = g[c10 + iz]; v[5] = g[c10 + jz]; v[6] = g[c11 + iz]; v[7] = g[c11 + jz];
 }
 
+// The 8 corners of the cell a probe last sampled, kept in registers across
+// its FIRE iterations: while the probe stays in that cell (most iterations
+// of a converging step) the sample needs no memory access.  The values are
+// the grid's, so results equal those of gather().
+template <typename T4>
+struct CellCache {
+    int ix = -1, iy = -1, iz = -1;
+    T4 v[8];
+};
+
+// corners() through the cache: C.v holds the corners of the cell of
+// (gx, gy, gz) afterwards; (ux, uy, uz) is the position in it.
+template <typename T4>
+__device__ __forceinline__ void corners_cached(const T4 *__restrict__ g, const DevArgs &A, double gx, double gy,
+                                               double gz, CellCache<T4> &C, double &ux, double &uy, double &uz)
+{
+    int ix, jx, iy, jy, iz, jz;
+    axis(gx, A.n0, ix, jx, ux);
+    axis(gy, A.n1, iy, jy, uy);
+    axis(gz, A.n2, iz, jz, uz);
+    if (ix != C.ix || iy != C.iy || iz != C.iz) {
+        const unsigned c00 = ((unsigned)ix * A.n1 + iy) * A.n2, c01 = ((unsigned)ix * A.n1 + jy) * A.n2;
+        const unsigned c10 = ((unsigned)jx * A.n1 + iy) * A.n2, c11 = ((unsigned)jx * A.n1 + jy) * A.n2;
+        C.v[0] = g[c00 + iz]; C.v[1] = g[c00 + jz]; C.v[2] = g[c01 + iz]; C.v[3] = g[c01 + jz];
+        C.v[4] = g[c10 + iz]; C.v[5] = g[c10 + jz]; C.v[6] = g[c11 + iz]; C.v[7] = g[c11 + jz];
+        C.ix = ix;
+        C.iy = iy;
+        C.iz = iz;
+    }
+}
+
+template <typename T4>
+__device__ __forceinline__ void gather_cached(const T4 *__restrict__ g, const DevArgs &A, double gx, double gy,
+                                              double gz, double out[4], bool want_e, CellCache<T4> &C)
+{
+    double ux, uy, uz;
+    corners_cached(g, A, gx, gy, gz, C, ux, uy, uz);
+    interp8(C.v, ux, uy, uz, out, want_e);
+}
+
 template <typename T4>
 __device__ __forceinline__ void gather(const T4 *__restrict__ g, const DevArgs &A, double gx, double gy, double gz,
                                        double out[4], bool want_e)
