@@ -107,15 +107,37 @@ __device__ __forceinline__ Corners corners(const int n[3], d3 g)
     return c;
 }
 
+// The corner values of the cell a probe sampled last (key: its first corner
+// index), kept in registers across FIRE iterations: a probe that stays in
+// that cell samples without memory accesses, with the same values.
+template <class V>
+struct CornerCache {
+    long key = -1;
+    V v[8];
+};
+
+struct d3v {
+    double x, y, z;
+};
+
 // FP32 float4 grid (Fx, Fy, Fz, E): all 8 loads issued before any use.
 struct GridF32 {
+    using Cache = CornerCache<float4>;
     const float4 *__restrict__ g;
     __device__ __forceinline__ void sample(const int n[3], d3 gc, double out[4]) const
     {
+        Cache C;
+        sample(n, gc, out, C);
+    }
+    __device__ __forceinline__ void sample(const int n[3], d3 gc, double out[4], Cache &C) const
+    {
         const Corners c = corners(n, gc);
-        float4 v[8];
+        if (c.idx[0] != C.key) {
 #pragma unroll
-        for (int k = 0; k < 8; k++) v[k] = g[c.idx[k]];
+            for (int k = 0; k < 8; k++) C.v[k] = g[c.idx[k]];
+            C.key = c.idx[0];
+        }
+        const float4 *v = C.v;
         out[0] = v[0].x * c.w[0]; out[1] = v[0].y * c.w[0]; out[2] = v[0].z * c.w[0]; out[3] = v[0].w * c.w[0];
 #pragma unroll
         for (int k = 1; k < 8; k++) {
@@ -127,13 +149,22 @@ struct GridF32 {
 
 // FP64 double4 grid of the exact device path.
 struct GridD4 {
+    using Cache = CornerCache<double4>;
     const double4 *__restrict__ g;
     __device__ __forceinline__ void sample(const int n[3], d3 gc, double out[4]) const
     {
+        Cache C;
+        sample(n, gc, out, C);
+    }
+    __device__ __forceinline__ void sample(const int n[3], d3 gc, double out[4], Cache &C) const
+    {
         const Corners c = corners(n, gc);
-        double4 v[8];
+        if (c.idx[0] != C.key) {
 #pragma unroll
-        for (int k = 0; k < 8; k++) v[k] = g[c.idx[k]];
+            for (int k = 0; k < 8; k++) C.v[k] = g[c.idx[k]];
+            C.key = c.idx[0];
+        }
+        const double4 *v = C.v;
         out[0] = v[0].x * c.w[0]; out[1] = v[0].y * c.w[0]; out[2] = v[0].z * c.w[0]; out[3] = v[0].w * c.w[0];
 #pragma unroll
         for (int k = 1; k < 8; k++) {
@@ -145,19 +176,26 @@ struct GridD4 {
 
 // FP64 (nx,ny,nz,3) grid of the host ABI.
 struct GridF64 {
+    using Cache = CornerCache<d3v>;
     const double *__restrict__ g;
     __device__ __forceinline__ void sample(const int n[3], d3 gc, double out[4]) const
     {
+        Cache C;
+        sample(n, gc, out, C);
+    }
+    __device__ __forceinline__ void sample(const int n[3], d3 gc, double out[4], Cache &C) const
+    {
         const Corners c = corners(n, gc);
-        double v[8][3];
+        if (c.idx[0] != C.key) {
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
-            v[k][0] = g[3 * c.idx[k]]; v[k][1] = g[3 * c.idx[k] + 1]; v[k][2] = g[3 * c.idx[k] + 2];
+            for (int k = 0; k < 8; k++) C.v[k] = d3v{g[3 * c.idx[k]], g[3 * c.idx[k] + 1], g[3 * c.idx[k] + 2]};
+            C.key = c.idx[0];
         }
-        out[0] = v[0][0] * c.w[0]; out[1] = v[0][1] * c.w[0]; out[2] = v[0][2] * c.w[0]; out[3] = 0;
+        const d3v *v = C.v;
+        out[0] = v[0].x * c.w[0]; out[1] = v[0].y * c.w[0]; out[2] = v[0].z * c.w[0]; out[3] = 0;
 #pragma unroll
         for (int k = 1; k < 8; k++) {
-            out[0] += v[k][0] * c.w[k]; out[1] += v[k][1] * c.w[k]; out[2] += v[k][2] * c.w[k];
+            out[0] += v[k].x * c.w[k]; out[1] += v[k].y * c.w[k]; out[2] += v[k].z * c.w[k];
         }
     }
 };
@@ -217,11 +255,12 @@ __global__ __launch_bounds__(64) void relax_kernel(RelaxArgs A, Grid G)
     const d3 dinv0{A.dinv[0], A.dinv[1], A.dinv[2]}, dinv1{A.dinv[3], A.dinv[4], A.dinv[5]},
         dinv2{A.dinv[6], A.dinv[7], A.dinv[8]};
     const d3 RA{A.rot[0], A.rot[1], A.rot[2]}, RB{A.rot[3], A.rot[4], A.rot[5]}, RC{A.rot[6], A.rot[7], A.rot[8]};
+    typename Grid::Cache cache;
 
     while (s < nz) {
         const d3 gc{dot3(r, dinv0), dot3(r, dinv1), dot3(r, dinv2)};
         double fe[4];
-        G.sample(A.n, gc, fe);
+        G.sample(A.n, gc, fe, cache);
         nevals++;
         const long o = ((long)i * A.ny + j) * nz + s;
         bool step_done = false;
