@@ -746,6 +746,8 @@ def main():
             "bytes_model": f"8 corners x 3 x {corner_bytes} B per force evaluation + 24 B per relaxed position",
             "achieved_fp32_corner_units": (96 * g["n_eval"] + 24 * n_pos) / (g["t_relax"] * 1e-3) / 1e9,
             "fp32_corner_units_note": "96 B per force evaluation (8 corners x 3 FP32) + 24 B per position, as for an FP32 grid",
+            "note": "the unit counts every evaluation's 8 corners; the kernel keeps a probe's cell corners in registers across "
+                    "its FIRE iterations and reads most of the rest from L1/L2, so `traffic` (measured HBM bytes) is far below it",
         },
         "grid_build": {
             "bound": "valu",
