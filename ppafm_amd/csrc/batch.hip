@@ -138,8 +138,10 @@ void relax_tables(const ppafm_batch &b, int n, int tile_y, std::vector<BatchRela
 
 int enqueue_grid(ppafm_batch &b, int nitems, long long nbricks)
 {
+    bool ortho = true;
+    for (const BatchGridItem &g : b.gitems) ortho = ortho && ortho_cell(g.dcell);
     return launch_ff_build_batch(b.model, nitems, b.d_gitems, b.d_bprefix, nbricks, b.atoms, b.grids, b.f64,
-                                 b.any_uncharged, b.st);
+                                 b.any_uncharged, ortho, b.st);
 }
 
 int enqueue_relax(ppafm_batch &b, const ppafm_batch_tuning &t, const BatchRelaxItem *items, const long long *prefix,

@@ -281,11 +281,13 @@ __device__ __forceinline__ double rsq_nr(double x)
 
 // FP64 pair with reciprocal square roots (rsq_nr) in place of IEEE divide
 // and sqrt; the Coulomb point at zero z offset shares the LJ pair's rsq.
-template <bool LJ, int NS, bool Z0, bool WE, bool QS = false>
+// ORTHO (grid cells with orthogonal axes): dxy2 = dx^2 + dy^2 + R2SAFE is
+// the same for all z points of a lane and comes from the caller.
+template <bool LJ, int NS, bool Z0, bool WE, bool QS = false, bool ORTHO = false>
 __device__ __forceinline__ void pair_fp64_fast(const FFArgs &A, double dx, double dy, double dz, double q, double c6,
-                                               double c12, double aL[4], double aE[4])
+                                               double c12, double aL[4], double aE[4], double dxy2 = 0.0)
 {
-    const double r2 = __fma_rn(dx, dx, __fma_rn(dy, dy, __fma_rn(dz, dz, A.r2safe_d)));
+    const double r2 = ORTHO ? __fma_rn(dz, dz, dxy2) : __fma_rn(dx, dx, __fma_rn(dy, dy, __fma_rn(dz, dz, A.r2safe_d)));
     double ir0 = 0, ir20 = 0;
     if (LJ || Z0) {
         ir0 = rsq_nr(r2);
@@ -309,7 +311,7 @@ __device__ __forceinline__ void pair_fp64_fast(const FFArgs &A, double dx, doubl
             ir = ir0; ir2 = ir20; dzs = dz;
         } else {
             dzs = dz + (double)A.sdz[s_];
-            ir = rsq_nr(__fma_rn(dx, dx, __fma_rn(dy, dy, __fma_rn(dzs, dzs, A.r2safe_d))));
+            ir = rsq_nr(ORTHO ? __fma_rn(dzs, dzs, dxy2) : __fma_rn(dx, dx, __fma_rn(dy, dy, __fma_rn(dzs, dzs, A.r2safe_d))));
             ir2 = ir * ir;
         }
         const double e = ir * (q * (double)A.sw[s_]);
@@ -323,7 +325,7 @@ __device__ __forceinline__ void pair_fp64_fast(const FFArgs &A, double dx, doubl
 // callers that need the grid to FP64 accuracy (the FIRE stop rule turns
 // grid perturbations of ~1e-7 into stopping-point changes of >1e-3 A on
 // ill-conditioned strokes).  Same brick mapping as ff_build_f32.
-template <bool LJ, int NS, bool Z0, bool WE, bool QS = false>
+template <bool LJ, int NS, bool Z0, bool WE, bool QS = false, bool ORTHO = false>
 __device__ __forceinline__ void ff_brick_f64(const FFArgs &A, const AtomRec *__restrict__ atoms, double4 *__restrict__ out,
                                              int b, double4 *lpos, double2 *llj)
 {
@@ -368,13 +370,29 @@ __device__ __forceinline__ void ff_brick_f64(const FFArgs &A, const AtomRec *__r
             llj[j] = make_double2(a.c6, a.c12);
         }
         __syncthreads();
-        for (int j = 0; j < nt; j++) {
-            const double4 a = lpos[j];
-            const double2 c = llj[j];
+        if constexpr (ORTHO) {
+            // orthogonal cell axes: the lane's points share x and y, so one
+            // dx, dy and dx^2 + dy^2 per atom
+            for (int j = 0; j < nt; j++) {
+                const double4 a = lpos[j];
+                const double2 c = llj[j];
+                const double dx = px[0] - a.x, dy = py[0] - a.y;
+                const double dxy2 = __fma_rn(dx, dx, __fma_rn(dy, dy, A.r2safe_d));
 #pragma unroll
-            for (int k = 0; k < kK; k++) {
-                if (k >= nk) break;
-                pair_fp64_fast<LJ, NS, Z0, WE, QS>(A, px[k] - a.x, py[k] - a.y, pz[k] - a.z, a.w, c.x, c.y, aL[k], aE[k]);
+                for (int k = 0; k < kK; k++) {
+                    if (k >= nk) break;
+                    pair_fp64_fast<LJ, NS, Z0, WE, QS, true>(A, dx, dy, pz[k] - a.z, a.w, c.x, c.y, aL[k], aE[k], dxy2);
+                }
+            }
+        } else {
+            for (int j = 0; j < nt; j++) {
+                const double4 a = lpos[j];
+                const double2 c = llj[j];
+#pragma unroll
+                for (int k = 0; k < kK; k++) {
+                    if (k >= nk) break;
+                    pair_fp64_fast<LJ, NS, Z0, WE, QS>(A, px[k] - a.x, py[k] - a.y, pz[k] - a.z, a.w, c.x, c.y, aL[k], aE[k]);
+                }
             }
         }
     }
@@ -411,13 +429,13 @@ __global__ __launch_bounds__(kThreads) void ff_build_f32(FFArgs A, const AtomRec
     ff_brick_f32<LJ, NS, Z0, WE>(A, atoms, out, blockIdx.x, lds, ldsd);
 }
 
-template <bool LJ, int NS, bool Z0, bool WE>
+template <bool LJ, int NS, bool Z0, bool WE, bool ORTHO>
 __global__ __launch_bounds__(kThreads) void ff_build_f64(FFArgs A, const AtomRec *__restrict__ atoms,
                                                          double4 *__restrict__ out)
 {
     __shared__ double4 lpos[kTile];  // brick-local x, y, z, q
     __shared__ double2 llj[kTile];   // c6, c12
-    ff_brick_f64<LJ, NS, Z0, WE>(A, atoms, out, blockIdx.x, lpos, llj);
+    ff_brick_f64<LJ, NS, Z0, WE, false, ORTHO>(A, atoms, out, blockIdx.x, lpos, llj);
 }
 
 // ---- batches: one launch over the bricks of every molecule.  Block b finds
@@ -459,7 +477,7 @@ __global__ __launch_bounds__(kThreads) void ff_build_f32_batch(FFArgs A0, const 
     ff_brick_f32<LJ, NS, Z0, WE, QS>(A, atoms + it.atom_off, grid + it.grid_off, (int)(bidx - prefix[m]), lds, ldsd);
 }
 
-template <bool LJ, int NS, bool Z0, bool WE, bool QS>
+template <bool LJ, int NS, bool Z0, bool WE, bool QS, bool ORTHO>
 __global__ __launch_bounds__(kThreads) void ff_build_f64_batch(FFArgs A0, const BatchGridItem *__restrict__ items,
                                                                const long long *__restrict__ prefix, int nitems,
                                                                const AtomRec *__restrict__ atoms, double4 *__restrict__ grid)
@@ -470,7 +488,7 @@ __global__ __launch_bounds__(kThreads) void ff_build_f64_batch(FFArgs A0, const 
     const int m = find_item(prefix, nitems, bidx);
     const BatchGridItem it = items[m];
     const FFArgs A = batch_args(A0, it);
-    ff_brick_f64<LJ, NS, Z0, WE, QS>(A, atoms + it.atom_off, grid + it.grid_off, (int)(bidx - prefix[m]), lpos, llj);
+    ff_brick_f64<LJ, NS, Z0, WE, QS, ORTHO>(A, atoms + it.atom_off, grid + it.grid_off, (int)(bidx - prefix[m]), lpos, llj);
 }
 
 // One launch: a single grid (batch == nullptr) or every grid of a batch.
@@ -480,15 +498,20 @@ struct BatchLaunch {
     int nitems;
     long long nbricks;
     bool qskip;  // some atoms are uncharged: skip their Coulomb terms
+    bool ortho;  // every grid has orthogonal cell axes
 };
+
 
 template <bool LJ, int NS, bool Z0, bool WE, bool QS>
 void launch_batch_t(const FFArgs &A, const AtomRec *atoms, void *out, bool f64, const BatchLaunch *batch, hipStream_t st)
 {
     dim3 block(kThreads), grid((unsigned)batch->nbricks);
-    if (f64)
-        hipLaunchKernelGGL((ff_build_f64_batch<LJ, NS, Z0, WE, QS>), grid, block, 0, st, A, batch->items, batch->prefix,
-                           batch->nitems, atoms, (double4 *)out);
+    if (f64 && batch->ortho)
+        hipLaunchKernelGGL((ff_build_f64_batch<LJ, NS, Z0, WE, QS, true>), grid, block, 0, st, A, batch->items,
+                           batch->prefix, batch->nitems, atoms, (double4 *)out);
+    else if (f64)
+        hipLaunchKernelGGL((ff_build_f64_batch<LJ, NS, Z0, WE, QS, false>), grid, block, 0, st, A, batch->items,
+                           batch->prefix, batch->nitems, atoms, (double4 *)out);
     else
         hipLaunchKernelGGL((ff_build_f32_batch<LJ, NS, Z0, WE, QS>), grid, block, 0, st, A, batch->items, batch->prefix,
                            batch->nitems, atoms, (float4 *)out);
@@ -504,8 +527,10 @@ void launch_t(const FFArgs &A, const AtomRec *atoms, void *out, bool f64, const 
         return;
     }
     dim3 grid((unsigned)A.nbx * A.nby * A.nbz);
-    if (f64)
-        hipLaunchKernelGGL((ff_build_f64<LJ, NS, Z0, WE>), grid, block, 0, st, A, atoms, (double4 *)out);
+    if (f64 && ortho_cell(A.dcell))
+        hipLaunchKernelGGL((ff_build_f64<LJ, NS, Z0, WE, true>), grid, block, 0, st, A, atoms, (double4 *)out);
+    else if (f64)
+        hipLaunchKernelGGL((ff_build_f64<LJ, NS, Z0, WE, false>), grid, block, 0, st, A, atoms, (double4 *)out);
     else
         hipLaunchKernelGGL((ff_build_f32<LJ, NS, Z0, WE>), grid, block, 0, st, A, atoms, (float4 *)out);
 }
@@ -602,14 +627,14 @@ long long ff_bricks(const int n[3])
 
 int launch_ff_build_batch(const ppafm_ff_model &m, int nitems, const BatchGridItem *items_dev,
                           const long long *prefix_dev, long long nbricks, const AtomRec *atoms_dev, void *grids,
-                          bool f64, bool qskip, hipStream_t st)
+                          bool f64, bool qskip, bool ortho, hipStream_t st)
 {
     PP_REQUIRE(m.n_stencil >= 0 && m.n_stencil <= 4, "n_stencil must be 0..4");
     PP_REQUIRE(nbricks >= 0 && nbricks < (1LL << 31), "batch too large for one launch");
     if (nbricks == 0) return 0;
     FFArgs A = {};
     const bool z0 = model_args(m, A);
-    BatchLaunch b{items_dev, prefix_dev, nitems, nbricks, qskip};
+    BatchLaunch b{items_dev, prefix_dev, nitems, nbricks, qskip, ortho};
     return dispatch(m, A, z0, atoms_dev, grids, f64, &b, st);
 }
 

@@ -43,6 +43,13 @@ struct GridGeom {
 
 GridGeom make_geom(const int n[3], const double cell[9]);
 
+// Step vectors (rows of d) along the axes only: the FP64 grid kernels then
+// share dx, dy between the z points of a lane.
+inline bool ortho_cell(const double d[9])
+{
+    return d[1] == 0 && d[2] == 0 && d[3] == 0 && d[5] == 0 && d[6] == 0 && d[7] == 0;
+}
+
 // Atom record for the FP32 grid kernel: coordinates relative to the brick
 // centre are formed in FP64 on the fly and split hi + lo, so that
 // d = (p - hi) - lo keeps FP32 relative accuracy in the pair distance.
@@ -86,7 +93,7 @@ long long ff_bricks(const int n[3]);
 // Every grid of a batch in one launch; prefix[k] = bricks of items < k.
 int launch_ff_build_batch(const ppafm_ff_model &m, int nitems, const BatchGridItem *items_dev,
                           const long long *prefix_dev, long long nbricks, const AtomRec *atoms_dev, void *grids,
-                          bool f64, bool qskip, hipStream_t st);
+                          bool f64, bool qskip, bool ortho, hipStream_t st);
 // Pair models of the reference C ABI (ff_models.hip).  Coefficients per
 // atom: LJ (c6, c12); Coulomb (kqq); vdW (c6), parameter ADamp; vdW with a
 // damping function (R, E0), parameter ADamp; D3 (c6, c8, R0^6, R0^8); Morse
