@@ -213,6 +213,7 @@ def run_batch(args):
     b.synchronize()
     t_step = e0.elapsed_ms(e1) / steps
     nev, npos = b.counts()
+    lane_use = b.lane_use()
     # stages one by one (same kernels, launched without the graph)
     stage = {}
     for name, mask in (("grid", 1), ("relax", 2), ("df", 4)):
@@ -223,7 +224,7 @@ def run_batch(args):
         stage[name] = e0.elapsed_ms(e1)
     natoms = np.array([len(m[1]) for m in mols])
     npts = int(np.prod(cfg["n"]))
-    res = dict(t_step=t_step, stage=stage, n_eval=nev, n_pos=npos, natoms_total=int(natoms.sum()), npts=npts * nb,
+    res = dict(t_step=t_step, stage=stage, n_eval=nev, n_pos=npos, lane_use=lane_use, natoms_total=int(natoms.sum()), npts=npts * nb,
                tuning=b.tuning, nitems=nb)
     if args.learned_samples > 0:
         res["learned"] = run_learned(args, b)
@@ -360,7 +361,7 @@ def run_learned(args, b):
         q = r[k]
         out[k] = {"ms_per_step": q["ms"], "positions_per_s": npos / (q["ms"] * 1e-3), "tuning": q["tuning"],
                   "force_evals_per_position": q["evals"] / npos, "fallback_steps": q["fallbacks"],
-                  "held_out_vs_min_A": q["vs_min"]}
+                  "lane_use": q["lane_use"], "held_out_vs_min_A": q["vs_min"]}
         if "vs_plain" in q:
             out[k]["held_out_vs_plain_A"] = q["vs_plain"]
         if "estimate_vs_min" in q:
@@ -772,6 +773,7 @@ def main():
             "relax_positions_per_s": bt["n_pos"] / (bt["stage"]["relax"] * 1e-3),
             "stage_ms": bt["stage"],
             "force_evals": bt["n_eval"],
+            "relax_lane_use": bt["lane_use"],
             "grid_build_tflops": 32 * pairs / (bt["stage"]["grid"] * 1e-3) / 1e12,
             "relax_algorithmic_gbs": (8 * 3 * corner_bytes * bt["n_eval"] + 24 * bt["n_pos"]) / (bt["stage"]["relax"] * 1e-3) / 1e9,
             "tuning": bt["tuning"],

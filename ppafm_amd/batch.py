@@ -315,6 +315,14 @@ class BatchScan:
         check(lib().ppafm_memcpy_d2h(c.ctypes.data, lib().ppafm_batch_buffer(self._h, BUF_COUNTERS), 24, self.stream), "download")
         return int(c[0]), int(c[1]), int(c[2])
 
+    def lane_use(self):
+        """Force evaluations of the last relaxation over its lane slots (64 per
+        loop pass of each wave): the fraction of SIMD lanes that did work while
+        a wave waited for its slowest column."""
+        c = np.empty(4, np.uint64)
+        check(lib().ppafm_memcpy_d2h(c.ctypes.data, lib().ppafm_batch_buffer(self._h, BUF_COUNTERS), 32, self.stream), "download")
+        return float(c[0]) / float(c[3]) if c[3] else float("nan")
+
     # ------------------------------------------------------ learned warm start
     def set_warmstart(self, net=None, enable=True, net_passes=1, newton_max=8, newton_max0=None, newton_retry=8,
                       newton_tol=1e-3, newton_cap=0.1):

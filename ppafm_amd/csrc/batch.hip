@@ -78,7 +78,7 @@ struct ppafm_batch {
     float *df = nullptr;
     AtomRec *atoms = nullptr;
     long long atom_cap = 0;  // atom records the atoms buffer holds
-    unsigned *ctl = nullptr;  // 8 queue heads (u32) then 2 counters (u64)
+    unsigned *ctl = nullptr;  // 8 queue heads (u32) then 4 counters (u64)
     BatchGridItem *d_gitems = nullptr;
     long long *d_bprefix = nullptr;
     BatchRelaxItem *d_ritems = nullptr;
@@ -224,7 +224,7 @@ int enqueue_stages(ppafm_batch &b, int stages)
 {
     int e = 0;
     if (stages & (2 | 16)) {
-        e = launch_fill_u32(b.ctl, 14, 0u, b.st);
+        e = launch_fill_u32(b.ctl, 16, 0u, b.st);
         if (e) return e;
     }
     if (stages & 1) {

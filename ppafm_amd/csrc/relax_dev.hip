@@ -226,10 +226,13 @@ __device__ __forceinline__ void relax_column_warm(const DevArgs &A, const T4 *__
     }
 }
 
-// Wave sum of the per-lane counters, one atomic per wave.
+// Wave sum of the per-lane counters, one atomic per wave.  `passes` (the
+// same in every lane) is the wave's loop passes; 64 lane slots each go to
+// counters[3] when with_slots.
 __device__ __forceinline__ void flush_counters(unsigned long long *counters, unsigned long long nevals,
                                                unsigned long long npos, int lane, unsigned long long nfb = 0,
-                                               bool with_fb = false)
+                                               bool with_fb = false, unsigned long long passes = 0,
+                                               bool with_slots = false)
 {
     if (!counters) return;
     for (int off = 32; off > 0; off >>= 1) {
@@ -241,6 +244,7 @@ __device__ __forceinline__ void flush_counters(unsigned long long *counters, uns
         atomicAdd(&counters[0], nevals);
         atomicAdd(&counters[1], npos);
         if (with_fb) atomicAdd(&counters[2], nfb);
+        if (with_slots) atomicAdd(&counters[3], 64ull * passes);
     }
 }
 
@@ -297,7 +301,7 @@ __global__ __launch_bounds__(256) void relax_batch_kernel(DevArgs A0, BatchArgs 
     const int lane = threadIdx.x & 63;
     const int xcd = blockIdx.x & 7;
     const long long per = (B.ntiles + 7) / 8;
-    unsigned long long nevals = 0, npos = 0, nfb = 0;
+    unsigned long long nevals = 0, npos = 0, nfb = 0, passes = 0;
     for (int q = 0; q < 8; q++) {
         const int part = (xcd + q) & 7;
         const long long lo = part * per;
@@ -327,11 +331,16 @@ __global__ __launch_bounds__(256) void relax_batch_kernel(DevArgs A0, BatchArgs 
             const int lt = (int)(tile - prefix[m]);
             const int ti = lt / it.tiles_y, tj = lt % it.tiles_y;
             const int i = ti * TX + lane / TY, j = tj * TY + lane % TY;
+            const unsigned long long before = nevals;
             if (WARM) relax_column_warm<T4, MODE>(A, grids + it.grid_off, i, j, i < A.nx && j < A.ny, nevals, npos, nfb);
             else relax_column<T4, MODE>(A, grids + it.grid_off, i, j, i < A.nx && j < A.ny, nevals, npos);
+            // the tile's loop passes: every pass evaluates once in each lane still running
+            unsigned long long d = nevals - before;
+            for (int off = 32; off > 0; off >>= 1) d = max(d, (unsigned long long)__shfl_xor(d, off, 64));
+            passes += d;
         }
     }
-    flush_counters(A0.counters, nevals, npos, lane, nfb, WARM);
+    flush_counters(A0.counters, nevals, npos, lane, nfb, WARM, passes, true);
 }
 
 // ---------------------------------------------------------------------------

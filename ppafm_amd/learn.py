@@ -140,9 +140,10 @@ def evaluate(batch, truth, net=None, enable=True, items=None, reps=0, tune=False
     batch.run()
     batch.synchronize()
     ev, npos, nfb = batch.counters()
+    lane_use = batch.lane_use()
     r = positions_of(batch, items)
     rt = _rows(batch, truth.ptr, items)
-    out = dict(evals=ev, positions=npos, fallbacks=nfb, vs_min=_stats(np.linalg.norm(r - rt, axis=1)), ms=ms,
+    out = dict(evals=ev, positions=npos, fallbacks=nfb, lane_use=lane_use, vs_min=_stats(np.linalg.norm(r - rt, axis=1)), ms=ms,
                positions_array=r, tuning=batch.tuning)
     if net is not None and enable:
         ks = range(batch.nitems) if items is None else items

@@ -74,6 +74,8 @@ def test_batch_equals_single_scans(run):
     ne, npos = b.counts()
     assert npos == b.total_positions()
     assert ne == sum(s.counts()[0] for s in singles)
+    # evaluations over lane slots of the waves' loop passes
+    assert 0 < b.lane_use() <= 1
 
 
 def test_batch_item_against_oracle(run):
