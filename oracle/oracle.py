@@ -61,44 +61,50 @@ def lib():
     if _lib is None:
         if not LIB_PATH.exists():
             raise RuntimeError(f"oracle library missing: {LIB_PATH} (run `make -C oracle`)")
-        L = ctypes.CDLL(str(LIB_PATH))
-        L.orc_grid_shape.argtypes = [_ip, _dp, _dp, _dp]
-        L.orc_lj_ff.argtypes = [_ip, _dp, ctypes.c_int, _dp, _dp, _dp, _dp]
-        L.orc_coulomb_ff.argtypes = [_ip, _dp, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, _dp]
-        L.orc_limit_vec_field.argtypes = [ctypes.c_long, _dp, ctypes.c_double]
-        L.orc_interp.argtypes = [_ip, _dp, _dp, ctypes.c_int, _dp, _dp]
-        L.orc_relax_stroke.argtypes = [_ip, _dp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _ip, _ip]
-        L.orc_relax_stroke.restype = ctypes.c_int
-        L.orc_relax_strokes.argtypes = [ctypes.c_int] * 3 + [_ip, _dp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _ip, _ip]
-        L.orc_relax_strokes.restype = ctypes.c_long
-        L.orc_relax_strokes_subset.argtypes = [ctypes.c_int, _ip, ctypes.c_int, ctypes.c_int, _ip, _dp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _ip]
-        L.orc_relax_strokes_subset.restype = ctypes.c_long
-        L.orc_ljc_qzs_ff.argtypes = [_ip, _dp, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
-        L.orc_relax_strokes_tilted.argtypes = [
-            ctypes.c_int, ctypes.c_int, _ip, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_double,
-            ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp, _ip,
-        ]
-        L.orc_convolve_z.argtypes = [ctypes.c_long, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]
-        L.orc_stable3.argtypes = [_dp]
-        L.orc_stable3.restype = ctypes.c_int
-        L.orc_force_jac.argtypes = [_ip, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp, _dp]
-        L.orc_relax_strokes_warm.argtypes = [ctypes.c_int] * 3 + [_ip, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int,
-                                                                  ctypes.c_int, ctypes.c_int, ctypes.c_double,
-                                                                  ctypes.c_double, _dp, _dp, _ip, _ip]
-        L.orc_relax_strokes_warm.restype = ctypes.c_long
-        L.orc_stiffness.argtypes = [_ip, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int, ctypes.c_long, _dp, _dp, _dp,
-                                    _dp, _dp, _dp, _dp]
-        L.orc_stiffness.restype = ctypes.c_long
-        L.orc_model_ff.argtypes = [ctypes.c_int, ctypes.c_double, _ip, _dp, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, _dp]
-        L.orc_radial.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp, _dp, _dp, _dp]
-        L.orc_relax_strokes_spline.argtypes = [ctypes.c_int] * 3 + [_ip, _dp, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
-        L.orc_relax_strokes_spline.restype = ctypes.c_long
-        L.orc_stiffness_spline.argtypes = [_ip, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_int,
-                                           ctypes.c_long, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
-        L.orc_stiffness_spline.restype = ctypes.c_long
-        L.orc_num_threads.restype = ctypes.c_int
-        _lib = L
+        _lib = load(LIB_PATH)
     return _lib
+
+
+def load(path):
+    """Load a build of pp_oracle.c from ``path`` with its argument types (the
+    tests load copies built with planted mistakes this way)."""
+    L = ctypes.CDLL(str(path))
+    L.orc_grid_shape.argtypes = [_ip, _dp, _dp, _dp]
+    L.orc_lj_ff.argtypes = [_ip, _dp, ctypes.c_int, _dp, _dp, _dp, _dp]
+    L.orc_coulomb_ff.argtypes = [_ip, _dp, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, _dp]
+    L.orc_limit_vec_field.argtypes = [ctypes.c_long, _dp, ctypes.c_double]
+    L.orc_interp.argtypes = [_ip, _dp, _dp, ctypes.c_int, _dp, _dp]
+    L.orc_relax_stroke.argtypes = [_ip, _dp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _ip, _ip]
+    L.orc_relax_stroke.restype = ctypes.c_int
+    L.orc_relax_strokes.argtypes = [ctypes.c_int] * 3 + [_ip, _dp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _ip, _ip]
+    L.orc_relax_strokes.restype = ctypes.c_long
+    L.orc_relax_strokes_subset.argtypes = [ctypes.c_int, _ip, ctypes.c_int, ctypes.c_int, _ip, _dp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _ip]
+    L.orc_relax_strokes_subset.restype = ctypes.c_long
+    L.orc_ljc_qzs_ff.argtypes = [_ip, _dp, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
+    L.orc_relax_strokes_tilted.argtypes = [
+        ctypes.c_int, ctypes.c_int, _ip, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_double,
+        ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp, _ip,
+    ]
+    L.orc_convolve_z.argtypes = [ctypes.c_long, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]
+    L.orc_stable3.argtypes = [_dp]
+    L.orc_stable3.restype = ctypes.c_int
+    L.orc_force_jac.argtypes = [_ip, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp, _dp]
+    L.orc_relax_strokes_warm.argtypes = [ctypes.c_int] * 3 + [_ip, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int,
+                                                              ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                                              ctypes.c_double, _dp, _dp, _ip, _ip]
+    L.orc_relax_strokes_warm.restype = ctypes.c_long
+    L.orc_stiffness.argtypes = [_ip, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int, ctypes.c_long, _dp, _dp, _dp,
+                                _dp, _dp, _dp, _dp]
+    L.orc_stiffness.restype = ctypes.c_long
+    L.orc_model_ff.argtypes = [ctypes.c_int, ctypes.c_double, _ip, _dp, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, _dp]
+    L.orc_radial.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp, _dp, _dp, _dp]
+    L.orc_relax_strokes_spline.argtypes = [ctypes.c_int] * 3 + [_ip, _dp, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
+    L.orc_relax_strokes_spline.restype = ctypes.c_long
+    L.orc_stiffness_spline.argtypes = [_ip, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, ctypes.c_double, ctypes.c_int,
+                                       ctypes.c_long, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+    L.orc_stiffness_spline.restype = ctypes.c_long
+    L.orc_num_threads.restype = ctypes.c_int
+    return L
 
 
 def num_threads():
@@ -417,7 +423,8 @@ def ljc_qzs_ff(nDim, lvec, atoms, cLJs, Qs, QZs):
     return out.transpose(2, 1, 0, 3).copy()
 
 
-def relax_tilted(FE, dinv, poss, nz, tipRot, stiffness, dpos0, relax_params, surfFF=0.0, max_iters=128, conv_f2=1e-8, f2safe=1e-8):
+def relax_tilted(FE, dinv, poss, nz, tipRot, stiffness, dpos0, relax_params, surfFF=0.0, max_iters=128, conv_f2=1e-8, f2safe=1e-8,
+                 _lib=None):
     """relaxStrokesTilted semantics in FP64 with exact trilinear sampling.
 
     FE (nx,ny,nz_g,4) indexed FE[ix,iy,iz]; dinv (3,3) maps a position relative
@@ -431,7 +438,7 @@ def relax_tilted(FE, dinv, poss, nz, tipRot, stiffness, dpos0, relax_params, sur
     paths = np.zeros((npix, nz, 4))
     nevals = np.zeros((npix, nz), dtype=np.int32)
     n = np.array([nx, ny, nzg], dtype=np.int32)
-    lib().orc_relax_strokes_tilted(
+    (_lib or lib()).orc_relax_strokes_tilted(
         npix, int(nz), _ptr(n, _ip), _ptr(fe_x), _ptr(_f64(dinv, (3, 3))), _ptr(_f64(tipRot, (3, 4))), _ptr(_f64(stiffness)),
         _ptr(_f64(dpos0)), _ptr(_f64(relax_params[:2])), float(surfFF), int(max_iters), float(conv_f2), float(f2safe),
         _ptr(poss), _ptr(FEs), _ptr(paths), _ptr(nevals, _ip),

@@ -453,9 +453,9 @@ int ppafm_batch_set_atoms(ppafm_batch *b, const ppafm_atom *atoms)
     // there are any; a change of that re-captures the graph
     bool any = false;
     for (long long k = 0; k < n && !any; k++) any = atoms[k].q == 0.0;
-    if (any != b->any_uncharged) {
+    if (any != b->any_uncharged || !b->exec) {
         b->any_uncharged = any;
-        if (b->exec) return capture(*b);
+        return capture(*b);
     }
     return 0;
 }
@@ -472,11 +472,15 @@ int ppafm_batch_set_atoms_n(ppafm_batch *b, const int *natoms, const ppafm_atom 
     PP_TRY_HIP(hipStreamSynchronize(b->st));
     bool recapture = false;
     if (total > b->atom_cap) {
-        // a larger buffer: the graph holds the old address, so capture again
+        // a larger buffer: the graph holds the old address, so it goes
+        // before the old buffer is freed; an error from here on leaves the
+        // plan without a graph (ppafm_batch_run refuses it), never with one
+        // that reads freed memory, and the next successful call captures
         const long long cap = total + total / 4;
         AtomRec *p = nullptr;
         PP_TRY_HIP(hipMalloc((void **)&p, cap * sizeof(AtomRec)));
-        if (b->atoms) PP_TRY_HIP(hipFree(b->atoms));
+        drop_graph(*b);
+        if (b->atoms) (void)hipFree(b->atoms);
         b->atoms = p;
         b->atom_cap = cap;
         recapture = true;
@@ -499,7 +503,9 @@ int ppafm_batch_set_atoms_n(ppafm_batch *b, const int *natoms, const ppafm_atom 
         b->any_uncharged = any;
         recapture = true;
     }
-    if (recapture && b->exec) return capture(*b);
+    // a plan always has a graph once created: capture when this call needs a
+    // new one or an earlier failed call dropped it
+    if (recapture || !b->exec) return capture(*b);
     return 0;
 }
 

@@ -1,0 +1,219 @@
+"""GPU parity at the benchmark workloads' own sizes.
+
+* c1: one 120-atom molecule, LJ + Coulomb grid 256x256x320 (double4),
+  256x256 pixels x 60 z-steps, 9-tap df: grid, every relaxed position, the
+  sample forces and df against the FP64 oracle (the oracle is bit-exact with
+  the compiled reference, ProbeParticle.cpp:682-701 relaxTipStrokes_omp).
+* c2: the 512-molecule batch (U[8,120] atoms, 128x128x160 grids,
+  128x128x30 scans) in one captured graph: 8 seeded items against the
+  oracle at the plain path's tolerances; every item checked for finite
+  outputs, iteration counts within the cap, and device counters equal to
+  the sums the per-position iteration counts imply.
+* c3: the learned warm start on that batch (network trained on step 0 of the
+  first 95 % of the molecules): sampled held-out and training items against
+  the FP64 oracle of the warm method (orc_relax_strokes_warm fed the GPU's
+  step-0 estimates), on the GPU-built grid and on the oracle's own grid;
+  every item finite and ended by the stop rule or the cap; the distance
+  from plain FIRE's results reported over the whole batch.
+
+Tolerances: grid 1e-5 relative (1e-3 eV/A floor), positions 1e-3 A, df
+1e-3 Hz, all at the maximum; max and p99 are printed."""
+
+import numpy as np
+import pytest
+
+import oracle as O
+from cases import case, grid_rel_err, tip_params
+from ppafm_amd import batch as B
+from ppafm_amd import common as C
+from ppafm_amd import learn as L
+from ppafm_amd import molgen as M
+from ppafm_amd.scan import PlainScan
+from ppafm_amd.warmstart import WarmStartNet
+
+pytestmark = pytest.mark.gpu
+
+GRID_TOL, POS_TOL, DF_TOL = 1e-5, 1e-3, 1e-3
+NB = 512
+MAX_ITERS = 1000
+
+
+def _stats(name, d):
+    print(f"{name}: max {d.max():.3g} p99 {np.percentile(d, 99):.3g} mean {d.mean():.3g}", flush=True)
+
+
+def _evals_from_iters(it):
+    """Force evaluations the ProbeParticle-mode kernel makes for reported
+    iteration counts: iterations + 1 (the converged step's evaluation) + 1
+    (the output sample), one fewer where the cap ended the step."""
+    it = np.asarray(it, np.int64)
+    return int(it.sum()) + 2 * it.size - int((it == MAX_ITERS).sum())
+
+
+# ------------------------------------------------------------------------ c1
+
+@pytest.fixture(scope="module")
+def c1():
+    print("case: c1, 120-atom molecule, 256x256x320 grid (double4), 256x256x60 scan", flush=True)
+    k = case(1)
+    cfg = k["cfg"]
+    s = PlainScan(k["lvec"], cfg["n"], cfg, tipR0=M.TIP_R0, k_lat=M.K_LAT, k_rad=M.K_RAD, iZPP=M.PROBE_Z,
+                  probe_q=M.PROBE_Q, amplitude=M.DF_AMPLITUDE, grid_f64=True)
+    s.set_molecule(k["xyz"], k["Zs"], k["qs"])
+    s.run()
+    tips = M.scan_tips(cfg) - k["lvec"][0]
+    print("c1: oracle relaxation", flush=True)
+    ref = O.relax_strokes(cfg["n"], k["lvec"][1:], k["FF"], tip_params(), tips)
+    return k, s, ref
+
+
+def test_c1_grid(c1):
+    k, s, _ = c1
+    G = s.grid_host()
+    assert np.isfinite(G).all()
+    err = grid_rel_err(G, k["FF"])
+    _stats("c1 grid |dF|/max(|F|,1e-3)", err)
+    assert err.max() < GRID_TOL
+
+
+def test_c1_positions_forces_df(c1):
+    k, s, (rs, fs, it, tot) = c1
+    d = np.linalg.norm(s.positions() - k["lvec"][0] - rs, axis=-1)
+    _stats("c1 positions |dr| [A]", d)
+    git = s.iterations()
+    print(f"c1 iterations: gpu mean {git.mean():.2f} oracle {it.mean():.2f}; differing at {(git != it).sum()} of {it.size}")
+    nev, npos = s.counts()
+    assert npos == it.size == 256 * 256 * 60
+    assert nev == _evals_from_iters(git)
+    assert d.max() < POS_TOL
+    dfz = np.abs(s.forces()[..., 2] - fs[..., 2])
+    _stats("c1 sample Fz [eV/A]", dfz)
+    df_ref = O.fz2df(fs[:, :, ::-1, 2], 0.1, M.K0, M.F0, amplitude=M.DF_AMPLITUDE)
+    ddf = np.abs(s.df_host() - df_ref)
+    _stats("c1 df [Hz]", ddf)
+    assert ddf.max() < DF_TOL
+
+
+# ------------------------------------------------------------------- c2 / c3
+
+def _sample_items():
+    rng = np.random.default_rng(20240)
+    return sorted({0, NB - 1, *rng.choice(np.arange(1, NB - 1), 6, replace=False).tolist()})
+
+
+HELD = list(range(NB - NB // 20, NB))  # the last 5 % of the molecules
+
+
+@pytest.fixture(scope="module")
+def c2():
+    print(f"case: c2, {NB} molecules (U[8,120] atoms), 128x128x160 grids, 128x128x30 scans, one graph", flush=True)
+    cfg = M.CONFIGS[2]
+    mols = [M.config_molecule(2, k) for k in range(NB)]
+    lvec = M.config_lvec(cfg)
+    b = B.BatchScan.plain([lvec] * NB, [cfg["n"]] * NB, [cfg] * NB, tipR0=M.TIP_R0, k_lat=M.K_LAT, k_rad=M.K_RAD,
+                          iZPP=M.PROBE_Z, probe_q=M.PROBE_Q, amplitude=M.DF_AMPLITUDE, molecules=mols, keep_iters=True)
+    b.run()
+    b.synchronize()
+    yield cfg, lvec, mols, b
+    b.close()
+
+
+def _oracle_field(cfg, lvec, mol):
+    xyz, Zs, qs = mol
+    cLJs = C.getAtomsLJ(M.PROBE_Z, Zs, C.loadSpecies())
+    Rs0 = xyz - lvec[0]
+    FL = O.limit_vec_field(O.lj_ff(cfg["n"], lvec[1:], Rs0, cLJs), 10.0)
+    FE = O.limit_vec_field(O.coulomb_ff(cfg["n"], lvec[1:], Rs0, qs * C.CoulombConst), 10.0)
+    return FL + M.PROBE_Q * FE, M.scan_tips(cfg) - lvec[0]
+
+
+def test_c2_sampled_items_vs_oracle(c2):
+    cfg, lvec, mols, b = c2
+    worst = dict(grid=0.0, pos=0.0, df=0.0)
+    for k in _sample_items():
+        print(f"case: c2 item {k} ({len(mols[k][1])} atoms) vs oracle", flush=True)
+        FF, tips = _oracle_field(cfg, lvec, mols[k])
+        err = grid_rel_err(b.grid(k), FF)
+        rs, fs, it, _ = O.relax_strokes(cfg["n"], lvec[1:], FF, tip_params(), tips)
+        d = np.linalg.norm(b.positions_raw(k).astype(np.float64) - rs, axis=-1)
+        df_ref = O.fz2df(fs[:, :, ::-1, 2], 0.1, M.K0, M.F0, amplitude=M.DF_AMPLITUDE)
+        ddf = np.abs(b.df_host(k) - df_ref)
+        print(f"item {k}: grid max {err.max():.3g} p99 {np.percentile(err, 99):.3g}; positions max {d.max():.3g} "
+              f"p99 {np.percentile(d, 99):.3g} A; df max {ddf.max():.3g} Hz; iterations differ at {(b.iterations(k) != it).sum()}",
+              flush=True)
+        worst = dict(grid=max(worst["grid"], err.max()), pos=max(worst["pos"], d.max()), df=max(worst["df"], ddf.max()))
+    print("c2 sampled items, worst:", worst)
+    assert worst["grid"] < GRID_TOL and worst["pos"] < POS_TOL and worst["df"] < DF_TOL
+
+
+def test_c2_every_item_properties(c2):
+    cfg, lvec, mols, b = c2
+    nz = cfg["nz"]
+    evals, capped, total = 0, 0, 0
+    for k in range(NB):
+        p = b.positions_raw(k)
+        f = b.forces(k)
+        it = b.iterations(k)
+        df = b.df_host(k)
+        assert p.shape == (128, 128, nz, 3) and np.isfinite(p).all(), k
+        assert np.isfinite(f).all() and np.isfinite(df).all(), k
+        assert it.min() >= 0 and it.max() <= MAX_ITERS, k
+        evals += _evals_from_iters(it)
+        capped += int((it == MAX_ITERS).sum())
+        total += it.size
+    nev, npos = b.counts()
+    print(f"c2: {total} positions, {nev} force evaluations ({nev / total:.2f} per position), {capped} capped at {MAX_ITERS}")
+    assert npos == total == NB * 128 * 128 * nz
+    assert nev == evals
+
+
+@pytest.fixture(scope="module")
+def c3(c2):
+    cfg, lvec, mols, b = c2
+    print(f"case: c3, network trained on step 0 of items 0..{HELD[0] - 1}, warm starts on all {NB}", flush=True)
+    b.set_plain()
+    b.run()
+    b.synchronize()
+    plain = L.positions_all(b).astype(np.float64)
+    P, T, truth = L.training_set(b, 400_000, seed=1, items=range(HELD[0]))
+    net = WarmStartNet(max_batch=8192, seed=3)
+    losses = L.train(net, P, T, epochs=6, batch=8192, lr=3e-3)
+    print(f"c3: loss {losses[:5].mean():.3f} -> {losses[-5:].mean():.4f} over {len(losses)} steps", flush=True)
+    b.set_warmstart(net)
+    b.run()
+    b.synchronize()
+    yield cfg, lvec, mols, b, net, plain
+    b.set_plain()
+
+
+def test_c3_sampled_items_vs_warm_oracle(c3):
+    cfg, lvec, mols, b, net, plain = c3
+    for k in (HELD[0], HELD[len(HELD) // 2], NB - 1, 7):
+        print(f"case: c3 item {k} ({len(mols[k][1])} atoms) vs the FP64 warm oracle", flush=True)
+        FF, tips = _oracle_field(cfg, lvec, mols[k])
+        Fg = np.ascontiguousarray(b.grid(k)[..., :3], dtype=np.float64)
+        rg = b.positions_raw(k).astype(np.float64)
+        kw = dict(starts0=b.predictions(k).astype(np.float64), newton_max0=6)
+        dg = np.linalg.norm(rg - O.relax_strokes_warm(cfg["n"], lvec[1:], Fg, tip_params(), tips, **kw)[0], axis=-1)
+        do = np.linalg.norm(rg - O.relax_strokes_warm(cfg["n"], lvec[1:], FF, tip_params(), tips, **kw)[0], axis=-1)
+        print(f"item {k}: on the GPU grid max {dg.max():.3g} p99 {np.percentile(dg, 99):.3g}; on the oracle grid max "
+              f"{do.max():.3g} p99 {np.percentile(do, 99):.3g} A", flush=True)
+        assert dg.max() < POS_TOL and do.max() < POS_TOL
+
+
+def test_c3_every_item_properties(c3):
+    cfg, lvec, mols, b, net, plain = c3
+    total = 0
+    for k in range(NB):
+        p = b.positions_raw(k)
+        it = b.iterations(k)
+        assert np.isfinite(p).all() and np.isfinite(b.df_host(k)).all(), k
+        assert it.min() >= 0 and it.max() <= MAX_ITERS, k
+        total += it.size
+    nev, npos, nfb = b.counters()[:3]
+    assert npos == total
+    warm = L.positions_all(b).astype(np.float64)
+    d = np.linalg.norm(warm - plain, axis=1)
+    print(f"c3: {nev / total:.2f} force evaluations per position, {nfb} fallback steps; |learned - plain FIRE|: "
+          f"p50 {np.median(d):.3g} p99 {np.percentile(d, 99):.3g} max {d.max():.3g} A; > 1e-3 A at {np.mean(d > 1e-3):.4f}, "
+          f"> 0.1 A at {(d > 0.1).sum()} positions")
