@@ -175,9 +175,13 @@ typedef struct {
     double coulomb_const;
     double probe_q;
     double fmax;
-    double fp64_radius;     /* atom-point pairs closer than this [A] are evaluated
-                               in FP64 (at least 0.2 A beyond the largest stencil
-                               offset); the rest in FP32 with FP64 accumulation */
+    double fp64_radius;     /* float4 grids only.  > 0: atom-point pairs closer
+                               than this [A] are evaluated in FP64 (at least 0.2 A
+                               beyond the largest stencil offset), the rest in FP32
+                               with FP64 accumulation.  0: every pair in FP32 with
+                               FP64 accumulation; only points where an FP32 pair
+                               overflows (an atom within ~0.005 A) are summed in
+                               FP64 */
 } ppafm_ff_model;
 
 /* One atom as the grid kernels read it: position relative to the grid

@@ -145,7 +145,7 @@ class BatchScan:
     def tilted(cls, lvecs, ns, rot=np.eye(3), pos0s=None, start=(-5.0, -5.0), end=(5.0, 5.0), scan_n=(32, 32), nz=20,
                zstep=0.1, tipR0=(0.0, 0.0, 3.0), k_lat=0.25, k_rad=30.0, relax_params=None, Qs=(-0.1, 0.0, 0.0, 0.0),
                QZs=(0.0, 0.0, 0.0, 0.0), iZPP=8, df_steps=10, k0=1800.0, f0=30300.0, surf_ff=0.0, molecules=None,
-               natoms=None, grid_f64=True, keep_iters=False, tuning=None, stream=None, species=None, tune_sample=32):
+               natoms=None, grid_f64=False, keep_iters=False, tuning=None, stream=None, species=None, tune_sample=32):
         """Batch of tilted scans with the OpenCL semantics of
         RelaxedScanner.run_relaxStrokesTilted (relax.cl:391-464) on the
         evalLJC_QZs_noPos grid (FF.cl:366-423): per item a grid lattice (4,3)
@@ -155,7 +155,9 @@ class BatchScan:
         tip stiffness (k_lat, k_lat, 0, k_rad) N/m, the Qs/QZs charge stencil
         and df from get_simple_df_weight(df_steps, dz=zstep) as AFMulator
         (AFMulator.py:125-133).  LJ parameters come from the REA table
-        (getAtomsREA with alphaFac -1, REA2LJ), as AFMulator takes them."""
+        (getAtomsREA with alphaFac -1, REA2LJ), as AFMulator takes them.
+        Grids are float4 with FP32 pairs (the reference's precision) unless
+        grid_f64."""
         from .ocl.relax import DEFAULT_relax_params, rotTip
 
         if molecules is not None:

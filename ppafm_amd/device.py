@@ -117,7 +117,10 @@ def grid_desc(n, cell, f64=False):
     return g
 
 
-FP64_RADIUS = 4.0  # [A] pairs closer than this are summed in FP64 by the grid kernel
+# [A] float4 grids of the ProbeParticle model: pairs closer than this are
+# summed in FP64 (the grid then meets 1e-5 relative); the OpenCL model's
+# float4 grids are FP32 throughout (0), as the reference's own
+FP64_RADIUS = {0: 4.0, 1: 0.0}
 
 
 def ff_model(mode, with_lj=True, stencil=(), with_energy=False, coulomb_const=None, probe_q=0.0, fmax=10.0, fp64_radius=None):
@@ -138,7 +141,7 @@ def ff_model(mode, with_lj=True, stencil=(), with_energy=False, coulomb_const=No
     m.coulomb_const = float(coulomb_const)
     m.probe_q = float(probe_q)
     m.fmax = float(fmax) if fmax is not None else 0.0
-    m.fp64_radius = float(FP64_RADIUS if fp64_radius is None else fp64_radius)
+    m.fp64_radius = float(FP64_RADIUS[int(mode)] if fp64_radius is None else fp64_radius)
     return m
 
 

@@ -164,6 +164,7 @@ class AFMulator:
         kCantilever=1800,
         colorscale="gray",
         minimize_memory=False,
+        grid_f64=False,
     ):
         self.scanner = _Scanner(np.array(tipStiffness, dtype=np.float32) / -common.eVA_Nm, np.array(self.relaxParams, dtype=np.float32))
         self.iZPP = iZPP
@@ -180,6 +181,11 @@ class AFMulator:
         self.sample_lvec = None
         self.colorscale = colorscale
         self.minimize_memory = minimize_memory
+        # point-charge force fields: float4 grids with FP32 pairs, as the
+        # reference's evalLJC_QZs_noPos (FF.cl:366-423); grid_f64=True sums
+        # every pair in FP64 into a double4 grid.  The Hartree and
+        # density-based force fields always use double4 grids.
+        self.grid_f64 = bool(grid_f64)
         self._grid = None
         self._nDim_alloc = None
         self._out = None
@@ -363,11 +369,12 @@ class AFMulator:
         if pot is not None:
             qs = None
         xyzq, cLJs, charged, REAs, Zs_pbc, rot_center = self._prepare_atoms(xyzs, Zs, qs, rho_sample, sample_lvec, rot, rot_center, REAs, full=True)
-        if self._grid is None or self._nDim_alloc != n:
-            # double4 grid summed in FP64 (see ppafm_amd.scan.PlainScan for why)
-            self._grid = D.DeviceArray(n + (4,), np.float64)
+        dtype = np.float64 if (self.grid_f64 or method != "point-charge") else np.float32
+        if self._grid is None or self._nDim_alloc != n or self._grid.dtype != dtype:
+            if self._nDim_alloc != n or getattr(self, "_ws", None) is None:
+                self._ws = FFcl.Workspace(n)
+            self._grid = D.DeviceArray(n + (4,), dtype)
             self._nDim_alloc = n
-            self._ws = FFcl.Workspace(n)
         lvec_ff = self._lvec_ff()
         cell = lvec_ff[1:]
         if method == "point-charge":
@@ -465,11 +472,11 @@ class AFMulator:
             model = D.ff_model(1, with_lj=True, stencil=stencil, with_energy=True, fmax=None)
             packs = [D.pack_atoms(reqs[i]["xyzq"], reqs[i]["cLJs"], model.coulomb_const) for i in idx]
             geo = tuple((reqs[i]["n"], reqs[i]["cell"].tobytes(), bytes(reqs[i]["scan"])) for i in idx)
-            pkey = (key, charged, geo, repr(tuning))
+            pkey = (key, charged, geo, repr(tuning), self.grid_f64)
             b = cache.get(pkey)
             if b is None:
                 items = [dict(n=reqs[i]["n"], cell=reqs[i]["cell"], natoms=len(p), scan=reqs[i]["scan"]) for i, p in zip(idx, packs)]
-                b = BatchScan(items, model, r0["rp"], r0["tip"], df_weights=r0["dfw"], df_scale=1.0, grid_f64=True, tuning=tuning)
+                b = BatchScan(items, model, r0["rp"], r0["tip"], df_weights=r0["dfw"], df_scale=1.0, grid_f64=self.grid_f64, tuning=tuning)
                 if len(cache) >= self._PLAN_CACHE:
                     cache.pop(next(iter(cache)))
                 cache[pkey] = b

@@ -4,7 +4,16 @@ oracle restatement of evalLJC_QZs_noPos, relaxStrokesTilted and convolveZ.
 The reference OpenCL path cannot run here and samples the grid through the
 hardware image filter; the oracle uses exact trilinear weights, as does the
 GPU kernel, so this test pins the GPU code to the OpenCL-mode algorithm, not
-to the vendor filter (parity with the hardware filter is unpinned)."""
+to the vendor filter (parity with the hardware filter is unpinned).
+
+Two grid storages: the default float4 grid with FP32 pairs (the reference's
+own precision) and grid_f64=True.  With FP32 pairs the grid error is
+reported and bounded (p99 1e-5, max 1e-3 relative); the relaxation and df
+are checked against the oracle relaxing on the GPU's own grid (the
+relaxation alone, max 1e-3 A / 1e-3 Hz), and end to end against the
+oracle's FP64 grid at the 99.9th percentile (the FIRE stop rule turns
+~1e-7 grid differences into stopping-point changes on a few strokes).
+With FP64 pairs every bar holds at the maximum end to end."""
 
 import numpy as np
 import pytest
@@ -17,11 +26,12 @@ from ppafm_amd.ocl.AFMulator import AFMulator
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(scope="module")
-def run():
-    print("case: AFMulator 32x32x20 scan, 20-atom molecule, default dz2-like tip charges", flush=True)
+@pytest.fixture(scope="module", params=[False, True], ids=["f32grid", "f64grid"])
+def run(request):
+    print(f"case: AFMulator 32x32x20 scan, 20-atom molecule, default dz2-like tip charges, grid_f64={request.param}", flush=True)
     xyz, Zs, qs = M.molecule(20, 11)
-    afm = AFMulator(scan_dim=(32, 32, 20), scan_window=((-2.0, -2.0, 7.0), (2.0, 2.0, 9.0)), df_steps=10)
+    afm = AFMulator(scan_dim=(32, 32, 20), scan_window=((-2.0, -2.0, 7.0), (2.0, 2.0, 9.0)), df_steps=10,
+                    grid_f64=request.param)
     X = afm(xyz, Zs, qs)
     return afm, xyz, Zs, qs, X
 
@@ -43,17 +53,21 @@ def test_afm_grid(run):
     G = afm.downloadFF()
     d = np.linalg.norm(G[..., :3] - FE[..., :3], axis=-1) / np.maximum(np.linalg.norm(FE[..., :3], axis=-1), 1e-3)
     de = np.abs(G[..., 3] - FE[..., 3]) / np.maximum(np.abs(FE[..., 3]), 1e-3)
-    print(f"AFM grid rel err: F max {d.max():.3g} p99 {np.percentile(d, 99):.3g}; E max {de.max():.3g}")
+    print(f"AFM grid rel err (grid_f64={afm.grid_f64}): F max {d.max():.3g} p99 {np.percentile(d, 99):.3g}; E max {de.max():.3g}")
     assert np.isfinite(G).all()
-    assert d.max() < 1e-5
+    if afm.grid_f64:
+        assert d.max() < 1e-5
+    else:
+        assert np.percentile(d, 99) < 1e-5 and d.max() < 1e-3
 
 
 def test_afm_relax_and_df(run):
     afm, xyz, Zs, qs, X = run
     atoms, cLJs, lvec = _oracle_inputs(afm, xyz, Zs, qs)
     FE = O.ljc_qzs_ff(afm.nDim, lvec, atoms, cLJs, np.float32(afm.Qs).astype(np.float64), np.float32(afm.QZs).astype(np.float64))
-    # end to end: the oracle relaxes on its own FP64 grid
-    G = FE
+    # FP64 grid: end to end, the oracle relaxes on its own FP64 grid;
+    # FP32 grid: the oracle relaxes on the GPU's grid (the relaxation alone)
+    G = FE if afm.grid_f64 else afm.downloadFF().astype(np.float64)
     nx, ny, nz = afm.scan_dim
     cell = lvec[1:]
     dinv = np.linalg.inv(cell).T * np.array(afm.nDim[:3])[:, None]
@@ -78,6 +92,16 @@ def test_afm_relax_and_df(run):
     print(f"AFM df max |dX| {dX.max():.3g} Hz (range {Xref.min():.3g} .. {Xref.max():.3g})")
     assert X.shape == (nx, ny, nz - afm.df_steps + 1)
     assert dX.max() < 1e-3
+    if not afm.grid_f64:
+        # end to end against the oracle's FP64 grid
+        FEs_o, paths_o, _ = O.relax_tilted(FE, dinv, poss.reshape(-1, 4), nz, tipRot, afm.stiffness.astype(np.float64), dpos0,
+                                           afm.relax_params[:2].astype(np.float64), max_iters=128,
+                                           conv_f2=float(np.float32(1e-8)), f2safe=float(np.float32(1e-8)))
+        dpo = np.linalg.norm(p_gpu.reshape(-1, nz, 3) - paths_o[..., :3], axis=-1)
+        dXo = np.abs(X - O.convolve_z(FEs_o[..., 2].reshape(nx, ny, nz), afm.dfWeight.astype(np.float64)))
+        print(f"FP32 grid end to end: paths max {dpo.max():.3g} p99.9 {np.percentile(dpo, 99.9):.3g} A, "
+              f"beyond 1e-3 A {(dpo > 1e-3).sum()} of {dpo.size}; df max {dXo.max():.3g} p99.9 {np.percentile(dXo, 99.9):.3g} Hz")
+        assert np.percentile(dpo, 99.9) < 1e-3 and np.percentile(dXo, 99.9) < 1e-3
 
 
 def test_afm_output_array_and_errors(run):

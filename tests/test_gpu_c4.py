@@ -3,10 +3,13 @@ atoms) with a 60-atom molecule, 3x3 xy images (17,928 atoms), the OpenCL
 LJ + point-charge grid on 400x400x300 points at 0.1 A, and tilted strokes
 (10 degrees about y, RelaxedScanner.run_relaxStrokesTilted semantics).
 
-The full grid is built on the GPU (one batch item).  Checked against the
-FP64 oracle:
+The full grid is built on the GPU (one batch item), as float4 with FP32
+pairs (the default of the OpenCL model) and as double4 with FP64 pairs.
+Checked against the FP64 oracle:
   * two 32x32x40 windows of the grid (at the molecule and near a corner of
-    the scan), 1e-5 relative with a 1e-3 eV/A floor;
+    the scan): FP64 pairs 1e-5 relative with a 1e-3 eV/A floor at the
+    maximum; FP32 pairs 1e-5 at the 99th percentile and 1e-3 at the maximum
+    (the FP32 error is printed);
   * two 12x12x60 tilted scans (over the molecule and at the window edge,
     where the tilted apex comes closest to the slab) relaxed by the oracle
     on the GPU-built grid: paths within 1e-3 A, tip-frame forces within
@@ -32,17 +35,18 @@ def _rot_y(deg):
     return np.array([[np.cos(t), 0, np.sin(t)], [0, 1, 0], [-np.sin(t), 0, np.cos(t)]])
 
 
-@pytest.fixture(scope="module")
-def c4():
+@pytest.fixture(scope="module", params=[False, True], ids=["f32grid", "f64grid"])
+def c4(request):
     cfg = M.CONFIGS[4]
-    print("case: c4 instance 0 (17,928 atoms), 400x400x300 grid, 10-degree tilt, scans 12x12x60", flush=True)
+    print(f"case: c4 instance 0 (17,928 atoms), 400x400x300 grid (grid_f64={request.param}), 10-degree tilt, scans 12x12x60",
+          flush=True)
     xyz, Zs, qs, _ = M.c4_system(0)
     lvec = M.config_lvec(cfg)
     out = {}
     for name, (s0, s1) in SCANS.items():
         b = B.BatchScan.tilted([lvec], [cfg["n"]], rot=_rot_y(cfg["tilt_deg"]), pos0s=[(0.0, 0.0, cfg["z_top"])],
                                start=s0, end=s1, scan_n=(12, 12), nz=cfg["nz"], zstep=cfg["dz"],
-                               molecules=[(xyz, Zs, qs)])
+                               molecules=[(xyz, Zs, qs)], grid_f64=request.param)
         b.run()
         b.synchronize()
         out[name] = b
@@ -63,8 +67,13 @@ def test_c4_grid_windows_match_oracle(c4):
         FE = O.ljc_qzs_ff(sub, lv, np.concatenate([xyz, qs[:, None]], 1), cLJs, np.array([-0.1, 0, 0, 0.0]), np.zeros(4))
         Gs = G[lo[0]:lo[0] + sub[0], lo[1]:lo[1] + sub[1], lo[2]:lo[2] + sub[2]]
         e = grid_rel_err(Gs, FE)
-        print(f"window at {tuple(lo)}: max rel err {e.max():.2e}, |F| max {np.linalg.norm(FE[..., :3], axis=-1).max():.2e} eV/A")
-        assert e.max() < 1e-5
+        f64 = bs["centre"].grid_f64
+        print(f"window at {tuple(lo)} (grid_f64={f64}): rel err max {e.max():.2e} p99 {np.percentile(e, 99):.2e}, "
+              f"|F| max {np.linalg.norm(FE[..., :3], axis=-1).max():.2e} eV/A")
+        if f64:
+            assert e.max() < 1e-5
+        else:
+            assert np.percentile(e, 99) < 1e-5 and e.max() < 1e-3
 
 
 @pytest.mark.parametrize("name", list(SCANS))
