@@ -40,16 +40,16 @@ def parse():
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--no-cpu", action="store_true")
-    p.add_argument("--grid", choices=["f64", "f32"], default="f64", help="grid storage of the measured plain path")
+    p.add_argument("--no-fp32", dest="fp32", action="store_false", help="skip the FP32 grid legs (c1, batch)")
     p.add_argument("--batch-items", type=int, default=512, help="molecules of the batched workload (0 skips it)")
-    p.add_argument("--learned-samples", type=int, default=1_000_000,
+    p.add_argument("--learned-samples", type=int, default=50_000_000,
                    help="training pairs of the warm-start network on the batch (0 skips the learned comparison)")
-    p.add_argument("--learned-epochs", type=float, default=8.0)
+    p.add_argument("--learned-epochs", type=float, default=2.0)
     p.add_argument("--c4-instances", type=int, default=16,
                    help="instances of the large tilted case (Au slab + molecule, 400x400x300 grid; 0 skips it)")
     p.add_argument("--generator-molecules", type=int, default=64,
                    help="molecules of the generator comparison, per call vs batched (0 skips it)")
-    p.add_argument("--c4-grid", choices=["f64", "f32"], default="f64", help="grid storage of the large tilted case")
+    p.add_argument("--c4-grid", choices=["f64", "f32"], default="f32", help="grid storage of the large tilted case")
     p.add_argument("--no-hartree", action="store_true", help="skip the LJ+Hartree AFMulator leg")
     return p.parse_args()
 
@@ -73,11 +73,13 @@ def quiet_stdout():
 
 
 def cpu_cores():
+    """Every core in this process's affinity set (the OpenMP threads of the
+    CPU baselines)."""
     try:
         n = len(os.sched_getaffinity(0))
     except AttributeError:
         n = os.cpu_count() or 1
-    return max(1, min(n, 16))
+    return max(1, n)
 
 
 def cpu_model():
@@ -90,12 +92,13 @@ def cpu_model():
     return "unknown"
 
 
-def run_gpu(args, cfg, xyz, Zs, qs, lvec):
+def run_gpu(args, cfg, xyz, Zs, qs, lvec, grid_f64=True, fp64_radius=None):
     from ppafm_amd import device as D
     from ppafm_amd import molgen as M
     from ppafm_amd.scan import PlainScan
 
-    s = PlainScan(lvec, cfg["n"], cfg, tipR0=M.TIP_R0, k_lat=M.K_LAT, k_rad=M.K_RAD, iZPP=M.PROBE_Z, probe_q=M.PROBE_Q, amplitude=M.DF_AMPLITUDE, grid_f64=args.grid == "f64")
+    s = PlainScan(lvec, cfg["n"], cfg, tipR0=M.TIP_R0, k_lat=M.K_LAT, k_rad=M.K_RAD, iZPP=M.PROBE_Z, probe_q=M.PROBE_Q,
+                  amplitude=M.DF_AMPLITUDE, grid_f64=grid_f64, fp64_radius=fp64_radius)
     s.set_molecule(xyz, Zs, qs)
     for _ in range(args.warmup):
         s.run()
@@ -199,7 +202,7 @@ def run_batch(args):
     lvec = M.config_lvec(cfg)
     b = B.BatchScan.plain([lvec] * nb, [cfg["n"]] * nb, [cfg] * nb, tipR0=M.TIP_R0, k_lat=M.K_LAT, k_rad=M.K_RAD,
                           iZPP=M.PROBE_Z, probe_q=M.PROBE_Q, amplitude=M.DF_AMPLITUDE, molecules=mols,
-                          grid_f64=args.grid == "f64", tuning="auto")
+                          grid_f64=True, tuning="auto")
     st = b.stream
     for _ in range(max(1, args.warmup)):
         b.run()
@@ -228,10 +231,86 @@ def run_batch(args):
                tuning=b.tuning, nitems=nb)
     if args.learned_samples > 0:
         res["learned"] = run_learned(args, b)
+    if args.fp32:
+        res["fp32"] = run_fp32_batch(args, b, mols, cfg, lvec)
     if not args.no_cpu:
         res["cpu"] = run_cpu_batch(args, b, mols, cfg)
     b.close()
     return res
+
+
+def _err_stats(d):
+    d = np.asarray(d, dtype=np.float64).ravel()
+    return {"max": float(d.max()), "p99": float(np.percentile(d, 99)), "p50": float(np.median(d))}
+
+
+def run_fp32_batch(args, b64, mols, cfg, lvec):
+    """The batch with FP32 grids (every pair FP32, FP64 partial sums): the
+    grid stage timed, and its grids, relaxed positions and df against the
+    FP64 batch (itself equal to the FP64 oracle within 1e-12 on the grid and
+    1e-6 A on positions): grid |dF|/max(|F|, 1e-3), positions |dr| (max,
+    p99, fraction beyond 1e-3 A), df |d df|."""
+    from ppafm_amd import batch as B
+    from ppafm_amd import device as D
+    from ppafm_amd import molgen as M
+
+    nb = b64.nitems
+    b64.set_plain()
+    b64.set_tuning(b64.tuning)
+    b64.run()
+    b64.synchronize()
+    b = B.BatchScan.plain([lvec] * nb, [cfg["n"]] * nb, [cfg] * nb, tipR0=M.TIP_R0, k_lat=M.K_LAT, k_rad=M.K_RAD,
+                          iZPP=M.PROBE_Z, probe_q=M.PROBE_Q, amplitude=M.DF_AMPLITUDE, molecules=mols, grid_f64=False,
+                          fp64_radius=0.0, tuning=b64.tuning)
+    b.run()
+    b.synchronize()
+    e0, e1 = D.Event(), D.Event()
+    e0.record(b.stream)
+    b.run_stages(1)
+    e1.record(b.stream)
+    b.synchronize()
+    t_grid = e0.elapsed_ms(e1)
+    e0.record(b.stream)
+    b.run()
+    e1.record(b.stream)
+    b.synchronize()
+    t_step = e0.elapsed_ms(e1)
+    pairs = int(np.prod(cfg["n"])) * sum(len(m[1]) for m in mols)
+    gerr, dpos, ddf = [], [], []
+    for k in range(nb):
+        if k % 64 == 0:  # grids: every 64th item (a grid pair is 84 + 42 MB on the host)
+            g64 = b64.grid(k)
+            g32 = b.grid(k).astype(np.float64)
+            d = np.linalg.norm(g32[..., :3] - g64[..., :3], axis=-1) / np.maximum(np.linalg.norm(g64[..., :3], axis=-1), 1e-3)
+            gerr.append(d.ravel())
+        dpos.append(np.linalg.norm(b.positions_raw(k).astype(np.float64) - b64.positions_raw(k), axis=-1).ravel())
+        ddf.append(np.abs(b.df_host(k) - b64.df_host(k)).ravel())
+    b.close()
+    dpos = np.concatenate(dpos)
+    return {"grid_ms": t_grid, "ms_per_step": t_step, "grid_pts_per_s": int(np.prod(cfg["n"])) * nb / (t_grid * 1e-3),
+            "grid_build_tflops": 32 * pairs / (t_grid * 1e-3) / 1e12,
+            "grid_rel_err": _err_stats(np.concatenate(gerr)), "grid_err_sample": f"items 0, 64, ..., {nb - 1 - (nb - 1) % 64}",
+            "positions_err_A": dict(_err_stats(dpos), frac_above_1e_3=float(np.mean(dpos > 1e-3))),
+            "df_err_Hz": _err_stats(np.concatenate(ddf))}
+
+
+def run_fp32_c1(args, cfg, xyz, Zs, qs, lvec, g64):
+    """c1 with the FP32 grid kernel (fp64_radius 0): stage times and the
+    FP32 grid, positions and df against the FP64 run (equal to the FP64
+    oracle within 5e-13 on the grid and 2e-6 A on positions)."""
+    g = run_gpu(args, cfg, xyz, Zs, qs, lvec, grid_f64=False, fp64_radius=0.0)
+    s32, s64 = g["scan"], g64["scan"]
+    G64 = s64.grid_host()
+    G32 = s32.grid_host().astype(np.float64)
+    gerr = np.linalg.norm(G32[..., :3] - G64[..., :3], axis=-1) / np.maximum(np.linalg.norm(G64[..., :3], axis=-1), 1e-3)
+    dpos = np.linalg.norm(s32.positions().astype(np.float64) - s64.positions(), axis=-1)
+    ddf = np.abs(s32.df_host() - s64.df_host())
+    pairs = int(np.prod(cfg["n"])) * len(xyz)
+    return {"ms_per_step": g["t_step"], "stage_ms": {"grid": g["t_grid"], "relax": g["t_relax"], "df": g["t_df"]},
+            "positions_per_s": g["n_pos"] / (g["t_step"] * 1e-3), "grid_pts_per_s": int(np.prod(cfg["n"])) / (g["t_grid"] * 1e-3),
+            "grid_build_tflops": 32 * pairs / (g["t_grid"] * 1e-3) / 1e12, "grid_rel_err": _err_stats(gerr),
+            "positions_err_A": dict(_err_stats(dpos), frac_above_1e_3=float(np.mean(dpos > 1e-3))),
+            "df_err_Hz": _err_stats(ddf)}
 
 
 def _cpu_env():
@@ -341,15 +420,30 @@ def run_learned(args, b):
             out_ms[name] = e0.elapsed_ms(e1)
         return out_ms
 
+    def steps_ms():
+        # W untimed warm-up runs, then exactly K timed runs of the graph
+        for _ in range(args.warmup):
+            b.run()
+        b.synchronize()
+        e0.record(st)
+        for _ in range(args.steps):
+            b.run()
+        e1.record(st)
+        b.synchronize()
+        return e0.elapsed_ms(e1) / args.steps
+
     b.set_warmstart(net)
     b.set_tuning(r["learned"]["tuning"])
     stage = stages((("grid", 1), ("estimates", 8), ("relax", 16), ("df", 4)))
+    r["learned"]["ms"] = steps_ms()
     b.set_warmstart(None)
     b.set_tuning(r["newton"]["tuning"])
     stage_newton = stages((("relax", 16),))
+    r["newton"]["ms"] = steps_ms()
     b.set_plain()
     b.set_tuning(r["plain"]["tuning"])
     stage_plain = stages((("relax", 16),))
+    r["plain"]["ms"] = steps_ms()
     npos = r["plain"]["positions"]
     ncols = sum(int(np.prod(b._scan_shape(k)[:2])) for k in range(nb))
     flop_col = 2 * (9 * 81 * 32 + 288 * 32 + 32 * 3)  # one network pass per column
@@ -379,6 +473,14 @@ def run_learned(args, b):
     # the relaxation alone (the grid build and df are the same work in every setting)
     out["speedup_learned_vs_plain_relaxation"] = stage_plain["relax"] / (stage["estimates"] + stage["relax"])
     out["speedup_newton_only_vs_plain_relaxation"] = stage_plain["relax"] / stage_newton["relax"]
+    out["speedup_learned_vs_newton_only_relaxation"] = stage_newton["relax"] / (stage["estimates"] + stage["relax"])
+    # agreement with plain FIRE (the reference's results) over the held-out items
+    lp = r["plain"]["positions_array"].astype(np.float64)
+    lw = r["learned"]["positions_array"].astype(np.float64)
+    d = np.linalg.norm(lw - lp, axis=1)
+    out["held_out_learned_vs_plain"] = {"p50_A": float(np.median(d)), "p99_A": float(np.percentile(d, 99)), "max_A": float(d.max()),
+                                        "frac_above_1e_3_A": float(np.mean(d > 1e-3)), "count_above_0_1_A": int((d > 0.1).sum()),
+                                        "positions": int(len(d))}
     return out
 
 
@@ -674,118 +776,177 @@ def run_cpu(args, cfg, xyz, Zs, qs, lvec, grid_host):
     return dict(value=value, unit="relaxed tip positions/s", cores=cores, kind=kind, sample=sample, grid_pts_per_s=nx * ny * nz / t_grid, t_grid_s=t_grid, t_relax_s=t_relax)
 
 
-def traffic_from_profiles():
-    """HBM bytes per relaxation launch from the committed rocprofv3 PMC summary
-    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
-    path = os.path.join(ROOT, "profiles", "relax_c1_pmc.json")
-    try:
-        d = json.load(open(path))
-        return float(d["hbm_bytes_per_launch"])
-    except (OSError, KeyError, ValueError):
-        return None
+def profile_figures():
+    """Figures read from the committed rocprofv3 summaries (profiles/*.json):
+    HBM bytes per launch (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel
+    calibration) and counter fractions of the kernels the line quotes."""
+    out = {}
+    for name, pre in (("relax_c1_pmc.json", "relax_c1_"), ("grid_batch_pmc.json", "grid_batch_"),
+                      ("warm_relax_pmc.json", "warm_relax_")):
+        try:
+            d = json.load(open(os.path.join(ROOT, "profiles", name)))
+        except (OSError, ValueError):
+            continue
+        for k, v in d.items():
+            if isinstance(v, (int, float)) and not isinstance(v, bool):
+                out[pre + k] = float(v)
+    return out
 
 
 def main():
     args = parse()
-    # OpenMP threads of the CPU legs: every core this process may use
-    os.environ["OMP_NUM_THREADS"] = str(cpu_cores())
+    # OpenMP threads of the CPU legs: every core this process may use (set
+    # before any OpenMP runtime starts)
+    cores = cpu_cores()
+    os.environ["OMP_NUM_THREADS"] = str(cores)
     os.environ.setdefault("OMP_PROC_BIND", "close")
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).", file=sys.stderr)
         sys.exit(2)
+    from ppafm_amd import device as D
     from ppafm_amd import molgen as M
 
+    peaks = {k: D.measure_peak(k) for k in D.PEAK_KINDS}
+    prof = profile_figures()
+
+    # ---- the metric's configuration: learned warm start + batching against
+    # plain FIRE on the 512-molecule batch (the headline line)
+    bt = run_batch(args) if args.batch_items > 0 else None
+    Lr = bt.get("learned") if bt else None
+    cfg2 = M.CONFIGS[2]
+    if Lr:
+        npos = bt["n_pos"]
+        ms = Lr["learned"]["ms_per_step"]
+        st = Lr["learned"]["stage_ms"]
+        pairs = bt["npts"] // bt["nitems"] * bt["natoms_total"]
+        grid_tf = 32 * pairs / (st["grid"] * 1e-3) / 1e12
+        relax_ms = st["estimates"] + st["relax"]
+        if st["grid"] >= relax_ms:
+            roof = {"kernel": "ff_build_f64_batch (FP64 pairs, double4 grids; the learned step's longest stage)",
+                    "bound": "valu", "achieved": grid_tf, "peak": 157.3, "unit": "TFLOP/s", "frac": grid_tf / 157.3,
+                    "traffic": prof.get("grid_batch_hbm_bytes_per_launch"),
+                    "algorithmic_unit": "32 FLOP per atom-point pair (LJ + Coulomb, force + energy)",
+                    "pairs_per_launch": pairs, "kernel_ms": st["grid"],
+                    "fp64_vendor_peak_tflops": 78.6, "frac_of_fp64_vendor_peak": grid_tf / 78.6,
+                    "fp64_measured_peak_tflops": peaks["fp64_fma"]}
+        else:
+            ev = Lr["learned"]["force_evals_per_position"] * npos
+            ach = (96 * ev + 24 * npos) / (st["relax"] * 1e-3) / 1e9
+            roof = {"kernel": "relax_refill_kernel (warm-started FIRE, the learned step's longest stage)", "bound": "latency",
+                    "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                    "traffic": prof.get("warm_relax_hbm_bytes_per_launch"),
+                    "algorithmic_unit": "96 B per force evaluation + 24 B per relaxed position", "kernel_ms": st["relax"]}
+        out = {
+            "metric": METRIC,
+            "value": npos / (ms * 1e-3),
+            "unit": "relaxed tip positions/s",
+            "n_gpus": 1,
+            "steps": args.steps,
+            "warmup": args.warmup,
+            "ms_per_step": ms,
+            "higher_is_better": True,
+            "scaling": "weak",
+            "vs_baseline": None,
+            "dtype": "f64",
+            "data": "synthetic",
+            "config": {
+                "workload": f"learned warm-start FIRE on the batch of {bt['nitems']} seeded molecules (U[8,120] atoms), each a "
+                            "128x128x160 LJ+Coulomb grid at 0.1 A (FP64) and a 128x128x30 scan, one captured graph "
+                            "(grid, step-0 network estimates, warm-started FIRE to |F| < 1e-4 eV/A, 9-tap df); "
+                            "against plain FIRE on the same batch",
+                "molecules": bt["nitems"], "tip_positions": npos, "grid_points": bt["npts"],
+                "training_pairs": Lr["training"]["pairs"], "held_out_molecules": Lr["training"]["held_out_molecules"],
+            },
+            "speedup_vs_plain_relaxation": Lr["speedup_learned_vs_plain_relaxation"],
+            "speedup_vs_plain_step": Lr["speedup_learned_vs_plain"],
+            "speedup_vs_newton_only_relaxation": Lr["speedup_learned_vs_newton_only_relaxation"],
+            "speedup_vs_newton_only_step": Lr["speedup_learned_vs_newton_only"],
+            "relax_positions_per_s": npos / (relax_ms * 1e-3),
+            "plain_relax_positions_per_s": npos / (Lr["plain"]["stage_ms"]["relax"] * 1e-3),
+            "grid_pts_per_s": bt["npts"] / (st["grid"] * 1e-3),
+            "stage_ms": st,
+            "roofline": roof,
+            "cpu_baseline": None,
+        }
+        if "cpu" in bt:
+            c = bt["cpu"]
+            out["cpu_baseline"] = {k: c[k] for k in ("value", "unit", "cores", "kind", "sample")}
+            out["cpu_baseline"]["affinity_cores"] = cores
+            out["speedup_vs_cpu"] = out["value"] / c["value"]
+        out["learned"] = Lr
+    else:
+        out = {"metric": METRIC, "value": None, "unit": "relaxed tip positions/s", "n_gpus": 1, "steps": args.steps,
+               "warmup": args.warmup, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+               "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+               "config": {"workload": "learned comparison skipped (--batch-items 0 or --learned-samples 0)"},
+               "roofline": None, "cpu_baseline": None}
+    if bt:
+        out["c2_plain_batch"] = {
+            "workload": f"{bt['nitems']} seeded molecules (U[8,120] atoms), each a 128x128x160 FP64 grid at 0.1 A and a "
+                        "128x128x30 scan, plain FIRE, one captured graph",
+            "positions": bt["n_pos"], "grid_points": bt["npts"], "ms_per_step": bt["t_step"],
+            "positions_per_s": bt["n_pos"] / (bt["t_step"] * 1e-3), "grid_pts_per_s": bt["npts"] / (bt["stage"]["grid"] * 1e-3),
+            "relax_positions_per_s": bt["n_pos"] / (bt["stage"]["relax"] * 1e-3), "stage_ms": bt["stage"],
+            "force_evals": bt["n_eval"], "relax_lane_use": bt["lane_use"],
+            "grid_build_tflops": 32 * (bt["npts"] // bt["nitems"] * bt["natoms_total"]) / (bt["stage"]["grid"] * 1e-3) / 1e12,
+            "grid_peak_tflops": 157.3,
+            "relax_algorithmic_gbs": (96 * bt["n_eval"] + 24 * bt["n_pos"]) / (bt["stage"]["relax"] * 1e-3) / 1e9,
+            "tuning": bt["tuning"]}
+        if "cpu" in bt:
+            out["c2_plain_batch"]["cpu_baseline"] = dict(bt["cpu"], affinity_cores=cores)
+            out["c2_plain_batch"]["speedup_vs_cpu"] = out["c2_plain_batch"]["positions_per_s"] / bt["cpu"]["value"]
+        if "fp32" in bt:
+            out["c2_fp32_grid"] = bt["fp32"]
+
+    # ---- c1: plain FIRE, one 120-atom molecule, the production-size scan
     cfg = M.CONFIGS[1]
     xyz, Zs, qs = M.config_molecule(1, 0, cfg["natoms"])
     lvec = M.config_lvec(cfg)
     g = run_gpu(args, cfg, xyz, Zs, qs, lvec)
     npts = int(np.prod(cfg["n"]))
     n_pos = g["n_pos"]
-    value = n_pos / (g["t_step"] * 1e-3)
-    corner_bytes = 8 if args.grid == "f64" else 4
-    alg_bytes = 8 * 3 * corner_bytes * g["n_eval"] + 24 * n_pos
+    alg_bytes = 96 * g["n_eval"] + 24 * n_pos
     achieved = alg_bytes / (g["t_relax"] * 1e-3) / 1e9
-    traffic = traffic_from_profiles()
-    from ppafm_amd import device as D
+    traffic = prof.get("relax_c1_hbm_bytes_per_launch")
+    from ppafm_amd.batch import device_info
 
-    peaks = {k: D.measure_peak(k) for k in D.PEAK_KINDS}
-    out = {
-        "metric": METRIC,
-        "value": value,
-        "unit": "relaxed tip positions/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": g["t_step"],
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64" if args.grid == "f64" else "f32 grid / f64 relax",
-        "data": "synthetic",
-        "config": {
-            "workload": "plain FIRE, one 120-atom molecule: LJ+Coulomb grid 256x256x320 at 0.1 A, 256x256 pixels x 60 z-steps, 9-tap df",
-            "molecule_seed": 1000000,
-            "grid_storage": args.grid,
-            "grid_points": npts,
-            "tip_positions": n_pos,
-        },
-        "grid_pts_per_s": npts / (g["t_grid"] * 1e-3),
-        "relax_positions_per_s": n_pos / (g["t_relax"] * 1e-3),
+    nsimd = 4 * int(device_info()[1])
+    c1 = {
+        "workload": "plain FIRE, one 120-atom molecule: LJ+Coulomb grid 256x256x320 at 0.1 A (FP64), 256x256 pixels x 60 "
+                    "z-steps, 9-tap df",
+        "positions_per_s": n_pos / (g["t_step"] * 1e-3), "ms_per_step": g["t_step"],
+        "grid_pts_per_s": npts / (g["t_grid"] * 1e-3), "relax_positions_per_s": n_pos / (g["t_relax"] * 1e-3),
         "stage_ms": {"grid": g["t_grid"], "relax": g["t_relax"], "df": g["t_df"]},
-        "host_wall_ms_per_step": g["wall_step_ms"],
-        "force_evals_per_step": g["n_eval"],
-        "mean_fire_iterations": g["mean_iters"],
+        "host_wall_ms_per_step": g["wall_step_ms"], "force_evals_per_step": g["n_eval"], "mean_fire_iterations": g["mean_iters"],
         "roofline": {
-            "kernel": "relax_kernel (FIRE)",
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
+            "kernel": "relax_dev_kernel (FIRE, one lane per pixel column)",
+            "bound": "latency",
+            "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
             "algorithmic_bytes_per_launch": alg_bytes,
-            "bytes_model": f"8 corners x 3 x {corner_bytes} B per force evaluation + 24 B per relaxed position",
-            "achieved_fp32_corner_units": (96 * g["n_eval"] + 24 * n_pos) / (g["t_relax"] * 1e-3) / 1e9,
-            "fp32_corner_units_note": "96 B per force evaluation (8 corners x 3 FP32) + 24 B per position, as for an FP32 grid",
-            "note": "the unit counts every evaluation's 8 corners; the kernel keeps a probe's cell corners in registers across "
-                    "its FIRE iterations and reads most of the rest from L1/L2, so `traffic` (measured HBM bytes) is far below it",
+            "bytes_model": "96 B per force evaluation (8 corners x 3 FP32) + 24 B per relaxed position",
+            "waves_per_simd": (cfg["scan_n"][0] // 8) * (cfg["scan_n"][1] // 8) / nsimd,
+            "physical_hbm_gbs": traffic / (g["t_relax"] * 1e-3) / 1e9 if traffic else None,
+            "physical_hbm_frac": traffic / (g["t_relax"] * 1e-3) / 1e9 / HBM_PEAK_GBS if traffic else None,
+            "valu_busy_frac": prof.get("relax_c1_valu_busy_frac"),
+            "note": "each pixel column is a serial chain of dependent FP64 force evaluations and the scan has one wave per "
+                    "SIMD, so instruction latency, not bandwidth, bounds the kernel; `traffic` and physical_hbm_* are the "
+                    "rocprof FETCH_SIZE + WRITE_SIZE bytes of one launch",
         },
         "grid_build": {
-            "bound": "valu",
-            "pairs": npts * len(xyz),
-            "achieved_tflops": 32 * npts * len(xyz) / (g["t_grid"] * 1e-3) / 1e12,
-            "peak_tflops": 78.6 if args.grid == "f64" else 157.3,
-            "peak_measured_tflops": peaks["fp64_fma"] if args.grid == "f64" else peaks["fp32_fma"],
-            "note": "32 algorithmic FLOP per atom-point pair (LJ + Coulomb, force + energy); FP64 pairs on the double4 grid",
+            "bound": "valu", "pairs": npts * len(xyz),
+            "achieved_tflops": 32 * npts * len(xyz) / (g["t_grid"] * 1e-3) / 1e12, "peak_tflops": 157.3,
+            "frac": 32 * npts * len(xyz) / (g["t_grid"] * 1e-3) / 1e12 / 157.3,
+            "fp64_vendor_peak_tflops": 78.6, "fp64_measured_peak_tflops": peaks["fp64_fma"],
+            "note": "32 algorithmic FLOP per atom-point pair (LJ + Coulomb, force + energy), FP64 pairs on the double4 grid, "
+                    "against the FP32 vector peak the grid roofline is fixed to",
         },
-        "peaks_measured_tflops": peaks,
     }
-    if args.batch_items > 0:
-        bt = run_batch(args)
-        pairs = bt["npts"] // bt["nitems"] * bt["natoms_total"]
-        out["batch"] = {
-            "workload": f"{bt['nitems']} seeded molecules (U[8,120] atoms), each a 128x128x160 grid at 0.1 A and a 128x128x30 scan, one captured graph",
-            "positions": bt["n_pos"],
-            "grid_points": bt["npts"],
-            "ms_per_step": bt["t_step"],
-            "positions_per_s": bt["n_pos"] / (bt["t_step"] * 1e-3),
-            "grid_pts_per_s": bt["npts"] / (bt["stage"]["grid"] * 1e-3),
-            "relax_positions_per_s": bt["n_pos"] / (bt["stage"]["relax"] * 1e-3),
-            "stage_ms": bt["stage"],
-            "force_evals": bt["n_eval"],
-            "relax_lane_use": bt["lane_use"],
-            "grid_build_tflops": 32 * pairs / (bt["stage"]["grid"] * 1e-3) / 1e12,
-            "relax_algorithmic_gbs": (8 * 3 * corner_bytes * bt["n_eval"] + 24 * bt["n_pos"]) / (bt["stage"]["relax"] * 1e-3) / 1e9,
-            "tuning": bt["tuning"],
-            "speedup_vs_single_molecule_path": (bt["n_pos"] / bt["t_step"]) / (n_pos / g["t_step"]),
-        }
-        if "cpu" in bt:
-            out["batch"]["cpu_baseline"] = bt["cpu"]
-            out["batch"]["speedup_vs_cpu"] = out["batch"]["positions_per_s"] / bt["cpu"]["value"]
-        if "learned" in bt:
-            out["learned"] = dict(workload="the batch above: step-0 network + Newton warm starts, FIRE to the same threshold",
-                                  **bt["learned"])
-            out["learned"]["learned"]["estimates_mfma"]["peak_measured_tflops"] = peaks["bf16_mfma"]
+    out["c1"] = c1
+    out["peaks_measured_tflops"] = peaks
+    if args.fp32:
+        print("[bench] c1 with the FP32 grid kernel", file=sys.stderr, flush=True)
+        out["c1_fp32_grid"] = run_fp32_c1(args, cfg, xyz, Zs, qs, lvec, g)
     if args.generator_molecules > 0:
         print(f"[bench] generator: {args.generator_molecules} molecules", file=sys.stderr, flush=True)
         out["generator"] = run_generator(args)
@@ -795,28 +956,27 @@ def main():
     if not args.no_hartree:
         print("[bench] hartree", file=sys.stderr, flush=True)
         out["hartree"] = run_hartree(args)
-    st = run_stiffness(g["scan"])
-    stiff_bytes = 6 * 8 * 4 * corner_bytes + 12 + 96
+    st_ = run_stiffness(g["scan"])
+    stiff_bytes = 6 * 8 * 4 * 8 + 12 + 96
     out["stiffness"] = {
-        "workload": "stiffnessMatrix (which 3, ddisp 0.05 A) at every relaxed position of the scan above, on its grid",
-        "positions": st["npos"],
-        "ms": st["ms"],
-        "positions_per_s": st["npos"] / (st["ms"] * 1e-3),
-        "algorithmic_gbs": stiff_bytes * st["npos"] / (st["ms"] * 1e-3) / 1e9,
-        "bytes_model": f"6 force evaluations x 8 corners x {4 * corner_bytes} B + 12 B position in + 96 B eigenvalues and eigenvectors out (FP64)",
-        "positions_with_non_positive_smallest_eigenvalue": st["n_unstable"],
-        "positions_with_nan": st["n_nan"],
+        "workload": "stiffnessMatrix (which 3, ddisp 0.05 A) at every relaxed position of the c1 scan, on its grid",
+        "positions": st_["npos"],
+        "ms": st_["ms"],
+        "positions_per_s": st_["npos"] / (st_["ms"] * 1e-3),
+        "algorithmic_gbs": stiff_bytes * st_["npos"] / (st_["ms"] * 1e-3) / 1e9,
+        "bytes_model": "6 force evaluations x 8 corners x 32 B + 12 B position in + 96 B eigenvalues and eigenvectors out (FP64)",
+        "positions_with_non_positive_smallest_eigenvalue": st_["n_unstable"],
+        "positions_with_nan": st_["n_nan"],
     }
     if not args.no_cpu:
         grid_host = g["scan"].grid_host()
-        cs = run_cpu_stiffness(cfg, lvec, grid_host, st)
+        cs = run_cpu_stiffness(cfg, lvec, grid_host, st_)
         out["stiffness"]["cpu_baseline"] = cs
         out["stiffness"]["speedup_vs_cpu"] = out["stiffness"]["positions_per_s"] / cs["value"]
         c = run_cpu(args, cfg, xyz, Zs, qs, lvec, grid_host)
-        out["cpu_baseline"] = {"value": c["value"], "unit": c["unit"], "cores": c["cores"], "kind": c["kind"], "sample": c["sample"], "grid_pts_per_s": c["grid_pts_per_s"]}
-        out["speedup_vs_cpu"] = value / c["value"]
-    else:
-        out["cpu_baseline"] = None
+        c1["cpu_baseline"] = {k: c[k] for k in ("value", "unit", "cores", "kind", "sample", "grid_pts_per_s")}
+        c1["cpu_baseline"]["affinity_cores"] = cores
+        c1["speedup_vs_cpu"] = c1["positions_per_s"] / c["value"]
     print(json.dumps(out), flush=True)
 
 

@@ -207,6 +207,7 @@ int ppafm_ff_build_dev(const ppafm_grid *grid, const ppafm_ff_model *model, int 
 int ppafm_ff_build(const ppafm_grid *grid, const ppafm_ff_model *model, int natoms, const double *atoms_xyzq,
                    const double *cLJs, void *out_dev, void *stream);
 
+
 /* Relaxation semantics.
  *  mode 0 (ProbeParticle.cpp): FIRE state (dt, alpha) reset for every tip
  *         position; output force = grid force sampled at the final position.
@@ -302,11 +303,16 @@ typedef struct {
 
 /* Launch shape of the persistent relaxation: pixel tiles of
  * (64 / tile_y) x tile_y columns per wave (tile_y in 1, 2, 4, 8, 16),
- * waves_per_block in 1..4, blocks_per_cu persistent workgroups per CU. */
+ * waves_per_block in 1..4, blocks_per_cu persistent workgroups per CU.
+ * refill 1: each lane takes the next pixel column (in tile order) as soon as
+ * its own is done, instead of the wave taking the next tile when all 64 are;
+ * used when every item shares one grid shape, cell and scan (otherwise the
+ * tile queue runs).  Results do not depend on the launch shape. */
 typedef struct {
     int tile_y;
     int waves_per_block;
     int blocks_per_cu;
+    int refill;
 } ppafm_batch_tuning;
 
 typedef struct ppafm_batch ppafm_batch;
@@ -432,6 +438,7 @@ int ppafm_malloc(void **ptr, unsigned long long bytes);
 int ppafm_free(void *ptr);
 int ppafm_memcpy_h2d(void *dst, const void *src, unsigned long long bytes, void *stream);
 int ppafm_memcpy_d2h(void *dst, const void *src, unsigned long long bytes, void *stream);
+int ppafm_memcpy_d2d(void *dst, const void *src, unsigned long long bytes, void *stream);
 int ppafm_memset(void *dst, int value, unsigned long long bytes, void *stream);
 int ppafm_synchronize(void *stream);
 int ppafm_device_count(void);

@@ -85,6 +85,13 @@ def load(path):
         ctypes.c_int, ctypes.c_int, _ip, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_double,
         ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp, _ip,
     ]
+    L.orc_relax_strokes_tilted_warm.argtypes = [
+        ctypes.c_int, ctypes.c_int, _ip, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int, ctypes.c_double,
+        ctypes.c_double, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp,
+        _ip, _ip,
+    ]
+    L.orc_relax_strokes_tilted_warm.restype = ctypes.c_long
+    L.orc_tilted_force_jac.argtypes = [_ip, _dp, _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int, _dp, _dp, _dp, _dp]
     L.orc_convolve_z.argtypes = [ctypes.c_long, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]
     L.orc_stable3.argtypes = [_dp]
     L.orc_stable3.restype = ctypes.c_int
@@ -444,6 +451,47 @@ def relax_tilted(FE, dinv, poss, nz, tipRot, stiffness, dpos0, relax_params, sur
         _ptr(poss), _ptr(FEs), _ptr(paths), _ptr(nevals, _ip),
     )
     return FEs, paths, nevals
+
+
+def relax_tilted_warm(FE, dinv, poss, nz, tipRot, stiffness, dpos0, relax_params, surfFF=0.0, max_iters=128,
+                      conv_f2=1e-8, f2safe=1e-8, starts0=None, newton_max=8, newton_max0=12, retry=8, tol=1e-3, cap=0.1):
+    """The warm method (see relax_strokes_warm) on relaxStrokesTilted
+    semantics, arguments as relax_tilted; starts0 (npix, 3) optional step-0
+    estimates.  Returns FEs, paths (npix, nz, 4), FIRE iterations and force
+    evaluations (npix, nz) and the number of fallback steps."""
+    nx, ny, nzg = FE.shape[:3]
+    fe_x = np.ascontiguousarray(np.asarray(FE, dtype=np.float64).transpose(2, 1, 0, 3))
+    poss = _f64(poss, (-1, 4))
+    npix = len(poss)
+    FEs = np.zeros((npix, nz, 4))
+    paths = np.zeros((npix, nz, 4))
+    iters = np.zeros((npix, nz), dtype=np.int32)
+    nev = np.zeros((npix, nz), dtype=np.int32)
+    n = np.array([nx, ny, nzg], dtype=np.int32)
+    st = None if starts0 is None else _f64(starts0, (npix, 3))
+    nfb = lib().orc_relax_strokes_tilted_warm(
+        npix, int(nz), _ptr(n, _ip), _ptr(fe_x), _ptr(_f64(dinv, (3, 3))), _ptr(_f64(tipRot, (3, 4))), _ptr(_f64(stiffness)),
+        _ptr(_f64(dpos0)), _ptr(_f64(relax_params[:2])), float(surfFF), int(max_iters), float(conv_f2), float(f2safe),
+        _ptr(poss), None if st is None else _ptr(st), int(newton_max), int(newton_max0), int(retry), float(tol), float(cap),
+        _ptr(FEs), _ptr(paths), _ptr(iters, _ip), _ptr(nev, _ip),
+    )
+    return FEs, paths, iters, nev, int(nfb)
+
+
+def tilted_force_jac(FE, dinv, tipRot, stiffness, dpos0, tips, rs, surfFF=0.0):
+    """Total probe force and its Jacobian in relaxStrokesTilted semantics at
+    positions rs (m, 3) for tips (m, 3): (m, 3), (m, 3, 3)."""
+    nx, ny, nzg = FE.shape[:3]
+    fe_x = np.ascontiguousarray(np.asarray(FE, dtype=np.float64).transpose(2, 1, 0, 3))
+    tips = _f64(tips, (-1, 3))
+    rs = _f64(rs, (-1, 3))
+    f = np.zeros_like(rs)
+    J = np.zeros((len(rs), 3, 3))
+    n = np.array([nx, ny, nzg], dtype=np.int32)
+    lib().orc_tilted_force_jac(_ptr(n, _ip), _ptr(fe_x), _ptr(_f64(dinv, (3, 3))), _ptr(_f64(tipRot, (3, 4))),
+                               _ptr(_f64(stiffness)), _ptr(_f64(dpos0)), float(surfFF), len(rs), _ptr(tips), _ptr(rs),
+                               _ptr(f), _ptr(J))
+    return f, J
 
 
 def convolve_z(F, w):

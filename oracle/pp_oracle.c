@@ -579,12 +579,14 @@ static int solve3(double J[3][3], const double b[3], double x[3])
 
 /* One Newton iteration from *r, the step cut to at most cap (damped Newton):
  * returns 1 and moves r when the step is finite; *st2 = squared length of
- * the full step; *stable = stability of J at the old r. */
+ * the full step; *stable = stability of J at the old r; *f2 = squared total
+ * force at the old r. */
 static int newton_iter(const orc_field *ff, const orc_tip *tip, v3 rtip, v3 *r, double cap, int *stable,
-                       double *st2)
+                       double *st2, double *f2)
 {
     double f[3], J[3][3], dr[3];
     force_jac(ff, tip, rtip, *r, f, J);
+    *f2 = f[0] * f[0] + f[1] * f[1] + f[2] * f[2];
     *stable = orc_stable3(&J[0][0]);
     const double b[3] = {-f[0], -f[1], -f[2]};
     if (!solve3(J, b, dr)) return 0;
@@ -609,17 +611,24 @@ void orc_force_jac(const int n[3], const double cell[9], const double *F, const 
     }
 }
 
-/* Newton iterations from *q (at most kmax, the first one already done by
- * the caller when first_done): 1 when they converge (full step < tol, every
- * iterate stable), *q then the converged point; counts evaluations. */
+/* Newton iterations from *q (at most kmax): 2 when an iterate with a
+ * stable Jacobian already meets the stop rule (|F|^2 < conv_f2; *q is that
+ * iterate, the step ends there), 1 when they converge (full step < tol,
+ * every iterate stable; *q the converged point), else 0; counts
+ * evaluations. */
 static int newton_run(const orc_field *ff, const orc_tip *tip, v3 rtip, v3 *q, int kmax, double tol, double cap,
-                      int *ne)
+                      double conv_f2, int *ne)
 {
     for (int k = 0; k < kmax; k++) {
         int stable = 0;
-        double st2 = 0;
-        const int moved = newton_iter(ff, tip, rtip, q, cap, &stable, &st2);
+        double st2 = 0, f2 = 0;
+        const v3 q0 = *q;
+        const int moved = newton_iter(ff, tip, rtip, q, cap, &stable, &st2, &f2);
         (*ne)++;
+        if (stable && f2 < conv_f2) {
+            *q = q0;
+            return 2;
+        }
         if (moved && stable && st2 < tol * tol) return 1;
         if (!(moved && stable)) return 0;
     }
@@ -631,7 +640,8 @@ static int newton_run(const orc_field *ff, const orc_tip *tip, v3 rtip, v3 *q, i
  * the current position; where it is stable and finite, Newton iterations
  * continue (newton_max in all, this step included) and on convergence FIRE
  * restarts at the zero with v = 0 and fresh dt, alpha, its iteration count
- * running on.  `check`: *r is a Newton zero; when the first evaluation there
+ * running on; when a later Newton iterate meets the stop rule the step ends
+ * there.  `check`: *r is a Newton zero; when the first evaluation there
  * already meets the stop rule the step ends at *r without a move (returns
  * 0), as after any later convergence of Newton.  Returns the iteration index
  * at which FIRE stopped. */
@@ -653,11 +663,15 @@ static int fire_retry(const orc_field *ff, const orc_tip *tip, const orc_relax *
             next_try = it + retry;
             v3 q = *r;
             int stable = 0;
-            double st2 = 0;
-            const int moved = newton_iter(ff, tip, rtip, &q, cap, &stable, &st2);
+            double st2 = 0, f2 = 0;
+            const int moved = newton_iter(ff, tip, rtip, &q, cap, &stable, &st2, &f2);
             if (moved && stable) {
                 int conv = st2 < tol * tol;
-                if (!conv) conv = newton_run(ff, tip, rtip, &q, newton_max - 1, tol, cap, ne);
+                if (!conv) conv = newton_run(ff, tip, rtip, &q, newton_max - 1, tol, cap, rp->conv_f2, ne);
+                if (conv == 2) {
+                    *r = q;
+                    return it;
+                }
                 if (conv) {
                     *r = q;
                     v = (v3){0, 0, 0};
@@ -706,19 +720,23 @@ static int warm_stroke(const orc_field *ff, const orc_tip *tip, const orc_relax 
         v3 drp = {rp_.x - rtip.x, rp_.y - rtip.y, rp_.z - rtip.z};
         rtip = nt;
         rp_.x = rtip.x + drp.x; rp_.y = rtip.y + drp.y; rp_.z = rtip.z + drp.z;
-        int ne = 0, check = 0;
+        int ne = 0, check = 0, settled = 0;
         const int kmax = i == 0 ? newton_max0 : newton_max;
         v3 q = rp_;
         if (i == 0 && start0) q = (v3){start0[0], start0[1], start0[2]};
-        if (kmax == 0) {
+        const int res = kmax == 0 ? -1 : newton_run(ff, tip, rtip, &q, kmax, tol, cap, rp->conv_f2, &ne);
+        if (res == -1) {
             rp_ = q;
-        } else if (newton_run(ff, tip, rtip, &q, kmax, tol, cap, &ne)) {
+        } else if (res == 2) {
+            rp_ = q;
+            settled = 1;
+        } else if (res == 1) {
             rp_ = q;
             check = 1;
         } else {
             nfb++;
         }
-        const int itr = fire_retry(ff, tip, rp, rtip, &rp_, check, retry, newton_max, tol, cap, &ne);
+        const int itr = settled ? 0 : fire_retry(ff, tip, rp, rtip, &rp_, check, retry, newton_max, tol, cap, &ne);
         v3 g = {v3_dot(rp_, (v3){ff->dinv[0], ff->dinv[1], ff->dinv[2]}),
                 v3_dot(rp_, (v3){ff->dinv[3], ff->dinv[4], ff->dinv[5]}),
                 v3_dot(rp_, (v3){ff->dinv[6], ff->dinv[7], ff->dinv[8]})};
@@ -1168,6 +1186,269 @@ void orc_relax_strokes_tilted(int npix, int nz, const int n[3], const double *FE
             for (int c = 0; c < 3; c++) { tp[c] += dtip[c]; pos[c] += dtip[c]; }
         }
     }
+}
+
+/* ------------------------------------- warm starts, OpenCL (tilted) semantics */
+
+/* The warm method of orc_relax_strokes_warm on relaxStrokesTilted's
+ * semantics (relax.cl:391-464): the tip springs act in the tip frame rows
+ * A, B, C (tipForce, relax.cl:37-41), the constant surface force along C,
+ * FIRE with dt 0.5, dtmin = dtmax / 10, F2SAFE in the square root, dt and
+ * alpha kept along the stroke (update_FIRE, relax.cl:105-126), at most
+ * max_iters steps, the stored force the last evaluation's grid force
+ * rotated into the tip frame.  Each step opens with Newton iterations from
+ * the continuation start (step 0: from starts0 when given); FIRE then ends
+ * the step, with a Newton try every `retry` iterations.  Field FE is
+ * x-fastest (nx, ny, nz, 4). */
+typedef struct {
+    const double *FE;
+    int n[3];
+    double dinv[9];
+    double R[9];     /* rows A, B, C */
+    double k[3], krad, L, d0[3], surf;
+} tw_field;
+
+/* Grid force fe (4 channels), total force f and its Jacobian at r.  fe and f
+ * round as in orc_relax_strokes_tilted (the same interpolation and tip-force
+ * expressions), so the warm method without Newton reproduces it exactly. */
+static void tw_force_jac(const tw_field *t, v3 tp, v3 r, double fe[4], double f[3], double J[3][3])
+{
+    const double g[3] = {r.x * t->dinv[0] + r.y * t->dinv[1] + r.z * t->dinv[2],
+                         r.x * t->dinv[3] + r.y * t->dinv[4] + r.z * t->dinv[5],
+                         r.x * t->dinv[6] + r.y * t->dinv[7] + r.z * t->dinv[8]};
+    fe_interp_xfast(t->FE, t->n, g[0], g[1], g[2], fe);
+    /* the trilinear interpolant's derivative in grid coordinates */
+    int lo[3], hi[3];
+    double wl[3], wh[3];
+    for (int a = 0; a < 3; a++) wrap_axis(g[a], t->n[a], &lo[a], &hi[a], &wl[a], &wh[a]);
+    double dFg[3][3] = {{0}};
+    for (int bx = 0; bx < 2; bx++)
+        for (int by = 0; by < 2; by++)
+            for (int bz = 0; bz < 2; bz++) {
+                const int ix = bx ? hi[0] : lo[0], iy = by ? hi[1] : lo[1], iz = bz ? hi[2] : lo[2];
+                const double *c = t->FE + 4 * ((long)ix + (long)t->n[0] * ((long)iy + (long)t->n[1] * iz));
+                const double w0 = bx ? wh[0] : wl[0], w1 = by ? wh[1] : wl[1], w2 = bz ? wh[2] : wl[2];
+                const double s0 = bx ? 1 : -1, s1 = by ? 1 : -1, s2 = bz ? 1 : -1;
+                for (int i = 0; i < 3; i++) {
+                    dFg[i][0] += c[i] * s0 * w1 * w2;
+                    dFg[i][1] += c[i] * w0 * s1 * w2;
+                    dFg[i][2] += c[i] * w0 * w1 * s2;
+                }
+            }
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            J[i][j] = dFg[i][0] * t->dinv[j] + dFg[i][1] * t->dinv[3 + j] + dFg[i][2] * t->dinv[6 + j];
+    /* tip force in the tip frame (relax.cl:37-41), as orc_relax_strokes_tilted */
+    const double *A = t->R, *B = t->R + 3, *C = t->R + 6;
+    const double dp[3] = {r.x - tp.x, r.y - tp.y, r.z - tp.z};
+    const double dq[3] = {dp[0] * A[0] + dp[1] * A[1] + dp[2] * A[2], dp[0] * B[0] + dp[1] * B[1] + dp[2] * B[2],
+                          dp[0] * C[0] + dp[1] * C[1] + dp[2] * C[2]};
+    const double rl = sqrt(dq[0] * dq[0] + dq[1] * dq[1] + dq[2] * dq[2]);
+    const double sr = t->krad * (rl - t->L) / rl;
+    double ft[3];
+    for (int c = 0; c < 3; c++) ft[c] = (dq[c] - t->d0[c]) * t->k[c] + dq[c] * sr;
+    for (int c = 0; c < 3; c++) f[c] = fe[c] + (A[c] * ft[0] + B[c] * ft[1] + C[c] * ft[2]);
+    for (int c = 0; c < 3; c++) f[c] += C[c] * t->surf;
+    /* spring Jacobian R^T Dq R, Dq = diag(k + a1) + a2 q q^T */
+    const double a1 = t->krad * (1 - t->L / rl), a2 = t->krad * t->L / (rl * rl * rl);
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double s = 0;
+            for (int a = 0; a < 3; a++)
+                for (int b = 0; b < 3; b++) {
+                    const double dqab = a2 * dq[a] * dq[b] + (a == b ? a1 + t->k[a] : 0);
+                    s += t->R[3 * a + i] * dqab * t->R[3 * b + j];
+                }
+            J[i][j] += s;
+        }
+}
+
+/* Total force and Jacobian of the tilted field at npts points (tests: the
+ * finite-difference pin).  Arguments as orc_relax_strokes_tilted. */
+void orc_tilted_force_jac(const int n[3], const double *FE, const double dinv[9], const double tipRot[12],
+                          const double stiffness[4], const double dpos0[4], double surf_ff, int npts, const double *tips,
+                          const double *rs, double *f_out, double *J_out)
+{
+    tw_field t;
+    t.FE = FE;
+    for (int a = 0; a < 3; a++) t.n[a] = n[a];
+    for (int a = 0; a < 9; a++) t.dinv[a] = dinv[a];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) t.R[3 * i + j] = tipRot[4 * i + j];
+    for (int a = 0; a < 3; a++) { t.k[a] = stiffness[a]; t.d0[a] = dpos0[a]; }
+    t.krad = stiffness[3];
+    t.L = dpos0[3];
+    t.surf = surf_ff;
+    for (int i = 0; i < npts; i++) {
+        double fe[4], J[3][3];
+        tw_force_jac(&t, (v3){tips[3 * i], tips[3 * i + 1], tips[3 * i + 2]}, (v3){rs[3 * i], rs[3 * i + 1], rs[3 * i + 2]},
+                     fe, f_out + 3 * i, J);
+        for (int a = 0; a < 9; a++) J_out[9 * i + a] = J[a / 3][a % 3];
+    }
+}
+
+/* One damped Newton iteration (see newton_iter) on the tilted field; fe of
+ * the old r. */
+static int tw_newton_iter(const tw_field *t, v3 tp, v3 *r, double cap, int *stable, double *st2, double *f2, double fe[4])
+{
+    double f[3], J[3][3], dr[3];
+    tw_force_jac(t, tp, *r, fe, f, J);
+    *f2 = f[0] * f[0] + f[1] * f[1] + f[2] * f[2];
+    *stable = orc_stable3(&J[0][0]);
+    const double b[3] = {-f[0], -f[1], -f[2]};
+    if (!solve3(J, b, dr)) return 0;
+    *st2 = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
+    if (!(*st2 < 1e6)) return 0;
+    const double sc = *st2 > cap * cap ? cap / sqrt(*st2) : 1.0;
+    r->x += dr[0] * sc; r->y += dr[1] * sc; r->z += dr[2] * sc;
+    return 1;
+}
+
+/* newton_run on the tilted field: 2 = a stable iterate meets the stop rule
+ * (*q that iterate, fe its grid force), 1 = converged, 0 = failed. */
+static int tw_newton_run(const tw_field *t, v3 tp, v3 *q, int kmax, double tol, double cap, double conv_f2, int *ne,
+                         double fe[4])
+{
+    for (int k = 0; k < kmax; k++) {
+        int stable = 0;
+        double st2 = 0, f2 = 0;
+        const v3 q0 = *q;
+        const int moved = tw_newton_iter(t, tp, q, cap, &stable, &st2, &f2, fe);
+        (*ne)++;
+        if (stable && f2 < conv_f2) {
+            *q = q0;
+            return 2;
+        }
+        if (moved && stable && st2 < tol * tol) return 1;
+        if (!(moved && stable)) return 0;
+    }
+    return 0;
+}
+
+/* FIRE of relaxStrokesTilted from *r with dt, alpha carried in *dt, *damp,
+ * a Newton try every `retry` iterations (as fire_retry); `check`: *r is a
+ * Newton zero.  fe: grid force of the last evaluation that ends the step.
+ * Returns the FIRE iterations taken. */
+static int tw_fire(const tw_field *t, v3 tp, v3 *r, double *dt, double *damp, double dtmax, double dtmin, double damp0,
+                   int max_iters, double conv_f2, double f2safe, int check, int retry, int newton_max, double tol,
+                   double cap, int *ne, double fe[4])
+{
+    v3 v = {0, 0, 0};
+    int it = 0, next_try = retry;
+    while (it < max_iters) {
+        double f[3], J[3][3];
+        tw_force_jac(t, tp, *r, fe, f, J);
+        (*ne)++;
+        const double ff2 = f[0] * f[0] + f[1] * f[1] + f[2] * f[2];
+        if (check) {
+            check = 0;
+            if (ff2 < conv_f2) return it;
+        }
+        if (retry > 0 && it >= next_try) {
+            next_try = it + retry;
+            v3 q = *r;
+            int stable = 0;
+            double st2 = 0, f2 = 0, feq[4];
+            const int moved = tw_newton_iter(t, tp, &q, cap, &stable, &st2, &f2, feq);
+            if (moved && stable) {
+                int conv = st2 < tol * tol;
+                if (!conv) conv = tw_newton_run(t, tp, &q, newton_max - 1, tol, cap, conv_f2, ne, feq);
+                if (conv == 2) {
+                    *r = q;
+                    for (int c = 0; c < 4; c++) fe[c] = feq[c];
+                    return it;
+                }
+                if (conv) {
+                    *r = q;
+                    v = (v3){0, 0, 0};
+                    check = 1;
+                }
+                continue;
+            }
+        }
+        const double vv = v.x * v.x + v.y * v.y + v.z * v.z, vf = v.x * f[0] + v.y * f[1] + v.z * f[2];
+        if (vf < 0) {
+            v = (v3){0, 0, 0};
+            *dt = fmax(dtmin, *dt * 0.5);
+            *damp = damp0;
+        } else {
+            const double cf = *damp * sqrt(vv / (ff2 + f2safe)), cv = 1 - *damp;
+            v.x = v.x * cv + f[0] * cf; v.y = v.y * cv + f[1] * cf; v.z = v.z * cv + f[2] * cf;
+            *dt = fmin(dtmax, *dt * 1.1);
+            *damp *= 0.99;
+        }
+        v.x += f[0] * *dt; v.y += f[1] * *dt; v.z += f[2] * *dt;
+        r->x += v.x * *dt; r->y += v.y * *dt; r->z += v.z * *dt;
+        it++;
+        if (ff2 < conv_f2) return it;
+    }
+    return max_iters;
+}
+
+long orc_relax_strokes_tilted_warm(int npix, int nz, const int n[3], const double *FE, const double dinv[9],
+                                   const double tipRot[12], const double stiffness[4], const double dpos0[4],
+                                   const double relax_params[2], double surf_ff, int max_iters, double conv_f2,
+                                   double f2safe, const double *poss, const double *starts0, int newton_max,
+                                   int newton_max0, int retry, double tol, double cap, double *FEs, double *paths,
+                                   int *iters, int *nevals)
+{
+    tw_field t;
+    t.FE = FE;
+    for (int a = 0; a < 3; a++) t.n[a] = n[a];
+    for (int a = 0; a < 9; a++) t.dinv[a] = dinv[a];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) t.R[3 * i + j] = tipRot[4 * i + j];
+    for (int a = 0; a < 3; a++) { t.k[a] = stiffness[a]; t.d0[a] = dpos0[a]; }
+    t.krad = stiffness[3];
+    t.L = dpos0[3];
+    t.surf = surf_ff;
+    long total = 0;
+    #pragma omp parallel for schedule(dynamic, 16) reduction(+ : total)
+    for (int ip = 0; ip < npix; ip++) {
+        const double *A = tipRot, *B = tipRot + 4, *C = tipRot + 8;
+        const v3 dtip = {C[0] * C[3], C[1] * C[3], C[2] * C[3]};
+        const v3 d0 = {A[0] * dpos0[0] + B[0] * dpos0[1] + C[0] * dpos0[2],
+                       A[1] * dpos0[0] + B[1] * dpos0[1] + C[1] * dpos0[2],
+                       A[2] * dpos0[0] + B[2] * dpos0[1] + C[2] * dpos0[2]};
+        v3 tp = {poss[4 * ip], poss[4 * ip + 1], poss[4 * ip + 2]};
+        v3 r = {tp.x + d0.x, tp.y + d0.y, tp.z + d0.z};
+        double dt = relax_params[0], damp = relax_params[1];
+        const double dtmax = dt, dtmin = dtmax * 0.1, damp0 = damp;
+        for (int iz = 0; iz < nz; iz++) {
+            int ne = 0, check = 0, settled = 0, itr = 0;
+            double fe[4] = {0, 0, 0, 0};
+            const int kmax = iz == 0 ? newton_max0 : newton_max;
+            v3 q = r;
+            if (iz == 0 && starts0 && isfinite(starts0[3 * ip]) && isfinite(starts0[3 * ip + 1]) && isfinite(starts0[3 * ip + 2]))
+                q = (v3){starts0[3 * ip], starts0[3 * ip + 1], starts0[3 * ip + 2]};
+            const int res = kmax == 0 ? -1 : tw_newton_run(&t, tp, &q, kmax, tol, cap, conv_f2, &ne, fe);
+            if (res == -1) {
+                r = q;
+            } else if (res == 2) {
+                r = q;
+                settled = 1;
+            } else if (res == 1) {
+                r = q;
+                check = 1;
+            } else {
+                total++;
+            }
+            if (!settled)
+                itr = tw_fire(&t, tp, &r, &dt, &damp, dtmax, dtmin, damp0, max_iters, conv_f2, f2safe, check, retry,
+                              newton_max, tol, cap, &ne, fe);
+            const long ind = (long)ip * nz + iz;
+            FEs[4 * ind + 0] = fe[0] * A[0] + fe[1] * A[1] + fe[2] * A[2];
+            FEs[4 * ind + 1] = fe[0] * B[0] + fe[1] * B[1] + fe[2] * B[2];
+            FEs[4 * ind + 2] = fe[0] * C[0] + fe[1] * C[1] + fe[2] * C[2];
+            FEs[4 * ind + 3] = fe[3];
+            paths[4 * ind + 0] = r.x; paths[4 * ind + 1] = r.y; paths[4 * ind + 2] = r.z; paths[4 * ind + 3] = 0;
+            if (iters) iters[ind] = itr;
+            if (nevals) nevals[ind] = ne;
+            tp.x += dtip.x; tp.y += dtip.y; tp.z += dtip.z;
+            r.x += dtip.x; r.y += dtip.y; r.z += dtip.z;
+        }
+    }
+    return total;
 }
 
 /* ------------------------------------------------------------ z convolution */

@@ -78,10 +78,11 @@ class BatchItem(ctypes.Structure):
 
 
 class BatchTuning(ctypes.Structure):
-    _fields_ = [("tile_y", c_int), ("waves_per_block", c_int), ("blocks_per_cu", c_int)]
+    _fields_ = [("tile_y", c_int), ("waves_per_block", c_int), ("blocks_per_cu", c_int), ("refill", c_int)]
 
     def as_dict(self):
-        return {"tile_y": self.tile_y, "waves_per_block": self.waves_per_block, "blocks_per_cu": self.blocks_per_cu}
+        return {"tile_y": self.tile_y, "waves_per_block": self.waves_per_block, "blocks_per_cu": self.blocks_per_cu,
+                "refill": self.refill}
 
 
 class WarmParams(ctypes.Structure):
@@ -168,6 +169,7 @@ SIGNATURES = [
     ("ppafm_free", c_int, [c_void_p]),
     ("ppafm_memcpy_h2d", c_int, [c_void_p, c_void_p, c_ull, c_void_p]),
     ("ppafm_memcpy_d2h", c_int, [c_void_p, c_void_p, c_ull, c_void_p]),
+    ("ppafm_memcpy_d2d", c_int, [c_void_p, c_void_p, c_ull, c_void_p]),
     ("ppafm_memset", c_int, [c_void_p, c_int, c_ull, c_void_p]),
     ("ppafm_synchronize", c_int, [c_void_p]),
     ("ppafm_device_count", c_int, []),

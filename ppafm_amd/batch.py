@@ -109,7 +109,7 @@ class BatchScan:
     @classmethod
     def plain(cls, lvecs, ns, scans, tipR0=(0.0, 0.0, 3.0), k_lat=0.25, k_rad=30.0, iZPP=8, probe_q=-0.1, tip="s",
               amplitude=0.8, k0=1800.0, f0=30300.0, Fmax=10.0, relax=None, molecules=None, natoms=None,
-              grid_f64=True, keep_iters=False, tuning=None, stream=None, species=None):
+              grid_f64=True, keep_iters=False, tuning=None, stream=None, species=None, fp64_radius=None):
         """ProbeParticle-semantics batch (PlainScan per item): lvecs, ns and
         scans are per-item grid lattices (4,3), grid sizes and scan dicts.
         molecules: (xyzs, Zs, qs) per item (uploaded), or natoms per item and
@@ -123,7 +123,7 @@ class BatchScan:
         for k, (lvec, n, scan) in enumerate(zip(lvecs, ns, scans)):
             model, rp, tip_d, scan_d, w, df_scale = plain_setup(
                 lvec, scan, tipR0=tipR0, k_lat=k_lat, k_rad=k_rad, probe_q=probe_q, tip=tip, amplitude=amplitude,
-                k0=k0, f0=f0, Fmax=Fmax, relax=relax)
+                k0=k0, f0=f0, Fmax=Fmax, relax=relax, fp64_radius=fp64_radius)
             if shared is None:
                 shared = (model, rp, tip_d, w, df_scale)
             elif len(w) != len(shared[3]) or not np.array_equal(w, shared[3]):
@@ -261,7 +261,7 @@ class BatchScan:
         """Problem class of the autotuner cache."""
         it = self.items[0]
         s = it["scan"]
-        return f"mode{self.rp.mode}-{'f64' if self.grid_f64 else 'f32'}-scan{s.nx}x{s.ny}x{s.nz}{self._warm_key}"
+        return f"mode{self.rp.mode}-{'f64' if self.grid_f64 else 'f32'}-scan{s.nx}x{s.ny}x{s.nz}{self._warm_key}-v2"
 
     # ------------------------------------------------------------ outputs
     def _download(self, which, off_elems, shape, dtype):
@@ -361,17 +361,27 @@ class BatchScan:
         check(lib().ppafm_batch_polish(self._h, float(conv_f2), int(max_iters), out.ptr, it.ptr), "batch_polish")
         return out, it
 
-    def ws_samples(self, truth, sel, jitter=0.0, seed=0):
+    def ws_samples(self, truth, sel, jitter=0.0, seed=0, out=None, offset=0):
         """Training pairs at the global position indices ``sel`` (host int64
         array): device arrays patches (n, 232) bf16 and targets (n, 3); the
-        patch centre is moved by up to ``jitter`` A (half along z)."""
+        patch centre is moved by up to ``jitter`` A (half along z), by a hash
+        of the position index and the seed.  ``out`` = (P, T): write into
+        those device arrays from row ``offset`` instead."""
         sel = np.ascontiguousarray(sel, dtype=np.int64)
         if len(sel) and (sel.min() < 0 or sel.max() >= self.total_positions()):
             raise ValueError("sample index out of range")
         sd = D.DeviceArray.from_host(sel if len(sel) else np.zeros(1, np.int64))
-        P = D.DeviceArray((max(len(sel), 1), 232), np.uint16)
-        T = D.DeviceArray((max(len(sel), 1), 3), np.float32)
-        check(lib().ppafm_batch_ws_samples(self._h, truth.ptr, len(sel), sd.ptr, P.ptr, T.ptr, float(jitter), int(seed) & 0xFFFFFFFF), "batch_ws_samples")
+        if out is None:
+            P = D.DeviceArray((max(len(sel), 1), 232), np.uint16)
+            T = D.DeviceArray((max(len(sel), 1), 3), np.float32)
+            offset = 0
+        else:
+            P, T = out
+            if offset < 0 or offset + len(sel) > P.shape[0] or P.shape[0] != T.shape[0]:
+                raise ValueError("samples do not fit the output arrays")
+        check(lib().ppafm_batch_ws_samples(self._h, truth.ptr, len(sel), sd.ptr, P.ptr + 464 * int(offset),
+                                           T.ptr + 12 * int(offset), float(jitter), int(seed) & 0xFFFFFFFF),
+              "batch_ws_samples")
         self.synchronize()
         return P, T
 
@@ -425,7 +435,7 @@ def tuned(batch, nsample=32, reps=2, path=None):
     cache = _load_cache(path)
     hit = cache.get(key)
     if isinstance(hit, dict) and {"tile_y", "waves_per_block", "blocks_per_cu"} <= set(hit):
-        return {k: int(hit[k]) for k in ("tile_y", "waves_per_block", "blocks_per_cu")}
+        return {k: int(hit.get(k, 0)) for k in ("tile_y", "waves_per_block", "blocks_per_cu", "refill")}
     best, ms = batch.tune(nsample=nsample, reps=reps)
     cache[key] = dict(best, ms=ms, nsample=int(min(nsample, batch.nitems)))
     try:

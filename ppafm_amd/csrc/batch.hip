@@ -62,7 +62,9 @@ struct ppafm_batch {
     std::vector<double> w;
     double df_scale = 1;
     bool keep_iters = false;
-    ppafm_batch_tuning tuning{8, 1, 16};
+    ppafm_batch_tuning tuning{8, 1, 16, 0};
+    bool homog = false;  // every item shares grid shape, cell and scan (lane refill possible)
+    std::vector<BatchRelaxItem> ritems;  // host copy of the relaxation table
     hipStream_t st = nullptr;
     bool own_stream = false;
     int ncu = 1;
@@ -136,6 +138,24 @@ void relax_tables(const ppafm_batch &b, int n, int tile_y, std::vector<BatchRela
     }
 }
 
+// Every item shares grid shape, cell and scan (only offsets differ): the
+// relaxation can refill lanes column by column with one set of uniforms.
+bool homogeneous(const std::vector<BatchRelaxItem> &it)
+{
+    if (it.empty()) return false;
+    const BatchRelaxItem &a = it[0];
+    if (a.nz <= 0) return false;
+    for (const BatchRelaxItem &r : it) {
+        if (r.nx != a.nx || r.ny != a.ny || r.nz != a.nz || r.tiles_y != a.tiles_y) return false;
+        for (int k = 0; k < 3; k++)
+            if (r.n[k] != a.n[k] || r.p0[k] != a.p0[k] || r.da[k] != a.da[k] || r.db[k] != a.db[k] || r.dstep[k] != a.dstep[k])
+                return false;
+        for (int k = 0; k < 9; k++)
+            if (r.dinv[k] != a.dinv[k]) return false;
+    }
+    return true;
+}
+
 int enqueue_grid(ppafm_batch &b, int nitems, long long nbricks)
 {
     bool ortho = true;
@@ -145,9 +165,11 @@ int enqueue_grid(ppafm_batch &b, int nitems, long long nbricks)
 }
 
 int enqueue_relax(ppafm_batch &b, const ppafm_batch_tuning &t, const BatchRelaxItem *items, const long long *prefix,
-                  int nitems, long long ntiles)
+                  int nitems, long long ntiles, const std::vector<BatchRelaxItem> &hitems)
 {
     RelaxBatchLaunch L;
+    L.refill = t.refill == 1 && homogeneous(hitems);
+    if (L.refill) L.item0 = hitems[0];
     L.rp = b.rp;
     L.tip = b.tip;
     L.items = items;
@@ -236,7 +258,7 @@ int enqueue_stages(ppafm_batch &b, int stages)
         if (e) return e;
     }
     if (stages & (2 | 16)) {
-        e = enqueue_relax(b, b.tuning, b.d_ritems, b.d_tprefix, b.nitems, b.ntiles);
+        e = enqueue_relax(b, b.tuning, b.d_ritems, b.d_tprefix, b.nitems, b.ntiles, b.ritems);
         if (e) return e;
     }
     if ((stages & 4) && !b.w.empty()) {
@@ -280,6 +302,7 @@ int apply_tuning(ppafm_batch &b)
     std::vector<BatchRelaxItem> it;
     std::vector<long long> prefix;
     relax_tables(b, b.nitems, b.tuning.tile_y, it, prefix);
+    b.homog = homogeneous(it);
     PP_TRY_HIP(hipStreamSynchronize(b.st));  // no run may still read the old tables
     drop_graph(b);
     if (b.d_ritems) { (void)hipFree(b.d_ritems); b.d_ritems = nullptr; }
@@ -289,6 +312,7 @@ int apply_tuning(ppafm_batch &b)
     e = upload(b.d_tprefix, prefix, b.st);
     if (e) return e;
     b.ntiles = prefix[b.nitems];
+    b.ritems = std::move(it);
     PP_TRY_HIP(hipStreamSynchronize(b.st));
     return capture(b);
 }
@@ -299,6 +323,7 @@ int check_tuning(const ppafm_batch_tuning &t)
                "tile_y must be 1, 2, 4, 8 or 16");
     PP_REQUIRE(t.waves_per_block >= 1 && t.waves_per_block <= 4, "waves_per_block must be 1..4");
     PP_REQUIRE(t.blocks_per_cu >= 1 && t.blocks_per_cu <= 64, "blocks_per_cu must be 1..64");
+    PP_REQUIRE(t.refill == 0 || t.refill == 1, "refill must be 0 or 1");
     return 0;
 }
 
@@ -552,14 +577,16 @@ int ppafm_batch_tune(ppafm_batch *b, int nsample, int reps, ppafm_batch_tuning *
         if (b->d_sitems) { (void)hipFree(b->d_sitems); b->d_sitems = nullptr; }
         if (b->d_sprefix) { (void)hipFree(b->d_sprefix); b->d_sprefix = nullptr; }
         if ((rc = upload(b->d_sitems, it, b->st)) || (rc = upload(b->d_sprefix, prefix, b->st))) break;
+        const int nrf = homogeneous(it) ? 2 : 1;  // lane refill only for one shared geometry
+        for (int rf = 0; rf < nrf; rf++)
         for (int wpb : wpbs)
             for (int wpc : wpcs) {
-                const ppafm_batch_tuning t{ty, wpb, std::max(1, wpc / wpb)};
+                const ppafm_batch_tuning t{ty, wpb, std::max(1, wpc / wpb), rf};
                 double tot = 0;
                 for (int r = 0; r <= reps && rc == 0; r++) {
                     if ((rc = launch_fill_u32(b->ctl, 12, 0u, b->st))) break;
                     if (hipEventRecord(e0, b->st) != hipSuccess) { rc = -1; break; }
-                    if ((rc = enqueue_relax(*b, t, b->d_sitems, b->d_sprefix, ns, prefix[ns]))) break;
+                    if ((rc = enqueue_relax(*b, t, b->d_sitems, b->d_sprefix, ns, prefix[ns], it))) break;
                     if (hipEventRecord(e1, b->st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
                         set_error("event failure while tuning");
                         rc = -1;

@@ -808,6 +808,14 @@ int ppafm_memcpy_d2h(void *dst, const void *src, unsigned long long bytes, void 
     return 0;
 }
 
+int ppafm_memcpy_d2d(void *dst, const void *src, unsigned long long bytes, void *stream)
+{
+    if (!bytes) return 0;
+    if (stream) PP_TRY_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, as_stream(stream)));
+    else PP_TRY_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    return 0;
+}
+
 int ppafm_memset(void *dst, int value, unsigned long long bytes, void *stream)
 {
     if (!bytes) return 0;

@@ -468,6 +468,10 @@ static bool model_args(const ppafm_ff_model &m, FFArgs &A)
         }
     for (int s = ns; s < 4; s++) { A.sw[s] = 0.f; A.sdz[s] = 0.f; }
     A.ns = ns;
+    double ws = 0, wdz = 0;
+    for (int s = 0; s < ns; s++) { ws += (double)A.sw[s]; wdz += (double)A.sw[s] * (double)A.sdz[s]; }
+    A.swsum = (float)ws;
+    A.swdz = (float)wdz;
     A.mode = m.mode;
     A.r2safe_d = m.mode == 0 ? (double)1.0e-8f : (double)1.0e-6f;  // Forces.h:12 / FF.cl:1
     A.r2safe = (float)A.r2safe_d;

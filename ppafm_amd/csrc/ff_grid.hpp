@@ -26,6 +26,7 @@ struct FFArgs {
     int ns;
     double dcell[9];
     float sw[4], sdz[4];
+    float swsum, swdz;  // sum of the stencil weights, of weight x z offset
     float r2safe;
     double r2safe_d;
     float near2;  // pairs with |d|^2 below this go to FP64

@@ -10,6 +10,17 @@
 //   * One v_rsq_f32 per pair serves the LJ term and the Coulomb point at
 //     zero z offset; orthogonal cells share dx, dy and dx^2 + dy^2 + R2SAFE
 //     across a lane's 4 z points.
+//   * Coulomb stencils with z offsets (tip charges Qs at QZs, or the pz /
+//     dz2 probe kinds) are finite differences whose terms cancel by orders
+//     of magnitude (the dz2-like -10/+20/-10 tip: ~10^3).  They are summed
+//     relative to the zero-offset distance r0 without forming the
+//     differences in floating point: with W = sum w_s,
+//       sum w_s / r_s   = W / r0   + sum w_s (1/r_s - 1/r0),
+//       1/r_s - 1/r0    = (r0^2 - r_s^2) / (r0 r_s (r0 + r_s)),
+//       r0^2 - r_s^2    = -dz_s (2 dz + dz_s)            (exact offsets),
+//       1/r_s^3 - 1/r0^3 = (1/r_s - 1/r0)(1/r_s^2 + 1/(r_s r0) + 1/r0^2),
+//     and the z offsets' own term sum w_s dz_s / r_s^3 the same way, so the
+//     FP32 rounding is relative to the stencil's sum, not to its terms.
 //   * The OpenCL model (mode 1) sums LJ and Coulomb into one force scale per
 //     pair (one set of 3 FMAs); the ProbeParticle model (mode 0) keeps the
 //     two fields apart for their separate clamps (io.py:766-778).
@@ -109,21 +120,43 @@ __device__ __forceinline__ void brick_f32(const FFArgs &A, const AtomRec *__rest
                     const float dz = (pz[k] - hi.z) - lo.z;
                     const float r2 = ORTHO ? __fmaf_rn(dz, dz, dxy2) : r2_f32(dx, dy, dz, A.r2safe);
                     float ir0 = 0.f, ir20 = 0.f;
-                    if (LJ || Z0) {
+                    if (LJ || NS > 0) {
                         ir0 = __builtin_amdgcn_rsqf(r2);
                         ir20 = ir0 * ir0;
                     }
-                    // s: force over distance at zero z offset; e: energy
-                    float s = 0.f, e = 0.f, sc = 0.f, ec = 0.f;
+                    // s: force over distance at zero z offset; e: energy;
+                    // sc, ec, fzc: the Coulomb stencil's force scale, energy
+                    // and extra z force
+                    float s = 0.f, e = 0.f, sc = 0.f, ec = 0.f, fzc = 0.f;
                     if (LJ) {
                         const float ir6 = ir20 * ir20 * ir20;
                         const float ir8 = ir6 * ir20;
                         s = __fmaf_rn(lj.y, ir6, -lj.x) * ir8;
                         if (WE) e = __fmaf_rn(lj.w, ir6, -lj.z) * ir6;
                     }
-                    if (Z0 && charged) {
+                    if (charged && NS == 1 && Z0) {  // one point at zero offset
                         ec = (hi.w * A.sw[0]) * ir0;
                         sc = ec * ir20;
+                    } else if (charged) {  // stencil with offsets, cancellation free
+                        const float r0 = r2 * ir0;
+                        float s1 = 0.f, s3 = 0.f, s3z = 0.f;
+#pragma unroll
+                        for (int s_ = Z0 ? 1 : 0; s_ < NS; s_++) {
+                            const float dzs = dz + A.sdz[s_];
+                            const float r2s = ORTHO ? __fmaf_rn(dzs, dzs, dxy2) : r2_f32(dx, dy, dzs, A.r2safe);
+                            const float irs = __builtin_amdgcn_rsqf(r2s);
+                            const float rs = r2s * irs;
+                            const float dr2 = -A.sdz[s_] * (dz + dz + A.sdz[s_]);  // r0^2 - rs^2
+                            const float d1 = (dr2 * ir0) * (irs * __builtin_amdgcn_rcpf(r0 + rs));
+                            const float d3 = d1 * __fmaf_rn(irs, irs, __fmaf_rn(irs, ir0, ir20));
+                            s1 = __fmaf_rn(A.sw[s_], d1, s1);
+                            s3 = __fmaf_rn(A.sw[s_], d3, s3);
+                            s3z = __fmaf_rn(A.sw[s_] * A.sdz[s_], d3, s3z);
+                        }
+                        const float ir30 = ir0 * ir20;
+                        ec = hi.w * __fmaf_rn(A.swsum, ir0, s1);
+                        sc = hi.w * __fmaf_rn(A.swsum, ir30, s3);
+                        fzc = hi.w * __fmaf_rn(A.swdz, ir30, s3z);
                     }
                     if (SEP) {
                         if (LJ) {
@@ -132,33 +165,18 @@ __device__ __forceinline__ void brick_f32(const FFArgs &A, const AtomRec *__rest
                             sL[k][2] = __fmaf_rn(dz, s, sL[k][2]);
                             if (WE) sL[k][3] += e;
                         }
-                        if (Z0 && charged) {
+                        if (charged) {
                             sE[k][0] = __fmaf_rn(dx, sc, sE[k][0]);
                             sE[k][1] = __fmaf_rn(dy, sc, sE[k][1]);
-                            sE[k][2] = __fmaf_rn(dz, sc, sE[k][2]);
+                            sE[k][2] = __fmaf_rn(dz, sc, NS == 1 && Z0 ? sE[k][2] : sE[k][2] + fzc);
                             if (WE) sE[k][3] += ec;
                         }
-                    } else if (LJ || (Z0 && charged)) {
+                    } else if (LJ || charged) {
                         const float st = s + sc;
                         sL[k][0] = __fmaf_rn(dx, st, sL[k][0]);
                         sL[k][1] = __fmaf_rn(dy, st, sL[k][1]);
-                        sL[k][2] = __fmaf_rn(dz, st, sL[k][2]);
+                        sL[k][2] = __fmaf_rn(dz, st, NS == 1 && Z0 ? sL[k][2] : sL[k][2] + fzc);
                         if (WE) sL[k][3] += e + ec;
-                    }
-                    // the other stencil points (z offsets sdz)
-                    if (charged) {
-#pragma unroll
-                        for (int s_ = Z0 ? 1 : 0; s_ < NS; s_++) {
-                            const float dzs = dz + A.sdz[s_];
-                            const float ir = __builtin_amdgcn_rsqf(ORTHO ? __fmaf_rn(dzs, dzs, dxy2) : r2_f32(dx, dy, dzs, A.r2safe));
-                            const float es = (hi.w * A.sw[s_]) * ir;
-                            const float ss = es * (ir * ir);
-                            float *acc = SEP ? sE[k] : sL[k];
-                            acc[0] = __fmaf_rn(dx, ss, acc[0]);
-                            acc[1] = __fmaf_rn(dy, ss, acc[1]);
-                            acc[2] = __fmaf_rn(dzs, ss, acc[2]);
-                            if (WE) acc[3] += es;
-                        }
                     }
                 }
             }

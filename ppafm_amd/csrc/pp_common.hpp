@@ -190,6 +190,10 @@ struct RelaxBatchLaunch {
     int tile_y;           // pixel tile: (64 / tile_y) x tile_y columns
     int waves_per_block;  // 1..4
     int nblocks;          // persistent workgroups
+    // lane refill (every item of item0's geometry): lanes take pixel
+    // columns one by one in tile order from per-XCD column queues
+    bool refill = false;
+    BatchRelaxItem item0{};
     // warm starts (relax_column_warm): on when `warm`; pred = step-0
     // estimates per pixel column (3 floats) or nullptr
     bool warm = false;

@@ -25,205 +25,253 @@ namespace pp {
 
 namespace {
 
-// One pixel column (i, j): every z step of the stroke, FIRE to convergence,
-// outputs in stroke order.  Counts force evaluations and finished positions.
-template <typename T4, int MODE>
-__device__ __forceinline__ void relax_column(const DevArgs &A, const T4 *__restrict__ grid, int i, int j, bool live,
-                                             unsigned long long &nevals, unsigned long long &npos)
+// The state of one pixel column's relaxation, advanced one force
+// evaluation per step: the stroke's z step s, the FIRE state and, with warm
+// starts, the Newton phase.  A column is done when s reaches nz.  The plain
+// and warm kernels, per tile or with lane refill, all run these functions,
+// so a column's results do not depend on the lane, wave or launch shape.
+template <typename T4>
+struct Column {
+    Stroke c;
+    long obase;  // output offset of the column's item (0: pre-offset outputs)
+    double tx, ty, tz;
+    Fire S;
+    int s, it, itr;
+    bool sample;  // mode 0: next gather is the output sample
+    CellCache<T4> cache;
+    // warm starts: Newton iterate q, iterations taken and allowed, FIRE
+    // iteration of the next Newton try; `opening` = the step's first Newton
+    // run (its failure is a fallback); `check` = S.r is a Newton zero not yet
+    // evaluated by FIRE
+    double qx, qy, qz;
+    int next_try, nk, kmax;
+    bool newton, opening, check;
+};
+
+// Step 0 of column (i, j) (nothing to do when !live).  pred: the column's
+// step-0 estimate (3 floats) or nullptr.
+template <typename T4, int MODE, bool WARM>
+__device__ __forceinline__ void column_init(const DevArgs &A, Column<T4> &C, int i, int j, bool live, long obase,
+                                            const float *pred)
 {
-    const int nz = A.nz;
-    const Stroke c = stroke_at(A, i, j);
-    double tx = 0, ty = 0, tz = 0;
-    int s = live ? 0 : nz;
-    if (live) tip_pos(A, c, 0, live, tx, ty, tz);
+    C.c = stroke_at(A, i, j);
+    C.obase = obase;
+    C.tx = C.ty = C.tz = 0;
+    C.s = live ? 0 : A.nz;
+    if (live) tip_pos(A, C.c, 0, live, C.tx, C.ty, C.tz);
     double ox, oy, oz;  // equilibrium offset in the world frame
     probe_offset<MODE>(A, ox, oy, oz);
-    Fire S{tx + ox, ty + oy, tz + oz, 0, 0, 0, A.dtmax, A.a0};
-    int it = 0, itr = 0;
-    bool sample = false;  // mode 0: next gather is the output sample
-    CellCache<T4> cache;
+    C.S = Fire{C.tx + ox, C.ty + oy, C.tz + oz, 0, 0, 0, A.dtmax, A.a0};
+    C.it = 0;
+    C.itr = 0;
+    C.sample = false;
+    C.cache.ix = C.cache.iy = C.cache.iz = -1;
+    if (WARM) {
+        C.next_try = A.newton_retry;
+        C.qx = C.S.rx; C.qy = C.S.ry; C.qz = C.S.rz;
+        if (pred && live && isfinite(pred[0]) && isfinite(pred[1]) && isfinite(pred[2])) {  // else the plain start
+            C.qx = (double)pred[0]; C.qy = (double)pred[1]; C.qz = (double)pred[2];
+        }
+        C.nk = 0;
+        C.kmax = A.newton_max0;
+        C.newton = live && C.kmax > 0;
+        C.opening = C.newton;
+        C.check = false;
+        if (live && !C.newton && pred) { C.S.rx = C.qx; C.S.ry = C.qy; C.S.rz = C.qz; }  // the estimate as it is
+    }
+}
 
-    while (s < nz) {
-        double fe[4], gx, gy, gz;
-        to_grid(A, S.rx, S.ry, S.rz, gx, gy, gz);
-        gather_cached(grid, A, gx, gy, gz, fe, MODE == 1, cache);
-        nevals++;
-        bool done = false;
-        const long o = c.col0 + s;
-        if (MODE == 0 && sample) {
-            store_pos(A, o, S.rx, S.ry, S.rz);
-            if (A.force) A.force[o] = make_float4((float)fe[0], (float)fe[1], (float)fe[2], 0.f);
-            if (A.iters) A.iters[o] = itr;
-            done = true;
-        } else {
-            const double ff = fire_move<MODE>(A, fe, tx, ty, tz, S);
-            it++;
-            const bool conv = ff < A.conv_f2;
-            if (conv || it >= A.max_iters) {
-                if (MODE == 0) {
-                    itr = conv ? it - 1 : A.max_iters;
-                    sample = true;
-                } else {
-                    if (A.force) A.force[o] = tip_frame_force(A, fe);
-                    store_pos(A, o, S.rx, S.ry, S.rz);
-                    if (A.iters) A.iters[o] = it;
-                    done = true;
-                }
+// One force evaluation of a plain column: FIRE to convergence at each z
+// step, outputs in stroke order.  Counts force evaluations and finished
+// positions.
+template <typename T4, int MODE>
+__device__ __forceinline__ void column_step(const DevArgs &A, const T4 *__restrict__ grid, Column<T4> &C,
+                                            unsigned long long &nevals, unsigned long long &npos)
+{
+    double fe[4], gx, gy, gz;
+    to_grid(A, C.S.rx, C.S.ry, C.S.rz, gx, gy, gz);
+    gather_cached(grid, A, gx, gy, gz, fe, MODE == 1, C.cache);
+    nevals++;
+    bool done = false;
+    const long o = C.obase + C.c.col0 + C.s;
+    if (MODE == 0 && C.sample) {
+        store_pos(A, o, C.S.rx, C.S.ry, C.S.rz);
+        if (A.force) A.force[o] = make_float4((float)fe[0], (float)fe[1], (float)fe[2], 0.f);
+        if (A.iters) A.iters[o] = C.itr;
+        done = true;
+    } else {
+        const double ff = fire_move<MODE>(A, fe, C.tx, C.ty, C.tz, C.S);
+        C.it++;
+        const bool conv = ff < A.conv_f2;
+        if (conv || C.it >= A.max_iters) {
+            if (MODE == 0) {
+                C.itr = conv ? C.it - 1 : A.max_iters;
+                C.sample = true;
+            } else {
+                if (A.force) A.force[o] = tip_frame_force(A, fe);
+                store_pos(A, o, C.S.rx, C.S.ry, C.S.rz);
+                if (A.iters) A.iters[o] = C.it;
+                done = true;
             }
         }
-        if (done) {
-            npos++;
-            s++;
-            if (s < nz) {
-                next_step<MODE>(A, c, s, live, tx, ty, tz, S);
-                it = 0;
-                sample = false;
-            }
+    }
+    if (done) {
+        npos++;
+        C.s++;
+        if (C.s < A.nz) {
+            next_step<MODE>(A, C.c, C.s, true, C.tx, C.ty, C.tz, C.S);
+            C.it = 0;
+            C.sample = false;
         }
     }
 }
 
-// The same column with warm starts.  Every step begins with Newton
-// iterations on the interpolated field (newton_from) from the continuation
-// start the reference uses (step 0: from the column's estimate A.pred, or
-// the plain start without one), each step cut to at most newton_cap.  When
-// they reach a stable zero (a full step shorter than sqrt(newton_tol2) with
-// a stable Jacobian at every iterate, at most newton_max of them, newton_max0
-// at step 0), FIRE runs from there; otherwise (an unstable Jacobian, a
-// non-finite step, no convergence) FIRE runs from the continuation start
-// itself, exactly as without warm starts (counted in nfb).  While FIRE runs,
-// every newton_retry of its iterations one Newton step is tried from its
-// current position with the corners FIRE just loaded; where the Jacobian is
-// stable Newton iterations take over as above, and on convergence FIRE
-// restarts there (velocity zero; mode 0 also resets dt and alpha), its
-// iteration count running on toward max_iters.  When FIRE's first
-// evaluation at a Newton zero already meets the stop rule, the step ends at
-// that zero without a move (output force: the grid force there).  Either way
-// FIRE ends the step by the reference's stop rule, and the next step
-// continues from FIRE's result.  Newton iterates and FIRE moves share one 8-corner load per
-// loop pass.  Newton evaluations count in nevals.
+// One force evaluation of a warm-started column.  Every step begins with
+// Newton iterations on the interpolated field (newton_from) from the
+// continuation start the reference uses (step 0: from the column's estimate,
+// or the plain start without one), each step cut to at most newton_cap.
+// When they reach a stable zero (a full step shorter than sqrt(newton_tol2)
+// with a stable Jacobian at every iterate, at most newton_max of them,
+// newton_max0 at step 0), FIRE runs from there; otherwise (an unstable
+// Jacobian, a non-finite step, no convergence) FIRE runs from the
+// continuation start itself, exactly as without warm starts (counted in
+// nfb).  While FIRE runs, every newton_retry of its iterations one Newton
+// step is tried from its current position with the corners FIRE just
+// loaded; where the Jacobian is stable Newton iterations take over as above,
+// and on convergence FIRE restarts there (velocity zero; mode 0 also resets
+// dt and alpha), its iteration count running on toward max_iters.  When a
+// Newton iterate (Jacobian stable) or FIRE's first evaluation at a Newton
+// zero already meets the stop rule, the step ends at that point without a
+// move (output force: the grid force there).  Either way FIRE ends the step by the reference's stop rule, and
+// the next step continues from FIRE's result.  Newton iterates and FIRE
+// moves share one 8-corner load per evaluation; Newton evaluations count in
+// nevals.
+template <typename T4, int MODE>
+__device__ __forceinline__ void column_step_warm(const DevArgs &A, const T4 *__restrict__ grid, Column<T4> &C,
+                                                 unsigned long long &nevals, unsigned long long &npos,
+                                                 unsigned long long &nfb)
+{
+    double gx, gy, gz, ux, uy, uz;
+    to_grid(A, C.newton ? C.qx : C.S.rx, C.newton ? C.qy : C.S.ry, C.newton ? C.qz : C.S.rz, gx, gy, gz);
+    T4 v[8];
+    corners(grid, A, gx, gy, gz, v, ux, uy, uz);
+    nevals++;
+    bool done = false;
+    const long o = C.obase + C.c.col0 + C.s;
+    const bool retry = !C.newton && !(MODE == 0 && C.sample) && A.newton_retry > 0 && C.it >= C.next_try;
+    bool fire = !C.newton;
+    double fe[4];
+    interp8(v, ux, uy, uz, fe, MODE == 1);
+    if (C.newton || retry) {
+        double px = C.newton ? C.qx : C.S.rx, py = C.newton ? C.qy : C.S.ry, pz = C.newton ? C.qz : C.S.rz;
+        bool stable;
+        double st2, f2;
+        const bool ok =
+            newton_from<T4, MODE>(A, v, ux, uy, uz, fe, C.tx, C.ty, C.tz, px, py, pz, A.newton_cap, stable, st2, f2) && stable;
+        if (retry) C.next_try = C.it + A.newton_retry;
+        if (C.newton && stable && f2 < A.conv_f2) {  // the iterate meets the stop rule: the step ends here
+            fire = false;
+            C.S.rx = C.qx; C.S.ry = C.qy; C.S.rz = C.qz;
+            store_pos(A, o, C.qx, C.qy, C.qz);
+            if (A.force)
+                A.force[o] = MODE == 0 ? make_float4((float)fe[0], (float)fe[1], (float)fe[2], 0.f) : tip_frame_force(A, fe);
+            if (A.iters) A.iters[o] = C.it;
+            C.newton = false;
+            C.opening = false;
+            done = true;
+        } else if (ok || C.newton) {
+            fire = false;
+            C.nk = C.newton ? C.nk + 1 : 1;
+            if (!C.newton) C.kmax = A.newton_max;
+            C.qx = px; C.qy = py; C.qz = pz;
+            if (ok && st2 < A.newton_tol2) {  // converged: FIRE from here
+                C.S.rx = C.qx; C.S.ry = C.qy; C.S.rz = C.qz;
+                C.S.vx = C.S.vy = C.S.vz = 0;
+                if (MODE == 0) { C.S.dt = A.dtmax; C.S.ac = A.a0; }
+                C.newton = false;
+                C.opening = false;
+                C.check = true;
+            } else if (!ok || C.nk >= C.kmax) {  // FIRE from where it was
+                if (C.opening) nfb++;
+                C.newton = false;
+                C.opening = false;
+            } else {
+                C.newton = true;
+            }
+        }
+    }
+    if (fire) {
+        if (MODE == 0 && C.sample) {
+            store_pos(A, o, C.S.rx, C.S.ry, C.S.rz);
+            if (A.force) A.force[o] = make_float4((float)fe[0], (float)fe[1], (float)fe[2], 0.f);
+            if (A.iters) A.iters[o] = C.itr;
+            done = true;
+        } else {
+            double fx, fy, fz;
+            total_force<MODE>(A, fe, C.tx, C.ty, C.tz, C.S.rx, C.S.ry, C.S.rz, fx, fy, fz);
+            const double f2 = fma(fx, fx, fma(fy, fy, fz * fz));
+            const bool settled = C.check && f2 < A.conv_f2;
+            C.check = false;
+            if (settled) {  // Newton's zero meets the stop rule: the step ends here, no move
+                store_pos(A, o, C.S.rx, C.S.ry, C.S.rz);
+                if (A.force)
+                    A.force[o] = MODE == 0 ? make_float4((float)fe[0], (float)fe[1], (float)fe[2], 0.f)
+                                           : tip_frame_force(A, fe);
+                if (A.iters) A.iters[o] = C.it;
+                done = true;
+            } else {
+                const double ff = fire_update<MODE>(A, fx, fy, fz, C.S);
+                C.it++;
+                const bool conv = ff < A.conv_f2;
+                if (conv || C.it >= A.max_iters) {
+                    if (MODE == 0) {
+                        C.itr = conv ? C.it - 1 : A.max_iters;
+                        C.sample = true;
+                    } else {
+                        if (A.force) A.force[o] = tip_frame_force(A, fe);
+                        store_pos(A, o, C.S.rx, C.S.ry, C.S.rz);
+                        if (A.iters) A.iters[o] = C.it;
+                        done = true;
+                    }
+                }
+            }
+        }
+    }
+    if (done) {
+        npos++;
+        C.s++;
+        if (C.s < A.nz) {
+            next_step<MODE>(A, C.c, C.s, true, C.tx, C.ty, C.tz, C.S);
+            C.it = 0;
+            C.next_try = A.newton_retry;
+            C.sample = false;
+            C.qx = C.S.rx; C.qy = C.S.ry; C.qz = C.S.rz;
+            C.nk = 0;
+            C.kmax = A.newton_max;
+            C.newton = C.opening = C.kmax > 0;
+            C.check = false;
+        }
+    }
+}
+
+// One pixel column (i, j) from start to end.
+template <typename T4, int MODE>
+__device__ __forceinline__ void relax_column(const DevArgs &A, const T4 *__restrict__ grid, int i, int j, bool live,
+                                             unsigned long long &nevals, unsigned long long &npos)
+{
+    Column<T4> C;
+    column_init<T4, MODE, false>(A, C, i, j, live, 0, nullptr);
+    while (C.s < A.nz) column_step<T4, MODE>(A, grid, C, nevals, npos);
+}
+
 template <typename T4, int MODE>
 __device__ __forceinline__ void relax_column_warm(const DevArgs &A, const T4 *__restrict__ grid, int i, int j,
                                                   bool live, unsigned long long &nevals, unsigned long long &npos,
                                                   unsigned long long &nfb)
 {
-    const int nz = A.nz;
-    const Stroke c = stroke_at(A, i, j);
-    double tx = 0, ty = 0, tz = 0;
-    int s = live ? 0 : nz;
-    if (live) tip_pos(A, c, 0, live, tx, ty, tz);
-    double ox, oy, oz;
-    probe_offset<MODE>(A, ox, oy, oz);
-    Fire S{tx + ox, ty + oy, tz + oz, 0, 0, 0, A.dtmax, A.a0};
-    int it = 0, itr = 0, next_try = A.newton_retry;
-    bool sample = false;
-    // Newton phase: iterate q, iterations taken and allowed; `opening` = the
-    // step's first Newton run (its failure is a fallback)
-    double qx = S.rx, qy = S.ry, qz = S.rz;
-    if (A.pred && live) {
-        const float *p = A.pred + 3 * ((long)i * A.ny + j);
-        if (isfinite(p[0]) && isfinite(p[1]) && isfinite(p[2])) {  // else the plain start
-            qx = (double)p[0]; qy = (double)p[1]; qz = (double)p[2];
-        }
-    }
-    int nk = 0, kmax = A.newton_max0;
-    bool newton = live && kmax > 0, opening = newton;
-    bool check = false;  // S.r is a Newton zero not yet evaluated by FIRE
-    if (live && !newton && A.pred) { S.rx = qx; S.ry = qy; S.rz = qz; }  // the estimate as it is
-
-    while (s < nz) {
-        double gx, gy, gz, ux, uy, uz;
-        to_grid(A, newton ? qx : S.rx, newton ? qy : S.ry, newton ? qz : S.rz, gx, gy, gz);
-        T4 v[8];
-        corners(grid, A, gx, gy, gz, v, ux, uy, uz);
-        nevals++;
-        bool done = false;
-        const long o = c.col0 + s;
-        const bool retry = !newton && !(MODE == 0 && sample) && A.newton_retry > 0 && it >= next_try;
-        bool fire = !newton;
-        double fe[4];
-        interp8(v, ux, uy, uz, fe, MODE == 1);
-        if (newton || retry) {
-            double px = newton ? qx : S.rx, py = newton ? qy : S.ry, pz = newton ? qz : S.rz;
-            bool stable;
-            double st2;
-            const bool ok =
-                newton_from<T4, MODE>(A, v, ux, uy, uz, fe, tx, ty, tz, px, py, pz, A.newton_cap, stable, st2) && stable;
-            if (retry) next_try = it + A.newton_retry;
-            if (ok || newton) {
-                fire = false;
-                nk = newton ? nk + 1 : 1;
-                if (!newton) kmax = A.newton_max;
-                qx = px; qy = py; qz = pz;
-                if (ok && st2 < A.newton_tol2) {  // converged: FIRE from here
-                    S.rx = qx; S.ry = qy; S.rz = qz;
-                    S.vx = S.vy = S.vz = 0;
-                    if (MODE == 0) { S.dt = A.dtmax; S.ac = A.a0; }
-                    newton = false;
-                    opening = false;
-                    check = true;
-                } else if (!ok || nk >= kmax) {  // FIRE from where it was
-                    if (opening) nfb++;
-                    newton = false;
-                    opening = false;
-                } else {
-                    newton = true;
-                }
-            }
-        }
-        if (fire) {
-            if (MODE == 0 && sample) {
-                store_pos(A, o, S.rx, S.ry, S.rz);
-                if (A.force) A.force[o] = make_float4((float)fe[0], (float)fe[1], (float)fe[2], 0.f);
-                if (A.iters) A.iters[o] = itr;
-                done = true;
-            } else {
-                double fx, fy, fz;
-                total_force<MODE>(A, fe, tx, ty, tz, S.rx, S.ry, S.rz, fx, fy, fz);
-                const double f2 = fma(fx, fx, fma(fy, fy, fz * fz));
-                const bool settled = check && f2 < A.conv_f2;
-                check = false;
-                if (settled) {  // Newton's zero meets the stop rule: the step ends here, no move
-                    store_pos(A, o, S.rx, S.ry, S.rz);
-                    if (A.force)
-                        A.force[o] = MODE == 0 ? make_float4((float)fe[0], (float)fe[1], (float)fe[2], 0.f)
-                                               : tip_frame_force(A, fe);
-                    if (A.iters) A.iters[o] = it;
-                    done = true;
-                } else {
-                    const double ff = fire_update<MODE>(A, fx, fy, fz, S);
-                    it++;
-                    const bool conv = ff < A.conv_f2;
-                    if (conv || it >= A.max_iters) {
-                        if (MODE == 0) {
-                            itr = conv ? it - 1 : A.max_iters;
-                            sample = true;
-                        } else {
-                            if (A.force) A.force[o] = tip_frame_force(A, fe);
-                            store_pos(A, o, S.rx, S.ry, S.rz);
-                            if (A.iters) A.iters[o] = it;
-                            done = true;
-                        }
-                    }
-                }
-            }
-        }
-        if (done) {
-            npos++;
-            s++;
-            if (s < nz) {
-                next_step<MODE>(A, c, s, live, tx, ty, tz, S);
-                it = 0;
-                next_try = A.newton_retry;
-                sample = false;
-                qx = S.rx; qy = S.ry; qz = S.rz;
-                nk = 0;
-                kmax = A.newton_max;
-                newton = opening = kmax > 0;
-                check = false;
-            }
-        }
-    }
+    Column<T4> C;
+    column_init<T4, MODE, true>(A, C, i, j, live, 0, A.pred ? A.pred + 3 * ((long)i * A.ny + j) : nullptr);
+    while (C.s < A.nz) column_step_warm<T4, MODE>(A, grid, C, nevals, npos, nfb);
 }
 
 // Wave sum of the per-lane counters, one atomic per wave.  `passes` (the
@@ -341,6 +389,79 @@ __global__ __launch_bounds__(256) void relax_batch_kernel(DevArgs A0, BatchArgs 
         }
     }
     flush_counters(A0.counters, nevals, npos, lane, nfb, WARM, passes, true);
+}
+
+// ---------------------------------------------------------------------------
+// Batches whose items share one geometry (grid shape, cell, scan; only the
+// grid, output and estimate offsets differ), with lane refill: a lane takes
+// the next pixel column as soon as its own is done, so a wave never waits
+// for its slowest column (a warm-started column that falls back to FIRE
+// costs ~20x an average one).  Columns are numbered item by item in tile
+// order (64 per tile, TX x TY pixels), so the 64 columns a wave starts with
+// form one pixel tile and later ones stay close to it.  The column range is
+// split into 8 contiguous parts, one per XCD (blockIdx % 8), each with its
+// own counter queue[part] (zero at launch); the lanes of a wave that need
+// work take consecutive columns with one atomic per wave, and the wave moves
+// on to the next part when its part is empty.  Every lane leaves the loop
+// once all parts are empty and its column is done, so the grid drains.
+template <typename T4, int MODE, int TY, bool WARM>
+__global__ __launch_bounds__(256) void relax_refill_kernel(DevArgs A, BatchArgs B, const BatchRelaxItem *__restrict__ items,
+                                                           const T4 *__restrict__ grids, long long cols_item)
+{
+    constexpr int TX = 64 / TY;
+    const int lane = threadIdx.x & 63;
+    const int xcd = blockIdx.x & 7;
+    const long long ncols = (long long)B.nitems * cols_item;
+    const long long per = (ncols + 7) / 8;
+    unsigned long long nevals = 0, npos = 0, nfb = 0, passes = 0;
+    Column<T4> C;
+    C.s = A.nz;  // no column yet
+    const T4 *grid = grids;
+    int q = 0;         // parts of the column range this wave found empty (uniform)
+    bool more = true;  // this lane may still find a column
+    while (true) {
+        bool need = more && C.s >= A.nz;
+        unsigned long long mask = __ballot(need);
+        while (mask != 0 && q < 8) {
+            const int part = (xcd + q) & 7;
+            const long long lo = part * per;
+            const long long hi = min(ncols, lo + per);
+            const int leader = __ffsll((long long)mask) - 1;
+            unsigned base = 0;
+            if (lane == leader) base = atomicAdd(&B.queue[part], (unsigned)__popcll(mask));
+            base = __shfl(base, leader, 64);
+            if (need) {
+                const long long col = lo + (long long)base + __popcll(mask & ((1ull << lane) - 1));
+                if (col < hi) {
+                    const long long m = col / cols_item;
+                    const long long lc = col - m * cols_item;
+                    const int t = (int)(lc >> 6), l = (int)(lc & 63);
+                    const int ti = t / A.tiles_y, tj = t - ti * A.tiles_y;
+                    const int i = ti * TX + l / TY, j = tj * TY + l % TY;
+                    const bool live = i < A.nx && j < A.ny;
+                    const BatchRelaxItem &it = items[m];
+                    grid = grids + it.grid_off;
+                    const float *pred =
+                        WARM && B.pred ? B.pred + 3 * ((long long)it.col_off + (long long)i * A.ny + j) : nullptr;
+                    column_init<T4, MODE, WARM>(A, C, i, j, live, it.out_off, pred);
+                    need = false;  // a column outside the scan (live false) is done at once
+                }
+            }
+            mask = __ballot(need);
+            if (mask != 0) q++;  // this part is empty
+        }
+        if (need) more = false;  // every part is empty
+        if (__ballot(C.s < A.nz) == 0) {
+            if (__ballot(more) == 0) break;
+            continue;
+        }
+        if (C.s < A.nz) {
+            if (WARM) column_step_warm<T4, MODE>(A, grid, C, nevals, npos, nfb);
+            else column_step<T4, MODE>(A, grid, C, nevals, npos);
+        }
+        passes++;
+    }
+    flush_counters(A.counters, nevals, npos, lane, nfb, WARM, passes, true);
 }
 
 // ---------------------------------------------------------------------------
@@ -569,6 +690,22 @@ static void fill_shared_args(DevArgs &A, const ppafm_relax_params &rp, const ppa
 }
 
 template <typename T4, int MODE, bool WARM>
+static int launch_refill_ty(int ty, dim3 grd, dim3 blk, hipStream_t st, const DevArgs &A, const BatchArgs &B,
+                            const BatchRelaxItem *items, const void *grids, long long cols_item)
+{
+    const T4 *g = (const T4 *)grids;
+    switch (ty) {
+    case 1: hipLaunchKernelGGL((relax_refill_kernel<T4, MODE, 1, WARM>), grd, blk, 0, st, A, B, items, g, cols_item); break;
+    case 2: hipLaunchKernelGGL((relax_refill_kernel<T4, MODE, 2, WARM>), grd, blk, 0, st, A, B, items, g, cols_item); break;
+    case 4: hipLaunchKernelGGL((relax_refill_kernel<T4, MODE, 4, WARM>), grd, blk, 0, st, A, B, items, g, cols_item); break;
+    case 8: hipLaunchKernelGGL((relax_refill_kernel<T4, MODE, 8, WARM>), grd, blk, 0, st, A, B, items, g, cols_item); break;
+    case 16: hipLaunchKernelGGL((relax_refill_kernel<T4, MODE, 16, WARM>), grd, blk, 0, st, A, B, items, g, cols_item); break;
+    default: set_error("tile_y must be 1, 2, 4, 8 or 16"); return -2;
+    }
+    return 0;
+}
+
+template <typename T4, int MODE, bool WARM>
 static int launch_batch_ty(int ty, dim3 grd, dim3 blk, hipStream_t st, const DevArgs &A, const BatchArgs &B,
                            const BatchRelaxItem *items, const long long *prefix, const void *grids)
 {
@@ -585,11 +722,33 @@ static int launch_batch_ty(int ty, dim3 grd, dim3 blk, hipStream_t st, const Dev
 }
 
 template <typename T4, int MODE>
-static int launch_batch_w(bool warm, int ty, dim3 grd, dim3 blk, hipStream_t st, const DevArgs &A, const BatchArgs &B,
-                          const BatchRelaxItem *items, const long long *prefix, const void *grids)
+static int launch_batch_w(const RelaxBatchLaunch &L, dim3 grd, dim3 blk, hipStream_t st, const DevArgs &A, const BatchArgs &B)
 {
-    if (warm) return launch_batch_ty<T4, MODE, true>(ty, grd, blk, st, A, B, items, prefix, grids);
-    return launch_batch_ty<T4, MODE, false>(ty, grd, blk, st, A, B, items, prefix, grids);
+    if (L.refill) {
+        // one geometry: the uniforms of item 0, columns counted in tiles
+        DevArgs G = A;
+        const BatchRelaxItem &it = L.item0;
+        G.n0 = it.n[0]; G.n1 = it.n[1]; G.n2 = it.n[2];
+        G.nx = it.nx; G.ny = it.ny; G.nz = it.nz;
+        int tiles_y = 1;
+        const long long cols_item = 64LL * relax_batch_tiles(it.nx, it.ny, L.tile_y, tiles_y);
+        G.tiles_y = tiles_y;
+        for (int k = 0; k < 9; k++) G.dinv[k] = it.dinv[k];
+        for (int k = 0; k < 3; k++) {
+            G.p0[k] = it.p0[k]; G.da[k] = it.da[k]; G.db[k] = it.db[k]; G.dstep[k] = it.dstep[k];
+        }
+        G.tips = nullptr;
+        G.pos = B.pos;
+        G.pos_f32 = 1;
+        G.force = B.force;
+        G.iters = B.iters;
+        G.pred = nullptr;
+        PP_REQUIRE((long long)L.nitems * cols_item < (1LL << 35), "batch too large for the column queues");
+        if (L.warm) return launch_refill_ty<T4, MODE, true>(L.tile_y, grd, blk, st, G, B, L.items, L.grids, cols_item);
+        return launch_refill_ty<T4, MODE, false>(L.tile_y, grd, blk, st, G, B, L.items, L.grids, cols_item);
+    }
+    if (L.warm) return launch_batch_ty<T4, MODE, true>(L.tile_y, grd, blk, st, A, B, L.items, L.prefix, L.grids);
+    return launch_batch_ty<T4, MODE, false>(L.tile_y, grd, blk, st, A, B, L.items, L.prefix, L.grids);
 }
 
 // ---------------------------------------------------------------------------
@@ -689,11 +848,11 @@ int launch_relax_batch(const RelaxBatchLaunch &L, hipStream_t st)
     if (L.ntiles == 0) return 0;
     dim3 grd((unsigned)L.nblocks), blk(64 * L.waves_per_block);
     if (L.f64) {
-        if (L.rp.mode == 0) return launch_batch_w<double4, 0>(warm, L.tile_y, grd, blk, st, A, B, L.items, L.prefix, L.grids);
-        return launch_batch_w<double4, 1>(warm, L.tile_y, grd, blk, st, A, B, L.items, L.prefix, L.grids);
+        if (L.rp.mode == 0) return launch_batch_w<double4, 0>(L, grd, blk, st, A, B);
+        return launch_batch_w<double4, 1>(L, grd, blk, st, A, B);
     }
-    if (L.rp.mode == 0) return launch_batch_w<float4, 0>(warm, L.tile_y, grd, blk, st, A, B, L.items, L.prefix, L.grids);
-    return launch_batch_w<float4, 1>(warm, L.tile_y, grd, blk, st, A, B, L.items, L.prefix, L.grids);
+    if (L.rp.mode == 0) return launch_batch_w<float4, 0>(L, grd, blk, st, A, B);
+    return launch_batch_w<float4, 1>(L, grd, blk, st, A, B);
 }
 
 }  // namespace pp

@@ -240,7 +240,7 @@ __device__ __forceinline__ double fire_move(const DevArgs &A, const double fe[4]
 // (ux, uy, uz) of r in it and the grid force fe there (interp8), all FP64.
 // A step longer than max_step is cut to that length (damped Newton).
 // Returns false, leaving r unchanged, when the step is not finite; st2 =
-// squared length of the full step.  `stable` tells whether the Jacobian at r
+// squared length of the full step, f2 = squared total force at the old r.  `stable` tells whether the Jacobian at r
 // has all eigenvalues in the left half plane (Routh-Hurwitz on its
 // characteristic polynomial l^3 + a2 l^2 + a1 l + a0): only then is a zero of
 // the force there a minimum that damped dynamics settle in, not a saddle or a
@@ -248,7 +248,8 @@ __device__ __forceinline__ double fire_move(const DevArgs &A, const double fe[4]
 template <typename T4, int MODE>
 __device__ __forceinline__ bool newton_from(const DevArgs &A, const T4 v[8], double ux, double uy, double uz,
                                             const double fe[4], double tx, double ty, double tz, double &rx,
-                                            double &ry, double &rz, double max_step, bool &stable, double &st2)
+                                            double &ry, double &rz, double max_step, bool &stable, double &st2,
+                                            double &f2)
 {
     double dT[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};  // dT[i][a] = dF_i / du_a
 #pragma unroll
@@ -271,6 +272,7 @@ __device__ __forceinline__ bool newton_from(const DevArgs &A, const T4 v[8], dou
         for (int j = 0; j < 3; j++) J[i][j] = fma(dT[i][2], A.dinv[6 + j], fma(dT[i][1], A.dinv[3 + j], dT[i][0] * A.dinv[j]));
     double fx, fy, fz;
     total_force<MODE>(A, fe, tx, ty, tz, rx, ry, rz, fx, fy, fz);
+    f2 = fma(fx, fx, fma(fy, fy, fz * fz));
     // springs in the tip frame q = R d: a(q) = ks (q - d0) + krad (|q| - L) q / |q|,
     // force R^T a, Jacobian R^T (da/dq) R
     const double dx = rx - tx, dy = ry - ty, dz = rz - tz;
@@ -333,12 +335,12 @@ __device__ __forceinline__ bool newton_step(const DevArgs &A, const T4 *__restri
                                             double tz, double &rx, double &ry, double &rz, double max_step,
                                             bool &stable)
 {
-    double gx, gy, gz, ux, uy, uz, st2, fe[4];
+    double gx, gy, gz, ux, uy, uz, st2, f2, fe[4];
     to_grid(A, rx, ry, rz, gx, gy, gz);
     T4 v[8];
     corners(g, A, gx, gy, gz, v, ux, uy, uz);
     interp8(v, ux, uy, uz, fe, false);
-    return newton_from<T4, MODE>(A, v, ux, uy, uz, fe, tx, ty, tz, rx, ry, rz, max_step, stable, st2);
+    return newton_from<T4, MODE>(A, v, ux, uy, uz, fe, tx, ty, tz, rx, ry, rz, max_step, stable, st2, f2);
 }
 
 __device__ __forceinline__ void store_pos(const DevArgs &A, long o, double rx, double ry, double rz)

@@ -47,29 +47,40 @@ def training_set(batch, nsamples, seed=0, items=None, conv_f2=POLISH_CONV_F2, ma
 
     Each column gives one pair per round, the patch centre moved by up to
     ``jitter`` A by a hash of (position, round seed); rounds repeat until
-    ``nsamples`` pairs are collected.  Pairs whose target exceeds
-    ``max_target`` (a polish that ended far away) are dropped."""
+    ``nsamples`` pairs are collected, each round in a fresh random order of
+    the columns.  Pairs whose target exceeds ``max_target`` (a polish that
+    ended far away) are dropped.  The pairs stay on the device: a round's
+    targets are made once to find the outliers, then the kept pairs are made
+    again (the same hash, so the same pairs) straight into the output."""
     truth, _ = batch.polish(conv_f2=conv_f2, max_iters=max_iters)
     batch.synchronize()
     rng = np.random.default_rng(seed)
     pool = first_steps(batch, items)
-    P_all, T_all, n = [], [], 0
-    r = 0
+    nsamples = int(nsamples)
+    P = D.DeviceArray((max(nsamples, 1), 232), np.uint16)
+    T = D.DeviceArray((max(nsamples, 1), 3), np.float32)
+    n, r = 0, 0
     while n < nsamples and len(pool):
-        sel = rng.permutation(pool)[: int(nsamples) - n]
-        P, T = batch.ws_samples(truth, sel, jitter=jitter, seed=seed * 7919 + r)
-        Ph, Th = P.download(), T.download()
-        keep = np.abs(Th).max(axis=1) <= max_target
-        P_all.append(Ph[keep]); T_all.append(Th[keep])
-        n += int(keep.sum())
+        sel = rng.permutation(pool)[: nsamples - n]
+        s = seed * 7919 + r
+        _, Tr = batch.ws_samples(truth, sel, jitter=jitter, seed=s)
+        keep = np.abs(Tr.download()[: len(sel)]).max(axis=1) <= max_target
+        del Tr, _
+        k = int(keep.sum())
+        if k:
+            batch.ws_samples(truth, sel[keep], jitter=jitter, seed=s, out=(P, T), offset=n)
+        n += k
         r += 1
-        del P, T
         if r > 1000:
             break
-    Ph = np.concatenate(P_all)[: int(nsamples)]
-    Th = np.concatenate(T_all)[: int(nsamples)]
-    perm = rng.permutation(len(Ph))
-    return D.DeviceArray.from_host(Ph[perm]), D.DeviceArray.from_host(Th[perm]), truth
+    if n < nsamples:
+        P2 = D.DeviceArray((max(n, 1), 232), np.uint16)
+        T2 = D.DeviceArray((max(n, 1), 3), np.float32)
+        if n:
+            check(lib().ppafm_memcpy_d2d(P2.ptr, P.ptr, 464 * n, None), "copy")
+            check(lib().ppafm_memcpy_d2d(T2.ptr, T.ptr, 12 * n, None), "copy")
+        P, T = P2, T2
+    return P, T, truth
 
 
 def train(net, P, T, epochs=1.0, batch=8192, lr=3e-3, lr_end_frac=0.02):

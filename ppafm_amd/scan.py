@@ -19,12 +19,12 @@ from . import device as D
 
 
 def plain_setup(lvec, scan, tipR0=(0.0, 0.0, 3.0), k_lat=0.25, k_rad=30.0, probe_q=-0.1, tip="s", amplitude=0.8,
-                k0=1800.0, f0=30300.0, Fmax=10.0, relax=None):
+                k0=1800.0, f0=30300.0, Fmax=10.0, relax=None, fp64_radius=None):
     """Descriptors of one ProbeParticle-semantics scan: (ff model, relax
     params, tip, scan, df weights, df scale).  Positions are relative to the
-    grid origin lvec[0]."""
+    grid origin lvec[0].  fp64_radius: see device.ff_model (float4 grids)."""
     lvec = np.asarray(lvec, dtype=np.float64)
-    model = D.ff_model(0, with_lj=True, stencil=D.PROBE_STENCILS[tip], probe_q=probe_q, fmax=Fmax)
+    model = D.ff_model(0, with_lj=True, stencil=D.PROBE_STENCILS[tip], probe_q=probe_q, fmax=Fmax, fp64_radius=fp64_radius)
     rp = D.relax_params(mode=0, **(relax or {}))
     kS = np.array([k_lat, k_lat, 0.0]) / -PPU.eVA_Nm
     tip_d = D.cpu_tip(tipR0[2], k_rad / -PPU.eVA_Nm, (tipR0[0], tipR0[1]), kS)
@@ -61,11 +61,13 @@ class PlainScan:
     reference everywhere; grid_f64=False stores float4 (half the bytes per
     gather), whose rounding the FIRE stop rule turns into >1e-3 A stopping
     differences on a few ill-conditioned strokes (about 1 in 10^5-10^6).
+    A float4 grid sums pairs closer than fp64_radius (default 4 A) in FP64;
+    fp64_radius=0 sums every pair in FP32 (the FP32 grid kernel).
     """
 
     def __init__(self, lvec, n, scan, tipR0=(0.0, 0.0, 3.0), k_lat=0.25, k_rad=30.0, iZPP=8, probe_q=-0.1, tip="s",
                  amplitude=0.8, k0=1800.0, f0=30300.0, Fmax=10.0, relax=None, species=None, stream=None,
-                 grid_f64=True):
+                 grid_f64=True, fp64_radius=None):
         self.lvec = np.asarray(lvec, dtype=np.float64)
         self.n = tuple(int(v) for v in n)
         self.scan = dict(scan)
@@ -74,7 +76,7 @@ class PlainScan:
         self.species = species or PPU.loadSpecies()
         self.model, self.rp, self.tip, self.scan_d, self.w, self.df_scale = plain_setup(
             self.lvec, self.scan, tipR0=tipR0, k_lat=k_lat, k_rad=k_rad, probe_q=probe_q, tip=tip, amplitude=amplitude,
-            k0=k0, f0=f0, Fmax=Fmax, relax=relax)
+            k0=k0, f0=f0, Fmax=Fmax, relax=relax, fp64_radius=fp64_radius)
         nx, ny = self.scan["scan_n"]
         nz = self.scan["nz"]
         self.grid_f64 = bool(grid_f64)

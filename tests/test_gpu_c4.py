@@ -8,7 +8,7 @@ pairs (the default of the OpenCL model) and as double4 with FP64 pairs.
 Checked against the FP64 oracle:
   * two 32x32x40 windows of the grid (at the molecule and near a corner of
     the scan): FP64 pairs 1e-5 relative with a 1e-3 eV/A floor at the
-    maximum; FP32 pairs 1e-5 at the 99th percentile and 1e-3 at the maximum
+    maximum; FP32 pairs 3e-5 at the 99th percentile and 1e-3 at the maximum
     (the FP32 error is printed);
   * two 12x12x60 tilted scans (over the molecule and at the window edge,
     where the tilted apex comes closest to the slab) relaxed by the oracle
@@ -73,7 +73,7 @@ def test_c4_grid_windows_match_oracle(c4):
         if f64:
             assert e.max() < 1e-5
         else:
-            assert np.percentile(e, 99) < 1e-5 and e.max() < 1e-3
+            assert np.percentile(e, 99) < 3e-5 and e.max() < 1e-3
 
 
 @pytest.mark.parametrize("name", list(SCANS))

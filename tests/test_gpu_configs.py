@@ -175,6 +175,7 @@ def c3(c2):
     b.run()
     b.synchronize()
     plain = L.positions_all(b).astype(np.float64)
+    plain_it = np.concatenate([b.iterations(k).ravel() for k in range(NB)])
     P, T, truth = L.training_set(b, 400_000, seed=1, items=range(HELD[0]))
     net = WarmStartNet(max_batch=8192, seed=3)
     losses = L.train(net, P, T, epochs=6, batch=8192, lr=3e-3)
@@ -182,12 +183,12 @@ def c3(c2):
     b.set_warmstart(net)
     b.run()
     b.synchronize()
-    yield cfg, lvec, mols, b, net, plain
+    yield cfg, lvec, mols, b, net, plain, plain_it
     b.set_plain()
 
 
 def test_c3_sampled_items_vs_warm_oracle(c3):
-    cfg, lvec, mols, b, net, plain = c3
+    cfg, lvec, mols, b, net, plain, plain_it = c3
     for k in (HELD[0], HELD[len(HELD) // 2], NB - 1, 7):
         print(f"case: c3 item {k} ({len(mols[k][1])} atoms) vs the FP64 warm oracle", flush=True)
         FF, tips = _oracle_field(cfg, lvec, mols[k])
@@ -202,18 +203,38 @@ def test_c3_sampled_items_vs_warm_oracle(c3):
 
 
 def test_c3_every_item_properties(c3):
-    cfg, lvec, mols, b, net, plain = c3
+    """Every item finite and ended by the stop rule or the cap.  Against
+    plain FIRE (the reference's results): positions differ by its stop-rule
+    scatter (|F| < 1e-4 eV/A stops up to |F| / k_lat ~ 6e-3 A from a minimum
+    along the lateral springs; the warm path converges to the minimum);
+    differences above 0.1 A are where the reference's FIRE does not meet its
+    stop rule within 1000 iterations (its result there is not a minimum and
+    depends on the trajectory) or, where it does, along soft directions
+    where the stop rule is met far from the minimum.  Bounded: among
+    positions where both runs met the stop rule, differences above 0.1 A at
+    fewer than 2e-4 of them and none above 1 A."""
+    cfg, lvec, mols, b, net, plain, plain_it = c3
     total = 0
+    warm_it = []
     for k in range(NB):
         p = b.positions_raw(k)
         it = b.iterations(k)
         assert np.isfinite(p).all() and np.isfinite(b.df_host(k)).all(), k
         assert it.min() >= 0 and it.max() <= MAX_ITERS, k
         total += it.size
+        warm_it.append(it.ravel())
+    warm_it = np.concatenate(warm_it)
     nev, npos, nfb = b.counters()[:3]
     assert npos == total
     warm = L.positions_all(b).astype(np.float64)
     d = np.linalg.norm(warm - plain, axis=1)
+    cap_p, cap_w = plain_it >= MAX_ITERS, warm_it >= MAX_ITERS
+    both = ~cap_p & ~cap_w
+    big = d > 0.1
     print(f"c3: {nev / total:.2f} force evaluations per position, {nfb} fallback steps; |learned - plain FIRE|: "
           f"p50 {np.median(d):.3g} p99 {np.percentile(d, 99):.3g} max {d.max():.3g} A; > 1e-3 A at {np.mean(d > 1e-3):.4f}, "
-          f"> 0.1 A at {(d > 0.1).sum()} positions")
+          f"> 0.1 A at {big.sum()} positions: plain FIRE capped at {(big & cap_p).sum()}, learned capped at "
+          f"{(big & cap_w & ~cap_p).sum()}, both met the stop rule at {(big & both).sum()} (max {d[both].max():.3g} A); "
+          f"capped: plain {cap_p.sum()}, learned {cap_w.sum()}")
+    assert (big & both).sum() < 2e-4 * both.sum()
+    assert d[both].max() < 1.0
