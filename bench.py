@@ -660,7 +660,9 @@ def run_c4(args):
     npts = int(np.prod(cfg["n"]))
     pairs = npts * natoms * ni
     tuning = b.tuning
+    learned = run_c4_learned(args, b, stage, nev, npos) if args.learned_samples > 0 else None
     cpu = None if args.no_cpu else run_cpu_c4(b, systems[0], cfg, lvec)
+    fp64_check = run_c4_fp64_check(b, systems[0], cfg, lvec, rot) if args.c4_grid == "f32" else None
     b.close()
     return {
         "workload": f"{ni} instances: Au(111) slab (3 layers, 1932 atoms) + 60-atom molecule, 3x3 images ({natoms} atoms), "
@@ -671,9 +673,90 @@ def run_c4(args):
         "grid_pairs_per_s": pairs / (stage["grid"] * 1e-3),
         "grid_build_tflops": 32 * pairs / (stage["grid"] * 1e-3) / 1e12,
         "force_evals_per_position": nev / npos, "tuning": tuning,
+        "learned": learned,
+        "fp32_grid_vs_fp64_instance0": fp64_check,
         "cpu_baseline": cpu,
         "speedup_vs_cpu": (npos / (t_step * 1e-3)) / cpu["value"] if cpu else None,
     }
+
+
+def run_c4_learned(args, b, stage_plain, nev_plain, npos):
+    """Learned warm start on the tilted instances, beside plain FIRE: the
+    network trained on step 0 of every instance but the last two (held out),
+    then warm starts (network at step 0, Newton from the continuation) on all
+    of them; relaxation-stage times, force evaluations and the held-out
+    instances' agreement with plain FIRE."""
+    from ppafm_amd import device as D
+    from ppafm_amd import learn as L
+    from ppafm_amd.batch import tuned
+    from ppafm_amd.warmstart import WarmStartNet
+
+    ni = b.nitems
+    held = list(range(max(1, ni - 2), ni))
+    b.set_plain()
+    b.run()
+    b.synchronize()
+    plain = L.positions_of(b, held).astype(np.float64)
+    n_train = min(args.learned_samples, 8_000_000)
+    P, T, truth = L.training_set(b, n_train, seed=5, items=range(held[0]))
+    net = WarmStartNet(max_batch=8192, seed=3)
+    losses = L.train(net, P, T, epochs=2.0, batch=8192, lr=3e-3)
+    del P, T
+    st = b.stream
+    e0, e1 = D.Event(), D.Event()
+    out = {"training_pairs": int(n_train), "loss_first": float(losses[:5].mean()), "loss_last": float(losses[-5:].mean())}
+    for key, model in (("newton", None), ("learned", net)):
+        b.set_warmstart(model)
+        b.set_tuning(tuned(b, nsample=2))
+        b.run()
+        b.synchronize()
+        stages = {}
+        for name, mask in (("estimates", 8), ("relax", 16)):
+            if name == "estimates" and model is None:
+                continue
+            b.run_stages(mask)
+            e0.record(st)
+            b.run_stages(mask)
+            e1.record(st)
+            b.synchronize()
+            stages[name] = e0.elapsed_ms(e1)
+        b.run()
+        b.synchronize()
+        ev, _, nfb = b.counters()
+        w = L.positions_of(b, held).astype(np.float64)
+        d = np.linalg.norm(w - plain, axis=1)
+        relax_ms = stages["relax"] + stages.get("estimates", 0.0)
+        out[key] = {"stage_ms": stages, "relax_positions_per_s": npos / (relax_ms * 1e-3),
+                    "speedup_vs_plain_relaxation": stage_plain["relax"] / relax_ms,
+                    "force_evals_per_position": ev / npos, "fallback_steps": nfb, "tuning": b.tuning,
+                    "held_out_vs_plain_A": {"p50": float(np.median(d)), "p99": float(np.percentile(d, 99)),
+                                            "max": float(d.max()), "frac_above_1e_3": float(np.mean(d > 1e-3)),
+                                            "count_above_0_1": int((d > 0.1).sum())}}
+    b.set_plain()
+    out["plain"] = {"relax_ms": stage_plain["relax"], "force_evals_per_position": nev_plain / npos}
+    return out
+
+
+def run_c4_fp64_check(b, system, cfg, lvec, rot):
+    """Instance 0 of the tilted batch again with an FP64 grid (every pair in
+    FP64): the FP32 grid's relaxed positions and df against it."""
+    from ppafm_amd import batch as B
+
+    xyz, Zs, qs, _ = system
+    b64 = B.BatchScan.tilted([lvec], [cfg["n"]], rot=rot, pos0s=[(0.0, 0.0, cfg["z_top"])],
+                             start=(cfg["scan_x"][0], cfg["scan_y"][0]), end=(cfg["scan_x"][1], cfg["scan_y"][1]),
+                             scan_n=cfg["scan_n"], nz=cfg["nz"], zstep=cfg["dz"], molecules=[(xyz, Zs, qs)], grid_f64=True,
+                             tuning=b.tuning)
+    b64.run()
+    b64.synchronize()
+    b.set_plain()
+    b.run()
+    b.synchronize()
+    dp = np.linalg.norm(b.positions_raw(0).astype(np.float64) - b64.positions_raw(0), axis=-1)
+    ddf = np.abs(b.df_raw(0) - b64.df_raw(0))
+    b64.close()
+    return {"positions_err_A": dict(_err_stats(dp), frac_above_1e_3=float(np.mean(dp > 1e-3))), "df_err_Hz": _err_stats(ddf),
+            "note": "instance 0 relaxed on its FP32 grid against the same scan on an FP64 grid"}
 
 
 def run_cpu_c4(b, system, cfg, lvec, planes=(60, 120, 180, 240), stride=10):

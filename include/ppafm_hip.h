@@ -207,7 +207,6 @@ int ppafm_ff_build_dev(const ppafm_grid *grid, const ppafm_ff_model *model, int 
 int ppafm_ff_build(const ppafm_grid *grid, const ppafm_ff_model *model, int natoms, const double *atoms_xyzq,
                    const double *cLJs, void *out_dev, void *stream);
 
-
 /* Relaxation semantics.
  *  mode 0 (ProbeParticle.cpp): FIRE state (dt, alpha) reset for every tip
  *         position; output force = grid force sampled at the final position.
@@ -242,15 +241,20 @@ typedef struct {
 
 /* Outputs (device, any may be NULL): pos (nx,ny,nz,3) float64 or float32
  * (pos_f32 != 0), force (nx,ny,nz,4) float32, iters (nx,ny,nz) int32,
- * counters[2] uint64 accumulated: total force evaluations, total positions.
- * Layout index ((i*ny + j)*nz + s) (column-major over the scan lattice
- * matches (nx,ny,nz) C order). */
+ * counters[2] uint64 accumulated: total force evaluations, total positions,
+ * status (nx,ny,nz) uint8 per position: 0 converged by the stop rule, bit 0
+ * set = the iteration cap ended the step without meeting it, bit 1 set = a
+ * non-finite position or grid force (no counterpart in the reference, whose
+ * relaxTipStroke test itr > maxIters, ProbeParticle.cpp:658-661, never
+ * fires).  Layout index ((i*ny + j)*nz + s) (column-major over the scan
+ * lattice matches (nx,ny,nz) C order). */
 typedef struct {
     void *pos;
     int pos_f32;
     void *force;
     int *iters;
     unsigned long long *counters;
+    unsigned char *status;
 } ppafm_relax_out;
 
 int ppafm_relax(const ppafm_grid *grid, const void *grid_dev, const ppafm_relax_params *rp, const ppafm_tip *tip,
@@ -321,7 +325,7 @@ typedef struct ppafm_ws_model ppafm_ws_model;
 /* model / rp / tip as for ppafm_ff_build_dev and ppafm_relax (rp->alg must
  * be 1).  df: nw (<= 64) host weights correlated with force channel 2 of
  * each column, times df_scale; nw = 0 skips df.  keep_iters: allocate the
- * per-position iteration output.  tuning NULL = defaults.  stream NULL = the
+ * per-position iteration and status outputs (status as in ppafm_relax_out).  tuning NULL = defaults.  stream NULL = the
  * plan creates its own.  Returns NULL on error (ppafm_last_error). */
 ppafm_batch *ppafm_batch_create(int nitems, const ppafm_batch_item *items, const ppafm_ff_model *model,
                                 const ppafm_relax_params *rp, const ppafm_tip *tip, int grid_f64, int nw,
@@ -351,7 +355,8 @@ int ppafm_batch_set_tuning(ppafm_batch *b, const ppafm_batch_tuning *tuning);
 int ppafm_batch_get_tuning(const ppafm_batch *b, ppafm_batch_tuning *tuning);
 /* Device buffers: 0 grids, 1 pos, 2 force, 3 iters, 4 df, 5 counters
  * (uint64[3]: force evaluations, positions, warm-start fallbacks), 6 packed
- * atoms, 7 step-0 estimates of the network (pixel columns, 3) float32. */
+ * atoms, 7 step-0 estimates of the network (pixel columns, 3) float32,
+ * 8 per-position status uint8 (with keep_iters). */
 void *ppafm_batch_buffer(const ppafm_batch *b, int which);
 /* Per-item element offsets (nitems + 1 entries each, NULL to skip): grid
  * points, positions, df values. */
@@ -390,6 +395,10 @@ typedef struct ppafm_warm_params {
     int newton_retry;  /* FIRE iterations between Newton tries from FIRE's position, 0..1000 (0: none) */
     double newton_tol; /* A: a shorter Newton step means converged */
     double newton_cap; /* A: longest Newton step */
+    int extrapolate;   /* 1: steps >= 2 start their Newton run from the continuation moved by the
+                          change of the probe's offset from the tip over the last two steps,
+                          u(s-1) - u(s-2) (a second-order predictor); FIRE fallbacks still start
+                          from the continuation itself */
 } ppafm_warm_params;
 int ppafm_batch_set_warmstart(ppafm_batch *b, const ppafm_warm_params *p, ppafm_ws_model *m);
 /* FIRE from the plan's relaxed positions (buffer 1) to |F|^2 < conv_f2
@@ -420,6 +429,12 @@ void ppafm_ws_destroy(ppafm_ws_model *m);
 int ppafm_ws_nparams(void);
 int ppafm_ws_set_params(ppafm_ws_model *m, const float *theta_host, int n);
 int ppafm_ws_get_params(const ppafm_ws_model *m, float *theta_host, int n);
+/* Training state for checkpoints: parameters, Adam first and second moments
+ * (ppafm_ws_nparams() floats each, host) and the Adam step count.  set_state
+ * restores all of them (set_params resets the moments and the step). */
+int ppafm_ws_get_state(const ppafm_ws_model *m, float *theta_host, float *m1_host, float *m2_host, long long *step);
+int ppafm_ws_set_state(ppafm_ws_model *m, const float *theta_host, const float *m1_host, const float *m2_host,
+                       long long step);
 /* out_dev (n, 3) float32 = network(patches_dev (n, 232) bf16). */
 int ppafm_ws_forward(ppafm_ws_model *m, int n, const unsigned short *patches_dev, float *out_dev, void *stream);
 /* Mean squared error over n <= max_batch samples and its gradient (host,

@@ -710,10 +710,11 @@ static int fire_retry(const orc_field *ff, const orc_tip *tip, const orc_relax *
  * start. */
 static int warm_stroke(const orc_field *ff, const orc_tip *tip, const orc_relax *rp, int nstep,
                        const double *rtips, const double *start0, int newton_max, int newton_max0, int retry,
-                       double tol, double cap, double *rs, double *fs, int *iters, int *nevals)
+                       double tol, double cap, int extrap, double *rs, double *fs, int *iters, int *nevals)
 {
     v3 rtip = {rtips[0], rtips[1], rtips[2]};
     v3 rp_ = {rtip.x + tip->rpp0[0], rtip.y + tip->rpp0[1], rtip.z + tip->rpp0[2]};
+    v3 u1 = {0, 0, 0};  /* probe offset from the tip two steps back (predictor) */
     int nfb = 0;
     for (int i = 0; i < nstep; i++) {
         v3 nt = {rtips[3 * i], rtips[3 * i + 1], rtips[3 * i + 2]};
@@ -724,6 +725,10 @@ static int warm_stroke(const orc_field *ff, const orc_tip *tip, const orc_relax 
         const int kmax = i == 0 ? newton_max0 : newton_max;
         v3 q = rp_;
         if (i == 0 && start0) q = (v3){start0[0], start0[1], start0[2]};
+        /* second-order predictor: the offset drp of the last step keeps its
+           last change drp - u1 */
+        if (extrap && i >= 2 && kmax > 0) { q.x += drp.x - u1.x; q.y += drp.y - u1.y; q.z += drp.z - u1.z; }
+        if (i >= 1) u1 = drp;
         const int res = kmax == 0 ? -1 : newton_run(ff, tip, rtip, &q, kmax, tol, cap, rp->conv_f2, &ne);
         if (res == -1) {
             rp_ = q;
@@ -754,8 +759,8 @@ static int warm_stroke(const orc_field *ff, const orc_tip *tip, const orc_relax 
  * same column order.  Returns the total number of fallback steps. */
 long orc_relax_strokes_warm(int nx, int ny, int nz, const int n[3], const double cell[9], const double *F,
                             const double tip[8], const double relax[7], const double *rtips, const double *starts0,
-                            int newton_max, int newton_max0, int retry, double tol, double cap, double *rs,
-                            double *fs, int *iters, int *nevals)
+                            int newton_max, int newton_max0, int retry, double tol, double cap, int extrap,
+                            double *rs, double *fs, int *iters, int *nevals)
 {
     orc_field ff; orc_tip t; orc_relax r;
     make_field(&ff, n, cell, F); make_tip(&t, tip); make_relax(&r, relax);
@@ -765,8 +770,8 @@ long orc_relax_strokes_warm(int nx, int ny, int nz, const int n[3], const double
         for (int iy = 0; iy < ny; iy++) {
             const long c = (long)ix + (long)iy * nx, off = c * nz;
             total += warm_stroke(&ff, &t, &r, nz, rtips + 3 * off, starts0 ? starts0 + 3 * c : 0, newton_max,
-                                 newton_max0, retry, tol, cap, rs + 3 * off, fs + 3 * off, iters ? iters + off : 0,
-                                 nevals ? nevals + off : 0);
+                                 newton_max0, retry, tol, cap, extrap, rs + 3 * off, fs + 3 * off,
+                                 iters ? iters + off : 0, nevals ? nevals + off : 0);
         }
     return total;
 }
@@ -1094,6 +1099,21 @@ void orc_ljc_qzs_ff(const int n[3], const double p0[3], const double dA[3], cons
  * tipRot rows A,B,C (3x4; C[3] = -zstep), stiffness (4), dpos0 (4, tip frame),
  * relax_params (dt, damp), surfFF (x used), maxIters, convF2, F2SAFE.
  * Outputs FEs and paths (npix*nz, 4).  nevals (npix*nz, optional). */
+/* Periodic cell of grid coordinate g along an axis of n points: lower
+ * node lo, upper node hi, fractional position t.  The remainder is taken in
+ * floating point (exact for every finite value), so a probe that has run far
+ * away still indexes inside the grid; a NaN coordinate maps to node 0. */
+static inline void xfast_axis(double g, int n, int *lo, int *hi, double *t)
+{
+    const double fl = floor(g);
+    *t = g - fl;
+    double m = fmod(fl, (double)n);
+    if (m < 0) m += n;
+    const int il = (m >= 0 && m < n) ? (int)m : 0;
+    *lo = il;
+    *hi = il + 1 == n ? 0 : il + 1;
+}
+
 static inline void fe_interp_xfast(const double *FE, const int n[3], double gx, double gy, double gz,
                                    double out[4])
 {
@@ -1101,12 +1121,8 @@ static inline void fe_interp_xfast(const double *FE, const int n[3], double gx, 
     double w0[3], w1[3];
     double g[3] = {gx, gy, gz};
     for (int a = 0; a < 3; a++) {
-        double fl = floor(g[a]);
-        double t = g[a] - fl;
-        long il = (long)fl % n[a];
-        if (il < 0) il += n[a];
-        i0[a] = (int)il;
-        i1[a] = (int)((il + 1) % n[a]);
+        double t;
+        xfast_axis(g[a], n[a], &i0[a], &i1[a], &t);
         w0[a] = 1 - t; w1[a] = t;
     }
     for (int c = 0; c < 4; c++) out[c] = 0;
@@ -1217,10 +1233,14 @@ static void tw_force_jac(const tw_field *t, v3 tp, v3 r, double fe[4], double f[
                          r.x * t->dinv[3] + r.y * t->dinv[4] + r.z * t->dinv[5],
                          r.x * t->dinv[6] + r.y * t->dinv[7] + r.z * t->dinv[8]};
     fe_interp_xfast(t->FE, t->n, g[0], g[1], g[2], fe);
-    /* the trilinear interpolant's derivative in grid coordinates */
+    /* the trilinear interpolant's derivative in grid coordinates, in the cell
+     * fe_interp_xfast interpolates in */
     int lo[3], hi[3];
     double wl[3], wh[3];
-    for (int a = 0; a < 3; a++) wrap_axis(g[a], t->n[a], &lo[a], &hi[a], &wl[a], &wh[a]);
+    for (int a = 0; a < 3; a++) {
+        xfast_axis(g[a], t->n[a], &lo[a], &hi[a], &wh[a]);
+        wl[a] = 1 - wh[a];
+    }
     double dFg[3][3] = {{0}};
     for (int bx = 0; bx < 2; bx++)
         for (int by = 0; by < 2; by++)
@@ -1389,8 +1409,8 @@ long orc_relax_strokes_tilted_warm(int npix, int nz, const int n[3], const doubl
                                    const double tipRot[12], const double stiffness[4], const double dpos0[4],
                                    const double relax_params[2], double surf_ff, int max_iters, double conv_f2,
                                    double f2safe, const double *poss, const double *starts0, int newton_max,
-                                   int newton_max0, int retry, double tol, double cap, double *FEs, double *paths,
-                                   int *iters, int *nevals)
+                                   int newton_max0, int retry, double tol, double cap, int extrap, double *FEs,
+                                   double *paths, int *iters, int *nevals)
 {
     tw_field t;
     t.FE = FE;
@@ -1414,6 +1434,7 @@ long orc_relax_strokes_tilted_warm(int npix, int nz, const int n[3], const doubl
         v3 r = {tp.x + d0.x, tp.y + d0.y, tp.z + d0.z};
         double dt = relax_params[0], damp = relax_params[1];
         const double dtmax = dt, dtmin = dtmax * 0.1, damp0 = damp;
+        v3 u1 = {0, 0, 0}, u2 = {0, 0, 0};  /* probe offsets from the tip one and two steps back */
         for (int iz = 0; iz < nz; iz++) {
             int ne = 0, check = 0, settled = 0, itr = 0;
             double fe[4] = {0, 0, 0, 0};
@@ -1421,6 +1442,8 @@ long orc_relax_strokes_tilted_warm(int npix, int nz, const int n[3], const doubl
             v3 q = r;
             if (iz == 0 && starts0 && isfinite(starts0[3 * ip]) && isfinite(starts0[3 * ip + 1]) && isfinite(starts0[3 * ip + 2]))
                 q = (v3){starts0[3 * ip], starts0[3 * ip + 1], starts0[3 * ip + 2]};
+            /* second-order predictor (as orc_relax_strokes_warm) */
+            if (extrap && iz >= 2 && kmax > 0) { q.x += u1.x - u2.x; q.y += u1.y - u2.y; q.z += u1.z - u2.z; }
             const int res = kmax == 0 ? -1 : tw_newton_run(&t, tp, &q, kmax, tol, cap, conv_f2, &ne, fe);
             if (res == -1) {
                 r = q;
@@ -1444,6 +1467,8 @@ long orc_relax_strokes_tilted_warm(int npix, int nz, const int n[3], const doubl
             paths[4 * ind + 0] = r.x; paths[4 * ind + 1] = r.y; paths[4 * ind + 2] = r.z; paths[4 * ind + 3] = 0;
             if (iters) iters[ind] = itr;
             if (nevals) nevals[ind] = ne;
+            u2 = u1;
+            u1 = (v3){r.x - tp.x, r.y - tp.y, r.z - tp.z};
             tp.x += dtip.x; tp.y += dtip.y; tp.z += dtip.z;
             r.x += dtip.x; r.y += dtip.y; r.z += dtip.z;
         }

@@ -70,7 +70,8 @@ class Scan(ctypes.Structure):
 
 
 class RelaxOut(ctypes.Structure):
-    _fields_ = [("pos", c_void_p), ("pos_f32", c_int), ("force", c_void_p), ("iters", c_void_p), ("counters", c_void_p)]
+    _fields_ = [("pos", c_void_p), ("pos_f32", c_int), ("force", c_void_p), ("iters", c_void_p), ("counters", c_void_p),
+                ("status", c_void_p)]
 
 
 class BatchItem(ctypes.Structure):
@@ -87,7 +88,7 @@ class BatchTuning(ctypes.Structure):
 
 class WarmParams(ctypes.Structure):
     _fields_ = [("enable", c_int), ("net_passes", c_int), ("newton_max", c_int), ("newton_max0", c_int),
-                ("newton_retry", c_int), ("newton_tol", c_double), ("newton_cap", c_double)]
+                ("newton_retry", c_int), ("newton_tol", c_double), ("newton_cap", c_double), ("extrapolate", c_int)]
 
 
 # (name, restype, argtypes) of every exported symbol declared in the header
@@ -162,6 +163,8 @@ SIGNATURES = [
     ("ppafm_ws_nparams", c_int, []),
     ("ppafm_ws_set_params", c_int, [c_void_p, c_void_p, c_int]),
     ("ppafm_ws_get_params", c_int, [c_void_p, c_void_p, c_int]),
+    ("ppafm_ws_get_state", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.POINTER(ctypes.c_longlong)]),
+    ("ppafm_ws_set_state", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_longlong]),
     ("ppafm_ws_forward", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     ("ppafm_ws_loss_grad", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     ("ppafm_ws_train", c_int, [c_void_p, c_long, c_void_p, c_void_p, c_int, c_long, ctypes.c_float, ctypes.c_float, c_void_p, c_void_p]),

@@ -35,6 +35,7 @@ from ._lib import HipError, check, lib
 from .scan import plain_setup
 
 BUF_GRIDS, BUF_POS, BUF_FORCE, BUF_ITERS, BUF_DF, BUF_COUNTERS, BUF_ATOMS = range(7)
+BUF_STATUS = 8
 
 
 def _scan_copy(dst, src):
@@ -296,10 +297,19 @@ class BatchScan:
     def iterations(self, k):
         return self._download(BUF_ITERS, self.pos_off[k] * 4, self._scan_shape(k), np.int32)
 
+    def status(self, k):
+        """Per-position status (nx, ny, nz) uint8 of item k (plans made with
+        keep_iters): 0 converged by the stop rule, bit 0 = the iteration cap
+        ended the step, bit 1 = non-finite position or grid force."""
+        return self._download(BUF_STATUS, self.pos_off[k], self._scan_shape(k), np.uint8)
+
     def df_raw(self, k):
-        """df (nx, ny, nz - nw + 1) in stroke order (index 0 = top)."""
+        """df (nx, ny, nz - nw + 1) in stroke order (index 0 = top); empty
+        along z when the scan has fewer steps than the df weights."""
         nx, ny, nz = self._scan_shape(k)
         nzout = nz - len(self.w) + 1
+        if nzout < 1:
+            return np.zeros((nx, ny, 0), np.float32)
         return self._download(BUF_DF, self.df_off[k] * 4, (nx, ny, nzout), np.float32)
 
     def df_host(self, k):
@@ -326,8 +336,8 @@ class BatchScan:
         return float(c[0]) / float(c[3]) if c[3] else float("nan")
 
     # ------------------------------------------------------ learned warm start
-    def set_warmstart(self, net=None, enable=True, net_passes=1, newton_max=8, newton_max0=None, newton_retry=8,
-                      newton_tol=1e-3, newton_cap=0.1):
+    def set_warmstart(self, net=None, enable=True, net_passes=1, newton_max=16, newton_max0=12, newton_retry=16,
+                      newton_tol=1e-3, newton_cap=0.3, extrapolate=True):
         """Warm starts (``enable`` False: plain FIRE).  Every step first runs
         up to ``newton_max`` Newton iterations on the interpolated field from
         the continuation start (step 0: ``newton_max0`` from the network's
@@ -337,12 +347,12 @@ class BatchScan:
         converged and FIRE starts there, otherwise FIRE starts from the
         continuation start (counters()[2]).  Every ``newton_retry`` FIRE
         iterations Newton is tried again from FIRE's position.  FIRE ends
-        every step by the plan's stop rule.  ``newton_max0`` defaults to 6
-        with a network, 12 without."""
-        if newton_max0 is None:
-            newton_max0 = 6 if net is not None else 12
+        every step by the plan's stop rule.  ``extrapolate``: steps from the third on
+        start their Newton run from the continuation moved by the last change
+        of the probe's offset from the tip (fallbacks still start FIRE from
+        the continuation itself)."""
         p = _lib.WarmParams(int(bool(enable)), int(net_passes), int(newton_max), int(newton_max0), int(newton_retry),
-                            float(newton_tol), float(newton_cap))
+                            float(newton_tol), float(newton_cap), int(bool(extrapolate)))
         self._ws = net if enable else None
         check(lib().ppafm_batch_set_warmstart(self._h, ctypes.byref(p), net._h if (net is not None and enable) else None),
               "batch_set_warmstart")

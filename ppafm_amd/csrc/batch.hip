@@ -77,6 +77,7 @@ struct ppafm_batch {
     float *pos = nullptr;
     float4 *force = nullptr;
     int *iters = nullptr;
+    unsigned char *status = nullptr;  // per-position status, with iters
     float *df = nullptr;
     AtomRec *atoms = nullptr;
     long long atom_cap = 0;  // atom records the atoms buffer holds
@@ -182,6 +183,7 @@ int enqueue_relax(ppafm_batch &b, const ppafm_batch_tuning &t, const BatchRelaxI
     L.pos = b.pos;
     L.force = b.force;
     L.iters = b.iters;
+    L.status = b.status;
     L.counters = b.counters();
     L.tile_y = t.tile_y;
     L.waves_per_block = t.waves_per_block;
@@ -194,6 +196,7 @@ int enqueue_relax(ppafm_batch &b, const ppafm_batch_tuning &t, const BatchRelaxI
         L.newton_retry = b.warm.newton_retry;
         L.newton_tol = b.warm.newton_tol;
         L.newton_cap = b.warm.newton_cap;
+        L.newton_extrap = b.warm.extrapolate != 0;
     }
     return launch_relax_batch(L, b.st);
 }
@@ -213,6 +216,7 @@ RelaxBatchLaunch base_launch(ppafm_batch &b)
     L.pos = b.pos;
     L.force = b.force;
     L.iters = b.iters;
+    L.status = b.status;
     L.counters = b.counters();
     L.tile_y = b.tuning.tile_y;
     L.waves_per_block = b.tuning.waves_per_block;
@@ -332,7 +336,7 @@ void free_plan(ppafm_batch *b)
     drop_graph(*b);
     void *bufs[] = {b->grids, b->pos, b->force, b->iters, b->df, b->atoms, b->ctl, b->d_gitems,
                     b->d_bprefix, b->d_ritems, b->d_tprefix, b->d_sitems, b->d_sprefix, b->pred, b->d_wprefix,
-                    b->d_pprefix};
+                    b->d_pprefix, b->status};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     if (b->own_stream && b->st) (void)hipStreamDestroy(b->st);
@@ -398,7 +402,10 @@ int build_plan(ppafm_batch *b, int nitems, const ppafm_batch_item *items)
     PP_TRY_HIP(hipMalloc(&b->grids, std::max<long long>(b->grid_off[nitems], 1) * esz));
     PP_TRY_HIP(hipMalloc((void **)&b->pos, std::max<long long>(npos, 1) * 12));
     PP_TRY_HIP(hipMalloc((void **)&b->force, std::max<long long>(npos, 1) * 16));
-    if (b->keep_iters) PP_TRY_HIP(hipMalloc((void **)&b->iters, std::max<long long>(npos, 1) * 4));
+    if (b->keep_iters) {
+        PP_TRY_HIP(hipMalloc((void **)&b->iters, std::max<long long>(npos, 1) * 4));
+        PP_TRY_HIP(hipMalloc((void **)&b->status, std::max<long long>(npos, 1)));
+    }
     PP_TRY_HIP(hipMalloc((void **)&b->df, std::max<long long>(b->df_off[nitems], 1) * 4));
     b->atom_cap = std::max<long long>(b->atom_off[nitems], 1);
     PP_TRY_HIP(hipMalloc((void **)&b->atoms, b->atom_cap * sizeof(AtomRec)));
@@ -645,6 +652,7 @@ void *ppafm_batch_buffer(const ppafm_batch *b, int which)
     case 5: return b->counters();
     case 6: return b->atoms;
     case 7: return b->pred;
+    case 8: return b->status;
     default: return nullptr;
     }
 }
@@ -670,6 +678,7 @@ int ppafm_batch_set_warmstart(ppafm_batch *b, const ppafm_warm_params *p, ppafm_
         PP_REQUIRE(p->newton_retry >= 0 && p->newton_retry <= 1000, "newton_retry must be 0..1000");
         PP_REQUIRE(p->newton_tol > 0 && p->newton_tol < 1, "newton_tol must be in (0, 1) A");
         PP_REQUIRE(p->newton_cap > 0 && p->newton_cap <= 1, "newton_cap must be in (0, 1] A");
+        PP_REQUIRE(p->extrapolate == 0 || p->extrapolate == 1, "extrapolate must be 0 or 1");
         PP_REQUIRE(!m || (p->net_passes >= 1 && p->net_passes <= 8), "net_passes must be 1..8");
     }
     PP_TRY_HIP(hipStreamSynchronize(b->st));

@@ -187,6 +187,7 @@ struct RelaxBatchLaunch {
     float4 *force;
     int *iters;
     unsigned long long *counters;
+    unsigned char *status = nullptr;  // per-position status, optional
     int tile_y;           // pixel tile: (64 / tile_y) x tile_y columns
     int waves_per_block;  // 1..4
     int nblocks;          // persistent workgroups
@@ -200,6 +201,7 @@ struct RelaxBatchLaunch {
     const float *pred = nullptr;
     int newton_max = 0, newton_max0 = 0, newton_retry = 0;
     double newton_tol = 0, newton_cap = 0;
+    bool newton_extrap = false;  // second-order predictor for steps >= 2
 };
 // Tiles of one scan for a given tile_y (also returns tiles along y).
 int relax_batch_tiles(int nx, int ny, int tile_y, int &tiles_y);

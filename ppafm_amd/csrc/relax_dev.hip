@@ -45,6 +45,7 @@ struct Column {
     // evaluated by FIRE
     double qx, qy, qz;
     int next_try, nk, kmax;
+    double ux1, uy1, uz1;  // probe offset from the tip at the previous step (predictor)
     bool newton, opening, check;
 };
 
@@ -73,6 +74,7 @@ __device__ __forceinline__ void column_init(const DevArgs &A, Column<T4> &C, int
             C.qx = (double)pred[0]; C.qy = (double)pred[1]; C.qz = (double)pred[2];
         }
         C.nk = 0;
+        C.ux1 = C.uy1 = C.uz1 = 0;
         C.kmax = A.newton_max0;
         C.newton = live && C.kmax > 0;
         C.opening = C.newton;
@@ -98,6 +100,7 @@ __device__ __forceinline__ void column_step(const DevArgs &A, const T4 *__restri
         store_pos(A, o, C.S.rx, C.S.ry, C.S.rz);
         if (A.force) A.force[o] = make_float4((float)fe[0], (float)fe[1], (float)fe[2], 0.f);
         if (A.iters) A.iters[o] = C.itr;
+        store_status(A, o, C.itr >= A.max_iters, C.S.rx, C.S.ry, C.S.rz, fe);
         done = true;
     } else {
         const double ff = fire_move<MODE>(A, fe, C.tx, C.ty, C.tz, C.S);
@@ -111,6 +114,7 @@ __device__ __forceinline__ void column_step(const DevArgs &A, const T4 *__restri
                 if (A.force) A.force[o] = tip_frame_force(A, fe);
                 store_pos(A, o, C.S.rx, C.S.ry, C.S.rz);
                 if (A.iters) A.iters[o] = C.it;
+                store_status(A, o, !conv, C.S.rx, C.S.ry, C.S.rz, fe);
                 done = true;
             }
         }
@@ -129,7 +133,9 @@ __device__ __forceinline__ void column_step(const DevArgs &A, const T4 *__restri
 // One force evaluation of a warm-started column.  Every step begins with
 // Newton iterations on the interpolated field (newton_from) from the
 // continuation start the reference uses (step 0: from the column's estimate,
-// or the plain start without one), each step cut to at most newton_cap.
+// or the plain start without one; with newton_extrap, steps >= 2 from the
+// continuation moved by the last change of the probe's offset from the tip,
+// u(s-1) - u(s-2)), each step cut to at most newton_cap.
 // When they reach a stable zero (a full step shorter than sqrt(newton_tol2)
 // with a stable Jacobian at every iterate, at most newton_max of them,
 // newton_max0 at step 0), FIRE runs from there; otherwise (an unstable
@@ -176,6 +182,7 @@ __device__ __forceinline__ void column_step_warm(const DevArgs &A, const T4 *__r
             if (A.force)
                 A.force[o] = MODE == 0 ? make_float4((float)fe[0], (float)fe[1], (float)fe[2], 0.f) : tip_frame_force(A, fe);
             if (A.iters) A.iters[o] = C.it;
+            store_status(A, o, false, C.qx, C.qy, C.qz, fe);
             C.newton = false;
             C.opening = false;
             done = true;
@@ -205,6 +212,7 @@ __device__ __forceinline__ void column_step_warm(const DevArgs &A, const T4 *__r
             store_pos(A, o, C.S.rx, C.S.ry, C.S.rz);
             if (A.force) A.force[o] = make_float4((float)fe[0], (float)fe[1], (float)fe[2], 0.f);
             if (A.iters) A.iters[o] = C.itr;
+            store_status(A, o, C.itr >= A.max_iters, C.S.rx, C.S.ry, C.S.rz, fe);
             done = true;
         } else {
             double fx, fy, fz;
@@ -218,6 +226,7 @@ __device__ __forceinline__ void column_step_warm(const DevArgs &A, const T4 *__r
                     A.force[o] = MODE == 0 ? make_float4((float)fe[0], (float)fe[1], (float)fe[2], 0.f)
                                            : tip_frame_force(A, fe);
                 if (A.iters) A.iters[o] = C.it;
+                store_status(A, o, false, C.S.rx, C.S.ry, C.S.rz, fe);
                 done = true;
             } else {
                 const double ff = fire_update<MODE>(A, fx, fy, fz, C.S);
@@ -231,6 +240,7 @@ __device__ __forceinline__ void column_step_warm(const DevArgs &A, const T4 *__r
                         if (A.force) A.force[o] = tip_frame_force(A, fe);
                         store_pos(A, o, C.S.rx, C.S.ry, C.S.rz);
                         if (A.iters) A.iters[o] = C.it;
+                        store_status(A, o, !conv, C.S.rx, C.S.ry, C.S.rz, fe);
                         done = true;
                     }
                 }
@@ -241,11 +251,17 @@ __device__ __forceinline__ void column_step_warm(const DevArgs &A, const T4 *__r
         npos++;
         C.s++;
         if (C.s < A.nz) {
+            // the probe's offset from the tip at the step just ended
+            const double ux = C.S.rx - C.tx, uy = C.S.ry - C.ty, uz = C.S.rz - C.tz;
             next_step<MODE>(A, C.c, C.s, true, C.tx, C.ty, C.tz, C.S);
             C.it = 0;
             C.next_try = A.newton_retry;
             C.sample = false;
             C.qx = C.S.rx; C.qy = C.S.ry; C.qz = C.S.rz;
+            if (A.newton_extrap && C.s >= 2) {  // second-order predictor: the offset keeps its last change
+                C.qx += ux - C.ux1; C.qy += uy - C.uy1; C.qz += uz - C.uz1;
+            }
+            C.ux1 = ux; C.uy1 = uy; C.uz1 = uz;
             C.nk = 0;
             C.kmax = A.newton_max;
             C.newton = C.opening = C.kmax > 0;
@@ -327,6 +343,7 @@ struct BatchArgs {
     float4 *force;
     int *iters;
     const float *pred;  // step-0 estimates (pixel columns, 3), WARM kernels only
+    unsigned char *status;
 };
 
 __device__ __forceinline__ int find_batch_item(const long long *__restrict__ prefix, int nitems, long long x)
@@ -375,6 +392,7 @@ __global__ __launch_bounds__(256) void relax_batch_kernel(DevArgs A0, BatchArgs 
             A.pos_f32 = 1;
             A.force = B.force ? B.force + it.out_off : nullptr;
             A.iters = B.iters ? B.iters + it.out_off : nullptr;
+            A.status = B.status ? B.status + it.out_off : nullptr;
             if (WARM) A.pred = B.pred ? B.pred + 3 * (long long)it.col_off : nullptr;
             const int lt = (int)(tile - prefix[m]);
             const int ti = lt / it.tiles_y, tj = lt % it.tiles_y;
@@ -556,6 +574,7 @@ __global__ __launch_bounds__(64) void relax_lds_kernel(DevArgs A, const T4 *__re
         // ---- relax this step, all lanes together
         int st = live ? 0 : 2;  // 0 = FIRE, 1 = output sample pending (mode 0), 2 = done
         int it = 0, itr = 0;
+        bool capped = false;
         float4 outf = make_float4(0.f, 0.f, 0.f, 0.f);
         while (__ballot(st < 2) != 0) {
             if (st < 2) {
@@ -588,6 +607,7 @@ __global__ __launch_bounds__(64) void relax_lds_kernel(DevArgs A, const T4 *__re
                     it++;
                     const bool conv = ff < A.conv_f2;
                     if (conv || it >= A.max_iters) {
+                        capped = !conv;
                         if (MODE == 0) {
                             itr = conv ? it - 1 : A.max_iters;
                             st = 1;
@@ -605,6 +625,8 @@ __global__ __launch_bounds__(64) void relax_lds_kernel(DevArgs A, const T4 *__re
             store_pos(A, o, S.rx, S.ry, S.rz);
             if (A.force) A.force[o] = outf;
             if (A.iters) A.iters[o] = itr;
+            const double fo[4] = {outf.x, outf.y, outf.z, 0.0};
+            store_status(A, o, capped, S.rx, S.ry, S.rz, fo);
             npos++;
         }
         __syncthreads();
@@ -641,6 +663,7 @@ int launch_relax_fast(const RelaxLaunch &L, const void *grid, bool f64, hipStrea
     A.krad = L.tip.krad; A.lrad = L.tip.lrad; A.surf_ff = L.tip.surf_ff;
     A.tips = L.tips;
     A.pos = L.out.pos; A.pos_f32 = L.out.pos_f32; A.force = (float4 *)L.out.force; A.iters = L.out.iters;
+    A.status = L.out.status;
     A.counters = L.out.counters;
     if (A.ntiles == 0 || A.nz == 0) return 0;
     const long nblk = ((A.ntiles + 7L) / 8) * 8;
@@ -742,6 +765,7 @@ static int launch_batch_w(const RelaxBatchLaunch &L, dim3 grd, dim3 blk, hipStre
         G.pos_f32 = 1;
         G.force = B.force;
         G.iters = B.iters;
+        G.status = B.status;
         G.pred = nullptr;
         PP_REQUIRE((long long)L.nitems * cols_item < (1LL << 35), "batch too large for the column queues");
         if (L.warm) return launch_refill_ty<T4, MODE, true>(L.tile_y, grd, blk, st, G, B, L.items, L.grids, cols_item);
@@ -843,8 +867,9 @@ int launch_relax_batch(const RelaxBatchLaunch &L, hipStream_t st)
         A.newton_tol2 = L.newton_tol * L.newton_tol;
         A.newton_cap = L.newton_cap;
         A.newton_retry = L.newton_retry;
+        A.newton_extrap = L.newton_extrap ? 1 : 0;
     }
-    BatchArgs B{L.nitems, L.ntiles, L.queue, L.pos, L.force, L.iters, warm ? L.pred : nullptr};
+    BatchArgs B{L.nitems, L.ntiles, L.queue, L.pos, L.force, L.iters, warm ? L.pred : nullptr, L.status};
     if (L.ntiles == 0) return 0;
     dim3 grd((unsigned)L.nblocks), blk(64 * L.waves_per_block);
     if (L.f64) {

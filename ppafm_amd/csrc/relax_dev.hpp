@@ -28,6 +28,7 @@ struct DevArgs {
     int pos_f32;
     float4 *force;
     int *iters;
+    unsigned char *status = nullptr;  // per-position status (store_status), optional
     unsigned long long *counters;
     // warm starts: step-0 estimates per pixel column (3 floats, nullptr: the
     // plain start), Newton iterations allowed from the continuation (steps
@@ -36,6 +37,7 @@ struct DevArgs {
     const float *pred;
     int newton_max, newton_max0, newton_retry;
     double newton_tol2, newton_cap;
+    int newton_extrap;  // 1: steps >= 2 start Newton at the second-order prediction
 };
 
 __device__ __forceinline__ double rsq_nr(double x)
@@ -353,6 +355,19 @@ __device__ __forceinline__ void store_pos(const DevArgs &A, long o, double rx, d
         double *p = (double *)A.pos + 3 * o;
         p[0] = rx; p[1] = ry; p[2] = rz;
     }
+}
+
+// Per-position status, the optional output the reference has no counterpart
+// for (relaxTipStroke's itr > maxIters test, ProbeParticle.cpp:658-661, can
+// never fire): bit 0 = the iteration cap ended the step without meeting the
+// stop rule, bit 1 = a non-finite position or grid force (a NaN grid or a
+// diverging probe); 0 = converged.
+__device__ __forceinline__ void store_status(const DevArgs &A, long o, bool capped, double rx, double ry, double rz,
+                                             const double fe[4])
+{
+    if (!A.status) return;
+    const bool finite = isfinite(rx) && isfinite(ry) && isfinite(rz) && isfinite(fe[0]) && isfinite(fe[1]) && isfinite(fe[2]);
+    A.status[o] = (unsigned char)((capped ? 1 : 0) | (finite ? 0 : 2));
 }
 
 // grid force rotated into the tip frame (relax.cl:446-452), .w = energy

@@ -1059,6 +1059,30 @@ int ppafm_ws_get_params(const ppafm_ws_model *w, float *theta, int n)
     return 0;
 }
 
+int ppafm_ws_get_state(const ppafm_ws_model *w, float *theta, float *m1, float *m2, long long *step)
+{
+    PP_REQUIRE(w && theta && m1 && m2 && step, "NULL argument");
+    PP_TRY_HIP(hipDeviceSynchronize());
+    PP_TRY_HIP(hipMemcpy(theta, w->m.theta, kNParam * 4, hipMemcpyDeviceToHost));
+    PP_TRY_HIP(hipMemcpy(m1, w->m.m1, kNParam * 4, hipMemcpyDeviceToHost));
+    PP_TRY_HIP(hipMemcpy(m2, w->m.m2, kNParam * 4, hipMemcpyDeviceToHost));
+    *step = w->m.step;
+    return 0;
+}
+
+int ppafm_ws_set_state(ppafm_ws_model *w, const float *theta, const float *m1, const float *m2, long long step)
+{
+    PP_REQUIRE(w && theta && m1 && m2, "NULL argument");
+    PP_REQUIRE(step >= 0, "step must be >= 0");
+    PP_TRY_HIP(hipMemcpy(w->m.theta, theta, kNParam * 4, hipMemcpyHostToDevice));
+    PP_TRY_HIP(hipMemcpy(w->m.m1, m1, kNParam * 4, hipMemcpyHostToDevice));
+    PP_TRY_HIP(hipMemcpy(w->m.m2, m2, kNParam * 4, hipMemcpyHostToDevice));
+    w->m.step = (long)step;
+    ws_refresh(w->m, nullptr);
+    PP_TRY_HIP(hipDeviceSynchronize());
+    return 0;
+}
+
 int ppafm_ws_forward(ppafm_ws_model *w, int n, const unsigned short *patches_dev, float *out_dev, void *stream)
 {
     PP_REQUIRE(w && patches_dev && out_dev, "NULL argument");

@@ -220,9 +220,12 @@ def ff_build(n, cell, xyzq, cLJs, model, out=None, stream=None, atoms_dev=None, 
     return out
 
 
-def relax(n, cell, grid_dev, rp, tip, scan, tips_dev=None, pos=None, force=None, iters=None, counters=None, pos_f32=True, stream=None):
+def relax(n, cell, grid_dev, rp, tip, scan, tips_dev=None, pos=None, force=None, iters=None, counters=None, pos_f32=True,
+          stream=None, status=None):
     """Run the relaxation kernel; outputs are DeviceArrays (allocated if None
-    and requested with True)."""
+    and requested with True).  Returns (pos, force, iters), and the uint8
+    per-position status array as a fourth item when ``status`` is given
+    (0 converged, bit 0 the iteration cap ended the step, bit 1 non-finite)."""
     shp = (scan.nx, scan.ny, scan.nz)
     if pos is True:
         pos = DeviceArray(shp + (3,), np.float32 if pos_f32 else np.float64)
@@ -230,17 +233,22 @@ def relax(n, cell, grid_dev, rp, tip, scan, tips_dev=None, pos=None, force=None,
         force = DeviceArray(shp + (4,), np.float32)
     if iters is True:
         iters = DeviceArray(shp, np.int32)
+    if status is True:
+        status = DeviceArray(shp, np.uint8)
     o = _lib.RelaxOut()
     o.pos = pos.ptr if pos is not None else None
     o.pos_f32 = int(pos_f32)
     o.force = force.ptr if force is not None else None
     o.iters = iters.ptr if iters is not None else None
     o.counters = counters.ptr if counters is not None else None
+    o.status = status.ptr if status is not None else None
     g = grid_desc(n, cell, f64=grid_dev.dtype == np.float64)
     check(
         lib().ppafm_relax(ctypes.byref(g), grid_dev.ptr, ctypes.byref(rp), ctypes.byref(tip), ctypes.byref(scan), tips_dev.ptr if tips_dev is not None else None, ctypes.byref(o), stream),
         "relax",
     )
+    if status is not None:
+        return pos, force, iters, status
     return pos, force, iters
 
 

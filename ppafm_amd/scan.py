@@ -84,6 +84,7 @@ class PlainScan:
         self.pos = D.DeviceArray((nx, ny, nz, 3), np.float32)
         self.force = D.DeviceArray((nx, ny, nz, 4), np.float32)
         self.iters = D.DeviceArray((nx, ny, nz), np.int32)
+        self.status_d = D.DeviceArray((nx, ny, nz), np.uint8)
         self.counters = D.DeviceArray((2,), np.uint64)
         self.counters.zero()
         self.nzout = nz - len(self.w) + 1
@@ -110,7 +111,8 @@ class PlainScan:
         check(lib().ppafm_ff_build_dev(ctypes.byref(g), ctypes.byref(self.model), self.natoms, self.atoms.ptr, self.grid.ptr, self.stream), "ff_build")
 
     def relax(self, counters=True):
-        D.relax(self.n, self.lvec[1:], self.grid, self.rp, self.tip, self.scan_d, pos=self.pos, force=self.force, iters=self.iters, counters=self.counters if counters else None, stream=self.stream)
+        D.relax(self.n, self.lvec[1:], self.grid, self.rp, self.tip, self.scan_d, pos=self.pos, force=self.force, iters=self.iters,
+                counters=self.counters if counters else None, stream=self.stream, status=self.status_d)
 
     def to_df(self):
         if self.nzout < 1:
@@ -182,6 +184,12 @@ class PlainScan:
 
     def iterations(self):
         return self.iters.download()
+
+    def status(self):
+        """Per-position status (nx, ny, nz) uint8: 0 converged by the stop
+        rule, bit 0 = the iteration cap ended the step, bit 1 = non-finite
+        position or grid force."""
+        return self.status_d.download()
 
     def df_host(self):
         """df (nx, ny, nzout) with z ascending, like Fz2df on the reversed stack."""

@@ -50,6 +50,26 @@ def pack_patches(P):
     return out
 
 
+CHECKPOINT_FORMAT = 1
+
+
+def read_checkpoint(path):
+    """A training checkpoint (WarmStartNet.save) as a dict: theta, m1, m2
+    (float32, NPARAM each), step, seed, meta (dict); the arrays are checked
+    for shape and type."""
+    with np.load(path, allow_pickle=False) as z:
+        d = {k: z[k] for k in z.files}
+    if int(d.get("format", -1)) != CHECKPOINT_FORMAT:
+        raise ValueError(f"{path}: not a warm-start checkpoint of format {CHECKPOINT_FORMAT}")
+    for k in ("theta", "m1", "m2"):
+        if d[k].dtype != np.float32 or d[k].shape != (NPARAM,):
+            raise ValueError(f"{path}: {k} must be {NPARAM} float32 values")
+    import json
+
+    return dict(theta=d["theta"], m1=d["m1"], m2=d["m2"], step=int(d["step"]), seed=int(d["seed"]),
+                meta=json.loads(str(d["meta"])))
+
+
 class WarmStartNet:
     """The network and its training state on the GPU."""
 
@@ -60,8 +80,39 @@ class WarmStartNet:
             raise HipError("ppafm_ws_create failed: " + lib().ppafm_last_error().decode(errors="replace"))
         self._h = h
         self.max_batch = int(max_batch)
+        self.seed = int(seed)
         assert lib().ppafm_ws_nparams() == NPARAM
         self.set_params(init_params(seed) if params is None else params)
+
+    # ------------------------------------------------------------ checkpoints
+    def state(self):
+        """(theta, Adam m1, Adam m2, step) from the GPU."""
+        th, m1, m2 = (np.empty(NPARAM, np.float32) for _ in range(3))
+        step = ctypes.c_longlong()
+        check(lib().ppafm_ws_get_state(self._h, th.ctypes.data, m1.ctypes.data, m2.ctypes.data, ctypes.byref(step)),
+              "ws_get_state")
+        return th, m1, m2, int(step.value)
+
+    def save(self, path, meta=None):
+        """Write a training checkpoint (npz): the parameters, both Adam moments,
+        the Adam step, the initialisation seed and ``meta`` (a JSON-serialisable
+        dict, e.g. the training-set seed and sample count) for a resume."""
+        import json
+
+        th, m1, m2, step = self.state()
+        np.savez(path, format=np.int64(CHECKPOINT_FORMAT), theta=th, m1=m1, m2=m2, step=np.int64(step),
+                 seed=np.int64(self.seed), meta=np.array(json.dumps(meta or {})))
+
+    @classmethod
+    def load(cls, path, max_batch=8192):
+        """A network with the checkpoint's parameters and Adam state: training
+        resumes where the saved one stopped."""
+        d = read_checkpoint(path)
+        net = cls(max_batch=max_batch, params=d["theta"], seed=d["seed"])
+        check(lib().ppafm_ws_set_state(net._h, d["theta"].ctypes.data, d["m1"].ctypes.data, d["m2"].ctypes.data,
+                                       d["step"]), "ws_set_state")
+        net.meta = d["meta"]
+        return net
 
     def set_params(self, theta):
         theta = np.ascontiguousarray(theta, dtype=np.float32)

@@ -70,6 +70,9 @@ def test_batch_equals_single_scans(run):
         assert np.array_equal(b.positions_raw(k), s.pos.download()), k
         assert np.array_equal(b.forces(k), s.forces()), k
         assert np.array_equal(b.iterations(k), s.iterations()), k
+        assert np.array_equal(b.status(k), s.status()), k
+        # mode 0: a capped step reports max_iters, a converged one fewer
+        assert np.array_equal(b.status(k), (b.iterations(k) >= b.rp.max_iters).astype(np.uint8)), k
         assert np.array_equal(b.df_host(k), s.df_host()), k
     ne, npos = b.counts()
     assert npos == b.total_positions()
@@ -248,7 +251,7 @@ def test_lane_refill_equals_tile_queue(mode):
             b.set_tuning({"tile_y": 4, "waves_per_block": 1, "blocks_per_cu": 1, "refill": refill})
             b.run()
             b.synchronize()
-            out[refill] = ([(b.positions_raw(k), b.forces(k), b.iterations(k), b.df_raw(k)) for k in range(NI)],
+            out[refill] = ([(b.positions_raw(k), b.forces(k), b.iterations(k), b.status(k), b.df_raw(k)) for k in range(NI)],
                            b.counters(), b.lane_use())
         for k in range(NI):
             for a, c in zip(out[0][0][k], out[1][0][k]):

@@ -111,3 +111,55 @@ def test_c4_tilted_scan_matches_oracle(c4, name):
     assert dp.max() < 1e-3
     assert dff < 1e-5 * max(1.0, np.abs(FEs).max())
     assert ddf < 1e-3
+
+
+@pytest.mark.parametrize("name", list(SCANS))
+def test_c4_warm_tilted_scan_matches_warm_oracle(c4, name):
+    """Warm starts (Newton from the continuation, FIRE to the stop rule) on the
+    tilted scans against the FP64 oracle of the same method in
+    relaxStrokesTilted semantics (orc_relax_strokes_tilted_warm), on the
+    GPU-built grid; and against plain tilted FIRE: the difference stays
+    within the stop rule's scatter and the warm path takes fewer force
+    evaluations."""
+    cfg, lvec, xyz, Zs, qs, bs = c4
+    b = bs[name]
+    it = b.items[0]
+    s = it["scan"]
+    nx, ny, nz = s.nx, s.ny, s.nz
+    i, j = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    poss = np.zeros((nx, ny, 4))
+    for c in range(3):
+        poss[..., c] = s.p0[c] + i * s.da[c] + j * s.db[c]
+    cell = np.asarray(it["cell"], dtype=np.float64)
+    dinv = np.linalg.inv(cell / np.array(cfg["n"])[:, None]).T
+    FE = b.grid(0)
+    tip = b.tip
+    tipRot = np.zeros((3, 4))
+    tipRot[:, :3] = np.array(tip.rot).reshape(3, 3)
+    tipRot[2, 3] = -cfg["dz"]
+    st = np.array([tip.kspring[0], tip.kspring[1], tip.kspring[2], tip.krad])
+    dpos0 = np.array([tip.d0[0], tip.d0[1], tip.d0[2], tip.lrad])
+    kw = dict(max_iters=128, conv_f2=float(np.float32(1e-8)), f2safe=float(np.float32(1e-8)))
+    plain = b.positions_raw(0).reshape(-1, nz, 3).astype(np.float64)
+    nev_plain = b.counters()[0]
+    b.set_warmstart(None)
+    b.run()
+    b.synchronize()
+    gp = b.positions_raw(0).reshape(-1, nz, 3)
+    gf = b.forces(0).reshape(-1, nz, 4)
+    nev_warm, _, nfb = b.counters()
+    b.set_plain()
+    b.run()
+    b.synchronize()
+    FEs, paths, itw, nevw, nfb_o = O.relax_tilted_warm(FE, dinv, poss.reshape(-1, 4), nz, tipRot, st, dpos0,
+                                                       np.array([b.rp.dt, b.rp.damping]), **kw)
+    dp = np.linalg.norm(gp - paths[..., :3], axis=-1)
+    dff = np.abs(gf - FEs).max()
+    dpl = np.linalg.norm(gp - plain, axis=-1)
+    print(f"{name} warm (grid_f64={b.grid_f64}): vs warm oracle paths max {dp.max():.2e} A, forces max {dff:.2e}; "
+          f"fallbacks GPU {nfb} oracle {nfb_o}; vs plain FIRE p50 {np.median(dpl):.2e} p99 {np.percentile(dpl, 99):.2e} "
+          f"max {dpl.max():.2e} A; evaluations warm {nev_warm} plain {nev_plain}")
+    assert dp.max() < 1e-3
+    assert dff < 1e-5 * max(1.0, np.abs(FEs).max())
+    assert np.percentile(dpl, 99) < 0.03
+    assert nev_warm < 0.5 * nev_plain

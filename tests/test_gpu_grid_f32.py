@@ -126,17 +126,25 @@ def test_overflow_guard_ocl():
     assert err.max() < 1e-3
 
 
-def test_overflow_guard_pp_atom_on_node():
-    print("case: ProbeParticle model, an atom exactly on a grid node (clamped fields)", flush=True)
+def test_overflow_guard_pp_atom_near_node():
+    """An atom 0.001 A from a grid node: the FP32 repulsion term 12 c12 r^-14
+    overflows there (r^-2 ~ 1e6), the guard sums the point again in FP64,
+    and the clamped fields agree with the oracle to FP64 accuracy (FP32
+    pairs would leave the direction of a 0.001 A offset wrong at 1e-4).
+    (An atom exactly on a node is a singular point: the pair's force there
+    is 0 * inf, and its direction depends on 1e-16 offsets of the node's
+    coordinates, so no result pins it.)"""
+    print("case: ProbeParticle model, an atom 0.001 A from a grid node (clamped fields)", flush=True)
     n = (32, 32, 32)
     lvec = np.array([[0.0, 0.0, 0.0], [3.2, 0, 0], [0, 3.2, 0], [0, 0, 3.2]])
-    xyz = np.array([[1.6, 1.6, 1.6], [0.75, 2.3, 1.1]])
+    node = np.array([1.6, 1.6, 1.6])
+    xyz = np.array([node + 0.001 / np.sqrt(3) * np.array([1.0, -1.0, 1.0]), [0.75, 2.3, 1.1]])
     G, FF = _pp_grid(n, lvec, xyz, np.array([6, 8]), np.array([0.2, -0.2]))
     # the clamped forces stay finite; the energy at the node (~1e52 eV)
     # exceeds the float range of the storage
     assert np.isfinite(G[..., :3]).all()
     err = grid_rel_err(G, FF)
-    print(f"atom on a node: max {err.max():.3g}; at the node GPU {G[16, 16, 16]} oracle {FF[16, 16, 16]}")
+    print(f"atom near a node: max {err.max():.3g}; at the node GPU {G[16, 16, 16]} oracle {FF[16, 16, 16]}")
     assert err[16, 16, 16] < 1e-6
     assert err.max() < 1e-3
 

@@ -87,6 +87,13 @@ def test_relax_c0(c0):
     assert npos == it.size
     assert nev == int(git.sum()) + 2 * it.size - int((git == 1000).sum())  # evals = iters+1 (+1 sample), iters when capped
     assert d.max() < POS_TOL
+    # per-position status: the cap (itr = 1000 in the oracle, which follows
+    # relaxTipStroke) and no non-finite position
+    st = s.status()
+    print(f"status: capped {(st & 1).sum()} (oracle {(it == 1000).sum()}), non-finite {(st & 2).sum() // 2}")
+    assert np.array_equal(st & 1, (git == 1000).astype(np.uint8))
+    assert not (st & 2).any()
+    assert abs(int((st & 1).sum()) - int((it == 1000).sum())) <= max(2, it.size // 10000)
 
 
 def test_force_and_df_c0(c0):

@@ -84,3 +84,30 @@ def test_training_reduces_loss():
     Lr, _ = W.loss_grad(th.astype(np.float64), Pr, T[:1024].astype(np.float64), bf=True)
     L, _ = net.loss_grad(D.DeviceArray.from_host(WS.pack_patches(P[:1024])), D.DeviceArray.from_host(T[:1024]))
     assert abs(L - Lr) <= 1e-2 * Lr
+
+
+def test_checkpoint_resumes_training(tmp_path):
+    """Train, save (parameters, Adam moments, step, seed), train on; a network
+    loaded from the checkpoint and trained on the same minibatches ends at the
+    same parameters and step."""
+    print("case: warm-start network checkpoint and resume", flush=True)
+    rng = np.random.default_rng(11)
+    n = 4096
+    P = D.DeviceArray.from_host(WS.pack_patches(rng.normal(size=(n, 5, 5, 3, 3)).astype(np.float32)))
+    T = D.DeviceArray.from_host(rng.normal(size=(n, 3)).astype(np.float32))
+    net = WS.WarmStartNet(max_batch=1024, seed=5)
+    net.train(P, T, batch=1024, steps=6, lr=1e-3)
+    path = tmp_path / "ck.npz"
+    net.save(path, meta={"pairs": n})
+    th0, m10, m20, step0 = net.state()
+    net.train(P, T, batch=1024, steps=4, lr=1e-3)
+    a = net.state()
+    net2 = WS.WarmStartNet.load(path, max_batch=1024)
+    th1, m11, m21, step1 = net2.state()
+    assert np.array_equal(th0, th1) and np.array_equal(m10, m11) and np.array_equal(m20, m21) and step0 == step1 == 6
+    assert net2.seed == 5 and net2.meta == {"pairs": n}
+    net2.train(P, T, batch=1024, steps=4, lr=1e-3)
+    b = net2.state()
+    assert a[3] == b[3] == 10
+    for x, y in zip(a[:3], b[:3]):
+        assert np.allclose(x, y, rtol=1e-5, atol=1e-7)

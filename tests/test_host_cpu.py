@@ -152,3 +152,29 @@ def test_molecule_generator_properties():
     # batch config: atom count drawn from U[8,120]
     counts = [len(M.config_molecule(2, i)[0]) for i in range(40)]
     assert min(counts) >= 8 and max(counts) <= 120 and len(set(counts)) > 10
+
+
+def test_warmstart_checkpoint_reader(tmp_path):
+    """The checkpoint format (WarmStartNet.save) read back, and malformed
+    files rejected, without a GPU."""
+    import json
+
+    from ppafm_amd import warmstart as W
+
+    rng = np.random.default_rng(0)
+    th, m1, m2 = (rng.normal(size=W.NPARAM).astype(np.float32) for _ in range(3))
+    p = tmp_path / "ck.npz"
+    np.savez(p, format=np.int64(W.CHECKPOINT_FORMAT), theta=th, m1=m1, m2=m2, step=np.int64(77), seed=np.int64(3),
+             meta=np.array(json.dumps({"pairs": 123})))
+    d = W.read_checkpoint(p)
+    assert np.array_equal(d["theta"], th) and np.array_equal(d["m1"], m1) and np.array_equal(d["m2"], m2)
+    assert d["step"] == 77 and d["seed"] == 3 and d["meta"] == {"pairs": 123}
+    bad = tmp_path / "bad.npz"
+    np.savez(bad, format=np.int64(W.CHECKPOINT_FORMAT), theta=th[:-1], m1=m1, m2=m2, step=np.int64(1), seed=np.int64(0),
+             meta=np.array("{}"))
+    with pytest.raises(ValueError):
+        W.read_checkpoint(bad)
+    old = tmp_path / "old.npz"
+    np.savez(old, format=np.int64(0), theta=th, m1=m1, m2=m2, step=np.int64(1), seed=np.int64(0), meta=np.array("{}"))
+    with pytest.raises(ValueError):
+        W.read_checkpoint(old)

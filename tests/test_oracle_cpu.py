@@ -567,11 +567,14 @@ def test_warm_results_are_stable_minima():
     f, J = O.force_jac(n, cell, F, _tip(), tips.reshape(-1, 3), rw.reshape(-1, 3))
     fn = np.linalg.norm(f, axis=1)
     # the positions a Newton iterate or FIRE's first evaluation at a Newton
-    # zero ended: the reference's stop rule holds there, at a stable point
+    # zero ended (no FIRE move): the reference's stop rule holds there, at a
+    # stable point (FIRE itself, in the few fallbacks, may stop at any point
+    # that meets the rule, as the reference's does)
     conv = fn**2 < 1e-8
     assert conv.mean() > 0.9
-    assert all(O.stable3(Jk) for Jk in J[conv])
-    assert (iw.reshape(-1)[conv] == 0).mean() > 0.9  # ended without a FIRE move
+    newton_ended = conv & (iw.reshape(-1) == 0)
+    assert all(O.stable3(Jk) for Jk in J[newton_ended])
+    assert newton_ended.sum() > 0.9 * conv.sum()
     # the sample after FIRE's stop (|F|^2 < 1e-8 before its last move)
     assert np.percentile(fn, 99) < 1e-3
     # plain FIRE stops within its stop-rule scatter of the same minima

@@ -91,14 +91,19 @@ MUTANTS = [
 
 
 def _build(tmp, name, old, new):
-    src = ORACLE_C.read_text()
-    assert src.count(old) == 1, f"{name}: the line to change is not unique in pp_oracle.c"
+    full = ORACLE_C.read_text()
+    # the change goes into orc_relax_strokes_tilted only
+    i = full.index("void orc_relax_strokes_tilted(int npix")
+    j = full.index("/* ----", i)
+    src = full[i:j]
+    assert src.count(old) == 1, f"{name}: the line to change is not unique in orc_relax_strokes_tilted"
     if name == "test_before_move":
         anchor = "                double vf = v[0] * f[0] + v[1] * f[1] + v[2] * f[2];\n"
         assert src.count(anchor) == 1
         src = src.replace(old, new).replace(anchor, anchor + "                if (ff2 < conv_f2) break;\n")
     else:
         src = src.replace(old, new)
+    src = full[:i] + src + full[j:]
     c_file = tmp / f"{name}.c"
     so = tmp / f"{name}.so"
     c_file.write_text(src)
