@@ -207,6 +207,18 @@ int ppafm_ff_build_dev(const ppafm_grid *grid, const ppafm_ff_model *model, int 
 int ppafm_ff_build(const ppafm_grid *grid, const ppafm_ff_model *model, int natoms, const double *atoms_xyzq,
                    const double *cLJs, void *out_dev, void *stream);
 
+/* ppafm_ff_build without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (serving loops,
+ * harnesses that forbid hipMalloc).  The packed atoms are staged in a fixed
+ * array of 65536 records (3 MiB) in device memory of each GPU the library
+ * runs on, held until the process ends; natoms above 65536 is an error.
+ * Calls on one GPU share that array and keep the last call's atoms: calls
+ * on one stream are ordered, calls that overlap on one GPU race.  Waits for
+ * earlier work on the stream before it stages the atoms (a stream
+ * synchronisation, so the call cannot be recorded into a HIP graph). */
+int ppafm_ff_build_noalloc(const ppafm_grid *grid, const ppafm_ff_model *model, int natoms, const double *atoms_xyzq,
+                           const double *cLJs, void *out_dev, void *stream);
+
 /* Relaxation semantics.
  *  mode 0 (ProbeParticle.cpp): FIRE state (dt, alpha) reset for every tip
  *         position; output force = grid force sampled at the final position.

@@ -124,6 +124,7 @@ SIGNATURES = [
     ("ppafm_pack_atoms", c_int, [c_int, _dp, _dp, c_double, c_void_p]),
     ("ppafm_ff_build_dev", c_int, [ctypes.POINTER(Grid), ctypes.POINTER(FFModel), c_int, c_void_p, c_void_p, c_void_p]),
     ("ppafm_ff_build", c_int, [ctypes.POINTER(Grid), ctypes.POINTER(FFModel), c_int, _dp, _dp, c_void_p, c_void_p]),
+    ("ppafm_ff_build_noalloc", c_int, [ctypes.POINTER(Grid), ctypes.POINTER(FFModel), c_int, _dp, _dp, c_void_p, c_void_p]),
     (
         "ppafm_relax",
         c_int,
