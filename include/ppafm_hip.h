@@ -407,10 +407,10 @@ typedef struct ppafm_warm_params {
     int newton_retry;  /* FIRE iterations between Newton tries from FIRE's position, 0..1000 (0: none) */
     double newton_tol; /* A: a shorter Newton step means converged */
     double newton_cap; /* A: longest Newton step */
-    int extrapolate;   /* 1: steps >= 2 start their Newton run from the continuation moved by the
-                          change of the probe's offset from the tip over the last two steps,
-                          u(s-1) - u(s-2) (a second-order predictor); FIRE fallbacks still start
-                          from the continuation itself */
+    double extrap_max; /* A, 0 = off: steps >= 2 start their Newton run from the continuation
+                          moved by the change of the probe's offset from the tip over the last
+                          two steps, du = u(s-1) - u(s-2) (a second-order predictor), where
+                          |du| <= extrap_max; FIRE fallbacks still start from the continuation */
 } ppafm_warm_params;
 int ppafm_batch_set_warmstart(ppafm_batch *b, const ppafm_warm_params *p, ppafm_ws_model *m);
 /* FIRE from the plan's relaxed positions (buffer 1) to |F|^2 < conv_f2

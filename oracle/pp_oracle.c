@@ -710,7 +710,7 @@ static int fire_retry(const orc_field *ff, const orc_tip *tip, const orc_relax *
  * start. */
 static int warm_stroke(const orc_field *ff, const orc_tip *tip, const orc_relax *rp, int nstep,
                        const double *rtips, const double *start0, int newton_max, int newton_max0, int retry,
-                       double tol, double cap, int extrap, double *rs, double *fs, int *iters, int *nevals)
+                       double tol, double cap, double extrap, double *rs, double *fs, int *iters, int *nevals)
 {
     v3 rtip = {rtips[0], rtips[1], rtips[2]};
     v3 rp_ = {rtip.x + tip->rpp0[0], rtip.y + tip->rpp0[1], rtip.z + tip->rpp0[2]};
@@ -726,8 +726,11 @@ static int warm_stroke(const orc_field *ff, const orc_tip *tip, const orc_relax 
         v3 q = rp_;
         if (i == 0 && start0) q = (v3){start0[0], start0[1], start0[2]};
         /* second-order predictor: the offset drp of the last step keeps its
-           last change drp - u1 */
-        if (extrap && i >= 2 && kmax > 0) { q.x += drp.x - u1.x; q.y += drp.y - u1.y; q.z += drp.z - u1.z; }
+           last change drp - u1 when that is at most extrap long */
+        if (extrap > 0 && i >= 2 && kmax > 0) {
+            const v3 e = {drp.x - u1.x, drp.y - u1.y, drp.z - u1.z};
+            if (e.x * e.x + e.y * e.y + e.z * e.z <= extrap * extrap) { q.x += e.x; q.y += e.y; q.z += e.z; }
+        }
         if (i >= 1) u1 = drp;
         const int res = kmax == 0 ? -1 : newton_run(ff, tip, rtip, &q, kmax, tol, cap, rp->conv_f2, &ne);
         if (res == -1) {
@@ -759,7 +762,7 @@ static int warm_stroke(const orc_field *ff, const orc_tip *tip, const orc_relax 
  * same column order.  Returns the total number of fallback steps. */
 long orc_relax_strokes_warm(int nx, int ny, int nz, const int n[3], const double cell[9], const double *F,
                             const double tip[8], const double relax[7], const double *rtips, const double *starts0,
-                            int newton_max, int newton_max0, int retry, double tol, double cap, int extrap,
+                            int newton_max, int newton_max0, int retry, double tol, double cap, double extrap,
                             double *rs, double *fs, int *iters, int *nevals)
 {
     orc_field ff; orc_tip t; orc_relax r;
@@ -1409,7 +1412,7 @@ long orc_relax_strokes_tilted_warm(int npix, int nz, const int n[3], const doubl
                                    const double tipRot[12], const double stiffness[4], const double dpos0[4],
                                    const double relax_params[2], double surf_ff, int max_iters, double conv_f2,
                                    double f2safe, const double *poss, const double *starts0, int newton_max,
-                                   int newton_max0, int retry, double tol, double cap, int extrap, double *FEs,
+                                   int newton_max0, int retry, double tol, double cap, double extrap, double *FEs,
                                    double *paths, int *iters, int *nevals)
 {
     tw_field t;
@@ -1443,7 +1446,10 @@ long orc_relax_strokes_tilted_warm(int npix, int nz, const int n[3], const doubl
             if (iz == 0 && starts0 && isfinite(starts0[3 * ip]) && isfinite(starts0[3 * ip + 1]) && isfinite(starts0[3 * ip + 2]))
                 q = (v3){starts0[3 * ip], starts0[3 * ip + 1], starts0[3 * ip + 2]};
             /* second-order predictor (as orc_relax_strokes_warm) */
-            if (extrap && iz >= 2 && kmax > 0) { q.x += u1.x - u2.x; q.y += u1.y - u2.y; q.z += u1.z - u2.z; }
+            if (extrap > 0 && iz >= 2 && kmax > 0) {
+                const v3 e = {u1.x - u2.x, u1.y - u2.y, u1.z - u2.z};
+                if (e.x * e.x + e.y * e.y + e.z * e.z <= extrap * extrap) { q.x += e.x; q.y += e.y; q.z += e.z; }
+            }
             const int res = kmax == 0 ? -1 : tw_newton_run(&t, tp, &q, kmax, tol, cap, conv_f2, &ne, fe);
             if (res == -1) {
                 r = q;

@@ -337,7 +337,7 @@ class BatchScan:
 
     # ------------------------------------------------------ learned warm start
     def set_warmstart(self, net=None, enable=True, net_passes=1, newton_max=16, newton_max0=12, newton_retry=16,
-                      newton_tol=1e-3, newton_cap=0.3, extrapolate=True):
+                      newton_tol=1e-3, newton_cap=0.3, extrap_max=1.0):
         """Warm starts (``enable`` False: plain FIRE).  Every step first runs
         up to ``newton_max`` Newton iterations on the interpolated field from
         the continuation start (step 0: ``newton_max0`` from the network's
@@ -347,12 +347,13 @@ class BatchScan:
         converged and FIRE starts there, otherwise FIRE starts from the
         continuation start (counters()[2]).  Every ``newton_retry`` FIRE
         iterations Newton is tried again from FIRE's position.  FIRE ends
-        every step by the plan's stop rule.  ``extrapolate``: steps from the third on
-        start their Newton run from the continuation moved by the last change
-        of the probe's offset from the tip (fallbacks still start FIRE from
-        the continuation itself)."""
+        every step by the plan's stop rule.  ``extrap_max`` (A, 0 = off):
+        steps from the third on start their Newton run from the continuation
+        moved by the last change of the probe's offset from the tip, where
+        that change is at most ``extrap_max`` long (fallbacks still start
+        FIRE from the continuation itself)."""
         p = _lib.WarmParams(int(bool(enable)), int(net_passes), int(newton_max), int(newton_max0), int(newton_retry),
-                            float(newton_tol), float(newton_cap), int(bool(extrapolate)))
+                            float(newton_tol), float(newton_cap), float(extrap_max))
         self._ws = net if enable else None
         check(lib().ppafm_batch_set_warmstart(self._h, ctypes.byref(p), net._h if (net is not None and enable) else None),
               "batch_set_warmstart")

@@ -201,7 +201,7 @@ struct RelaxBatchLaunch {
     const float *pred = nullptr;
     int newton_max = 0, newton_max0 = 0, newton_retry = 0;
     double newton_tol = 0, newton_cap = 0;
-    bool newton_extrap = false;  // second-order predictor for steps >= 2
+    double newton_extrap = 0;  // second-order predictor for steps >= 2 up to this step (A), 0 = off
 };
 // Tiles of one scan for a given tile_y (also returns tiles along y).
 int relax_batch_tiles(int nx, int ny, int tile_y, int &tiles_y);

@@ -133,9 +133,10 @@ __device__ __forceinline__ void column_step(const DevArgs &A, const T4 *__restri
 // One force evaluation of a warm-started column.  Every step begins with
 // Newton iterations on the interpolated field (newton_from) from the
 // continuation start the reference uses (step 0: from the column's estimate,
-// or the plain start without one; with newton_extrap, steps >= 2 from the
-// continuation moved by the last change of the probe's offset from the tip,
-// u(s-1) - u(s-2)), each step cut to at most newton_cap.
+// or the plain start without one; with newton_extrap2 > 0, steps >= 2 from
+// the continuation moved by the last change of the probe's offset from the
+// tip, u(s-1) - u(s-2), when that change is at most sqrt(newton_extrap2)
+// long), each step cut to at most newton_cap.
 // When they reach a stable zero (a full step shorter than sqrt(newton_tol2)
 // with a stable Jacobian at every iterate, at most newton_max of them,
 // newton_max0 at step 0), FIRE runs from there; otherwise (an unstable
@@ -258,8 +259,11 @@ __device__ __forceinline__ void column_step_warm(const DevArgs &A, const T4 *__r
             C.next_try = A.newton_retry;
             C.sample = false;
             C.qx = C.S.rx; C.qy = C.S.ry; C.qz = C.S.rz;
-            if (A.newton_extrap && C.s >= 2) {  // second-order predictor: the offset keeps its last change
-                C.qx += ux - C.ux1; C.qy += uy - C.uy1; C.qz += uz - C.uz1;
+            if (A.newton_extrap2 > 0 && C.s >= 2) {  // second-order predictor: the offset keeps its last change
+                const double ex = ux - C.ux1, ey = uy - C.uy1, ez = uz - C.uz1;
+                if (fma(ex, ex, fma(ey, ey, ez * ez)) <= A.newton_extrap2) {
+                    C.qx += ex; C.qy += ey; C.qz += ez;
+                }
             }
             C.ux1 = ux; C.uy1 = uy; C.uz1 = uz;
             C.nk = 0;
@@ -867,7 +871,7 @@ int launch_relax_batch(const RelaxBatchLaunch &L, hipStream_t st)
         A.newton_tol2 = L.newton_tol * L.newton_tol;
         A.newton_cap = L.newton_cap;
         A.newton_retry = L.newton_retry;
-        A.newton_extrap = L.newton_extrap ? 1 : 0;
+        A.newton_extrap2 = L.newton_extrap * L.newton_extrap;
     }
     BatchArgs B{L.nitems, L.ntiles, L.queue, L.pos, L.force, L.iters, warm ? L.pred : nullptr, L.status};
     if (L.ntiles == 0) return 0;

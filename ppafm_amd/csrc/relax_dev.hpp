@@ -37,7 +37,8 @@ struct DevArgs {
     const float *pred;
     int newton_max, newton_max0, newton_retry;
     double newton_tol2, newton_cap;
-    int newton_extrap;  // 1: steps >= 2 start Newton at the second-order prediction
+    double newton_extrap2;  // > 0: steps >= 2 start Newton at the second-order prediction when its
+                            // shift is at most sqrt(newton_extrap2)
 };
 
 __device__ __forceinline__ double rsq_nr(double x)
