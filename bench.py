@@ -915,7 +915,8 @@ def main():
         else:
             ev = Lr["learned"]["force_evals_per_position"] * npos
             ach = (96 * ev + 24 * npos) / (st["relax"] * 1e-3) / 1e9
-            roof = {"kernel": "relax_refill_kernel (warm-started FIRE, the learned step's longest stage)", "bound": "latency",
+            kname = "relax_refill_kernel" if Lr["learned"]["tuning"].get("refill") else "relax_batch_kernel"
+            roof = {"kernel": f"{kname} (warm-started FIRE, the learned step's longest stage)", "bound": "latency",
                     "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
                     "traffic": prof.get("warm_relax_hbm_bytes_per_launch"),
                     "algorithmic_unit": "96 B per force evaluation + 24 B per relaxed position", "kernel_ms": st["relax"]}
