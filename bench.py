@@ -244,6 +244,16 @@ def _err_stats(d):
     return {"max": float(d.max()), "p99": float(np.percentile(d, 99)), "p50": float(np.median(d))}
 
 
+ITER_BINS = [0, 1, 2, 5, 10, 20, 50, 100, 200, 500, 1000, 1001]
+
+
+def iteration_histogram(it):
+    """Counts of FIRE iterations per position in the bins [0,1), [1,2), [2,5),
+    ..., [500,1000), [1000] (the cap)."""
+    h, _ = np.histogram(np.asarray(it).ravel(), bins=ITER_BINS)
+    return {"bins": ITER_BINS, "counts": [int(v) for v in h]}
+
+
 def run_fp32_batch(args, b64, mols, cfg, lvec):
     """The batch with FP32 grids (every pair FP32, FP64 partial sums): the
     grid stage timed, and its grids, relaxed positions and df against the
@@ -302,15 +312,21 @@ def run_fp32_c1(args, cfg, xyz, Zs, qs, lvec, g64):
     s32, s64 = g["scan"], g64["scan"]
     G64 = s64.grid_host()
     G32 = s32.grid_host().astype(np.float64)
-    gerr = np.linalg.norm(G32[..., :3] - G64[..., :3], axis=-1) / np.maximum(np.linalg.norm(G64[..., :3], axis=-1), 1e-3)
+    fref = np.maximum(np.linalg.norm(G64[..., :3], axis=-1), 1e-3)
+    gerr = np.linalg.norm(G32[..., :3] - G64[..., :3], axis=-1) / fref
+    gcomp = {c: _err_stats(np.abs(G32[..., k] - G64[..., k]) / fref) for k, c in enumerate("xyz")}
     dpos = np.linalg.norm(s32.positions().astype(np.float64) - s64.positions(), axis=-1)
     ddf = np.abs(s32.df_host() - s64.df_host())
+    it32, it64 = s32.iterations(), s64.iterations()
     pairs = int(np.prod(cfg["n"])) * len(xyz)
     return {"ms_per_step": g["t_step"], "stage_ms": {"grid": g["t_grid"], "relax": g["t_relax"], "df": g["t_df"]},
             "positions_per_s": g["n_pos"] / (g["t_step"] * 1e-3), "grid_pts_per_s": int(np.prod(cfg["n"])) / (g["t_grid"] * 1e-3),
             "grid_build_tflops": 32 * pairs / (g["t_grid"] * 1e-3) / 1e12, "grid_rel_err": _err_stats(gerr),
+            "grid_rel_err_per_component": gcomp,
             "positions_err_A": dict(_err_stats(dpos), frac_above_1e_3=float(np.mean(dpos > 1e-3))),
-            "df_err_Hz": _err_stats(ddf)}
+            "df_err_Hz": _err_stats(ddf),
+            "iterations": {"fp32_grid": iteration_histogram(it32), "fp64_grid": iteration_histogram(it64),
+                           "positions_with_other_count": int((it32 != it64).sum())}}
 
 
 def _cpu_env():
@@ -1002,6 +1018,7 @@ def main():
         "grid_pts_per_s": npts / (g["t_grid"] * 1e-3), "relax_positions_per_s": n_pos / (g["t_relax"] * 1e-3),
         "stage_ms": {"grid": g["t_grid"], "relax": g["t_relax"], "df": g["t_df"]},
         "host_wall_ms_per_step": g["wall_step_ms"], "force_evals_per_step": g["n_eval"], "mean_fire_iterations": g["mean_iters"],
+        "iterations_histogram": iteration_histogram(g["scan"].iterations()),
         "roofline": {
             "kernel": "relax_dev_kernel (FIRE, one lane per pixel column)",
             "bound": "latency",

@@ -374,7 +374,7 @@ def stiffness(n, cell, F, tip, rTips, rPPs, which=0, ddisp=0.05, want_dynmat=Fal
     return out + (D,) if want_dynmat else out
 
 
-def relax_strokes_warm(n, cell, F, tip, rTips, starts0=None, newton_max=16, newton_max0=12, retry=16, tol=1e-3,
+def relax_strokes_warm(n, cell, F, tip, rTips, starts0=None, newton_max=16, newton_max0=12, retry=0, tol=1e-3,
                        cap=0.3, relax=DEFAULT_RELAX, extrap=1.0):
     """Warm-start strokes (Newton from the continuation start, then FIRE
     with a Newton try every ``retry`` iterations; probeStart 1) on
@@ -455,7 +455,7 @@ def relax_tilted(FE, dinv, poss, nz, tipRot, stiffness, dpos0, relax_params, sur
 
 
 def relax_tilted_warm(FE, dinv, poss, nz, tipRot, stiffness, dpos0, relax_params, surfFF=0.0, max_iters=128,
-                      conv_f2=1e-8, f2safe=1e-8, starts0=None, newton_max=16, newton_max0=12, retry=16, tol=1e-3, cap=0.3,
+                      conv_f2=1e-8, f2safe=1e-8, starts0=None, newton_max=16, newton_max0=12, retry=0, tol=1e-3, cap=0.3,
                       extrap=1.0):
     """The warm method (see relax_strokes_warm) on relaxStrokesTilted
     semantics, arguments as relax_tilted; starts0 (npix, 3) optional step-0

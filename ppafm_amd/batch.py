@@ -336,7 +336,7 @@ class BatchScan:
         return float(c[0]) / float(c[3]) if c[3] else float("nan")
 
     # ------------------------------------------------------ learned warm start
-    def set_warmstart(self, net=None, enable=True, net_passes=1, newton_max=16, newton_max0=12, newton_retry=16,
+    def set_warmstart(self, net=None, enable=True, net_passes=1, newton_max=16, newton_max0=12, newton_retry=0,
                       newton_tol=1e-3, newton_cap=0.3, extrap_max=1.0):
         """Warm starts (``enable`` False: plain FIRE).  Every step first runs
         up to ``newton_max`` Newton iterations on the interpolated field from

@@ -41,15 +41,13 @@ struct DevArgs {
                             // shift is at most sqrt(newton_extrap2)
 };
 
+// 1/sqrt(x): v_rsq_f64 (5.2e-8 relative on gfx950) and one Newton step
+// (4.3e-15 relative, measured with a probe kernel)
 __device__ __forceinline__ double rsq_nr(double x)
 {
-    double y = __builtin_amdgcn_rsq(x);
-#pragma unroll
-    for (int k = 0; k < 2; k++) {
-        const double h = 0.5 * x * y;
-        y = fma(y, fma(-h, y, 0.5), y);
-    }
-    return y;
+    const double y = __builtin_amdgcn_rsq(x);
+    const double h = 0.5 * x * y;
+    return fma(y, fma(-h, y, 0.5), y);
 }
 
 // grid index coordinates of a position (rows of dinv)
@@ -210,7 +208,8 @@ __device__ __forceinline__ double fire_update(const DevArgs &A, double fx, doubl
         S.dt = MODE == 0 ? S.dt * A.fdec : fmax(A.dtmin, S.dt * A.fdec);
         S.ac = A.a0;
     } else {
-        const double cf = vv > 0 ? S.ac * (vv * rsq_nr(vv)) * rsq_nr(MODE == 0 ? ff : ff + A.f2safe) : 0.0;
+        // alpha |v| / |f| = alpha v^2 / sqrt(v^2 f^2): one reciprocal square root
+        const double cf = vv > 0 ? S.ac * (vv * rsq_nr(vv * (MODE == 0 ? ff : ff + A.f2safe))) : 0.0;
         const double cv = 1 - S.ac;
         S.vx = fma(fx, cf, S.vx * cv);
         S.vy = fma(fy, cf, S.vy * cv);
