@@ -222,11 +222,15 @@ __device__ __forceinline__ void pair_fp64_fast(const FFArgs &A, double dx, doubl
         ir20 = ir0 * ir0;
     }
     if (LJ) {
+        // c6 and c12 arrive as 6 c6 and 12 c12 (scaled when staged):
+        //   s = (12 c12 r^-12 - 6 c6 r^-6) r^-2 = fma(12 c12, r^-6, -6 c6) r^-8,
+        //   12 E = r^-6 fma(12 c12, r^-6, -12 c6)  (E = c12 r^-12 - c6 r^-6; the
+        //   epilogue divides the sum by 12): 8 FP64 instructions for the force
+        //   instead of 11
         const double ir6 = ir20 * ir20 * ir20;
-        const double e6 = c6 * ir6, e12 = c12 * ir6 * ir6;
-        const double s = __fma_rn(12.0, e12, -6.0 * e6) * ir20;
+        const double s = __fma_rn(c12, ir6, -c6) * (ir6 * ir20);
         aL[0] = __fma_rn(dx, s, aL[0]); aL[1] = __fma_rn(dy, s, aL[1]); aL[2] = __fma_rn(dz, s, aL[2]);
-        if (WE) aL[3] += e12 - e6;
+        if (WE) aL[3] = __fma_rn(ir6, __fma_rn(c12, ir6, -2.0 * c6), aL[3]);
     }
     // QS (batches holding uncharged atoms): every lane of the brick reads the
     // same atom, so this branch is uniform; an uncharged atom's Coulomb terms
@@ -295,7 +299,7 @@ __device__ __forceinline__ void ff_brick_f64(const FFArgs &A, const AtomRec *__r
         for (int j = t; j < nt; j += kThreads) {
             const AtomRec a = atoms[i0 + j];
             lpos[j] = make_double4(a.x - ccx, a.y - ccy, a.z - ccz, a.q);
-            llj[j] = make_double2(a.c6, a.c12);
+            llj[j] = make_double2(6.0 * a.c6, 12.0 * a.c12);  // the scaling pair_fp64_fast expects
         }
         __syncthreads();
         if constexpr (ORTHO) {
@@ -330,6 +334,7 @@ __device__ __forceinline__ void ff_brick_f64(const FFArgs &A, const AtomRec *__r
     for (int k = 0; k < kK; k++) {
         const int iz = z0 + zl + kZLanes * k;
         if (iz >= A.n[2]) break;
+        if (WE) aL[k][3] *= 1.0 / 12.0;  // the pairs summed 12 E
         double gx, gy, gz, e;
         if (A.mode == 0) {
             clamp_vec(aL[k][0], aL[k][1], aL[k][2], A.fmax);
