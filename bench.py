@@ -400,6 +400,7 @@ def run_learned(args, b):
     figures), then plain FIRE, warm starts without the network (Newton from
     the plain start at step 0) and with it, each as the whole captured
     graph (grid build, relaxation, df) timed over 3 runs."""
+    from ppafm_amd import batch as B
     from ppafm_amd import device as D
     from ppafm_amd import learn as L
     from ppafm_amd.warmstart import NPARAM, WarmStartNet
@@ -477,6 +478,8 @@ def run_learned(args, b):
         if "estimate_vs_min" in q:
             out[k]["held_out_step0_estimate_vs_min_A"] = q["estimate_vs_min"]
     out["learned"]["stage_ms"] = stage
+    # step 0 is plain FIRE where the total force at the estimate exceeds this (counted in fallback_steps)
+    out["learned"]["residual_max_eV_per_A"] = B.RESIDUAL_MAX
     out["newton"]["stage_ms"] = stage_newton
     out["plain"]["stage_ms"] = stage_plain
     out["learned"]["estimates_mfma"] = {"flop_per_column": flop_col, "columns": ncols,

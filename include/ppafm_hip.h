@@ -411,6 +411,10 @@ typedef struct ppafm_warm_params {
                           moved by the change of the probe's offset from the tip over the last
                           two steps, du = u(s-1) - u(s-2) (a second-order predictor), where
                           |du| <= extrap_max; FIRE fallbacks still start from the continuation */
+    double residual_max; /* eV/A, 0 = off: with a model and newton_max0 > 0, step 0 evaluates the
+                            total force at the network's estimate first; where |F| there exceeds
+                            residual_max (or is not finite) the column's step 0 is plain FIRE from
+                            the reference's start (counted as a fallback step) */
 } ppafm_warm_params;
 int ppafm_batch_set_warmstart(ppafm_batch *b, const ppafm_warm_params *p, ppafm_ws_model *m);
 /* FIRE from the plan's relaxed positions (buffer 1) to |F|^2 < conv_f2

@@ -18,7 +18,7 @@ long orc_relax_strokes(int nx, int ny, int nz, const int n[3], const double cell
 long orc_relax_strokes_warm(int nx, int ny, int nz, const int n[3], const double cell[9], const double *F,
                             const double tip[8], const double relax[7], const double *rtips, const double *starts0,
                             int newton_max, int newton_max0, int retry, double tol, double cap, double extrap,
-                            double *rs, double *fs, int *iters, int *nevals);
+                            double res_max, double *rs, double *fs, int *iters, int *nevals);
 long orc_stiffness(const int n[3], const double cell[9], const double *F, const double tip[8], double ddisp, int which,
                    long np, const double *rtips, const double *rpps, double *evals, double *ev1, double *ev2,
                    double *ev3, double *dynmat);
@@ -33,8 +33,8 @@ long orc_relax_strokes_tilted_warm(int npix, int nz, const int n[3], const doubl
                                    const double tipRot[12], const double stiffness[4], const double dpos0[4],
                                    const double relax_params[2], double surf_ff, int max_iters, double conv_f2,
                                    double f2safe, const double *poss, const double *starts0, int newton_max,
-                                   int newton_max0, int retry, double tol, double cap, double extrap, double *FEs,
-                                   double *paths, int *iters, int *nevals);
+                                   int newton_max0, int retry, double tol, double cap, double extrap,
+                                   double res_max, double *FEs, double *paths, int *iters, int *nevals);
 void orc_model_ff(int model, double par, const int n[3], const double cell[9], int natoms, const double *Rs,
                   const double *coefs, int ncoef, double *F, double *E);
 void orc_convolve_z(long ncol, int nzin, int nw, const double *in, const double *w, double *out);
@@ -100,7 +100,7 @@ int main(void)
     const double relax[7] = {1000, 1e-8, 0.1, 0.1, 1.1, 0.5, 0.99};
     for (int ps = -1; ps <= 1; ps++) orc_relax_strokes(sx, sy, sz, n, cell, F, tip, relax, ps, 1, tips, rs, fs, it, ne);
     orc_relax_strokes(sx, sy, sz, n, cell, F, tip, relax, 1, 0, tips, rs, fs, it, ne);
-    orc_relax_strokes_warm(sx, sy, sz, n, cell, F, tip, relax, tips, 0, 8, 12, 8, 1e-3, 0.1, 0.05, rs, fs, it, ne);
+    orc_relax_strokes_warm(sx, sy, sz, n, cell, F, tip, relax, tips, 0, 8, 12, 8, 1e-3, 0.1, 0.05, 0.0, rs, fs, it, ne);
     double *ev = take(sizeof(double) * 3 * ns), *v1 = take(sizeof(double) * 3 * ns), *v2 = take(sizeof(double) * 3 * ns),
            *v3 = take(sizeof(double) * 3 * ns), *dm = take(sizeof(double) * 9 * ns);
     orc_stiffness(n, cell, F, tip, 0.05, 3, ns, tips, rs, ev, v1, v2, v3, dm);
@@ -127,7 +127,10 @@ int main(void)
     }
     orc_relax_strokes_tilted(npix, sz, n, FE, dinv, rot, st, d0, rp, 0.01, 128, 1e-8, 1e-8, poss, FEs, paths, ne);
     orc_relax_strokes_tilted_warm(npix, sz, n, FE, dinv, rot, st, d0, rp, 0.01, 128, 1e-8, 1e-8, poss, starts, 8, 12, 8,
-                                  1e-3, 0.1, 0.05, FEs, paths, it, ne);
+                                  1e-3, 0.1, 0.05, 0.5, FEs, paths, it, ne);
+    /* step-0 estimates with the residual gate (starts: one per column, C order of the columns) */
+    orc_relax_strokes_warm(sx, sy, sz, n, cell, F, tip, relax, tips, starts, 8, 12, 8, 1e-3, 0.1, 0.05, 0.5, rs, fs, it,
+                           ne);
     double *w = take(sizeof(double) * 4), *cv = take(sizeof(double) * npix * (sz - 3));
     for (int k = 0; k < 4; k++) w[k] = urand(&seed);
     orc_convolve_z(npix, sz, 4, fs, w, cv);

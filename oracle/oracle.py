@@ -88,7 +88,7 @@ def load(path):
     L.orc_relax_strokes_tilted_warm.argtypes = [
         ctypes.c_int, ctypes.c_int, _ip, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int, ctypes.c_double,
         ctypes.c_double, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
-        ctypes.c_double, _dp, _dp, _ip, _ip,
+        ctypes.c_double, ctypes.c_double, _dp, _dp, _ip, _ip,
     ]
     L.orc_relax_strokes_tilted_warm.restype = ctypes.c_long
     L.orc_tilted_force_jac.argtypes = [_ip, _dp, _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int, _dp, _dp, _dp, _dp]
@@ -98,7 +98,8 @@ def load(path):
     L.orc_force_jac.argtypes = [_ip, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp, _dp]
     L.orc_relax_strokes_warm.argtypes = [ctypes.c_int] * 3 + [_ip, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int,
                                                               ctypes.c_int, ctypes.c_int, ctypes.c_double,
-                                                              ctypes.c_double, ctypes.c_double, _dp, _dp, _ip, _ip]
+                                                              ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp,
+                                                              _dp, _ip, _ip]
     L.orc_relax_strokes_warm.restype = ctypes.c_long
     L.orc_stiffness.argtypes = [_ip, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int, ctypes.c_long, _dp, _dp, _dp,
                                 _dp, _dp, _dp, _dp]
@@ -375,11 +376,13 @@ def stiffness(n, cell, F, tip, rTips, rPPs, which=0, ddisp=0.05, want_dynmat=Fal
 
 
 def relax_strokes_warm(n, cell, F, tip, rTips, starts0=None, newton_max=16, newton_max0=12, retry=0, tol=1e-3,
-                       cap=0.3, relax=DEFAULT_RELAX, extrap=1.0):
+                       cap=0.3, relax=DEFAULT_RELAX, extrap=1.0, res_max=0.0):
     """Warm-start strokes (Newton from the continuation start, then FIRE
     with a Newton try every ``retry`` iterations; probeStart 1) on
-    (nx,ny,nz,3) tip positions; starts0 (nx,ny,3) optional step-0 estimates.  Returns rs, fs, iters, nevals (per
-    position) and the number of fallback steps."""
+    (nx,ny,nz,3) tip positions; starts0 (nx,ny,3) optional step-0 estimates;
+    res_max > 0: a column whose total force at its estimate exceeds res_max
+    runs step 0 as plain FIRE from the plain start.  Returns rs, fs, iters,
+    nevals (per position) and the number of fallback steps."""
     rTips = _f64(rTips)
     nx, ny, nz, _ = rTips.shape
     rs = np.zeros_like(rTips)
@@ -390,7 +393,8 @@ def relax_strokes_warm(n, cell, F, tip, rTips, starts0=None, newton_max=16, newt
     st = None if starts0 is None else _f64(starts0, (nx, ny, 3))
     nfb = lib().orc_relax_strokes_warm(nx, ny, nz, _ptr(n, _ip), _ptr(_f64(cell, (3, 3))), _ptr(_f64(F)), _ptr(_f64(tip)),
                                        _ptr(_f64(relax)), _ptr(rTips), None if st is None else _ptr(st), int(newton_max),
-                                       int(newton_max0), int(retry), float(tol), float(cap), float(extrap), _ptr(rs),
+                                       int(newton_max0), int(retry), float(tol), float(cap), float(extrap),
+                                       float(res_max), _ptr(rs),
                                        _ptr(fs), _ptr(iters, _ip),
                                        _ptr(nev, _ip))
     return rs, fs, iters, nev, int(nfb)
@@ -456,11 +460,12 @@ def relax_tilted(FE, dinv, poss, nz, tipRot, stiffness, dpos0, relax_params, sur
 
 def relax_tilted_warm(FE, dinv, poss, nz, tipRot, stiffness, dpos0, relax_params, surfFF=0.0, max_iters=128,
                       conv_f2=1e-8, f2safe=1e-8, starts0=None, newton_max=16, newton_max0=12, retry=0, tol=1e-3, cap=0.3,
-                      extrap=1.0):
+                      extrap=1.0, res_max=0.0):
     """The warm method (see relax_strokes_warm) on relaxStrokesTilted
     semantics, arguments as relax_tilted; starts0 (npix, 3) optional step-0
-    estimates.  Returns FEs, paths (npix, nz, 4), FIRE iterations and force
-    evaluations (npix, nz) and the number of fallback steps."""
+    estimates (res_max: the residual gate of relax_strokes_warm).  Returns
+    FEs, paths (npix, nz, 4), FIRE iterations and force evaluations
+    (npix, nz) and the number of fallback steps."""
     nx, ny, nzg = FE.shape[:3]
     fe_x = np.ascontiguousarray(np.asarray(FE, dtype=np.float64).transpose(2, 1, 0, 3))
     poss = _f64(poss, (-1, 4))
@@ -475,7 +480,7 @@ def relax_tilted_warm(FE, dinv, poss, nz, tipRot, stiffness, dpos0, relax_params
         npix, int(nz), _ptr(n, _ip), _ptr(fe_x), _ptr(_f64(dinv, (3, 3))), _ptr(_f64(tipRot, (3, 4))), _ptr(_f64(stiffness)),
         _ptr(_f64(dpos0)), _ptr(_f64(relax_params[:2])), float(surfFF), int(max_iters), float(conv_f2), float(f2safe),
         _ptr(poss), None if st is None else _ptr(st), int(newton_max), int(newton_max0), int(retry), float(tol), float(cap),
-        float(extrap), _ptr(FEs), _ptr(paths), _ptr(iters, _ip), _ptr(nev, _ip),
+        float(extrap), float(res_max), _ptr(FEs), _ptr(paths), _ptr(iters, _ip), _ptr(nev, _ip),
     )
     return FEs, paths, iters, nev, int(nfb)
 

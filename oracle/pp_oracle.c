@@ -710,7 +710,8 @@ static int fire_retry(const orc_field *ff, const orc_tip *tip, const orc_relax *
  * start. */
 static int warm_stroke(const orc_field *ff, const orc_tip *tip, const orc_relax *rp, int nstep,
                        const double *rtips, const double *start0, int newton_max, int newton_max0, int retry,
-                       double tol, double cap, double extrap, double *rs, double *fs, int *iters, int *nevals)
+                       double tol, double cap, double extrap, double res_max, double *rs, double *fs, int *iters,
+                       int *nevals)
 {
     v3 rtip = {rtips[0], rtips[1], rtips[2]};
     v3 rp_ = {rtip.x + tip->rpp0[0], rtip.y + tip->rpp0[1], rtip.z + tip->rpp0[2]};
@@ -732,7 +733,17 @@ static int warm_stroke(const orc_field *ff, const orc_tip *tip, const orc_relax 
             if (e.x * e.x + e.y * e.y + e.z * e.z <= extrap * extrap) { q.x += e.x; q.y += e.y; q.z += e.z; }
         }
         if (i >= 1) u1 = drp;
-        const int res = kmax == 0 ? -1 : newton_run(ff, tip, rtip, &q, kmax, tol, cap, rp->conv_f2, &ne);
+        /* residual gate: where the total force at the step-0 estimate is
+           larger than res_max (or not finite), step 0 is plain FIRE from the
+           continuation start (one evaluation at the estimate, a fallback) */
+        int gated = 0;
+        if (i == 0 && start0 && res_max > 0 && kmax > 0) {
+            const v3 f0 = pp_force(ff, tip, rtip, q);
+            ne++;
+            gated = !(v3_norm2(f0) <= res_max * res_max);
+            if (!gated) ne--;  /* the Newton run's first evaluation is this one */
+        }
+        const int res = gated ? 0 : kmax == 0 ? -1 : newton_run(ff, tip, rtip, &q, kmax, tol, cap, rp->conv_f2, &ne);
         if (res == -1) {
             rp_ = q;
         } else if (res == 2) {
@@ -763,7 +774,7 @@ static int warm_stroke(const orc_field *ff, const orc_tip *tip, const orc_relax 
 long orc_relax_strokes_warm(int nx, int ny, int nz, const int n[3], const double cell[9], const double *F,
                             const double tip[8], const double relax[7], const double *rtips, const double *starts0,
                             int newton_max, int newton_max0, int retry, double tol, double cap, double extrap,
-                            double *rs, double *fs, int *iters, int *nevals)
+                            double res_max, double *rs, double *fs, int *iters, int *nevals)
 {
     orc_field ff; orc_tip t; orc_relax r;
     make_field(&ff, n, cell, F); make_tip(&t, tip); make_relax(&r, relax);
@@ -773,7 +784,7 @@ long orc_relax_strokes_warm(int nx, int ny, int nz, const int n[3], const double
         for (int iy = 0; iy < ny; iy++) {
             const long c = (long)ix + (long)iy * nx, off = c * nz;
             total += warm_stroke(&ff, &t, &r, nz, rtips + 3 * off, starts0 ? starts0 + 3 * c : 0, newton_max,
-                                 newton_max0, retry, tol, cap, extrap, rs + 3 * off, fs + 3 * off,
+                                 newton_max0, retry, tol, cap, extrap, res_max, rs + 3 * off, fs + 3 * off,
                                  iters ? iters + off : 0, nevals ? nevals + off : 0);
         }
     return total;
@@ -1412,8 +1423,8 @@ long orc_relax_strokes_tilted_warm(int npix, int nz, const int n[3], const doubl
                                    const double tipRot[12], const double stiffness[4], const double dpos0[4],
                                    const double relax_params[2], double surf_ff, int max_iters, double conv_f2,
                                    double f2safe, const double *poss, const double *starts0, int newton_max,
-                                   int newton_max0, int retry, double tol, double cap, double extrap, double *FEs,
-                                   double *paths, int *iters, int *nevals)
+                                   int newton_max0, int retry, double tol, double cap, double extrap,
+                                   double res_max, double *FEs, double *paths, int *iters, int *nevals)
 {
     tw_field t;
     t.FE = FE;
@@ -1443,14 +1454,25 @@ long orc_relax_strokes_tilted_warm(int npix, int nz, const int n[3], const doubl
             double fe[4] = {0, 0, 0, 0};
             const int kmax = iz == 0 ? newton_max0 : newton_max;
             v3 q = r;
-            if (iz == 0 && starts0 && isfinite(starts0[3 * ip]) && isfinite(starts0[3 * ip + 1]) && isfinite(starts0[3 * ip + 2]))
+            int est = 0;
+            if (iz == 0 && starts0 && isfinite(starts0[3 * ip]) && isfinite(starts0[3 * ip + 1]) && isfinite(starts0[3 * ip + 2])) {
                 q = (v3){starts0[3 * ip], starts0[3 * ip + 1], starts0[3 * ip + 2]};
+                est = 1;
+            }
             /* second-order predictor (as orc_relax_strokes_warm) */
             if (extrap > 0 && iz >= 2 && kmax > 0) {
                 const v3 e = {u1.x - u2.x, u1.y - u2.y, u1.z - u2.z};
                 if (e.x * e.x + e.y * e.y + e.z * e.z <= extrap * extrap) { q.x += e.x; q.y += e.y; q.z += e.z; }
             }
-            const int res = kmax == 0 ? -1 : tw_newton_run(&t, tp, &q, kmax, tol, cap, conv_f2, &ne, fe);
+            /* residual gate at the step-0 estimate (as orc_relax_strokes_warm) */
+            int gated = 0;
+            if (est && res_max > 0 && kmax > 0) {
+                double f0[3], J0[3][3], fe0[4];
+                tw_force_jac(&t, tp, q, fe0, f0, J0);
+                gated = !(f0[0] * f0[0] + f0[1] * f0[1] + f0[2] * f0[2] <= res_max * res_max);
+                if (gated) ne++;
+            }
+            const int res = gated ? 0 : kmax == 0 ? -1 : tw_newton_run(&t, tp, &q, kmax, tol, cap, conv_f2, &ne, fe);
             if (res == -1) {
                 r = q;
             } else if (res == 2) {

@@ -88,7 +88,8 @@ class BatchTuning(ctypes.Structure):
 
 class WarmParams(ctypes.Structure):
     _fields_ = [("enable", c_int), ("net_passes", c_int), ("newton_max", c_int), ("newton_max0", c_int),
-                ("newton_retry", c_int), ("newton_tol", c_double), ("newton_cap", c_double), ("extrap_max", c_double)]
+                ("newton_retry", c_int), ("newton_tol", c_double), ("newton_cap", c_double), ("extrap_max", c_double),
+                ("residual_max", c_double)]
 
 
 # (name, restype, argtypes) of every exported symbol declared in the header

@@ -34,6 +34,11 @@ from . import device as D
 from ._lib import HipError, check, lib
 from .scan import plain_setup
 
+# Default step-0 residual gate of the learned warm start (eV/A): an estimate
+# where the total force is larger than this is not used; the column's step 0
+# is plain FIRE from the reference's start.
+RESIDUAL_MAX = 0.5
+
 BUF_GRIDS, BUF_POS, BUF_FORCE, BUF_ITERS, BUF_DF, BUF_COUNTERS, BUF_ATOMS = range(7)
 BUF_STATUS = 8
 
@@ -337,7 +342,7 @@ class BatchScan:
 
     # ------------------------------------------------------ learned warm start
     def set_warmstart(self, net=None, enable=True, net_passes=1, newton_max=16, newton_max0=12, newton_retry=0,
-                      newton_tol=1e-3, newton_cap=0.3, extrap_max=1.0):
+                      newton_tol=1e-3, newton_cap=0.3, extrap_max=1.0, residual_max=RESIDUAL_MAX):
         """Warm starts (``enable`` False: plain FIRE).  Every step first runs
         up to ``newton_max`` Newton iterations on the interpolated field from
         the continuation start (step 0: ``newton_max0`` from the network's
@@ -351,9 +356,13 @@ class BatchScan:
         steps from the third on start their Newton run from the continuation
         moved by the last change of the probe's offset from the tip, where
         that change is at most ``extrap_max`` long (fallbacks still start
-        FIRE from the continuation itself)."""
+        FIRE from the continuation itself).  ``residual_max`` (eV/A, 0 =
+        off): with a network and ``newton_max0`` > 0, the total force at
+        each column's estimate is evaluated first, and where its magnitude
+        exceeds ``residual_max`` that column's step 0 is plain FIRE from the
+        reference's start instead (a fallback step in counters()[2])."""
         p = _lib.WarmParams(int(bool(enable)), int(net_passes), int(newton_max), int(newton_max0), int(newton_retry),
-                            float(newton_tol), float(newton_cap), float(extrap_max))
+                            float(newton_tol), float(newton_cap), float(extrap_max), float(residual_max))
         self._ws = net if enable else None
         check(lib().ppafm_batch_set_warmstart(self._h, ctypes.byref(p), net._h if (net is not None and enable) else None),
               "batch_set_warmstart")

@@ -197,6 +197,7 @@ int enqueue_relax(ppafm_batch &b, const ppafm_batch_tuning &t, const BatchRelaxI
         L.newton_tol = b.warm.newton_tol;
         L.newton_cap = b.warm.newton_cap;
         L.newton_extrap = b.warm.extrap_max;
+        L.residual_max = b.warm.residual_max;
     }
     return launch_relax_batch(L, b.st);
 }
@@ -679,6 +680,7 @@ int ppafm_batch_set_warmstart(ppafm_batch *b, const ppafm_warm_params *p, ppafm_
         PP_REQUIRE(p->newton_tol > 0 && p->newton_tol < 1, "newton_tol must be in (0, 1) A");
         PP_REQUIRE(p->newton_cap > 0 && p->newton_cap <= 1, "newton_cap must be in (0, 1] A");
         PP_REQUIRE(p->extrap_max >= 0 && p->extrap_max <= 1, "extrap_max must be in [0, 1] A");
+        PP_REQUIRE(p->residual_max >= 0 && p->residual_max <= 1e6, "residual_max must be in [0, 1e6] eV/A");
         PP_REQUIRE(!m || (p->net_passes >= 1 && p->net_passes <= 8), "net_passes must be 1..8");
     }
     PP_TRY_HIP(hipStreamSynchronize(b->st));

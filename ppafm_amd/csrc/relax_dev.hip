@@ -47,6 +47,7 @@ struct Column {
     int next_try, nk, kmax;
     double ux1, uy1, uz1;  // probe offset from the tip at the previous step (predictor)
     bool newton, opening, check;
+    bool gate;  // step 0's first Newton evaluation is at the network's estimate: test its residual
 };
 
 // Step 0 of column (i, j) (nothing to do when !live).  pred: the column's
@@ -70,8 +71,10 @@ __device__ __forceinline__ void column_init(const DevArgs &A, Column<T4> &C, int
     if (WARM) {
         C.next_try = A.newton_retry;
         C.qx = C.S.rx; C.qy = C.S.ry; C.qz = C.S.rz;
+        bool est = false;
         if (pred && live && isfinite(pred[0]) && isfinite(pred[1]) && isfinite(pred[2])) {  // else the plain start
             C.qx = (double)pred[0]; C.qy = (double)pred[1]; C.qz = (double)pred[2];
+            est = true;
         }
         C.nk = 0;
         C.ux1 = C.uy1 = C.uz1 = 0;
@@ -79,6 +82,7 @@ __device__ __forceinline__ void column_init(const DevArgs &A, Column<T4> &C, int
         C.newton = live && C.kmax > 0;
         C.opening = C.newton;
         C.check = false;
+        C.gate = C.newton && est && A.ws_res2 > 0;
         if (live && !C.newton && pred) { C.S.rx = C.qx; C.S.ry = C.qy; C.S.rz = C.qz; }  // the estimate as it is
     }
 }
@@ -142,7 +146,10 @@ __device__ __forceinline__ void column_step(const DevArgs &A, const T4 *__restri
 // newton_max0 at step 0), FIRE runs from there; otherwise (an unstable
 // Jacobian, a non-finite step, no convergence) FIRE runs from the
 // continuation start itself, exactly as without warm starts (counted in
-// nfb).  While FIRE runs, every newton_retry of its iterations one Newton
+// nfb).  With ws_res2 > 0 the first evaluation at the network's step-0
+// estimate also tests the estimate's residual: where |F|^2 there exceeds
+// ws_res2 (or is not finite), step 0 is plain FIRE from the plain start
+// (counted in nfb).  While FIRE runs, every newton_retry of its iterations one Newton
 // step is tried from its current position with the corners FIRE just
 // loaded; where the Jacobian is stable Newton iterations take over as above,
 // and on convergence FIRE restarts there (velocity zero; mode 0 also resets
@@ -176,7 +183,13 @@ __device__ __forceinline__ void column_step_warm(const DevArgs &A, const T4 *__r
         const bool ok =
             newton_from<T4, MODE>(A, v, ux, uy, uz, fe, C.tx, C.ty, C.tz, px, py, pz, A.newton_cap, stable, st2, f2) && stable;
         if (retry) C.next_try = C.it + A.newton_retry;
-        if (C.newton && stable && f2 < A.conv_f2) {  // the iterate meets the stop rule: the step ends here
+        const bool gated = C.gate && !(f2 <= A.ws_res2);
+        C.gate = false;
+        if (gated) {  // the estimate's residual |F| exceeds the threshold: plain FIRE from the plain start (C.S)
+            nfb++;
+            C.newton = false;
+            C.opening = false;
+        } else if (C.newton && stable && f2 < A.conv_f2) {  // the iterate meets the stop rule: the step ends here
             fire = false;
             C.S.rx = C.qx; C.S.ry = C.qy; C.S.rz = C.qz;
             store_pos(A, o, C.qx, C.qy, C.qz);
@@ -872,6 +885,7 @@ int launch_relax_batch(const RelaxBatchLaunch &L, hipStream_t st)
         A.newton_cap = L.newton_cap;
         A.newton_retry = L.newton_retry;
         A.newton_extrap2 = L.newton_extrap * L.newton_extrap;
+        A.ws_res2 = L.residual_max * L.residual_max;
     }
     BatchArgs B{L.nitems, L.ntiles, L.queue, L.pos, L.force, L.iters, warm ? L.pred : nullptr, L.status};
     if (L.ntiles == 0) return 0;

@@ -39,6 +39,7 @@ struct DevArgs {
     double newton_tol2, newton_cap;
     double newton_extrap2;  // > 0: steps >= 2 start Newton at the second-order prediction when its
                             // shift is at most sqrt(newton_extrap2)
+    double ws_res2;         // > 0: step 0 falls back to plain FIRE where |F|^2 at the estimate exceeds it
 };
 
 // 1/sqrt(x): v_rsq_f64 (5.2e-8 relative on gfx950) and one Newton step

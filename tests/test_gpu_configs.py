@@ -194,7 +194,7 @@ def test_c3_sampled_items_vs_warm_oracle(c3):
         FF, tips = _oracle_field(cfg, lvec, mols[k])
         Fg = np.ascontiguousarray(b.grid(k)[..., :3], dtype=np.float64)
         rg = b.positions_raw(k).astype(np.float64)
-        kw = dict(starts0=b.predictions(k).astype(np.float64))
+        kw = dict(starts0=b.predictions(k).astype(np.float64), res_max=B.RESIDUAL_MAX)
         dg = np.linalg.norm(rg - O.relax_strokes_warm(cfg["n"], lvec[1:], Fg, tip_params(), tips, **kw)[0], axis=-1)
         do = np.linalg.norm(rg - O.relax_strokes_warm(cfg["n"], lvec[1:], FF, tip_params(), tips, **kw)[0], axis=-1)
         print(f"item {k}: on the GPU grid max {dg.max():.3g} p99 {np.percentile(dg, 99):.3g}; on the oracle grid max "

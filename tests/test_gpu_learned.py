@@ -117,7 +117,7 @@ def test_warm_path_matches_fp64_oracle(setup, with_net):
         b.synchronize()
         rg = b.positions_raw(k).astype(np.float64)
         starts0 = b.predictions(k).astype(np.float64) if with_net else None
-        kw = dict(starts0=starts0)
+        kw = dict(starts0=starts0, res_max=B.RESIDUAL_MAX if with_net else 0.0)
         rw_g = O.relax_strokes_warm(cfg["n"], lvec[1:], Fg, tip_params(), tips, **kw)[0]
         rw_o = O.relax_strokes_warm(cfg["n"], lvec[1:], FF, tip_params(), tips, **kw)[0]
         dg = np.linalg.norm(rg - rw_g, axis=-1)
@@ -184,6 +184,39 @@ def test_network_gain(setup):
     # Newton iterations at step 0
     assert r["learned"]["estimate_vs_min"]["p50"] < 0.3 * ru["estimate_vs_min"]["p50"]
     assert e_l < e_n
+
+
+def test_residual_gate(setup):
+    """The step-0 residual gate on the GPU (oracle pins: tests/test_oracle_cpu.py):
+    a threshold every estimate fails reproduces warm starts without a step-0
+    Newton run bit for bit, with one more evaluation and one more fallback per
+    pixel column (the gate's evaluation at the estimate); one no estimate fails
+    reproduces the ungated run; the default gates between those (printed)."""
+    cases, b, net, losses, truth = setup
+    print("case: 12 molecules, residual gate at 1e-9, off, 1e6 and the default", flush=True)
+
+    def run(**kw):
+        b.set_warmstart(**kw)
+        b.run()
+        b.synchronize()
+        return ([b.positions_raw(k).copy() for k in range(NITEMS)], [b.iterations(k).copy() for k in range(NITEMS)],
+                b.counters())
+
+    ref = run(net=None, newton_max0=0)
+    every = run(net=net, residual_max=1e-9)
+    off = run(net=net, residual_max=0.0)
+    none = run(net=net, residual_max=1e6)
+    default = run(net=net)
+    b.set_plain()
+    ncols = ref[2][1] // 20  # every scan has 20 steps
+    for k in range(NITEMS):
+        assert np.array_equal(every[0][k], ref[0][k]) and np.array_equal(every[1][k], ref[1][k]), k
+        assert np.array_equal(none[0][k], off[0][k]) and np.array_equal(none[1][k], off[1][k]), k
+    assert every[2] == (ref[2][0] + ncols, ref[2][1], ref[2][2] + ncols)
+    assert none[2] == off[2]
+    print(f"fallback steps: gate off {off[2][2]}, default ({B.RESIDUAL_MAX} eV/A) {default[2][2]}, every column gated "
+          f"{every[2][2]} ({ncols} columns); evaluations {off[2][0]}, {default[2][0]}, {every[2][0]}")
+    assert off[2][2] <= default[2][2] <= off[2][2] + ncols
 
 
 def test_warm_runs_are_deterministic(setup):

@@ -584,6 +584,48 @@ def test_warm_results_are_stable_minima():
     assert nev.sum() < 0.3 * (it.sum() + npos)
 
 
+def test_residual_gate_selects_plain_fire_per_column():
+    """The step-0 residual gate: where the total force at a column's
+    estimate is larger than res_max, step 0 is plain FIRE from the plain
+    start.  A gate every estimate fails (res_max tiny) reproduces warm starts
+    without a step-0 Newton run bit for bit, with one more evaluation per
+    column (the gate's) and one more fallback per column; a gate none fails
+    (res_max huge) reproduces the ungated run; a threshold between two
+    columns' residuals splits the columns into exactly those two runs, by
+    their own residuals (the force at the estimates from orc_force_jac)."""
+    from cases import case
+    from ppafm_amd import molgen as M
+
+    cs = case(0, small=True)
+    n, cell, F = cs["cfg"]["n"], cs["lvec"][1:], cs["FF"]
+    tips = (M.scan_tips(cs["cfg"]) - cs["lvec"][0])[::3, ::3]
+    nx, ny, nz, _ = tips.shape
+    rs, _, _, _ = O.relax_strokes(n, cell, F, _tip(), tips)
+    rng = np.random.default_rng(11)
+    # estimates: step 0's relaxed positions moved by up to 0.4 A laterally and 0.2 A along z
+    starts = rs[:, :, 0] + rng.uniform(-1, 1, (nx, ny, 3)) * np.array([0.4, 0.4, 0.2])
+    ref = O.relax_strokes_warm(n, cell, F, _tip(), tips, newton_max0=0)
+    off = O.relax_strokes_warm(n, cell, F, _tip(), tips, starts0=starts)
+    every = O.relax_strokes_warm(n, cell, F, _tip(), tips, starts0=starts, res_max=1e-12)
+    none = O.relax_strokes_warm(n, cell, F, _tip(), tips, starts0=starts, res_max=1e6)
+    for a, b in zip(every[:3], ref[:3]):
+        assert np.array_equal(a, b)
+    assert (every[3][:, :, 0] == ref[3][:, :, 0] + 1).all() and np.array_equal(every[3][:, :, 1:], ref[3][:, :, 1:])
+    assert every[4] == ref[4] + nx * ny
+    for a, b in zip(none, off):
+        assert np.array_equal(a, b)
+    f, _ = O.force_jac(n, cell, F, _tip(), tips[:, :, 0].reshape(-1, 3), starts.reshape(-1, 3))
+    res = np.sort(np.linalg.norm(f, axis=1))
+    k = len(res) // 2
+    thr = 0.5 * (res[k - 1] + res[k])
+    assert res[k - 1] < thr < res[k]
+    mid = O.relax_strokes_warm(n, cell, F, _tip(), tips, starts0=starts, res_max=thr)
+    gated = (np.linalg.norm(f, axis=1) > thr).reshape(nx, ny)
+    assert gated.sum() == len(res) - k
+    assert np.array_equal(mid[0][gated], ref[0][gated]) and np.array_equal(mid[0][~gated], off[0][~gated])
+    assert np.array_equal(mid[2][gated], ref[2][gated]) and np.array_equal(mid[2][~gated], off[2][~gated])
+
+
 def test_warm_rejects_saddle_on_a_repulsive_bump():
     F, cell = _bump_field()
     n = F.shape[:3]

@@ -90,3 +90,36 @@ def test_warm_results_meet_the_stop_rule_at_stable_points():
     d = np.linalg.norm(pw[..., :3] - paths[..., :3], axis=-1)
     assert np.percentile(d, 99) < 0.03
     assert nevw.sum() < 0.5 * nev.sum()
+
+
+def test_residual_gate_selects_plain_fire_per_column():
+    """The step-0 residual gate in relaxStrokesTilted semantics (as
+    orc_relax_strokes_warm's, tests/test_oracle_cpu.py): every estimate gated
+    equals warm starts without a step-0 Newton run bit for bit (one more
+    evaluation at step 0 and one more fallback per column), none gated equals
+    the ungated run, and a threshold between two columns' residuals splits
+    the columns by their own residuals (the force from orc_tilted_force_jac)."""
+    FE, dinv, tipRot, st, dpos0, poss = _setup()
+    args = (FE, dinv, poss, 16, tipRot, st, dpos0, (0.5, 0.1))
+    tips0 = poss[:, :3]
+    rng = np.random.default_rng(5)
+    starts = tips0 + tipRot[:, :3].T @ dpos0[:3] + rng.uniform(-1, 1, (len(poss), 3)) * np.array([0.4, 0.4, 0.2])
+    ref = O.relax_tilted_warm(*args, newton_max0=0, surfFF=0.01, **KW)
+    off = O.relax_tilted_warm(*args, starts0=starts, surfFF=0.01, **KW)
+    every = O.relax_tilted_warm(*args, starts0=starts, res_max=1e-12, surfFF=0.01, **KW)
+    none = O.relax_tilted_warm(*args, starts0=starts, res_max=1e6, surfFF=0.01, **KW)
+    for a, b in zip(every[:3], ref[:3]):
+        assert np.array_equal(a, b)
+    assert (every[3][:, 0] == ref[3][:, 0] + 1).all() and np.array_equal(every[3][:, 1:], ref[3][:, 1:])
+    assert every[4] == ref[4] + len(poss)
+    for a, b in zip(none, off):
+        assert np.array_equal(a, b)
+    f, _ = O.tilted_force_jac(FE, dinv, tipRot, st, dpos0, tips0, starts, surfFF=0.01)
+    res = np.sort(np.linalg.norm(f, axis=1))
+    k = len(res) // 2
+    thr = 0.5 * (res[k - 1] + res[k])
+    assert res[k - 1] < thr < res[k]
+    mid = O.relax_tilted_warm(*args, starts0=starts, res_max=thr, surfFF=0.01, **KW)
+    gated = np.linalg.norm(f, axis=1) > thr
+    assert gated.sum() == len(res) - k
+    assert np.array_equal(mid[1][gated], ref[1][gated]) and np.array_equal(mid[1][~gated], off[1][~gated])
