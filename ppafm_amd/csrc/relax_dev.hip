@@ -90,8 +90,8 @@ __device__ __forceinline__ void column_init(const DevArgs &A, Column<T4> &C, int
 // One force evaluation of a plain column: FIRE to convergence at each z
 // step, outputs in stroke order.  Counts force evaluations and finished
 // positions.
-template <typename T4, int MODE>
-__device__ __forceinline__ void column_step(const DevArgs &A, const T4 *__restrict__ grid, Column<T4> &C,
+template <typename T4, int MODE, int K>
+__device__ __forceinline__ void column_step(const DevArgs &A, const T4 *__restrict__ grid, Column<T4> &C, Stage<K> &G,
                                             unsigned long long &nevals, unsigned long long &npos)
 {
     double fe[4], gx, gy, gz;
@@ -101,10 +101,8 @@ __device__ __forceinline__ void column_step(const DevArgs &A, const T4 *__restri
     bool done = false;
     const long o = C.obase + C.c.col0 + C.s;
     if (MODE == 0 && C.sample) {
-        store_pos(A, o, C.S.rx, C.S.ry, C.S.rz);
-        if (A.force) A.force[o] = make_float4((float)fe[0], (float)fe[1], (float)fe[2], 0.f);
-        if (A.iters) A.iters[o] = C.itr;
-        store_status(A, o, C.itr >= A.max_iters, C.S.rx, C.S.ry, C.S.rz, fe);
+        put_record<K>(A, G, o, C.s, C.S.rx, C.S.ry, C.S.rz, make_float4((float)fe[0], (float)fe[1], (float)fe[2], 0.f),
+                      C.itr, A.status ? status_byte(C.itr >= A.max_iters, C.S.rx, C.S.ry, C.S.rz, fe) : 0);
         done = true;
     } else {
         const double ff = fire_move<MODE>(A, fe, C.tx, C.ty, C.tz, C.S);
@@ -115,10 +113,8 @@ __device__ __forceinline__ void column_step(const DevArgs &A, const T4 *__restri
                 C.itr = conv ? C.it - 1 : A.max_iters;
                 C.sample = true;
             } else {
-                if (A.force) A.force[o] = tip_frame_force(A, fe);
-                store_pos(A, o, C.S.rx, C.S.ry, C.S.rz);
-                if (A.iters) A.iters[o] = C.it;
-                store_status(A, o, !conv, C.S.rx, C.S.ry, C.S.rz, fe);
+                put_record<K>(A, G, o, C.s, C.S.rx, C.S.ry, C.S.rz, tip_frame_force(A, fe), C.it,
+                              A.status ? status_byte(!conv, C.S.rx, C.S.ry, C.S.rz, fe) : 0);
                 done = true;
             }
         }
@@ -160,9 +156,9 @@ __device__ __forceinline__ void column_step(const DevArgs &A, const T4 *__restri
 // the next step continues from FIRE's result.  Newton iterates and FIRE
 // moves share one 8-corner load per evaluation; Newton evaluations count in
 // nevals.
-template <typename T4, int MODE>
+template <typename T4, int MODE, int K>
 __device__ __forceinline__ void column_step_warm(const DevArgs &A, const T4 *__restrict__ grid, Column<T4> &C,
-                                                 unsigned long long &nevals, unsigned long long &npos,
+                                                 Stage<K> &G, unsigned long long &nevals, unsigned long long &npos,
                                                  unsigned long long &nfb)
 {
     double gx, gy, gz, ux, uy, uz;
@@ -192,11 +188,9 @@ __device__ __forceinline__ void column_step_warm(const DevArgs &A, const T4 *__r
         } else if (C.newton && stable && f2 < A.conv_f2) {  // the iterate meets the stop rule: the step ends here
             fire = false;
             C.S.rx = C.qx; C.S.ry = C.qy; C.S.rz = C.qz;
-            store_pos(A, o, C.qx, C.qy, C.qz);
-            if (A.force)
-                A.force[o] = MODE == 0 ? make_float4((float)fe[0], (float)fe[1], (float)fe[2], 0.f) : tip_frame_force(A, fe);
-            if (A.iters) A.iters[o] = C.it;
-            store_status(A, o, false, C.qx, C.qy, C.qz, fe);
+            put_record<K>(A, G, o, C.s, C.qx, C.qy, C.qz,
+                          MODE == 0 ? make_float4((float)fe[0], (float)fe[1], (float)fe[2], 0.f) : tip_frame_force(A, fe),
+                          C.it, A.status ? status_byte(false, C.qx, C.qy, C.qz, fe) : 0);
             C.newton = false;
             C.opening = false;
             done = true;
@@ -223,10 +217,8 @@ __device__ __forceinline__ void column_step_warm(const DevArgs &A, const T4 *__r
     }
     if (fire) {
         if (MODE == 0 && C.sample) {
-            store_pos(A, o, C.S.rx, C.S.ry, C.S.rz);
-            if (A.force) A.force[o] = make_float4((float)fe[0], (float)fe[1], (float)fe[2], 0.f);
-            if (A.iters) A.iters[o] = C.itr;
-            store_status(A, o, C.itr >= A.max_iters, C.S.rx, C.S.ry, C.S.rz, fe);
+            put_record<K>(A, G, o, C.s, C.S.rx, C.S.ry, C.S.rz, make_float4((float)fe[0], (float)fe[1], (float)fe[2], 0.f),
+                          C.itr, A.status ? status_byte(C.itr >= A.max_iters, C.S.rx, C.S.ry, C.S.rz, fe) : 0);
             done = true;
         } else {
             double fx, fy, fz;
@@ -235,12 +227,9 @@ __device__ __forceinline__ void column_step_warm(const DevArgs &A, const T4 *__r
             const bool settled = C.check && f2 < A.conv_f2;
             C.check = false;
             if (settled) {  // Newton's zero meets the stop rule: the step ends here, no move
-                store_pos(A, o, C.S.rx, C.S.ry, C.S.rz);
-                if (A.force)
-                    A.force[o] = MODE == 0 ? make_float4((float)fe[0], (float)fe[1], (float)fe[2], 0.f)
-                                           : tip_frame_force(A, fe);
-                if (A.iters) A.iters[o] = C.it;
-                store_status(A, o, false, C.S.rx, C.S.ry, C.S.rz, fe);
+                put_record<K>(A, G, o, C.s, C.S.rx, C.S.ry, C.S.rz,
+                              MODE == 0 ? make_float4((float)fe[0], (float)fe[1], (float)fe[2], 0.f) : tip_frame_force(A, fe),
+                              C.it, A.status ? status_byte(false, C.S.rx, C.S.ry, C.S.rz, fe) : 0);
                 done = true;
             } else {
                 const double ff = fire_update<MODE>(A, fx, fy, fz, C.S);
@@ -251,10 +240,8 @@ __device__ __forceinline__ void column_step_warm(const DevArgs &A, const T4 *__r
                         C.itr = conv ? C.it - 1 : A.max_iters;
                         C.sample = true;
                     } else {
-                        if (A.force) A.force[o] = tip_frame_force(A, fe);
-                        store_pos(A, o, C.S.rx, C.S.ry, C.S.rz);
-                        if (A.iters) A.iters[o] = C.it;
-                        store_status(A, o, !conv, C.S.rx, C.S.ry, C.S.rz, fe);
+                        put_record<K>(A, G, o, C.s, C.S.rx, C.S.ry, C.S.rz, tip_frame_force(A, fe), C.it,
+                                      A.status ? status_byte(!conv, C.S.rx, C.S.ry, C.S.rz, fe) : 0);
                         done = true;
                     }
                 }
@@ -288,23 +275,23 @@ __device__ __forceinline__ void column_step_warm(const DevArgs &A, const T4 *__r
 }
 
 // One pixel column (i, j) from start to end.
-template <typename T4, int MODE>
+template <typename T4, int MODE, int K>
 __device__ __forceinline__ void relax_column(const DevArgs &A, const T4 *__restrict__ grid, int i, int j, bool live,
-                                             unsigned long long &nevals, unsigned long long &npos)
+                                             Stage<K> &G, unsigned long long &nevals, unsigned long long &npos)
 {
     Column<T4> C;
     column_init<T4, MODE, false>(A, C, i, j, live, 0, nullptr);
-    while (C.s < A.nz) column_step<T4, MODE>(A, grid, C, nevals, npos);
+    while (C.s < A.nz) column_step<T4, MODE, K>(A, grid, C, G, nevals, npos);
 }
 
-template <typename T4, int MODE>
+template <typename T4, int MODE, int K>
 __device__ __forceinline__ void relax_column_warm(const DevArgs &A, const T4 *__restrict__ grid, int i, int j,
-                                                  bool live, unsigned long long &nevals, unsigned long long &npos,
-                                                  unsigned long long &nfb)
+                                                  bool live, Stage<K> &G, unsigned long long &nevals,
+                                                  unsigned long long &npos, unsigned long long &nfb)
 {
     Column<T4> C;
     column_init<T4, MODE, true>(A, C, i, j, live, 0, A.pred ? A.pred + 3 * ((long)i * A.ny + j) : nullptr);
-    while (C.s < A.nz) column_step_warm<T4, MODE>(A, grid, C, nevals, npos, nfb);
+    while (C.s < A.nz) column_step_warm<T4, MODE, K>(A, grid, C, G, nevals, npos, nfb);
 }
 
 // Wave sum of the per-lane counters, one atomic per wave.  `passes` (the
@@ -329,9 +316,22 @@ __device__ __forceinline__ void flush_counters(unsigned long long *counters, uns
     }
 }
 
+// records of 8 steps per lane: 23 KB of LDS per 64-lane block
+constexpr int kStageSingle = 8;
+// warm-started batches: 4 steps per lane, 11.5 KB per wave, so LDS does not
+// limit their occupancy; plain-FIRE batches store directly (staged, they ran
+// 3-7 % slower for the same output)
+template <bool WARM>
+constexpr int stage_batch()
+{
+    return WARM ? 4 : 1;
+}
+
 template <typename T4, int MODE>
 __global__ __launch_bounds__(64) void relax_dev_kernel(DevArgs A, const T4 *__restrict__ grid)
 {
+    __shared__ __align__(16) unsigned char lds[kStageSingle * 64 * kStageRecordBytes];
+    Stage<kStageSingle> G = make_stage<kStageSingle>(lds, 64, threadIdx.x);
     const int per_xcd = (A.ntiles + 7) / 8;
     const int b = blockIdx.x;
     const int tile = (b % 8) * per_xcd + b / 8;
@@ -340,7 +340,7 @@ __global__ __launch_bounds__(64) void relax_dev_kernel(DevArgs A, const T4 *__re
     const int lane = threadIdx.x;
     const int i = ti * 8 + lane / 8, j = tj * 8 + lane % 8;
     unsigned long long nevals = 0, npos = 0;
-    relax_column<T4, MODE>(A, grid, i, j, i < A.nx && j < A.ny, nevals, npos);
+    relax_column<T4, MODE, kStageSingle>(A, grid, i, j, i < A.nx && j < A.ny, G, nevals, npos);
     flush_counters(A.counters, nevals, npos, lane);
 }
 
@@ -380,6 +380,9 @@ __global__ __launch_bounds__(256) void relax_batch_kernel(DevArgs A0, BatchArgs 
                                                           const T4 *__restrict__ grids)
 {
     constexpr int TX = 64 / TY;
+    constexpr int KS = stage_batch<WARM>();
+    extern __shared__ __align__(16) unsigned char dyn_lds[];  // blockDim.x * KS records (none when KS = 1)
+    Stage<KS> G = make_stage<KS>(dyn_lds, blockDim.x, threadIdx.x);
     const int lane = threadIdx.x & 63;
     const int xcd = blockIdx.x & 7;
     const long long per = (B.ntiles + 7) / 8;
@@ -415,8 +418,9 @@ __global__ __launch_bounds__(256) void relax_batch_kernel(DevArgs A0, BatchArgs 
             const int ti = lt / it.tiles_y, tj = lt % it.tiles_y;
             const int i = ti * TX + lane / TY, j = tj * TY + lane % TY;
             const unsigned long long before = nevals;
-            if (WARM) relax_column_warm<T4, MODE>(A, grids + it.grid_off, i, j, i < A.nx && j < A.ny, nevals, npos, nfb);
-            else relax_column<T4, MODE>(A, grids + it.grid_off, i, j, i < A.nx && j < A.ny, nevals, npos);
+            if (WARM)
+                relax_column_warm<T4, MODE, KS>(A, grids + it.grid_off, i, j, i < A.nx && j < A.ny, G, nevals, npos, nfb);
+            else relax_column<T4, MODE, KS>(A, grids + it.grid_off, i, j, i < A.nx && j < A.ny, G, nevals, npos);
             // the tile's loop passes: every pass evaluates once in each lane still running
             unsigned long long d = nevals - before;
             for (int off = 32; off > 0; off >>= 1) d = max(d, (unsigned long long)__shfl_xor(d, off, 64));
@@ -444,6 +448,9 @@ __global__ __launch_bounds__(256) void relax_refill_kernel(DevArgs A, BatchArgs 
                                                            const T4 *__restrict__ grids, long long cols_item)
 {
     constexpr int TX = 64 / TY;
+    constexpr int KS = stage_batch<WARM>();
+    extern __shared__ __align__(16) unsigned char dyn_lds[];  // blockDim.x * KS records (none when KS = 1)
+    Stage<KS> G = make_stage<KS>(dyn_lds, blockDim.x, threadIdx.x);
     const int lane = threadIdx.x & 63;
     const int xcd = blockIdx.x & 7;
     const long long ncols = (long long)B.nitems * cols_item;
@@ -491,8 +498,8 @@ __global__ __launch_bounds__(256) void relax_refill_kernel(DevArgs A, BatchArgs 
             continue;
         }
         if (C.s < A.nz) {
-            if (WARM) column_step_warm<T4, MODE>(A, grid, C, nevals, npos, nfb);
-            else column_step<T4, MODE>(A, grid, C, nevals, npos);
+            if (WARM) column_step_warm<T4, MODE, KS>(A, grid, C, G, nevals, npos, nfb);
+            else column_step<T4, MODE, KS>(A, grid, C, G, nevals, npos);
         }
         passes++;
     }
@@ -734,12 +741,13 @@ static int launch_refill_ty(int ty, dim3 grd, dim3 blk, hipStream_t st, const De
                             const BatchRelaxItem *items, const void *grids, long long cols_item)
 {
     const T4 *g = (const T4 *)grids;
+    const size_t lds = stage_batch<WARM>() > 1 ? (size_t)blk.x * stage_batch<WARM>() * kStageRecordBytes : 0;
     switch (ty) {
-    case 1: hipLaunchKernelGGL((relax_refill_kernel<T4, MODE, 1, WARM>), grd, blk, 0, st, A, B, items, g, cols_item); break;
-    case 2: hipLaunchKernelGGL((relax_refill_kernel<T4, MODE, 2, WARM>), grd, blk, 0, st, A, B, items, g, cols_item); break;
-    case 4: hipLaunchKernelGGL((relax_refill_kernel<T4, MODE, 4, WARM>), grd, blk, 0, st, A, B, items, g, cols_item); break;
-    case 8: hipLaunchKernelGGL((relax_refill_kernel<T4, MODE, 8, WARM>), grd, blk, 0, st, A, B, items, g, cols_item); break;
-    case 16: hipLaunchKernelGGL((relax_refill_kernel<T4, MODE, 16, WARM>), grd, blk, 0, st, A, B, items, g, cols_item); break;
+    case 1: hipLaunchKernelGGL((relax_refill_kernel<T4, MODE, 1, WARM>), grd, blk, lds, st, A, B, items, g, cols_item); break;
+    case 2: hipLaunchKernelGGL((relax_refill_kernel<T4, MODE, 2, WARM>), grd, blk, lds, st, A, B, items, g, cols_item); break;
+    case 4: hipLaunchKernelGGL((relax_refill_kernel<T4, MODE, 4, WARM>), grd, blk, lds, st, A, B, items, g, cols_item); break;
+    case 8: hipLaunchKernelGGL((relax_refill_kernel<T4, MODE, 8, WARM>), grd, blk, lds, st, A, B, items, g, cols_item); break;
+    case 16: hipLaunchKernelGGL((relax_refill_kernel<T4, MODE, 16, WARM>), grd, blk, lds, st, A, B, items, g, cols_item); break;
     default: set_error("tile_y must be 1, 2, 4, 8 or 16"); return -2;
     }
     return 0;
@@ -750,12 +758,13 @@ static int launch_batch_ty(int ty, dim3 grd, dim3 blk, hipStream_t st, const Dev
                            const BatchRelaxItem *items, const long long *prefix, const void *grids)
 {
     const T4 *g = (const T4 *)grids;
+    const size_t lds = stage_batch<WARM>() > 1 ? (size_t)blk.x * stage_batch<WARM>() * kStageRecordBytes : 0;
     switch (ty) {
-    case 1: hipLaunchKernelGGL((relax_batch_kernel<T4, MODE, 1, WARM>), grd, blk, 0, st, A, B, items, prefix, g); break;
-    case 2: hipLaunchKernelGGL((relax_batch_kernel<T4, MODE, 2, WARM>), grd, blk, 0, st, A, B, items, prefix, g); break;
-    case 4: hipLaunchKernelGGL((relax_batch_kernel<T4, MODE, 4, WARM>), grd, blk, 0, st, A, B, items, prefix, g); break;
-    case 8: hipLaunchKernelGGL((relax_batch_kernel<T4, MODE, 8, WARM>), grd, blk, 0, st, A, B, items, prefix, g); break;
-    case 16: hipLaunchKernelGGL((relax_batch_kernel<T4, MODE, 16, WARM>), grd, blk, 0, st, A, B, items, prefix, g); break;
+    case 1: hipLaunchKernelGGL((relax_batch_kernel<T4, MODE, 1, WARM>), grd, blk, lds, st, A, B, items, prefix, g); break;
+    case 2: hipLaunchKernelGGL((relax_batch_kernel<T4, MODE, 2, WARM>), grd, blk, lds, st, A, B, items, prefix, g); break;
+    case 4: hipLaunchKernelGGL((relax_batch_kernel<T4, MODE, 4, WARM>), grd, blk, lds, st, A, B, items, prefix, g); break;
+    case 8: hipLaunchKernelGGL((relax_batch_kernel<T4, MODE, 8, WARM>), grd, blk, lds, st, A, B, items, prefix, g); break;
+    case 16: hipLaunchKernelGGL((relax_batch_kernel<T4, MODE, 16, WARM>), grd, blk, lds, st, A, B, items, prefix, g); break;
     default: set_error("tile_y must be 1, 2, 4, 8 or 16"); return -2;
     }
     return 0;

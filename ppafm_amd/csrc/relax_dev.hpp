@@ -371,6 +371,105 @@ __device__ __forceinline__ void store_status(const DevArgs &A, long o, bool capp
     A.status[o] = (unsigned char)((capped ? 1 : 0) | (finite ? 0 : 2));
 }
 
+__device__ __forceinline__ unsigned char status_byte(bool capped, double rx, double ry, double rz, const double fe[4])
+{
+    const bool finite = isfinite(rx) && isfinite(ry) && isfinite(rz) && isfinite(fe[0]) && isfinite(fe[1]) && isfinite(fe[2]);
+    return (unsigned char)((capped ? 1 : 0) | (finite ? 0 : 2));
+}
+
+// Output records staged per lane in LDS.  A lane finishes a z step every
+// ~100 force evaluations, and its column's records lie nz apart from its
+// neighbours', so records written one at a time reach memory as partial
+// lines (c1: 5x the output bytes).  Staged, the records of K consecutive
+// steps of a column go out together: one contiguous chunk per output array.
+// K = 1 stores directly.  Layout (T threads, index k*T + t): force float4,
+// then x, y, z of the position (FP64), iteration count, status byte.
+template <int K>
+struct Stage {
+    float4 *f;
+    double *p;
+    int *it;
+    unsigned char *st;
+    int T, t;
+};
+
+constexpr int kStageRecordBytes = 16 + 24 + 4 + 1;
+
+template <int K>
+__device__ __forceinline__ Stage<K> make_stage(unsigned char *lds, int T, int t)
+{
+    Stage<K> G;
+    G.f = (float4 *)lds;
+    G.p = (double *)(lds + 16 * K * T);
+    G.it = (int *)(lds + 40 * K * T);
+    G.st = lds + 44 * K * T;
+    G.T = T;
+    G.t = t;
+    return G;
+}
+
+// The record of step s of a column (output offset o) into the lane's
+// slot s % K; the slots go out when the chunk is full or s is the last step.
+template <int K>
+__device__ __forceinline__ void put_record(const DevArgs &A, Stage<K> &G, long o, int s, double rx, double ry, double rz,
+                                           float4 f, int iters, unsigned char status)
+{
+    if (K == 1) {
+        store_pos(A, o, rx, ry, rz);
+        if (A.force) A.force[o] = f;
+        if (A.iters) A.iters[o] = iters;
+        if (A.status) A.status[o] = status;
+        return;
+    }
+    const int k = s % K;
+    const int KT = K * G.T;
+    const int i = k * G.T + G.t;
+    G.f[i] = f;
+    G.p[i] = rx; G.p[KT + i] = ry; G.p[2 * KT + i] = rz;
+    G.it[i] = iters;
+    G.st[i] = status;
+    if (K >= 8 && k == K - 1) {  // a full chunk: fixed trip counts, so the LDS reads are issued ahead of the
+                                 // stores (K >= 8 only: the batch kernels, K = 4, would lose a wave per SIMD
+                                 // to the registers this takes)
+        const long o0 = o - (K - 1);
+#pragma unroll
+        for (int j = 0; j < K; j++) {
+            const int q = j * G.T + G.t;
+            store_pos(A, o0 + j, G.p[q], G.p[KT + q], G.p[2 * KT + q]);
+        }
+        if (A.force) {
+            float4 v[K];
+#pragma unroll
+            for (int j = 0; j < K; j++) v[j] = G.f[j * G.T + G.t];
+#pragma unroll
+            for (int j = 0; j < K; j++) A.force[o0 + j] = v[j];
+        }
+        if (A.iters) {
+            int v[K];
+#pragma unroll
+            for (int j = 0; j < K; j++) v[j] = G.it[j * G.T + G.t];
+#pragma unroll
+            for (int j = 0; j < K; j++) A.iters[o0 + j] = v[j];
+        }
+        if (A.status) {
+            unsigned char v[K];
+#pragma unroll
+            for (int j = 0; j < K; j++) v[j] = G.st[j * G.T + G.t];
+#pragma unroll
+            for (int j = 0; j < K; j++) A.status[o0 + j] = v[j];
+        }
+    } else if (k == K - 1 || s == A.nz - 1) {  // a chunk of the batch kernels, or a column's last, partial one
+        const long o0 = o - k;
+        for (int j = 0; j <= k; j++) {
+            const int q = j * G.T + G.t;
+            store_pos(A, o0 + j, G.p[q], G.p[KT + q], G.p[2 * KT + q]);
+            if (A.force) A.force[o0 + j] = G.f[q];
+            if (A.iters) A.iters[o0 + j] = G.it[q];
+            if (A.status) A.status[o0 + j] = G.st[q];
+        }
+    }
+}
+
 // grid force rotated into the tip frame (relax.cl:446-452), .w = energy
 __device__ __forceinline__ float4 tip_frame_force(const DevArgs &A, const double fe[4])
 {
