@@ -2,16 +2,18 @@
 // citations there) with a leaner instruction stream, because each pixel
 // column is a serial chain of ~10^4 dependent force evaluations and the
 // per-iteration latency sets the kernel time:
-//   * 1/l and sqrt(vv/ff) from v_rsq_f64 plus two Newton steps instead of
-//     IEEE divide / square root sequences;
+//   * 1/l and |v|/|f| from v_rsq_f64 plus one Newton step (one reciprocal
+//     square root per FIRE move) instead of IEEE divide / square root
+//     sequences;
 //   * fused multiply-adds written out (contraction is off, so every kernel
 //     that shares these helpers rounds identically: a batch reproduces the
 //     single-scan results bit for bit; this path is not bit-matched to the
 //     CPU reference);
 //   * 32-bit grid indices, ProbeParticle or OpenCL semantics chosen at
 //     compile time, per-lane tip base so the loop touches few uniforms.
-// Lanes run independent state machines over (z step, iteration); an
-// LDS-staged lock-step variant (relax_lds_kernel) is kept as an option.
+// Lanes run independent state machines over (z step, iteration) and stage
+// their output records in LDS (Stage, relax_dev.hpp); an LDS-staged
+// lock-step variant (relax_lds_kernel) is kept as an option.
 
 #include <cstdlib>
 
