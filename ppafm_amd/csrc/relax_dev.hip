@@ -156,8 +156,8 @@ __device__ __forceinline__ void column_step(const DevArgs &A, const T4 *__restri
 // zero already meets the stop rule, the step ends at that point without a
 // move (output force: the grid force there).  Either way FIRE ends the step by the reference's stop rule, and
 // the next step continues from FIRE's result.  Newton iterates and FIRE
-// moves share one 8-corner load per evaluation; Newton evaluations count in
-// nevals.
+// moves share the cell cache of the corners (a load only when the cell
+// changes); Newton evaluations count in nevals.
 template <typename T4, int MODE, int K>
 __device__ __forceinline__ void column_step_warm(const DevArgs &A, const T4 *__restrict__ grid, Column<T4> &C,
                                                  Stage<K> &G, unsigned long long &nevals, unsigned long long &npos,
@@ -165,8 +165,9 @@ __device__ __forceinline__ void column_step_warm(const DevArgs &A, const T4 *__r
 {
     double gx, gy, gz, ux, uy, uz;
     to_grid(A, C.newton ? C.qx : C.S.rx, C.newton ? C.qy : C.S.ry, C.newton ? C.qz : C.S.rz, gx, gy, gz);
-    T4 v[8];
-    corners(grid, A, gx, gy, gz, v, ux, uy, uz);
+    // the cell cache: Newton iterates near convergence and FIRE's moves mostly stay in one cell
+    corners_cached(grid, A, gx, gy, gz, C.cache, ux, uy, uz);
+    const T4 *v = C.cache.v;
     nevals++;
     bool done = false;
     const long o = C.obase + C.c.col0 + C.s;
