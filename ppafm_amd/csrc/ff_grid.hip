@@ -197,36 +197,37 @@ __device__ __forceinline__ void ff_brick_f32(const FFArgs &A, const AtomRec *__r
     }
 }
 
-// 1/sqrt(x): v_rsq_f64 (measured on gfx950: 5.2e-8 relative) and one
-// Newton step (4.3e-15 relative; a second would give 2.6e-16 for 4 more
-// FP64 instructions per pair, ~13% of the pair's cost)
-__device__ __forceinline__ double rsq_nr(double x)
+// 2/sqrt(x): v_rsq_f64 (measured on gfx950: 5.2e-8 relative) refined by
+// one Newton step, y (3 - x y^2), which is twice the step y + y (1/2 - x y^2 / 2)
+// to 1/sqrt(x) (4.3e-15 relative) in three FP64 instructions instead of four
+__device__ __forceinline__ double rsq2_nr(double x)
 {
     const double y = __builtin_amdgcn_rsq(x);
-    const double h = 0.5 * x * y;
-    return __fma_rn(y, __fma_rn(-h, y, 0.5), y);
+    return y * __fma_rn(-(x * y), y, 3.0);
 }
 
-// FP64 pair with reciprocal square roots (rsq_nr) in place of IEEE divide
-// and sqrt; the Coulomb point at zero z offset shares the LJ pair's rsq.
-// ORTHO (grid cells with orthogonal axes): dxy2 = dx^2 + dy^2 + R2SAFE is
-// the same for all z points of a lane and comes from the caller.
+// FP64 pair with reciprocal square roots in place of IEEE divide and sqrt;
+// the Coulomb point at zero z offset shares the LJ pair's rsq.  ORTHO (grid
+// cells with orthogonal axes): dxy2 = dx^2 + dy^2 + R2SAFE is the same for
+// all z points of a lane and comes from the caller.
+// The pair works with Y = 2/r (rsq2_nr), and the factors of 2 go into the
+// staged coefficients and the sums (exact powers of two, undone by the
+// caller): q arrives as q/2 and c12 as 12 c12 / 64, c6 as 6 c6; then
+//   LJ:      256 s = fma(12 c12 / 64, Y^6, -6 c6) Y^8   (s = (12 c12 r^-12 - 6 c6 r^-6) r^-2)
+//            768 E = Y^6 fma(12 c12 / 64, Y^6, -12 c6)    (E = c12 r^-12 - c6 r^-6)
+//   Coulomb: e = (q/2) sw Y = q sw / r,  4 e r^-2 = e Y^2
+// so aL[0..2] sum 256 F_LJ, aL[3] 768 E_LJ, aE[0..2] 4 F_C and aE[3] E_C.
 template <bool LJ, int NS, bool Z0, bool WE, bool QS = false, bool ORTHO = false>
 __device__ __forceinline__ void pair_fp64_fast(const FFArgs &A, double dx, double dy, double dz, double q, double c6,
                                                double c12, double aL[4], double aE[4], double dxy2 = 0.0)
 {
     const double r2 = ORTHO ? __fma_rn(dz, dz, dxy2) : __fma_rn(dx, dx, __fma_rn(dy, dy, __fma_rn(dz, dz, A.r2safe_d)));
-    double ir0 = 0, ir20 = 0;
+    double ir0 = 0, ir20 = 0;  // Y = 2/r and Y^2
     if (LJ || Z0) {
-        ir0 = rsq_nr(r2);
+        ir0 = rsq2_nr(r2);
         ir20 = ir0 * ir0;
     }
     if (LJ) {
-        // c6 and c12 arrive as 6 c6 and 12 c12 (scaled when staged):
-        //   s = (12 c12 r^-12 - 6 c6 r^-6) r^-2 = fma(12 c12, r^-6, -6 c6) r^-8,
-        //   12 E = r^-6 fma(12 c12, r^-6, -12 c6)  (E = c12 r^-12 - c6 r^-6; the
-        //   epilogue divides the sum by 12): 8 FP64 instructions for the force
-        //   instead of 11
         const double ir6 = ir20 * ir20 * ir20;
         const double s = __fma_rn(c12, ir6, -c6) * (ir6 * ir20);
         aL[0] = __fma_rn(dx, s, aL[0]); aL[1] = __fma_rn(dy, s, aL[1]); aL[2] = __fma_rn(dz, s, aL[2]);
@@ -243,7 +244,7 @@ __device__ __forceinline__ void pair_fp64_fast(const FFArgs &A, double dx, doubl
             ir = ir0; ir2 = ir20; dzs = dz;
         } else {
             dzs = dz + (double)A.sdz[s_];
-            ir = rsq_nr(ORTHO ? __fma_rn(dzs, dzs, dxy2) : __fma_rn(dx, dx, __fma_rn(dy, dy, __fma_rn(dzs, dzs, A.r2safe_d))));
+            ir = rsq2_nr(ORTHO ? __fma_rn(dzs, dzs, dxy2) : __fma_rn(dx, dx, __fma_rn(dy, dy, __fma_rn(dzs, dzs, A.r2safe_d))));
             ir2 = ir * ir;
         }
         const double e = ir * (q * (double)A.sw[s_]);
@@ -298,8 +299,8 @@ __device__ __forceinline__ void ff_brick_f64(const FFArgs &A, const AtomRec *__r
         __syncthreads();
         for (int j = t; j < nt; j += kThreads) {
             const AtomRec a = atoms[i0 + j];
-            lpos[j] = make_double4(a.x - ccx, a.y - ccy, a.z - ccz, a.q);
-            llj[j] = make_double2(6.0 * a.c6, 12.0 * a.c12);  // the scaling pair_fp64_fast expects
+            lpos[j] = make_double4(a.x - ccx, a.y - ccy, a.z - ccz, 0.5 * a.q);
+            llj[j] = make_double2(6.0 * a.c6, (12.0 * a.c12) * (1.0 / 64.0));  // the scaling pair_fp64_fast expects
         }
         __syncthreads();
         if constexpr (ORTHO) {
@@ -334,7 +335,13 @@ __device__ __forceinline__ void ff_brick_f64(const FFArgs &A, const AtomRec *__r
     for (int k = 0; k < kK; k++) {
         const int iz = z0 + zl + kZLanes * k;
         if (iz >= A.n[2]) break;
-        if (WE) aL[k][3] *= 1.0 / 12.0;  // the pairs summed 12 E
+        // the pairs summed 256 F_LJ, 768 E_LJ and 4 F_C (pair_fp64_fast)
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            aL[k][c] *= 1.0 / 256.0;
+            aE[k][c] *= 0.25;
+        }
+        if (WE) aL[k][3] *= 1.0 / 768.0;
         double gx, gy, gz, e;
         if (A.mode == 0) {
             clamp_vec(aL[k][0], aL[k][1], aL[k][2], A.fmax);
