@@ -247,7 +247,8 @@ __device__ __forceinline__ void pair_fp64_fast(const FFArgs &A, double dx, doubl
             ir = rsq2_nr(ORTHO ? __fma_rn(dzs, dzs, dxy2) : __fma_rn(dx, dx, __fma_rn(dy, dy, __fma_rn(dzs, dzs, A.r2safe_d))));
             ir2 = ir * ir;
         }
-        const double e = ir * (q * (double)A.sw[s_]);
+        // one Coulomb point (NS == 1): q arrives multiplied by sw[0] already
+        const double e = ir * (NS == 1 ? q : q * (double)A.sw[s_]);
         const double s = e * ir2;
         aE[0] = __fma_rn(dx, s, aE[0]); aE[1] = __fma_rn(dy, s, aE[1]); aE[2] = __fma_rn(dzs, s, aE[2]);
         if (WE) aE[3] += e;
@@ -299,22 +300,37 @@ __device__ __forceinline__ void ff_brick_f64(const FFArgs &A, const AtomRec *__r
         __syncthreads();
         for (int j = t; j < nt; j += kThreads) {
             const AtomRec a = atoms[i0 + j];
-            lpos[j] = make_double4(a.x - ccx, a.y - ccy, a.z - ccz, 0.5 * a.q);
+            lpos[j] = make_double4(a.x - ccx, a.y - ccy, a.z - ccz, 0.5 * (NS == 1 ? a.q * (double)A.sw[0] : a.q));
             llj[j] = make_double2(6.0 * a.c6, (12.0 * a.c12) * (1.0 / 64.0));  // the scaling pair_fp64_fast expects
         }
         __syncthreads();
         if constexpr (ORTHO) {
             // orthogonal cell axes: the lane's points share x and y, so one
-            // dx, dy and dx^2 + dy^2 per atom
-            for (int j = 0; j < nt; j++) {
-                const double4 a = lpos[j];
-                const double2 c = llj[j];
-                const double dx = px[0] - a.x, dy = py[0] - a.y;
-                const double dxy2 = __fma_rn(dx, dx, __fma_rn(dy, dy, A.r2safe_d));
+            // dx, dy and dx^2 + dy^2 per atom.  Bricks with all kK points in
+            // the grid (all but the top ones) take a loop without the
+            // per-point test, two atoms per pass
+            if (nk == kK) {
+#pragma unroll 2
+                for (int j = 0; j < nt; j++) {
+                    const double4 a = lpos[j];
+                    const double2 c = llj[j];
+                    const double dx = px[0] - a.x, dy = py[0] - a.y;
+                    const double dxy2 = __fma_rn(dx, dx, __fma_rn(dy, dy, A.r2safe_d));
 #pragma unroll
-                for (int k = 0; k < kK; k++) {
-                    if (k >= nk) break;
-                    pair_fp64_fast<LJ, NS, Z0, WE, QS, true>(A, dx, dy, pz[k] - a.z, a.w, c.x, c.y, aL[k], aE[k], dxy2);
+                    for (int k = 0; k < kK; k++)
+                        pair_fp64_fast<LJ, NS, Z0, WE, QS, true>(A, dx, dy, pz[k] - a.z, a.w, c.x, c.y, aL[k], aE[k], dxy2);
+                }
+            } else {
+                for (int j = 0; j < nt; j++) {
+                    const double4 a = lpos[j];
+                    const double2 c = llj[j];
+                    const double dx = px[0] - a.x, dy = py[0] - a.y;
+                    const double dxy2 = __fma_rn(dx, dx, __fma_rn(dy, dy, A.r2safe_d));
+#pragma unroll
+                    for (int k = 0; k < kK; k++) {
+                        if (k >= nk) break;
+                        pair_fp64_fast<LJ, NS, Z0, WE, QS, true>(A, dx, dy, pz[k] - a.z, a.w, c.x, c.y, aL[k], aE[k], dxy2);
+                    }
                 }
             }
         } else {
