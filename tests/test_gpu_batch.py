@@ -221,6 +221,41 @@ def test_batch_with_uncharged_atoms_equals_single_scans(f64):
     b.close()
 
 
+@pytest.mark.parametrize("mode,warm", [("plain", False), ("plain", True), ("tilted", True)])
+def test_partial_record_chunks(mode, warm):
+    """Warm-started batches stage a lane's output records in LDS and write
+    the records of 4 consecutive steps of a column together; a column whose
+    step count is not a multiple of 4 ends on a partial chunk.  A 7-step
+    scan reproduces the first 7 steps of the same 8-step scan bit for bit
+    (positions, forces, iterations, status)."""
+    print(f"case: 3 molecules, {mode} batch, warm {warm}, 7 against 8 steps", flush=True)
+    mols = [M.molecule(12 + 5 * k, 80 + k) for k in range(3)]
+    outs = []
+    for nz in (7, 8):
+        cfg, _ = small_config(n=(48, 48, 80), scan_n=(20, 13), nz=nz, natoms=20, seed=5)
+        cfg = dict(cfg, scan_x=(-1.0, 1.0), scan_y=(-0.9, 0.9))
+        lvec = M.config_lvec(cfg)
+        if mode == "plain":
+            b = B.BatchScan.plain([lvec] * 3, [cfg["n"]] * 3, [cfg] * 3, tipR0=M.TIP_R0, k_lat=M.K_LAT, k_rad=M.K_RAD,
+                                  iZPP=M.PROBE_Z, probe_q=M.PROBE_Q, amplitude=M.DF_AMPLITUDE, molecules=mols,
+                                  keep_iters=True)
+        else:
+            t = np.deg2rad(10.0)
+            rot = np.array([[np.cos(t), 0, np.sin(t)], [0, 1, 0], [-np.sin(t), 0, np.cos(t)]])
+            b = B.BatchScan.tilted([lvec] * 3, [cfg["n"]] * 3, rot=rot, pos0s=[(0.0, 0.0, 6.5)] * 3, start=(-1.0, -0.9),
+                                   end=(1.0, 0.9), scan_n=(20, 13), nz=nz, molecules=mols, keep_iters=True)
+        if warm:
+            b.set_warmstart(None)
+        b.run()
+        b.synchronize()
+        outs.append([(b.positions_raw(k), b.forces(k), b.iterations(k), b.status(k)) for k in range(3)])
+        b.close()
+    for k in range(3):
+        for a7, a8 in zip(outs[0][k], outs[1][k]):
+            assert a7.shape[2] == 7 and a8.shape[2] == 8
+            assert np.array_equal(a7, a8[:, :, :7]), k
+
+
 @pytest.mark.parametrize("mode", ["plain", "tilted"])
 def test_lane_refill_equals_tile_queue(mode):
     """Items of one geometry (here 24 molecules, a 66x45 scan that leaves
