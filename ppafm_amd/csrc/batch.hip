@@ -171,6 +171,9 @@ int enqueue_relax(ppafm_batch &b, const ppafm_batch_tuning &t, const BatchRelaxI
     RelaxBatchLaunch L;
     L.refill = t.refill == 1 && homogeneous(hitems);
     if (L.refill) L.item0 = hitems[0];
+    L.ortho = true;
+    for (const BatchRelaxItem &r : hitems)
+        for (int k : {1, 2, 3, 5, 6, 7}) L.ortho = L.ortho && r.dinv[k] == 0;
     L.rp = b.rp;
     L.tip = b.tip;
     L.items = items;

@@ -201,8 +201,9 @@ struct RelaxBatchLaunch {
     const float *pred = nullptr;
     int newton_max = 0, newton_max0 = 0, newton_retry = 0;
     double newton_tol = 0, newton_cap = 0;
-    double newton_extrap = 0;
-    double residual_max = 0;  // eV/A, 0 = off (step-0 estimate residual gate)  // second-order predictor for steps >= 2 up to this step (A), 0 = off
+    double newton_extrap = 0;  // second-order predictor for steps >= 2 up to this step (A), 0 = off
+    double residual_max = 0;   // eV/A, 0 = off (step-0 estimate residual gate)
+    bool ortho = false;        // every item's dinv is diagonal (orthogonal cell axes)
 };
 // Tiles of one scan for a given tile_y (also returns tiles along y).
 int relax_batch_tiles(int nx, int ny, int tile_y, int &tiles_y);

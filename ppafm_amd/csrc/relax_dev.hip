@@ -92,12 +92,12 @@ __device__ __forceinline__ void column_init(const DevArgs &A, Column<T4> &C, int
 // One force evaluation of a plain column: FIRE to convergence at each z
 // step, outputs in stroke order.  Counts force evaluations and finished
 // positions.
-template <typename T4, int MODE, int K>
+template <typename T4, int MODE, int K, bool ORTHO = false>
 __device__ __forceinline__ void column_step(const DevArgs &A, const T4 *__restrict__ grid, Column<T4> &C, Stage<K> &G,
                                             unsigned long long &nevals, unsigned long long &npos)
 {
     double fe[4], gx, gy, gz;
-    to_grid(A, C.S.rx, C.S.ry, C.S.rz, gx, gy, gz);
+    to_grid<ORTHO>(A, C.S.rx, C.S.ry, C.S.rz, gx, gy, gz);
     gather_cached(grid, A, gx, gy, gz, fe, MODE == 1, C.cache);
     nevals++;
     bool done = false;
@@ -158,13 +158,13 @@ __device__ __forceinline__ void column_step(const DevArgs &A, const T4 *__restri
 // the next step continues from FIRE's result.  Newton iterates and FIRE
 // moves share the cell cache of the corners (a load only when the cell
 // changes); Newton evaluations count in nevals.
-template <typename T4, int MODE, int K>
+template <typename T4, int MODE, int K, bool ORTHO = false>
 __device__ __forceinline__ void column_step_warm(const DevArgs &A, const T4 *__restrict__ grid, Column<T4> &C,
                                                  Stage<K> &G, unsigned long long &nevals, unsigned long long &npos,
                                                  unsigned long long &nfb)
 {
     double gx, gy, gz, ux, uy, uz;
-    to_grid(A, C.newton ? C.qx : C.S.rx, C.newton ? C.qy : C.S.ry, C.newton ? C.qz : C.S.rz, gx, gy, gz);
+    to_grid<ORTHO>(A, C.newton ? C.qx : C.S.rx, C.newton ? C.qy : C.S.ry, C.newton ? C.qz : C.S.rz, gx, gy, gz);
     // the cell cache: Newton iterates near convergence and FIRE's moves mostly stay in one cell
     corners_cached(grid, A, gx, gy, gz, C.cache, ux, uy, uz);
     const T4 *v = C.cache.v;
@@ -180,7 +180,8 @@ __device__ __forceinline__ void column_step_warm(const DevArgs &A, const T4 *__r
         bool stable;
         double st2, f2;
         const bool ok =
-            newton_from<T4, MODE>(A, v, ux, uy, uz, fe, C.tx, C.ty, C.tz, px, py, pz, A.newton_cap, stable, st2, f2) && stable;
+            newton_from<T4, MODE, ORTHO>(A, v, ux, uy, uz, fe, C.tx, C.ty, C.tz, px, py, pz, A.newton_cap, stable, st2, f2) &&
+            stable;
         if (retry) C.next_try = C.it + A.newton_retry;
         const bool gated = C.gate && !(f2 <= A.ws_res2);
         C.gate = false;
@@ -278,23 +279,23 @@ __device__ __forceinline__ void column_step_warm(const DevArgs &A, const T4 *__r
 }
 
 // One pixel column (i, j) from start to end.
-template <typename T4, int MODE, int K>
+template <typename T4, int MODE, int K, bool ORTHO = false>
 __device__ __forceinline__ void relax_column(const DevArgs &A, const T4 *__restrict__ grid, int i, int j, bool live,
                                              Stage<K> &G, unsigned long long &nevals, unsigned long long &npos)
 {
     Column<T4> C;
     column_init<T4, MODE, false>(A, C, i, j, live, 0, nullptr);
-    while (C.s < A.nz) column_step<T4, MODE, K>(A, grid, C, G, nevals, npos);
+    while (C.s < A.nz) column_step<T4, MODE, K, ORTHO>(A, grid, C, G, nevals, npos);
 }
 
-template <typename T4, int MODE, int K>
+template <typename T4, int MODE, int K, bool ORTHO = false>
 __device__ __forceinline__ void relax_column_warm(const DevArgs &A, const T4 *__restrict__ grid, int i, int j,
                                                   bool live, Stage<K> &G, unsigned long long &nevals,
                                                   unsigned long long &npos, unsigned long long &nfb)
 {
     Column<T4> C;
     column_init<T4, MODE, true>(A, C, i, j, live, 0, A.pred ? A.pred + 3 * ((long)i * A.ny + j) : nullptr);
-    while (C.s < A.nz) column_step_warm<T4, MODE, K>(A, grid, C, G, nevals, npos, nfb);
+    while (C.s < A.nz) column_step_warm<T4, MODE, K, ORTHO>(A, grid, C, G, nevals, npos, nfb);
 }
 
 // Wave sum of the per-lane counters, one atomic per wave.  `passes` (the
@@ -330,7 +331,7 @@ constexpr int stage_batch()
     return WARM ? 4 : 1;
 }
 
-template <typename T4, int MODE>
+template <typename T4, int MODE, bool ORTHO>
 __global__ __launch_bounds__(64) void relax_dev_kernel(DevArgs A, const T4 *__restrict__ grid)
 {
     __shared__ __align__(16) unsigned char lds[kStageSingle * 64 * kStageRecordBytes];
@@ -343,7 +344,7 @@ __global__ __launch_bounds__(64) void relax_dev_kernel(DevArgs A, const T4 *__re
     const int lane = threadIdx.x;
     const int i = ti * 8 + lane / 8, j = tj * 8 + lane % 8;
     unsigned long long nevals = 0, npos = 0;
-    relax_column<T4, MODE, kStageSingle>(A, grid, i, j, i < A.nx && j < A.ny, G, nevals, npos);
+    relax_column<T4, MODE, kStageSingle, ORTHO>(A, grid, i, j, i < A.nx && j < A.ny, G, nevals, npos);
     flush_counters(A.counters, nevals, npos, lane);
 }
 
@@ -377,7 +378,7 @@ __device__ __forceinline__ int find_batch_item(const long long *__restrict__ pre
     return lo;
 }
 
-template <typename T4, int MODE, int TY, bool WARM>
+template <typename T4, int MODE, int TY, bool WARM, bool ORTHO>
 __global__ __launch_bounds__(256) void relax_batch_kernel(DevArgs A0, BatchArgs B, const BatchRelaxItem *__restrict__ items,
                                                           const long long *__restrict__ prefix,
                                                           const T4 *__restrict__ grids)
@@ -422,8 +423,9 @@ __global__ __launch_bounds__(256) void relax_batch_kernel(DevArgs A0, BatchArgs 
             const int i = ti * TX + lane / TY, j = tj * TY + lane % TY;
             const unsigned long long before = nevals;
             if (WARM)
-                relax_column_warm<T4, MODE, KS>(A, grids + it.grid_off, i, j, i < A.nx && j < A.ny, G, nevals, npos, nfb);
-            else relax_column<T4, MODE, KS>(A, grids + it.grid_off, i, j, i < A.nx && j < A.ny, G, nevals, npos);
+                relax_column_warm<T4, MODE, KS, ORTHO>(A, grids + it.grid_off, i, j, i < A.nx && j < A.ny, G, nevals, npos,
+                                                       nfb);
+            else relax_column<T4, MODE, KS, ORTHO>(A, grids + it.grid_off, i, j, i < A.nx && j < A.ny, G, nevals, npos);
             // the tile's loop passes: every pass evaluates once in each lane still running
             unsigned long long d = nevals - before;
             for (int off = 32; off > 0; off >>= 1) d = max(d, (unsigned long long)__shfl_xor(d, off, 64));
@@ -446,7 +448,7 @@ __global__ __launch_bounds__(256) void relax_batch_kernel(DevArgs A0, BatchArgs 
 // work take consecutive columns with one atomic per wave, and the wave moves
 // on to the next part when its part is empty.  Every lane leaves the loop
 // once all parts are empty and its column is done, so the grid drains.
-template <typename T4, int MODE, int TY, bool WARM>
+template <typename T4, int MODE, int TY, bool WARM, bool ORTHO>
 __global__ __launch_bounds__(256) void relax_refill_kernel(DevArgs A, BatchArgs B, const BatchRelaxItem *__restrict__ items,
                                                            const T4 *__restrict__ grids, long long cols_item)
 {
@@ -501,8 +503,8 @@ __global__ __launch_bounds__(256) void relax_refill_kernel(DevArgs A, BatchArgs 
             continue;
         }
         if (C.s < A.nz) {
-            if (WARM) column_step_warm<T4, MODE, KS>(A, grid, C, G, nevals, npos, nfb);
-            else column_step<T4, MODE, KS>(A, grid, C, G, nevals, npos);
+            if (WARM) column_step_warm<T4, MODE, KS, ORTHO>(A, grid, C, G, nevals, npos, nfb);
+            else column_step<T4, MODE, KS, ORTHO>(A, grid, C, G, nevals, npos);
         }
         passes++;
     }
@@ -717,12 +719,25 @@ int launch_relax_fast(const RelaxLaunch &L, const void *grid, bool f64, hipStrea
         }
         return 0;
     }
+    // orthogonal cell axes (every configuration here): the diagonal of dinv alone
+    const bool ortho = A.dinv[1] == 0 && A.dinv[2] == 0 && A.dinv[3] == 0 && A.dinv[5] == 0 && A.dinv[6] == 0 &&
+                       A.dinv[7] == 0;
     if (f64) {
-        if (L.rp.mode == 0) hipLaunchKernelGGL((relax_dev_kernel<double4, 0>), grd, blk, 0, st, A, (const double4 *)grid);
-        else hipLaunchKernelGGL((relax_dev_kernel<double4, 1>), grd, blk, 0, st, A, (const double4 *)grid);
+        if (L.rp.mode == 0) {
+            if (ortho) hipLaunchKernelGGL((relax_dev_kernel<double4, 0, true>), grd, blk, 0, st, A, (const double4 *)grid);
+            else hipLaunchKernelGGL((relax_dev_kernel<double4, 0, false>), grd, blk, 0, st, A, (const double4 *)grid);
+        } else {
+            if (ortho) hipLaunchKernelGGL((relax_dev_kernel<double4, 1, true>), grd, blk, 0, st, A, (const double4 *)grid);
+            else hipLaunchKernelGGL((relax_dev_kernel<double4, 1, false>), grd, blk, 0, st, A, (const double4 *)grid);
+        }
     } else {
-        if (L.rp.mode == 0) hipLaunchKernelGGL((relax_dev_kernel<float4, 0>), grd, blk, 0, st, A, (const float4 *)grid);
-        else hipLaunchKernelGGL((relax_dev_kernel<float4, 1>), grd, blk, 0, st, A, (const float4 *)grid);
+        if (L.rp.mode == 0) {
+            if (ortho) hipLaunchKernelGGL((relax_dev_kernel<float4, 0, true>), grd, blk, 0, st, A, (const float4 *)grid);
+            else hipLaunchKernelGGL((relax_dev_kernel<float4, 0, false>), grd, blk, 0, st, A, (const float4 *)grid);
+        } else {
+            if (ortho) hipLaunchKernelGGL((relax_dev_kernel<float4, 1, true>), grd, blk, 0, st, A, (const float4 *)grid);
+            else hipLaunchKernelGGL((relax_dev_kernel<float4, 1, false>), grd, blk, 0, st, A, (const float4 *)grid);
+        }
     }
     return 0;
 }
@@ -739,35 +754,35 @@ static void fill_shared_args(DevArgs &A, const ppafm_relax_params &rp, const ppa
     A.krad = tip.krad; A.lrad = tip.lrad; A.surf_ff = tip.surf_ff;
 }
 
-template <typename T4, int MODE, bool WARM>
+template <typename T4, int MODE, bool WARM, bool ORTHO>
 static int launch_refill_ty(int ty, dim3 grd, dim3 blk, hipStream_t st, const DevArgs &A, const BatchArgs &B,
                             const BatchRelaxItem *items, const void *grids, long long cols_item)
 {
     const T4 *g = (const T4 *)grids;
     const size_t lds = stage_batch<WARM>() > 1 ? (size_t)blk.x * stage_batch<WARM>() * kStageRecordBytes : 0;
     switch (ty) {
-    case 1: hipLaunchKernelGGL((relax_refill_kernel<T4, MODE, 1, WARM>), grd, blk, lds, st, A, B, items, g, cols_item); break;
-    case 2: hipLaunchKernelGGL((relax_refill_kernel<T4, MODE, 2, WARM>), grd, blk, lds, st, A, B, items, g, cols_item); break;
-    case 4: hipLaunchKernelGGL((relax_refill_kernel<T4, MODE, 4, WARM>), grd, blk, lds, st, A, B, items, g, cols_item); break;
-    case 8: hipLaunchKernelGGL((relax_refill_kernel<T4, MODE, 8, WARM>), grd, blk, lds, st, A, B, items, g, cols_item); break;
-    case 16: hipLaunchKernelGGL((relax_refill_kernel<T4, MODE, 16, WARM>), grd, blk, lds, st, A, B, items, g, cols_item); break;
+    case 1: hipLaunchKernelGGL((relax_refill_kernel<T4, MODE, 1, WARM, ORTHO>), grd, blk, lds, st, A, B, items, g, cols_item); break;
+    case 2: hipLaunchKernelGGL((relax_refill_kernel<T4, MODE, 2, WARM, ORTHO>), grd, blk, lds, st, A, B, items, g, cols_item); break;
+    case 4: hipLaunchKernelGGL((relax_refill_kernel<T4, MODE, 4, WARM, ORTHO>), grd, blk, lds, st, A, B, items, g, cols_item); break;
+    case 8: hipLaunchKernelGGL((relax_refill_kernel<T4, MODE, 8, WARM, ORTHO>), grd, blk, lds, st, A, B, items, g, cols_item); break;
+    case 16: hipLaunchKernelGGL((relax_refill_kernel<T4, MODE, 16, WARM, ORTHO>), grd, blk, lds, st, A, B, items, g, cols_item); break;
     default: set_error("tile_y must be 1, 2, 4, 8 or 16"); return -2;
     }
     return 0;
 }
 
-template <typename T4, int MODE, bool WARM>
+template <typename T4, int MODE, bool WARM, bool ORTHO>
 static int launch_batch_ty(int ty, dim3 grd, dim3 blk, hipStream_t st, const DevArgs &A, const BatchArgs &B,
                            const BatchRelaxItem *items, const long long *prefix, const void *grids)
 {
     const T4 *g = (const T4 *)grids;
     const size_t lds = stage_batch<WARM>() > 1 ? (size_t)blk.x * stage_batch<WARM>() * kStageRecordBytes : 0;
     switch (ty) {
-    case 1: hipLaunchKernelGGL((relax_batch_kernel<T4, MODE, 1, WARM>), grd, blk, lds, st, A, B, items, prefix, g); break;
-    case 2: hipLaunchKernelGGL((relax_batch_kernel<T4, MODE, 2, WARM>), grd, blk, lds, st, A, B, items, prefix, g); break;
-    case 4: hipLaunchKernelGGL((relax_batch_kernel<T4, MODE, 4, WARM>), grd, blk, lds, st, A, B, items, prefix, g); break;
-    case 8: hipLaunchKernelGGL((relax_batch_kernel<T4, MODE, 8, WARM>), grd, blk, lds, st, A, B, items, prefix, g); break;
-    case 16: hipLaunchKernelGGL((relax_batch_kernel<T4, MODE, 16, WARM>), grd, blk, lds, st, A, B, items, prefix, g); break;
+    case 1: hipLaunchKernelGGL((relax_batch_kernel<T4, MODE, 1, WARM, ORTHO>), grd, blk, lds, st, A, B, items, prefix, g); break;
+    case 2: hipLaunchKernelGGL((relax_batch_kernel<T4, MODE, 2, WARM, ORTHO>), grd, blk, lds, st, A, B, items, prefix, g); break;
+    case 4: hipLaunchKernelGGL((relax_batch_kernel<T4, MODE, 4, WARM, ORTHO>), grd, blk, lds, st, A, B, items, prefix, g); break;
+    case 8: hipLaunchKernelGGL((relax_batch_kernel<T4, MODE, 8, WARM, ORTHO>), grd, blk, lds, st, A, B, items, prefix, g); break;
+    case 16: hipLaunchKernelGGL((relax_batch_kernel<T4, MODE, 16, WARM, ORTHO>), grd, blk, lds, st, A, B, items, prefix, g); break;
     default: set_error("tile_y must be 1, 2, 4, 8 or 16"); return -2;
     }
     return 0;
@@ -797,11 +812,20 @@ static int launch_batch_w(const RelaxBatchLaunch &L, dim3 grd, dim3 blk, hipStre
         G.status = B.status;
         G.pred = nullptr;
         PP_REQUIRE((long long)L.nitems * cols_item < (1LL << 35), "batch too large for the column queues");
-        if (L.warm) return launch_refill_ty<T4, MODE, true>(L.tile_y, grd, blk, st, G, B, L.items, L.grids, cols_item);
-        return launch_refill_ty<T4, MODE, false>(L.tile_y, grd, blk, st, G, B, L.items, L.grids, cols_item);
+        if (L.ortho) {
+            if (L.warm) return launch_refill_ty<T4, MODE, true, true>(L.tile_y, grd, blk, st, G, B, L.items, L.grids, cols_item);
+            return launch_refill_ty<T4, MODE, false, true>(L.tile_y, grd, blk, st, G, B, L.items, L.grids, cols_item);
+        }
+        if (L.warm) return launch_refill_ty<T4, MODE, true, false>(L.tile_y, grd, blk, st, G, B, L.items, L.grids, cols_item);
+        return launch_refill_ty<T4, MODE, false, false>(L.tile_y, grd, blk, st, G, B, L.items, L.grids, cols_item);
     }
-    if (L.warm) return launch_batch_ty<T4, MODE, true>(L.tile_y, grd, blk, st, A, B, L.items, L.prefix, L.grids);
-    return launch_batch_ty<T4, MODE, false>(L.tile_y, grd, blk, st, A, B, L.items, L.prefix, L.grids);
+    // orthogonal cell axes in every item (every configuration here): the diagonal of dinv alone
+    if (L.ortho) {
+        if (L.warm) return launch_batch_ty<T4, MODE, true, true>(L.tile_y, grd, blk, st, A, B, L.items, L.prefix, L.grids);
+        return launch_batch_ty<T4, MODE, false, true>(L.tile_y, grd, blk, st, A, B, L.items, L.prefix, L.grids);
+    }
+    if (L.warm) return launch_batch_ty<T4, MODE, true, false>(L.tile_y, grd, blk, st, A, B, L.items, L.prefix, L.grids);
+    return launch_batch_ty<T4, MODE, false, false>(L.tile_y, grd, blk, st, A, B, L.items, L.prefix, L.grids);
 }
 
 // ---------------------------------------------------------------------------

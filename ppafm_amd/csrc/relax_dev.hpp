@@ -51,10 +51,20 @@ __device__ __forceinline__ double rsq_nr(double x)
     return fma(y, fma(-h, y, 0.5), y);
 }
 
-// grid index coordinates of a position (rows of dinv)
+// grid index coordinates of a position (rows of dinv).  ORTHO: dinv is
+// diagonal, and the diagonal products alone are the same numbers for finite
+// positions (the zero terms add a signed zero, which no later operation
+// tells apart)
+template <bool ORTHO = false>
 __device__ __forceinline__ void to_grid(const DevArgs &A, double rx, double ry, double rz, double &gx, double &gy,
                                         double &gz)
 {
+    if (ORTHO) {
+        gx = rx * A.dinv[0];
+        gy = ry * A.dinv[4];
+        gz = rz * A.dinv[8];
+        return;
+    }
     gx = fma(rz, A.dinv[2], fma(ry, A.dinv[1], rx * A.dinv[0]));
     gy = fma(rz, A.dinv[5], fma(ry, A.dinv[4], rx * A.dinv[3]));
     gz = fma(rz, A.dinv[8], fma(ry, A.dinv[7], rx * A.dinv[6]));
@@ -248,7 +258,7 @@ __device__ __forceinline__ double fire_move(const DevArgs &A, const double fe[4]
 // characteristic polynomial l^3 + a2 l^2 + a1 l + a0): only then is a zero of
 // the force there a minimum that damped dynamics settle in, not a saddle or a
 // maximum such as the point straight above a repulsive atom.
-template <typename T4, int MODE>
+template <typename T4, int MODE, bool ORTHO = false>
 __device__ __forceinline__ bool newton_from(const DevArgs &A, const T4 v[8], double ux, double uy, double uz,
                                             const double fe[4], double tx, double ty, double tz, double &rx,
                                             double &ry, double &rz, double max_step, bool &stable, double &st2,
@@ -272,7 +282,9 @@ __device__ __forceinline__ bool newton_from(const DevArgs &A, const T4 v[8], dou
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
-        for (int j = 0; j < 3; j++) J[i][j] = fma(dT[i][2], A.dinv[6 + j], fma(dT[i][1], A.dinv[3 + j], dT[i][0] * A.dinv[j]));
+        for (int j = 0; j < 3; j++)  // ORTHO: the diagonal alone, the same numbers (as to_grid)
+            J[i][j] = ORTHO ? dT[i][j] * A.dinv[4 * j]
+                            : fma(dT[i][2], A.dinv[6 + j], fma(dT[i][1], A.dinv[3 + j], dT[i][0] * A.dinv[j]));
     double fx, fy, fz;
     total_force<MODE>(A, fe, tx, ty, tz, rx, ry, rz, fx, fy, fz);
     f2 = fma(fx, fx, fma(fy, fy, fz * fz));
