@@ -171,9 +171,13 @@ int enqueue_relax(ppafm_batch &b, const ppafm_batch_tuning &t, const BatchRelaxI
     RelaxBatchLaunch L;
     L.refill = t.refill == 1 && homogeneous(hitems);
     if (L.refill) L.item0 = hitems[0];
+    // the orthogonal kernels: every item's dinv diagonal and its grid under
+    // 4 GiB (they read grids through buffer resources with 32-bit offsets)
     L.ortho = true;
-    for (const BatchRelaxItem &r : hitems)
+    for (const BatchRelaxItem &r : hitems) {
         for (int k : {1, 2, 3, 5, 6, 7}) L.ortho = L.ortho && r.dinv[k] == 0;
+        L.ortho = L.ortho && (unsigned long long)r.n[0] * r.n[1] * r.n[2] * (b.f64 ? 32 : 16) <= 0xFFFFFFFFull;
+    }
     L.rp = b.rp;
     L.tip = b.tip;
     L.items = items;

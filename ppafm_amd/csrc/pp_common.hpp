@@ -203,7 +203,7 @@ struct RelaxBatchLaunch {
     double newton_tol = 0, newton_cap = 0;
     double newton_extrap = 0;  // second-order predictor for steps >= 2 up to this step (A), 0 = off
     double residual_max = 0;   // eV/A, 0 = off (step-0 estimate residual gate)
-    bool ortho = false;        // every item's dinv is diagonal (orthogonal cell axes)
+    bool ortho = false;        // every item's dinv is diagonal (orthogonal cell axes) and its grid under 4 GiB
 };
 // Tiles of one scan for a given tile_y (also returns tiles along y).
 int relax_batch_tiles(int nx, int ny, int tile_y, int &tiles_y);

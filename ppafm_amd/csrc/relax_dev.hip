@@ -98,7 +98,9 @@ __device__ __forceinline__ void column_step(const DevArgs &A, const T4 *__restri
 {
     double fe[4], gx, gy, gz;
     to_grid<ORTHO>(A, C.S.rx, C.S.ry, C.S.rz, gx, gy, gz);
-    gather_cached(grid, A, gx, gy, gz, fe, MODE == 1, C.cache);
+    // the orthogonal kernels also read the grid through a buffer resource
+    // (the host picks them for grids under 4 GiB only)
+    gather_cached<T4, ORTHO, MODE == 1>(grid, A, gx, gy, gz, fe, C.cache);
     nevals++;
     bool done = false;
     const long o = C.obase + C.c.col0 + C.s;
@@ -166,7 +168,7 @@ __device__ __forceinline__ void column_step_warm(const DevArgs &A, const T4 *__r
     double gx, gy, gz, ux, uy, uz;
     to_grid<ORTHO>(A, C.newton ? C.qx : C.S.rx, C.newton ? C.qy : C.S.ry, C.newton ? C.qz : C.S.rz, gx, gy, gz);
     // the cell cache: Newton iterates near convergence and FIRE's moves mostly stay in one cell
-    corners_cached(grid, A, gx, gy, gz, C.cache, ux, uy, uz);
+    corners_cached<T4, ORTHO, MODE == 1>(grid, A, gx, gy, gz, C.cache, ux, uy, uz);
     const T4 *v = C.cache.v;
     nevals++;
     bool done = false;
@@ -719,9 +721,12 @@ int launch_relax_fast(const RelaxLaunch &L, const void *grid, bool f64, hipStrea
         }
         return 0;
     }
-    // orthogonal cell axes (every configuration here): the diagonal of dinv alone
+    // orthogonal cell axes (every configuration here): the diagonal of dinv
+    // alone, and grid reads through a buffer resource (32-bit offsets: the
+    // grid under 4 GiB)
+    const unsigned long long gbytes = (unsigned long long)A.n0 * A.n1 * A.n2 * (f64 ? 32 : 16);
     const bool ortho = A.dinv[1] == 0 && A.dinv[2] == 0 && A.dinv[3] == 0 && A.dinv[5] == 0 && A.dinv[6] == 0 &&
-                       A.dinv[7] == 0;
+                       A.dinv[7] == 0 && gbytes <= 0xFFFFFFFFull;
     if (f64) {
         if (L.rp.mode == 0) {
             if (ortho) hipLaunchKernelGGL((relax_dev_kernel<double4, 0, true>), grd, blk, 0, st, A, (const double4 *)grid);

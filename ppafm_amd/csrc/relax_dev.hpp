@@ -130,9 +130,38 @@ struct CellCache {
     T4 v[8];
 };
 
+typedef unsigned pp_u4 __attribute__((ext_vector_type(4)));
+typedef unsigned pp_u2 __attribute__((ext_vector_type(2)));
+typedef double pp_d2 __attribute__((ext_vector_type(2)));
+typedef float pp_f4 __attribute__((ext_vector_type(4)));
+
+// One grid corner through a buffer resource r (the grid's base in scalar
+// registers, a 32-bit byte offset per load: no 64-bit address arithmetic
+// per corner; the grid must be under 4 GiB).  WANT_W: the energy component
+// too (mode 1); mode 0 reads x, y, z.
+template <typename T4, bool WANT_W>
+__device__ __forceinline__ T4 buf_corner(__amdgpu_buffer_rsrc_t r, unsigned idx)
+{
+    if constexpr (sizeof(T4) == 32) {
+        const pp_d2 xy = __builtin_bit_cast(pp_d2, __builtin_amdgcn_raw_buffer_load_b128(r, idx * 32u, 0, 0));
+        if constexpr (WANT_W) {
+            const pp_d2 zw = __builtin_bit_cast(pp_d2, __builtin_amdgcn_raw_buffer_load_b128(r, idx * 32u + 16u, 0, 0));
+            return T4{xy.x, xy.y, zw.x, zw.y};
+        } else {
+            const double z = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, idx * 32u + 16u, 0, 0));
+            return T4{xy.x, xy.y, z, 0.0};
+        }
+    } else {
+        const pp_f4 v = __builtin_bit_cast(pp_f4, __builtin_amdgcn_raw_buffer_load_b128(r, idx * 16u, 0, 0));
+        return T4{v.x, v.y, v.z, WANT_W ? v.w : 0.f};
+    }
+}
+
 // corners() through the cache: C.v holds the corners of the cell of
-// (gx, gy, gz) afterwards; (ux, uy, uz) is the position in it.
-template <typename T4>
+// (gx, gy, gz) afterwards; (ux, uy, uz) is the position in it.  BUF: the
+// loads go through a buffer resource on g (buf_corner; the caller makes
+// sure the grid is under 4 GiB).
+template <typename T4, bool BUF = false, bool WANT_W = true>
 __device__ __forceinline__ void corners_cached(const T4 *__restrict__ g, const DevArgs &A, double gx, double gy,
                                                double gz, CellCache<T4> &C, double &ux, double &uy, double &uz)
 {
@@ -143,21 +172,30 @@ __device__ __forceinline__ void corners_cached(const T4 *__restrict__ g, const D
     if (ix != C.ix || iy != C.iy || iz != C.iz) {
         const unsigned c00 = ((unsigned)ix * A.n1 + iy) * A.n2, c01 = ((unsigned)ix * A.n1 + jy) * A.n2;
         const unsigned c10 = ((unsigned)jx * A.n1 + iy) * A.n2, c11 = ((unsigned)jx * A.n1 + jy) * A.n2;
-        C.v[0] = g[c00 + iz]; C.v[1] = g[c00 + jz]; C.v[2] = g[c01 + iz]; C.v[3] = g[c01 + jz];
-        C.v[4] = g[c10 + iz]; C.v[5] = g[c10 + jz]; C.v[6] = g[c11 + iz]; C.v[7] = g[c11 + jz];
+        if constexpr (BUF) {
+            const __amdgpu_buffer_rsrc_t r =
+                __builtin_amdgcn_make_buffer_rsrc(const_cast<T4 *>(g), (short)0, (int)0xFFFFFFFF, 0x00020000);
+            C.v[0] = buf_corner<T4, WANT_W>(r, c00 + iz); C.v[1] = buf_corner<T4, WANT_W>(r, c00 + jz);
+            C.v[2] = buf_corner<T4, WANT_W>(r, c01 + iz); C.v[3] = buf_corner<T4, WANT_W>(r, c01 + jz);
+            C.v[4] = buf_corner<T4, WANT_W>(r, c10 + iz); C.v[5] = buf_corner<T4, WANT_W>(r, c10 + jz);
+            C.v[6] = buf_corner<T4, WANT_W>(r, c11 + iz); C.v[7] = buf_corner<T4, WANT_W>(r, c11 + jz);
+        } else {
+            C.v[0] = g[c00 + iz]; C.v[1] = g[c00 + jz]; C.v[2] = g[c01 + iz]; C.v[3] = g[c01 + jz];
+            C.v[4] = g[c10 + iz]; C.v[5] = g[c10 + jz]; C.v[6] = g[c11 + iz]; C.v[7] = g[c11 + jz];
+        }
         C.ix = ix;
         C.iy = iy;
         C.iz = iz;
     }
 }
 
-template <typename T4>
+template <typename T4, bool BUF = false, bool WANT_E = true>
 __device__ __forceinline__ void gather_cached(const T4 *__restrict__ g, const DevArgs &A, double gx, double gy,
-                                              double gz, double out[4], bool want_e, CellCache<T4> &C)
+                                              double gz, double out[4], CellCache<T4> &C)
 {
     double ux, uy, uz;
-    corners_cached(g, A, gx, gy, gz, C, ux, uy, uz);
-    interp8(C.v, ux, uy, uz, out, want_e);
+    corners_cached<T4, BUF, WANT_E>(g, A, gx, gy, gz, C, ux, uy, uz);
+    interp8(C.v, ux, uy, uz, out, WANT_E);
 }
 
 template <typename T4>
