@@ -130,8 +130,6 @@ struct CellCache {
     T4 v[8];
 };
 
-typedef unsigned pp_u4 __attribute__((ext_vector_type(4)));
-typedef unsigned pp_u2 __attribute__((ext_vector_type(2)));
 typedef double pp_d2 __attribute__((ext_vector_type(2)));
 typedef float pp_f4 __attribute__((ext_vector_type(4)));
 
