@@ -192,20 +192,20 @@ __global__ __launch_bounds__(64) void ws_fwd2_kernel(int S, const unsigned short
     }
 }
 
-__device__ __forceinline__ float wave_sum(float v)
-{
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
 // Layer 3, loss and the head of the backward pass; one thread per sample.
-// grad / loss are accumulated with one atomic per wave per value.
-// Samples s >= nvalid are padding: they produce outputs but no gradient.
+// With grad, each block of 256 samples also leaves the sums of its samples'
+// head gradient terms (dW3, db3, db2 and the loss, kHeadVals values) in
+// part[block]; ws_head_sum_kernel adds the blocks up in order (no atomics:
+// the same sums every run).  Samples s >= nvalid are padding: they produce
+// outputs but no gradient.
+constexpr int kHeadVals = kN3 * kN2 + kN3 + kN2 + 1;
+constexpr int kHeadRow = kN3 + kN2 + kN2 + 2;  // dy, bf16(h), dz, loss term, pad: an odd stride of floats
+
 __global__ __launch_bounds__(256) void ws_head_kernel(int S, int nvalid, float inv_n, const float *__restrict__ Z2,
                                                       const float *__restrict__ H2, Net net,
                                                       const float *__restrict__ T, float *__restrict__ out,
-                                                      float *__restrict__ grad, float *__restrict__ loss,
-                                                      unsigned short *__restrict__ dZ2, unsigned short *__restrict__ dZ2T)
+                                                      float *__restrict__ part, unsigned short *__restrict__ dZ2,
+                                                      unsigned short *__restrict__ dZ2T)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = s < S;
@@ -222,43 +222,63 @@ __global__ __launch_bounds__(256) void ws_head_kernel(int S, int nvalid, float i
         o[m] = a;
         if (out && live) out[(long)s * kN3 + m] = a;
     }
-    if (!grad) return;
+    if (!part) return;  // uniform: forward only
+    __shared__ float sh[256][kHeadRow];
+    const int t = threadIdx.x;
     float dy[kN3], lsum = 0.f;
 #pragma unroll
     for (int m = 0; m < kN3; m++) {
         const float d = valid ? o[m] - T[(long)s * kN3 + m] : 0.f;
         lsum += d * d;
         dy[m] = 2.f * d * inv_n;
+        sh[t][m] = dy[m];
     }
-    const int lane = threadIdx.x & 63;
-    // dW3, db3
 #pragma unroll
-    for (int m = 0; m < kN3; m++) {
-#pragma unroll
-        for (int j = 0; j < kN2; j++) {
-            const float g = wave_sum(dy[m] * h[j]);
-            if (lane == 0) atomicAdd(&grad[kOffW3 + m * kN2 + j], g);
-        }
-        const float g = wave_sum(dy[m]);
-        if (lane == 0) atomicAdd(&grad[kOffB3 + m], g);
-    }
-    const float ls = wave_sum(lsum * inv_n);
-    if (lane == 0) atomicAdd(loss, ls);
-    // dZ2 = (dY W3) * silu'(Z2), db2
+    for (int j = 0; j < kN2; j++) sh[t][kN3 + j] = h[j];
+    sh[t][kN3 + 2 * kN2] = lsum * inv_n;
+    // dZ2 = (dY W3) * silu'(Z2)
 #pragma unroll
     for (int j = 0; j < kN2; j++) {
         float dh = 0.f;
 #pragma unroll
         for (int m = 0; m < kN3; m++) dh = fmaf(dy[m], bfr(W3[m * kN2 + j]), dh);
         const float dz = live ? dh * dsilu(Z2[(long)s * kN2 + j]) : 0.f;
-        const float g = wave_sum(dz);
-        if (lane == 0) atomicAdd(&grad[kOffB2 + j], g);
+        sh[t][kN3 + kN2 + j] = dz;
         if (live) {
             const unsigned short b = f2bf(dz);
             dZ2[(long)s * kN2 + j] = b;
             dZ2T[(long)j * S + s] = b;
         }
     }
+    __syncthreads();
+    // the block's sums: thread v < kHeadVals takes value v over the 256 samples
+    if (t < kHeadVals) {
+        float acc = 0.f;
+        if (t < kN3 * kN2) {  // dW3[m][j] = sum dy[m] bf16(h[j])
+            const int m = t / kN2, j = t % kN2;
+            for (int q = 0; q < 256; q++) acc = fmaf(sh[q][m], sh[q][kN3 + j], acc);
+        } else {
+            // db3 (dy), db2 (dz), the loss term
+            const int c = t < kN3 * kN2 + kN3 ? t - kN3 * kN2 : t < kN3 * kN2 + kN3 + kN2 ? kN3 + kN2 + (t - kN3 * kN2 - kN3)
+                                                                                            : kN3 + 2 * kN2;
+            for (int q = 0; q < 256; q++) acc += sh[q][c];
+        }
+        part[(long)blockIdx.x * kHeadVals + t] = acc;
+    }
+}
+
+// The head's gradient and loss from the blocks' sums, added in block order.
+__global__ __launch_bounds__(256) void ws_head_sum_kernel(int nblocks, const float *__restrict__ part,
+                                                          float *__restrict__ grad, float *__restrict__ loss)
+{
+    const int v = threadIdx.x;
+    if (v >= kHeadVals) return;
+    float acc = 0.f;
+    for (int b = 0; b < nblocks; b++) acc += part[(long)b * kHeadVals + v];
+    if (v < kN3 * kN2) grad[kOffW3 + v] += acc;
+    else if (v < kN3 * kN2 + kN3) grad[kOffB3 + v - kN3 * kN2] += acc;
+    else if (v < kHeadVals - 1) grad[kOffB2 + v - kN3 * kN2 - kN3] += acc;
+    else loss[0] += acc;
 }
 
 // dW2[j][k2] += sum_s dZ2[s][j] H1[s][k2]: M = 32, N = 96 per block column,
@@ -922,6 +942,7 @@ struct WsModel {
     unsigned short *H1 = nullptr, *H1T = nullptr, *A1T = nullptr, *dZ2 = nullptr, *dZ2T = nullptr, *dZ1T = nullptr;
     unsigned short *Ppad = nullptr;  // padded copy of a partial last minibatch
     float *Tpad = nullptr;
+    float *part = nullptr;  // per-block sums of the head's gradient terms
     long step = 0;
     Net net() const { return Net{theta, W1b, W2b, W2bT}; }
 };
@@ -954,13 +975,14 @@ int ws_alloc(WsModel &m, int cap)
     PP_TRY_HIP(hipMalloc((void **)&m.dZ1T, c * 9 * kN1 * 2));
     PP_TRY_HIP(hipMalloc((void **)&m.Ppad, c * kPatchStride * 2));
     PP_TRY_HIP(hipMalloc((void **)&m.Tpad, c * kN3 * 4));
+    PP_TRY_HIP(hipMalloc((void **)&m.part, ((c + 255) / 256) * kHeadVals * 4));
     return 0;
 }
 
 void ws_free(WsModel &m)
 {
     void *p[] = {m.theta, m.grad, m.m1, m.m2, m.loss, m.W1b, m.W2b, m.W2bT, m.Z1, m.Z2, m.H2,
-                 m.H1, m.H1T, m.A1T, m.dZ2, m.dZ2T, m.dZ1T, m.Ppad, m.Tpad};
+                 m.H1, m.H1T, m.A1T, m.dZ2, m.dZ2T, m.dZ1T, m.Ppad, m.Tpad, m.part};
     for (void *q : p)
         if (q) (void)hipFree(q);
 }
@@ -984,9 +1006,11 @@ int ws_pass(WsModel &m, int S, int nvalid, const unsigned short *P, const float 
                        train ? m.H1T : nullptr, train ? m.A1T : nullptr);
     hipLaunchKernelGGL(ws_fwd2_kernel, dim3(S / 64), dim3(64), 0, st, S, m.H1, net, m.Z2, m.H2);
     const float inv_n = nvalid > 0 ? 1.f / (3.f * (float)nvalid) : 0.f;
-    hipLaunchKernelGGL(ws_head_kernel, dim3((S + 255) / 256), dim3(256), 0, st, S, nvalid, inv_n, m.Z2, m.H2, net, T,
-                       out, train ? m.grad : nullptr, m.loss, m.dZ2, m.dZ2T);
+    const int nhb = (S + 255) / 256;
+    hipLaunchKernelGGL(ws_head_kernel, dim3(nhb), dim3(256), 0, st, S, nvalid, inv_n, m.Z2, m.H2, net, T, out,
+                       train ? m.part : nullptr, m.dZ2, m.dZ2T);
     if (!train) return 0;
+    hipLaunchKernelGGL(ws_head_sum_kernel, dim3(1), dim3(256), 0, st, nhb, m.part, m.grad, m.loss);
     // dW2: K = S split into chunks of up to 512 samples
     int ks = 512;
     while (S % ks) ks >>= 1;
