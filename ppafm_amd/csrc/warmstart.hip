@@ -356,10 +356,11 @@ __global__ __launch_bounds__(64) void ws_bwd1a_kernel(int S, const unsigned shor
     }
 }
 
-// dW1[j][k] += sum_{(s,p)} dZ1[(s,p)][j] A1[(s,p)][k]: M = 32, N = 96, K = kr
-// rows of the (s,p) range per block.
+// dW1[j][k] = sum_{(s,p)} dZ1[(s,p)][j] A1[(s,p)][k]: M = 32, N = 96, K = kr
+// rows of the (s,p) range per block, whose sums go to part[block][j][k]
+// (ws_w1_sum_kernel adds the blocks up in order).
 __global__ __launch_bounds__(64) void ws_bwd1b_kernel(long K, int kr, const unsigned short *__restrict__ dZ1T,
-                                                       const unsigned short *__restrict__ A1T, float *__restrict__ grad)
+                                                       const unsigned short *__restrict__ A1T, float *__restrict__ part)
 {
     const int l = threadIdx.x;
     const long r0 = (long)blockIdx.x * kr;
@@ -387,8 +388,19 @@ __global__ __launch_bounds__(64) void ws_bwd1b_kernel(long K, int kr, const unsi
             for (int i = 0; i < 4; i++) {
                 const int j = 16 * m + 4 * (l >> 4) + i;
                 const int k = 16 * t + (l & 15);
-                if (k < kK1) atomicAdd(&grad[kOffW1 + j * kK1 + k], acc[m][t][i]);
+                if (k < kK1) part[(long)blockIdx.x * (kN1 * kK1) + j * kK1 + k] = acc[m][t][i];
             }
+}
+
+// dW1 from ws_bwd1b_kernel's block sums, added in block order.
+__global__ __launch_bounds__(256) void ws_w1_sum_kernel(int nblocks, const float *__restrict__ part,
+                                                        float *__restrict__ grad)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= kN1 * kK1) return;
+    float acc = 0.f;
+    for (int b = 0; b < nblocks; b++) acc += part[(long)b * (kN1 * kK1) + v];
+    grad[kOffW1 + v] += acc;
 }
 
 // Adam on the FP32 master parameters; refreshes the bf16 copies and clears
@@ -934,6 +946,8 @@ namespace {
 
 // --------------------------------------------------------------- model object
 
+constexpr int kW1Rows = 576;  // (s, p) rows per ws_bwd1b_kernel block: 128 blocks for 8192 samples
+
 struct WsModel {
     float *theta = nullptr, *grad = nullptr, *m1 = nullptr, *m2 = nullptr, *loss = nullptr;
     unsigned short *W1b = nullptr, *W2b = nullptr, *W2bT = nullptr;
@@ -942,7 +956,8 @@ struct WsModel {
     unsigned short *H1 = nullptr, *H1T = nullptr, *A1T = nullptr, *dZ2 = nullptr, *dZ2T = nullptr, *dZ1T = nullptr;
     unsigned short *Ppad = nullptr;  // padded copy of a partial last minibatch
     float *Tpad = nullptr;
-    float *part = nullptr;  // per-block sums of the head's gradient terms
+    float *part = nullptr;   // per-block sums of the head's gradient terms
+    float *part1 = nullptr;  // per-block sums of dW1 (ws_bwd1b_kernel)
     long step = 0;
     Net net() const { return Net{theta, W1b, W2b, W2bT}; }
 };
@@ -976,13 +991,14 @@ int ws_alloc(WsModel &m, int cap)
     PP_TRY_HIP(hipMalloc((void **)&m.Ppad, c * kPatchStride * 2));
     PP_TRY_HIP(hipMalloc((void **)&m.Tpad, c * kN3 * 4));
     PP_TRY_HIP(hipMalloc((void **)&m.part, ((c + 255) / 256) * kHeadVals * 4));
+    PP_TRY_HIP(hipMalloc((void **)&m.part1, (9 * c / kW1Rows) * (kN1 * kK1) * 4));
     return 0;
 }
 
 void ws_free(WsModel &m)
 {
     void *p[] = {m.theta, m.grad, m.m1, m.m2, m.loss, m.W1b, m.W2b, m.W2bT, m.Z1, m.Z2, m.H2,
-                 m.H1, m.H1T, m.A1T, m.dZ2, m.dZ2T, m.dZ1T, m.Ppad, m.Tpad, m.part};
+                 m.H1, m.H1T, m.A1T, m.dZ2, m.dZ2T, m.dZ1T, m.Ppad, m.Tpad, m.part, m.part1};
     for (void *q : p)
         if (q) (void)hipFree(q);
 }
@@ -1016,10 +1032,11 @@ int ws_pass(WsModel &m, int S, int nvalid, const unsigned short *P, const float 
     while (S % ks) ks >>= 1;
     hipLaunchKernelGGL(ws_bwd2_kernel, dim3(3, S / ks), dim3(64), 0, st, S, ks, m.dZ2T, m.H1T, m.grad);
     hipLaunchKernelGGL(ws_bwd1a_kernel, dim3(S / 64), dim3(64), 0, st, S, m.dZ2, net, m.Z1, m.dZ1T, m.grad);
+    // dW1: K = 9 S rows in blocks of kW1Rows (a divisor of 9 S as S is a multiple of 64)
     const long K = 9L * S;
-    int kr = 4608;
-    while (K % kr) kr >>= 1;
-    hipLaunchKernelGGL(ws_bwd1b_kernel, dim3((unsigned)(K / kr)), dim3(64), 0, st, K, kr, m.dZ1T, m.A1T, m.grad);
+    const int nb1 = (int)(K / kW1Rows);
+    hipLaunchKernelGGL(ws_bwd1b_kernel, dim3((unsigned)nb1), dim3(64), 0, st, K, kW1Rows, m.dZ1T, m.A1T, m.part1);
+    hipLaunchKernelGGL(ws_w1_sum_kernel, dim3((kN1 * kK1 + 255) / 256), dim3(256), 0, st, nb1, m.part1, m.grad);
     return 0;
 }
 
