@@ -319,7 +319,8 @@ __global__ __launch_bounds__(64) void ws_bwd2_kernel(int S, int ks, const unsign
 }
 
 // dA2 = dZ2 W2 (M = 64 samples, N = 288, K = 32), dZ1 = dA2 * silu'(Z1),
-// written transposed as dZ1T[c][s*9 + p]; db1.
+// written transposed as dZ1T[c][s*9 + p]; db1.  Block (tile, y) takes the
+// 16-column slices 6y .. 6y+5 of the 18.
 __global__ __launch_bounds__(64) void ws_bwd1a_kernel(int S, const unsigned short *__restrict__ dZ2, Net net,
                                                        const float *__restrict__ Z1, unsigned short *__restrict__ dZ1T,
                                                        float *__restrict__ grad)
@@ -330,7 +331,7 @@ __global__ __launch_bounds__(64) void ws_bwd1a_kernel(int S, const unsigned shor
 #pragma unroll
     for (int r = 0; r < 4; r++) a[r] = ld8(dZ2 + (long)(64 * tile + 16 * r + (l & 15)) * kN2 + 8 * (l >> 4));
     float db[2] = {0.f, 0.f};
-    for (int t = 0; t < kK2 / 16; t++) {
+    for (int t = 6 * blockIdx.y; t < 6 * blockIdx.y + 6; t++) {
         const int k2 = 16 * t + (l & 15);
         const bf16x8 b = ld8(net.W2bT + (long)k2 * kN2 + 8 * (l >> 4));
         const int p = k2 >> 5, c = k2 & 31;
@@ -1031,7 +1032,8 @@ int ws_pass(WsModel &m, int S, int nvalid, const unsigned short *P, const float 
     int ks = 512;
     while (S % ks) ks >>= 1;
     hipLaunchKernelGGL(ws_bwd2_kernel, dim3(3, S / ks), dim3(64), 0, st, S, ks, m.dZ2T, m.H1T, m.grad);
-    hipLaunchKernelGGL(ws_bwd1a_kernel, dim3(S / 64), dim3(64), 0, st, S, m.dZ2, net, m.Z1, m.dZ1T, m.grad);
+    static_assert(kK2 / 16 == 18, "ws_bwd1a_kernel splits 18 column slices over 3 blocks");
+    hipLaunchKernelGGL(ws_bwd1a_kernel, dim3(S / 64, 3), dim3(64), 0, st, S, m.dZ2, net, m.Z1, m.dZ1T, m.grad);
     // dW1: K = 9 S rows in blocks of kW1Rows (a divisor of 9 S as S is a multiple of 64)
     const long K = 9L * S;
     const int nb1 = (int)(K / kW1Rows);
