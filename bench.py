@@ -466,7 +466,9 @@ def run_learned(args, b):
     flop_col = 2 * (9 * 81 * 32 + 288 * 32 + 32 * 3)  # one network pass per column
     out = {"training": {"pairs": int(P.shape[0]), "held_out_molecules": len(held), "data_s": t_data, "train_s": t_train,
                         "steps": nsteps, "batch": 8192, "epochs": args.learned_epochs,
-                        "samples_per_s": nsteps * 8192 / t_train, "loss_first": float(losses[:5].mean()),
+                        "samples_per_s": nsteps * 8192 / t_train,
+                        # forward + backward ~ 3 forward passes of flop_col per sample (bf16 MFMA layers 1-2)
+                        "tflops": 3 * flop_col * nsteps * 8192 / t_train / 1e12, "loss_first": float(losses[:5].mean()),
                         "loss_last": float(losses[-5:].mean()), "parameters": NPARAM}}
     for k in ("plain", "newton", "learned"):
         q = r[k]
