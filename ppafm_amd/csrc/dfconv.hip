@@ -20,6 +20,10 @@ namespace pp {
 namespace {
 
 constexpr int kMaxW = 64;
+// LDS staging of the df convolution: about this many inputs per block, and
+// columns longer than the byte budget use the unstaged kernel.
+constexpr int kConvLdsInputs = 2048;
+constexpr size_t kConvLdsBytes = 32768;
 
 struct ConvArgs {
     long ncol;
@@ -41,6 +45,37 @@ __global__ __launch_bounds__(256) void convolve_z_kernel(ConvArgs A, const T *__
     double acc = 0;
     for (int k = 0; k < A.nw; k++) acc += (double)p[(long)k * A.in_step] * A.w[k];
     out[gid] = (T)(acc * A.scale);
+}
+
+// The same convolution with a block's columns staged in LDS: the block reads
+// its columns' inputs once, with neighbouring lanes on neighbouring inputs,
+// and every output then sums its nw terms from LDS in the order above, so
+// the results are the same. The kernel above reads each input nw times
+// through the vector cache.
+template <typename T>
+__global__ __launch_bounds__(256) void convolve_z_lds_kernel(ConvArgs A, int cols_per_block, const T *__restrict__ in,
+                                                             T *__restrict__ out)
+{
+    extern __shared__ unsigned char conv_lds[];
+    T *s = (T *)conv_lds;
+    const long c0 = (long)blockIdx.x * cols_per_block;
+    const int nc = (int)min((long)cols_per_block, A.ncol - c0);
+    const int nin = nc * A.nz;
+    const T *src = in + c0 * A.in_stride + A.in_offset;
+    for (int i = threadIdx.x; i < nin; i += blockDim.x) {
+        const int c = i / A.nz, z = i - c * A.nz;
+        s[i] = src[(long)c * A.in_stride + (long)z * A.in_step];
+    }
+    __syncthreads();
+    const int nout = nc * A.nzout;
+    T *o = out + c0 * A.nzout;
+    for (int i = threadIdx.x; i < nout; i += blockDim.x) {
+        const int c = i / A.nzout, j = i - c * A.nzout;
+        const T *p = s + c * A.nz + j;
+        double acc = 0;
+        for (int k = 0; k < A.nw; k++) acc += (double)p[k] * A.w[k];
+        o[i] = (T)(acc * A.scale);
+    }
 }
 
 // io.limit_vec_field (io.py:766-778) on a float64 (npts,3) device array.
@@ -114,6 +149,21 @@ int launch_convolve_z(long ncol, int nz, int in_stride, int in_step, int in_offs
     for (int k = 0; k < kMaxW; k++) A.w[k] = k < nw ? w[k] : 0.0;
     const long total = ncol * A.nzout;
     if (total == 0) return 0;
+    const size_t esz = f64 ? sizeof(double) : sizeof(float);
+    if ((size_t)nz * esz <= kConvLdsBytes) {
+        // Columns per block: about 2048 inputs, within the LDS budget.
+        const int cpb = (int)std::max<long>(1, std::min<long>(ncol, kConvLdsInputs / nz));
+        const long nb = (ncol + cpb - 1) / cpb;
+        PP_REQUIRE(nb < (1L << 31), "too many columns");
+        const size_t lds = (size_t)cpb * nz * esz;
+        if (f64)
+            hipLaunchKernelGGL(convolve_z_lds_kernel<double>, dim3((unsigned)nb), dim3(256), lds, st, A, cpb,
+                               (const double *)in, (double *)out);
+        else
+            hipLaunchKernelGGL(convolve_z_lds_kernel<float>, dim3((unsigned)nb), dim3(256), lds, st, A, cpb,
+                               (const float *)in, (float *)out);
+        return 0;
+    }
     const long nblk = (total + 255) / 256;
     PP_REQUIRE(nblk < (1L << 31), "too many columns");
     if (f64)
