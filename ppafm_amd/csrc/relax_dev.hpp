@@ -458,6 +458,24 @@ __device__ __forceinline__ Stage<K> make_stage(unsigned char *lds, int T, int t)
 
 // The record of step s of a column (output offset o) into the lane's
 // slot s % K; the slots go out when the chunk is full or s is the last step.
+// Copy a lane's K staged values (slots J..K-1, T lanes apart in LDS) to
+// dst[J..K-1]: every LDS read is issued before the first store, and the
+// values live in registers (a local array here was kept in scratch memory,
+// stored and read back at every flush).
+template <typename V, int J, int K>
+struct FlushChunk {
+    static __device__ __forceinline__ void run(V *dst, const V *src, int T, int t)
+    {
+        const V v = src[J * T + t];
+        FlushChunk<V, J + 1, K>::run(dst, src, T, t);
+        dst[J] = v;
+    }
+};
+template <typename V, int K>
+struct FlushChunk<V, K, K> {
+    static __device__ __forceinline__ void run(V *, const V *, int, int) {}
+};
+
 template <int K>
 __device__ __forceinline__ void put_record(const DevArgs &A, Stage<K> &G, long o, int s, double rx, double ry, double rz,
                                            float4 f, int iters, unsigned char status)
@@ -485,27 +503,9 @@ __device__ __forceinline__ void put_record(const DevArgs &A, Stage<K> &G, long o
             const int q = j * G.T + G.t;
             store_pos(A, o0 + j, G.p[q], G.p[KT + q], G.p[2 * KT + q]);
         }
-        if (A.force) {
-            float4 v[K];
-#pragma unroll
-            for (int j = 0; j < K; j++) v[j] = G.f[j * G.T + G.t];
-#pragma unroll
-            for (int j = 0; j < K; j++) A.force[o0 + j] = v[j];
-        }
-        if (A.iters) {
-            int v[K];
-#pragma unroll
-            for (int j = 0; j < K; j++) v[j] = G.it[j * G.T + G.t];
-#pragma unroll
-            for (int j = 0; j < K; j++) A.iters[o0 + j] = v[j];
-        }
-        if (A.status) {
-            unsigned char v[K];
-#pragma unroll
-            for (int j = 0; j < K; j++) v[j] = G.st[j * G.T + G.t];
-#pragma unroll
-            for (int j = 0; j < K; j++) A.status[o0 + j] = v[j];
-        }
+        if (A.force) FlushChunk<float4, 0, K>::run(A.force + o0, G.f, G.T, G.t);
+        if (A.iters) FlushChunk<int, 0, K>::run(A.iters + o0, G.it, G.T, G.t);
+        if (A.status) FlushChunk<unsigned char, 0, K>::run(A.status + o0, G.st, G.T, G.t);
     } else if (k == K - 1 || s == A.nz - 1) {  // a chunk of the batch kernels, or a column's last, partial one
         const long o0 = o - k;
         for (int j = 0; j <= k; j++) {
