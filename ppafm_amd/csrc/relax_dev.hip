@@ -322,7 +322,7 @@ __device__ __forceinline__ void flush_counters(unsigned long long *counters, uns
     }
 }
 
-// records of 8 steps per lane: 23 KB of LDS per 64-lane block
+// records of 8 steps per lane: 23 KB of LDS per wave
 constexpr int kStageSingle = 8;
 // warm-started batches: 4 steps per lane, 11.5 KB per wave, so LDS does not
 // limit their occupancy; plain-FIRE batches store directly (staged, they ran
@@ -333,17 +333,19 @@ constexpr int stage_batch()
     return WARM ? 4 : 1;
 }
 
-template <typename T4, int MODE, bool ORTHO>
-__global__ __launch_bounds__(64) void relax_dev_kernel(DevArgs A, const T4 *__restrict__ grid)
+// WPB waves (tiles) per workgroup.  Block b runs WPB consecutive tiles of
+// part b % 8 (one contiguous part of the tiles per XCD).
+template <typename T4, int MODE, bool ORTHO, int WPB>
+__global__ __launch_bounds__(64 * WPB) void relax_dev_kernel(DevArgs A, const T4 *__restrict__ grid)
 {
-    __shared__ __align__(16) unsigned char lds[kStageSingle * 64 * kStageRecordBytes];
-    Stage<kStageSingle> G = make_stage<kStageSingle>(lds, 64, threadIdx.x);
-    const int per_xcd = (A.ntiles + 7) / 8;
+    __shared__ __align__(16) unsigned char lds[kStageSingle * 64 * WPB * kStageRecordBytes];
+    Stage<kStageSingle> G = make_stage<kStageSingle>(lds, 64 * WPB, threadIdx.x);
+    const int per_xcd = ((A.ntiles + WPB - 1) / WPB + 7) / 8;
     const int b = blockIdx.x;
-    const int tile = (b % 8) * per_xcd + b / 8;
+    const int tile = ((b % 8) * per_xcd + b / 8) * WPB + (int)(threadIdx.x / 64);
     if (tile >= A.ntiles) return;
     const int ti = tile / A.tiles_y, tj = tile % A.tiles_y;
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x % 64;
     const int i = ti * 8 + lane / 8, j = tj * 8 + lane % 8;
     unsigned long long nevals = 0, npos = 0;
     relax_column<T4, MODE, kStageSingle, ORTHO>(A, grid, i, j, i < A.nx && j < A.ny, G, nevals, npos);
@@ -669,6 +671,38 @@ __global__ __launch_bounds__(64) void relax_lds_kernel(DevArgs A, const T4 *__re
 
 // 0 on launch, 1 when the parameters need the generic kernel (leap-frog,
 // max_iters < 1), negative on error.
+// SIMDs of the current device (4 per CU), 0 if unknown
+static int simd_count()
+{
+    static int cache[64];
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0) return 0;
+    if (dev < 64 && cache[dev] > 0) return cache[dev];
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    if (dev < 64) cache[dev] = 4 * cus;
+    return 4 * cus;
+}
+
+// Single-scan launch.  With one-wave workgroups the dispatcher has been
+// seen to place two waves of this launch on one SIMD while another SIMD of
+// the CU idles, depending on what ran before it (c1 right after its grid
+// build: 8.9-9.3 ms, against 7.1-7.4 ms after another FIRE launch).  A scan
+// with at most one tile per SIMD (c1: 1024 tiles) therefore uses four-wave
+// workgroups, whose waves go to the CU's four SIMDs; larger scans keep
+// one-wave workgroups, which LDS lets 6 to a CU.
+template <typename T4, int MODE, bool ORTHO>
+static void launch_single(const DevArgs &A, const void *grid, hipStream_t st)
+{
+    const int simds = simd_count();
+    if (simds > 0 && A.ntiles <= simds) {
+        const long nb = (((A.ntiles + 3L) / 4 + 7) / 8) * 8;
+        hipLaunchKernelGGL((relax_dev_kernel<T4, MODE, ORTHO, 4>), dim3((unsigned)nb), dim3(256), 0, st, A, (const T4 *)grid);
+    } else {
+        const long nb = ((A.ntiles + 7L) / 8) * 8;
+        hipLaunchKernelGGL((relax_dev_kernel<T4, MODE, ORTHO, 1>), dim3((unsigned)nb), dim3(64), 0, st, A, (const T4 *)grid);
+    }
+}
+
 int launch_relax_fast(const RelaxLaunch &L, const void *grid, bool f64, hipStream_t st)
 {
     if (L.rp.alg != 1 || L.rp.max_iters < 1) return 1;
@@ -729,19 +763,19 @@ int launch_relax_fast(const RelaxLaunch &L, const void *grid, bool f64, hipStrea
                        A.dinv[7] == 0 && gbytes <= 0xFFFFFFFFull;
     if (f64) {
         if (L.rp.mode == 0) {
-            if (ortho) hipLaunchKernelGGL((relax_dev_kernel<double4, 0, true>), grd, blk, 0, st, A, (const double4 *)grid);
-            else hipLaunchKernelGGL((relax_dev_kernel<double4, 0, false>), grd, blk, 0, st, A, (const double4 *)grid);
+            if (ortho) launch_single<double4, 0, true>(A, grid, st);
+            else launch_single<double4, 0, false>(A, grid, st);
         } else {
-            if (ortho) hipLaunchKernelGGL((relax_dev_kernel<double4, 1, true>), grd, blk, 0, st, A, (const double4 *)grid);
-            else hipLaunchKernelGGL((relax_dev_kernel<double4, 1, false>), grd, blk, 0, st, A, (const double4 *)grid);
+            if (ortho) launch_single<double4, 1, true>(A, grid, st);
+            else launch_single<double4, 1, false>(A, grid, st);
         }
     } else {
         if (L.rp.mode == 0) {
-            if (ortho) hipLaunchKernelGGL((relax_dev_kernel<float4, 0, true>), grd, blk, 0, st, A, (const float4 *)grid);
-            else hipLaunchKernelGGL((relax_dev_kernel<float4, 0, false>), grd, blk, 0, st, A, (const float4 *)grid);
+            if (ortho) launch_single<float4, 0, true>(A, grid, st);
+            else launch_single<float4, 0, false>(A, grid, st);
         } else {
-            if (ortho) hipLaunchKernelGGL((relax_dev_kernel<float4, 1, true>), grd, blk, 0, st, A, (const float4 *)grid);
-            else hipLaunchKernelGGL((relax_dev_kernel<float4, 1, false>), grd, blk, 0, st, A, (const float4 *)grid);
+            if (ortho) launch_single<float4, 1, true>(A, grid, st);
+            else launch_single<float4, 1, false>(A, grid, st);
         }
     }
     return 0;

@@ -110,6 +110,38 @@ def test_force_and_df_c0(c0):
     assert ddf.max() < DF_TOL
 
 
+def test_relax_more_tiles_than_simds():
+    """A 264 x 264 scan (1,089 tiles of 8 x 8 columns, more than the GPU's
+    1,024 SIMDs) runs one-wave workgroups; a scan with at most one tile per
+    SIMD runs four-wave workgroups.  The large scan is checked against the
+    oracle, and its first 128 x 128 columns, scanned again on their own
+    (same descriptor, four-wave workgroups), must give the same bits."""
+    from ppafm_amd import device as D
+    k = case(0)
+    cfg = dict(k["cfg"])
+    cfg["scan_n"] = (264, 264)
+    cfg["nz"] = 3
+    print("case: config 0 molecule, 160x160x200 grid (double4), 264x264x3 scan", flush=True)
+    tips = M.scan_tips(cfg) - k["lvec"][0]
+    rs, fs, it, tot = O.relax_strokes(cfg["n"], k["lvec"][1:], k["FF"], tip_params(), tips)
+    s = PlainScan(k["lvec"], cfg["n"], cfg, tipR0=M.TIP_R0, k_lat=M.K_LAT, k_rad=M.K_RAD, iZPP=M.PROBE_Z, probe_q=M.PROBE_Q,
+                  amplitude=M.DF_AMPLITUDE)
+    s.set_molecule(k["xyz"], k["Zs"], k["qs"])
+    s.build_grid()
+    s.relax()
+    pos = s.positions() - k["lvec"][0]
+    d = np.linalg.norm(pos - rs, axis=-1)
+    git = s.iterations()
+    print(f"positions |dr| [A]: max {d.max():.3g} p99 {np.percentile(d, 99):.3g}; iterations differing {(git != it).sum()} of {it.size}")
+    assert d.max() < POS_TOL
+    sub = D.scan_desc(128, 128, cfg["nz"], s.scan_d.p0, s.scan_d.da, s.scan_d.db, s.scan_d.dstep)
+    p2, f2, i2 = D.relax(s.n, s.lvec[1:], s.grid, s.rp, s.tip, sub, pos=True, force=True, iters=True)
+    D.synchronize()
+    assert np.array_equal(p2.download(), s.pos.download()[:128, :128])
+    assert np.array_equal(f2.download(), s.force.download()[:128, :128])
+    assert np.array_equal(i2.download(), s.iters.download()[:128, :128])
+
+
 def test_repeat_is_deterministic(c0):
     k, s, _ = c0
     p0 = s.positions().copy()
