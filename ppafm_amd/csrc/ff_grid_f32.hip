@@ -32,6 +32,8 @@
 //     is FP32.
 // The epilogue clamps and combines as the FP64 kernel does.
 
+#include <type_traits>
+
 #include "ff_grid.hpp"
 
 #pragma clang fp contract(off)
@@ -100,7 +102,10 @@ __device__ __forceinline__ void brick_f32(const FFArgs &A, const AtomRec *__rest
             for (int k = 0; k < kK; k++)
 #pragma unroll
                 for (int c = 0; c < 4; c++) { sL[k][c] = 0.f; sE[k][c] = 0.f; }
-            for (int j = j0; j < j1; j++) {
+            // one atom against the lane's points; all: every point of the
+            // lane is inside the grid (all bricks but the top ones), so
+            // no per-point test
+            auto atom = [&](int j, auto all) {
                 const float4 hi = lds[j].hi;
                 const float4 lo = lds[j].lo;
                 const float4 lj = lds[j].lj;
@@ -114,7 +119,7 @@ __device__ __forceinline__ void brick_f32(const FFArgs &A, const AtomRec *__rest
                 }
 #pragma unroll
                 for (int k = 0; k < kK; k++) {
-                    if (k >= nk) break;
+                    if (!decltype(all)::value && k >= nk) break;
                     const float dx = ORTHO ? dx0 : (px[k] - hi.x) - lo.x;
                     const float dy = ORTHO ? dy0 : (py[k] - hi.y) - lo.y;
                     const float dz = (pz[k] - hi.z) - lo.z;
@@ -127,12 +132,14 @@ __device__ __forceinline__ void brick_f32(const FFArgs &A, const AtomRec *__rest
                     // s: force over distance at zero z offset; e: energy;
                     // sc, ec, fzc: the Coulomb stencil's force scale, energy
                     // and extra z force
-                    float s = 0.f, e = 0.f, sc = 0.f, ec = 0.f, fzc = 0.f;
+                    // s = s0 * ir8 and e = e0 * ir6; the combined sum folds
+                    // these products into the additions of the Coulomb terms
+                    float s0 = 0.f, e0 = 0.f, ir6 = 0.f, ir8 = 0.f, sc = 0.f, ec = 0.f, fzc = 0.f;
                     if (LJ) {
-                        const float ir6 = ir20 * ir20 * ir20;
-                        const float ir8 = ir6 * ir20;
-                        s = __fmaf_rn(lj.y, ir6, -lj.x) * ir8;
-                        if (WE) e = __fmaf_rn(lj.w, ir6, -lj.z) * ir6;
+                        ir6 = ir20 * ir20 * ir20;
+                        ir8 = ir6 * ir20;
+                        s0 = __fmaf_rn(lj.y, ir6, -lj.x);
+                        if (WE) e0 = __fmaf_rn(lj.w, ir6, -lj.z);
                     }
                     if (charged && NS == 1 && Z0) {  // one point at zero offset
                         ec = (hi.w * A.sw[0]) * ir0;
@@ -160,10 +167,11 @@ __device__ __forceinline__ void brick_f32(const FFArgs &A, const AtomRec *__rest
                     }
                     if (SEP) {
                         if (LJ) {
+                            const float s = s0 * ir8;
                             sL[k][0] = __fmaf_rn(dx, s, sL[k][0]);
                             sL[k][1] = __fmaf_rn(dy, s, sL[k][1]);
                             sL[k][2] = __fmaf_rn(dz, s, sL[k][2]);
-                            if (WE) sL[k][3] += e;
+                            if (WE) sL[k][3] = __fmaf_rn(e0, ir6, sL[k][3]);
                         }
                         if (charged) {
                             sE[k][0] = __fmaf_rn(dx, sc, sE[k][0]);
@@ -172,13 +180,18 @@ __device__ __forceinline__ void brick_f32(const FFArgs &A, const AtomRec *__rest
                             if (WE) sE[k][3] += ec;
                         }
                     } else if (LJ || charged) {
-                        const float st = s + sc;
+                        const float st = LJ ? __fmaf_rn(s0, ir8, sc) : sc;
                         sL[k][0] = __fmaf_rn(dx, st, sL[k][0]);
                         sL[k][1] = __fmaf_rn(dy, st, sL[k][1]);
                         sL[k][2] = __fmaf_rn(dz, st, NS == 1 && Z0 ? sL[k][2] : sL[k][2] + fzc);
-                        if (WE) sL[k][3] += e + ec;
+                        if (WE) sL[k][3] += LJ ? __fmaf_rn(e0, ir6, ec) : ec;
                     }
                 }
+                        };
+            if (nk == kK) {
+                for (int j = j0; j < j1; j++) atom(j, std::true_type{});
+            } else {
+                for (int j = j0; j < j1; j++) atom(j, std::false_type{});
             }
 #pragma unroll
             for (int k = 0; k < kK; k++)
