@@ -122,6 +122,8 @@ __device__ __forceinline__ void brick_f32(const FFArgs &A, const AtomRec *__rest
                     if (!decltype(all)::value && k >= nk) break;
                     const float dx = ORTHO ? dx0 : (px[k] - hi.x) - lo.x;
                     const float dy = ORTHO ? dy0 : (py[k] - hi.y) - lo.y;
+                    // (pz - hi) is exact for a close atom (the operands are
+                    // within a factor of 2), which the near pairs' r^-13 needs
                     const float dz = (pz[k] - hi.z) - lo.z;
                     const float r2 = ORTHO ? __fmaf_rn(dz, dz, dxy2) : r2_f32(dx, dy, dz, A.r2safe);
                     float ir0 = 0.f, ir20 = 0.f;
@@ -180,7 +182,8 @@ __device__ __forceinline__ void brick_f32(const FFArgs &A, const AtomRec *__rest
                             if (WE) sE[k][3] += ec;
                         }
                     } else if (LJ || charged) {
-                        const float st = LJ ? __fmaf_rn(s0, ir8, sc) : sc;
+                        // s0 ir8 + ec ir20 = (s0 ir6 + ec) ir20 (one Coulomb point)
+                        const float st = LJ ? (NS == 1 && Z0 ? __fmaf_rn(s0, ir6, ec) * ir20 : __fmaf_rn(s0, ir8, sc)) : sc;
                         sL[k][0] = __fmaf_rn(dx, st, sL[k][0]);
                         sL[k][1] = __fmaf_rn(dy, st, sL[k][1]);
                         sL[k][2] = __fmaf_rn(dz, st, NS == 1 && Z0 ? sL[k][2] : sL[k][2] + fzc);
