@@ -322,8 +322,12 @@ typedef struct {
  * waves_per_block in 1..4, blocks_per_cu persistent workgroups per CU.
  * refill 1: each lane takes the next pixel column (in tile order) as soon as
  * its own is done, instead of the wave taking the next tile when all 64 are;
- * used when every item shares one grid shape, cell and scan (otherwise the
- * tile queue runs).  Results do not depend on the launch shape. */
+ * refill 2 and 3: the wave takes 64 (2: one tile) or 128 (3) consecutive
+ * columns at a time into a pool of its own, and a lane whose column is done
+ * takes the pool's next one, so a wave's lanes stay on neighbouring columns.
+ * Refill is used when every item shares one grid shape, cell and scan
+ * (otherwise the tile queue runs).  Results do not depend on the launch
+ * shape. */
 typedef struct {
     int tile_y;
     int waves_per_block;

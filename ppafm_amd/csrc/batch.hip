@@ -169,7 +169,8 @@ int enqueue_relax(ppafm_batch &b, const ppafm_batch_tuning &t, const BatchRelaxI
                   int nitems, long long ntiles, const std::vector<BatchRelaxItem> &hitems)
 {
     RelaxBatchLaunch L;
-    L.refill = t.refill == 1 && homogeneous(hitems);
+    L.refill = t.refill >= 1 && homogeneous(hitems);
+    L.refill_chunk = L.refill && t.refill >= 2 ? 64 * (t.refill - 1) : 0;
     if (L.refill) L.item0 = hitems[0];
     // the orthogonal kernels: every item's dinv diagonal and its grid under
     // 4 GiB (they read grids through buffer resources with 32-bit offsets)
@@ -335,7 +336,7 @@ int check_tuning(const ppafm_batch_tuning &t)
                "tile_y must be 1, 2, 4, 8 or 16");
     PP_REQUIRE(t.waves_per_block >= 1 && t.waves_per_block <= 4, "waves_per_block must be 1..4");
     PP_REQUIRE(t.blocks_per_cu >= 1 && t.blocks_per_cu <= 64, "blocks_per_cu must be 1..64");
-    PP_REQUIRE(t.refill == 0 || t.refill == 1, "refill must be 0 or 1");
+    PP_REQUIRE(t.refill >= 0 && t.refill <= 3, "refill must be 0, 1, 2 or 3");
     return 0;
 }
 
@@ -592,7 +593,10 @@ int ppafm_batch_tune(ppafm_batch *b, int nsample, int reps, ppafm_batch_tuning *
         if (b->d_sitems) { (void)hipFree(b->d_sitems); b->d_sitems = nullptr; }
         if (b->d_sprefix) { (void)hipFree(b->d_sprefix); b->d_sprefix = nullptr; }
         if ((rc = upload(b->d_sitems, it, b->st)) || (rc = upload(b->d_sprefix, prefix, b->st))) break;
-        const int nrf = homogeneous(it) ? 2 : 1;  // lane refill only for one shared geometry
+        // lane refill only for one shared geometry: the tile queue and one
+        // column at a time (wave pools, refill 2 and 3, measured slower than
+        // the tile queue on every shape, are not candidates)
+        const int nrf = homogeneous(it) ? 2 : 1;
         for (int rf = 0; rf < nrf; rf++)
         for (int wpb : wpbs)
             for (int wpc : wpcs) {

@@ -192,8 +192,11 @@ struct RelaxBatchLaunch {
     int waves_per_block;  // 1..4
     int nblocks;          // persistent workgroups
     // lane refill (every item of item0's geometry): lanes take pixel
-    // columns one by one in tile order from per-XCD column queues
+    // columns one by one in tile order from per-XCD column queues, or, with
+    // refill_chunk > 0, from a pool of that many consecutive columns that
+    // their wave takes at once
     bool refill = false;
+    int refill_chunk = 0;
     BatchRelaxItem item0{};
     // warm starts (relax_column_warm): on when `warm`; pred = step-0
     // estimates per pixel column (3 floats) or nullptr

@@ -92,7 +92,11 @@ __device__ __forceinline__ void column_init(const DevArgs &A, Column<T4> &C, int
 // One force evaluation of a plain column: FIRE to convergence at each z
 // step, outputs in stroke order.  Counts force evaluations and finished
 // positions.
-template <typename T4, int MODE, int K, bool ORTHO = false>
+// BUF: grid loads through a buffer resource (orthogonal kernels whose
+// lanes share one grid); the lanes of a refill wave may each read another
+// grid, and a buffer resource per lane would be a loop over the lanes'
+// distinct grids at every load, so they use plain global loads.
+template <typename T4, int MODE, int K, bool ORTHO = false, bool BUF = ORTHO>
 __device__ __forceinline__ void column_step(const DevArgs &A, const T4 *__restrict__ grid, Column<T4> &C, Stage<K> &G,
                                             unsigned long long &nevals, unsigned long long &npos)
 {
@@ -100,7 +104,7 @@ __device__ __forceinline__ void column_step(const DevArgs &A, const T4 *__restri
     to_grid<ORTHO>(A, C.S.rx, C.S.ry, C.S.rz, gx, gy, gz);
     // the orthogonal kernels also read the grid through a buffer resource
     // (the host picks them for grids under 4 GiB only)
-    gather_cached<T4, ORTHO, MODE == 1>(grid, A, gx, gy, gz, fe, C.cache);
+    gather_cached<T4, BUF, MODE == 1>(grid, A, gx, gy, gz, fe, C.cache);
     nevals++;
     bool done = false;
     const long o = C.obase + C.c.col0 + C.s;
@@ -160,7 +164,7 @@ __device__ __forceinline__ void column_step(const DevArgs &A, const T4 *__restri
 // the next step continues from FIRE's result.  Newton iterates and FIRE
 // moves share the cell cache of the corners (a load only when the cell
 // changes); Newton evaluations count in nevals.
-template <typename T4, int MODE, int K, bool ORTHO = false>
+template <typename T4, int MODE, int K, bool ORTHO = false, bool BUF = ORTHO>
 __device__ __forceinline__ void column_step_warm(const DevArgs &A, const T4 *__restrict__ grid, Column<T4> &C,
                                                  Stage<K> &G, unsigned long long &nevals, unsigned long long &npos,
                                                  unsigned long long &nfb)
@@ -168,7 +172,7 @@ __device__ __forceinline__ void column_step_warm(const DevArgs &A, const T4 *__r
     double gx, gy, gz, ux, uy, uz;
     to_grid<ORTHO>(A, C.newton ? C.qx : C.S.rx, C.newton ? C.qy : C.S.ry, C.newton ? C.qz : C.S.rz, gx, gy, gz);
     // the cell cache: Newton iterates near convergence and FIRE's moves mostly stay in one cell
-    corners_cached<T4, ORTHO, MODE == 1>(grid, A, gx, gy, gz, C.cache, ux, uy, uz);
+    corners_cached<T4, BUF, MODE == 1>(grid, A, gx, gy, gz, C.cache, ux, uy, uz);
     const T4 *v = C.cache.v;
     nevals++;
     bool done = false;
@@ -369,6 +373,10 @@ struct BatchArgs {
     int *iters;
     const float *pred;  // step-0 estimates (pixel columns, 3), WARM kernels only
     unsigned char *status;
+    // relax_refill_kernel: 0, lanes take columns one by one from the XCD's
+    // queue; > 0, a wave takes this many consecutive columns at once into a
+    // pool of its own and its lanes take the pool's columns in order
+    int chunk;
 };
 
 __device__ __forceinline__ int find_batch_item(const long long *__restrict__ prefix, int nitems, long long x)
@@ -470,6 +478,7 @@ __global__ __launch_bounds__(256) void relax_refill_kernel(DevArgs A, BatchArgs 
     const T4 *grid = grids;
     int q = 0;         // parts of the column range this wave found empty (uniform)
     bool more = true;  // this lane may still find a column
+    long long pool_lo = 0, pool_hi = 0;  // the wave's pool (B.chunk > 0): columns not yet taken (uniform)
     while (true) {
         bool need = more && C.s >= A.nz;
         unsigned long long mask = __ballot(need);
@@ -477,13 +486,33 @@ __global__ __launch_bounds__(256) void relax_refill_kernel(DevArgs A, BatchArgs 
             const int part = (xcd + q) & 7;
             const long long lo = part * per;
             const long long hi = min(ncols, lo + per);
-            const int leader = __ffsll((long long)mask) - 1;
-            unsigned base = 0;
-            if (lane == leader) base = atomicAdd(&B.queue[part], (unsigned)__popcll(mask));
-            base = __shfl(base, leader, 64);
+            long long col0;  // this pass hands out columns col0, col0 + 1, ... by lane rank
+            long long colend = hi;
+            if (B.chunk > 0) {
+                if (pool_lo >= pool_hi) {
+                    unsigned base = 0;
+                    if (lane == 0) base = atomicAdd(&B.queue[part], (unsigned)B.chunk);
+                    base = __shfl(base, 0, 64);
+                    if (lo + (long long)base >= hi) {  // this part is empty
+                        q++;
+                        continue;
+                    }
+                    pool_lo = lo + (long long)base;
+                    pool_hi = min(hi, pool_lo + B.chunk);
+                }
+                col0 = pool_lo;
+                colend = pool_hi;
+                pool_lo = min(pool_hi, pool_lo + (long long)__popcll(mask));
+            } else {
+                const int leader = __ffsll((long long)mask) - 1;
+                unsigned base = 0;
+                if (lane == leader) base = atomicAdd(&B.queue[part], (unsigned)__popcll(mask));
+                base = __shfl(base, leader, 64);
+                col0 = lo + (long long)base;
+            }
             if (need) {
-                const long long col = lo + (long long)base + __popcll(mask & ((1ull << lane) - 1));
-                if (col < hi) {
+                const long long col = col0 + __popcll(mask & ((1ull << lane) - 1));
+                if (col < colend) {
                     const long long m = col / cols_item;
                     const long long lc = col - m * cols_item;
                     const int t = (int)(lc >> 6), l = (int)(lc & 63);
@@ -499,7 +528,7 @@ __global__ __launch_bounds__(256) void relax_refill_kernel(DevArgs A, BatchArgs 
                 }
             }
             mask = __ballot(need);
-            if (mask != 0) q++;  // this part is empty
+            if (mask != 0 && B.chunk == 0) q++;  // this part is empty
         }
         if (need) more = false;  // every part is empty
         if (__ballot(C.s < A.nz) == 0) {
@@ -507,8 +536,8 @@ __global__ __launch_bounds__(256) void relax_refill_kernel(DevArgs A, BatchArgs 
             continue;
         }
         if (C.s < A.nz) {
-            if (WARM) column_step_warm<T4, MODE, KS, ORTHO>(A, grid, C, G, nevals, npos, nfb);
-            else column_step<T4, MODE, KS, ORTHO>(A, grid, C, G, nevals, npos);
+            if (WARM) column_step_warm<T4, MODE, KS, ORTHO, false>(A, grid, C, G, nevals, npos, nfb);
+            else column_step<T4, MODE, KS, ORTHO, false>(A, grid, C, G, nevals, npos);
         }
         passes++;
     }
@@ -962,7 +991,7 @@ int launch_relax_batch(const RelaxBatchLaunch &L, hipStream_t st)
         A.newton_extrap2 = L.newton_extrap * L.newton_extrap;
         A.ws_res2 = L.residual_max * L.residual_max;
     }
-    BatchArgs B{L.nitems, L.ntiles, L.queue, L.pos, L.force, L.iters, warm ? L.pred : nullptr, L.status};
+    BatchArgs B{L.nitems, L.ntiles, L.queue, L.pos, L.force, L.iters, warm ? L.pred : nullptr, L.status, L.refill_chunk};
     if (L.ntiles == 0) return 0;
     dim3 grd((unsigned)L.nblocks), blk(64 * L.waves_per_block);
     if (L.f64) {

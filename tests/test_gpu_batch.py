@@ -261,7 +261,8 @@ def test_lane_refill_equals_tile_queue(mode):
     """Items of one geometry (here 24 molecules, a 66x45 scan that leaves
     partial tiles; 71,280 pixel columns, more than the 16,384 lanes of one
     wave per CU, so lanes do take further columns): relaxing with lane
-    refill (each lane takes the next column when its own is done) gives
+    refill (each lane takes the next column when its own is done, from the
+    XCD's queue or from a pool of one or two tiles its wave took) gives
     every output bit for bit as the tile queue does, plain and
     warm-started, with the same counters.  Plain FIRE (columns of ~10^2
     evaluations) also gets a higher lane use from refill; warm-started
@@ -286,17 +287,19 @@ def test_lane_refill_equals_tile_queue(mode):
         if warm:
             b.set_warmstart(None)
         out = {}
-        for refill in (0, 1):
+        for refill in (0, 1, 2, 3):
             b.set_tuning({"tile_y": 4, "waves_per_block": 1, "blocks_per_cu": 1, "refill": refill})
             b.run()
             b.synchronize()
             out[refill] = ([(b.positions_raw(k), b.forces(k), b.iterations(k), b.status(k), b.df_raw(k)) for k in range(NI)],
                            b.counters(), b.lane_use())
-        for k in range(NI):
-            for a, c in zip(out[0][0][k], out[1][0][k]):
-                assert np.array_equal(a, c), (warm, k)
-        assert out[0][1] == out[1][1], (warm, out[0][1], out[1][1])
-        print(f"warm {warm}: lane use tile queue {out[0][2]:.3f}, lane refill {out[1][2]:.3f}")
+        for rf in (1, 2, 3):
+            for k in range(NI):
+                for a, c in zip(out[0][0][k], out[rf][0][k]):
+                    assert np.array_equal(a, c), (warm, rf, k)
+            assert out[0][1] == out[rf][1], (warm, rf, out[0][1], out[rf][1])
+        print(f"warm {warm}: lane use tile queue {out[0][2]:.3f}, lane refill {out[1][2]:.3f}, "
+              f"wave pools of one tile {out[2][2]:.3f} and two {out[3][2]:.3f}")
         if not warm:
-            assert out[1][2] > out[0][2]
+            assert out[1][2] > out[0][2] and out[2][2] > out[0][2]
     b.close()
