@@ -346,10 +346,16 @@ __global__ __launch_bounds__(64 * WPB) void relax_dev_kernel(DevArgs A, const T4
     Stage<kStageSingle> G = make_stage<kStageSingle>(lds, 64 * WPB, threadIdx.x);
     const int per_xcd = ((A.ntiles + WPB - 1) / WPB + 7) / 8;
     const int b = blockIdx.x;
-    const int tile = ((b % 8) * per_xcd + b / 8) * WPB + (int)(threadIdx.x / 64);
+    int tile, lane;
+    if constexpr (WPB == 1) {
+        tile = (b % 8) * per_xcd + b / 8;
+        lane = threadIdx.x;
+    } else {
+        tile = ((b % 8) * per_xcd + b / 8) * WPB + (int)(threadIdx.x / 64);
+        lane = threadIdx.x % 64;
+    }
     if (tile >= A.ntiles) return;
     const int ti = tile / A.tiles_y, tj = tile % A.tiles_y;
-    const int lane = threadIdx.x % 64;
     const int i = ti * 8 + lane / 8, j = tj * 8 + lane % 8;
     unsigned long long nevals = 0, npos = 0;
     relax_column<T4, MODE, kStageSingle, ORTHO>(A, grid, i, j, i < A.nx && j < A.ny, G, nevals, npos);
@@ -714,21 +720,31 @@ static int simd_count()
 
 // Single-scan launch.  With one-wave workgroups the dispatcher has been
 // seen to place two waves of this launch on one SIMD while another SIMD of
-// the CU idles, depending on what ran before it (c1 right after its grid
-// build: 8.9-9.3 ms, against 7.1-7.4 ms after another FIRE launch).  A scan
-// with at most one tile per SIMD (c1: 1024 tiles) therefore uses four-wave
-// workgroups, whose waves go to the CU's four SIMDs; larger scans keep
-// one-wave workgroups, which LDS lets 6 to a CU.
+// the CU idles, depending on what ran before it (c1, 1024 tiles, right after
+// its grid build: 8.9-9.3 ms, against 7.1-7.4 ms after another FIRE launch).
+// A scan with more than 3 tiles per CU and at most one per SIMD (c1)
+// therefore uses four-wave workgroups, one per CU with its waves on the
+// CU's four SIMDs.  Smaller scans keep one-wave workgroups spread over the
+// CUs (four-wave workgroups there leave CUs idle: a 256-tile scan 3.80 ->
+// 4.07 ms; two-wave workgroups on 512 tiles ran 1 % slower), and so do
+// scans beyond one tile per SIMD, which LDS lets 6 to a CU.
+template <typename T4, int MODE, bool ORTHO, int WPB>
+static void launch_single_w(const DevArgs &A, const void *grid, hipStream_t st)
+{
+    const long nb = (((A.ntiles + WPB - 1L) / WPB + 7) / 8) * 8;
+    hipLaunchKernelGGL((relax_dev_kernel<T4, MODE, ORTHO, WPB>), dim3((unsigned)nb), dim3(64 * WPB), 0, st, A,
+                       (const T4 *)grid);
+}
+
 template <typename T4, int MODE, bool ORTHO>
 static void launch_single(const DevArgs &A, const void *grid, hipStream_t st)
 {
     const int simds = simd_count();
-    if (simds > 0 && A.ntiles <= simds) {
-        const long nb = (((A.ntiles + 3L) / 4 + 7) / 8) * 8;
-        hipLaunchKernelGGL((relax_dev_kernel<T4, MODE, ORTHO, 4>), dim3((unsigned)nb), dim3(256), 0, st, A, (const T4 *)grid);
+    const int cus = simds / 4;
+    if (cus > 0 && A.ntiles > 3L * cus && A.ntiles <= simds) {
+        launch_single_w<T4, MODE, ORTHO, 4>(A, grid, st);
     } else {
-        const long nb = ((A.ntiles + 7L) / 8) * 8;
-        hipLaunchKernelGGL((relax_dev_kernel<T4, MODE, ORTHO, 1>), dim3((unsigned)nb), dim3(64), 0, st, A, (const T4 *)grid);
+        launch_single_w<T4, MODE, ORTHO, 1>(A, grid, st);
     }
 }
 

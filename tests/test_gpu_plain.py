@@ -112,10 +112,12 @@ def test_force_and_df_c0(c0):
 
 def test_relax_more_tiles_than_simds():
     """A 264 x 264 scan (1,089 tiles of 8 x 8 columns, more than the GPU's
-    1,024 SIMDs) runs one-wave workgroups; a scan with at most one tile per
-    SIMD runs four-wave workgroups.  The large scan is checked against the
-    oracle, and its first 128 x 128 columns, scanned again on their own
-    (same descriptor, four-wave workgroups), must give the same bits."""
+    1,024 SIMDs) runs one-wave workgroups; scans with more than 3 tiles per
+    CU (768 on the GPU's 256 CUs) and at most one per SIMD run four-wave
+    workgroups.  The large scan is checked against the oracle, and its first
+    232 x 232 columns (841 tiles: four-wave workgroups) and 128 x 128
+    columns (256 tiles: one-wave workgroups), scanned again on their own
+    with the same descriptor, must give the same bits."""
     from ppafm_amd import device as D
     k = case(0)
     cfg = dict(k["cfg"])
@@ -134,12 +136,13 @@ def test_relax_more_tiles_than_simds():
     git = s.iterations()
     print(f"positions |dr| [A]: max {d.max():.3g} p99 {np.percentile(d, 99):.3g}; iterations differing {(git != it).sum()} of {it.size}")
     assert d.max() < POS_TOL
-    sub = D.scan_desc(128, 128, cfg["nz"], s.scan_d.p0, s.scan_d.da, s.scan_d.db, s.scan_d.dstep)
-    p2, f2, i2 = D.relax(s.n, s.lvec[1:], s.grid, s.rp, s.tip, sub, pos=True, force=True, iters=True)
-    D.synchronize()
-    assert np.array_equal(p2.download(), s.pos.download()[:128, :128])
-    assert np.array_equal(f2.download(), s.force.download()[:128, :128])
-    assert np.array_equal(i2.download(), s.iters.download()[:128, :128])
+    for m in (232, 128):
+        sub = D.scan_desc(m, m, cfg["nz"], s.scan_d.p0, s.scan_d.da, s.scan_d.db, s.scan_d.dstep)
+        p2, f2, i2 = D.relax(s.n, s.lvec[1:], s.grid, s.rp, s.tip, sub, pos=True, force=True, iters=True)
+        D.synchronize()
+        assert np.array_equal(p2.download(), s.pos.download()[:m, :m]), m
+        assert np.array_equal(f2.download(), s.force.download()[:m, :m]), m
+        assert np.array_equal(i2.download(), s.iters.download()[:m, :m]), m
 
 
 def test_repeat_is_deterministic(c0):
